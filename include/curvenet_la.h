@@ -1,0 +1,263 @@
+/*
+ * curvenet_la.h -- C ABI of the MI355X (gfx950) profile-curves linear-algebra
+ * core: cut-mesh build, cotangent-Laplacian assembly, SpMV/SpMM, IC(0) and
+ * preconditioned conjugate gradients.
+ *
+ * The reference (V-Sekai-fire/TOOL_godot_curvenet) specifies this path in Lean
+ * (the lean/Curvenet/ spec files) and states that its C++ runtime `src/curvenet/*.h`
+ * "mirrors this 1:1" (SparseLinAlg.lean:24, HalfedgeBuilder.lean:4).  Each
+ * entry point below names the spec function it replaces.
+ *
+ * Conventions
+ *   - plain C types only; indices are int32, "none" is -1
+ *     (SlangCodegen/HalfedgeBuilder.lean:22);
+ *   - pointers named d_* are device pointers (hipMalloc'd or torch CUDA
+ *     memory); h_* are host pointers;
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream);
+ *   - calls are synchronous with respect to the host when they return sizes
+ *     or status computed on the device; otherwise they are stream-ordered;
+ *   - multi-column blocks are n x k row-major ("interleaved", x[i*k + c]),
+ *     the layout of SlangCodegen/SpmvMulti.lean:113;
+ *   - CSR is row_ptr[rows+1], col[nnz] ascending per row, val[nnz]
+ *     (SparseLinAlg.lean:35-41);
+ *   - handles are not thread-safe; use one stream per handle.
+ */
+#ifndef CURVENET_LA_H
+#define CURVENET_LA_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cn_status {
+  CN_OK = 0,
+  CN_MAXIT = 1,        /* iteration cap reached before the tolerance        */
+  CN_BREAKDOWN = 2,    /* p^T A p == 0 (SparseLinAlg.lean:118)               */
+  CN_NOT_SPD = 3,      /* non-positive IC(0) pivot (IncompleteCholesky.lean:53-57) */
+  CN_BAD_ARG = -1,
+  CN_HIP_ERROR = -2,
+  CN_NO_DEVICE = -3,
+  CN_OOM = -4,
+  CN_NOT_MANIFOLD = -5
+} cn_status;
+
+/* Human-readable text of the last error on the calling thread. */
+const char *cn_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int32_t cn_version(void);
+/* CN_OK when a HIP device is visible, CN_NO_DEVICE otherwise. */
+cn_status cn_device_check(void);
+
+/* ------------------------------------------------------------------------
+ * Topology (L1)
+ * ---------------------------------------------------------------------- */
+
+/* HalfedgeBuilder.fromTriangles (HalfedgeBuilder.lean:34-90): interior
+ * halfedges 3f+{0,1,2} = t0->t1, t1->t2, t2->t0; twin = first reverse match;
+ * one boundary halfedge per orphan in orphan order; boundary `next` = first
+ * boundary halfedge leaving the target, self-loop otherwise.
+ * d_target/d_twin/d_next/d_face need capacity 6*nf.  *h_nh receives nh. */
+cn_status cn_halfedge_from_triangles(int32_t nv, int32_t nf,
+                                     const int32_t *d_tris, int32_t *d_target,
+                                     int32_t *d_twin, int32_t *d_next,
+                                     int32_t *d_face, int32_t *h_nh,
+                                     void *stream);
+
+/* CutAlgorithm.subdivideEdge / splitFace / insertCrack
+ * (CutAlgorithm.lean:49-94, 115-163, 212-243).  Host arrays with room for
+ * two more halfedges; returns the new halfedge count in *h_nh and updates
+ * *h_nv / *h_nf.  O(1) pointer surgery (splitFace walks one loop). */
+cn_status cn_cut_subdivide_edge(int32_t *h_nv, int32_t *h_nf, int32_t *h_nh,
+                                int32_t h, int32_t *h_target, int32_t *h_twin,
+                                int32_t *h_next, int32_t *h_face);
+cn_status cn_cut_split_face(int32_t *h_nv, int32_t *h_nf, int32_t *h_nh,
+                            int32_t a, int32_t b, int32_t *h_target,
+                            int32_t *h_twin, int32_t *h_next, int32_t *h_face);
+cn_status cn_cut_insert_crack(int32_t *h_nv, int32_t *h_nf, int32_t *h_nh,
+                              int32_t h_anchor, int32_t *h_target,
+                              int32_t *h_twin, int32_t *h_next,
+                              int32_t *h_face);
+
+/* Seam duplication.  The reference records the curve->seam tracer as an
+ * open axiom (Features/KnownUnknowns/CurveSegmentTracing.lean:3-7) and its
+ * C++ `cut_algorithm::insert_crack_at_endpoint` is absent, so the cut rule is
+ * this library's: the mesh is cut open along the given undirected edges.
+ * Around every vertex touching a cut edge the face corners split into
+ * wedges (corners joined across uncut interior edges).  The wedge holding
+ * the smallest corner slot 3f+l keeps the vertex index; the others get new
+ * vertices nv, nv+1, ... in (vertex, wedge-min-slot) order, appended like
+ * CutAlgorithm's new vertices (CutAlgorithm.lean:40-47).  Every copy of a
+ * vertex on a cut edge becomes a constrained (sample) vertex
+ * (CutVertexKind.sample, CutMesh.lean:39-43).
+ *   d_new_tris[3 nf]; d_origin / d_constrained capacity nv + 2*ncut.
+ *   *h_nv_out = vertex count after the cut. */
+cn_status cn_cut_seams(int32_t nv, int32_t nf, const int32_t *d_tris,
+                       int32_t ncut, const int32_t *d_cut_edges,
+                       int32_t *d_new_tris, int32_t *d_origin,
+                       uint8_t *d_constrained, int32_t *h_nv_out, void *stream);
+
+/* SurfaceProjection.projectToVertices (SurfaceProjection.lean:46-64):
+ * nearest vertex by squared distance, strict <, lowest index wins ties. */
+cn_status cn_project_to_vertices(int32_t nv, const double *d_pos, int32_t nq,
+                                 const double *d_q, int32_t *d_index,
+                                 void *stream);
+
+/* ------------------------------------------------------------------------
+ * Cut Laplacian system (L2): VtLhV of CutMeshLaplacian.assembleVtLhV
+ * (CutMeshLaplacian.lean:56-93), stored sparse and split into the free
+ * block A_ff (rows/cols = mesh vertices) and A_fc (cols = sample vertices;
+ * the spec's sample rows and columns of VtLhV are zero, HarmonicSolve.lean:14-16).
+ * Per entry the sum runs over incoming halfedges p in ascending order, then
+ * over q in p's face in ascending order -- the order of the spec's two dense
+ * matMul loops -- so FP64 values match the spec bit for bit.
+ * ---------------------------------------------------------------------- */
+typedef struct cn_system_t *cn_system;
+
+typedef struct cn_system_info {
+  int32_t nv, nf, nh, n_free, n_con;
+  int64_t nnz_ff, nnz_fc;
+  const int32_t *d_ff_ptr, *d_ff_col;   /* A_ff, CSR */
+  const double *d_ff_val;
+  const float *d_ff_val32;              /* FP32 mirror */
+  const int32_t *d_fc_ptr, *d_fc_col;   /* A_fc, CSR (cols = con index) */
+  const double *d_fc_val;
+  const int32_t *d_free_of, *d_con_of;  /* per vertex: row / col, or -1 */
+  const int32_t *d_free_vertex, *d_con_vertex; /* inverse maps */
+  const int32_t *d_he_target, *d_he_twin, *d_he_next, *d_he_face;
+} cn_system_info;
+
+/* Build the system of a (cut) triangle mesh.  d_tris[3 nf], d_constrained[nv]
+ * (1 = sample vertex), d_pos[3 nv] FP64.  Copies its inputs. */
+cn_status cn_system_create(int32_t nv, int32_t nf, const int32_t *d_tris,
+                           const uint8_t *d_constrained, const double *d_pos,
+                           cn_system *out, void *stream);
+/* A system holding a caller-assembled SPD matrix as its A_ff (no mesh, no
+ * A_fc): the `SparseMatrixCSR` argument of SparseLinAlg.cg
+ * (SparseLinAlg.lean:35-41, 104).  Copies its inputs. */
+cn_status cn_system_from_csr(int32_t n, const int32_t *d_ptr,
+                             const int32_t *d_col, const double *d_val,
+                             cn_system *out, void *stream);
+cn_status cn_system_get_info(cn_system sys, cn_system_info *info);
+/* Re-assemble values for new vertex positions (same topology). */
+cn_status cn_system_set_positions(cn_system sys, const double *d_pos,
+                                  void *stream);
+cn_status cn_system_destroy(cn_system sys);
+
+/* HarmonicSolve.solveMulti right-hand side (HarmonicSolve.lean:35-45,57-64)
+ * restricted to free rows: d_rhs[n_free*k] = -(Vt Lh C Fc), with
+ * d_Xc[n_con*k] the sample values per constrained column, summed in the
+ * spec's order (FP64, bit-exact). */
+cn_status cn_harmonic_rhs(cn_system sys, int32_t k, const double *d_Xc,
+                          double *d_rhs, void *stream);
+
+/* ------------------------------------------------------------------------
+ * SpMV / SpMM (L3): SparseLinAlg.spmv (SparseLinAlg.lean:44-53) and the
+ * SpmvMulti kernel (SlangCodegen/SpmvMulti.lean:78-119) without its k<=16
+ * cap.  The f64 forms sum each row left to right with separate multiply and
+ * add, like the spec, so they are bit-exact against it.
+ * ---------------------------------------------------------------------- */
+cn_status cn_spmv_f64(int32_t rows, const int32_t *d_ptr, const int32_t *d_col,
+                      const double *d_val, const double *d_x, double *d_y,
+                      void *stream);
+cn_status cn_spmv_f32(int32_t rows, const int32_t *d_ptr, const int32_t *d_col,
+                      const float *d_val, const float *d_x, float *d_y,
+                      void *stream);
+cn_status cn_spmm_f64(int32_t rows, const int32_t *d_ptr, const int32_t *d_col,
+                      const double *d_val, int32_t k, const double *d_X,
+                      double *d_Y, void *stream);
+cn_status cn_spmm_f32(int32_t rows, const int32_t *d_ptr, const int32_t *d_col,
+                      const float *d_val, int32_t k, const float *d_X,
+                      float *d_Y, void *stream);
+
+/* ------------------------------------------------------------------------
+ * IC(0) (IncompleteCholesky.lean:49-101; CSR no-fill form :26-33)
+ * ---------------------------------------------------------------------- */
+typedef struct cn_ic0_t *cn_ic0;
+
+enum {
+  CN_ORDER_COLOR = 0,   /* GraphColoring.greedyColor order (GraphColoring.lean:41-56):
+                           rows grouped by colour, the colour-pass schedule of
+                           SlangCodegen/IncompleteCholesky.lean:10-12 */
+  CN_ORDER_NATURAL = 1  /* natural order, rows grouped by dependency level */
+};
+
+typedef struct cn_ic0_info {
+  int32_t n, nlevels, ordering;
+  int64_t nnz_l;        /* incl. diagonal */
+  double shift;         /* diagonal shift used (0 unless a pivot failed) */
+  const int32_t *d_perm;            /* perm[new] = old */
+  const int32_t *d_a_ptr, *d_a_col; /* permuted A */
+  const double *d_a_val;
+  const int32_t *d_l_ptr, *d_l_col; /* strictly lower L, CSR */
+  const double *d_l_val, *d_l_diag;
+} cn_ic0_info;
+
+/* Factor A_ff (permuted by `ordering`).  Pivots are FP64 and follow the
+ * spec's recurrences in its summation order.  A non-positive pivot returns
+ * CN_NOT_SPD unless `allow_shift` is set, in which case the factor of
+ * A + s*diag(A) is taken for the smallest s in {1e-4,1e-3,...,1} that works
+ * (reported in info.shift). */
+cn_status cn_ic0_factor(cn_system sys, int32_t ordering, int32_t allow_shift,
+                        cn_ic0 *out, void *stream);
+cn_status cn_ic0_get_info(cn_ic0 f, cn_ic0_info *info);
+/* z = (L L^T)^{-1} r in the system's vertex order (FP64), d_r/d_z n x k. */
+cn_status cn_ic0_apply(cn_ic0 f, int32_t k, const double *d_r, double *d_z,
+                       void *stream);
+cn_status cn_ic0_destroy(cn_ic0 f);
+
+/* ------------------------------------------------------------------------
+ * Preconditioned conjugate gradients (SparseLinAlg.cg / cgWithGuess,
+ * SparseLinAlg.lean:104-163) on A_ff, k right-hand sides in lockstep.
+ * ---------------------------------------------------------------------- */
+enum { CN_PRECOND_JACOBI = 0, CN_PRECOND_IC0 = 1 };
+enum {
+  CN_PREC_FP64_PLAIN = 0, /* FP64; one kernel per BLAS-1/SpMV step, alpha/beta
+                             and the stop test on the host (3 host round
+                             trips per iteration, the dispatch chain of the
+                             reference's GPU CG) */
+  CN_PREC_FP64 = 1,       /* FP64; scalars on the device, fused kernels */
+  CN_PREC_MIXED = 2       /* FP32 SpMM / IC(0) / search directions inside an
+                             FP64 iterative-refinement loop: FP64 residual
+                             b - A x, FP64 dots and alpha/beta, FP64 update */
+};
+
+typedef struct cn_pcg_opts {
+  int32_t precond;        /* CN_PRECOND_* (IC0 needs a factor)               */
+  int32_t precision;      /* CN_PREC_*                                        */
+  double tol_rel;         /* stop when ||b - A x||_2 <= tol_rel ||b||_2 per
+                             column, recomputed in FP64                      */
+  int32_t max_iter;       /* total inner iterations cap                       */
+  double inner_tol;       /* MIXED: relative reduction per refinement step    */
+  int32_t max_refine;     /* MIXED: refinement steps cap                      */
+  int32_t check_every;    /* device-side stop flags are read every n iters    */
+  int32_t warm_start;     /* 1: d_x holds x0 (cgWithGuess), 0: x0 = 0 (cg)    */
+} cn_pcg_opts;
+
+typedef struct cn_solve_info {
+  int32_t status;         /* cn_status                                        */
+  int32_t iters;          /* inner iterations performed                       */
+  int32_t refinements;    /* FP64 residual recomputations (MIXED)             */
+  double relres_max;      /* max over columns of ||b-Ax||/||b|| (FP64)        */
+} cn_solve_info;
+
+/* Fill opts with defaults (IC0, MIXED, tol 1e-9, 20000 iters, inner 1e-5). */
+void cn_pcg_default_opts(cn_pcg_opts *opts);
+
+/* Solve A_ff X = B for k columns.  d_b / d_x are n_free x k FP64.
+ * `ic0` may be NULL for Jacobi.  Promoted (sample) rows are not part of
+ * A_ff; the caller overlays their values (DeformSolve.lean:108-109). */
+cn_status cn_pcg_solve(cn_system sys, cn_ic0 ic0, const cn_pcg_opts *opts,
+                       int32_t k, const double *d_b, double *d_x,
+                       cn_solve_info *info, void *stream);
+
+/* Per-kernel timing of the last cn_pcg_solve on this thread, gathered with
+ * HIP events around every launch of the named kernel when enabled.
+ * name: "spmm" (FP32 SpMM of the inner loop).  Returns launches timed;
+ * *ms_total gets their summed duration. */
+void cn_profile_enable(int32_t on);
+int32_t cn_profile_query(const char *name, double *ms_total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

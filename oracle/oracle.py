@@ -1,0 +1,396 @@
+"""ctypes front end of liboracle.so (cn_oracle.c), numpy in / numpy out.
+
+TEST INFRASTRUCTURE ONLY: the checker the GPU path is compared against.
+Every function names the spec definition it restates (see cn_oracle.c for
+file:line citations).  Parity status of each function is listed in
+PINS below: every function is pinned by at least one independent check in
+tests/test_oracle_pins.py or tests/test_oracle_kat.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+# function -> the independent checks that pin it
+PINS = {
+    "he_from_triangles_spec": "spec KATs (counts, manifold?) HalfedgeBuilder.lean:115-136",
+    "he_from_triangles": "index-identical to the literal spec builder on random meshes",
+    "he_manifold": "spec KATs + planted defects",
+    "subdivide_edge/split_face/insert_crack": "CutAlgorithm.lean:269-507 KATs",
+    "cut_edges": "cylinder seam closed form (64 copies at nv+j), Euler/partition invariants, "
+                 "brute-force wedge recount",
+    "cot/triangle/polygon Laplacian": "closed forms (equilateral -1/(2 sqrt 3)), row sums, symmetry",
+    "assemble_vtlhv_dense": "CutMeshLaplacian.lean:101-179 KATs, linear precision",
+    "cut_laplacian_csr": "bit-identical to the dense spec transcription on small cut meshes",
+    "harmonic_rhs_csr": "bit-identical to the dense spec transcription; HarmonicSolve KATs",
+    "spmv/spmm": "identity KAT; numpy reference",
+    "cg": "SparseLinAlg.lean:171-287 KATs; dense GE",
+    "ic0_factor": "LL^T == A on the kept pattern; IncompleteCholesky.lean:128-186 KATs; "
+                  "full-pattern IC(0) == dense Cholesky",
+    "ic0_solve": "LL^T x == b",
+    "pcg": "dense GE / numpy solve",
+    "greedy_color": "GraphColoring.lean:79-117 KATs, validity",
+    "dense solve/LU/Cholesky": "DenseLinAlg.lean:294-489 KATs, numpy.linalg",
+    "hsc_eliminate_vertex": "HSC KATs (HierarchicalSparsifyCompensate.lean:134-231), "
+                            "dense Schur complement",
+}
+
+
+def build() -> pathlib.Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+_lib = None
+i32, i64, f64 = C.c_int32, C.c_int64, C.c_double
+P = C.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+        _lib.orc_cot_at_vertex.restype = f64
+        _lib.orc_dot.restype = f64
+        _lib.orc_hsc_degree_weight.restype = f64
+        _lib.orc_hsc_edge_weight.restype = f64
+        for name in ("orc_he_from_triangles_spec", "orc_he_from_triangles", "orc_he_manifold",
+                     "orc_subdivide_edge", "orc_split_face", "orc_insert_crack", "orc_cut_edges",
+                     "orc_cut_laplacian_csr", "orc_cg", "orc_ic0_factor", "orc_pcg",
+                     "orc_greedy_color", "orc_hsc_eliminate_vertex"):
+            getattr(_lib, name).restype = i32
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(P)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ------------------------------------------------------------------ topology --
+def he_from_triangles(nv, tris, spec=False):
+    tris = _i(np.asarray(tris).reshape(-1, 3))
+    nf = tris.shape[0]
+    cap = max(6 * nf, 1)
+    t, w, n, f = (np.zeros(cap, np.int32) for _ in range(4))
+    fn = lib().orc_he_from_triangles_spec if spec else lib().orc_he_from_triangles
+    nh = fn(nv, nf, _p(tris), _p(t), _p(w), _p(n), _p(f))
+    return {"target": t[:nh].copy(), "twin": w[:nh].copy(), "next": n[:nh].copy(),
+            "face": f[:nh].copy(), "nv": nv, "nf": nf, "nh": nh}
+
+
+def he_manifold(he) -> int:
+    t, w, n, f = (_i(he[k]) for k in ("target", "twin", "next", "face"))
+    return lib().orc_he_manifold(he["nv"], he["nf"], he["nh"], _p(t), _p(w), _p(n), _p(f))
+
+
+def _surgery(fn, he, *args):
+    nh = he["nh"]
+    arrs = [np.concatenate([_i(he[k]), np.zeros(2, np.int32)])
+            for k in ("target", "twin", "next", "face")]
+    nv, nf = C.c_int32(he["nv"]), C.c_int32(he["nf"])
+    nh2 = fn(C.byref(nv), C.byref(nf), nh, *args, *(_p(a) for a in arrs))
+    return {"target": arrs[0][:nh2], "twin": arrs[1][:nh2], "next": arrs[2][:nh2],
+            "face": arrs[3][:nh2], "nv": nv.value, "nf": nf.value, "nh": nh2}
+
+
+def subdivide_edge(he, h):
+    return _surgery(lib().orc_subdivide_edge, he, h)
+
+
+def split_face(he, a, b):
+    return _surgery(lib().orc_split_face, he, a, b)
+
+
+def insert_crack(he, h):
+    return _surgery(lib().orc_insert_crack, he, h)
+
+
+def cut_edges(nv, tris, edges):
+    tris = _i(np.asarray(tris).reshape(-1, 3))
+    edges = _i(np.asarray(edges).reshape(-1, 2))
+    cap = nv + 2 * edges.shape[0] + 1
+    nt = np.zeros_like(tris)
+    origin = np.zeros(cap, np.int32)
+    con = np.zeros(cap, np.uint8)
+    nv2 = lib().orc_cut_edges(nv, tris.shape[0], _p(tris), edges.shape[0], _p(edges), _p(nt),
+                              _p(origin), _p(con))
+    if nv2 < 0:
+        raise ValueError(f"orc_cut_edges failed ({nv2})")
+    return nt, origin[:nv2].copy(), con[:nv2].copy(), nv2
+
+
+def project_to_vertices(knots, pos):
+    """SurfaceProjection.projectToVertices (SurfaceProjection.lean:46-64),
+    direct transcription (strict <, lowest index wins)."""
+    knots = _d(np.asarray(knots).reshape(-1, 3))
+    pos = _d(np.asarray(pos).reshape(-1, 3))
+    out = np.empty(knots.shape[0], np.int32)
+    for i, k in enumerate(knots):
+        best, bi = 1e308, -1
+        d = pos - k[None]
+        dd = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+        for v in range(pos.shape[0]):
+            if dd[v] < best:
+                best, bi = dd[v], v
+        out[i] = bi
+    return out
+
+
+# ----------------------------------------------------------------- Laplacian --
+def cot_at_vertex(apex, o1, o2):
+    return lib().orc_cot_at_vertex(_p(_d(apex)), _p(_d(o1)), _p(_d(o2)))
+
+
+def triangle_cot_laplacian(a, b, c):
+    m = np.zeros(9)
+    lib().orc_triangle_cot_laplacian(_p(_d(a)), _p(_d(b)), _p(_d(c)), _p(m))
+    return m.reshape(3, 3)
+
+
+def polygon_cot_laplacian(poly):
+    poly = _d(np.asarray(poly).reshape(-1, 3))
+    n = poly.shape[0]
+    m = np.zeros(n * n)
+    lib().orc_polygon_cot_laplacian(n, _p(poly), _p(m))
+    return m.reshape(n, n)
+
+
+def assemble_vtlhv_dense(he, is_sample, pos):
+    nv = he["nv"]
+    out = np.zeros(nv * nv)
+    lib().orc_assemble_vtlhv_dense(nv, he["nf"], he["nh"], _p(_i(he["target"])),
+                                   _p(_i(he["next"])), _p(_i(he["face"])),
+                                   _p(np.ascontiguousarray(is_sample, np.uint8)),
+                                   _p(_d(pos)), _p(out))
+    return out.reshape(nv, nv)
+
+
+def harmonic_rhs_dense(he, col_of_vertex, pos, Fc):
+    Fc = _d(np.asarray(Fc))
+    k = Fc.shape[1] if Fc.ndim == 2 else 1
+    nv = he["nv"]
+    out = np.zeros(nv * k)
+    lib().orc_harmonic_rhs_dense(nv, he["nf"], he["nh"], _p(_i(he["target"])), _p(_i(he["next"])),
+                                 _p(_i(he["face"])), _p(_i(col_of_vertex)), _p(_d(pos)), k,
+                                 _p(Fc), _p(out))
+    return out.reshape(nv, k)
+
+
+def cut_laplacian_csr(he, constrained, pos):
+    """Returns dict with free_of, con_of, A_ff=(ptr,col,val), A_fc=(ptr,col,val)."""
+    nv, nh = he["nv"], he["nh"]
+    con = np.ascontiguousarray(constrained, np.uint8)
+    cap = nv + 6 * he["nf"] + 16
+    free_of = np.zeros(nv, np.int32)
+    con_of = np.zeros(nv, np.int32)
+    ffp, fcp = np.zeros(nv + 1, np.int32), np.zeros(nv + 1, np.int32)
+    ffc, fcc = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+    ffv, fcv = np.zeros(cap), np.zeros(cap)
+    nnz = np.zeros(2, np.int64)
+    nfree = lib().orc_cut_laplacian_csr(nv, he["nf"], nh, _p(_i(he["target"])), _p(_i(he["next"])),
+                                        _p(_i(he["face"])), _p(con), _p(_d(pos)), _p(free_of),
+                                        _p(con_of), _p(ffp), _p(ffc), _p(ffv), _p(fcp), _p(fcc),
+                                        _p(fcv), C.c_int64(cap), _p(nnz))
+    if nfree < 0:
+        raise ValueError("orc_cut_laplacian_csr overflow")
+    a, b = int(nnz[0]), int(nnz[1])
+    return {"n_free": nfree, "free_of": free_of, "con_of": con_of,
+            "A_ff": (ffp[:nfree + 1].copy(), ffc[:a].copy(), ffv[:a].copy()),
+            "A_fc": (fcp[:nfree + 1].copy(), fcc[:b].copy(), fcv[:b].copy())}
+
+
+def harmonic_rhs_csr(he, free_of, con_of, pos, Xc):
+    Xc = _d(np.asarray(Xc))
+    k = Xc.shape[1] if Xc.ndim == 2 else 1
+    nfree = int((np.asarray(free_of) >= 0).sum())
+    out = np.zeros(nfree * k)
+    lib().orc_harmonic_rhs_csr(he["nv"], he["nh"], _p(_i(he["target"])), _p(_i(he["next"])),
+                               _p(_i(he["face"])), _p(_i(free_of)), _p(_i(con_of)), _p(_d(pos)),
+                               k, _p(Xc), _p(out))
+    return out.reshape(nfree, k)
+
+
+# ------------------------------------------------------------ sparse linalg --
+def spmv(ptr, col, val, x):
+    n = len(ptr) - 1
+    y = np.zeros(n)
+    lib().orc_spmv(n, _p(_i(ptr)), _p(_i(col)), _p(_d(val)), _p(_d(x)), _p(y))
+    return y
+
+
+def spmm(ptr, col, val, X):
+    X = _d(X)
+    n, k = len(ptr) - 1, X.shape[1]
+    Y = np.zeros(n * k)
+    lib().orc_spmm(n, _p(_i(ptr)), _p(_i(col)), _p(_d(val)), k, _p(X), _p(Y))
+    return Y.reshape(n, k)
+
+
+def dot(a, b):
+    a, b = _d(a), _d(b)
+    return lib().orc_dot(len(a), _p(a), _p(b))
+
+
+def cg(ptr, col, val, b, max_iter, tol, x0=None):
+    """SparseLinAlg.cg / cgWithGuess. Returns (x, iterations)."""
+    n = len(ptr) - 1
+    x = np.zeros(n)
+    b = _d(b)
+    it = lib().orc_cg(n, _p(_i(ptr)), _p(_i(col)), _p(_d(val)), _p(b),
+                      _p(_d(x0)) if x0 is not None else None, max_iter, C.c_double(tol), _p(x))
+    return x, it
+
+
+def ic0_factor3(A, pattern):
+    L = np.zeros(9)
+    lib().orc_ic0_factor3(_p(_d(np.asarray(A).reshape(-1))), _p(_d(np.asarray(pattern).reshape(-1))),
+                          _p(L))
+    return L.reshape(3, 3)
+
+
+def ic0_factor(ptr, col, val):
+    """CSR IC(0); returns (l_ptr, l_col, l_val) incl. diagonal (last per row)."""
+    n = len(ptr) - 1
+    ptr, col, val = _i(ptr), _i(col), _d(val)
+    nl = int(sum(1 for i in range(n) for p in range(ptr[i], ptr[i + 1]) if col[p] <= i))
+    lp = np.zeros(n + 1, np.int32)
+    lc = np.zeros(max(nl, 1), np.int32)
+    lv = np.zeros(max(nl, 1))
+    st = lib().orc_ic0_factor(n, _p(ptr), _p(col), _p(val), _p(lp), _p(lc), _p(lv))
+    if st != 0:
+        raise ArithmeticError(f"IC(0) breakdown at row {-st - 1}")
+    return lp, lc[:nl], lv[:nl]
+
+
+def ic0_solve(lp, lc, lv, b):
+    n = len(lp) - 1
+    x = np.zeros(n)
+    lib().orc_ic0_solve(n, _p(_i(lp)), _p(_i(lc)), _p(_d(lv)), _p(_d(b)), _p(x))
+    return x
+
+
+def pcg(ptr, col, val, b, tol_abs, max_iter=100000, factor=None, x0=None):
+    """Jacobi (factor None) or IC(0) PCG, FP64.  Returns (x, iters)."""
+    n = len(ptr) - 1
+    x = np.zeros(n)
+    if factor is None:
+        lp = lc = lv = None
+        pc = 0
+    else:
+        lp, lc, lv = (_i(factor[0]), _i(factor[1]), _d(factor[2]))
+        pc = 1
+    it = lib().orc_pcg(n, _p(_i(ptr)), _p(_i(col)), _p(_d(val)), pc,
+                       _p(lp) if pc else None, _p(lc) if pc else None, _p(lv) if pc else None,
+                       _p(_d(b)), _p(_d(x0)) if x0 is not None else None, max_iter, C.c_double(tol_abs),
+                       _p(x))
+    return x, it
+
+
+def permute_csr(ptr, col, val, perm):
+    n = len(ptr) - 1
+    optr = np.zeros(n + 1, np.int32)
+    ocol = np.zeros(len(col), np.int32)
+    oval = np.zeros(len(col))
+    lib().orc_permute_csr(n, _p(_i(ptr)), _p(_i(col)), _p(_d(val)), _p(_i(perm)), _p(optr),
+                          _p(ocol), _p(oval))
+    return optr, ocol, oval
+
+
+def greedy_color(ptr, col):
+    n = len(ptr) - 1
+    color = np.zeros(n, np.int32)
+    nc = lib().orc_greedy_color(n, _p(_i(ptr)), _p(_i(col)), _p(color))
+    return color, nc
+
+
+# -------------------------------------------------------------------- dense --
+def dense_solve(A, b):
+    A = _d(A)
+    n = A.shape[0]
+    x = np.zeros(n)
+    lib().orc_dense_solve(n, _p(A), _p(_d(b)), _p(x))
+    return x
+
+
+def dense_solve_multi(A, B):
+    A, B = _d(A), _d(B)
+    n, k = B.shape
+    X = np.zeros((n, k))
+    lib().orc_dense_solve_multi(n, k, _p(A), _p(B), _p(X))
+    return X
+
+
+def lu_factor(A):
+    A = _d(A)
+    n = A.shape[0]
+    a = np.zeros_like(A)
+    piv = np.zeros(n, np.int32)
+    lib().orc_lu_factor(n, _p(A), _p(a), _p(piv))
+    return a, piv
+
+
+def lu_solve(a, piv, b):
+    n = a.shape[0]
+    x = np.zeros(n)
+    lib().orc_lu_solve(n, _p(_d(a)), _p(_i(piv)), _p(_d(b)), _p(x))
+    return x
+
+
+def cholesky_factor(A):
+    A = _d(A)
+    L = np.zeros_like(A)
+    lib().orc_cholesky_factor(A.shape[0], _p(A), _p(L))
+    return L
+
+
+def cholesky_solve(L, b):
+    L = _d(L)
+    x = np.zeros(L.shape[0])
+    lib().orc_cholesky_solve(L.shape[0], _p(L), _p(_d(b)), _p(x))
+    return x
+
+
+# ---------------------------------------------------------------------- HSC --
+def hsc_eliminate_vertex(edges, v):
+    """edges: list of (u, w, weight) with u < w."""
+    e = list(edges)
+    ne = len(e)
+    eu = _i([a for a, _, _ in e] or [0])
+    ev = _i([b for _, b, _ in e] or [0])
+    ew = _d([c for _, _, c in e] or [0.0])
+    cap = ne + ne * ne + 4
+    ou, ov, ow = np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(cap)
+    m = lib().orc_hsc_eliminate_vertex(ne, _p(eu), _p(ev), _p(ew), v, _p(ou), _p(ov), _p(ow))
+    return [(int(ou[i]), int(ov[i]), float(ow[i])) for i in range(m)]
+
+
+def hsc_edge_weight(edges, a, b):
+    e = list(edges)
+    eu = _i([x for x, _, _ in e] or [0])
+    ev = _i([y for _, y, _ in e] or [0])
+    ew = _d([z for _, _, z in e] or [0.0])
+    return lib().orc_hsc_edge_weight(len(e), _p(eu), _p(ev), _p(ew), a, b)
+
+
+def hsc_degree_weight(edges, v):
+    e = list(edges)
+    eu = _i([x for x, _, _ in e] or [0])
+    ev = _i([y for _, y, _ in e] or [0])
+    ew = _d([z for _, _, z in e] or [0.0])
+    return lib().orc_hsc_degree_weight(len(e), _p(eu), _p(ev), _p(ew), v)

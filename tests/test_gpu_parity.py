@@ -1,0 +1,333 @@
+"""GPU parity: the gfx950 library against the FP64 oracle on identical inputs.
+
+Integer/index outputs and the FP64 assembly, SpMV, IC(0) factor and apply
+are compared bit for bit (np.array_equal); FP32 kernels and solves against
+the stated tolerances."""
+import numpy as np
+import pytest
+
+import oracle as O
+from kat_data import (DIAG234, EQ_TRI, IDENTITY3, LAPLACIAN4, QUAD, SINGLE_TRI, SPD2, STRIP,
+                      TET, TRI3, TRIANGLE, csr_np, he)
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh(name):
+    from tool_godot_curvenet_amd import meshgen as G
+    return G.config_mesh(name)
+
+
+def _random_mesh(seed, n=6):
+    """Jittered n x n grid patch (open boundary): small enough for the
+    literal O(F^2) spec builder."""
+    rng = np.random.default_rng(seed)
+    tris = []
+    for r in range(n - 1):
+        for c in range(n - 1):
+            a, b, d, e = r * n + c, r * n + c + 1, (r + 1) * n + c, (r + 1) * n + c + 1
+            if rng.random() < 0.5:
+                tris += [(a, b, e), (a, e, d)]
+            else:
+                tris += [(a, b, d), (b, e, d)]
+    order = rng.permutation(len(tris))
+    tris = np.asarray(tris, np.int32)[order]
+    y, x = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    pos = np.stack([x.ravel(), y.ravel(), np.zeros(n * n)], 1) + rng.uniform(-0.2, 0.2, (n * n, 3))
+    return n * n, tris, pos
+
+
+# ------------------------------------------------------------- halfedges --
+@pytest.mark.parametrize("case", [SINGLE_TRI, STRIP, TET])
+def test_halfedges_spec_kats(gpu, case):
+    from tool_godot_curvenet_amd import from_triangles
+    nv, tris = case
+    g = from_triangles(nv, tris)
+    o = O.he_from_triangles(nv, tris, spec=True)
+    for k in ("target", "twin", "next", "face"):
+        assert np.array_equal(g[k], o[k]), k
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_halfedges_random_patch(gpu, seed):
+    from tool_godot_curvenet_amd import from_triangles
+    nv, tris, _ = _random_mesh(seed)
+    g = from_triangles(nv, tris)
+    o = O.he_from_triangles(nv, tris, spec=True)
+    for k in ("target", "twin", "next", "face"):
+        assert np.array_equal(g[k], o[k]), k
+    assert O.he_manifold(g) == 15
+
+
+@pytest.mark.parametrize("name", ["cylinder64", "capsules100k"])
+def test_halfedges_config_meshes(gpu, name):
+    from tool_godot_curvenet_amd import from_triangles
+    m = _mesh(name)
+    g = from_triangles(m.nv, m.tris)
+    o = O.he_from_triangles(m.nv, m.tris)
+    for k in ("target", "twin", "next", "face"):
+        assert np.array_equal(g[k], o[k]), k
+
+
+# -------------------------------------------------------------- seam cut --
+@pytest.mark.parametrize("name", ["cylinder64", "icosphere6", "capsules100k"])
+def test_cut_seams_bit_exact(gpu, name):
+    from tool_godot_curvenet_amd import cut_seams
+    m = _mesh(name)
+    nt, origin, con, nv2 = cut_seams(m.nv, m.tris, m.cut_edges)
+    ont, oorigin, ocon, onv2 = O.cut_edges(m.nv, m.tris, m.cut_edges)
+    assert nv2 == onv2
+    assert np.array_equal(nt, ont)
+    assert np.array_equal(origin, oorigin)
+    assert np.array_equal(con, ocon)
+
+
+def test_cut_cylinder_closed_form(gpu):
+    """Seam ring 32 of the 64x64 cylinder: 64 copies at nv + j in ring order."""
+    from tool_godot_curvenet_amd import cut_seams
+    m = _mesh("cylinder64")
+    nt, origin, con, nv2 = cut_seams(m.nv, m.tris, m.cut_edges)
+    assert nv2 == 4096 + 64
+    assert np.array_equal(origin[4096:], 32 * 64 + np.arange(64))
+    assert con.sum() == 128
+
+
+def test_cut_bad_edge_rejected(gpu):
+    from tool_godot_curvenet_amd import cut_seams
+    from tool_godot_curvenet_amd._lib import CurvenetError
+    m = _mesh("cylinder64")
+    with pytest.raises(CurvenetError):
+        cut_seams(m.nv, m.tris, np.array([[0, 2000]], np.int32))
+
+
+# -------------------------------------------------------------- Laplacian --
+def _system(name):
+    from tool_godot_curvenet_amd import CutLaplacian
+    m = _mesh(name)
+    lap = CutLaplacian.from_mesh(m)
+    return m, lap
+
+
+def _oracle_system(lap):
+    h = O.he_from_triangles(lap.pos.shape[0], lap.tris)
+    return h, O.cut_laplacian_csr(h, lap.constrained, lap.pos)
+
+
+@pytest.mark.parametrize("name", ["cylinder64", "icosphere6"])
+def test_cut_laplacian_bit_exact(gpu, name):
+    m, lap = _system(name)
+    h, o = _oracle_system(lap)
+    assert lap.n_free == o["n_free"]
+    A = lap.A_ff()
+    for got, want in zip((A.row_ptr, A.col_idx, A.values), o["A_ff"]):
+        assert np.array_equal(got, want)
+    B = lap.A_fc()
+    for got, want in zip((B.row_ptr, B.col_idx, B.values), o["A_fc"]):
+        assert np.array_equal(got, want)
+    free_of, con_of = lap.maps()
+    assert np.array_equal(free_of, o["free_of"]) and np.array_equal(con_of, o["con_of"])
+    he_g = lap.halfedges()
+    for k in ("target", "twin", "next", "face"):
+        assert np.array_equal(he_g[k], h[k])
+
+
+def test_cut_laplacian_matches_dense_spec(gpu):
+    """Small cut patch: GPU CSR == the literal dense assembleVtLhV."""
+    from tool_godot_curvenet_amd import CutLaplacian, cut_seams
+    nv, tris, pos = _random_mesh(7, n=7)
+    # cut along the middle row, interior edges only
+    cut = np.array([(3 * 7 + c, 3 * 7 + c + 1) for c in range(1, 5)], np.int32)
+    nt, origin, con, nv2 = cut_seams(nv, tris, cut)
+    lap = CutLaplacian(pos[origin], nt, con)
+    h = O.he_from_triangles(nv2, nt, spec=True)
+    D = O.assemble_vtlhv_dense(h, con, pos[origin])
+    free = np.flatnonzero(con == 0)
+    Aff = D[np.ix_(free, free)]
+    A = lap.A_ff().dense()
+    assert np.array_equal(A, Aff)
+
+
+@pytest.mark.parametrize("name", ["cylinder64", "icosphere6"])
+def test_harmonic_rhs_bit_exact(gpu, name):
+    from tool_godot_curvenet_amd import meshgen as G
+    m, lap = _system(name)
+    h, o = _oracle_system(lap)
+    Xc = G.seeded_boundary(lap.n_con, 3)
+    got = lap.harmonic_rhs(Xc)
+    want = O.harmonic_rhs_csr(h, o["free_of"], o["con_of"], lap.pos, Xc)
+    assert np.array_equal(got, want)
+
+
+# ------------------------------------------------------------ SpMV / SpMM --
+@pytest.mark.parametrize("k", [1, 3, 8])
+def test_spmm_f64_bit_exact(gpu, k):
+    from tool_godot_curvenet_amd import spmm
+    m, lap = _system("cylinder64")
+    A = lap.A_ff()
+    X = np.random.default_rng(k).standard_normal((A.rows, k))
+    got = spmm(A, X, "f64")
+    want = O.spmm(A.row_ptr, A.col_idx, A.values, X)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("k", [1, 3, 4, 64, 256])
+def test_spmm_f32(gpu, k):
+    from tool_godot_curvenet_amd import spmm
+    m, lap = _system("cylinder64")
+    A = lap.A_ff()
+    X = np.random.default_rng(k).standard_normal((A.rows, k)).astype(np.float32)
+    got = spmm(A, X, "f32").astype(np.float64)
+    want = O.spmm(A.row_ptr, A.col_idx, A.values, X.astype(np.float64))
+    absA = O.spmm(A.row_ptr, A.col_idx, np.abs(A.values), np.abs(X).astype(np.float64))
+    # FP32 values and FMA accumulation: |err| <= c * eps32 * (|A||x|)
+    assert np.all(np.abs(got - want) <= 1e-5 * absA + 1e-30)
+
+
+def test_spmv_kat(gpu):
+    from tool_godot_curvenet_amd import CsrMatrix, spmv
+    p, c, v = csr_np(IDENTITY3)
+    A = CsrMatrix(3, 3, p, c, v)
+    assert np.array_equal(spmv(A, [1.0, 2.0, 3.0]), [1.0, 2.0, 3.0])
+
+
+# ------------------------------------------------------------------ IC(0) --
+@pytest.mark.parametrize("ordering", ["color", "natural"])
+def test_ic0_factor_bit_exact(gpu, ordering):
+    from tool_godot_curvenet_amd import IC0
+    m, lap = _system("cylinder64")
+    f = IC0(lap, ordering)
+    A = lap.A_ff()
+    perm = f.perm()
+    if ordering == "color":
+        color, nc = O.greedy_color(A.row_ptr, A.col_idx)
+        assert f.info.nlevels == nc
+        assert np.array_equal(perm, np.argsort(color, kind="stable"))
+    else:
+        assert np.array_equal(perm, np.arange(A.rows)) or f.info.nlevels > 1
+    pa = f.permuted_matrix()
+    optr, ocol, oval = O.permute_csr(A.row_ptr, A.col_idx, A.values, perm)
+    assert np.array_equal(pa.row_ptr, optr) and np.array_equal(pa.col_idx, ocol)
+    assert np.array_equal(pa.values, oval)
+    assert f.info.shift == 0.0
+    lp, lc, lv = O.ic0_factor(optr, ocol, oval)
+    gp, gc, gv, gd = f.factor()
+    # oracle keeps the diagonal last in each row; the GPU stores it apart
+    n = A.rows
+    strict = np.concatenate([np.arange(lp[i], lp[i + 1] - 1) for i in range(n)])
+    assert np.array_equal(gc, lc[strict]) and np.array_equal(gv, lv[strict])
+    assert np.array_equal(gd, lv[lp[1:] - 1])
+
+
+def test_ic0_apply_bit_exact(gpu):
+    from tool_godot_curvenet_amd import IC0
+    m, lap = _system("cylinder64")
+    f = IC0(lap, "color")
+    A = lap.A_ff()
+    perm = f.perm()
+    optr, ocol, oval = O.permute_csr(A.row_ptr, A.col_idx, A.values, perm)
+    lp, lc, lv = O.ic0_factor(optr, ocol, oval)
+    r = np.random.default_rng(5).standard_normal((A.rows, 3))
+    got = f.apply(r)
+    for c in range(3):
+        want_p = O.ic0_solve(lp, lc, lv, r[perm, c])
+        want = np.empty_like(want_p)
+        want[perm] = want_p
+        assert np.array_equal(got[:, c], want)
+
+
+# -------------------------------------------------------------------- PCG --
+def _relres(A, X, B):
+    R = B - O.spmm(A.row_ptr, A.col_idx, A.values, X)
+    return np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64", "fp64_plain"])
+@pytest.mark.parametrize("precond", ["ic0", "jacobi"])
+def test_pcg_cylinder(gpu, precision, precond):
+    from tool_godot_curvenet_amd import meshgen as G
+    m, lap = _system("cylinder64")
+    A = lap.A_ff()
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, 3))
+    x, info = lap.solve(B, precond=precond, precision=precision, tol=1e-9)
+    X = x.cpu().numpy()
+    assert info.status == 0, info.status
+    rr = _relres(A, X, B)
+    assert np.all(rr <= 1e-9), rr
+    assert info.relres_max <= 1e-9
+
+
+def test_pcg_solution_matches_oracle(gpu):
+    """Positions within 1e-9 relative of the oracle's FP64 IC(0)-PCG."""
+    from tool_godot_curvenet_amd import meshgen as G
+    m, lap = _system("cylinder64")
+    A = lap.A_ff()
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, 3))
+    x, info = lap.solve(B, precond="ic0", precision="mixed", tol=1e-12)
+    X = x.cpu().numpy()
+    lp, lc, lv = O.ic0_factor(A.row_ptr, A.col_idx, A.values)
+    for c in range(3):
+        xo, it = O.pcg(A.row_ptr, A.col_idx, A.values, B[:, c],
+                       1e-13 * np.linalg.norm(B[:, c]), factor=(lp, lc, lv))
+        err = np.linalg.norm(X[:, c] - xo) / np.linalg.norm(xo)
+        assert err <= 1e-9, err
+
+
+def test_pcg_wide_k(gpu):
+    from tool_godot_curvenet_amd import meshgen as G
+    m, lap = _system("cylinder64")
+    A = lap.A_ff()
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, 96, seed=3))
+    x, info = lap.solve(B, precond="ic0", precision="mixed", tol=1e-9)
+    assert info.status == 0
+    assert np.all(_relres(A, x.cpu().numpy(), B) <= 1e-9)
+
+
+def test_pcg_warm_start_zero_iters(gpu):
+    """cgWithGuess seeded with the solution converges in 0 iterations
+    (SparseLinAlg.lean:272-277)."""
+    from tool_godot_curvenet_amd import meshgen as G
+    m, lap = _system("cylinder64")
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, 3))
+    x, _ = lap.solve(B, tol=1e-11)
+    x2, info = lap.solve(B, tol=1e-9, x0=x)
+    assert info.iters == 0 and info.status == 0
+
+
+@pytest.mark.parametrize("A,b,expect,maxit,eps", [
+    (IDENTITY3, [1.0, 2.0, 3.0], [1.0, 2.0, 3.0], 10, 1e-9),
+    (DIAG234, [4.0, 9.0, 16.0], [2.0, 3.0, 4.0], 10, 1e-9),
+    (SPD2, [3.0, 3.0], [1.0, 1.0], 50, 1e-8),
+    (LAPLACIAN4, [1.0, 0.0, 0.0, 0.0], [0.8, 0.6, 0.4, 0.2], 200, 1e-7),
+])
+def test_cg_spec_kats_on_gpu(gpu, A, b, expect, maxit, eps):   # SparseLinAlg.lean:171-246
+    from tool_godot_curvenet_amd import CsrMatrix, cg
+    p, c, v = csr_np(A)
+    x = cg(CsrMatrix(len(p) - 1, len(p) - 1, p, c, v), b, maxit, 1e-12)
+    assert np.all(np.abs(x - np.asarray(expect)) < eps)
+
+
+def test_cg_solver_kernels_feature_on_gpu(gpu):                # SolverKernels.lean:9-18
+    from tool_godot_curvenet_amd import CsrMatrix, cg
+    p, c, v = csr_np(TRI3)
+    b = np.array([1.0, 0.0, -1.0])
+    x = cg(CsrMatrix(3, 3, p, c, v), b, 100, 1e-12)
+    r = b - O.spmv(p, c, v, x)
+    assert O.dot(r, r) < 1e-9
+
+
+def test_harmonic_solve_triangle_with_sample(gpu):             # HarmonicSolve.lean:104-115
+    from tool_godot_curvenet_amd import CutLaplacian, harmonic_solve
+    lap = CutLaplacian(EQ_TRI, [(0, 1, 2)], [1, 0, 0])
+    X, info = harmonic_solve(lap, [[5.0, 10.0, 15.0]], precond="jacobi", precision="fp64")
+    assert np.all(np.abs(X[1] - [5.0, 10.0, 15.0]) < 1e-9)
+    assert np.all(np.abs(X[2] - [5.0, 10.0, 15.0]) < 1e-9)
+    assert np.all(X[0] == 0.0)
+
+
+# ------------------------------------------------------------- projection --
+def test_project_to_vertices(gpu):
+    from tool_godot_curvenet_amd import project_to_vertices
+    m = _mesh("cylinder64")
+    q = np.random.default_rng(3).uniform(-1, 1, (200, 3))
+    q[:5] = m.pos[[0, 5, 5, 77, 4000]]                 # exact hits incl. repeats
+    assert np.array_equal(project_to_vertices(q, m.pos), O.project_to_vertices(q, m.pos))

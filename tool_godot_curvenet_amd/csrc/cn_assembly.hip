@@ -1,0 +1,480 @@
+// cn_assembly.hip -- cut-mesh cotangent Laplacian VtLhV as CSR on gfx950.
+//
+// Compiled with -ffp-contract=off: every FP64 value here must equal the
+// spec's (CutMeshLaplacian.lean:56-93, PolygonLaplacian.lean:32-93) bit for
+// bit, so no multiply-add may be fused.
+//
+// Layout: per face 9 doubles (the triangle cot Laplacian of the face loop
+// that starts at halfedge 3f, whose targets are t1, t2, t0); per vertex the
+// ascending list of incoming interior halfedges; per free row the sorted
+// column list.  A row's values are a segmented reduction over its incoming
+// halfedges in ascending order -- one thread per row walks them, so each
+// entry is summed in exactly the spec's order without atomics.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "cn_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace cn {
+
+__device__ __forceinline__ int32_t tgt3(const int32_t *tris, int32_t h) {
+  int32_t f = h / 3, l = h - 3 * f;
+  return tris[3 * f + (l == 2 ? 0 : l + 1)];
+}
+
+// PolygonLaplacian.cotAtVertex (:32-40)
+__device__ __forceinline__ double cot_at(const double *ap, const double *o1,
+                                         const double *o2) {
+  double ax = o1[0] - ap[0], ay = o1[1] - ap[1], az = o1[2] - ap[2];
+  double bx = o2[0] - ap[0], by = o2[1] - ap[1], bz = o2[2] - ap[2];
+  double dot = ax * bx + ay * by + az * bz;
+  double cx = ay * bz - az * by;
+  double cy = az * bx - ax * bz;
+  double cz = ax * by - ay * bx;
+  double cl = sqrt(cx * cx + cy * cy + cz * cz);
+  return dot / cl;
+}
+
+// triangleCotLaplacian (:56-71) on the face loop (t1, t2, t0)
+__global__ void k_face_cot(int32_t nf, const int32_t *tris, const double *pos,
+                           double *m) {
+  int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  double a[3], b[3], c[3];
+  int32_t ia = tris[3 * f + 1], ib = tris[3 * f + 2], ic = tris[3 * f];
+  for (int d = 0; d < 3; ++d) {
+    a[d] = pos[3 * ia + d];
+    b[d] = pos[3 * ib + d];
+    c[d] = pos[3 * ic + d];
+  }
+  double cotA = cot_at(a, b, c), cotB = cot_at(b, a, c), cotC = cot_at(c, a, b);
+  double w_bc = cotA * 0.5, w_ac = cotB * 0.5, w_ab = cotC * 0.5;
+  double *o = m + 9 * (int64_t)f;
+  o[0] = w_ab + w_ac; o[1] = -w_ab;       o[2] = -w_ac;
+  o[3] = -w_ab;       o[4] = w_ab + w_bc; o[5] = -w_bc;
+  o[6] = -w_ac;       o[7] = -w_bc;       o[8] = w_ac + w_bc;
+}
+
+__global__ void k_flags(int32_t nv, const uint8_t *constrained, int32_t *fr,
+                        int32_t *co) {
+  int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  fr[v] = constrained[v] ? 0 : 1;
+  co[v] = constrained[v] ? 1 : 0;
+}
+
+__global__ void k_maps(int32_t nv, const uint8_t *constrained, int32_t *free_of,
+                       int32_t *con_of, int32_t *free_vertex,
+                       int32_t *con_vertex) {
+  int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  if (constrained[v]) {
+    con_vertex[con_of[v]] = v;
+    free_of[v] = -1;
+  } else {
+    free_vertex[free_of[v]] = v;
+    con_of[v] = -1;
+  }
+}
+
+__global__ void k_cand_count(int32_t nv, const int32_t *iptr, int32_t *cnt) {
+  int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  cnt[v] = 1 + 2 * (iptr[v + 1] - iptr[v]);
+}
+
+// Candidate columns of row v: v and the other corners of every face around
+// v (the faces of its incoming halfedges); sorted and de-duplicated in
+// place.  Counts go to the free row's slot.
+__global__ void k_cand_fill(int32_t nv, const int32_t *tris,
+                            const int32_t *iptr, const int32_t *ihe,
+                            const int32_t *cptr, int32_t *cand,
+                            const uint8_t *constrained, const int32_t *free_of,
+                            int32_t *ucnt, int32_t *cnt_ff, int32_t *cnt_fc) {
+  int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  int32_t *c = cand + cptr[v];
+  int32_t n = 0;
+  c[n++] = v;
+  for (int32_t a = iptr[v]; a < iptr[v + 1]; ++a) {
+    int32_t p = ihe[a], f = p / 3;
+    for (int l = 0; l < 3; ++l) {
+      int32_t w = tris[3 * f + l];
+      if (w != v) c[n++] = w;
+    }
+  }
+  for (int32_t i = 1; i < n; ++i) {
+    int32_t x = c[i], j = i - 1;
+    while (j >= 0 && c[j] > x) { c[j + 1] = c[j]; --j; }
+    c[j + 1] = x;
+  }
+  int32_t u = 0, nff = 0, nfc = 0;
+  for (int32_t i = 0; i < n; ++i)
+    if (i == 0 || c[i] != c[i - 1]) {
+      c[u++] = c[i];
+      if (constrained[c[i]]) ++nfc; else ++nff;
+    }
+  ucnt[v] = u;
+  int32_t r = free_of[v];
+  if (r >= 0) { cnt_ff[r] = nff; cnt_fc[r] = nfc; }
+}
+
+__global__ void k_pattern_fill(int32_t n_free, const int32_t *free_vertex,
+                               const int32_t *cptr, const int32_t *cand,
+                               const int32_t *ucnt, const uint8_t *constrained,
+                               const int32_t *free_of, const int32_t *con_of,
+                               const int32_t *ff_ptr, const int32_t *fc_ptr,
+                               int32_t *ff_col, int32_t *fc_col) {
+  int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_free) return;
+  int32_t v = free_vertex[r];
+  const int32_t *c = cand + cptr[v];
+  int32_t a = ff_ptr[r], b = fc_ptr[r];
+  for (int32_t i = 0; i < ucnt[v]; ++i) {
+    int32_t w = c[i];
+    if (constrained[w]) fc_col[b++] = con_of[w];
+    else ff_col[a++] = free_of[w];
+  }
+}
+
+__device__ __forceinline__ int32_t find_col(const int32_t *col, int32_t a,
+                                            int32_t b, int32_t key) {
+  while (a < b) {
+    int32_t m = (a + b) >> 1;
+    if (col[m] < key) a = m + 1; else b = m;
+  }
+  return a;
+}
+
+// Row values: for p in inc(v) ascending, for each distinct target j of p's
+// face: t = sum over q (ascending) targeting j of Lh[p][q]; entry(j) += t.
+__global__ void k_row_values(int32_t n_free, const int32_t *free_vertex,
+                             const int32_t *tris, const int32_t *iptr,
+                             const int32_t *ihe, const double *m,
+                             const uint8_t *constrained,
+                             const int32_t *free_of, const int32_t *con_of,
+                             const int32_t *ff_ptr, const int32_t *ff_col,
+                             double *ff_val, const int32_t *fc_ptr,
+                             const int32_t *fc_col, double *fc_val) {
+  int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_free) return;
+  int32_t v = free_vertex[r];
+  int32_t a0 = ff_ptr[r], a1 = ff_ptr[r + 1], b0 = fc_ptr[r], b1 = fc_ptr[r + 1];
+  for (int32_t i = a0; i < a1; ++i) ff_val[i] = 0.0;
+  for (int32_t i = b0; i < b1; ++i) fc_val[i] = 0.0;
+  for (int32_t a = iptr[v]; a < iptr[v + 1]; ++a) {
+    int32_t p = ihe[a], f = p / 3, lp = p - 3 * f;
+    const double *mf = m + 9 * (int64_t)f + 3 * lp;
+    int32_t tj[3] = {tgt3(tris, 3 * f), tgt3(tris, 3 * f + 1), tgt3(tris, 3 * f + 2)};
+    for (int lq = 0; lq < 3; ++lq) {
+      int32_t j = tj[lq];
+      bool seen = false;
+      for (int e = 0; e < lq; ++e) seen |= (tj[e] == j);
+      if (seen) continue;
+      double t = 0.0;
+      for (int e = lq; e < 3; ++e)
+        if (tj[e] == j) t = t + (0.0 + mf[e]);
+      if (constrained[j]) {
+        int32_t pos = find_col(fc_col, b0, b1, con_of[j]);
+        fc_val[pos] = fc_val[pos] + t;
+      } else {
+        int32_t pos = find_col(ff_col, a0, a1, free_of[j]);
+        ff_val[pos] = ff_val[pos] + t;
+      }
+    }
+  }
+}
+
+__global__ void k_mirror(int64_t n, const double *x, float *y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+
+__global__ void k_diag(int32_t n, const int32_t *ptr, const int32_t *col,
+                       const double *val, double *d) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = 0.0;
+  for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p)
+    if (col[p] == i) { x = val[p]; break; }
+  d[i] = x;
+}
+
+// RHS of HarmonicSolve.solveMulti on the free rows (spec summation order).
+__global__ void k_harmonic_rhs(int32_t n_free, int32_t k,
+                               const int32_t *free_vertex, const int32_t *tris,
+                               const int32_t *iptr, const int32_t *ihe,
+                               const double *m, const int32_t *con_of,
+                               const double *Xc, double *rhs) {
+  int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)n_free * k) return;
+  int32_t r = (int32_t)(id / k), c = (int32_t)(id - (int64_t)r * k);
+  int32_t v = free_vertex[r];
+  double s = 0.0;
+  for (int32_t a = iptr[v]; a < iptr[v + 1]; ++a) {
+    int32_t p = ihe[a], f = p / 3, lp = p - 3 * f;
+    const double *mf = m + 9 * (int64_t)f + 3 * lp;
+    double t = 0.0;
+    for (int lq = 0; lq < 3; ++lq) {
+      int32_t cc = con_of[tgt3(tris, 3 * f + lq)];
+      if (cc >= 0) t = t + (0.0 + mf[lq]) * Xc[(int64_t)cc * k + c];
+    }
+    s = s + t;
+  }
+  rhs[id] = -s;
+}
+
+static void assemble_values(cn_system_t *S, hipStream_t s) {
+  if (S->nf > 0)
+    k_face_cot<<<grid_for(S->nf, 256), 256, 0, s>>>(S->nf, S->tris.get(),
+                                                   S->pos.get(),
+                                                   S->face_m.get());
+  if (S->n_free > 0)
+    k_row_values<<<grid_for(S->n_free, 128), 128, 0, s>>>(
+        S->n_free, S->free_vertex.get(), S->tris.get(), S->inc.ptr.get(),
+        S->inc.he.get(), S->face_m.get(), S->constrained.get(),
+        S->free_of.get(), S->con_of.get(), S->ff_ptr.get(), S->ff_col.get(),
+        S->ff_val.get(), S->fc_ptr.get(), S->fc_col.get(), S->fc_val.get());
+  if (S->nnz_ff > 0)
+    k_mirror<<<grid_for(S->nnz_ff, 256), 256, 0, s>>>(S->nnz_ff, S->ff_val.get(),
+                                                     S->ff_val32.get());
+  if (S->nnz_fc > 0)
+    k_mirror<<<grid_for(S->nnz_fc, 256), 256, 0, s>>>(S->nnz_fc, S->fc_val.get(),
+                                                     S->fc_val32.get());
+  if (S->n_free > 0)
+    k_diag<<<grid_for(S->n_free, 256), 256, 0, s>>>(S->n_free, S->ff_ptr.get(),
+                                                   S->ff_col.get(),
+                                                   S->ff_val.get(),
+                                                   S->diag.get());
+  CN_HIP(hipGetLastError());
+  CN_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace cn
+
+using namespace cn;
+
+extern "C" {
+
+cn_status cn_system_create(int32_t nv, int32_t nf, const int32_t *d_tris,
+                           const uint8_t *d_constrained, const double *d_pos,
+                           cn_system *out, void *stream) {
+  CN_API_BEGIN
+  if (!out || nv <= 0 || nf < 0 || !d_constrained || !d_pos || (nf > 0 && !d_tris) ||
+      nf > INT_MAX / 6) {
+    set_error("cn_system_create: bad arguments");
+    return CN_BAD_ARG;
+  }
+  *out = nullptr;
+  hipStream_t s = as_stream(stream);
+  cn_system_t *S = new cn_system_t();
+  try {
+    S->nv = nv;
+    S->nf = nf;
+    S->tris.alloc((size_t)3 * nf + 1);
+    S->constrained.alloc((size_t)nv);
+    S->pos.alloc((size_t)3 * nv);
+    if (nf > 0)
+      CN_HIP(hipMemcpyAsync(S->tris.get(), d_tris, sizeof(int32_t) * 3 * (size_t)nf,
+                            hipMemcpyDeviceToDevice, s));
+    CN_HIP(hipMemcpyAsync(S->constrained.get(), d_constrained, (size_t)nv,
+                          hipMemcpyDeviceToDevice, s));
+    CN_HIP(hipMemcpyAsync(S->pos.get(), d_pos, sizeof(double) * 3 * (size_t)nv,
+                          hipMemcpyDeviceToDevice, s));
+    {  // index range check
+      DevBuf<int32_t> dummy_t((size_t)6 * nf + 1), dummy_w((size_t)6 * nf + 1),
+          dummy_n((size_t)6 * nf + 1), dummy_f((size_t)6 * nf + 1);
+      int32_t nh = 0;
+      cn_status st = cn_halfedge_from_triangles(nv, nf, S->tris.get(), dummy_t.get(),
+                                                dummy_w.get(), dummy_n.get(),
+                                                dummy_f.get(), &nh, stream);
+      if (st != CN_OK) { delete S; return st; }
+      S->he.nv = nv;
+      S->he.nf = nf;
+      S->he.nh = nh;
+      S->he.target = std::move(dummy_t);
+      S->he.twin = std::move(dummy_w);
+      S->he.next = std::move(dummy_n);
+      S->he.face = std::move(dummy_f);
+    }
+    build_incoming(nv, nf, S->tris.get(), S->inc, s);
+    // free / constrained maps
+    S->free_of.alloc((size_t)nv + 1);
+    S->con_of.alloc((size_t)nv + 1);
+    k_flags<<<grid_for(nv, 256), 256, 0, s>>>(nv, S->constrained.get(),
+                                             S->free_of.get(), S->con_of.get());
+    CN_HIP(hipGetLastError());
+    S->n_free = (int32_t)exclusive_scan_i32(S->free_of.get(), nv, s);
+    S->n_con = (int32_t)exclusive_scan_i32(S->con_of.get(), nv, s);
+    S->free_vertex.alloc((size_t)S->n_free + 1);
+    S->con_vertex.alloc((size_t)S->n_con + 1);
+    k_maps<<<grid_for(nv, 256), 256, 0, s>>>(nv, S->constrained.get(),
+                                            S->free_of.get(), S->con_of.get(),
+                                            S->free_vertex.get(),
+                                            S->con_vertex.get());
+    CN_HIP(hipGetLastError());
+    // pattern
+    DevBuf<int32_t> cptr((size_t)nv + 1), ucnt((size_t)nv + 1);
+    k_cand_count<<<grid_for(nv, 256), 256, 0, s>>>(nv, S->inc.ptr.get(), cptr.get());
+    int64_t ncand = exclusive_scan_i32(cptr.get(), nv, s);
+    DevBuf<int32_t> cand((size_t)ncand + 1);
+    S->ff_ptr.alloc((size_t)S->n_free + 1);
+    S->fc_ptr.alloc((size_t)S->n_free + 1);
+    CN_HIP(hipMemsetAsync(S->ff_ptr.get(), 0, sizeof(int32_t) * ((size_t)S->n_free + 1), s));
+    CN_HIP(hipMemsetAsync(S->fc_ptr.get(), 0, sizeof(int32_t) * ((size_t)S->n_free + 1), s));
+    k_cand_fill<<<grid_for(nv, 128), 128, 0, s>>>(
+        nv, S->tris.get(), S->inc.ptr.get(), S->inc.he.get(), cptr.get(),
+        cand.get(), S->constrained.get(), S->free_of.get(), ucnt.get(),
+        S->ff_ptr.get(), S->fc_ptr.get());
+    CN_HIP(hipGetLastError());
+    S->nnz_ff = exclusive_scan_i32(S->ff_ptr.get(), (int64_t)S->n_free + 1, s);
+    S->nnz_fc = exclusive_scan_i32(S->fc_ptr.get(), (int64_t)S->n_free + 1, s);
+    if (S->nnz_ff > INT_MAX || S->nnz_fc > INT_MAX) {
+      delete S;
+      set_error("nnz exceeds int32");
+      return CN_BAD_ARG;
+    }
+    S->ff_col.alloc((size_t)S->nnz_ff + 1);
+    S->fc_col.alloc((size_t)S->nnz_fc + 1);
+    S->ff_val.alloc((size_t)S->nnz_ff + 1);
+    S->fc_val.alloc((size_t)S->nnz_fc + 1);
+    S->ff_val32.alloc((size_t)S->nnz_ff + 1);
+    S->fc_val32.alloc((size_t)S->nnz_fc + 1);
+    S->diag.alloc((size_t)S->n_free + 1);
+    S->face_m.alloc((size_t)9 * nf + 1);
+    if (S->n_free > 0)
+      k_pattern_fill<<<grid_for(S->n_free, 256), 256, 0, s>>>(
+          S->n_free, S->free_vertex.get(), cptr.get(), cand.get(), ucnt.get(),
+          S->constrained.get(), S->free_of.get(), S->con_of.get(),
+          S->ff_ptr.get(), S->fc_ptr.get(), S->ff_col.get(), S->fc_col.get());
+    CN_HIP(hipGetLastError());
+    assemble_values(S, s);
+  } catch (...) {
+    delete S;
+    throw;
+  }
+  *out = S;
+  return CN_OK;
+  CN_API_END
+}
+
+cn_status cn_system_from_csr(int32_t n, const int32_t *d_ptr, const int32_t *d_col,
+                             const double *d_val, cn_system *out, void *stream) {
+  CN_API_BEGIN
+  if (!out || n < 0 || (n > 0 && (!d_ptr || !d_col || !d_val))) {
+    set_error("cn_system_from_csr: bad arguments");
+    return CN_BAD_ARG;
+  }
+  *out = nullptr;
+  hipStream_t s = as_stream(stream);
+  int32_t nnz = 0;
+  if (n > 0) {
+    CN_HIP(hipMemcpyAsync(&nnz, d_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CN_HIP(hipStreamSynchronize(s));
+  }
+  if (nnz < 0) { set_error("cn_system_from_csr: negative nnz"); return CN_BAD_ARG; }
+  cn_system_t *S = new cn_system_t();
+  try {
+    S->n_free = n;
+    S->nnz_ff = nnz;
+    S->ff_ptr.alloc((size_t)n + 1);
+    S->ff_col.alloc((size_t)nnz + 1);
+    S->ff_val.alloc((size_t)nnz + 1);
+    S->ff_val32.alloc((size_t)nnz + 1);
+    S->fc_ptr.alloc((size_t)n + 1);
+    S->fc_col.alloc(1);
+    S->fc_val.alloc(1);
+    S->fc_val32.alloc(1);
+    S->diag.alloc((size_t)n + 1);
+    S->free_vertex.alloc((size_t)n + 1);
+    if (n > 0) {
+      CN_HIP(hipMemcpyAsync(S->ff_ptr.get(), d_ptr, sizeof(int32_t) * ((size_t)n + 1),
+                            hipMemcpyDeviceToDevice, s));
+      CN_HIP(hipMemsetAsync(S->fc_ptr.get(), 0, sizeof(int32_t) * ((size_t)n + 1), s));
+    }
+    if (nnz > 0) {
+      CN_HIP(hipMemcpyAsync(S->ff_col.get(), d_col, sizeof(int32_t) * (size_t)nnz,
+                            hipMemcpyDeviceToDevice, s));
+      CN_HIP(hipMemcpyAsync(S->ff_val.get(), d_val, sizeof(double) * (size_t)nnz,
+                            hipMemcpyDeviceToDevice, s));
+      k_mirror<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, S->ff_val.get(), S->ff_val32.get());
+    }
+    if (n > 0)
+      k_diag<<<grid_for(n, 256), 256, 0, s>>>(n, S->ff_ptr.get(), S->ff_col.get(),
+                                             S->ff_val.get(), S->diag.get());
+    CN_HIP(hipGetLastError());
+    CN_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    delete S;
+    throw;
+  }
+  *out = S;
+  return CN_OK;
+  CN_API_END
+}
+
+cn_status cn_system_get_info(cn_system S, cn_system_info *info) {
+  if (!S || !info) { set_error("null handle"); return CN_BAD_ARG; }
+  info->nv = S->nv;
+  info->nf = S->nf;
+  info->nh = S->he.nh;
+  info->n_free = S->n_free;
+  info->n_con = S->n_con;
+  info->nnz_ff = S->nnz_ff;
+  info->nnz_fc = S->nnz_fc;
+  info->d_ff_ptr = S->ff_ptr.get();
+  info->d_ff_col = S->ff_col.get();
+  info->d_ff_val = S->ff_val.get();
+  info->d_ff_val32 = S->ff_val32.get();
+  info->d_fc_ptr = S->fc_ptr.get();
+  info->d_fc_col = S->fc_col.get();
+  info->d_fc_val = S->fc_val.get();
+  info->d_free_of = S->free_of.get();
+  info->d_con_of = S->con_of.get();
+  info->d_free_vertex = S->free_vertex.get();
+  info->d_con_vertex = S->con_vertex.get();
+  info->d_he_target = S->he.target.get();
+  info->d_he_twin = S->he.twin.get();
+  info->d_he_next = S->he.next.get();
+  info->d_he_face = S->he.face.get();
+  return CN_OK;
+}
+
+cn_status cn_system_set_positions(cn_system S, const double *d_pos, void *stream) {
+  CN_API_BEGIN
+  if (!S || !d_pos || S->nv == 0) { set_error("bad arguments (or system without a mesh)"); return CN_BAD_ARG; }
+  hipStream_t s = as_stream(stream);
+  CN_HIP(hipMemcpyAsync(S->pos.get(), d_pos, sizeof(double) * 3 * (size_t)S->nv,
+                        hipMemcpyDeviceToDevice, s));
+  assemble_values(S, s);
+  return CN_OK;
+  CN_API_END
+}
+
+cn_status cn_system_destroy(cn_system S) {
+  delete S;
+  return CN_OK;
+}
+
+cn_status cn_harmonic_rhs(cn_system S, int32_t k, const double *d_Xc,
+                          double *d_rhs, void *stream) {
+  CN_API_BEGIN
+  if (!S || k <= 0 || (S->n_con > 0 && !d_Xc) || !d_rhs || (S->nv == 0 && S->n_free > 0)) {
+    set_error("cn_harmonic_rhs: bad arguments");
+    return CN_BAD_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  int64_t tot = (int64_t)S->n_free * k;
+  if (tot > 0)
+    k_harmonic_rhs<<<grid_for(tot, 256), 256, 0, s>>>(
+        S->n_free, k, S->free_vertex.get(), S->tris.get(), S->inc.ptr.get(),
+        S->inc.he.get(), S->face_m.get(), S->con_of.get(), d_Xc, d_rhs);
+  CN_HIP(hipGetLastError());
+  return CN_OK;
+  CN_API_END
+}
+
+}  // extern "C"

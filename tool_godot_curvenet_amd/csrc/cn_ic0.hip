@@ -1,0 +1,431 @@
+// cn_ic0.hip -- no-fill incomplete Cholesky on gfx950.
+//
+// The factor follows IncompleteCholesky.factor3's recurrences
+// (IncompleteCholesky.lean:49-69) in CSR form (:26-33):
+//   L_ij = (A_ij - sum_{k<j} L_ik L_jk) / L_jj,  L_ii = sqrt(A_ii - sum L_ik^2)
+// with each sum taken over k ascending, FP64, no fused multiply-add, so a
+// factor computed here equals the oracle's bit for bit on the same ordering.
+//
+// Scheduling: rows are renumbered so that every dependency level is a
+// contiguous row range.  CN_ORDER_COLOR uses GraphColoring.greedyColor
+// (GraphColoring.lean:41-56) -- no two rows of a colour are adjacent, so a
+// colour is a level and the whole triangular solve is 2*C passes (C ~ 4-7 on
+// triangle meshes), the colour-pass schedule the reference planned for its
+// IncompleteCholesky kernel (SlangCodegen/IncompleteCholesky.lean:10-12).
+// CN_ORDER_NATURAL keeps the natural order and groups rows by dependency
+// depth (more passes, stronger preconditioner).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+#include "cn_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace cn {
+
+__global__ void k_perm_count(int32_t n, const int32_t *perm, const int32_t *ptr,
+                             int32_t *cnt) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t o = perm[i];
+  cnt[i] = ptr[o + 1] - ptr[o];
+}
+
+__global__ void k_perm_fill(int32_t n, const int32_t *perm, const int32_t *iperm,
+                            const int32_t *ptr, const int32_t *col,
+                            const double *val, const int32_t *optr,
+                            int32_t *ocol, double *oval) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t o = perm[i], base = optr[i], m = ptr[o + 1] - ptr[o];
+  for (int32_t t = 0; t < m; ++t) {
+    int32_t c = iperm[col[ptr[o] + t]];
+    double v = val[ptr[o] + t];
+    int32_t q = base + t - 1;
+    while (q >= base && ocol[q] > c) { ocol[q + 1] = ocol[q]; oval[q + 1] = oval[q]; --q; }
+    ocol[q + 1] = c;
+    oval[q + 1] = v;
+  }
+}
+
+__global__ void k_tri_count(int32_t n, const int32_t *ptr, const int32_t *col,
+                            int32_t *lc, int32_t *uc, int32_t *nodiag) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t l = 0, u = 0, d = 0;
+  for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+    int32_t c = col[p];
+    if (c < i) ++l; else if (c > i) ++u; else d = 1;
+  }
+  lc[i] = l;
+  uc[i] = u;
+  if (!d) atomicMin(nodiag, i);
+}
+
+__global__ void k_tri_fill(int32_t n, const int32_t *ptr, const int32_t *col,
+                           const double *val, const int32_t *lp, int32_t *lcol,
+                           double *lval, double *adiag, const int32_t *up,
+                           int32_t *ucol) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t a = lp[i], b = up[i];
+  for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+    int32_t c = col[p];
+    if (c < i) { lcol[a] = c; lval[a] = val[p]; ++a; }
+    else if (c > i) { ucol[b++] = c; }
+    else adiag[i] = val[p];
+  }
+}
+
+// u_map[e] = index in L of entry (row = ucol[e], col = i)
+__global__ void k_u_map(int32_t n, const int32_t *up, const int32_t *ucol,
+                        const int32_t *lp, const int32_t *lcol, int32_t *umap) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int32_t e = up[i]; e < up[i + 1]; ++e) {
+    int32_t r = ucol[e];
+    int32_t a = lp[r], b = lp[r + 1];
+    while (a < b) {
+      int32_t m = (a + b) >> 1;
+      if (lcol[m] < i) a = m + 1; else b = m;
+    }
+    umap[e] = a;
+  }
+}
+
+// One level of the factorisation: rows [r0, r1) depend only on earlier rows.
+__global__ void k_ic0_rows(int32_t r0, int32_t r1, const int32_t *lp,
+                           const int32_t *lcol, double *lval,
+                           const double *adiag, double *ldiag, double shift,
+                           int32_t *fail) {
+  int32_t i = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= r1) return;
+  int32_t a0 = lp[i], a1 = lp[i + 1];
+  for (int32_t a = a0; a < a1; ++a) {
+    int32_t j = lcol[a];
+    double t = lval[a];
+    int32_t c = a0, b = lp[j], b1 = lp[j + 1];
+    while (c < a && b < b1) {
+      int32_t kc = lcol[c], kb = lcol[b];
+      if (kc == kb) { t = t - lval[c] * lval[b]; ++c; ++b; }
+      else if (kc < kb) ++c;
+      else ++b;
+    }
+    lval[a] = t / ldiag[j];
+  }
+  double s = adiag[i];
+  if (shift != 0.0) s = s + shift * adiag[i];
+  for (int32_t a = a0; a < a1; ++a) s = s - lval[a] * lval[a];
+  if (!(s > 0.0)) {
+    atomicMin(fail, i);
+    ldiag[i] = 1.0;
+  } else {
+    ldiag[i] = sqrt(s);
+  }
+}
+
+__global__ void k_gather_u(int64_t nnz, const int32_t *umap, const double *lval,
+                           double *uval) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nnz) uval[e] = lval[umap[e]];
+}
+
+__global__ void k_to_f32(int64_t n, const double *x, float *y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+
+__global__ void k_rdiag32(int32_t n, const double *d, float *r) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = (float)(1.0 / d[i]);
+}
+
+// Exact (spec-order) sweeps for cn_ic0_apply: thread per (row, column).
+__global__ void k_fwd_exact(int32_t r0, int32_t r1, int32_t k, const int32_t *lp,
+                            const int32_t *lcol, const double *lval,
+                            const double *ldiag, const double *b, double *y) {
+  int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t rows = r1 - r0;
+  if (id >= rows * k) return;
+  int32_t i = r0 + (int32_t)(id / k), c = (int32_t)(id % k);
+  double s = b[(int64_t)i * k + c];
+  for (int32_t p = lp[i]; p < lp[i + 1]; ++p)
+    s = s - lval[p] * y[(int64_t)lcol[p] * k + c];
+  y[(int64_t)i * k + c] = s / ldiag[i];
+}
+
+__global__ void k_bwd_exact(int32_t r0, int32_t r1, int32_t k, const int32_t *up,
+                            const int32_t *ucol, const double *uval,
+                            const double *ldiag, const double *y, double *x) {
+  int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t rows = r1 - r0;
+  if (id >= rows * k) return;
+  int32_t i = r0 + (int32_t)(id / k), c = (int32_t)(id % k);
+  double s = y[(int64_t)i * k + c];
+  for (int32_t p = up[i]; p < up[i + 1]; ++p)
+    s = s - uval[p] * x[(int64_t)ucol[p] * k + c];
+  x[(int64_t)i * k + c] = s / ldiag[i];
+}
+
+__global__ void k_gather_rows(int32_t n, int32_t k, const int32_t *perm,
+                              const double *src, double *dst) {
+  int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)n * k) return;
+  int64_t i = id / k, c = id % k;
+  dst[id] = src[(int64_t)perm[i] * k + c];
+}
+
+__global__ void k_scatter_rows(int32_t n, int32_t k, const int32_t *perm,
+                               const double *src, double *dst) {
+  int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)n * k) return;
+  int64_t i = id / k, c = id % k;
+  dst[(int64_t)perm[i] * k + c] = src[id];
+}
+
+// GraphColoring.greedyColor (GraphColoring.lean:41-56): vertex order,
+// smallest colour not used by an earlier neighbour.
+static int32_t greedy_color(int32_t n, const std::vector<int32_t> &ptr,
+                            const std::vector<int32_t> &col,
+                            std::vector<int32_t> &color) {
+  color.assign(n, 0);
+  std::vector<int32_t> mark;
+  int32_t ncol = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    int32_t deg = ptr[i + 1] - ptr[i];
+    if ((int32_t)mark.size() < deg + 2) mark.resize(deg + 2, -1);
+    for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+      int32_t j = col[p];
+      if (j < i && color[j] <= deg) mark[color[j]] = i;
+    }
+    int32_t c = 0;
+    while (c <= deg && mark[c] == i) ++c;
+    color[i] = c;
+    ncol = std::max(ncol, c + 1);
+  }
+  return ncol;
+}
+
+static int32_t natural_levels(int32_t n, const std::vector<int32_t> &ptr,
+                              const std::vector<int32_t> &col,
+                              std::vector<int32_t> &level) {
+  level.assign(n, 0);
+  int32_t nl = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    int32_t l = 0;
+    for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+      int32_t j = col[p];
+      if (j < i) l = std::max(l, level[j] + 1);
+    }
+    level[i] = l;
+    nl = std::max(nl, l + 1);
+  }
+  return nl;
+}
+
+}  // namespace cn
+
+using namespace cn;
+
+extern "C" {
+
+cn_status cn_ic0_factor(cn_system S, int32_t ordering, int32_t allow_shift,
+                        cn_ic0 *out, void *stream) {
+  CN_API_BEGIN
+  if (!S || !out || (ordering != CN_ORDER_COLOR && ordering != CN_ORDER_NATURAL)) {
+    set_error("cn_ic0_factor: bad arguments");
+    return CN_BAD_ARG;
+  }
+  *out = nullptr;
+  hipStream_t s = as_stream(stream);
+  const int32_t n = S->n_free;
+  std::vector<int32_t> ptr(n + 1), col(S->nnz_ff);
+  CN_HIP(hipMemcpyAsync(ptr.data(), S->ff_ptr.get(), sizeof(int32_t) * (n + 1),
+                        hipMemcpyDeviceToHost, s));
+  if (S->nnz_ff > 0)
+    CN_HIP(hipMemcpyAsync(col.data(), S->ff_col.get(), sizeof(int32_t) * S->nnz_ff,
+                          hipMemcpyDeviceToHost, s));
+  CN_HIP(hipStreamSynchronize(s));
+  std::vector<int32_t> lev;
+  int32_t nl = ordering == CN_ORDER_COLOR ? greedy_color(n, ptr, col, lev)
+                                          : natural_levels(n, ptr, col, lev);
+  cn_ic0_t *F = new cn_ic0_t();
+  try {
+    F->n = n;
+    F->ordering = ordering;
+    F->ncolors = nl;
+    F->level_ptr.assign(nl + 1, 0);
+    for (int32_t i = 0; i < n; ++i) F->level_ptr[lev[i] + 1]++;
+    for (int32_t c = 0; c < nl; ++c) F->level_ptr[c + 1] += F->level_ptr[c];
+    std::vector<int32_t> perm(n), iperm(n), fill(F->level_ptr.begin(), F->level_ptr.end() - 1);
+    for (int32_t i = 0; i < n; ++i) perm[fill[lev[i]]++] = i;  // stable
+    for (int32_t i = 0; i < n; ++i) iperm[perm[i]] = i;
+    F->perm.alloc(n + 1);
+    F->iperm.alloc(n + 1);
+    CN_HIP(hipMemcpyAsync(F->perm.get(), perm.data(), sizeof(int32_t) * n,
+                          hipMemcpyHostToDevice, s));
+    CN_HIP(hipMemcpyAsync(F->iperm.get(), iperm.data(), sizeof(int32_t) * n,
+                          hipMemcpyHostToDevice, s));
+    const int64_t nnz = S->nnz_ff;
+    F->a_ptr.alloc(n + 1);
+    F->a_col.alloc(nnz + 1);
+    F->a_val.alloc(nnz + 1);
+    F->a_val32.alloc(nnz + 1);
+    CN_HIP(hipMemsetAsync(F->a_ptr.get(), 0, sizeof(int32_t) * (n + 1), s));
+    if (n > 0)
+      k_perm_count<<<grid_for(n, 256), 256, 0, s>>>(n, F->perm.get(),
+                                                   S->ff_ptr.get(), F->a_ptr.get());
+    exclusive_scan_i32(F->a_ptr.get(), (int64_t)n + 1, s);
+    if (n > 0)
+      k_perm_fill<<<grid_for(n, 128), 128, 0, s>>>(
+          n, F->perm.get(), F->iperm.get(), S->ff_ptr.get(), S->ff_col.get(),
+          S->ff_val.get(), F->a_ptr.get(), F->a_col.get(), F->a_val.get());
+    if (nnz > 0)
+      k_to_f32<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, F->a_val.get(), F->a_val32.get());
+    // split into L (strict lower) / U (strict upper) / diagonal
+    F->l_ptr.alloc(n + 1);
+    F->u_ptr.alloc(n + 1);
+    DevBuf<int32_t> nodiag(1);
+    int32_t big = INT_MAX;
+    CN_HIP(hipMemcpyAsync(nodiag.get(), &big, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    CN_HIP(hipMemsetAsync(F->l_ptr.get(), 0, sizeof(int32_t) * (n + 1), s));
+    CN_HIP(hipMemsetAsync(F->u_ptr.get(), 0, sizeof(int32_t) * (n + 1), s));
+    if (n > 0)
+      k_tri_count<<<grid_for(n, 256), 256, 0, s>>>(n, F->a_ptr.get(), F->a_col.get(),
+                                                  F->l_ptr.get(), F->u_ptr.get(),
+                                                  nodiag.get());
+    int64_t nl_nz = exclusive_scan_i32(F->l_ptr.get(), (int64_t)n + 1, s);
+    int64_t nu_nz = exclusive_scan_i32(F->u_ptr.get(), (int64_t)n + 1, s);
+    int32_t hnd = 0;
+    CN_HIP(hipMemcpyAsync(&hnd, nodiag.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CN_HIP(hipStreamSynchronize(s));
+    if (hnd != INT_MAX) {
+      delete F;
+      set_error("cn_ic0_factor: row %d has no diagonal entry", hnd);
+      return CN_NOT_SPD;
+    }
+    F->nnz_l = nl_nz + n;
+    F->l_col.alloc(nl_nz + 1);
+    F->l_val.alloc(nl_nz + 1);
+    F->l_val32.alloc(nl_nz + 1);
+    F->u_col.alloc(nu_nz + 1);
+    F->u_val.alloc(nu_nz + 1);
+    F->u_val32.alloc(nu_nz + 1);
+    F->u_map.alloc(nu_nz + 1);
+    F->l_diag.alloc(n + 1);
+    F->l_rdiag32.alloc(n + 1);
+    DevBuf<double> adiag(n + 1), lval0(nl_nz + 1);
+    if (n > 0) {
+      k_tri_fill<<<grid_for(n, 256), 256, 0, s>>>(
+          n, F->a_ptr.get(), F->a_col.get(), F->a_val.get(), F->l_ptr.get(),
+          F->l_col.get(), lval0.get(), adiag.get(), F->u_ptr.get(), F->u_col.get());
+      k_u_map<<<grid_for(n, 256), 256, 0, s>>>(n, F->u_ptr.get(), F->u_col.get(),
+                                              F->l_ptr.get(), F->l_col.get(),
+                                              F->u_map.get());
+    }
+    CN_HIP(hipGetLastError());
+    const double shifts[] = {0.0, 1e-4, 1e-3, 1e-2, 1e-1, 1.0};
+    const int nshift = allow_shift ? 6 : 1;
+    DevBuf<int32_t> fail(1);
+    int32_t hf = INT_MAX;
+    for (int si = 0; si < nshift; ++si) {
+      CN_HIP(hipMemcpyAsync(F->l_val.get(), lval0.get(), sizeof(double) * (nl_nz + 1),
+                            hipMemcpyDeviceToDevice, s));
+      CN_HIP(hipMemcpyAsync(fail.get(), &big, sizeof(int32_t), hipMemcpyHostToDevice, s));
+      for (int32_t c = 0; c < nl; ++c) {
+        int32_t r0 = F->level_ptr[c], r1 = F->level_ptr[c + 1];
+        if (r1 > r0)
+          k_ic0_rows<<<grid_for(r1 - r0, 64), 64, 0, s>>>(
+              r0, r1, F->l_ptr.get(), F->l_col.get(), F->l_val.get(), adiag.get(),
+              F->l_diag.get(), shifts[si], fail.get());
+      }
+      CN_HIP(hipGetLastError());
+      CN_HIP(hipMemcpyAsync(&hf, fail.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      CN_HIP(hipStreamSynchronize(s));
+      if (hf == INT_MAX) { F->shift = shifts[si]; break; }
+    }
+    if (hf != INT_MAX) {
+      delete F;
+      set_error("cn_ic0_factor: non-positive pivot at permuted row %d", hf);
+      return CN_NOT_SPD;
+    }
+    if (nu_nz > 0)
+      k_gather_u<<<grid_for(nu_nz, 256), 256, 0, s>>>(nu_nz, F->u_map.get(),
+                                                     F->l_val.get(), F->u_val.get());
+    if (nl_nz > 0)
+      k_to_f32<<<grid_for(nl_nz, 256), 256, 0, s>>>(nl_nz, F->l_val.get(), F->l_val32.get());
+    if (nu_nz > 0)
+      k_to_f32<<<grid_for(nu_nz, 256), 256, 0, s>>>(nu_nz, F->u_val.get(), F->u_val32.get());
+    if (n > 0)
+      k_rdiag32<<<grid_for(n, 256), 256, 0, s>>>(n, F->l_diag.get(), F->l_rdiag32.get());
+    CN_HIP(hipGetLastError());
+    CN_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    delete F;
+    throw;
+  }
+  *out = F;
+  return CN_OK;
+  CN_API_END
+}
+
+cn_status cn_ic0_get_info(cn_ic0 F, cn_ic0_info *info) {
+  if (!F || !info) { set_error("null handle"); return CN_BAD_ARG; }
+  info->n = F->n;
+  info->nlevels = F->ncolors;
+  info->ordering = F->ordering;
+  info->nnz_l = F->nnz_l;
+  info->shift = F->shift;
+  info->d_perm = F->perm.get();
+  info->d_a_ptr = F->a_ptr.get();
+  info->d_a_col = F->a_col.get();
+  info->d_a_val = F->a_val.get();
+  info->d_l_ptr = F->l_ptr.get();
+  info->d_l_col = F->l_col.get();
+  info->d_l_val = F->l_val.get();
+  info->d_l_diag = F->l_diag.get();
+  return CN_OK;
+}
+
+cn_status cn_ic0_apply(cn_ic0 F, int32_t k, const double *d_r, double *d_z,
+                       void *stream) {
+  CN_API_BEGIN
+  if (!F || k <= 0 || !d_r || !d_z) { set_error("cn_ic0_apply: bad arguments"); return CN_BAD_ARG; }
+  hipStream_t s = as_stream(stream);
+  const int32_t n = F->n;
+  int64_t tot = (int64_t)n * k;
+  if (tot == 0) return CN_OK;
+  DevBuf<double> rp(tot), y(tot), x(tot);
+  k_gather_rows<<<grid_for(tot, 256), 256, 0, s>>>(n, k, F->perm.get(), d_r, rp.get());
+  for (int32_t c = 0; c < F->ncolors; ++c) {
+    int32_t r0 = F->level_ptr[c], r1 = F->level_ptr[c + 1];
+    int64_t t = (int64_t)(r1 - r0) * k;
+    if (t > 0)
+      k_fwd_exact<<<grid_for(t, 256), 256, 0, s>>>(r0, r1, k, F->l_ptr.get(),
+                                                  F->l_col.get(), F->l_val.get(),
+                                                  F->l_diag.get(), rp.get(), y.get());
+  }
+  for (int32_t c = F->ncolors - 1; c >= 0; --c) {
+    int32_t r0 = F->level_ptr[c], r1 = F->level_ptr[c + 1];
+    int64_t t = (int64_t)(r1 - r0) * k;
+    if (t > 0)
+      k_bwd_exact<<<grid_for(t, 256), 256, 0, s>>>(r0, r1, k, F->u_ptr.get(),
+                                                  F->u_col.get(), F->u_val.get(),
+                                                  F->l_diag.get(), y.get(), x.get());
+  }
+  k_scatter_rows<<<grid_for(tot, 256), 256, 0, s>>>(n, k, F->perm.get(), x.get(), d_z);
+  CN_HIP(hipGetLastError());
+  CN_HIP(hipStreamSynchronize(s));
+  return CN_OK;
+  CN_API_END
+}
+
+cn_status cn_ic0_destroy(cn_ic0 F) {
+  delete F;
+  return CN_OK;
+}
+
+}  // extern "C"

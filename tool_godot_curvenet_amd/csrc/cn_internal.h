@@ -1,0 +1,154 @@
+// cn_internal.h -- shared internals of the gfx950 curvenet linear-algebra
+// library: error plumbing, device buffers, handle structs, kernel helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/curvenet_la.h"
+
+namespace cn {
+
+// Last error message of the calling thread (cn_last_error()).
+void set_error(const char *fmt, ...);
+
+struct HipError {
+  hipError_t err;
+  const char *what;
+};
+
+#define CN_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) throw ::cn::HipError{e_, #call};                  \
+  } while (0)
+
+// Converts exceptions thrown inside an entry point into a cn_status.
+#define CN_API_BEGIN try {
+#define CN_API_END                                                          \
+  }                                                                         \
+  catch (const ::cn::HipError &e) {                                         \
+    ::cn::set_error("HIP error %d (%s) in %s", (int)e.err,                 \
+                    hipGetErrorString(e.err), e.what);                      \
+    return CN_HIP_ERROR;                                                    \
+  }                                                                         \
+  catch (const std::bad_alloc &) {                                          \
+    ::cn::set_error("host allocation failed");                             \
+    return CN_OOM;                                                          \
+  }
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Owning device buffer (hipMalloc / hipFree).  Setup-time objects only; the
+// solver's per-iteration work never allocates.
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) CN_HIP(hipMalloc(&p, sizeof(T) * count));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  T *get() const { return p; }
+};
+
+inline unsigned grid_for(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 0x7fffffff) g = 0x7fffffff;
+  return (unsigned)g;
+}
+
+// ---------------------------------------------------------------- scans --
+// Exclusive prefix sum of int32 counts in place; returns the total.  `tmp`
+// is scratch (grown as needed).  Synchronises the stream to read the total.
+int64_t exclusive_scan_i32(int32_t *d, int64_t n, hipStream_t s);
+int64_t exclusive_scan_i64(int64_t *d, int64_t n, hipStream_t s);
+
+// ----------------------------------------------------------- topology --
+// Device halfedge mesh in SoA form (the spec's Halfedge record,
+// Halfedge.lean:30-35, with -1 for `none`).
+struct HeMesh {
+  int32_t nv = 0, nf = 0, nh = 0;
+  DevBuf<int32_t> target, twin, next, face;
+};
+
+// Build the halfedge mesh of a triangle list on the device
+// (HalfedgeBuilder.fromTriangles semantics, index-identical).
+void build_halfedges(int32_t nv, int32_t nf, const int32_t *d_tris,
+                     int32_t *target, int32_t *twin, int32_t *next,
+                     int32_t *face, int32_t *nh_out, hipStream_t s);
+
+// Vertex buckets: for every vertex, the ascending list of interior
+// halfedges whose target (or source) is the vertex.
+struct Buckets {
+  DevBuf<int32_t> ptr;  // nv + 1
+  DevBuf<int32_t> he;   // 3 nf
+};
+void build_incoming(int32_t nv, int32_t nf, const int32_t *d_tris, Buckets &b,
+                    hipStream_t s);  // by target (interior halfedges)
+void build_outgoing(int32_t nv, int32_t nf, const int32_t *d_tris, Buckets &b,
+                    hipStream_t s);  // by source (interior halfedges)
+
+struct SolverWork;
+void free_solver_work(SolverWork *w);
+
+}  // namespace cn
+
+// ------------------------------------------------------------ handles --
+// The reduced cut-Laplacian system A_ff (free x free) and the coupling
+// block A_fc (free x constrained), plus the cut mesh they came from.
+struct cn_system_t {
+  int32_t nv = 0, nf = 0, n_free = 0, n_con = 0;
+  int64_t nnz_ff = 0, nnz_fc = 0;
+  cn::DevBuf<int32_t> tris;         // cut-mesh triangles (3 nf)
+  cn::DevBuf<uint8_t> constrained;  // nv
+  cn::DevBuf<double> pos;           // 3 nv
+  cn::HeMesh he;                    // halfedges of the cut mesh
+  cn::Buckets inc;                  // incoming interior halfedges per vertex
+  cn::DevBuf<int32_t> free_of, con_of, free_vertex, con_vertex;
+  cn::DevBuf<double> face_m;        // 9 per face: triangle cot Laplacian
+  cn::DevBuf<int32_t> ff_ptr, ff_col, fc_ptr, fc_col;
+  cn::DevBuf<double> ff_val, fc_val;
+  cn::DevBuf<float> ff_val32, fc_val32;
+  cn::DevBuf<double> diag;          // diagonal of A_ff
+  cn::SolverWork *work = nullptr;   // solver workspace, grown on demand
+  ~cn_system_t() { cn::free_solver_work(work); }
+};
+
+struct cn_ic0_t {
+  int32_t n = 0, ncolors = 0, ordering = 0;
+  double shift = 0.0;
+  std::vector<int32_t> level_ptr;   // host copy: rows of level c are
+                                    // [level_ptr[c], level_ptr[c+1])
+  cn::DevBuf<int32_t> perm, iperm;  // perm[new] = old
+  // permuted A (FP64 + FP32) for the solver
+  cn::DevBuf<int32_t> a_ptr, a_col;
+  cn::DevBuf<double> a_val;
+  cn::DevBuf<float> a_val32;
+  // L strictly lower (CSR rows) and U = L^T strictly upper (CSR rows)
+  cn::DevBuf<int32_t> l_ptr, l_col, u_ptr, u_col, u_map;
+  cn::DevBuf<double> l_val, u_val, l_diag;
+  cn::DevBuf<float> l_val32, u_val32, l_rdiag32;
+  cn::DevBuf<double> l_rdiag64;     // made on first FP64 use
+  int64_t nnz_l = 0;
+};

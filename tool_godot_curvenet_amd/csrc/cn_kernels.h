@@ -1,0 +1,188 @@
+// cn_kernels.h -- device building blocks shared by the SpMV/SpMM, IC(0) and
+// PCG kernels.
+//
+// Two row mappings cover all right-hand-side counts k:
+//   * narrow (k <= 4): a group of 8 lanes per CSR row; lanes stride over the
+//     row's entries (coalesced col/val reads), gather k values of x per entry
+//     and combine with a butterfly over the group.  8 rows per wavefront.
+//   * wide (k > 4): one wavefront per (row, column tile).  The row's col/val
+//     stream is wave-uniform (scalar loads, readfirstlane), every lane owns
+//     VEC consecutive columns, so each entry costs one coalesced 64*VEC-wide
+//     gather of the n x k row-major block (SpmvMulti.lean:113 layout).
+// Partial dot products are accumulated in FP64 per lane, reduced across the
+// 4 waves of a block through LDS in a fixed order, and written as one row of
+// a [block][k] partial buffer; a second kernel sums the partials in a fixed
+// order.  Results are therefore deterministic from run to run.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cn {
+
+constexpr int kBlock = 256;    // 4 wavefronts
+constexpr int kWaves = 4;
+constexpr int kGroup = 8;      // lanes per row in the narrow mapping
+constexpr int kMaxPartBlocks = 1024;
+
+__device__ __forceinline__ int32_t uniform(int32_t x) {
+  return __builtin_amdgcn_readfirstlane(x);
+}
+
+template <typename T, int VEC>
+struct VecIO;
+template <>
+struct VecIO<float, 4> {
+  static __device__ __forceinline__ void ld(const float *p, float (&v)[4]) {
+    float4 t = *reinterpret_cast<const float4 *>(p);
+    v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+  }
+  static __device__ __forceinline__ void st(float *p, const float (&v)[4]) {
+    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+  }
+};
+template <>
+struct VecIO<float, 2> {
+  static __device__ __forceinline__ void ld(const float *p, float (&v)[2]) {
+    float2 t = *reinterpret_cast<const float2 *>(p);
+    v[0] = t.x; v[1] = t.y;
+  }
+  static __device__ __forceinline__ void st(float *p, const float (&v)[2]) {
+    *reinterpret_cast<float2 *>(p) = make_float2(v[0], v[1]);
+  }
+};
+template <>
+struct VecIO<double, 2> {
+  static __device__ __forceinline__ void ld(const double *p, double (&v)[2]) {
+    double2 t = *reinterpret_cast<const double2 *>(p);
+    v[0] = t.x; v[1] = t.y;
+  }
+  static __device__ __forceinline__ void st(double *p, const double (&v)[2]) {
+    *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+  }
+};
+template <typename T>
+struct VecIO<T, 1> {
+  static __device__ __forceinline__ void ld(const T *p, T (&v)[1]) { v[0] = *p; }
+  static __device__ __forceinline__ void st(T *p, const T (&v)[1]) { *p = v[0]; }
+};
+
+// acc[m] += sum_{e in [a,b)} val[e] * X[col[e]*k + c0 + m]   (wide mapping)
+template <typename T, int VEC>
+__device__ __forceinline__ void gather_wide(int32_t a, int32_t b,
+                                            const int32_t *__restrict__ col,
+                                            const T *__restrict__ val,
+                                            const T *__restrict__ X, int64_t k,
+                                            int64_t c0, bool valid,
+                                            T (&acc)[VEC]) {
+  int32_t e = a;
+  for (; e + 4 <= b; e += 4) {
+    int32_t j0 = col[e], j1 = col[e + 1], j2 = col[e + 2], j3 = col[e + 3];
+    T v0 = val[e], v1 = val[e + 1], v2 = val[e + 2], v3 = val[e + 3];
+    if (valid) {
+      T x0[VEC], x1[VEC], x2[VEC], x3[VEC];
+      VecIO<T, VEC>::ld(X + (int64_t)j0 * k + c0, x0);
+      VecIO<T, VEC>::ld(X + (int64_t)j1 * k + c0, x1);
+      VecIO<T, VEC>::ld(X + (int64_t)j2 * k + c0, x2);
+      VecIO<T, VEC>::ld(X + (int64_t)j3 * k + c0, x3);
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) {
+        acc[m] = fma(v0, x0[m], acc[m]);
+        acc[m] = fma(v1, x1[m], acc[m]);
+        acc[m] = fma(v2, x2[m], acc[m]);
+        acc[m] = fma(v3, x3[m], acc[m]);
+      }
+    }
+  }
+  for (; e < b; ++e) {
+    int32_t j = col[e];
+    T v = val[e];
+    if (valid) {
+      T x[VEC];
+      VecIO<T, VEC>::ld(X + (int64_t)j * k + c0, x);
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) acc[m] = fma(v, x[m], acc[m]);
+    }
+  }
+}
+
+// narrow mapping: lane g of the row's 8-lane group
+template <typename T, int K>
+__device__ __forceinline__ void gather_narrow(int32_t a, int32_t b, int g,
+                                              const int32_t *__restrict__ col,
+                                              const T *__restrict__ val,
+                                              const T *__restrict__ X,
+                                              T (&acc)[K]) {
+  for (int32_t e = a + g; e < b; e += kGroup) {
+    int32_t j = col[e];
+    T v = val[e];
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] = fma(v, X[(int64_t)j * K + c], acc[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    acc[c] += __shfl_xor(acc[c], 4, kGroup);
+    acc[c] += __shfl_xor(acc[c], 2, kGroup);
+    acc[c] += __shfl_xor(acc[c], 1, kGroup);
+  }
+}
+
+// Block reduction of per-lane FP64 partials for the wide mapping:
+// part[lane*VEC+m] of each wave summed over the 4 waves (fixed order).
+template <int VEC, int NP>
+__device__ __forceinline__ void block_partials_wide(double (&p)[NP][VEC],
+                                                    double *__restrict__ out,
+                                                    int64_t k, int64_t c0,
+                                                    int64_t stride_np) {
+  __shared__ double sh[kWaves][NP][64 * VEC];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NP; ++q)
+#pragma unroll
+    for (int m = 0; m < VEC; ++m) sh[w][q][lane * VEC + m] = p[q][m];
+  __syncthreads();
+  if (w == 0) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) {
+        int64_t c = c0 + m;
+        double s = sh[0][q][lane * VEC + m];
+        s += sh[1][q][lane * VEC + m];
+        s += sh[2][q][lane * VEC + m];
+        s += sh[3][q][lane * VEC + m];
+        if (c < k) out[q * stride_np + (int64_t)blockIdx.x * k + c] = s;
+      }
+  }
+}
+
+// narrow: lane g < K of every 8-lane group holds the partials of column g;
+// the 32 groups of the block are summed in order.
+template <int K, int NP>
+__device__ __forceinline__ void block_partials_narrow(const double (&p)[NP],
+                                                      double *__restrict__ out,
+                                                      int64_t stride_np) {
+  __shared__ double sh[NP][kBlock / kGroup][K];
+  const int grp = threadIdx.x / kGroup, g = threadIdx.x % kGroup;
+  if (g < K)
+#pragma unroll
+    for (int q = 0; q < NP; ++q) sh[q][grp][g] = p[q];
+  __syncthreads();
+  if (threadIdx.x < NP * K) {
+    int q = threadIdx.x / K, c = threadIdx.x % K;
+    double s = 0.0;
+    for (int i = 0; i < kBlock / kGroup; ++i) s += sh[q][i][c];
+    out[q * stride_np + (int64_t)blockIdx.x * K + c] = s;
+  }
+}
+
+// register-array element by a lane-varying index (K <= 4)
+template <typename T, int K>
+__device__ __forceinline__ T pick(const T (&a)[K], int g) {
+  T r = a[0];
+#pragma unroll
+  for (int c = 1; c < K; ++c) r = (g == c) ? a[c] : r;
+  return r;
+}
+
+}  // namespace cn

@@ -1,0 +1,390 @@
+"""Python host side of the gfx950 library, named after the spec functions.
+
+Every call goes through the C ABI (``_lib``); device memory is held in torch
+CUDA tensors (plumbing only).  Functions mirror the reference's Lean
+definitions:
+
+=========================  ==================================================
+``from_triangles``         HalfedgeBuilder.fromTriangles (HalfedgeBuilder.lean:34-90)
+``cut_seams``              seam duplication (cn_cut_seams; builder-defined rule)
+``project_to_vertices``    SurfaceProjection.projectToVertices (:46-64)
+``CutLaplacian``           CutMeshLaplacian.assembleVtLhV (:86-93), split A_ff / A_fc
+``CutLaplacian.harmonic_rhs``  HarmonicSolve.solveMulti RHS (:35-45, 57-64)
+``spmv`` / ``spmm``        SparseLinAlg.spmv (:44-53) / SpmvMulti
+``cg`` / ``cg_with_guess`` SparseLinAlg.cg / cgWithGuess (:104-163)
+``IC0``                    IncompleteCholesky (CSR no-fill, :26-101)
+``harmonic_solve``         HarmonicSolve.solveMulti (:51-66) on the sparse path
+=========================  ==================================================
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device visible: the gfx950 library needs a GPU "
+                           "(there is no CPU fallback)")
+    return torch
+
+
+def _dev(a, dtype):
+    torch = _torch()
+    t = torch.as_tensor(np.ascontiguousarray(a))
+    return t.to(device="cuda", dtype=dtype).contiguous()
+
+
+def _stream():
+    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None and t.numel() else C.c_void_p(0)
+
+
+def _np(ptr: int, count: int, dtype) -> np.ndarray:
+    dt = np.dtype(dtype)
+    _torch().cuda.synchronize()
+    return np.frombuffer(L.d2h(ptr, count * dt.itemsize), dtype=dt).copy()
+
+
+def _as_tensor(x, dtype):
+    torch = _torch()
+    if isinstance(x, torch.Tensor):
+        return x.to(device="cuda", dtype=dtype).contiguous()
+    return _dev(x, dtype)
+
+
+# ------------------------------------------------------------------ topology --
+def from_triangles(nv: int, tris) -> dict:
+    """HalfedgeBuilder.fromTriangles on the GPU; returns numpy arrays
+    target / twin / next / face (-1 = none)."""
+    torch = _torch()
+    lib = L.load()
+    t = _dev(np.asarray(tris, dtype=np.int32).reshape(-1, 3), torch.int32)
+    nf = t.shape[0]
+    cap = max(6 * nf, 1)
+    bufs = [torch.empty(cap, dtype=torch.int32, device="cuda") for _ in range(4)]
+    nh = C.c_int32(0)
+    L.check(lib.cn_halfedge_from_triangles(nv, nf, _p(t), *(_p(b) for b in bufs),
+                                           C.byref(nh), _stream()), "from_triangles")
+    out = {k: b[:nh.value].cpu().numpy() for k, b in zip(("target", "twin", "next", "face"), bufs)}
+    out["nv"], out["nf"], out["nh"] = nv, nf, nh.value
+    return out
+
+
+def cut_seams(nv: int, tris, cut_edges):
+    """Cut the mesh open along the given edges.  Returns (new_tris, origin,
+    constrained, nv_after)."""
+    torch = _torch()
+    lib = L.load()
+    t = _dev(np.asarray(tris, dtype=np.int32).reshape(-1, 3), torch.int32)
+    e = _dev(np.asarray(cut_edges, dtype=np.int32).reshape(-1, 2), torch.int32)
+    nf, ncut = t.shape[0], e.shape[0]
+    cap = nv + 2 * ncut + 1
+    nt = torch.empty_like(t)
+    origin = torch.empty(cap, dtype=torch.int32, device="cuda")
+    con = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    nv2 = C.c_int32(0)
+    L.check(lib.cn_cut_seams(nv, nf, _p(t), ncut, _p(e), _p(nt), _p(origin), _p(con),
+                             C.byref(nv2), _stream()), "cut_seams")
+    n2 = nv2.value
+    return (nt.cpu().numpy(), origin[:n2].cpu().numpy(), con[:n2].cpu().numpy(), n2)
+
+
+def project_to_vertices(knots, pos) -> np.ndarray:
+    torch = _torch()
+    lib = L.load()
+    q = _dev(np.asarray(knots, dtype=np.float64).reshape(-1, 3), torch.float64)
+    p = _dev(np.asarray(pos, dtype=np.float64).reshape(-1, 3), torch.float64)
+    out = torch.empty(q.shape[0], dtype=torch.int32, device="cuda")
+    L.check(lib.cn_project_to_vertices(p.shape[0], _p(p), q.shape[0], _p(q), _p(out),
+                                       _stream()), "project_to_vertices")
+    return out.cpu().numpy()
+
+
+# ----------------------------------------------------------------- matrices --
+@dataclass
+class CsrMatrix:
+    """SparseLinAlg.SparseMatrixCSR (SparseLinAlg.lean:35-41)."""
+    rows: int
+    cols: int
+    row_ptr: np.ndarray
+    col_idx: np.ndarray
+    values: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_ptr[-1])
+
+    def dense(self) -> np.ndarray:
+        d = np.zeros((self.rows, self.cols))
+        for i in range(self.rows):
+            a, b = self.row_ptr[i], self.row_ptr[i + 1]
+            d[i, self.col_idx[a:b]] += self.values[a:b]
+        return d
+
+    def device(self):
+        torch = _torch()
+        return (_dev(self.row_ptr.astype(np.int32), torch.int32),
+                _dev(self.col_idx.astype(np.int32), torch.int32),
+                _dev(self.values.astype(np.float64), torch.float64))
+
+
+def spmm(A: CsrMatrix, X, precision: str = "f64"):
+    """Y = A X for an n x k (or n) block; f64 is the spec-exact form."""
+    torch = _torch()
+    lib = L.load()
+    X = np.asarray(X) if not isinstance(X, torch.Tensor) else X
+    vec = X.ndim == 1
+    k = 1 if vec else X.shape[1]
+    ptr, col, val = A.device()
+    if precision == "f64":
+        x = _as_tensor(X, torch.float64)
+        y = torch.empty((A.rows, k), dtype=torch.float64, device="cuda")
+        L.check(lib.cn_spmm_f64(A.rows, _p(ptr), _p(col), _p(val), k, _p(x), _p(y),
+                                _stream()), "spmm_f64")
+    else:
+        x = _as_tensor(X, torch.float32)
+        v32 = val.float()
+        y = torch.empty((A.rows, k), dtype=torch.float32, device="cuda")
+        L.check(lib.cn_spmm_f32(A.rows, _p(ptr), _p(col), _p(v32), k, _p(x), _p(y),
+                                _stream()), "spmm_f32")
+    y = y.cpu().numpy()
+    return y.reshape(-1) if vec else y
+
+
+def spmv(A: CsrMatrix, x, precision: str = "f64"):
+    return spmm(A, np.asarray(x).reshape(-1), precision)
+
+
+# ------------------------------------------------------------------ systems --
+class _System:
+    """Owner of a cn_system handle."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+        info = L.SystemInfo()
+        L.check(L.load().cn_system_get_info(self._h, C.byref(info)), "system_get_info")
+        self.info = info
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                L.load().cn_system_destroy(h)
+            except Exception:
+                pass
+            self._h = C.c_void_p(0)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def n_free(self) -> int:
+        return self.info.n_free
+
+    def A_ff(self) -> CsrMatrix:
+        i = self.info
+        n = i.n_free
+        return CsrMatrix(n, n, _np(i.d_ff_ptr, n + 1, np.int32), _np(i.d_ff_col, i.nnz_ff, np.int32),
+                         _np(i.d_ff_val, i.nnz_ff, np.float64))
+
+    def solve(self, b, precond: str = "ic0", precision: str = "mixed", tol: float = 1e-9,
+              x0=None, ic0: "IC0 | None" = None, max_iter: int = 20000,
+              inner_tol: float = 1e-5, check_every: int = 8, max_refine: int = 30):
+        """Solve A_ff X = B (n x k or n).  Returns (x, SolveInfo)."""
+        torch = _torch()
+        lib = L.load()
+        bt = _as_tensor(b, torch.float64)
+        vec = bt.dim() == 1
+        bt = bt.reshape(self.n_free, -1).contiguous()
+        k = bt.shape[1]
+        x = (_as_tensor(x0, torch.float64).reshape(self.n_free, k).clone()
+             if x0 is not None else torch.zeros_like(bt))
+        o = L.PcgOpts()
+        lib.cn_pcg_default_opts(C.byref(o))
+        o.precond = {"ic0": L.CN_PRECOND_IC0, "jacobi": L.CN_PRECOND_JACOBI}[precond]
+        o.precision = {"mixed": L.CN_PREC_MIXED, "fp64": L.CN_PREC_FP64,
+                       "fp64_plain": L.CN_PREC_FP64_PLAIN}[precision]
+        o.tol_rel, o.max_iter, o.inner_tol = tol, max_iter, inner_tol
+        o.check_every, o.max_refine, o.warm_start = check_every, max_refine, int(x0 is not None)
+        if precond == "ic0" and ic0 is None:
+            ic0 = IC0(self)
+        info = L.SolveInfo()
+        st = lib.cn_pcg_solve(self._h, ic0.handle if ic0 is not None else C.c_void_p(0),
+                              C.byref(o), k, _p(bt), _p(x), C.byref(info), _stream())
+        L.check(st, "pcg_solve", ok=(L.CN_OK, L.CN_MAXIT, L.CN_BREAKDOWN))
+        return (x.reshape(-1) if vec else x), info
+
+
+class CutLaplacian(_System):
+    """VtLhV of a cut triangle mesh (CutMeshLaplacian.assembleVtLhV), stored
+    as the free block A_ff and the coupling block A_fc."""
+
+    def __init__(self, pos, tris, constrained):
+        torch = _torch()
+        lib = L.load()
+        self.pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+        self.tris = np.asarray(tris, dtype=np.int32).reshape(-1, 3)
+        self.constrained = np.asarray(constrained, dtype=np.uint8).reshape(-1)
+        t = _dev(self.tris, torch.int32)
+        c = _dev(self.constrained, torch.uint8)
+        p = _dev(self.pos, torch.float64)
+        h = C.c_void_p(0)
+        L.check(lib.cn_system_create(self.pos.shape[0], self.tris.shape[0], _p(t), _p(c), _p(p),
+                                     C.byref(h), _stream()), "system_create")
+        super().__init__(h.value)
+
+    @classmethod
+    def from_mesh(cls, mesh) -> "CutLaplacian":
+        """Cut a generated mesh along its cut edges and build the system."""
+        nt, origin, con, nv2 = cut_seams(mesh.nv, mesh.tris, mesh.cut_edges)
+        pos = mesh.pos[origin]
+        obj = cls(pos, nt, con)
+        obj.origin = origin
+        return obj
+
+    @property
+    def n_con(self) -> int:
+        return self.info.n_con
+
+    def A_fc(self) -> CsrMatrix:
+        i = self.info
+        n = i.n_free
+        return CsrMatrix(n, i.n_con, _np(i.d_fc_ptr, n + 1, np.int32),
+                         _np(i.d_fc_col, i.nnz_fc, np.int32), _np(i.d_fc_val, i.nnz_fc, np.float64))
+
+    def maps(self):
+        i = self.info
+        return (_np(i.d_free_of, i.nv, np.int32), _np(i.d_con_of, i.nv, np.int32))
+
+    def halfedges(self) -> dict:
+        i = self.info
+        return {k: _np(getattr(i, "d_he_" + k), i.nh, np.int32)
+                for k in ("target", "twin", "next", "face")}
+
+    def harmonic_rhs(self, Xc, as_numpy: bool = True):
+        """-(Vt Lh C Xc) on the free rows, spec summation order (FP64)."""
+        torch = _torch()
+        lib = L.load()
+        x = _as_tensor(Xc, torch.float64).reshape(self.n_con, -1).contiguous()
+        k = x.shape[1]
+        out = torch.empty((self.n_free, k), dtype=torch.float64, device="cuda")
+        L.check(lib.cn_harmonic_rhs(self._h, k, _p(x), _p(out), _stream()), "harmonic_rhs")
+        return out.cpu().numpy() if as_numpy else out
+
+    def set_positions(self, pos):
+        torch = _torch()
+        self.pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+        p = _dev(self.pos, torch.float64)
+        L.check(L.load().cn_system_set_positions(self._h, _p(p), _stream()), "set_positions")
+
+
+class CsrSystem(_System):
+    """A caller-assembled SPD matrix as a solver system."""
+
+    def __init__(self, A: CsrMatrix):
+        ptr, col, val = A.device()
+        h = C.c_void_p(0)
+        L.check(L.load().cn_system_from_csr(A.rows, _p(ptr), _p(col), _p(val), C.byref(h),
+                                            _stream()), "system_from_csr")
+        super().__init__(h.value)
+
+
+class IC0:
+    """No-fill incomplete Cholesky factor of a system's A_ff."""
+
+    def __init__(self, system: _System, ordering: str = "color", allow_shift: bool = True):
+        lib = L.load()
+        self.system = system
+        h = C.c_void_p(0)
+        order = {"color": L.CN_ORDER_COLOR, "natural": L.CN_ORDER_NATURAL}[ordering]
+        L.check(lib.cn_ic0_factor(system.handle, order, int(allow_shift), C.byref(h), _stream()),
+                "ic0_factor")
+        self._h = h
+        info = L.Ic0Info()
+        L.check(lib.cn_ic0_get_info(h, C.byref(info)), "ic0_get_info")
+        self.info = info
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                L.load().cn_ic0_destroy(h)
+            except Exception:
+                pass
+            self._h = C.c_void_p(0)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def perm(self) -> np.ndarray:
+        return _np(self.info.d_perm, self.info.n, np.int32)
+
+    def permuted_matrix(self) -> CsrMatrix:
+        i = self.info
+        n = i.n
+        ptr = _np(i.d_a_ptr, n + 1, np.int32)
+        return CsrMatrix(n, n, ptr, _np(i.d_a_col, int(ptr[-1]), np.int32),
+                         _np(i.d_a_val, int(ptr[-1]), np.float64))
+
+    def factor(self):
+        """(l_ptr, l_col, l_val, diag) of the strictly lower part + diagonal."""
+        i = self.info
+        n = i.n
+        ptr = _np(i.d_l_ptr, n + 1, np.int32)
+        nnz = int(ptr[-1])
+        return ptr, _np(i.d_l_col, nnz, np.int32), _np(i.d_l_val, nnz, np.float64), \
+            _np(i.d_l_diag, n, np.float64)
+
+    def apply(self, r):
+        torch = _torch()
+        rt = _as_tensor(r, torch.float64)
+        vec = rt.dim() == 1
+        rt = rt.reshape(self.info.n, -1).contiguous()
+        z = torch.empty_like(rt)
+        L.check(L.load().cn_ic0_apply(self._h, rt.shape[1], _p(rt), _p(z), _stream()), "ic0_apply")
+        z = z.cpu().numpy()
+        return z.reshape(-1) if vec else z
+
+
+# ------------------------------------------------------------- spec mirrors --
+def cg_with_guess(A: CsrMatrix, b, x0, maxIter: int, tol: float):
+    """SparseLinAlg.cgWithGuess: Jacobi PCG from x0, stop when ||r|| < tol
+    (absolute, as the spec).  FP64 on the GPU."""
+    b = np.asarray(b, dtype=np.float64).reshape(-1)
+    nb = float(np.linalg.norm(b))
+    sysm = CsrSystem(A)
+    if nb == 0.0:
+        return np.zeros_like(b)
+    x, info = sysm.solve(b, precond="jacobi", precision="fp64_plain", tol=tol / nb, x0=x0,
+                         max_iter=maxIter)
+    return x.cpu().numpy()
+
+
+def cg(A: CsrMatrix, b, maxIter: int, tol: float):
+    """SparseLinAlg.cg (SparseLinAlg.lean:104-130)."""
+    return cg_with_guess(A, b, None, maxIter, tol)
+
+
+def harmonic_solve(lap: CutLaplacian, Xc, **solve_kw):
+    """HarmonicSolve.solveMulti on the sparse path: returns the nv x k block
+    with free rows solved and sample rows 0 (the caller overlays them,
+    HarmonicSolve.lean:14-16)."""
+    b = lap.harmonic_rhs(Xc, as_numpy=False)
+    x, info = lap.solve(b, **solve_kw)
+    free_of, _ = lap.maps()
+    out = np.zeros((lap.info.nv, b.shape[1]))
+    xf = x.cpu().numpy()
+    sel = free_of >= 0
+    out[sel] = xf[free_of[sel]]
+    return out, info
