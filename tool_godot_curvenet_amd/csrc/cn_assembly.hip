@@ -179,10 +179,10 @@ __global__ void k_row_values(int32_t n_free, const int32_t *free_vertex,
         if (tj[e] == j) t = t + (0.0 + mf[e]);
       if (constrained[j]) {
         int32_t pos = find_col(fc_col, b0, b1, con_of[j]);
-        fc_val[pos] = fc_val[pos] + t;
+        if (pos < b1 && fc_col[pos] == con_of[j]) fc_val[pos] = fc_val[pos] + t;
       } else {
         int32_t pos = find_col(ff_col, a0, a1, free_of[j]);
-        ff_val[pos] = ff_val[pos] + t;
+        if (pos < a1 && ff_col[pos] == free_of[j]) ff_val[pos] = ff_val[pos] + t;
       }
     }
   }

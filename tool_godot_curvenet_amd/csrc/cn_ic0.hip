@@ -82,15 +82,20 @@ __global__ void k_tri_fill(int32_t n, const int32_t *ptr, const int32_t *col,
 
 // u_map[e] = index in L of entry (row = ucol[e], col = i)
 __global__ void k_u_map(int32_t n, const int32_t *up, const int32_t *ucol,
-                        const int32_t *lp, const int32_t *lcol, int32_t *umap) {
+                        const int32_t *lp, const int32_t *lcol, int32_t *umap,
+                        int32_t *asym) {
   int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int32_t e = up[i]; e < up[i + 1]; ++e) {
     int32_t r = ucol[e];
-    int32_t a = lp[r], b = lp[r + 1];
+    int32_t a = lp[r], b = lp[r + 1], end = b;
     while (a < b) {
       int32_t m = (a + b) >> 1;
       if (lcol[m] < i) a = m + 1; else b = m;
+    }
+    if (a >= end || lcol[a] != i) {  // pattern not symmetric
+      atomicOr(asym, 1);
+      a = lp[r];
     }
     umap[e] = a;
   }
@@ -318,15 +323,27 @@ cn_status cn_ic0_factor(cn_system S, int32_t ordering, int32_t allow_shift,
     F->l_diag.alloc(n + 1);
     F->l_rdiag32.alloc(n + 1);
     DevBuf<double> adiag(n + 1), lval0(nl_nz + 1);
+    DevBuf<int32_t> asymf(1);
+    CN_HIP(hipMemsetAsync(asymf.get(), 0, sizeof(int32_t), s));
     if (n > 0) {
       k_tri_fill<<<grid_for(n, 256), 256, 0, s>>>(
           n, F->a_ptr.get(), F->a_col.get(), F->a_val.get(), F->l_ptr.get(),
           F->l_col.get(), lval0.get(), adiag.get(), F->u_ptr.get(), F->u_col.get());
       k_u_map<<<grid_for(n, 256), 256, 0, s>>>(n, F->u_ptr.get(), F->u_col.get(),
                                               F->l_ptr.get(), F->l_col.get(),
-                                              F->u_map.get());
+                                              F->u_map.get(), asymf.get());
     }
     CN_HIP(hipGetLastError());
+    {
+      int32_t asym = 0;
+      CN_HIP(hipMemcpyAsync(&asym, asymf.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      CN_HIP(hipStreamSynchronize(s));
+      if (asym != 0) {
+        delete F;
+        set_error("cn_ic0_factor: matrix pattern is not symmetric");
+        return CN_BAD_ARG;
+      }
+    }
     const double shifts[] = {0.0, 1e-4, 1e-3, 1e-2, 1e-1, 1.0};
     const int nshift = allow_shift ? 6 : 1;
     DevBuf<int32_t> fail(1);

@@ -509,6 +509,11 @@ cn_status cn_cut_seams(int32_t nv, int32_t nf, const int32_t *d_tris,
         nv, on_cut.get(), inc.ptr.get(), inc.he.get(), wstart.get(), extra.get());
   CN_HIP(hipGetLastError());
   int64_t nextra = exclusive_scan_i32(extra.get(), nv, s);
+  if (nextra > 2ll * ncut) {  // only a non-manifold fan can split this much
+    set_error("cn_cut_seams: %lld wedge copies exceed 2*ncut (non-manifold vertex?)",
+              (long long)nextra);
+    return CN_NOT_MANIFOLD;
+  }
   if (nv > 0)
     k_init_origin<<<grid_for(nv, 256), 256, 0, s>>>(nv, on_cut.get(), d_origin,
                                                    d_constrained);
