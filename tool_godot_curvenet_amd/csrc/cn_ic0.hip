@@ -143,9 +143,18 @@ __global__ void k_to_f32(int64_t n, const double *x, float *y) {
   if (i < n) y[i] = (float)x[i];
 }
 
-__global__ void k_rdiag32(int32_t n, const double *d, float *r) {
+__global__ void k_rdiag(int32_t n, const double *d, float *r32, double *r64) {
   int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) r[i] = (float)(1.0 / d[i]);
+  if (i >= n) return;
+  double r = 1.0 / d[i];
+  r32[i] = (float)r;
+  r64[i] = r;
+}
+
+__global__ void k_renumber(int64_t nnz, const int32_t *perm, const int32_t *col,
+                           int32_t *out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nnz) out[e] = perm[col[e]];
 }
 
 // Exact (spec-order) sweeps for cn_ic0_apply: thread per (row, column).
@@ -376,8 +385,19 @@ cn_status cn_ic0_factor(cn_system S, int32_t ordering, int32_t allow_shift,
       k_to_f32<<<grid_for(nl_nz, 256), 256, 0, s>>>(nl_nz, F->l_val.get(), F->l_val32.get());
     if (nu_nz > 0)
       k_to_f32<<<grid_for(nu_nz, 256), 256, 0, s>>>(nu_nz, F->u_val.get(), F->u_val32.get());
-    if (n > 0)
-      k_rdiag32<<<grid_for(n, 256), 256, 0, s>>>(n, F->l_diag.get(), F->l_rdiag32.get());
+    F->l_rdiag64.alloc(n + 1);
+    F->l_col_nat.alloc(nl_nz + 1);
+    F->u_col_nat.alloc(nu_nz + 1);
+    if (n > 0) {
+      k_rdiag<<<grid_for(n, 256), 256, 0, s>>>(n, F->l_diag.get(), F->l_rdiag32.get(),
+                                              F->l_rdiag64.get());
+    }
+    if (nl_nz > 0)
+      k_renumber<<<grid_for(nl_nz, 256), 256, 0, s>>>(nl_nz, F->perm.get(), F->l_col.get(),
+                                                     F->l_col_nat.get());
+    if (nu_nz > 0)
+      k_renumber<<<grid_for(nu_nz, 256), 256, 0, s>>>(nu_nz, F->perm.get(), F->u_col.get(),
+                                                     F->u_col_nat.get());
     CN_HIP(hipGetLastError());
     CN_HIP(hipStreamSynchronize(s));
   } catch (...) {

@@ -149,6 +149,10 @@ struct cn_ic0_t {
   cn::DevBuf<int32_t> l_ptr, l_col, u_ptr, u_col, u_map;
   cn::DevBuf<double> l_val, u_val, l_diag;
   cn::DevBuf<float> l_val32, u_val32, l_rdiag32;
-  cn::DevBuf<double> l_rdiag64;     // made on first FP64 use
+  cn::DevBuf<double> l_rdiag64;     // 1 / L_ii by level position
+  // solver copies: column indices of L / U in the original vertex numbering,
+  // so the solver keeps its vectors in natural (mesh) order and only the
+  // triangular passes walk rows through `perm`
+  cn::DevBuf<int32_t> l_col_nat, u_col_nat;
   int64_t nnz_l = 0;
 };

@@ -68,40 +68,33 @@ struct VecIO<T, 1> {
 };
 
 // acc[m] += sum_{e in [a,b)} val[e] * X[col[e]*k + c0 + m]   (wide mapping)
-template <typename T, int VEC>
+// Entries are taken CH at a time; the tail of a chunk re-reads the row's last
+// entry with weight 0, so every gather of a chunk is issued back to back
+// (one round-trip of memory latency per chunk instead of one per entry).
+template <typename T, int VEC, int CH = 8>
 __device__ __forceinline__ void gather_wide(int32_t a, int32_t b,
                                             const int32_t *__restrict__ col,
                                             const T *__restrict__ val,
                                             const T *__restrict__ X, int64_t k,
                                             int64_t c0, bool valid,
                                             T (&acc)[VEC]) {
-  int32_t e = a;
-  for (; e + 4 <= b; e += 4) {
-    int32_t j0 = col[e], j1 = col[e + 1], j2 = col[e + 2], j3 = col[e + 3];
-    T v0 = val[e], v1 = val[e + 1], v2 = val[e + 2], v3 = val[e + 3];
-    if (valid) {
-      T x0[VEC], x1[VEC], x2[VEC], x3[VEC];
-      VecIO<T, VEC>::ld(X + (int64_t)j0 * k + c0, x0);
-      VecIO<T, VEC>::ld(X + (int64_t)j1 * k + c0, x1);
-      VecIO<T, VEC>::ld(X + (int64_t)j2 * k + c0, x2);
-      VecIO<T, VEC>::ld(X + (int64_t)j3 * k + c0, x3);
+  for (int32_t e = a; e < b; e += CH) {
+    int32_t j[CH];
+    T v[CH];
 #pragma unroll
-      for (int m = 0; m < VEC; ++m) {
-        acc[m] = fma(v0, x0[m], acc[m]);
-        acc[m] = fma(v1, x1[m], acc[m]);
-        acc[m] = fma(v2, x2[m], acc[m]);
-        acc[m] = fma(v3, x3[m], acc[m]);
-      }
+    for (int u = 0; u < CH; ++u) {
+      const int32_t q = min(e + u, b - 1);
+      j[u] = col[q];
+      v[u] = (e + u < b) ? val[q] : T(0);
     }
-  }
-  for (; e < b; ++e) {
-    int32_t j = col[e];
-    T v = val[e];
     if (valid) {
-      T x[VEC];
-      VecIO<T, VEC>::ld(X + (int64_t)j * k + c0, x);
+      T x[CH][VEC];
 #pragma unroll
-      for (int m = 0; m < VEC; ++m) acc[m] = fma(v, x[m], acc[m]);
+      for (int u = 0; u < CH; ++u) VecIO<T, VEC>::ld(X + (int64_t)j[u] * k + c0, x[u]);
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+#pragma unroll
+        for (int m = 0; m < VEC; ++m) acc[m] = fma(v[u], x[u][m], acc[m]);
     }
   }
 }

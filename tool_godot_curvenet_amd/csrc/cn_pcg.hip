@@ -1,29 +1,31 @@
 // cn_pcg.hip -- preconditioned conjugate gradients on gfx950.
 //
-// SparseLinAlg.cg / cgWithGuess (SparseLinAlg.lean:104-163) for k
-// right-hand sides in lockstep (per-column alpha/beta, as the reference's
-// *Multi kernels, SlangCodegen/AxpyMulti.lean:68-69).
+// SparseLinAlg.cg / cgWithGuess (SparseLinAlg.lean:104-163) for k right-hand
+// sides in lockstep (per-column alpha/beta, as the reference's *Multi
+// kernels, SlangCodegen/AxpyMulti.lean:68-69).  All vectors are n x k
+// row-major in the system's vertex order; only the IC(0) passes walk rows in
+// level (colour) order through `perm`.
 //
-// CN_PREC_MIXED: an FP64 iterative-refinement loop
-//     r = b - A x (FP64 values, FP64 vectors)      -- the true residual
-//     solve A d = r / ||r|| with FP32 PCG            -- SpMM, IC(0), p/z/q in FP32
-//     x += ||r|| d (FP64)
-// FP32 is used for everything that streams (matrix values, factor, search
-// directions); dots, alpha/beta and the stop test are FP64.
+// One iteration (C = IC(0) levels), all scalars on the device:
+//   SpMM   q = A p              (+ partial p.q)   -> alpha = rz / p.q
+//   C fwd  r -= alpha q; y = L^-1 r  (+ partial r.r, y.y)
+//                                                 -> rz' = y.y (= r.M^-1 r),
+//                                                    beta, stop test
+//   C bwd  z = L^-T y; d += alpha p; p = z + beta p
+// so the vectors move 12 times per iteration (SpMM 2, fwd 4, bwd 6) and
+// there are 2C + 3 launches.  The host reads a 4-int control block every
+// `check_every` iterations; kernels return at once when no column is active.
 //
-// One inner iteration is (C = number of IC(0) levels):
-//     SpMM q = A p  (+ partial p.q)     -> reduce: alpha = rz / p.q
-//     C forward passes, each fused with d += alpha p, r -= alpha q (+ r.r)
-//     C backward passes (+ r.z)         -> reduce: beta, stop flags
-//     p = z + beta p
-// All scalars live on the device; the host only reads the active-column
-// counter every `check_every` iterations.  Kernels read that counter first
-// and return at once when it is zero.
-//
-// CN_PREC_FP64 runs the same kernels in FP64.  CN_PREC_FP64_PLAIN is the
-// unfused baseline: separate SpMV / dot / axpy / preconditioner / saxpby
-// launches with alpha, the stop test and beta computed on the host
-// (three device->host round trips per iteration).
+// CN_PREC_MIXED keeps FP32 vectors (r, y/z, p, q, d) and FP32 matrix/factor
+// values, with FP64 dots and scalars, plus "reliable updates": when a
+// column's iterated residual has dropped 10x since the last update, the host
+// folds d into the FP64 solution x, recomputes r = b - A x in FP64 and
+// continues CG with that residual and the same search direction.  The
+// Krylov space is kept (no restart), so the iteration count stays near the
+// FP64 count, while the stop test and x are FP64.  CN_PREC_FP64 runs the
+// same loop in FP64.  CN_PREC_FP64_PLAIN is the unfused baseline: one launch
+// per BLAS-1 / SpMV / preconditioner step with alpha, the stop test and beta
+// computed on the host (three device->host round trips per iteration).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,6 +39,20 @@
 
 namespace cn {
 
+// per-column scalar slots (st[slot * k + c])
+enum {
+  ST_ALPHA = 0, ST_BETA, ST_GAMMA, ST_RZ, ST_TOL2, ST_RREF, ST_S, ST_INVS,
+  ST_BB, ST_RR, ST_TMP, ST_N
+};
+// control block
+enum { CTL_ACTIVE = 0, CTL_BREAKDOWN = 1, CTL_ONE = 2, CTL_TRIGGER = 3, CTL_JUST = 4,
+       CTL_N = 8 };
+// CTL_JUST counts columns stopped by this iteration's k_reduce_beta: the
+// backward pass that follows still has to apply their last d += alpha p.
+__device__ __forceinline__ bool bwd_idle(const int32_t *ctl) {
+  return ctl[CTL_ACTIVE] == 0 && ctl[CTL_JUST] == 0;
+}
+
 // ================================================================ kernels ==
 #define NARROW_LOOP(row, n0, n1)                                              \
   for (int64_t row = (int64_t)(n0) + (int64_t)blockIdx.x * (kBlock / kGroup) + \
@@ -45,14 +61,18 @@ namespace cn {
 #define WIDE_LOOP(row, n0, n1)                                                \
   for (int32_t row = (n0) + blockIdx.x * kWaves + (threadIdx.x >> 6);        \
        row < (n1); row += gridDim.x * kWaves)
+#define WIDE_COLS(VEC)                                                        \
+  const int lane = threadIdx.x & 63;                                          \
+  const int64_t c0 = (int64_t)blockIdx.y * 64 * (VEC) + lane * (VEC);         \
+  const bool valid = c0 < k;
 
 // ---- SpMM with partial p.q -------------------------------------------
 template <typename T, int K>
 __global__ void __launch_bounds__(kBlock)
 kn_spmm_dot(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
             const T *__restrict__ val, const T *__restrict__ P, T *__restrict__ Q,
-            double *__restrict__ part, const int32_t *__restrict__ n_active) {
-  if (*n_active == 0) return;
+            double *__restrict__ part, const int32_t *__restrict__ ctl) {
+  if (ctl[CTL_ACTIVE] == 0) return;
   const int g = threadIdx.x % kGroup;
   double pd[1] = {0.0};
   NARROW_LOOP(row, 0, n) {
@@ -72,11 +92,9 @@ __global__ void __launch_bounds__(kBlock)
 kw_spmm_dot(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
             const T *__restrict__ val, int64_t k, const T *__restrict__ P,
             T *__restrict__ Q, double *__restrict__ part,
-            const int32_t *__restrict__ n_active) {
-  if (*n_active == 0) return;
-  const int lane = threadIdx.x & 63;
-  const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
-  const bool valid = c0 < k;
+            const int32_t *__restrict__ ctl) {
+  if (ctl[CTL_ACTIVE] == 0) return;
+  WIDE_COLS(VEC)
   double pd[1][VEC] = {};
   WIDE_LOOP(row, 0, n) {
     const int32_t r = uniform(row);
@@ -121,9 +139,7 @@ kw_resid(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__
          const double *__restrict__ val, int64_t k, const double *__restrict__ X,
          const double *__restrict__ B, double *__restrict__ R,
          double *__restrict__ part) {
-  const int lane = threadIdx.x & 63;
-  const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
-  const bool valid = c0 < k;
+  WIDE_COLS(VEC)
   double pd[1][VEC] = {};
   WIDE_LOOP(row, 0, n) {
     const int32_t r = uniform(row);
@@ -144,154 +160,164 @@ kw_resid(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__
   block_partials_wide<VEC, 1>(pd, part, k, c0, 0);
 }
 
-// ---- IC(0) forward pass over one level, optionally fused with the CG
-//      update d += alpha p, r -= alpha q; partial r.r -----------------------
+// ---- IC(0) forward pass over level positions [r0, r1):
+//      (FUSED) r -= alpha q;  y = (r - sum L y) / L_ii;  partials r.r, y.y
 template <typename T, int K, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
-kn_fwd(int32_t r0, int32_t r1, const int32_t *__restrict__ lp,
-       const int32_t *__restrict__ lcol, const T *__restrict__ lval,
-       const T *__restrict__ rdiag, const double *__restrict__ alpha,
-       const T *__restrict__ P, const T *__restrict__ Q, T *__restrict__ D,
-       T *__restrict__ R, T *__restrict__ Y, double *__restrict__ part,
-       const int32_t *__restrict__ n_active) {
-  if (*n_active == 0) return;
+kn_fwd(int32_t r0, int32_t r1, const int32_t *__restrict__ rowmap,
+       const int32_t *__restrict__ lp, const int32_t *__restrict__ lcol,
+       const T *__restrict__ lval, const T *__restrict__ rdiag,
+       const double *__restrict__ alpha, const T *__restrict__ Q, T *__restrict__ R,
+       T *__restrict__ Y, double *__restrict__ part, int64_t stride,
+       const int32_t *__restrict__ ctl) {
+  if (ctl[CTL_ACTIVE] == 0) return;
   const int g = threadIdx.x % kGroup;
-  double pd[1] = {0.0};
+  double pd[2] = {0.0, 0.0};
   const T al = (FUSED && g < K) ? (T)alpha[g] : T(0);
-  NARROW_LOOP(row, r0, r1) {
+  NARROW_LOOP(pos, r0, r1) {
+    const int64_t row = rowmap[pos];
     T acc[K] = {};
-    gather_narrow<T, K>(lp[row], lp[row + 1], g, lcol, lval, Y, acc);
+    gather_narrow<T, K>(lp[pos], lp[pos + 1], g, lcol, lval, Y, acc);
     if (g < K) {
       const int64_t o = row * K + g;
       T r = R[o];
       if (FUSED) {
-        D[o] = fma(al, P[o], D[o]);
         r = fma(-al, Q[o], r);
         R[o] = r;
       }
+      T y = (r - pick<T, K>(acc, g)) * rdiag[pos];
+      Y[o] = y;
       pd[0] += (double)r * (double)r;
-      Y[o] = (r - pick<T, K>(acc, g)) * rdiag[row];
+      pd[1] += (double)y * (double)y;
     }
   }
-  block_partials_narrow<K, 1>(pd, part, 0);
+  block_partials_narrow<K, 2>(pd, part, stride);
 }
 
 template <typename T, int VEC, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
-kw_fwd(int32_t r0, int32_t r1, const int32_t *__restrict__ lp,
-       const int32_t *__restrict__ lcol, const T *__restrict__ lval,
-       const T *__restrict__ rdiag, const double *__restrict__ alpha, int64_t k,
-       const T *__restrict__ P, const T *__restrict__ Q, T *__restrict__ D,
+kw_fwd(int32_t r0, int32_t r1, const int32_t *__restrict__ rowmap,
+       const int32_t *__restrict__ lp, const int32_t *__restrict__ lcol,
+       const T *__restrict__ lval, const T *__restrict__ rdiag,
+       const double *__restrict__ alpha, int64_t k, const T *__restrict__ Q,
        T *__restrict__ R, T *__restrict__ Y, double *__restrict__ part,
-       const int32_t *__restrict__ n_active) {
-  if (*n_active == 0) return;
-  const int lane = threadIdx.x & 63;
-  const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
-  const bool valid = c0 < k;
-  double pd[1][VEC] = {};
+       int64_t stride, const int32_t *__restrict__ ctl) {
+  if (ctl[CTL_ACTIVE] == 0) return;
+  WIDE_COLS(VEC)
+  double pd[2][VEC] = {};
   T al[VEC];
 #pragma unroll
   for (int m = 0; m < VEC; ++m) al[m] = (FUSED && valid) ? (T)alpha[c0 + m] : T(0);
-  WIDE_LOOP(row, r0, r1) {
-    const int32_t r = uniform(row);
-    const int32_t a = uniform(lp[r]), b = uniform(lp[r + 1]);
+  WIDE_LOOP(pos, r0, r1) {
+    const int32_t ps = uniform(pos);
+    const int32_t r = uniform(rowmap[ps]);
+    const int32_t a = uniform(lp[ps]), b = uniform(lp[ps + 1]);
     T acc[VEC] = {};
-    gather_wide<T, VEC>(a, b, lcol, lval, Y, k, c0, valid, acc);
+    gather_wide<T, VEC, 4>(a, b, lcol, lval, Y, k, c0, valid, acc);
     if (valid) {
       const int64_t o = (int64_t)r * k + c0;
       T rv[VEC];
       VecIO<T, VEC>::ld(R + o, rv);
       if (FUSED) {
-        T pv[VEC], qv[VEC], dv[VEC];
-        VecIO<T, VEC>::ld(P + o, pv);
+        T qv[VEC];
         VecIO<T, VEC>::ld(Q + o, qv);
-        VecIO<T, VEC>::ld(D + o, dv);
 #pragma unroll
-        for (int m = 0; m < VEC; ++m) {
-          dv[m] = fma(al[m], pv[m], dv[m]);
-          rv[m] = fma(-al[m], qv[m], rv[m]);
-        }
-        VecIO<T, VEC>::st(D + o, dv);
+        for (int m = 0; m < VEC; ++m) rv[m] = fma(-al[m], qv[m], rv[m]);
         VecIO<T, VEC>::st(R + o, rv);
       }
-      const T rd = rdiag[r];
+      const T rd = rdiag[ps];
 #pragma unroll
       for (int m = 0; m < VEC; ++m) {
-        pd[0][m] += (double)rv[m] * (double)rv[m];
         acc[m] = (rv[m] - acc[m]) * rd;
+        pd[0][m] += (double)rv[m] * (double)rv[m];
+        pd[1][m] += (double)acc[m] * (double)acc[m];
       }
       VecIO<T, VEC>::st(Y + o, acc);
     }
   }
-  block_partials_wide<VEC, 1>(pd, part, k, c0, 0);
+  block_partials_wide<VEC, 2>(pd, part, k, c0, stride);
 }
 
-// ---- IC(0) backward pass over one level (in place on Z), partial r.z ------
+// ---- IC(0) backward pass over level positions [r0, r1), in place on Z
+//      (which holds y):  z = (y - sum U z) / L_ii;
+//      d += alpha p;  p = gamma z + beta p        (per-column alpha/beta/gamma)
 template <typename T, int K>
 __global__ void __launch_bounds__(kBlock)
-kn_bwd(int32_t r0, int32_t r1, const int32_t *__restrict__ up,
-       const int32_t *__restrict__ ucol, const T *__restrict__ uval,
-       const T *__restrict__ rdiag, const T *__restrict__ R, T *__restrict__ Z,
-       double *__restrict__ part, const int32_t *__restrict__ n_active) {
-  if (*n_active == 0) return;
+kn_bwd(int32_t r0, int32_t r1, const int32_t *__restrict__ rowmap,
+       const int32_t *__restrict__ up, const int32_t *__restrict__ ucol,
+       const T *__restrict__ uval, const T *__restrict__ rdiag,
+       const double *__restrict__ st, int64_t k, T *__restrict__ Z,
+       T *__restrict__ P, T *__restrict__ D, const int32_t *__restrict__ ctl) {
+  if (bwd_idle(ctl)) return;
   const int g = threadIdx.x % kGroup;
-  double pd[1] = {0.0};
-  NARROW_LOOP(row, r0, r1) {
+  const int gc = g < K ? g : 0;
+  const T al = (T)st[ST_ALPHA * k + gc], be = (T)st[ST_BETA * k + gc],
+          ga = (T)st[ST_GAMMA * k + gc];
+  NARROW_LOOP(pos, r0, r1) {
+    const int64_t row = rowmap[pos];
     T acc[K] = {};
-    gather_narrow<T, K>(up[row], up[row + 1], g, ucol, uval, Z, acc);
+    gather_narrow<T, K>(up[pos], up[pos + 1], g, ucol, uval, Z, acc);
     if (g < K) {
       const int64_t o = row * K + g;
-      T z = (Z[o] - pick<T, K>(acc, g)) * rdiag[row];
+      T z = (Z[o] - pick<T, K>(acc, g)) * rdiag[pos];
       Z[o] = z;
-      pd[0] += (double)R[o] * (double)z;
+      T p = P[o];
+      D[o] = fma(al, p, D[o]);
+      P[o] = fma(be, p, ga * z);
     }
   }
-  block_partials_narrow<K, 1>(pd, part, 0);
 }
 
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
-kw_bwd(int32_t r0, int32_t r1, const int32_t *__restrict__ up,
-       const int32_t *__restrict__ ucol, const T *__restrict__ uval,
-       const T *__restrict__ rdiag, int64_t k, const T *__restrict__ R,
-       T *__restrict__ Z, double *__restrict__ part,
-       const int32_t *__restrict__ n_active) {
-  if (*n_active == 0) return;
-  const int lane = threadIdx.x & 63;
-  const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
-  const bool valid = c0 < k;
-  double pd[1][VEC] = {};
-  WIDE_LOOP(row, r0, r1) {
-    const int32_t r = uniform(row);
-    const int32_t a = uniform(up[r]), b = uniform(up[r + 1]);
+kw_bwd(int32_t r0, int32_t r1, const int32_t *__restrict__ rowmap,
+       const int32_t *__restrict__ up, const int32_t *__restrict__ ucol,
+       const T *__restrict__ uval, const T *__restrict__ rdiag,
+       const double *__restrict__ st, int64_t k, T *__restrict__ Z,
+       T *__restrict__ P, T *__restrict__ D, const int32_t *__restrict__ ctl) {
+  if (bwd_idle(ctl)) return;
+  WIDE_COLS(VEC)
+  T al[VEC], be[VEC], ga[VEC];
+#pragma unroll
+  for (int m = 0; m < VEC; ++m) {
+    al[m] = valid ? (T)st[ST_ALPHA * k + c0 + m] : T(0);
+    be[m] = valid ? (T)st[ST_BETA * k + c0 + m] : T(0);
+    ga[m] = valid ? (T)st[ST_GAMMA * k + c0 + m] : T(0);
+  }
+  WIDE_LOOP(pos, r0, r1) {
+    const int32_t ps = uniform(pos);
+    const int32_t r = uniform(rowmap[ps]);
+    const int32_t a = uniform(up[ps]), b = uniform(up[ps + 1]);
     T acc[VEC] = {};
-    gather_wide<T, VEC>(a, b, ucol, uval, Z, k, c0, valid, acc);
+    gather_wide<T, VEC, 4>(a, b, ucol, uval, Z, k, c0, valid, acc);
     if (valid) {
       const int64_t o = (int64_t)r * k + c0;
-      T yv[VEC], rv[VEC];
+      T yv[VEC], pv[VEC], dv[VEC];
       VecIO<T, VEC>::ld(Z + o, yv);
-      VecIO<T, VEC>::ld(R + o, rv);
-      const T rd = rdiag[r];
+      VecIO<T, VEC>::ld(P + o, pv);
+      VecIO<T, VEC>::ld(D + o, dv);
+      const T rd = rdiag[ps];
 #pragma unroll
       for (int m = 0; m < VEC; ++m) {
         acc[m] = (yv[m] - acc[m]) * rd;
-        pd[0][m] += (double)rv[m] * (double)acc[m];
+        dv[m] = fma(al[m], pv[m], dv[m]);
+        pv[m] = fma(be[m], pv[m], ga[m] * acc[m]);
       }
       VecIO<T, VEC>::st(Z + o, acc);
+      VecIO<T, VEC>::st(P + o, pv);
+      VecIO<T, VEC>::st(D + o, dv);
     }
   }
-  block_partials_wide<VEC, 1>(pd, part, k, c0, 0);
 }
 
-// ---- Jacobi (SparseLinAlg.applyJacobi, zero diagonal -> 0), fused update,
-//      partials r.r and r.z ------------------------------------------------
+// ---- Jacobi (SparseLinAlg.applyJacobi, zero diagonal -> 0): z = r / d,
+//      (FUSED) r -= alpha q first; partials r.r and r.z -----------------------
 template <typename T, int K, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
-kn_jacobi(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alpha,
-          const T *__restrict__ P, const T *__restrict__ Q, T *__restrict__ D,
-          T *__restrict__ R, T *__restrict__ Z, double *__restrict__ part,
-          int64_t stride, const int32_t *__restrict__ n_active) {
-  if (*n_active == 0) return;
+kn_jac(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alpha,
+       const T *__restrict__ Q, T *__restrict__ R, T *__restrict__ Z,
+       double *__restrict__ part, int64_t stride, const int32_t *__restrict__ ctl) {
+  if (ctl[CTL_ACTIVE] == 0) return;
   const int g = threadIdx.x % kGroup;
   double pd[2] = {0.0, 0.0};
   const T al = (FUSED && g < K) ? (T)alpha[g] : T(0);
@@ -300,7 +326,6 @@ kn_jacobi(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alph
       const int64_t o = row * K + g;
       T r = R[o];
       if (FUSED) {
-        D[o] = fma(al, P[o], D[o]);
         r = fma(-al, Q[o], r);
         R[o] = r;
       }
@@ -315,15 +340,11 @@ kn_jacobi(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alph
 
 template <typename T, int VEC, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
-kw_jacobi(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alpha,
-          int64_t k, const T *__restrict__ P, const T *__restrict__ Q,
-          T *__restrict__ D, T *__restrict__ R, T *__restrict__ Z,
-          double *__restrict__ part, int64_t stride,
-          const int32_t *__restrict__ n_active) {
-  if (*n_active == 0) return;
-  const int lane = threadIdx.x & 63;
-  const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
-  const bool valid = c0 < k;
+kw_jac(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alpha,
+       int64_t k, const T *__restrict__ Q, T *__restrict__ R, T *__restrict__ Z,
+       double *__restrict__ part, int64_t stride, const int32_t *__restrict__ ctl) {
+  if (ctl[CTL_ACTIVE] == 0) return;
+  WIDE_COLS(VEC)
   double pd[2][VEC] = {};
   T al[VEC];
 #pragma unroll
@@ -334,16 +355,10 @@ kw_jacobi(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alph
       T rv[VEC], zv[VEC];
       VecIO<T, VEC>::ld(R + o, rv);
       if (FUSED) {
-        T pv[VEC], qv[VEC], dv[VEC];
-        VecIO<T, VEC>::ld(P + o, pv);
+        T qv[VEC];
         VecIO<T, VEC>::ld(Q + o, qv);
-        VecIO<T, VEC>::ld(D + o, dv);
 #pragma unroll
-        for (int m = 0; m < VEC; ++m) {
-          dv[m] = fma(al[m], pv[m], dv[m]);
-          rv[m] = fma(-al[m], qv[m], rv[m]);
-        }
-        VecIO<T, VEC>::st(D + o, dv);
+        for (int m = 0; m < VEC; ++m) rv[m] = fma(-al[m], qv[m], rv[m]);
         VecIO<T, VEC>::st(R + o, rv);
       }
       const T di = dinv[row];
@@ -359,7 +374,7 @@ kw_jacobi(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alph
   block_partials_wide<VEC, 2>(pd, part, k, c0, stride);
 }
 
-// ---- generic two-vector column dot (plain path): partial a.b -------------
+// ---- generic two-vector column dot: partial a.b -------------------------
 template <typename T, int K>
 __global__ void __launch_bounds__(kBlock)
 kn_dot(int32_t n, const T *__restrict__ A, const T *__restrict__ B,
@@ -376,9 +391,7 @@ template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
 kw_dot(int32_t n, int64_t k, const T *__restrict__ A, const T *__restrict__ B,
        double *__restrict__ part) {
-  const int lane = threadIdx.x & 63;
-  const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
-  const bool valid = c0 < k;
+  WIDE_COLS(VEC)
   double pd[1][VEC] = {};
   WIDE_LOOP(row, 0, n) {
     if (valid) {
@@ -410,14 +423,44 @@ __device__ __forceinline__ double col_reduce(const double *__restrict__ part,
   return t;
 }
 
-enum { ST_ALPHA = 0, ST_BETA, ST_RZ, ST_TOL2, ST_NORM, ST_SCALE, ST_BB, ST_RR, ST_TMP, ST_N };
+// First stage of the partial reduction: block (x, y) sums rows
+// [y*chunk, (y+1)*chunk) of `np` quantities for 64 columns, in a fixed
+// order, so that the second stage reads at most kRedRows rows.
+constexpr int kRedRows = 64;
+__global__ void __launch_bounds__(kBlock)
+k_reduce_stage1(int32_t nrows, int32_t chunk, int64_t k, const double *__restrict__ part,
+                int64_t stride, int np, double *__restrict__ out, int64_t ostride) {
+  __shared__ double sh[4][64];
+  const int64_t c = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int s = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int32_t b0 = blockIdx.y * chunk, b1 = min(nrows, b0 + chunk);
+  for (int q = 0; q < np; ++q) {
+    double v = 0.0;
+    if (c < k)
+      for (int32_t b = b0 + s; b < b1; b += 4) v += part[q * stride + (int64_t)b * k + c];
+    sh[s][lane] = v;
+    __syncthreads();
+    if (s == 0 && c < k) {
+      double t = sh[0][lane];
+      t += sh[1][lane];
+      t += sh[2][lane];
+      t += sh[3][lane];
+      out[q * ostride + (int64_t)blockIdx.y * k + c] = t;
+    }
+    __syncthreads();
+  }
+}
 
-// alpha = rz / p.q  (breakdown p.q == 0 deactivates the column)
+// alpha = rz / p.q for active columns (p.q == 0 or non-finite: breakdown,
+// SparseLinAlg.lean:118 -- the column stops)
 __global__ void __launch_bounds__(kBlock)
 k_reduce_alpha(int32_t nrows, int64_t k, const double *__restrict__ part,
                double *__restrict__ st, int32_t *__restrict__ active,
                int32_t *__restrict__ ctl) {
-  if (ctl[0] == 0) return;
+  const bool idle = ctl[CTL_ACTIVE] == 0;
+  __syncthreads();  // every thread has read ctl before block 0 clears JUST
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[CTL_JUST] = 0;
+  if (idle) return;
   __shared__ double sh[4][64];
   const int64_t c = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   double pq = col_reduce(part, nrows, k, c, sh);
@@ -428,52 +471,56 @@ k_reduce_alpha(int32_t nrows, int64_t k, const double *__restrict__ part,
         a = st[ST_RZ * k + c] / pq;
       } else {
         active[c] = 0;
-        atomicSub(&ctl[0], 1);
-        atomicOr(&ctl[1], 1);
+        atomicSub(&ctl[CTL_ACTIVE], 1);
+        atomicOr(&ctl[CTL_BREAKDOWN], 1);
       }
     }
     st[ST_ALPHA * k + c] = a;
   }
 }
 
-// after z = M^-1 r: rr and rz partials; mode 0 = first (sets rz and the
-// inner tolerance), mode 1 = iteration (beta, stop test)
+// After the forward pass: r.r and r.M^-1.r partials.
+//   mode 0 (iteration): stop test rr <= tol2, reliable-update trigger
+//          rr < delta2 * rref, beta = rz'/rz, gamma = 1 (inactive: beta 1,
+//          gamma 0 -> p, d frozen)
+//   mode 1 (after a residual replacement / start): rz = rz', rref = rr;
+//          continuing columns keep p (beta 1, gamma 0), restarted ones take
+//          p = z (beta 0, gamma 1)
 __global__ void __launch_bounds__(kBlock)
 k_reduce_beta(int32_t nrows, int64_t k, const double *__restrict__ prr,
               const double *__restrict__ prz, double *__restrict__ st,
-              int32_t *__restrict__ active, int32_t *__restrict__ ctl, int mode,
-              double inner_tol2) {
-  if (ctl[0] == 0) return;
+              int32_t *__restrict__ active, const int32_t *__restrict__ restart,
+              int32_t *__restrict__ ctl, int mode, double delta2) {
+  if (ctl[CTL_ACTIVE] == 0) return;
   __shared__ double sh[4][64];
   const int64_t c = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   double rr = col_reduce(prr, nrows, k, c, sh);
   double rz = col_reduce(prz, nrows, k, c, sh);
   if (threadIdx.x < 64 && c < k) {
-    double beta = 0.0;
+    double beta = 1.0, gamma = 0.0;
     if (active[c]) {
-      if (mode == 0) {
-        if (rr > 0.0 && isfinite(rr)) {
-          st[ST_TOL2 * k + c] = inner_tol2 * rr;
-          st[ST_RZ * k + c] = rz;
-        } else {
-          active[c] = 0;
-          atomicSub(&ctl[0], 1);
-        }
-      } else if (!(rr > st[ST_TOL2 * k + c]) || !isfinite(rr)) {
+      if (mode == 1) {
+        st[ST_RZ * k + c] = rz;
+        st[ST_RREF * k + c] = rr;
+        if (restart[c]) { beta = 0.0; gamma = 1.0; }
+      } else if (!(rr > st[ST_TOL2 * k + c]) || !isfinite(rr) || !isfinite(rz)) {
         active[c] = 0;
-        atomicSub(&ctl[0], 1);
-        if (!isfinite(rr)) atomicOr(&ctl[1], 1);
+        atomicSub(&ctl[CTL_ACTIVE], 1);
+        atomicAdd(&ctl[CTL_JUST], 1);
+        if (!isfinite(rr) || !isfinite(rz)) atomicOr(&ctl[CTL_BREAKDOWN], 1);
       } else {
         double old = st[ST_RZ * k + c];
         beta = old == 0.0 ? 0.0 : rz / old;
+        gamma = 1.0;
         st[ST_RZ * k + c] = rz;
+        if (rr < delta2 * st[ST_RREF * k + c]) atomicOr(&ctl[CTL_TRIGGER], 1);
       }
     }
     st[ST_BETA * k + c] = beta;
+    st[ST_GAMMA * k + c] = gamma;
   }
 }
 
-// plain reduction into st[slot]
 __global__ void __launch_bounds__(kBlock)
 k_reduce_plain(int32_t nrows, int64_t k, const double *__restrict__ part,
                double *__restrict__ out) {
@@ -484,64 +531,56 @@ k_reduce_plain(int32_t nrows, int64_t k, const double *__restrict__ part,
 }
 
 // ---- elementwise -----------------------------------------------------------
-// P = Z + beta P
+// plain path: P = Z + beta P
 template <typename T>
 __global__ void k_p_update(int64_t total, int64_t k, const double *__restrict__ beta,
-                           const T *__restrict__ Z, T *__restrict__ P,
-                           const int32_t *__restrict__ ctl) {
-  if (ctl[0] == 0) return;
+                           const T *__restrict__ Z, T *__restrict__ P) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x)
+    P[i] = fma((T)beta[i % k], P[i], Z[i]);
+}
+
+// Jacobi second half: d += alpha p; p = gamma z + beta p
+template <typename T>
+__global__ void k_pd_update(int64_t total, int64_t k, const double *__restrict__ st,
+                            const T *__restrict__ Z, T *__restrict__ P,
+                            T *__restrict__ D, const int32_t *__restrict__ ctl) {
+  if (bwd_idle(ctl)) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
-    int64_t c = i % k;
-    P[i] = fma((T)beta[c], P[i], Z[i]);
+    const int64_t c = i % k;
+    T p = P[i];
+    D[i] = fma((T)st[ST_ALPHA * k + c], p, D[i]);
+    P[i] = fma((T)st[ST_BETA * k + c], p, (T)st[ST_GAMMA * k + c] * Z[i]);
   }
 }
 
-// Y = (T)(X * scale_c)
+// Y = (T)(X * s_c)
 template <typename T>
-__global__ void k_scale_in(int64_t total, int64_t k, const double *__restrict__ scale,
+__global__ void k_scale_in(int64_t total, int64_t k, const double *__restrict__ s,
                            const double *__restrict__ X, T *__restrict__ Y) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x)
-    Y[i] = (T)(X[i] * scale[i % k]);
+    Y[i] = (T)(X[i] * s[i % k]);
 }
 
-// X += norm_c * D
+// X += invs_c * D; D = 0
 template <typename T>
-__global__ void k_x_update(int64_t total, int64_t k, const double *__restrict__ norm,
-                           const T *__restrict__ D, double *__restrict__ X) {
+__global__ void k_x_update(int64_t total, int64_t k, const double *__restrict__ invs,
+                           T *__restrict__ D, double *__restrict__ X) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x)
-    X[i] = fma(norm[i % k], (double)D[i], X[i]);
+       i += (int64_t)gridDim.x * blockDim.x) {
+    X[i] = fma(invs[i % k], (double)D[i], X[i]);
+    D[i] = T(0);
+  }
 }
 
-// y += a_c x  (plain axpy with per-column scalar, SlangCodegen/AxpyMulti.lean)
 template <typename T>
 __global__ void k_axpy(int64_t total, int64_t k, const double *__restrict__ a,
                        double sign, const T *__restrict__ x, T *__restrict__ y) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x)
     y[i] = y[i] + (T)(sign * a[i % k]) * x[i];
-}
-
-__global__ void k_perm_gather(int32_t n, int64_t k, const int32_t *__restrict__ perm,
-                              const double *__restrict__ src, double *__restrict__ dst) {
-  int64_t tot = (int64_t)n * k;
-  for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < tot;
-       id += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = id / k, c = id - i * k;
-    dst[id] = src[(int64_t)(perm ? perm[i] : i) * k + c];
-  }
-}
-
-__global__ void k_perm_scatter(int32_t n, int64_t k, const int32_t *__restrict__ perm,
-                               const double *__restrict__ src, double *__restrict__ dst) {
-  int64_t tot = (int64_t)n * k;
-  for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < tot;
-       id += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = id / k, c = id - i * k;
-    dst[(int64_t)(perm ? perm[i] : i) * k + c] = src[id];
-  }
 }
 
 __global__ void k_dinv(int32_t n, const double *__restrict__ d, double *__restrict__ d64,
@@ -551,11 +590,6 @@ __global__ void k_dinv(int32_t n, const double *__restrict__ d, double *__restri
   double v = d[i] == 0.0 ? 0.0 : 1.0 / d[i];
   d64[i] = v;
   d32[i] = (float)v;
-}
-
-__global__ void k_rdiag64(int32_t n, const double *__restrict__ d, double *__restrict__ r) {
-  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) r[i] = 1.0 / d[i];
 }
 
 // ============================================================== profiling ==
@@ -570,7 +604,7 @@ struct Prof {
 static thread_local Prof g_prof;
 
 struct ProfScope {
-  Prof::Rec *rec = nullptr;
+  int idx = -1;
   hipStream_t s;
   ProfScope(const char *name, hipStream_t st) : s(st) {
     Prof &P = g_prof;
@@ -586,17 +620,16 @@ struct ProfScope {
     }
     hipEvent_t a = P.pool[P.used++], b = P.pool[P.used++];
     P.recs.push_back({name, a, b});
-    rec = &P.recs.back();
+    idx = (int)P.recs.size() - 1;
     (void)hipEventRecord(a, s);
   }
   ~ProfScope() {
-    if (rec) (void)hipEventRecord(rec->b, s);
+    if (idx >= 0) (void)hipEventRecord(g_prof.recs[idx].b, s);
   }
 };
 
 static void prof_reset() {
   g_prof.recs.clear();
-  g_prof.recs.reserve(32768);
   g_prof.used = 0;
   g_prof.result.clear();
 }
@@ -620,12 +653,13 @@ static void prof_collect() {
 // ============================================================== workspace ==
 struct SolverWork {
   int32_t n = 0, k = 0, part_rows = 0;
-  DevBuf<double> xp, rp, bp;          // FP64 n x k: x, true residual, b
+  DevBuf<double> xp, rp;              // FP64 n x k: x, true residual
   DevBuf<double> buf[5];              // inner D, R, Z, P, Q (8 bytes/elem)
   DevBuf<double> part;                // 3 x part_rows x k
+  DevBuf<double> red;                 // 2 x kRedRows x k, first-stage sums
   DevBuf<double> st;                  // ST_N x k per-column scalars
-  DevBuf<int32_t> active;             // k
-  DevBuf<int32_t> ctl;                // [0] active count, [1] breakdown, [2] always 1
+  DevBuf<int32_t> active, restart;    // k
+  DevBuf<int32_t> ctl;                // CTL_N ints
   DevBuf<double> dinv64;
   DevBuf<float> dinv32;
   int32_t *h_ctl = nullptr;           // pinned
@@ -638,8 +672,7 @@ struct SolverWork {
 
 void free_solver_work(SolverWork *w) { delete w; }
 
-static SolverWork *ensure_work(cn_system_t *S, int32_t k, int32_t part_rows,
-                               hipStream_t s) {
+static SolverWork *ensure_work(cn_system_t *S, int32_t k, int32_t part_rows) {
   SolverWork *W = S->work;
   const int32_t n = S->n_free;
   if (W && W->n == n && W->k >= k && W->part_rows >= part_rows) return W;
@@ -654,21 +687,18 @@ static SolverWork *ensure_work(cn_system_t *S, int32_t k, int32_t part_rows,
   size_t nk = (size_t)n * kk + 4;
   W->xp.alloc(nk);
   W->rp.alloc(nk);
-  W->bp.alloc(nk);
   for (auto &b : W->buf) b.alloc(nk);
   W->part.alloc((size_t)3 * pr * kk + 4);
+  W->red.alloc((size_t)2 * kRedRows * kk + 4);
   W->st.alloc((size_t)ST_N * kk + 4);
   W->active.alloc((size_t)kk + 4);
-  W->ctl.alloc(8);
+  W->restart.alloc((size_t)kk + 4);
+  W->ctl.alloc(CTL_N);
   W->dinv64.alloc((size_t)n + 1);
   W->dinv32.alloc((size_t)n + 1);
-  CN_HIP(hipHostMalloc((void **)&W->h_ctl, sizeof(int32_t) * 8, hipHostMallocDefault));
+  CN_HIP(hipHostMalloc((void **)&W->h_ctl, sizeof(int32_t) * CTL_N, hipHostMallocDefault));
   CN_HIP(hipHostMalloc((void **)&W->h_st, sizeof(double) * ((size_t)ST_N * kk + 4),
                        hipHostMallocDefault));
-  if (n > 0)
-    k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
-                                           W->dinv32.get());
-  CN_HIP(hipGetLastError());
   S->work = W;
   return W;
 }
@@ -681,7 +711,6 @@ static unsigned blocks_for_rows(int64_t rows, int rows_per_block, unsigned cap) 
   return (unsigned)g;
 }
 
-// grid for the row mappings; rows per block-iteration: narrow 32, wide 4
 struct Grid {
   dim3 g;
   int32_t rows;  // partial rows written (= g.x)
@@ -710,142 +739,131 @@ static Grid grid_rows(int64_t rows, int64_t k, unsigned cap) {
 static const unsigned kCapSpmm = 1024;
 static const unsigned kCapLevel = 512;
 
-#define DISPATCH_NARROW(K_, KERNEL, ...)                       \
-  switch (K_) {                                                \
-    case 1: KERNEL<1> __VA_ARGS__; break;                      \
-    case 2: KERNEL<2> __VA_ARGS__; break;                      \
-    case 3: KERNEL<3> __VA_ARGS__; break;                      \
-    default: KERNEL<4> __VA_ARGS__; break;                     \
-  }
+// dispatch: NARROW(K) for k <= 4, WIDE(V) with V the vector width
+#define DISPATCH_K(k, T, NARROW, WIDE)                                        \
+  do {                                                                        \
+    if ((k) <= 4) {                                                           \
+      switch (k) {                                                            \
+        case 1: NARROW(1); break;                                             \
+        case 2: NARROW(2); break;                                             \
+        case 3: NARROW(3); break;                                             \
+        default: NARROW(4); break;                                            \
+      }                                                                       \
+    } else {                                                                  \
+      int v_ = wide_vec<T>(k);                                                \
+      if (v_ == 4) { WIDE(4); }                                               \
+      else if (v_ == 2) { WIDE(2); }                                          \
+      else { WIDE(1); }                                                       \
+    }                                                                         \
+  } while (0)
 
-// Matrix + preconditioner view of one solve (permuted if IC(0)).
+// the FP64 path never selects V = 4; map it to an existing instantiation
+template <typename T, int V> struct WideV { static constexpr int v = V; };
+template <> struct WideV<double, 4> { static constexpr int v = 2; };
+
+// Matrix + preconditioner view of one solve.
 template <typename T>
 struct SysView {
   int32_t n;
   const int32_t *ptr, *col;
   const T *val;
   const double *val64;
-  // IC(0)
   const cn_ic0_t *ic0 = nullptr;
-  const int32_t *lp, *lcol, *up, *ucol;
+  const int32_t *rowmap, *lp, *lcol, *up, *ucol;
   const T *lval, *uval, *rdiag;
-  // Jacobi
   const T *dinv;
 };
 
 template <typename T>
-static void launch_spmm_dot(const SysView<T> &V, int64_t k, const T *P, T *Q,
-                            double *part, const int32_t *ctl, hipStream_t s,
-                            int32_t *rows_out) {
+static int32_t launch_spmm_dot(const SysView<T> &V, int64_t k, const T *P, T *Q,
+                               double *part, const int32_t *ctl, hipStream_t s) {
   Grid G = grid_rows<T>(V.n, k, kCapSpmm);
-  *rows_out = G.rows;
   ProfScope ps("spmm", s);
-  if (k <= 4) {
-#define KN_SPMM(KK) kn_spmm_dot<T, KK><<<G.g, kBlock, 0, s>>>(V.n, V.ptr, V.col, V.val, P, Q, part, ctl)
-    switch (k) { case 1: KN_SPMM(1); break; case 2: KN_SPMM(2); break; case 3: KN_SPMM(3); break; default: KN_SPMM(4); }
-#undef KN_SPMM
-  } else {
-    int v = wide_vec<T>(k);
-    if (v == 4) kw_spmm_dot<T, (sizeof(T) == 4 ? 4 : 2)><<<G.g, kBlock, 0, s>>>(V.n, V.ptr, V.col, V.val, k, P, Q, part, ctl);
-    else if (v == 2) kw_spmm_dot<T, 2><<<G.g, kBlock, 0, s>>>(V.n, V.ptr, V.col, V.val, k, P, Q, part, ctl);
-    else kw_spmm_dot<T, 1><<<G.g, kBlock, 0, s>>>(V.n, V.ptr, V.col, V.val, k, P, Q, part, ctl);
-  }
+#define N_(KK) kn_spmm_dot<T, KK><<<G.g, kBlock, 0, s>>>(V.n, V.ptr, V.col, V.val, P, Q, part, ctl)
+#define W_(VV) kw_spmm_dot<T, WideV<T, VV>::v><<<G.g, kBlock, 0, s>>>(V.n, V.ptr, V.col, V.val, k, P, Q, part, ctl)
+  DISPATCH_K(k, T, N_, W_);
+#undef N_
+#undef W_
+  return G.rows;
 }
 
-static void launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
-                         const double *val, int64_t k, const double *X,
-                         const double *B, double *R, double *part, hipStream_t s,
-                         int32_t *rows_out) {
+static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
+                            const double *val, int64_t k, const double *X,
+                            const double *B, double *R, double *part, hipStream_t s) {
   Grid G = grid_rows<double>(n, k, kCapSpmm);
-  *rows_out = G.rows;
   ProfScope ps("resid", s);
-  if (k <= 4) {
-#define KN_RES(KK) kn_resid<KK><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, X, B, R, part)
-    switch (k) { case 1: KN_RES(1); break; case 2: KN_RES(2); break; case 3: KN_RES(3); break; default: KN_RES(4); }
-#undef KN_RES
-  } else if (k % 2 == 0) {
-    kw_resid<2><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, k, X, B, R, part);
-  } else {
-    kw_resid<1><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, k, X, B, R, part);
-  }
+#define N_(KK) kn_resid<KK><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, X, B, R, part)
+#define W_(VV) kw_resid<WideV<double, VV>::v><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, k, X, B, R, part)
+  DISPATCH_K(k, double, N_, W_);
+#undef N_
+#undef W_
+  return G.rows;
 }
 
-// z = M^-1 r (fused with the CG update when FUSED); partials rr into
-// part[0 .. rows), rz into part[stride ...].  Returns partial rows.
+// First half of z = M^-1 r: (FUSED: r -= alpha q) then y = L^-1 r (IC(0))
+// or z = r / d (Jacobi); partials r.r -> part[0..rows) and r.M^-1.r ->
+// part[stride..].  Returns the partial rows written.
 template <typename T, bool FUSED>
-static int32_t launch_precond(const SysView<T> &V, int64_t k, const double *alpha,
-                              const T *P, const T *Q, T *D, T *R, T *Z,
-                              double *part, int64_t stride, const int32_t *ctl,
-                              hipStream_t s) {
-  const int v = wide_vec<T>(k);
+static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
+                          const T *Q, T *R, T *Z, double *part, int64_t stride,
+                          const int32_t *ctl, hipStream_t s) {
   if (!V.ic0) {
     Grid G = grid_rows<T>(V.n, k, kCapSpmm);
-    ProfScope ps("jacobi", s);
-    if (k <= 4) {
-#define KN_J(KK) kn_jacobi<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(V.n, V.dinv, alpha, P, Q, D, R, Z, part, stride, ctl)
-      switch (k) { case 1: KN_J(1); break; case 2: KN_J(2); break; case 3: KN_J(3); break; default: KN_J(4); }
-#undef KN_J
-    } else if (v == 4) {
-      kw_jacobi<T, (sizeof(T) == 4 ? 4 : 2), FUSED><<<G.g, kBlock, 0, s>>>(V.n, V.dinv, alpha, k, P, Q, D, R, Z, part, stride, ctl);
-    } else if (v == 2) {
-      kw_jacobi<T, 2, FUSED><<<G.g, kBlock, 0, s>>>(V.n, V.dinv, alpha, k, P, Q, D, R, Z, part, stride, ctl);
-    } else {
-      kw_jacobi<T, 1, FUSED><<<G.g, kBlock, 0, s>>>(V.n, V.dinv, alpha, k, P, Q, D, R, Z, part, stride, ctl);
-    }
+    ProfScope ps("fwd", s);
+#define N_(KK) kn_jac<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(V.n, V.dinv, alpha, Q, R, Z, part, stride, ctl)
+#define W_(VV) kw_jac<T, WideV<T, VV>::v, FUSED><<<G.g, kBlock, 0, s>>>(V.n, V.dinv, alpha, k, Q, R, Z, part, stride, ctl)
+    DISPATCH_K(k, T, N_, W_);
+#undef N_
+#undef W_
     return G.rows;
   }
   const std::vector<int32_t> &lv = V.ic0->level_ptr;
   const int32_t nl = (int32_t)lv.size() - 1;
   int32_t off = 0;
-  {
-    ProfScope ps("fwd", s);
-    for (int32_t c = 0; c < nl; ++c) {
-      int32_t r0 = lv[c], r1 = lv[c + 1];
-      if (r1 <= r0) continue;
-      Grid G = grid_rows<T>(r1 - r0, k, kCapLevel);
-      double *pp = part + (int64_t)off * k;
-      if (k <= 4) {
-#define KN_F(KK) kn_fwd<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(r0, r1, V.lp, V.lcol, V.lval, V.rdiag, alpha, P, Q, D, R, Z, pp, ctl)
-        switch (k) { case 1: KN_F(1); break; case 2: KN_F(2); break; case 3: KN_F(3); break; default: KN_F(4); }
-#undef KN_F
-      } else if (v == 4) {
-        kw_fwd<T, (sizeof(T) == 4 ? 4 : 2), FUSED><<<G.g, kBlock, 0, s>>>(r0, r1, V.lp, V.lcol, V.lval, V.rdiag, alpha, k, P, Q, D, R, Z, pp, ctl);
-      } else if (v == 2) {
-        kw_fwd<T, 2, FUSED><<<G.g, kBlock, 0, s>>>(r0, r1, V.lp, V.lcol, V.lval, V.rdiag, alpha, k, P, Q, D, R, Z, pp, ctl);
-      } else {
-        kw_fwd<T, 1, FUSED><<<G.g, kBlock, 0, s>>>(r0, r1, V.lp, V.lcol, V.lval, V.rdiag, alpha, k, P, Q, D, R, Z, pp, ctl);
-      }
-      off += G.rows;
-    }
+  ProfScope ps("fwd", s);
+  for (int32_t c = 0; c < nl; ++c) {
+    int32_t r0 = lv[c], r1 = lv[c + 1];
+    if (r1 <= r0) continue;
+    Grid G = grid_rows<T>(r1 - r0, k, kCapLevel);
+    double *pp = part + (int64_t)off * k;
+#define N_(KK) kn_fwd<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(r0, r1, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, Q, R, Z, pp, stride, ctl)
+#define W_(VV) kw_fwd<T, WideV<T, VV>::v, FUSED><<<G.g, kBlock, 0, s>>>(r0, r1, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, k, Q, R, Z, pp, stride, ctl)
+    DISPATCH_K(k, T, N_, W_);
+#undef N_
+#undef W_
+    off += G.rows;
   }
-  int32_t off2 = 0;
-  {
-    ProfScope ps("bwd", s);
-    for (int32_t c = nl - 1; c >= 0; --c) {
-      int32_t r0 = lv[c], r1 = lv[c + 1];
-      if (r1 <= r0) continue;
-      Grid G = grid_rows<T>(r1 - r0, k, kCapLevel);
-      double *pp = part + stride + (int64_t)off2 * k;
-      if (k <= 4) {
-#define KN_B(KK) kn_bwd<T, KK><<<G.g, kBlock, 0, s>>>(r0, r1, V.up, V.ucol, V.uval, V.rdiag, R, Z, pp, ctl)
-        switch (k) { case 1: KN_B(1); break; case 2: KN_B(2); break; case 3: KN_B(3); break; default: KN_B(4); }
-#undef KN_B
-      } else if (v == 4) {
-        kw_bwd<T, (sizeof(T) == 4 ? 4 : 2)><<<G.g, kBlock, 0, s>>>(r0, r1, V.up, V.ucol, V.uval, V.rdiag, k, R, Z, pp, ctl);
-      } else if (v == 2) {
-        kw_bwd<T, 2><<<G.g, kBlock, 0, s>>>(r0, r1, V.up, V.ucol, V.uval, V.rdiag, k, R, Z, pp, ctl);
-      } else {
-        kw_bwd<T, 1><<<G.g, kBlock, 0, s>>>(r0, r1, V.up, V.ucol, V.uval, V.rdiag, k, R, Z, pp, ctl);
-      }
-      off2 += G.rows;
-    }
-  }
-  return std::max(off, off2);  // both passes write the same row count
+  return off;
 }
 
-// partial rows the preconditioner needs (for workspace sizing)
+// Second half: z = L^-T y (IC(0)); d += alpha p; p = gamma z + beta p.
 template <typename T>
-static int32_t precond_rows(const SysView<T> &V, int64_t k) {
+static void launch_bwd(const SysView<T> &V, int64_t k, const double *st, T *Z, T *P,
+                       T *D, const int32_t *ctl, hipStream_t s) {
+  if (!V.ic0) {
+    ProfScope ps("bwd", s);
+    const int64_t total = (int64_t)V.n * k;
+    unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192));
+    k_pd_update<T><<<g, 256, 0, s>>>(total, k, st, Z, P, D, ctl);
+    return;
+  }
+  const std::vector<int32_t> &lv = V.ic0->level_ptr;
+  const int32_t nl = (int32_t)lv.size() - 1;
+  ProfScope ps("bwd", s);
+  for (int32_t c = nl - 1; c >= 0; --c) {
+    int32_t r0 = lv[c], r1 = lv[c + 1];
+    if (r1 <= r0) continue;
+    Grid G = grid_rows<T>(r1 - r0, k, kCapLevel);
+#define N_(KK) kn_bwd<T, KK><<<G.g, kBlock, 0, s>>>(r0, r1, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
+#define W_(VV) kw_bwd<T, WideV<T, VV>::v><<<G.g, kBlock, 0, s>>>(r0, r1, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
+    DISPATCH_K(k, T, N_, W_);
+#undef N_
+#undef W_
+  }
+}
+
+template <typename T>
+static int32_t fwd_rows(const SysView<T> &V, int64_t k) {
   if (!V.ic0) return grid_rows<T>(V.n, k, kCapSpmm).rows;
   const std::vector<int32_t> &lv = V.ic0->level_ptr;
   int32_t off = 0;
@@ -855,20 +873,15 @@ static int32_t precond_rows(const SysView<T> &V, int64_t k) {
 }
 
 template <typename T>
-static void launch_dot(int32_t n, int64_t k, const T *A, const T *B, double *part,
-                       hipStream_t s, int32_t *rows_out) {
+static int32_t launch_dot(int32_t n, int64_t k, const T *A, const T *B, double *part,
+                          hipStream_t s) {
   Grid G = grid_rows<T>(n, k, kCapSpmm);
-  *rows_out = G.rows;
-  if (k <= 4) {
-#define KN_D(KK) kn_dot<T, KK><<<G.g, kBlock, 0, s>>>(n, A, B, part)
-    switch (k) { case 1: KN_D(1); break; case 2: KN_D(2); break; case 3: KN_D(3); break; default: KN_D(4); }
-#undef KN_D
-  } else {
-    int v = wide_vec<T>(k);
-    if (v == 4) kw_dot<T, (sizeof(T) == 4 ? 4 : 2)><<<G.g, kBlock, 0, s>>>(n, k, A, B, part);
-    else if (v == 2) kw_dot<T, 2><<<G.g, kBlock, 0, s>>>(n, k, A, B, part);
-    else kw_dot<T, 1><<<G.g, kBlock, 0, s>>>(n, k, A, B, part);
-  }
+#define N_(KK) kn_dot<T, KK><<<G.g, kBlock, 0, s>>>(n, A, B, part)
+#define W_(VV) kw_dot<T, WideV<T, VV>::v><<<G.g, kBlock, 0, s>>>(n, k, A, B, part)
+  DISPATCH_K(k, T, N_, W_);
+#undef N_
+#undef W_
+  return G.rows;
 }
 
 static unsigned ew_blocks(int64_t total) {
@@ -878,60 +891,337 @@ static unsigned ew_blocks(int64_t total) {
   return (unsigned)g;
 }
 
-// ========================================================= inner PCG solve ==
-// Solves A D = R (R given, D zeroed) for the active columns.  Returns the
-// number of iterations run.  `active`/ctl[0] must be set by the caller.
+// Two-stage column reduction: returns the buffer and row count the final
+// k_reduce_* kernel should read (stride between quantities = rows * k).
+struct Red {
+  const double *p;
+  int32_t rows;
+  int64_t stride;
+};
+static Red reduce_rows(SolverWork *W, const double *part, int32_t nrows, int64_t k,
+                       int64_t stride, int np, hipStream_t s) {
+  if (nrows <= kRedRows) return Red{part, nrows, stride};
+  int32_t S = kRedRows;
+  int32_t chunk = (nrows + S - 1) / S;
+  S = (nrows + chunk - 1) / chunk;
+  dim3 g((unsigned)((k + 63) / 64), (unsigned)S);
+  const int64_t ostride = (int64_t)kRedRows * k;
+  k_reduce_stage1<<<g, kBlock, 0, s>>>(nrows, chunk, k, part, stride, np, W->red.get(),
+                                       ostride);
+  return Red{W->red.get(), S, ostride};
+}
+
 template <typename T>
-static int32_t inner_pcg(const SysView<T> &V, SolverWork *W, int64_t k,
-                         int32_t max_it, double inner_tol, int32_t check_every,
-                         hipStream_t s) {
-  const int32_t n = V.n;
+static SysView<T> make_view(cn_system_t *S, cn_ic0_t *F, SolverWork *W);
+
+template <>
+SysView<float> make_view<float>(cn_system_t *S, cn_ic0_t *F, SolverWork *W) {
+  SysView<float> V{};
+  V.n = S->n_free;
+  V.ptr = S->ff_ptr.get(); V.col = S->ff_col.get(); V.val = S->ff_val32.get();
+  V.val64 = S->ff_val.get();
+  if (F) {
+    V.ic0 = F;
+    V.rowmap = F->perm.get();
+    V.lp = F->l_ptr.get(); V.lcol = F->l_col_nat.get(); V.lval = F->l_val32.get();
+    V.up = F->u_ptr.get(); V.ucol = F->u_col_nat.get(); V.uval = F->u_val32.get();
+    V.rdiag = F->l_rdiag32.get();
+  }
+  V.dinv = W->dinv32.get();
+  return V;
+}
+
+template <>
+SysView<double> make_view<double>(cn_system_t *S, cn_ic0_t *F, SolverWork *W) {
+  SysView<double> V{};
+  V.n = S->n_free;
+  V.ptr = S->ff_ptr.get(); V.col = S->ff_col.get(); V.val = S->ff_val.get();
+  V.val64 = S->ff_val.get();
+  if (F) {
+    V.ic0 = F;
+    V.rowmap = F->perm.get();
+    V.lp = F->l_ptr.get(); V.lcol = F->l_col_nat.get(); V.lval = F->l_val.get();
+    V.up = F->u_ptr.get(); V.ucol = F->u_col_nat.get(); V.uval = F->u_val.get();
+    V.rdiag = F->l_rdiag64.get();
+  }
+  V.dinv = W->dinv64.get();
+  return V;
+}
+
+// ========================================= fused solve (MIXED and FP64) ==
+template <typename T>
+static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
+                             int32_t k, const double *d_b, double *d_x,
+                             cn_solve_info *info, hipStream_t s) {
+  const int32_t n = S->n_free;
   const int64_t total = (int64_t)n * k;
+  SysView<T> V0{};
+  V0.n = n;
+  V0.ic0 = F;
+  int32_t need = std::max<int32_t>(fwd_rows<T>(V0, k), grid_rows<T>(n, k, kCapSpmm).rows);
+  need = std::max<int32_t>(need, grid_rows<double>(n, k, kCapSpmm).rows);
+  SolverWork *W = ensure_work(S, k, need);
+  k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
+                                         W->dinv32.get());
+  SysView<T> V = make_view<T>(S, F, W);
+  double *st = W->st.get();
+  double *hs = W->h_st;
+  int32_t *ctl = W->ctl.get();
+  int32_t *hc = W->h_ctl;
+  const int64_t stride = (int64_t)W->part_rows * k;
+  double *p0 = W->part.get(), *p1 = p0 + stride;
+  const unsigned red = (unsigned)((k + 63) / 64);
   T *D = reinterpret_cast<T *>(W->buf[0].get());
   T *R = reinterpret_cast<T *>(W->buf[1].get());
   T *Z = reinterpret_cast<T *>(W->buf[2].get());
   T *P = reinterpret_cast<T *>(W->buf[3].get());
   T *Q = reinterpret_cast<T *>(W->buf[4].get());
-  double *st = W->st.get();
-  int32_t *ctl = W->ctl.get();
-  const int64_t stride = (int64_t)W->part_rows * k;
-  double *p0 = W->part.get(), *p1 = p0 + stride, *p2 = p1 + stride;
-  const unsigned red_blocks = (unsigned)((k + 63) / 64);
+  double *X = W->xp.get(), *R64 = W->rp.get();
+  const double theta = 0.5;  // iterated-residual target below tol
+  const double delta2 = sizeof(T) == 4 ? 1e-2 : 1e-8;
 
-  // z0 = M^-1 r0, rz0, rr0
-  int32_t pr = launch_precond<T, false>(V, k, st + ST_ALPHA * k, P, Q, D, R, Z, p1,
-                                        stride, ctl, s);
-  k_reduce_beta<<<red_blocks, kBlock, 0, s>>>(pr, k, p1, p1 + stride, st,
-                                              W->active.get(), ctl, 0,
-                                              inner_tol * inner_tol);
-  CN_HIP(hipMemcpyAsync(P, Z, sizeof(T) * total, hipMemcpyDeviceToDevice, s));
-  int32_t it = 0;
-  while (it < max_it) {
-    int32_t rows = 0;
-    launch_spmm_dot<T>(V, k, P, Q, p0, ctl, s, &rows);
-    k_reduce_alpha<<<red_blocks, kBlock, 0, s>>>(rows, k, p0, st, W->active.get(), ctl);
-    pr = launch_precond<T, true>(V, k, st + ST_ALPHA * k, P, Q, D, R, Z, p1, stride,
-                                 ctl, s);
-    k_reduce_beta<<<red_blocks, kBlock, 0, s>>>(pr, k, p1, p1 + stride, st,
-                                                W->active.get(), ctl, 1, 0.0);
-    k_p_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_BETA * k, Z, P, ctl);
-    ++it;
-    if (it % check_every == 0 || it == max_it) {
-      CN_HIP(hipMemcpyAsync(W->h_ctl, ctl, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
-      CN_HIP(hipStreamSynchronize(s));
-      if (W->h_ctl[0] == 0) break;
-    }
-  }
-  (void)p2;
+  // x0, true residual, ||b||
+  if (o->warm_start)
+    CN_HIP(hipMemcpyAsync(X, d_x, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
+  else
+    CN_HIP(hipMemsetAsync(X, 0, sizeof(double) * total, s));
+  int32_t rows = launch_dot<double>(n, k, d_b, d_b, p0, s);
+  Red rd = reduce_rows(W, p0, rows, k, stride, 1, s);
+  k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_BB * k);
+  rows = launch_resid(n, V.ptr, V.col, V.val64, k, X, d_b, R64, p0, s);
+  rd = reduce_rows(W, p0, rows, k, stride, 1, s);
+  k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
+  CN_HIP(hipMemsetAsync(D, 0, sizeof(T) * total, s));
+  CN_HIP(hipMemsetAsync(P, 0, sizeof(T) * total, s));
   CN_HIP(hipGetLastError());
-  return it;
+
+  std::vector<int32_t> act(k, 0), rst(k, 0);
+  int32_t iters = 0, updates = 0;
+  cn_status status = CN_OK;
+  double relmax = 0.0;
+  bool first = true;
+  for (;;) {
+    // ---- read the true FP64 residual, decide which columns continue
+    CN_HIP(hipMemcpyAsync(hs + ST_BB * k, st + ST_BB * k, sizeof(double) * 2 * k,
+                          hipMemcpyDeviceToHost, s));  // BB and RR adjacent
+    CN_HIP(hipStreamSynchronize(s));
+    relmax = 0.0;
+    int32_t nact = 0;
+    for (int32_t c = 0; c < k; ++c) {
+      double bb = hs[ST_BB * k + c], rr = hs[ST_RR * k + c];
+      double rel = bb > 0.0 ? std::sqrt(rr / bb) : (rr > 0.0 ? INFINITY : 0.0);
+      if (!std::isfinite(rel)) rel = INFINITY;
+      relmax = std::max(relmax, rel);
+      bool a = rel > o->tol_rel && bb > 0.0;
+      // a column the iterated residual had stopped, but whose true residual
+      // is not there yet, restarts (p = z)
+      rst[c] = (a && (first || !act[c])) ? 1 : 0;
+      act[c] = a ? 1 : 0;
+      nact += a ? 1 : 0;
+    }
+    if (nact == 0) { status = CN_OK; break; }
+    if (!std::isfinite(relmax)) { status = CN_BREAKDOWN; break; }
+    if (iters >= o->max_iter || updates >= o->max_refine * 8) { status = CN_MAXIT; break; }
+    // ---- per-column scale s = 1/||b|| (fixed for the solve)
+    if (first) {
+      for (int32_t c = 0; c < k; ++c) {
+        double bb = hs[ST_BB * k + c];
+        hs[ST_S * k + c] = bb > 0.0 ? 1.0 / std::sqrt(bb) : 0.0;
+        hs[ST_INVS * k + c] = bb > 0.0 ? std::sqrt(bb) : 0.0;
+        double t = theta * o->tol_rel;
+        hs[ST_TOL2 * k + c] = t * t;
+        hs[ST_RREF * k + c] = 0.0;
+      }
+      CN_HIP(hipMemcpyAsync(st + ST_TOL2 * k, hs + ST_TOL2 * k, sizeof(double) * 4 * k,
+                            hipMemcpyHostToDevice, s));  // TOL2, RREF, S, INVS
+    }
+    first = false;
+    CN_HIP(hipMemcpyAsync(W->active.get(), act.data(), sizeof(int32_t) * k,
+                          hipMemcpyHostToDevice, s));
+    CN_HIP(hipMemcpyAsync(W->restart.get(), rst.data(), sizeof(int32_t) * k,
+                          hipMemcpyHostToDevice, s));
+    hc[CTL_ACTIVE] = nact;
+    hc[CTL_BREAKDOWN] = 0;
+    hc[CTL_ONE] = 1;
+    hc[CTL_TRIGGER] = 0;
+    hc[CTL_JUST] = 0;
+    CN_HIP(hipMemcpyAsync(ctl, hc, sizeof(int32_t) * 5, hipMemcpyHostToDevice, s));
+    CN_HIP(hipMemsetAsync(st + ST_ALPHA * k, 0, sizeof(double) * k, s));
+    k_scale_in<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_S * k, R64, R);
+    // z = M^-1 r; rz; continuing columns keep p, restarted ones take p = z
+    rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * k, Q, R, Z, p1, stride, ctl, s);
+    rd = reduce_rows(W, p1, rows, k, stride, 2, s);
+    k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, rd.p + rd.stride, st,
+                                         W->active.get(), W->restart.get(), ctl, 1, delta2);
+    launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
+    CN_HIP(hipGetLastError());
+    // ---- CG iterations until every column stops or an update is due
+    int32_t it = 0;
+    const int32_t budget = o->max_iter - iters;
+    const int32_t every = std::max(1, o->check_every);
+    while (it < budget) {
+      rows = launch_spmm_dot<T>(V, k, P, Q, p0, ctl, s);
+      rd = reduce_rows(W, p0, rows, k, stride, 1, s);
+      k_reduce_alpha<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st, W->active.get(), ctl);
+      rows = launch_fwd<T, true>(V, k, st + ST_ALPHA * k, Q, R, Z, p1, stride, ctl, s);
+      rd = reduce_rows(W, p1, rows, k, stride, 2, s);
+      k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, rd.p + rd.stride, st,
+                                           W->active.get(), W->restart.get(), ctl, 0, delta2);
+      launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
+      ++it;
+      if (it % every == 0 || it == budget) {
+        CN_HIP(hipMemcpyAsync(hc, ctl, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
+        CN_HIP(hipStreamSynchronize(s));
+        if (hc[CTL_ACTIVE] == 0 || hc[CTL_TRIGGER]) break;
+      }
+    }
+    CN_HIP(hipGetLastError());
+    iters += it;
+    // ---- reliable update: x += d / s (FP64), r = b - A x (FP64)
+    k_x_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_INVS * k, D, X);
+    rows = launch_resid(n, V.ptr, V.col, V.val64, k, X, d_b, R64, p0, s);
+    rd = reduce_rows(W, p0, rows, k, stride, 1, s);
+    k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
+    CN_HIP(hipGetLastError());
+    ++updates;
+    if (it == 0) { status = CN_MAXIT; break; }
+  }
+  CN_HIP(hipMemcpyAsync(d_x, X, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
+  CN_HIP(hipStreamSynchronize(s));
+  if (info) {
+    info->status = status;
+    info->iters = iters;
+    info->refinements = updates;
+    info->relres_max = relmax;
+  }
+  return status;
+}
+
+// ================================ unfused FP64 baseline, scalars on host ==
+static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
+                             int32_t k, const double *d_b, double *d_x,
+                             cn_solve_info *info, hipStream_t s) {
+  const int32_t n = S->n_free;
+  const int64_t total = (int64_t)n * k;
+  SysView<double> V0{};
+  V0.n = n;
+  V0.ic0 = F;
+  int32_t need = std::max<int32_t>(fwd_rows<double>(V0, k),
+                                   grid_rows<double>(n, k, kCapSpmm).rows);
+  SolverWork *W = ensure_work(S, k, need);
+  k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
+                                         W->dinv32.get());
+  SysView<double> V = make_view<double>(S, F, W);
+  double *st = W->st.get();
+  const int64_t stride = (int64_t)W->part_rows * k;
+  const unsigned red = (unsigned)((k + 63) / 64);
+  double *X = W->xp.get(), *R = W->rp.get();
+  double *Z = W->buf[2].get(), *P = W->buf[3].get(), *Q = W->buf[4].get();
+  double *Dd = W->buf[0].get();
+  int32_t *ctl = W->ctl.get();
+  W->h_ctl[CTL_ACTIVE] = 1; W->h_ctl[CTL_BREAKDOWN] = 0; W->h_ctl[CTL_ONE] = 1;
+  W->h_ctl[CTL_TRIGGER] = 0;
+  W->h_ctl[CTL_JUST] = 0;
+  CN_HIP(hipMemcpyAsync(ctl, W->h_ctl, sizeof(int32_t) * 5, hipMemcpyHostToDevice, s));
+  // preconditioner-only coefficients: alpha 0, beta 1, gamma 0 (p, d untouched)
+  {
+    std::vector<double> h(3 * (size_t)k);
+    for (int32_t c = 0; c < k; ++c) { h[c] = 0.0; h[k + c] = 1.0; h[2 * k + c] = 0.0; }
+    CN_HIP(hipMemcpyAsync(st + ST_ALPHA * k, h.data(), sizeof(double) * 3 * k,
+                          hipMemcpyHostToDevice, s));
+    CN_HIP(hipStreamSynchronize(s));
+  }
+  auto dot_to_host = [&](const double *a, const double *b, double *h) {
+    int32_t rows = launch_dot<double>(n, k, a, b, W->part.get(), s);
+    Red rd = reduce_rows(W, W->part.get(), rows, k, stride, 1, s);
+    k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_TMP * k);
+    CN_HIP(hipMemcpyAsync(h, st + ST_TMP * k, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    CN_HIP(hipStreamSynchronize(s));
+  };
+  auto apply_m = [&]() {  // Z = M^-1 R
+    launch_fwd<double, false>(V, k, st + ST_ALPHA * k, Q, R, Z, W->part.get(), stride,
+                              ctl, s);
+    if (F) launch_bwd<double>(V, k, st, Z, P, Dd, ctl, s);
+  };
+  if (o->warm_start) {
+    CN_HIP(hipMemcpyAsync(X, d_x, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
+    launch_resid(n, V.ptr, V.col, V.val64, k, X, d_b, R, W->part.get(), s);
+  } else {
+    CN_HIP(hipMemsetAsync(X, 0, sizeof(double) * total, s));
+    CN_HIP(hipMemcpyAsync(R, d_b, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
+  }
+  CN_HIP(hipMemsetAsync(P, 0, sizeof(double) * total, s));
+  std::vector<double> bb(k), rz(k), tmp(k), alpha(k), beta(k), rr(k);
+  dot_to_host(d_b, d_b, bb.data());
+  apply_m();
+  CN_HIP(hipMemcpyAsync(P, Z, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
+  dot_to_host(R, Z, rz.data());
+  std::vector<int32_t> act(k, 1);
+  dot_to_host(R, R, rr.data());
+  for (int32_t c = 0; c < k; ++c)
+    if (!(rr[c] > o->tol_rel * o->tol_rel * bb[c])) act[c] = 0;
+  int32_t it = 0;
+  cn_status status = CN_MAXIT;
+  bool any = false;
+  for (int32_t c = 0; c < k; ++c) any |= act[c] != 0;
+  if (!any) status = CN_OK;
+  while (any && it < o->max_iter) {
+    launch_spmm_dot<double>(V, k, P, Q, W->part.get(), ctl, s);      // SpMV
+    dot_to_host(P, Q, tmp.data());                                   // p.Ap
+    for (int32_t c = 0; c < k; ++c) {
+      if (act[c] && tmp[c] == 0.0) { act[c] = 0; status = CN_BREAKDOWN; }
+      alpha[c] = act[c] ? rz[c] / tmp[c] : 0.0;
+    }
+    CN_HIP(hipMemcpyAsync(st + ST_TMP * k, alpha.data(), sizeof(double) * k,
+                          hipMemcpyHostToDevice, s));
+    k_axpy<double><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_TMP * k, 1.0, P, X);
+    k_axpy<double><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_TMP * k, -1.0, Q, R);
+    ++it;
+    dot_to_host(R, R, rr.data());                                    // stop test
+    any = false;
+    for (int32_t c = 0; c < k; ++c) {
+      if (act[c] && !(rr[c] > o->tol_rel * o->tol_rel * bb[c])) act[c] = 0;
+      any |= act[c] != 0;
+    }
+    if (!any) { if (status == CN_MAXIT) status = CN_OK; break; }
+    apply_m();                                                       // z = M^-1 r
+    dot_to_host(R, Z, tmp.data());                                   // r.z
+    for (int32_t c = 0; c < k; ++c) {
+      beta[c] = act[c] ? (rz[c] == 0.0 ? 0.0 : tmp[c] / rz[c]) : 0.0;
+      if (act[c]) rz[c] = tmp[c];
+    }
+    CN_HIP(hipMemcpyAsync(st + ST_TMP * k, beta.data(), sizeof(double) * k,
+                          hipMemcpyHostToDevice, s));
+    k_p_update<double><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_TMP * k, Z, P);
+  }
+  int32_t rows = launch_resid(n, V.ptr, V.col, V.val64, k, X, d_b, R, W->part.get(), s);
+  Red rdf = reduce_rows(W, W->part.get(), rows, k, stride, 1, s);
+  k_reduce_plain<<<red, kBlock, 0, s>>>(rdf.rows, k, rdf.p, st + ST_TMP * k);
+  CN_HIP(hipMemcpyAsync(tmp.data(), st + ST_TMP * k, sizeof(double) * k,
+                        hipMemcpyDeviceToHost, s));
+  CN_HIP(hipMemcpyAsync(d_x, X, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
+  CN_HIP(hipGetLastError());
+  CN_HIP(hipStreamSynchronize(s));
+  double relmax = 0.0;
+  for (int32_t c = 0; c < k; ++c) {
+    double rel = bb[c] > 0.0 ? std::sqrt(tmp[c] / bb[c]) : 0.0;
+    relmax = std::max(relmax, rel);
+  }
+  if (status == CN_OK && relmax > o->tol_rel * 10.0) status = CN_MAXIT;
+  if (info) {
+    info->status = status;
+    info->iters = it;
+    info->refinements = 0;
+    info->relres_max = relmax;
+  }
+  return status;
 }
 
 }  // namespace cn
 
 using namespace cn;
 
-// ============================================================ entry point ==
+// ============================================================ entry points ==
 extern "C" {
 
 void cn_pcg_default_opts(cn_pcg_opts *o) {
@@ -961,276 +1251,9 @@ int32_t cn_profile_query(const char *name, double *ms_total) {
   return it->second.first;
 }
 
-}  // extern "C"
-
-namespace cn {
-
-template <typename T>
-static SysView<T> make_view(cn_system_t *S, cn_ic0_t *F, SolverWork *W);
-
-template <>
-SysView<float> make_view<float>(cn_system_t *S, cn_ic0_t *F, SolverWork *W) {
-  SysView<float> V{};
-  V.n = S->n_free;
-  if (F) {
-    V.ptr = F->a_ptr.get(); V.col = F->a_col.get(); V.val = F->a_val32.get();
-    V.val64 = F->a_val.get();
-    V.ic0 = F;
-    V.lp = F->l_ptr.get(); V.lcol = F->l_col.get(); V.lval = F->l_val32.get();
-    V.up = F->u_ptr.get(); V.ucol = F->u_col.get(); V.uval = F->u_val32.get();
-    V.rdiag = F->l_rdiag32.get();
-  } else {
-    V.ptr = S->ff_ptr.get(); V.col = S->ff_col.get(); V.val = S->ff_val32.get();
-    V.val64 = S->ff_val.get();
-  }
-  V.dinv = W->dinv32.get();
-  return V;
-}
-
-template <>
-SysView<double> make_view<double>(cn_system_t *S, cn_ic0_t *F, SolverWork *W) {
-  SysView<double> V{};
-  V.n = S->n_free;
-  if (F) {
-    V.ptr = F->a_ptr.get(); V.col = F->a_col.get(); V.val = F->a_val.get();
-    V.val64 = F->a_val.get();
-    V.ic0 = F;
-    V.lp = F->l_ptr.get(); V.lcol = F->l_col.get(); V.lval = F->l_val.get();
-    V.up = F->u_ptr.get(); V.ucol = F->u_col.get(); V.uval = F->u_val.get();
-    V.rdiag = F->l_rdiag64.get();
-  } else {
-    V.ptr = S->ff_ptr.get(); V.col = S->ff_col.get(); V.val = S->ff_val.get();
-    V.val64 = S->ff_val.get();
-  }
-  V.dinv = W->dinv64.get();
-  return V;
-}
-
-// refinement loop shared by MIXED (T = float) and FP64 (T = double)
-template <typename T>
-static cn_status refine_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
-                              int32_t k, const double *d_b, double *d_x,
-                              cn_solve_info *info, hipStream_t s) {
-  const int32_t n = S->n_free;
-  const int64_t total = (int64_t)n * k;
-  SysView<T> V0{};
-  V0.n = n;
-  V0.ic0 = F;
-  int32_t need = std::max<int32_t>(precond_rows<T>(V0, k),
-                                   grid_rows<T>(n, k, kCapSpmm).rows);
-  need = std::max<int32_t>(need, grid_rows<double>(n, k, kCapSpmm).rows);
-  SolverWork *W = ensure_work(S, k, need, s);
-  SysView<T> V = make_view<T>(S, F, W);
-  const int32_t *perm = F ? F->perm.get() : nullptr;
-  double *st = W->st.get();
-  double *hs = W->h_st;
-  const int64_t stride = (int64_t)W->part_rows * k;
-  const unsigned red_blocks = (unsigned)((k + 63) / 64);
-  int32_t rows = 0;
-  k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
-                                         W->dinv32.get());
-
-  k_perm_gather<<<ew_blocks(total), 256, 0, s>>>(n, k, perm, d_b, W->bp.get());
-  if (o->warm_start)
-    k_perm_gather<<<ew_blocks(total), 256, 0, s>>>(n, k, perm, d_x, W->xp.get());
-  else
-    CN_HIP(hipMemsetAsync(W->xp.get(), 0, sizeof(double) * total, s));
-  launch_dot<double>(n, k, W->bp.get(), W->bp.get(), W->part.get(), s, &rows);
-  k_reduce_plain<<<red_blocks, kBlock, 0, s>>>(rows, k, W->part.get(), st + ST_BB * k);
-  launch_resid(n, V.ptr, V.col, V.val64, k, W->xp.get(), W->bp.get(), W->rp.get(),
-               W->part.get(), s, &rows);
-  k_reduce_plain<<<red_blocks, kBlock, 0, s>>>(rows, k, W->part.get(), st + ST_RR * k);
-  CN_HIP(hipGetLastError());
-
-  int32_t iters = 0, refine = 0;
-  cn_status status = CN_OK;
-  double relmax = 0.0;
-  std::vector<int32_t> act(k);
-  std::vector<double> scale(k), norm(k);
-  bool first = true;
-  for (;;) {
-    CN_HIP(hipMemcpyAsync(hs + ST_BB * k, st + ST_BB * k, sizeof(double) * 2 * k,
-                          hipMemcpyDeviceToHost, s));  // BB and RR are adjacent
-    CN_HIP(hipStreamSynchronize(s));
-    relmax = 0.0;
-    double relmax_active = 0.0;
-    int32_t nact = 0;
-    for (int32_t c = 0; c < k; ++c) {
-      double bb = hs[ST_BB * k + c], rr = hs[ST_RR * k + c];
-      double rel = bb > 0.0 ? std::sqrt(rr / bb) : (rr > 0.0 ? INFINITY : 0.0);
-      if (!std::isfinite(rel)) rel = INFINITY;
-      relmax = std::max(relmax, rel);
-      bool a = rel > o->tol_rel;
-      act[c] = a ? 1 : 0;
-      scale[c] = a ? 1.0 / std::sqrt(rr) : 0.0;
-      norm[c] = a ? std::sqrt(rr) : 0.0;
-      if (a) { ++nact; relmax_active = std::max(relmax_active, rel); }
-    }
-    if (nact == 0) { status = CN_OK; break; }
-    if (!std::isfinite(relmax)) { status = CN_BREAKDOWN; break; }
-    if (refine >= o->max_refine || iters >= o->max_iter) { status = CN_MAXIT; break; }
-    // inner relative target: enough to reach tol with a 10x margin, never
-    // tighter than FP32 can deliver (MIXED)
-    double inner = 0.1 * o->tol_rel / relmax_active;
-    if (sizeof(T) == 4) inner = std::max(inner, o->inner_tol);
-    else inner = std::max(inner, 1e-15);
-    CN_HIP(hipMemcpyAsync(W->active.get(), act.data(), sizeof(int32_t) * k,
-                          hipMemcpyHostToDevice, s));
-    W->h_ctl[0] = nact;
-    W->h_ctl[1] = 0;
-    W->h_ctl[2] = 1;
-    CN_HIP(hipMemcpyAsync(W->ctl.get(), W->h_ctl, sizeof(int32_t) * 3,
-                          hipMemcpyHostToDevice, s));
-    std::memcpy(hs + ST_SCALE * k, scale.data(), sizeof(double) * k);
-    std::memcpy(hs + ST_NORM * k, norm.data(), sizeof(double) * k);
-    CN_HIP(hipMemcpyAsync(st + ST_NORM * k, hs + ST_NORM * k, sizeof(double) * 2 * k,
-                          hipMemcpyHostToDevice, s));  // NORM, SCALE adjacent
-    T *D = reinterpret_cast<T *>(W->buf[0].get());
-    T *R = reinterpret_cast<T *>(W->buf[1].get());
-    CN_HIP(hipMemsetAsync(D, 0, sizeof(T) * total, s));
-    k_scale_in<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_SCALE * k,
-                                                   W->rp.get(), R);
-    int32_t budget = o->max_iter - iters;
-    int32_t it = inner_pcg<T>(V, W, k, budget, inner, std::max(1, o->check_every), s);
-    iters += it;
-    k_x_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_NORM * k, D,
-                                                   W->xp.get());
-    launch_resid(n, V.ptr, V.col, V.val64, k, W->xp.get(), W->bp.get(), W->rp.get(),
-                 W->part.get(), s, &rows);
-    k_reduce_plain<<<red_blocks, kBlock, 0, s>>>(rows, k, W->part.get(), st + ST_RR * k);
-    CN_HIP(hipGetLastError());
-    ++refine;
-    if (it <= 1 && W->h_ctl[1]) { status = CN_BREAKDOWN; break; }
-  }
-  k_perm_scatter<<<ew_blocks(total), 256, 0, s>>>(n, k, perm, W->xp.get(), d_x);
-  CN_HIP(hipGetLastError());
-  CN_HIP(hipStreamSynchronize(s));
-  (void)stride;
-  if (info) {
-    info->status = status;
-    info->iters = iters;
-    info->refinements = refine;
-    info->relres_max = relmax;
-  }
-  return status;
-}
-
-// Unfused FP64 baseline: one launch per step, scalars on the host.
-static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
-                             int32_t k, const double *d_b, double *d_x,
-                             cn_solve_info *info, hipStream_t s) {
-  const int32_t n = S->n_free;
-  const int64_t total = (int64_t)n * k;
-  SysView<double> V0{};
-  V0.n = n;
-  V0.ic0 = F;
-  int32_t need = std::max<int32_t>(precond_rows<double>(V0, k),
-                                   grid_rows<double>(n, k, kCapSpmm).rows);
-  SolverWork *W = ensure_work(S, k, need, s);
-  SysView<double> V = make_view<double>(S, F, W);
-  k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
-                                         W->dinv32.get());
-  const int32_t *perm = F ? F->perm.get() : nullptr;
-  double *st = W->st.get();
-  double *hs = W->h_st;
-  const int64_t stride = (int64_t)W->part_rows * k;
-  const unsigned red_blocks = (unsigned)((k + 63) / 64);
-  double *X = W->xp.get(), *B = W->bp.get(), *R = W->rp.get();
-  double *Z = W->buf[2].get(), *P = W->buf[3].get(), *Q = W->buf[4].get();
-  int32_t *ctl = W->ctl.get();
-  W->h_ctl[0] = 1; W->h_ctl[1] = 0; W->h_ctl[2] = 1;
-  CN_HIP(hipMemcpyAsync(ctl, W->h_ctl, sizeof(int32_t) * 3, hipMemcpyHostToDevice, s));
-  int32_t rows = 0;
-  auto dot_to_host = [&](const double *a, const double *b, double *h) {
-    launch_dot<double>(n, k, a, b, W->part.get(), s, &rows);
-    k_reduce_plain<<<red_blocks, kBlock, 0, s>>>(rows, k, W->part.get(), st + ST_TMP * k);
-    CN_HIP(hipMemcpyAsync(h, st + ST_TMP * k, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-    CN_HIP(hipStreamSynchronize(s));
-  };
-  auto apply_m = [&]() {  // Z = M^-1 R (no fused update)
-    launch_precond<double, false>(V, k, st + ST_ALPHA * k, P, Q, X, R, Z, W->part.get(),
-                                  stride, ctl, s);
-  };
-  k_perm_gather<<<ew_blocks(total), 256, 0, s>>>(n, k, perm, d_b, B);
-  if (o->warm_start) {
-    k_perm_gather<<<ew_blocks(total), 256, 0, s>>>(n, k, perm, d_x, X);
-    launch_resid(n, V.ptr, V.col, V.val64, k, X, B, R, W->part.get(), s, &rows);
-  } else {
-    CN_HIP(hipMemsetAsync(X, 0, sizeof(double) * total, s));
-    CN_HIP(hipMemcpyAsync(R, B, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
-  }
-  std::vector<double> bb(k), rz(k), tmp(k), alpha(k), beta(k), rr(k);
-  dot_to_host(B, B, bb.data());
-  apply_m();
-  CN_HIP(hipMemcpyAsync(P, Z, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
-  dot_to_host(R, Z, rz.data());
-  std::vector<int32_t> act(k, 1);
-  dot_to_host(R, R, rr.data());
-  for (int32_t c = 0; c < k; ++c)
-    if (!(rr[c] > o->tol_rel * o->tol_rel * bb[c])) act[c] = 0;
-  int32_t it = 0;
-  cn_status status = CN_MAXIT;
-  bool any = false;
-  for (int32_t c = 0; c < k; ++c) any |= act[c] != 0;
-  if (!any) status = CN_OK;
-  while (any && it < o->max_iter) {
-    launch_spmm_dot<double>(V, k, P, Q, W->part.get(), ctl, s, &rows);   // SpMV
-    dot_to_host(P, Q, tmp.data());                                        // p.Ap
-    for (int32_t c = 0; c < k; ++c) {
-      if (act[c] && tmp[c] == 0.0) { act[c] = 0; status = CN_BREAKDOWN; }
-      alpha[c] = act[c] ? rz[c] / tmp[c] : 0.0;
-    }
-    CN_HIP(hipMemcpyAsync(st + ST_ALPHA * k, alpha.data(), sizeof(double) * k,
-                          hipMemcpyHostToDevice, s));
-    k_axpy<double><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_ALPHA * k, 1.0, P, X);
-    k_axpy<double><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_ALPHA * k, -1.0, Q, R);
-    ++it;
-    dot_to_host(R, R, rr.data());                                         // stop test
-    any = false;
-    for (int32_t c = 0; c < k; ++c) {
-      if (act[c] && !(rr[c] > o->tol_rel * o->tol_rel * bb[c])) act[c] = 0;
-      any |= act[c] != 0;
-    }
-    if (!any) { if (status == CN_MAXIT) status = CN_OK; break; }
-    apply_m();                                                            // z = M^-1 r
-    dot_to_host(R, Z, tmp.data());                                        // r.z
-    for (int32_t c = 0; c < k; ++c) {
-      beta[c] = act[c] ? (rz[c] == 0.0 ? 0.0 : tmp[c] / rz[c]) : 0.0;
-      if (act[c]) rz[c] = tmp[c];
-    }
-    CN_HIP(hipMemcpyAsync(st + ST_BETA * k, beta.data(), sizeof(double) * k,
-                          hipMemcpyHostToDevice, s));
-    k_p_update<double><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_BETA * k, Z, P, ctl);
-  }
-  // true residual
-  launch_resid(n, V.ptr, V.col, V.val64, k, X, B, R, W->part.get(), s, &rows);
-  k_reduce_plain<<<red_blocks, kBlock, 0, s>>>(rows, k, W->part.get(), st + ST_TMP * k);
-  CN_HIP(hipMemcpyAsync(tmp.data(), st + ST_TMP * k, sizeof(double) * k,
-                        hipMemcpyDeviceToHost, s));
-  k_perm_scatter<<<ew_blocks(total), 256, 0, s>>>(n, k, perm, X, d_x);
-  CN_HIP(hipGetLastError());
-  CN_HIP(hipStreamSynchronize(s));
-  double relmax = 0.0;
-  for (int32_t c = 0; c < k; ++c) {
-    double rel = bb[c] > 0.0 ? std::sqrt(tmp[c] / bb[c]) : 0.0;
-    relmax = std::max(relmax, rel);
-  }
-  if (status == CN_OK && relmax > o->tol_rel * 10.0) status = CN_MAXIT;
-  if (info) {
-    info->status = status;
-    info->iters = it;
-    info->refinements = 0;
-    info->relres_max = relmax;
-  }
-  (void)hs;
-  return status;
-}
-
-}  // namespace cn
-
-extern "C" cn_status cn_pcg_solve(cn_system S, cn_ic0 F, const cn_pcg_opts *o,
-                                  int32_t k, const double *d_b, double *d_x,
-                                  cn_solve_info *info, void *stream) {
+cn_status cn_pcg_solve(cn_system S, cn_ic0 F, const cn_pcg_opts *o, int32_t k,
+                       const double *d_b, double *d_x, cn_solve_info *info,
+                       void *stream) {
   CN_API_BEGIN
   if (!S || !o || k <= 0 || !d_b || !d_x) {
     set_error("cn_pcg_solve: bad arguments");
@@ -1250,17 +1273,12 @@ extern "C" cn_status cn_pcg_solve(cn_system S, cn_ic0 F, const cn_pcg_opts *o,
     if (info) { info->status = CN_OK; info->iters = 0; info->refinements = 0; info->relres_max = 0.0; }
     return CN_OK;
   }
-  if (F && !F->l_rdiag64.get()) {
-    F->l_rdiag64.alloc((size_t)F->n + 1);
-    k_rdiag64<<<grid_for(F->n, 256), 256, 0, s>>>(F->n, F->l_diag.get(), F->l_rdiag64.get());
-    CN_HIP(hipGetLastError());
-  }
   prof_reset();
   cn_status st;
   switch (o->precision) {
     case CN_PREC_FP64_PLAIN: st = plain_solve(S, F, o, k, d_b, d_x, info, s); break;
-    case CN_PREC_FP64: st = refine_solve<double>(S, F, o, k, d_b, d_x, info, s); break;
-    case CN_PREC_MIXED: st = refine_solve<float>(S, F, o, k, d_b, d_x, info, s); break;
+    case CN_PREC_FP64: st = fused_solve<double>(S, F, o, k, d_b, d_x, info, s); break;
+    case CN_PREC_MIXED: st = fused_solve<float>(S, F, o, k, d_b, d_x, info, s); break;
     default:
       set_error("cn_pcg_solve: unknown precision %d", o->precision);
       return CN_BAD_ARG;
@@ -1269,3 +1287,5 @@ extern "C" cn_status cn_pcg_solve(cn_system S, cn_ic0 F, const cn_pcg_opts *o,
   return st;
   CN_API_END
 }
+
+}  // extern "C"
