@@ -35,8 +35,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--poses", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-9)
+    ap.add_argument("--ordering", default="block", choices=["block", "color", "natural"],
+                    help="IC(0) ordering (block multicolour, point colour, natural levels)")
+    ap.add_argument("--block-size", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-poses", type=int, default=8, help="poses in the CPU baseline sample")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--plain-poses", type=int, default=2,
                     help="poses solved by the plain FP64 one-solve-per-launch path")
     args = ap.parse_args()
@@ -55,11 +60,21 @@ def main():
     lib = _lib.load()
     dev = torch.device("cuda:0")
 
+    if args.copy_mb > 0:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters
+        # calibrate the HBM traffic counters of the solver kernels
+        src = torch.ones(args.copy_mb * 262144, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     # ---------------------------------------------------------- setup ------
     t0 = time.time()
     mesh = meshgen.config_mesh("capsules100k")
     lap = cn.CutLaplacian.from_mesh(mesh)
-    ic0 = cn.IC0(lap, "color")
+    ic0 = cn.IC0(lap, args.ordering, block_size=args.block_size)
     n, ncon = lap.n_free, lap.n_con
     nnz = lap.info.nnz_ff
     P = args.poses
@@ -186,8 +201,9 @@ def main():
         "data": "synthetic (seeded capsule union, seeded constraint displacements)",
         "config": {"workload": "capsule union 12 capsules / 16 ring cuts, "
                                f"{mesh.nv} verts, n_free={n}, nnz={nnz}, "
-                               f"{P} poses x 3 RHS (k={k}), IC(0) colour order "
-                               f"({ic0.info.nlevels} colours), tol {args.tol}"},
+                               f"{P} poses x 3 RHS (k={k}), IC(0) {args.ordering} order "
+                               f"({ic0.info.nlevels} colours, {ic0.info.nblocks} blocks), "
+                               f"tol {args.tol}"},
         "roofline": {"bound": "hbm", "achieved": spmm_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": spmm_gbs / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "kw_spmm_dot<float,4>", "avg_ms": spmm_avg_ms,

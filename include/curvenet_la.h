@@ -178,11 +178,22 @@ enum {
   CN_ORDER_COLOR = 0,   /* GraphColoring.greedyColor order (GraphColoring.lean:41-56):
                            rows grouped by colour, the colour-pass schedule of
                            SlangCodegen/IncompleteCholesky.lean:10-12 */
-  CN_ORDER_NATURAL = 1  /* natural order, rows grouped by dependency level */
+  CN_ORDER_NATURAL = 1, /* natural order, rows grouped by dependency level */
+  CN_ORDER_BLOCK = 2    /* block multicolour: compact BFS patches of
+                           `block_size` rows (natural order inside a patch),
+                           patches coloured by greedyColor on the patch
+                           graph; a patch is walked sequentially, patches of
+                           a colour in parallel */
 };
 
+typedef struct cn_ic0_opts {
+  int32_t ordering;     /* CN_ORDER_*                                  */
+  int32_t block_size;   /* CN_ORDER_BLOCK patch size (default 32)      */
+  int32_t allow_shift;  /* see cn_ic0_factor                           */
+} cn_ic0_opts;
+
 typedef struct cn_ic0_info {
-  int32_t n, nlevels, ordering;
+  int32_t n, nlevels, ordering, nblocks, max_block;
   int64_t nnz_l;        /* incl. diagonal */
   double shift;         /* diagonal shift used (0 unless a pivot failed) */
   const int32_t *d_perm;            /* perm[new] = old */
@@ -199,6 +210,9 @@ typedef struct cn_ic0_info {
  * (reported in info.shift). */
 cn_status cn_ic0_factor(cn_system sys, int32_t ordering, int32_t allow_shift,
                         cn_ic0 *out, void *stream);
+/* cn_ic0_factor with an options struct (block ordering and its size). */
+cn_status cn_ic0_factor_ex(cn_system sys, const cn_ic0_opts *opts, cn_ic0 *out,
+                           void *stream);
 cn_status cn_ic0_get_info(cn_ic0 f, cn_ic0_info *info);
 /* z = (L L^T)^{-1} r in the system's vertex order (FP64), d_r/d_z n x k. */
 cn_status cn_ic0_apply(cn_ic0 f, int32_t k, const double *d_r, double *d_z,

@@ -191,7 +191,7 @@ def test_spmv_kat(gpu):
 
 
 # ------------------------------------------------------------------ IC(0) --
-@pytest.mark.parametrize("ordering", ["color", "natural"])
+@pytest.mark.parametrize("ordering", ["color", "natural", "block"])
 def test_ic0_factor_bit_exact(gpu, ordering):
     from tool_godot_curvenet_amd import IC0
     m, lap = _system("cylinder64")
@@ -202,8 +202,11 @@ def test_ic0_factor_bit_exact(gpu, ordering):
         color, nc = O.greedy_color(A.row_ptr, A.col_idx)
         assert f.info.nlevels == nc
         assert np.array_equal(perm, np.argsort(color, kind="stable"))
-    else:
+    elif ordering == "natural":
         assert np.array_equal(perm, np.arange(A.rows)) or f.info.nlevels > 1
+    else:
+        assert f.info.max_block <= 32 and f.info.nblocks >= A.rows // 32
+        assert np.array_equal(np.sort(perm), np.arange(A.rows))
     pa = f.permuted_matrix()
     optr, ocol, oval = O.permute_csr(A.row_ptr, A.col_idx, A.values, perm)
     assert np.array_equal(pa.row_ptr, optr) and np.array_equal(pa.col_idx, ocol)
@@ -218,10 +221,11 @@ def test_ic0_factor_bit_exact(gpu, ordering):
     assert np.array_equal(gd, lv[lp[1:] - 1])
 
 
-def test_ic0_apply_bit_exact(gpu):
+@pytest.mark.parametrize("ordering", ["color", "block"])
+def test_ic0_apply_bit_exact(gpu, ordering):
     from tool_godot_curvenet_amd import IC0
     m, lap = _system("cylinder64")
-    f = IC0(lap, "color")
+    f = IC0(lap, ordering)
     A = lap.A_ff()
     perm = f.perm()
     optr, ocol, oval = O.permute_csr(A.row_ptr, A.col_idx, A.values, perm)
@@ -242,13 +246,17 @@ def _relres(A, X, B):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64", "fp64_plain"])
-@pytest.mark.parametrize("precond", ["ic0", "jacobi"])
-def test_pcg_cylinder(gpu, precision, precond):
+@pytest.mark.parametrize("precond", ["ic0-block", "ic0-color", "jacobi"])
+@pytest.mark.parametrize("k", [3, 96])
+def test_pcg_cylinder(gpu, precision, precond, k):
+    from tool_godot_curvenet_amd import IC0
     from tool_godot_curvenet_amd import meshgen as G
     m, lap = _system("cylinder64")
     A = lap.A_ff()
-    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, 3))
-    x, info = lap.solve(B, precond=precond, precision=precision, tol=1e-9)
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=k))
+    ic0 = IC0(lap, precond.split("-")[1]) if precond.startswith("ic0") else None
+    x, info = lap.solve(B, precond=precond.split("-")[0], precision=precision, tol=1e-9,
+                        ic0=ic0)
     X = x.cpu().numpy()
     assert info.status == 0, info.status
     rr = _relres(A, X, B)

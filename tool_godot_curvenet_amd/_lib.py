@@ -18,7 +18,7 @@ CN_BAD_ARG, CN_HIP_ERROR, CN_NO_DEVICE, CN_OOM, CN_NOT_MANIFOLD = -1, -2, -3, -4
 STATUS_NAMES = {0: "OK", 1: "MAXIT", 2: "BREAKDOWN", 3: "NOT_SPD", -1: "BAD_ARG",
                 -2: "HIP_ERROR", -3: "NO_DEVICE", -4: "OOM", -5: "NOT_MANIFOLD"}
 
-CN_ORDER_COLOR, CN_ORDER_NATURAL = 0, 1
+CN_ORDER_COLOR, CN_ORDER_NATURAL, CN_ORDER_BLOCK = 0, 1, 2
 CN_PRECOND_JACOBI, CN_PRECOND_IC0 = 0, 1
 CN_PREC_FP64_PLAIN, CN_PREC_FP64, CN_PREC_MIXED = 0, 1, 2
 
@@ -39,8 +39,13 @@ class SystemInfo(C.Structure):
                 ("d_he_next", vp), ("d_he_face", vp)]
 
 
+class Ic0Opts(C.Structure):
+    _fields_ = [("ordering", i32), ("block_size", i32), ("allow_shift", i32)]
+
+
 class Ic0Info(C.Structure):
-    _fields_ = [("n", i32), ("nlevels", i32), ("ordering", i32), ("nnz_l", i64),
+    _fields_ = [("n", i32), ("nlevels", i32), ("ordering", i32), ("nblocks", i32),
+                ("max_block", i32), ("nnz_l", i64),
                 ("shift", f64), ("d_perm", vp), ("d_a_ptr", vp), ("d_a_col", vp),
                 ("d_a_val", vp), ("d_l_ptr", vp), ("d_l_col", vp), ("d_l_val", vp),
                 ("d_l_diag", vp)]
@@ -77,6 +82,7 @@ _SIGS = {
     "cn_spmm_f64": (i32, [i32, vp, vp, vp, i32, vp, vp, vp]),
     "cn_spmm_f32": (i32, [i32, vp, vp, vp, i32, vp, vp, vp]),
     "cn_ic0_factor": (i32, [vp, i32, i32, C.POINTER(vp), vp]),
+    "cn_ic0_factor_ex": (i32, [vp, C.POINTER(Ic0Opts), C.POINTER(vp), vp]),
     "cn_ic0_get_info": (i32, [vp, C.POINTER(Ic0Info)]),
     "cn_ic0_apply": (i32, [vp, i32, vp, vp, vp]),
     "cn_ic0_destroy": (i32, [vp]),

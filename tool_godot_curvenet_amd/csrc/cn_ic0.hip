@@ -101,34 +101,38 @@ __global__ void k_u_map(int32_t n, const int32_t *up, const int32_t *ucol,
   }
 }
 
-// One level of the factorisation: rows [r0, r1) depend only on earlier rows.
-__global__ void k_ic0_rows(int32_t r0, int32_t r1, const int32_t *lp,
-                           const int32_t *lcol, double *lval,
-                           const double *adiag, double *ldiag, double shift,
-                           int32_t *fail) {
-  int32_t i = r0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= r1) return;
-  int32_t a0 = lp[i], a1 = lp[i + 1];
-  for (int32_t a = a0; a < a1; ++a) {
-    int32_t j = lcol[a];
-    double t = lval[a];
-    int32_t c = a0, b = lp[j], b1 = lp[j + 1];
-    while (c < a && b < b1) {
-      int32_t kc = lcol[c], kb = lcol[b];
-      if (kc == kb) { t = t - lval[c] * lval[b]; ++c; ++b; }
-      else if (kc < kb) ++c;
-      else ++b;
+// One level of the factorisation: the blocks [b0, b1) depend only on earlier
+// levels and on earlier rows of themselves; one thread per block walks its
+// rows in order.
+__global__ void k_ic0_blocks(int32_t b0, int32_t b1, const int32_t *blk_ptr,
+                             const int32_t *lp, const int32_t *lcol, double *lval,
+                             const double *adiag, double *ldiag, double shift,
+                             int32_t *fail) {
+  int32_t b = b0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= b1) return;
+  for (int32_t i = blk_ptr[b]; i < blk_ptr[b + 1]; ++i) {
+    int32_t a0 = lp[i], a1 = lp[i + 1];
+    for (int32_t a = a0; a < a1; ++a) {
+      int32_t j = lcol[a];
+      double t = lval[a];
+      int32_t c = a0, q = lp[j], q1 = lp[j + 1];
+      while (c < a && q < q1) {
+        int32_t kc = lcol[c], kq = lcol[q];
+        if (kc == kq) { t = t - lval[c] * lval[q]; ++c; ++q; }
+        else if (kc < kq) ++c;
+        else ++q;
+      }
+      lval[a] = t / ldiag[j];
     }
-    lval[a] = t / ldiag[j];
-  }
-  double s = adiag[i];
-  if (shift != 0.0) s = s + shift * adiag[i];
-  for (int32_t a = a0; a < a1; ++a) s = s - lval[a] * lval[a];
-  if (!(s > 0.0)) {
-    atomicMin(fail, i);
-    ldiag[i] = 1.0;
-  } else {
-    ldiag[i] = sqrt(s);
+    double sd = adiag[i];
+    if (shift != 0.0) sd = sd + shift * adiag[i];
+    for (int32_t a = a0; a < a1; ++a) sd = sd - lval[a] * lval[a];
+    if (!(sd > 0.0)) {
+      atomicMin(fail, i);
+      ldiag[i] = 1.0;
+    } else {
+      ldiag[i] = sqrt(sd);
+    }
   }
 }
 
@@ -157,31 +161,34 @@ __global__ void k_renumber(int64_t nnz, const int32_t *perm, const int32_t *col,
   if (e < nnz) out[e] = perm[col[e]];
 }
 
-// Exact (spec-order) sweeps for cn_ic0_apply: thread per (row, column).
-__global__ void k_fwd_exact(int32_t r0, int32_t r1, int32_t k, const int32_t *lp,
-                            const int32_t *lcol, const double *lval,
+// Exact (spec-order) sweeps for cn_ic0_apply: thread per (block, column),
+// the block's rows in order (forward) or reverse order (backward).
+__global__ void k_fwd_exact(int32_t b0, int32_t b1, int32_t k, const int32_t *blk_ptr,
+                            const int32_t *lp, const int32_t *lcol, const double *lval,
                             const double *ldiag, const double *b, double *y) {
   int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t rows = r1 - r0;
-  if (id >= rows * k) return;
-  int32_t i = r0 + (int32_t)(id / k), c = (int32_t)(id % k);
-  double s = b[(int64_t)i * k + c];
-  for (int32_t p = lp[i]; p < lp[i + 1]; ++p)
-    s = s - lval[p] * y[(int64_t)lcol[p] * k + c];
-  y[(int64_t)i * k + c] = s / ldiag[i];
+  if (id >= (int64_t)(b1 - b0) * k) return;
+  int32_t blk = b0 + (int32_t)(id / k), c = (int32_t)(id % k);
+  for (int32_t i = blk_ptr[blk]; i < blk_ptr[blk + 1]; ++i) {
+    double sum = b[(int64_t)i * k + c];
+    for (int32_t p = lp[i]; p < lp[i + 1]; ++p)
+      sum = sum - lval[p] * y[(int64_t)lcol[p] * k + c];
+    y[(int64_t)i * k + c] = sum / ldiag[i];
+  }
 }
 
-__global__ void k_bwd_exact(int32_t r0, int32_t r1, int32_t k, const int32_t *up,
-                            const int32_t *ucol, const double *uval,
+__global__ void k_bwd_exact(int32_t b0, int32_t b1, int32_t k, const int32_t *blk_ptr,
+                            const int32_t *up, const int32_t *ucol, const double *uval,
                             const double *ldiag, const double *y, double *x) {
   int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t rows = r1 - r0;
-  if (id >= rows * k) return;
-  int32_t i = r0 + (int32_t)(id / k), c = (int32_t)(id % k);
-  double s = y[(int64_t)i * k + c];
-  for (int32_t p = up[i]; p < up[i + 1]; ++p)
-    s = s - uval[p] * x[(int64_t)ucol[p] * k + c];
-  x[(int64_t)i * k + c] = s / ldiag[i];
+  if (id >= (int64_t)(b1 - b0) * k) return;
+  int32_t blk = b0 + (int32_t)(id / k), c = (int32_t)(id % k);
+  for (int32_t i = blk_ptr[blk + 1] - 1; i >= blk_ptr[blk]; --i) {
+    double sum = y[(int64_t)i * k + c];
+    for (int32_t p = up[i]; p < up[i + 1]; ++p)
+      sum = sum - uval[p] * x[(int64_t)ucol[p] * k + c];
+    x[(int64_t)i * k + c] = sum / ldiag[i];
+  }
 }
 
 __global__ void k_gather_rows(int32_t n, int32_t k, const int32_t *perm,
@@ -240,6 +247,50 @@ static int32_t natural_levels(int32_t n, const std::vector<int32_t> &ptr,
   return nl;
 }
 
+// Block multicolour ordering: compact BFS patches of up to `bs` rows grown
+// from the lowest unassigned row, patches coloured with greedyColor
+// (GraphColoring.lean:41-56) on the patch adjacency graph.
+static int32_t block_levels(int32_t n, const std::vector<int32_t> &ptr,
+                            const std::vector<int32_t> &col, int32_t bs,
+                            std::vector<int32_t> &blk, std::vector<int32_t> &bcolor) {
+  blk.assign(n, -1);
+  std::vector<int32_t> queue;
+  queue.reserve(bs);
+  int32_t nb = 0;
+  for (int32_t s0 = 0; s0 < n; ++s0) {
+    if (blk[s0] >= 0) continue;
+    queue.clear();
+    queue.push_back(s0);
+    blk[s0] = nb;
+    int32_t cnt = 1;
+    for (size_t h = 0; h < queue.size() && cnt < bs; ++h) {
+      int32_t u = queue[h];
+      for (int32_t p = ptr[u]; p < ptr[u + 1] && cnt < bs; ++p) {
+        int32_t w = col[p];
+        if (blk[w] < 0) { blk[w] = nb; ++cnt; queue.push_back(w); }
+      }
+    }
+    ++nb;
+  }
+  // patch graph (CSR, sorted, no self loops)
+  std::vector<int64_t> keys;
+  keys.reserve(ptr[n]);
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+      int32_t a = blk[i], b = blk[col[p]];
+      if (a != b) keys.push_back(((int64_t)a << 32) | (uint32_t)b);
+    }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  std::vector<int32_t> gptr(nb + 1, 0), gcol(keys.size());
+  for (size_t e = 0; e < keys.size(); ++e) {
+    gptr[(keys[e] >> 32) + 1]++;
+    gcol[e] = (int32_t)(keys[e] & 0xffffffff);
+  }
+  for (int32_t b = 0; b < nb; ++b) gptr[b + 1] += gptr[b];
+  return greedy_color(nb, gptr, gcol, bcolor);
+}
+
 }  // namespace cn
 
 using namespace cn;
@@ -248,11 +299,25 @@ extern "C" {
 
 cn_status cn_ic0_factor(cn_system S, int32_t ordering, int32_t allow_shift,
                         cn_ic0 *out, void *stream) {
+  cn_ic0_opts o;
+  o.ordering = ordering;
+  o.block_size = 32;
+  o.allow_shift = allow_shift;
+  return cn_ic0_factor_ex(S, &o, out, stream);
+}
+
+cn_status cn_ic0_factor_ex(cn_system S, const cn_ic0_opts *opt, cn_ic0 *out,
+                           void *stream) {
   CN_API_BEGIN
-  if (!S || !out || (ordering != CN_ORDER_COLOR && ordering != CN_ORDER_NATURAL)) {
+  if (!S || !out || !opt ||
+      (opt->ordering != CN_ORDER_COLOR && opt->ordering != CN_ORDER_NATURAL &&
+       opt->ordering != CN_ORDER_BLOCK) ||
+      (opt->ordering == CN_ORDER_BLOCK && (opt->block_size < 1 || opt->block_size > 4096))) {
     set_error("cn_ic0_factor: bad arguments");
     return CN_BAD_ARG;
   }
+  const int32_t ordering = opt->ordering;
+  const int32_t allow_shift = opt->allow_shift;
   *out = nullptr;
   hipStream_t s = as_stream(stream);
   const int32_t n = S->n_free;
@@ -263,19 +328,57 @@ cn_status cn_ic0_factor(cn_system S, int32_t ordering, int32_t allow_shift,
     CN_HIP(hipMemcpyAsync(col.data(), S->ff_col.get(), sizeof(int32_t) * S->nnz_ff,
                           hipMemcpyDeviceToHost, s));
   CN_HIP(hipStreamSynchronize(s));
-  std::vector<int32_t> lev;
-  int32_t nl = ordering == CN_ORDER_COLOR ? greedy_color(n, ptr, col, lev)
-                                          : natural_levels(n, ptr, col, lev);
+  // per row: level and block id; rows sort by (level, block, row)
+  std::vector<int32_t> lev, blk;
+  int32_t nl = 0;
+  if (ordering == CN_ORDER_BLOCK) {
+    std::vector<int32_t> bcolor;
+    nl = block_levels(n, ptr, col, opt->block_size, blk, bcolor);
+    lev.resize(n);
+    for (int32_t i = 0; i < n; ++i) lev[i] = bcolor[blk[i]];
+  } else {
+    nl = ordering == CN_ORDER_COLOR ? greedy_color(n, ptr, col, lev)
+                                    : natural_levels(n, ptr, col, lev);
+    blk.resize(n);
+    for (int32_t i = 0; i < n; ++i) blk[i] = i;
+  }
   cn_ic0_t *F = new cn_ic0_t();
   try {
     F->n = n;
     F->ordering = ordering;
     F->ncolors = nl;
+    F->block_size = ordering == CN_ORDER_BLOCK ? opt->block_size : 1;
+    std::vector<int32_t> perm(n);
+    for (int32_t i = 0; i < n; ++i) perm[i] = i;
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) {
+      if (lev[a] != lev[b]) return lev[a] < lev[b];
+      if (blk[a] != blk[b]) return blk[a] < blk[b];
+      return a < b;
+    });
     F->level_ptr.assign(nl + 1, 0);
-    for (int32_t i = 0; i < n; ++i) F->level_ptr[lev[i] + 1]++;
-    for (int32_t c = 0; c < nl; ++c) F->level_ptr[c + 1] += F->level_ptr[c];
-    std::vector<int32_t> perm(n), iperm(n), fill(F->level_ptr.begin(), F->level_ptr.end() - 1);
-    for (int32_t i = 0; i < n; ++i) perm[fill[lev[i]]++] = i;  // stable
+    F->level_blk.assign(nl + 1, 0);
+    std::vector<int32_t> bptr;
+    bptr.reserve(n + 1);
+    for (int32_t q = 0; q < n; ++q) {
+      int32_t r = perm[q];
+      F->level_ptr[lev[r] + 1]++;
+      if (q == 0 || blk[perm[q - 1]] != blk[r]) {
+        bptr.push_back(q);
+        F->level_blk[lev[r] + 1]++;
+      }
+    }
+    bptr.push_back(n);
+    for (int32_t c = 0; c < nl; ++c) {
+      F->level_ptr[c + 1] += F->level_ptr[c];
+      F->level_blk[c + 1] += F->level_blk[c];
+    }
+    const int32_t nblk = (int32_t)bptr.size() - 1;
+    for (int32_t b = 0; b < nblk; ++b) F->max_block = std::max(F->max_block, bptr[b + 1] - bptr[b]);
+    F->blk_ptr.alloc((size_t)nblk + 1);
+    CN_HIP(hipMemcpyAsync(F->blk_ptr.get(), bptr.data(), sizeof(int32_t) * (nblk + 1),
+                          hipMemcpyHostToDevice, s));
+    CN_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> iperm(n);
     for (int32_t i = 0; i < n; ++i) iperm[perm[i]] = i;
     F->perm.alloc(n + 1);
     F->iperm.alloc(n + 1);
@@ -362,11 +465,11 @@ cn_status cn_ic0_factor(cn_system S, int32_t ordering, int32_t allow_shift,
                             hipMemcpyDeviceToDevice, s));
       CN_HIP(hipMemcpyAsync(fail.get(), &big, sizeof(int32_t), hipMemcpyHostToDevice, s));
       for (int32_t c = 0; c < nl; ++c) {
-        int32_t r0 = F->level_ptr[c], r1 = F->level_ptr[c + 1];
-        if (r1 > r0)
-          k_ic0_rows<<<grid_for(r1 - r0, 64), 64, 0, s>>>(
-              r0, r1, F->l_ptr.get(), F->l_col.get(), F->l_val.get(), adiag.get(),
-              F->l_diag.get(), shifts[si], fail.get());
+        int32_t b0 = F->level_blk[c], b1 = F->level_blk[c + 1];
+        if (b1 > b0)
+          k_ic0_blocks<<<grid_for(b1 - b0, 64), 64, 0, s>>>(
+              b0, b1, F->blk_ptr.get(), F->l_ptr.get(), F->l_col.get(), F->l_val.get(),
+              adiag.get(), F->l_diag.get(), shifts[si], fail.get());
       }
       CN_HIP(hipGetLastError());
       CN_HIP(hipMemcpyAsync(&hf, fail.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -413,6 +516,8 @@ cn_status cn_ic0_get_info(cn_ic0 F, cn_ic0_info *info) {
   if (!F || !info) { set_error("null handle"); return CN_BAD_ARG; }
   info->n = F->n;
   info->nlevels = F->ncolors;
+  info->nblocks = F->level_blk.empty() ? 0 : F->level_blk.back();
+  info->max_block = F->max_block;
   info->ordering = F->ordering;
   info->nnz_l = F->nnz_l;
   info->shift = F->shift;
@@ -438,20 +543,22 @@ cn_status cn_ic0_apply(cn_ic0 F, int32_t k, const double *d_r, double *d_z,
   DevBuf<double> rp(tot), y(tot), x(tot);
   k_gather_rows<<<grid_for(tot, 256), 256, 0, s>>>(n, k, F->perm.get(), d_r, rp.get());
   for (int32_t c = 0; c < F->ncolors; ++c) {
-    int32_t r0 = F->level_ptr[c], r1 = F->level_ptr[c + 1];
-    int64_t t = (int64_t)(r1 - r0) * k;
+    int32_t b0 = F->level_blk[c], b1 = F->level_blk[c + 1];
+    int64_t t = (int64_t)(b1 - b0) * k;
     if (t > 0)
-      k_fwd_exact<<<grid_for(t, 256), 256, 0, s>>>(r0, r1, k, F->l_ptr.get(),
-                                                  F->l_col.get(), F->l_val.get(),
-                                                  F->l_diag.get(), rp.get(), y.get());
+      k_fwd_exact<<<grid_for(t, 256), 256, 0, s>>>(b0, b1, k, F->blk_ptr.get(),
+                                                  F->l_ptr.get(), F->l_col.get(),
+                                                  F->l_val.get(), F->l_diag.get(),
+                                                  rp.get(), y.get());
   }
   for (int32_t c = F->ncolors - 1; c >= 0; --c) {
-    int32_t r0 = F->level_ptr[c], r1 = F->level_ptr[c + 1];
-    int64_t t = (int64_t)(r1 - r0) * k;
+    int32_t b0 = F->level_blk[c], b1 = F->level_blk[c + 1];
+    int64_t t = (int64_t)(b1 - b0) * k;
     if (t > 0)
-      k_bwd_exact<<<grid_for(t, 256), 256, 0, s>>>(r0, r1, k, F->u_ptr.get(),
-                                                  F->u_col.get(), F->u_val.get(),
-                                                  F->l_diag.get(), y.get(), x.get());
+      k_bwd_exact<<<grid_for(t, 256), 256, 0, s>>>(b0, b1, k, F->blk_ptr.get(),
+                                                  F->u_ptr.get(), F->u_col.get(),
+                                                  F->u_val.get(), F->l_diag.get(),
+                                                  y.get(), x.get());
   }
   k_scatter_rows<<<grid_for(tot, 256), 256, 0, s>>>(n, k, F->perm.get(), x.get(), d_z);
   CN_HIP(hipGetLastError());

@@ -136,10 +136,18 @@ struct cn_system_t {
 };
 
 struct cn_ic0_t {
-  int32_t n = 0, ncolors = 0, ordering = 0;
+  int32_t n = 0, ncolors = 0, ordering = 0, block_size = 1, max_block = 1;
   double shift = 0.0;
-  std::vector<int32_t> level_ptr;   // host copy: rows of level c are
+  // Rows are renumbered into levels (colours), each level into blocks, each
+  // block a run of consecutive positions.  Rows of a block depend on earlier
+  // rows of the same block and on earlier levels only, so one thread or
+  // wavefront walks a block in order while all blocks of a level run in
+  // parallel.  Point orderings have one row per block.
+  std::vector<int32_t> level_ptr;   // host: positions of level c are
                                     // [level_ptr[c], level_ptr[c+1])
+  std::vector<int32_t> level_blk;   // host: blocks of level c are
+                                    // [level_blk[c], level_blk[c+1])
+  cn::DevBuf<int32_t> blk_ptr;      // positions of block b: [blk_ptr[b], blk_ptr[b+1])
   cn::DevBuf<int32_t> perm, iperm;  // perm[new] = old
   // permuted A (FP64 + FP32) for the solver
   cn::DevBuf<int32_t> a_ptr, a_col;

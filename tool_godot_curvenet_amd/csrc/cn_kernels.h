@@ -29,6 +29,27 @@ __device__ __forceinline__ int32_t uniform(int32_t x) {
   return __builtin_amdgcn_readfirstlane(x);
 }
 
+// MI355X dispatches workgroups round-robin over its 8 XCDs, each with its
+// own 4 MB L2.  Row kernels therefore give each XCD one contiguous eighth of
+// the rows, and the XCD's row slots (its waves or 8-lane groups) interleave
+// over that eighth: the rows in flight on an XCD form one compact window
+// that slides forward, so a row's neighbours -- the x rows it gathers, a
+// few ring-lengths away in mesh order -- are still in that XCD's L2.
+constexpr int kXcd = 8;
+struct RowIter {
+  int32_t first, end, stride;
+};
+// slots: row slots per block (4 waves, or 32 groups); slot: this thread's
+__device__ __forceinline__ RowIter row_iter(int32_t n0, int32_t n1, int32_t slots,
+                                            int32_t slot) {
+  const int32_t nb = gridDim.x;
+  if (nb % kXcd != 0) return RowIter{n0 + (int32_t)blockIdx.x * slots + slot, n1, nb * slots};
+  const int32_t x = blockIdx.x % kXcd, lb = blockIdx.x / kXcd, nbx = nb / kXcd;
+  const int32_t chunk = (n1 - n0 + kXcd - 1) / kXcd;
+  const int32_t cs = n0 + x * chunk;
+  return RowIter{cs + lb * slots + slot, min(n1, cs + chunk), nbx * slots};
+}
+
 template <typename T, int VEC>
 struct VecIO;
 template <>
@@ -75,7 +96,7 @@ template <typename T, int VEC, int CH = 8>
 __device__ __forceinline__ void gather_wide(int32_t a, int32_t b,
                                             const int32_t *__restrict__ col,
                                             const T *__restrict__ val,
-                                            const T *__restrict__ X, int64_t k,
+                                            const T *X, int64_t k,
                                             int64_t c0, bool valid,
                                             T (&acc)[VEC]) {
   for (int32_t e = a; e < b; e += CH) {
@@ -104,7 +125,7 @@ template <typename T, int K>
 __device__ __forceinline__ void gather_narrow(int32_t a, int32_t b, int g,
                                               const int32_t *__restrict__ col,
                                               const T *__restrict__ val,
-                                              const T *__restrict__ X,
+                                              const T *X,
                                               T (&acc)[K]) {
   for (int32_t e = a + g; e < b; e += kGroup) {
     int32_t j = col[e];

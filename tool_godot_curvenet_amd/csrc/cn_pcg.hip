@@ -54,13 +54,13 @@ __device__ __forceinline__ bool bwd_idle(const int32_t *ctl) {
 }
 
 // ================================================================ kernels ==
+// rows of an XCD's contiguous eighth, interleaved over its row slots
 #define NARROW_LOOP(row, n0, n1)                                              \
-  for (int64_t row = (int64_t)(n0) + (int64_t)blockIdx.x * (kBlock / kGroup) + \
-                     threadIdx.x / kGroup;                                     \
-       row < (n1); row += (int64_t)gridDim.x * (kBlock / kGroup))
+  const RowIter ri_ = row_iter((n0), (n1), kBlock / kGroup, threadIdx.x / kGroup); \
+  for (int64_t row = ri_.first; row < ri_.end; row += ri_.stride)
 #define WIDE_LOOP(row, n0, n1)                                                \
-  for (int32_t row = (n0) + blockIdx.x * kWaves + (threadIdx.x >> 6);        \
-       row < (n1); row += gridDim.x * kWaves)
+  const RowIter ri_ = row_iter((n0), (n1), kWaves, threadIdx.x >> 6);         \
+  for (int32_t row = ri_.first; row < ri_.end; row += ri_.stride)
 #define WIDE_COLS(VEC)                                                        \
   const int lane = threadIdx.x & 63;                                          \
   const int64_t c0 = (int64_t)blockIdx.y * 64 * (VEC) + lane * (VEC);         \
@@ -160,21 +160,26 @@ kw_resid(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__
   block_partials_wide<VEC, 1>(pd, part, k, c0, 0);
 }
 
-// ---- IC(0) forward pass over level positions [r0, r1):
+// ---- IC(0) forward pass over the blocks [b0, b1) of one level:
 //      (FUSED) r -= alpha q;  y = (r - sum L y) / L_ii;  partials r.r, y.y
+// A block's rows are walked in order by the same lanes that own the
+// columns they write, so reads of rows written earlier in the block are
+// same-thread reads.
 template <typename T, int K, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
-kn_fwd(int32_t r0, int32_t r1, const int32_t *__restrict__ rowmap,
-       const int32_t *__restrict__ lp, const int32_t *__restrict__ lcol,
-       const T *__restrict__ lval, const T *__restrict__ rdiag,
-       const double *__restrict__ alpha, const T *__restrict__ Q, T *__restrict__ R,
-       T *__restrict__ Y, double *__restrict__ part, int64_t stride,
-       const int32_t *__restrict__ ctl) {
+kn_fwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
+       const int32_t *__restrict__ rowmap, const int32_t *__restrict__ lp,
+       const int32_t *__restrict__ lcol, const T *__restrict__ lval,
+       const T *__restrict__ rdiag, const double *__restrict__ alpha,
+       const T *__restrict__ Q, T *__restrict__ R, T *Y, double *__restrict__ part,
+       int64_t stride, const int32_t *__restrict__ ctl) {
+  // one row per block: 8-lane groups, lanes stride over the row's entries
   if (ctl[CTL_ACTIVE] == 0) return;
   const int g = threadIdx.x % kGroup;
   double pd[2] = {0.0, 0.0};
   const T al = (FUSED && g < K) ? (T)alpha[g] : T(0);
-  NARROW_LOOP(pos, r0, r1) {
+  NARROW_LOOP(bi, b0, b1) {
+    const int32_t pos = blk_ptr[bi];
     const int64_t row = rowmap[pos];
     T acc[K] = {};
     gather_narrow<T, K>(lp[pos], lp[pos + 1], g, lcol, lval, Y, acc);
@@ -194,13 +199,77 @@ kn_fwd(int32_t r0, int32_t r1, const int32_t *__restrict__ rowmap,
   block_partials_narrow<K, 2>(pd, part, stride);
 }
 
+// Per-thread partial reduction (thread t holds K columns) for the
+// thread-per-block kernels: fixed-order sum over the block's 256 threads.
+template <int K, int NP>
+__device__ __forceinline__ void block_partials_thread(const double (&p)[NP][K],
+                                                      double *__restrict__ out,
+                                                      int64_t stride_np) {
+  __shared__ double sh[NP][K][kBlock];
+#pragma unroll
+  for (int q = 0; q < NP; ++q)
+#pragma unroll
+    for (int c = 0; c < K; ++c) sh[q][c][threadIdx.x] = p[q][c];
+  __syncthreads();
+  if (threadIdx.x < NP * K) {
+    const int q = threadIdx.x / K, c = threadIdx.x % K;
+    double t = 0.0;
+    for (int i = 0; i < kBlock; ++i) t += sh[q][c][i];
+    out[q * stride_np + (int64_t)blockIdx.x * K + c] = t;
+  }
+}
+
+// multi-row blocks, k <= 4: one thread per block, all K columns
+template <typename T, int K, bool FUSED>
+__global__ void __launch_bounds__(kBlock)
+kt_fwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
+       const int32_t *__restrict__ rowmap, const int32_t *__restrict__ lp,
+       const int32_t *__restrict__ lcol, const T *__restrict__ lval,
+       const T *__restrict__ rdiag, const double *__restrict__ alpha,
+       const T *__restrict__ Q, T *__restrict__ R, T *Y, double *__restrict__ part,
+       int64_t stride, const int32_t *__restrict__ ctl) {
+  if (ctl[CTL_ACTIVE] == 0) return;
+  double pd[2][K] = {};
+  T al[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) al[c] = FUSED ? (T)alpha[c] : T(0);
+  const RowIter ri = row_iter(b0, b1, kBlock, threadIdx.x);
+  for (int32_t bi = ri.first; bi < ri.end; bi += ri.stride) {
+    for (int32_t pos = blk_ptr[bi]; pos < blk_ptr[bi + 1]; ++pos) {
+      const int64_t row = rowmap[pos];
+      T acc[K] = {};
+      for (int32_t e = lp[pos]; e < lp[pos + 1]; ++e) {
+        const int64_t j = lcol[e];
+        const T v = lval[e];
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc[c] = fma(v, Y[j * K + c], acc[c]);
+      }
+      const T rd = rdiag[pos];
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        const int64_t o = row * K + c;
+        T r = R[o];
+        if (FUSED) {
+          r = fma(-al[c], Q[o], r);
+          R[o] = r;
+        }
+        T y = (r - acc[c]) * rd;
+        Y[o] = y;
+        pd[0][c] += (double)r * (double)r;
+        pd[1][c] += (double)y * (double)y;
+      }
+    }
+  }
+  block_partials_thread<K, 2>(pd, part, stride);
+}
+
 template <typename T, int VEC, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
-kw_fwd(int32_t r0, int32_t r1, const int32_t *__restrict__ rowmap,
-       const int32_t *__restrict__ lp, const int32_t *__restrict__ lcol,
-       const T *__restrict__ lval, const T *__restrict__ rdiag,
-       const double *__restrict__ alpha, int64_t k, const T *__restrict__ Q,
-       T *__restrict__ R, T *__restrict__ Y, double *__restrict__ part,
+kw_fwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
+       const int32_t *__restrict__ rowmap, const int32_t *__restrict__ lp,
+       const int32_t *__restrict__ lcol, const T *__restrict__ lval,
+       const T *__restrict__ rdiag, const double *__restrict__ alpha, int64_t k,
+       const T *__restrict__ Q, T *__restrict__ R, T *Y, double *__restrict__ part,
        int64_t stride, const int32_t *__restrict__ ctl) {
   if (ctl[CTL_ACTIVE] == 0) return;
   WIDE_COLS(VEC)
@@ -208,52 +277,56 @@ kw_fwd(int32_t r0, int32_t r1, const int32_t *__restrict__ rowmap,
   T al[VEC];
 #pragma unroll
   for (int m = 0; m < VEC; ++m) al[m] = (FUSED && valid) ? (T)alpha[c0 + m] : T(0);
-  WIDE_LOOP(pos, r0, r1) {
-    const int32_t ps = uniform(pos);
-    const int32_t r = uniform(rowmap[ps]);
-    const int32_t a = uniform(lp[ps]), b = uniform(lp[ps + 1]);
-    T acc[VEC] = {};
-    gather_wide<T, VEC, 4>(a, b, lcol, lval, Y, k, c0, valid, acc);
-    if (valid) {
-      const int64_t o = (int64_t)r * k + c0;
-      T rv[VEC];
-      VecIO<T, VEC>::ld(R + o, rv);
-      if (FUSED) {
-        T qv[VEC];
-        VecIO<T, VEC>::ld(Q + o, qv);
+  WIDE_LOOP(bi, b0, b1) {
+    const int32_t bq = uniform(bi);
+    const int32_t p0 = uniform(blk_ptr[bq]), p1 = uniform(blk_ptr[bq + 1]);
+    for (int32_t ps = p0; ps < p1; ++ps) {
+      const int32_t r = uniform(rowmap[ps]);
+      const int32_t a = uniform(lp[ps]), b = uniform(lp[ps + 1]);
+      T acc[VEC] = {};
+      gather_wide<T, VEC, 4>(a, b, lcol, lval, Y, k, c0, valid, acc);
+      if (valid) {
+        const int64_t o = (int64_t)r * k + c0;
+        T rv[VEC];
+        VecIO<T, VEC>::ld(R + o, rv);
+        if (FUSED) {
+          T qv[VEC];
+          VecIO<T, VEC>::ld(Q + o, qv);
 #pragma unroll
-        for (int m = 0; m < VEC; ++m) rv[m] = fma(-al[m], qv[m], rv[m]);
-        VecIO<T, VEC>::st(R + o, rv);
-      }
-      const T rd = rdiag[ps];
+          for (int m = 0; m < VEC; ++m) rv[m] = fma(-al[m], qv[m], rv[m]);
+          VecIO<T, VEC>::st(R + o, rv);
+        }
+        const T rd = rdiag[ps];
 #pragma unroll
-      for (int m = 0; m < VEC; ++m) {
-        acc[m] = (rv[m] - acc[m]) * rd;
-        pd[0][m] += (double)rv[m] * (double)rv[m];
-        pd[1][m] += (double)acc[m] * (double)acc[m];
+        for (int m = 0; m < VEC; ++m) {
+          acc[m] = (rv[m] - acc[m]) * rd;
+          pd[0][m] += (double)rv[m] * (double)rv[m];
+          pd[1][m] += (double)acc[m] * (double)acc[m];
+        }
+        VecIO<T, VEC>::st(Y + o, acc);
       }
-      VecIO<T, VEC>::st(Y + o, acc);
     }
   }
   block_partials_wide<VEC, 2>(pd, part, k, c0, stride);
 }
 
-// ---- IC(0) backward pass over level positions [r0, r1), in place on Z
-//      (which holds y):  z = (y - sum U z) / L_ii;
-//      d += alpha p;  p = gamma z + beta p        (per-column alpha/beta/gamma)
+// ---- IC(0) backward pass over the blocks [b0, b1) of one level, in place
+//      on Z (which holds y), block rows in reverse:
+//      z = (y - sum U z) / L_ii;  d += alpha p;  p = gamma z + beta p
 template <typename T, int K>
 __global__ void __launch_bounds__(kBlock)
-kn_bwd(int32_t r0, int32_t r1, const int32_t *__restrict__ rowmap,
-       const int32_t *__restrict__ up, const int32_t *__restrict__ ucol,
-       const T *__restrict__ uval, const T *__restrict__ rdiag,
-       const double *__restrict__ st, int64_t k, T *__restrict__ Z,
+kn_bwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
+       const int32_t *__restrict__ rowmap, const int32_t *__restrict__ up,
+       const int32_t *__restrict__ ucol, const T *__restrict__ uval,
+       const T *__restrict__ rdiag, const double *__restrict__ st, int64_t k, T *Z,
        T *__restrict__ P, T *__restrict__ D, const int32_t *__restrict__ ctl) {
   if (bwd_idle(ctl)) return;
   const int g = threadIdx.x % kGroup;
   const int gc = g < K ? g : 0;
   const T al = (T)st[ST_ALPHA * k + gc], be = (T)st[ST_BETA * k + gc],
           ga = (T)st[ST_GAMMA * k + gc];
-  NARROW_LOOP(pos, r0, r1) {
+  NARROW_LOOP(bi, b0, b1) {
+    const int32_t pos = blk_ptr[bi];
     const int64_t row = rowmap[pos];
     T acc[K] = {};
     gather_narrow<T, K>(up[pos], up[pos + 1], g, ucol, uval, Z, acc);
@@ -268,12 +341,52 @@ kn_bwd(int32_t r0, int32_t r1, const int32_t *__restrict__ rowmap,
   }
 }
 
+template <typename T, int K>
+__global__ void __launch_bounds__(kBlock)
+kt_bwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
+       const int32_t *__restrict__ rowmap, const int32_t *__restrict__ up,
+       const int32_t *__restrict__ ucol, const T *__restrict__ uval,
+       const T *__restrict__ rdiag, const double *__restrict__ st, int64_t k, T *Z,
+       T *__restrict__ P, T *__restrict__ D, const int32_t *__restrict__ ctl) {
+  if (bwd_idle(ctl)) return;
+  T al[K], be[K], ga[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    al[c] = (T)st[ST_ALPHA * k + c];
+    be[c] = (T)st[ST_BETA * k + c];
+    ga[c] = (T)st[ST_GAMMA * k + c];
+  }
+  const RowIter ri = row_iter(b0, b1, kBlock, threadIdx.x);
+  for (int32_t bi = ri.first; bi < ri.end; bi += ri.stride) {
+    for (int32_t pos = blk_ptr[bi + 1] - 1; pos >= blk_ptr[bi]; --pos) {
+      const int64_t row = rowmap[pos];
+      T acc[K] = {};
+      for (int32_t e = up[pos]; e < up[pos + 1]; ++e) {
+        const int64_t j = ucol[e];
+        const T v = uval[e];
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc[c] = fma(v, Z[j * K + c], acc[c]);
+      }
+      const T rd = rdiag[pos];
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        const int64_t o = row * K + c;
+        T z = (Z[o] - acc[c]) * rd;
+        Z[o] = z;
+        T p = P[o];
+        D[o] = fma(al[c], p, D[o]);
+        P[o] = fma(be[c], p, ga[c] * z);
+      }
+    }
+  }
+}
+
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
-kw_bwd(int32_t r0, int32_t r1, const int32_t *__restrict__ rowmap,
-       const int32_t *__restrict__ up, const int32_t *__restrict__ ucol,
-       const T *__restrict__ uval, const T *__restrict__ rdiag,
-       const double *__restrict__ st, int64_t k, T *__restrict__ Z,
+kw_bwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
+       const int32_t *__restrict__ rowmap, const int32_t *__restrict__ up,
+       const int32_t *__restrict__ ucol, const T *__restrict__ uval,
+       const T *__restrict__ rdiag, const double *__restrict__ st, int64_t k, T *Z,
        T *__restrict__ P, T *__restrict__ D, const int32_t *__restrict__ ctl) {
   if (bwd_idle(ctl)) return;
   WIDE_COLS(VEC)
@@ -284,28 +397,31 @@ kw_bwd(int32_t r0, int32_t r1, const int32_t *__restrict__ rowmap,
     be[m] = valid ? (T)st[ST_BETA * k + c0 + m] : T(0);
     ga[m] = valid ? (T)st[ST_GAMMA * k + c0 + m] : T(0);
   }
-  WIDE_LOOP(pos, r0, r1) {
-    const int32_t ps = uniform(pos);
-    const int32_t r = uniform(rowmap[ps]);
-    const int32_t a = uniform(up[ps]), b = uniform(up[ps + 1]);
-    T acc[VEC] = {};
-    gather_wide<T, VEC, 4>(a, b, ucol, uval, Z, k, c0, valid, acc);
-    if (valid) {
-      const int64_t o = (int64_t)r * k + c0;
-      T yv[VEC], pv[VEC], dv[VEC];
-      VecIO<T, VEC>::ld(Z + o, yv);
-      VecIO<T, VEC>::ld(P + o, pv);
-      VecIO<T, VEC>::ld(D + o, dv);
-      const T rd = rdiag[ps];
+  WIDE_LOOP(bi, b0, b1) {
+    const int32_t bq = uniform(bi);
+    const int32_t p0 = uniform(blk_ptr[bq]), p1 = uniform(blk_ptr[bq + 1]);
+    for (int32_t ps = p1 - 1; ps >= p0; --ps) {
+      const int32_t r = uniform(rowmap[ps]);
+      const int32_t a = uniform(up[ps]), b = uniform(up[ps + 1]);
+      T acc[VEC] = {};
+      gather_wide<T, VEC, 4>(a, b, ucol, uval, Z, k, c0, valid, acc);
+      if (valid) {
+        const int64_t o = (int64_t)r * k + c0;
+        T yv[VEC], pv[VEC], dv[VEC];
+        VecIO<T, VEC>::ld(Z + o, yv);
+        VecIO<T, VEC>::ld(P + o, pv);
+        VecIO<T, VEC>::ld(D + o, dv);
+        const T rd = rdiag[ps];
 #pragma unroll
-      for (int m = 0; m < VEC; ++m) {
-        acc[m] = (yv[m] - acc[m]) * rd;
-        dv[m] = fma(al[m], pv[m], dv[m]);
-        pv[m] = fma(be[m], pv[m], ga[m] * acc[m]);
+        for (int m = 0; m < VEC; ++m) {
+          acc[m] = (yv[m] - acc[m]) * rd;
+          dv[m] = fma(al[m], pv[m], dv[m]);
+          pv[m] = fma(be[m], pv[m], ga[m] * acc[m]);
+        }
+        VecIO<T, VEC>::st(Z + o, acc);
+        VecIO<T, VEC>::st(P + o, pv);
+        VecIO<T, VEC>::st(D + o, dv);
       }
-      VecIO<T, VEC>::st(Z + o, acc);
-      VecIO<T, VEC>::st(P + o, pv);
-      VecIO<T, VEC>::st(D + o, dv);
     }
   }
 }
@@ -704,10 +820,13 @@ static SolverWork *ensure_work(cn_system_t *S, int32_t k, int32_t part_rows) {
 }
 
 // ================================================================ launches ==
+// blocks for a row kernel: multiples of 8 (one share per XCD) once there
+// are enough rows, capped
 static unsigned blocks_for_rows(int64_t rows, int rows_per_block, unsigned cap) {
   int64_t g = (rows + rows_per_block - 1) / rows_per_block;
   if (g < 1) g = 1;
   if (g > cap) g = cap;
+  if (g >= 64) g -= g % 8;
   return (unsigned)g;
 }
 
@@ -732,6 +851,25 @@ static Grid grid_rows(int64_t rows, int64_t k, unsigned cap) {
     G.g = dim3(blocks_for_rows(rows, kWaves, cap),
                (unsigned)((k + 64 * v - 1) / (64 * v)), 1);
   }
+  G.rows = (int32_t)G.g.x;
+  return G;
+}
+
+// Column-tile width for the IC(0) passes.  Multi-row blocks are walked
+// sequentially, so a level needs enough (block, tile) wavefronts to hide the
+// latency of that walk: narrow the tiles until there are ~4096 of them.
+template <typename T>
+static int sweep_vec(int64_t k, int64_t nblocks, bool multi) {
+  int v = wide_vec<T>(k);
+  if (!multi) return v;
+  while (v > 1 && nblocks * ((k + 64 * v - 1) / (64 * v)) < 4096) v /= 2;
+  return v;
+}
+
+template <typename T>
+static Grid grid_vec(int64_t rows, int64_t k, int v, unsigned cap) {
+  Grid G;
+  G.g = dim3(blocks_for_rows(rows, kWaves, cap), (unsigned)((k + 64 * v - 1) / (64 * v)), 1);
   G.rows = (int32_t)G.g.x;
   return G;
 }
@@ -769,7 +907,7 @@ struct SysView {
   const T *val;
   const double *val64;
   const cn_ic0_t *ic0 = nullptr;
-  const int32_t *rowmap, *lp, *lcol, *up, *ucol;
+  const int32_t *rowmap, *blk, *lp, *lcol, *up, *ucol;
   const T *lval, *uval, *rdiag;
   const T *dinv;
 };
@@ -817,20 +955,36 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
 #undef W_
     return G.rows;
   }
-  const std::vector<int32_t> &lv = V.ic0->level_ptr;
-  const int32_t nl = (int32_t)lv.size() - 1;
+  const std::vector<int32_t> &lb = V.ic0->level_blk;
+  const int32_t nl = (int32_t)lb.size() - 1;
+  const bool multi = V.ic0->max_block > 1;
   int32_t off = 0;
   ProfScope ps("fwd", s);
   for (int32_t c = 0; c < nl; ++c) {
-    int32_t r0 = lv[c], r1 = lv[c + 1];
-    if (r1 <= r0) continue;
-    Grid G = grid_rows<T>(r1 - r0, k, kCapLevel);
+    int32_t b0 = lb[c], b1 = lb[c + 1];
+    if (b1 <= b0) continue;
     double *pp = part + (int64_t)off * k;
-#define N_(KK) kn_fwd<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(r0, r1, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, Q, R, Z, pp, stride, ctl)
-#define W_(VV) kw_fwd<T, WideV<T, VV>::v, FUSED><<<G.g, kBlock, 0, s>>>(r0, r1, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, k, Q, R, Z, pp, stride, ctl)
-    DISPATCH_K(k, T, N_, W_);
+    if (k <= 4 && multi) {
+      dim3 g(blocks_for_rows(b1 - b0, kBlock, kCapLevel));
+#define N_(KK) kt_fwd<T, KK, FUSED><<<g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, Q, R, Z, pp, stride, ctl)
+      switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
 #undef N_
+      off += g.x;
+      continue;
+    }
+    if (k > 4) {
+      const int v = sweep_vec<T>(k, b1 - b0, multi);
+      Grid G = grid_vec<T>(b1 - b0, k, v, kCapLevel);
+#define W_(VV) kw_fwd<T, WideV<T, VV>::v, FUSED><<<G.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, k, Q, R, Z, pp, stride, ctl)
+      if (v == 4) { W_(4); } else if (v == 2) { W_(2); } else { W_(1); }
 #undef W_
+      off += G.rows;
+      continue;
+    }
+    Grid G = grid_rows<T>(b1 - b0, k, kCapLevel);
+#define N_(KK) kn_fwd<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, Q, R, Z, pp, stride, ctl)
+    switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
+#undef N_
     off += G.rows;
   }
   return off;
@@ -847,28 +1001,47 @@ static void launch_bwd(const SysView<T> &V, int64_t k, const double *st, T *Z, T
     k_pd_update<T><<<g, 256, 0, s>>>(total, k, st, Z, P, D, ctl);
     return;
   }
-  const std::vector<int32_t> &lv = V.ic0->level_ptr;
-  const int32_t nl = (int32_t)lv.size() - 1;
+  const std::vector<int32_t> &lb = V.ic0->level_blk;
+  const int32_t nl = (int32_t)lb.size() - 1;
+  const bool multi = V.ic0->max_block > 1;
   ProfScope ps("bwd", s);
   for (int32_t c = nl - 1; c >= 0; --c) {
-    int32_t r0 = lv[c], r1 = lv[c + 1];
-    if (r1 <= r0) continue;
-    Grid G = grid_rows<T>(r1 - r0, k, kCapLevel);
-#define N_(KK) kn_bwd<T, KK><<<G.g, kBlock, 0, s>>>(r0, r1, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
-#define W_(VV) kw_bwd<T, WideV<T, VV>::v><<<G.g, kBlock, 0, s>>>(r0, r1, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
-    DISPATCH_K(k, T, N_, W_);
+    int32_t b0 = lb[c], b1 = lb[c + 1];
+    if (b1 <= b0) continue;
+    if (k <= 4 && multi) {
+      dim3 g(blocks_for_rows(b1 - b0, kBlock, kCapLevel));
+#define N_(KK) kt_bwd<T, KK><<<g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
+      switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
 #undef N_
+      continue;
+    }
+    if (k > 4) {
+      const int v = sweep_vec<T>(k, b1 - b0, multi);
+      Grid G = grid_vec<T>(b1 - b0, k, v, kCapLevel);
+#define W_(VV) kw_bwd<T, WideV<T, VV>::v><<<G.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
+      if (v == 4) { W_(4); } else if (v == 2) { W_(2); } else { W_(1); }
 #undef W_
+      continue;
+    }
+    Grid G = grid_rows<T>(b1 - b0, k, kCapLevel);
+#define N_(KK) kn_bwd<T, KK><<<G.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
+    switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
+#undef N_
   }
 }
 
 template <typename T>
 static int32_t fwd_rows(const SysView<T> &V, int64_t k) {
   if (!V.ic0) return grid_rows<T>(V.n, k, kCapSpmm).rows;
-  const std::vector<int32_t> &lv = V.ic0->level_ptr;
+  const std::vector<int32_t> &lb = V.ic0->level_blk;
+  const bool multi = V.ic0->max_block > 1;
   int32_t off = 0;
-  for (size_t c = 0; c + 1 < lv.size(); ++c)
-    if (lv[c + 1] > lv[c]) off += grid_rows<T>(lv[c + 1] - lv[c], k, kCapLevel).rows;
+  for (size_t c = 0; c + 1 < lb.size(); ++c) {
+    int32_t nb = lb[c + 1] - lb[c];
+    if (nb <= 0) continue;
+    off += (k <= 4 && multi) ? (int32_t)blocks_for_rows(nb, kBlock, kCapLevel)
+                             : grid_rows<T>(nb, k, kCapLevel).rows;
+  }
   return off;
 }
 
@@ -923,6 +1096,7 @@ SysView<float> make_view<float>(cn_system_t *S, cn_ic0_t *F, SolverWork *W) {
   if (F) {
     V.ic0 = F;
     V.rowmap = F->perm.get();
+    V.blk = F->blk_ptr.get();
     V.lp = F->l_ptr.get(); V.lcol = F->l_col_nat.get(); V.lval = F->l_val32.get();
     V.up = F->u_ptr.get(); V.ucol = F->u_col_nat.get(); V.uval = F->u_val32.get();
     V.rdiag = F->l_rdiag32.get();
@@ -940,6 +1114,7 @@ SysView<double> make_view<double>(cn_system_t *S, cn_ic0_t *F, SolverWork *W) {
   if (F) {
     V.ic0 = F;
     V.rowmap = F->perm.get();
+    V.blk = F->blk_ptr.get();
     V.lp = F->l_ptr.get(); V.lcol = F->l_col_nat.get(); V.lval = F->l_val.get();
     V.up = F->u_ptr.get(); V.ucol = F->u_col_nat.get(); V.uval = F->u_val.get();
     V.rdiag = F->l_rdiag64.get();

@@ -302,12 +302,17 @@ class CsrSystem(_System):
 class IC0:
     """No-fill incomplete Cholesky factor of a system's A_ff."""
 
-    def __init__(self, system: _System, ordering: str = "color", allow_shift: bool = True):
+    def __init__(self, system: _System, ordering: str = "block", allow_shift: bool = True,
+                 block_size: int = 32):
         lib = L.load()
         self.system = system
         h = C.c_void_p(0)
-        order = {"color": L.CN_ORDER_COLOR, "natural": L.CN_ORDER_NATURAL}[ordering]
-        L.check(lib.cn_ic0_factor(system.handle, order, int(allow_shift), C.byref(h), _stream()),
+        o = L.Ic0Opts()
+        o.ordering = {"color": L.CN_ORDER_COLOR, "natural": L.CN_ORDER_NATURAL,
+                      "block": L.CN_ORDER_BLOCK}[ordering]
+        o.block_size = block_size
+        o.allow_shift = int(allow_shift)
+        L.check(lib.cn_ic0_factor_ex(system.handle, C.byref(o), C.byref(h), _stream()),
                 "ic0_factor")
         self._h = h
         info = L.Ic0Info()
