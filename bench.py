@@ -37,7 +37,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--ordering", default="block", choices=["block", "color", "natural"],
                     help="IC(0) ordering (block multicolour, point colour, natural levels)")
-    ap.add_argument("--block-size", type=int, default=32)
+    ap.add_argument("--block-size", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-poses", type=int, default=8, help="poses in the CPU baseline sample")
     ap.add_argument("--copy-mb", type=int, default=0,

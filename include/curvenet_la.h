@@ -188,7 +188,7 @@ enum {
 
 typedef struct cn_ic0_opts {
   int32_t ordering;     /* CN_ORDER_*                                  */
-  int32_t block_size;   /* CN_ORDER_BLOCK patch size (default 32)      */
+  int32_t block_size;   /* CN_ORDER_BLOCK patch size (default 16)      */
   int32_t allow_shift;  /* see cn_ic0_factor                           */
 } cn_ic0_opts;
 

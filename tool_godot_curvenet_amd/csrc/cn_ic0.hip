@@ -301,7 +301,7 @@ cn_status cn_ic0_factor(cn_system S, int32_t ordering, int32_t allow_shift,
                         cn_ic0 *out, void *stream) {
   cn_ic0_opts o;
   o.ordering = ordering;
-  o.block_size = 32;
+  o.block_size = 16;
   o.allow_shift = allow_shift;
   return cn_ic0_factor_ex(S, &o, out, stream);
 }

@@ -303,7 +303,7 @@ class IC0:
     """No-fill incomplete Cholesky factor of a system's A_ff."""
 
     def __init__(self, system: _System, ordering: str = "block", allow_shift: bool = True,
-                 block_size: int = 32):
+                 block_size: int = 16):
         lib = L.load()
         self.system = system
         h = C.c_void_p(0)
