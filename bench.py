@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--ordering", default="block", choices=["block", "color", "natural"],
                     help="IC(0) ordering (block multicolour, point colour, natural levels)")
     ap.add_argument("--block-size", type=int, default=16)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
+                    help="inner CG arithmetic (FP32 with FP64 updates, or FP64)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-poses", type=int, default=8, help="poses in the CPU baseline sample")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -89,7 +91,7 @@ def main():
     opts = _lib.PcgOpts()
     lib.cn_pcg_default_opts(opts)
     opts.precond = _lib.CN_PRECOND_IC0
-    opts.precision = _lib.CN_PREC_MIXED
+    opts.precision = _lib.CN_PREC_MIXED if args.precision == "mixed" else _lib.CN_PREC_FP64
     opts.tol_rel = args.tol
     stream = torch.cuda.current_stream().cuda_stream
     import ctypes as C
@@ -197,7 +199,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner / f64 refinement",
+        "dtype": "f32 inner / f64 refinement" if args.precision == "mixed" else "f64",
         "data": "synthetic (seeded capsule union, seeded constraint displacements)",
         "config": {"workload": "capsule union 12 capsules / 16 ring cuts, "
                                f"{mesh.nv} verts, n_free={n}, nnz={nnz}, "

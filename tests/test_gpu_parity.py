@@ -264,6 +264,51 @@ def test_pcg_cylinder(gpu, precision, precond, k):
     assert info.relres_max <= 1e-9
 
 
+@pytest.mark.parametrize("name", ["cylinder64", "icosphere6"])
+@pytest.mark.parametrize("ordering", ["block", "color", "natural"])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp64_plain", 1e-11),
+                                           ("mixed", 2e-5)])
+@pytest.mark.parametrize("k", [3, 100])
+def test_pcg_first_step_is_ic0_direction(gpu, name, ordering, precision, tol, k):
+    """From x0 = 0 one PCG step gives x1 = alpha0 M^-1 b, so x1 / |x1| must
+    be the oracle's IC(0) solve of b (on the permuted system, mapped back)
+    normalised: this pins the solver's triangular sweeps (every kernel form:
+    point and block levels, narrow and wide column tiles) to the oracle."""
+    from tool_godot_curvenet_amd import IC0
+    from tool_godot_curvenet_amd import meshgen as G
+    m, lap = _system(name)
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=40 + k))
+    ic0 = IC0(lap, ordering)
+    x, info = lap.solve(B, precond="ic0", precision=precision, tol=1e-9, ic0=ic0,
+                        max_iter=1)
+    assert info.iters == 1
+    X = x.cpu().numpy()
+    perm = ic0.perm()
+    Ap = ic0.permuted_matrix()
+    lp, lc, lv = O.ic0_factor(Ap.row_ptr, Ap.col_idx, Ap.values)
+    worst = 0.0
+    for c in range(k):
+        z = np.empty(lap.n_free)
+        z[perm] = O.ic0_solve(lp, lc, lv, B[perm, c])
+        e = np.linalg.norm(X[:, c] / np.linalg.norm(X[:, c]) - z / np.linalg.norm(z))
+        worst = max(worst, e)
+    assert worst <= tol, worst
+
+
+@pytest.mark.parametrize("ordering", ["block", "color"])
+@pytest.mark.parametrize("k", [7, 100, 300])
+def test_pcg_mixed_ragged_k(gpu, ordering, k):
+    """Column counts that leave the last column tile partly empty."""
+    from tool_godot_curvenet_amd import IC0
+    from tool_godot_curvenet_amd import meshgen as G
+    m, lap = _system("icosphere6")
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=50 + k))
+    x, info = lap.solve(B, precond="ic0", precision="mixed", tol=1e-9,
+                        ic0=IC0(lap, ordering))
+    assert info.status == 0
+    assert np.all(_relres(lap.A_ff(), x.cpu().numpy(), B) <= 1e-9)
+
+
 def test_pcg_solution_matches_oracle(gpu):
     """Positions within 1e-9 relative of the oracle's FP64 IC(0)-PCG."""
     from tool_godot_curvenet_amd import meshgen as G

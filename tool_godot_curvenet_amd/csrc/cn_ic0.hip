@@ -291,6 +291,116 @@ static int32_t block_levels(int32_t n, const std::vector<int32_t> &ptr,
   return greedy_color(nb, gptr, gcol, bcolor);
 }
 
+// ---- triangles prepared for the solver's block sweeps -------------------
+// position -> its block and the block's [first, end) positions
+__global__ void k_pos_block(int32_t nblk, const int32_t *__restrict__ blk_ptr,
+                            int32_t *__restrict__ pblk, int32_t *__restrict__ pb0,
+                            int32_t *__restrict__ pb1) {
+  int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  const int32_t p0 = blk_ptr[b], p1 = blk_ptr[b + 1];
+  for (int32_t p = p0; p < p1; ++p) {
+    pblk[p] = b;
+    pb0[p] = p0;
+    pb1[p] = p1;
+  }
+}
+
+// Columns are positions.  An L entry (column < row) is inside the row's
+// block when its column >= the block's first position; a U entry (column >
+// row) when its column < the block's end.
+__device__ __forceinline__ bool in_block(bool upper, int32_t q, int32_t p0, int32_t p1) {
+  return upper ? (q < p1) : (q >= p0);
+}
+
+// entries in other blocks beyond the fixed-width table
+__global__ void k_sweep_count(int32_t n, bool upper, const int32_t *__restrict__ tp,
+                              const int32_t *__restrict__ tc, const int32_t *__restrict__ pb0,
+                              const int32_t *__restrict__ pb1, int32_t *__restrict__ ovn) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t nx = 0;
+  for (int32_t e = tp[i]; e < tp[i + 1]; ++e)
+    if (!in_block(upper, tc[e], pb0[i], pb1[i])) ++nx;
+  ovn[i] = nx > cn_ic0_t::kSweepEll ? nx - cn_ic0_t::kSweepEll : 0;
+}
+
+__global__ void k_sweep_fill(int32_t n, bool upper, int32_t mb, const int32_t *__restrict__ tp,
+                             const int32_t *__restrict__ tc, const double *__restrict__ tv,
+                             const int32_t *__restrict__ perm, const int32_t *__restrict__ pblk,
+                             const int32_t *__restrict__ pb0, const int32_t *__restrict__ pb1,
+                             const double *__restrict__ rdiag, int32_t *__restrict__ ell_col,
+                             int32_t *__restrict__ ell_cnt, double *__restrict__ ell_val,
+                             float *__restrict__ ell_val32, const int32_t *__restrict__ ov_ptr,
+                             int32_t *__restrict__ ov_col, double *__restrict__ ov_val,
+                             float *__restrict__ ov_val32, double *__restrict__ dense,
+                             float *__restrict__ dense32) {
+  constexpr int W = cn_ic0_t::kSweepEll;
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t p0 = pb0[i], p1 = pb1[i], t = i - p0;
+  const int64_t base = ((int64_t)pblk[i] * mb + t) * mb;
+  int32_t nx = 0, ov = ov_ptr[i];
+  for (int32_t e = tp[i]; e < tp[i + 1]; ++e) {
+    const int32_t q = tc[e];
+    const double v = tv[e];
+    if (in_block(upper, q, p0, p1)) {
+      dense[base + (q - p0)] = v;
+      dense32[base + (q - p0)] = (float)v;
+    } else if (nx < W) {
+      ell_col[(int64_t)i * W + nx] = perm[q];
+      ell_val[(int64_t)i * W + nx] = v;
+      ell_val32[(int64_t)i * W + nx] = (float)v;
+      ++nx;
+    } else {
+      ov_col[ov] = perm[q];
+      ov_val[ov] = v;
+      ov_val32[ov] = (float)v;
+      ++ov;
+    }
+  }
+  ell_cnt[i] = nx;
+  const int32_t pad = nx > 0 ? ell_col[(int64_t)i * W + nx - 1] : 0;
+  for (int32_t u = nx; u < W; ++u) {
+    ell_col[(int64_t)i * W + u] = pad;
+    ell_val[(int64_t)i * W + u] = 0.0;
+    ell_val32[(int64_t)i * W + u] = 0.0f;
+  }
+  dense[base + t] = rdiag[i];
+  dense32[base + t] = (float)rdiag[i];
+}
+
+static void build_sweep(cn_ic0_t::Sweep &S, int32_t n, int32_t nblk, int32_t mb, bool upper,
+                        const int32_t *tp, const int32_t *tc, const double *tv,
+                        const int32_t *perm, const int32_t *pblk, const int32_t *pb0,
+                        const int32_t *pb1, const double *rdiag, hipStream_t s) {
+  constexpr int W = cn_ic0_t::kSweepEll;
+  S.ov_ptr.alloc(n + 1);
+  CN_HIP(hipMemsetAsync(S.ov_ptr.get(), 0, sizeof(int32_t) * (n + 1), s));
+  if (n > 0)
+    k_sweep_count<<<grid_for(n, 256), 256, 0, s>>>(n, upper, tp, tc, pb0, pb1,
+                                                  S.ov_ptr.get());
+  const int64_t nov = exclusive_scan_i32(S.ov_ptr.get(), (int64_t)n + 1, s);
+  S.ell_col.alloc((size_t)n * W + 1);
+  S.ell_cnt.alloc(n + 1);
+  S.ell_val.alloc((size_t)n * W + 1);
+  S.ell_val32.alloc((size_t)n * W + 1);
+  S.ov_col.alloc(nov + 1);
+  S.ov_val.alloc(nov + 1);
+  S.ov_val32.alloc(nov + 1);
+  const size_t nd = (size_t)nblk * mb * mb;
+  S.dense.alloc(nd + 1);
+  S.dense32.alloc(nd + 1);
+  CN_HIP(hipMemsetAsync(S.dense.get(), 0, sizeof(double) * (nd + 1), s));
+  CN_HIP(hipMemsetAsync(S.dense32.get(), 0, sizeof(float) * (nd + 1), s));
+  if (n > 0)
+    k_sweep_fill<<<grid_for(n, 256), 256, 0, s>>>(
+        n, upper, mb, tp, tc, tv, perm, pblk, pb0, pb1, rdiag, S.ell_col.get(),
+        S.ell_cnt.get(), S.ell_val.get(), S.ell_val32.get(), S.ov_ptr.get(), S.ov_col.get(),
+        S.ov_val.get(), S.ov_val32.get(), S.dense.get(), S.dense32.get());
+  CN_HIP(hipGetLastError());
+}
+
 }  // namespace cn
 
 using namespace cn;
@@ -502,6 +612,22 @@ cn_status cn_ic0_factor_ex(cn_system S, const cn_ic0_opts *opt, cn_ic0 *out,
       k_renumber<<<grid_for(nu_nz, 256), 256, 0, s>>>(nu_nz, F->perm.get(), F->u_col.get(),
                                                      F->u_col_nat.get());
     CN_HIP(hipGetLastError());
+    F->sweep_mb = 0;
+    if (F->max_block > 1 && F->max_block <= 32)
+      F->sweep_mb = F->max_block <= 8 ? 8 : (F->max_block <= 16 ? 16 : 32);
+    if (F->sweep_mb > 0) {
+      DevBuf<int32_t> pblk(n + 1), pb0(n + 1), pb1(n + 1);
+      if (nblk > 0)
+        k_pos_block<<<grid_for(nblk, 256), 256, 0, s>>>(nblk, F->blk_ptr.get(), pblk.get(),
+                                                       pb0.get(), pb1.get());
+      build_sweep(F->lsw, n, nblk, F->sweep_mb, false, F->l_ptr.get(), F->l_col.get(),
+                  F->l_val.get(), F->perm.get(), pblk.get(), pb0.get(), pb1.get(),
+                  F->l_rdiag64.get(), s);
+      build_sweep(F->usw, n, nblk, F->sweep_mb, true, F->u_ptr.get(), F->u_col.get(),
+                  F->u_val.get(), F->perm.get(), pblk.get(), pb0.get(), pb1.get(),
+                  F->l_rdiag64.get(), s);
+      CN_HIP(hipStreamSynchronize(s));
+    }
     CN_HIP(hipStreamSynchronize(s));
   } catch (...) {
     delete F;

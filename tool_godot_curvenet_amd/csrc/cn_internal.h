@@ -163,4 +163,21 @@ struct cn_ic0_t {
   // triangular passes walk rows through `perm`
   cn::DevBuf<int32_t> l_col_nat, u_col_nat;
   int64_t nnz_l = 0;
+  // L (forward) and U (backward) prepared for the solver's block sweeps.
+  // Row `ps` of a triangle is split into its entries in other blocks (final
+  // when the block's level runs) and its entries inside its own block:
+  //  - other blocks: the first kSweepEll in a fixed-width table indexed by
+  //    position (natural column; unused slots repeat the row's last entry
+  //    with weight 0), the rest in an overflow CSR;
+  //  - own block: a dense mb x mb tile per block, [t][u] = entry of block
+  //    row t in block column u, with 1 / L_tt on the diagonal.
+  // mb = 0 when there are no multi-row blocks or a block exceeds 32 rows.
+  static constexpr int kSweepEll = 4;
+  struct Sweep {
+    cn::DevBuf<int32_t> ell_col, ell_cnt, ov_ptr, ov_col;
+    cn::DevBuf<double> ell_val, ov_val, dense;
+    cn::DevBuf<float> ell_val32, ov_val32, dense32;
+  };
+  int32_t sweep_mb = 0;
+  Sweep lsw, usw;
 };

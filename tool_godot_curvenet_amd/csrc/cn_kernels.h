@@ -143,17 +143,18 @@ __device__ __forceinline__ void gather_narrow(int32_t a, int32_t b, int g,
 
 // Block reduction of per-lane FP64 partials for the wide mapping:
 // part[lane*VEC+m] of each wave summed over the 4 waves (fixed order).
+// `sh` holds kWaves * NP * 64 * VEC doubles of shared memory.
 template <int VEC, int NP>
-__device__ __forceinline__ void block_partials_wide(double (&p)[NP][VEC],
-                                                    double *__restrict__ out,
-                                                    int64_t k, int64_t c0,
-                                                    int64_t stride_np) {
-  __shared__ double sh[kWaves][NP][64 * VEC];
+__device__ __forceinline__ void block_partials_wide_in(double (&p)[NP][VEC],
+                                                       double *__restrict__ out,
+                                                       int64_t k, int64_t c0,
+                                                       int64_t stride_np, double *sh) {
+  constexpr int TW = 64 * VEC;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < NP; ++q)
 #pragma unroll
-    for (int m = 0; m < VEC; ++m) sh[w][q][lane * VEC + m] = p[q][m];
+    for (int m = 0; m < VEC; ++m) sh[(w * NP + q) * TW + lane * VEC + m] = p[q][m];
   __syncthreads();
   if (w == 0) {
 #pragma unroll
@@ -161,13 +162,23 @@ __device__ __forceinline__ void block_partials_wide(double (&p)[NP][VEC],
 #pragma unroll
       for (int m = 0; m < VEC; ++m) {
         int64_t c = c0 + m;
-        double s = sh[0][q][lane * VEC + m];
-        s += sh[1][q][lane * VEC + m];
-        s += sh[2][q][lane * VEC + m];
-        s += sh[3][q][lane * VEC + m];
+        const int o = q * TW + lane * VEC + m;
+        double s = sh[o];
+        s += sh[NP * TW + o];
+        s += sh[2 * NP * TW + o];
+        s += sh[3 * NP * TW + o];
         if (c < k) out[q * stride_np + (int64_t)blockIdx.x * k + c] = s;
       }
   }
+}
+
+template <int VEC, int NP>
+__device__ __forceinline__ void block_partials_wide(double (&p)[NP][VEC],
+                                                    double *__restrict__ out,
+                                                    int64_t k, int64_t c0,
+                                                    int64_t stride_np) {
+  __shared__ double sh[kWaves * NP * 64 * VEC];
+  block_partials_wide_in<VEC, NP>(p, out, k, c0, stride_np, sh);
 }
 
 // narrow: lane g < K of every 8-lane group holds the partials of column g;

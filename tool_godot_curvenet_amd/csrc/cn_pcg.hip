@@ -426,6 +426,289 @@ kw_bwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
   }
 }
 
+// ---- Block sweeps: one workgroup per (block, column tile) ---------------
+// A block's rows depend on earlier levels (final when the level starts) and
+// on earlier rows of the same block.  Phase 1 gathers the first kind for
+// the block's rows (wave w takes rows w, w+4, ..., up to 4 rows at once so
+// their loads are in flight together; the first kSweepEll entries of a row
+// come from a fixed-width table, so no row-pointer load precedes them);
+// phase 2 solves the dense in-block triangle, one thread per column, with
+// the column in registers and the triangle in scalar registers; phase 3
+// writes the rows back.
+template <typename T>
+struct SweepView {
+  const int32_t *ell_col, *ell_cnt, *ov_ptr, *ov_col;
+  const T *ell_val, *ov_val, *dense;
+};
+
+#define SWEEP_PARAMS(T)                                                        \
+  const int32_t *__restrict__ ell_col, const int32_t *__restrict__ ell_cnt,   \
+      const int32_t *__restrict__ ov_ptr, const int32_t *__restrict__ ov_col, \
+      const T *__restrict__ ell_val, const T *__restrict__ ov_val,            \
+      const T *__restrict__ dense
+#define SWEEP_ARGS(sw) sw.ell_col, sw.ell_cnt, sw.ov_ptr, sw.ov_col, sw.ell_val, sw.ov_val, sw.dense
+
+template <typename T, int VEC, int MB>
+struct BlockShared {
+  static constexpr int TW = 64 * VEC;
+  static constexpr size_t kLoc = sizeof(T) * MB * TW;
+  static constexpr size_t kPart = sizeof(double) * kWaves * 2 * TW;
+  static constexpr size_t kBytes = kLoc > kPart ? kLoc : kPart;
+};
+
+// acc[g] += sum over row ps[g]'s entries in other blocks of val * X[col]
+template <typename T, int VEC, int G>
+__device__ __forceinline__ void sweep_gather(const int32_t (&ps)[G], const bool (&on)[G],
+                                             const int32_t *__restrict__ ell_col,
+                                             const int32_t *__restrict__ ell_cnt,
+                                             const int32_t *__restrict__ ov_ptr,
+                                             const int32_t *__restrict__ ov_col,
+                                             const T *__restrict__ ell_val,
+                                             const T *__restrict__ ov_val, const T *X,
+                                             int64_t k, int64_t c0, bool valid,
+                                             T (&acc)[G][VEC]) {
+  constexpr int W = cn_ic0_t::kSweepEll;
+  int32_t cnt[G], oa[G], ob[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    cnt[g] = on[g] ? uniform(ell_cnt[ps[g]]) : 0;
+    oa[g] = uniform(ov_ptr[ps[g]]);
+    ob[g] = on[g] ? uniform(ov_ptr[ps[g] + 1]) : oa[g];
+  }
+  int32_t jc[G][W];
+  T vv[G][W];
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      jc[g][u] = uniform(ell_col[(int64_t)ps[g] * W + u]);
+      vv[g][u] = ell_val[(int64_t)ps[g] * W + u];
+    }
+  T x[G][W][VEC];
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+    if (cnt[g] > 0 && valid) {
+#pragma unroll
+      for (int u = 0; u < W; ++u) VecIO<T, VEC>::ld(X + (int64_t)jc[g][u] * k + c0, x[g][u]);
+    }
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+    if (cnt[g] > 0 && valid) {
+#pragma unroll
+      for (int u = 0; u < W; ++u)
+#pragma unroll
+        for (int m = 0; m < VEC; ++m) acc[g][m] = fma(vv[g][u], x[g][u][m], acc[g][m]);
+    }
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+    if (ob[g] > oa[g]) gather_wide<T, VEC, 4>(oa[g], ob[g], ov_col, ov_val, X, k, c0, valid, acc[g]);
+}
+
+// In-block triangle of block bq for the column `c` of the tile in `loc`
+// (rows [0, nb)): forward y_t = (s_t - sum_{u<t} L_tu y_u) / L_tt, or
+// backward z_t = (s_t - sum_{u>t} U_tu z_u) / L_tt.
+template <typename T, int MB, bool UPPER>
+__device__ __forceinline__ void block_solve(T *loc, int TW, int c, int32_t nb,
+                                            const T *__restrict__ tri) {
+  T y[MB];
+#pragma unroll
+  for (int t = 0; t < MB; ++t) y[t] = t < nb ? loc[t * TW + c] : T(0);
+  if (!UPPER) {
+#pragma unroll
+    for (int t = 0; t < MB; ++t) {
+      T v = y[t];
+#pragma unroll
+      for (int u = 0; u < t; ++u) v = fma(-tri[t * MB + u], y[u], v);
+      y[t] = v * tri[t * MB + t];
+    }
+  } else {
+#pragma unroll
+    for (int t = MB - 1; t >= 0; --t) {
+      T v = y[t];
+#pragma unroll
+      for (int u = t + 1; u < MB; ++u) v = fma(-tri[t * MB + u], y[u], v);
+      y[t] = v * tri[t * MB + t];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < MB; ++t)
+    if (t < nb) loc[t * TW + c] = y[t];
+}
+
+// forward: (FUSED: r -= alpha q) then y = L^-1 r over the blocks [b0, b1);
+// partials r.r and y.y
+template <typename T, int VEC, int MB, bool FUSED>
+__global__ void __launch_bounds__(kBlock)
+kb_fwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
+       const int32_t *__restrict__ rowmap, SWEEP_PARAMS(T),
+       const double *__restrict__ alpha, int64_t k, const T *__restrict__ Q,
+       T *__restrict__ R, T *Y, double *__restrict__ part, int64_t stride,
+       const int32_t *__restrict__ ctl) {
+  if (ctl[CTL_ACTIVE] == 0) return;
+  using SH = BlockShared<T, VEC, MB>;
+  constexpr int TW = SH::TW, RPW = MB / kWaves, G = RPW < 4 ? RPW : 4;
+  __shared__ __align__(16) unsigned char smem[SH::kBytes];
+  T *loc = reinterpret_cast<T *>(smem);
+  WIDE_COLS(VEC)
+  const int wave = threadIdx.x >> 6;
+  double pd[2][VEC] = {};
+  T al[VEC];
+#pragma unroll
+  for (int m = 0; m < VEC; ++m) al[m] = (FUSED && valid) ? (T)alpha[c0 + m] : T(0);
+  const RowIter ri = row_iter(b0, b1, 1, 0);
+  for (int32_t bq = ri.first; bq < ri.end; bq += ri.stride) {
+    const int32_t p0 = uniform(blk_ptr[bq]), nb = uniform(blk_ptr[bq + 1]) - p0;
+#pragma unroll 1
+    for (int g0 = 0; g0 < RPW; g0 += G) {
+      if (wave + kWaves * g0 >= nb) break;
+      int32_t ps[G], rows[G];
+      bool on[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int t = wave + kWaves * (g0 + g);
+        on[g] = t < nb;
+        ps[g] = p0 + (on[g] ? t : 0);
+        rows[g] = uniform(rowmap[ps[g]]);
+      }
+      T acc[G][VEC] = {};
+      sweep_gather<T, VEC, G>(ps, on, ell_col, ell_cnt, ov_ptr, ov_col, ell_val, ov_val, Y, k,
+                              c0, valid, acc);
+      T rv[G][VEC] = {}, qv[G][VEC] = {};
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (on[g] && valid) {
+          const int64_t o = (int64_t)rows[g] * k + c0;
+          VecIO<T, VEC>::ld(R + o, rv[g]);
+          if (FUSED) VecIO<T, VEC>::ld(Q + o, qv[g]);
+        }
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (on[g]) {
+          if (FUSED) {
+#pragma unroll
+            for (int m = 0; m < VEC; ++m) rv[g][m] = fma(-al[m], qv[g][m], rv[g][m]);
+            if (valid) VecIO<T, VEC>::st(R + (int64_t)rows[g] * k + c0, rv[g]);
+          }
+#pragma unroll
+          for (int m = 0; m < VEC; ++m) {
+            pd[0][m] += (double)rv[g][m] * (double)rv[g][m];
+            acc[g][m] = rv[g][m] - acc[g][m];
+          }
+          VecIO<T, VEC>::st(loc + (wave + kWaves * (g0 + g)) * TW + lane * VEC, acc[g]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < TW)
+      block_solve<T, MB, false>(loc, TW, threadIdx.x, nb, dense + (int64_t)bq * MB * MB);
+    __syncthreads();
+#pragma unroll 1
+    for (int t = wave; t < nb; t += kWaves) {
+      const int32_t r = uniform(rowmap[p0 + t]);
+      T y[VEC];
+      VecIO<T, VEC>::ld(loc + t * TW + lane * VEC, y);
+      if (valid) VecIO<T, VEC>::st(Y + (int64_t)r * k + c0, y);
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) pd[1][m] += (double)y[m] * (double)y[m];
+    }
+    __syncthreads();
+  }
+  block_partials_wide_in<VEC, 2>(pd, part, k, c0, stride, reinterpret_cast<double *>(smem));
+}
+
+// backward, in place on Z (holding y): z = L^-T y over the blocks [b0, b1);
+// d += alpha p;  p = gamma z + beta p
+template <typename T, int VEC, int MB>
+__global__ void __launch_bounds__(kBlock)
+kb_bwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
+       const int32_t *__restrict__ rowmap, SWEEP_PARAMS(T),
+       const double *__restrict__ st, int64_t k, T *Z, T *__restrict__ P,
+       T *__restrict__ D, const int32_t *__restrict__ ctl) {
+  if (bwd_idle(ctl)) return;
+  using SH = BlockShared<T, VEC, MB>;
+  constexpr int TW = SH::TW, RPW = MB / kWaves, G = RPW < 4 ? RPW : 4;
+  __shared__ __align__(16) unsigned char smem[SH::kBytes];
+  T *loc = reinterpret_cast<T *>(smem);
+  WIDE_COLS(VEC)
+  const int wave = threadIdx.x >> 6;
+  T al[VEC], be[VEC], ga[VEC];
+#pragma unroll
+  for (int m = 0; m < VEC; ++m) {
+    al[m] = valid ? (T)st[ST_ALPHA * k + c0 + m] : T(0);
+    be[m] = valid ? (T)st[ST_BETA * k + c0 + m] : T(0);
+    ga[m] = valid ? (T)st[ST_GAMMA * k + c0 + m] : T(0);
+  }
+  const RowIter ri = row_iter(b0, b1, 1, 0);
+  for (int32_t bq = ri.first; bq < ri.end; bq += ri.stride) {
+    const int32_t p0 = uniform(blk_ptr[bq]), nb = uniform(blk_ptr[bq + 1]) - p0;
+#pragma unroll 1
+    for (int g0 = 0; g0 < RPW; g0 += G) {
+      if (wave + kWaves * g0 >= nb) break;
+      int32_t ps[G], rows[G];
+      bool on[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int t = wave + kWaves * (g0 + g);
+        on[g] = t < nb;
+        ps[g] = p0 + (on[g] ? t : 0);
+        rows[g] = uniform(rowmap[ps[g]]);
+      }
+      T acc[G][VEC] = {};
+      sweep_gather<T, VEC, G>(ps, on, ell_col, ell_cnt, ov_ptr, ov_col, ell_val, ov_val, Z, k,
+                              c0, valid, acc);
+      T yv[G][VEC] = {};
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (on[g] && valid) VecIO<T, VEC>::ld(Z + (int64_t)rows[g] * k + c0, yv[g]);
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (on[g]) {
+#pragma unroll
+          for (int m = 0; m < VEC; ++m) acc[g][m] = yv[g][m] - acc[g][m];
+          VecIO<T, VEC>::st(loc + (wave + kWaves * (g0 + g)) * TW + lane * VEC, acc[g]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < TW)
+      block_solve<T, MB, true>(loc, TW, threadIdx.x, nb, dense + (int64_t)bq * MB * MB);
+    __syncthreads();
+#pragma unroll 1
+    for (int g0 = 0; g0 < RPW; g0 += G) {
+      if (wave + kWaves * g0 >= nb) break;
+      int32_t rows[G];
+      bool on[G];
+      T z[G][VEC] = {}, pv[G][VEC] = {}, dv[G][VEC] = {};
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int t = wave + kWaves * (g0 + g);
+        on[g] = t < nb;
+        rows[g] = uniform(rowmap[p0 + (on[g] ? t : 0)]);
+        if (on[g]) {
+          VecIO<T, VEC>::ld(loc + t * TW + lane * VEC, z[g]);
+          if (valid) {
+            const int64_t o = (int64_t)rows[g] * k + c0;
+            VecIO<T, VEC>::ld(P + o, pv[g]);
+            VecIO<T, VEC>::ld(D + o, dv[g]);
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (on[g] && valid) {
+          const int64_t o = (int64_t)rows[g] * k + c0;
+#pragma unroll
+          for (int m = 0; m < VEC; ++m) {
+            dv[g][m] = fma(al[m], pv[g][m], dv[g][m]);
+            pv[g][m] = fma(be[m], pv[g][m], ga[m] * z[g][m]);
+          }
+          VecIO<T, VEC>::st(Z + o, z[g]);
+          VecIO<T, VEC>::st(P + o, pv[g]);
+          VecIO<T, VEC>::st(D + o, dv[g]);
+        }
+    }
+    __syncthreads();
+  }
+}
+
 // ---- Jacobi (SparseLinAlg.applyJacobi, zero diagonal -> 0): z = r / d,
 //      (FUSED) r -= alpha q first; partials r.r and r.z -----------------------
 template <typename T, int K, bool FUSED>
@@ -841,6 +1124,22 @@ static int wide_vec(int64_t k) {
   return (k % 2 == 0) ? 2 : 1;
 }
 
+// Blocks of kBlock threads the device holds at once (8 per CU).  A row
+// sweep whose grid exceeds this runs its last column tiles as a second,
+// half-empty wave of blocks; capping the grid at one full wave avoids it.
+static unsigned resident_blocks() {
+  static unsigned cached = 0;
+  if (cached == 0) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    cached = (unsigned)cus * 8;
+  }
+  return cached;
+}
+
 template <typename T>
 static Grid grid_rows(int64_t rows, int64_t k, unsigned cap) {
   Grid G;
@@ -848,8 +1147,9 @@ static Grid grid_rows(int64_t rows, int64_t k, unsigned cap) {
     G.g = dim3(blocks_for_rows(rows, kBlock / kGroup, cap), 1, 1);
   } else {
     int v = wide_vec<T>(k);
-    G.g = dim3(blocks_for_rows(rows, kWaves, cap),
-               (unsigned)((k + 64 * v - 1) / (64 * v)), 1);
+    const unsigned tiles = (unsigned)((k + 64 * v - 1) / (64 * v));
+    const unsigned fit = std::max(8u, resident_blocks() / tiles);
+    G.g = dim3(blocks_for_rows(rows, kWaves, std::min(cap, fit)), tiles, 1);
   }
   G.rows = (int32_t)G.g.x;
   return G;
@@ -910,6 +1210,9 @@ struct SysView {
   const int32_t *rowmap, *blk, *lp, *lcol, *up, *ucol;
   const T *lval, *uval, *rdiag;
   const T *dinv;
+  // block sweeps (forward: L, backward: U); mb = 0 when not available
+  int32_t mb = 0;
+  SweepView<T> lsw{}, usw{};
 };
 
 template <typename T>
@@ -938,6 +1241,101 @@ static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
   return G.rows;
 }
 
+// How one level of an IC(0) sweep is launched.
+enum SweepKind { SW_NARROW, SW_THREAD, SW_WIDE, SW_BLOCK };
+struct LevelGrid {
+  SweepKind kind;
+  dim3 g;
+  int v;        // vector width (SW_WIDE, SW_BLOCK)
+  int32_t rows; // partial rows written by the forward pass
+};
+
+// Vector width of a block sweep: the widest that fits k without leaving the
+// tile mostly empty.
+template <typename T>
+static int block_vec(int64_t k) {
+  int v = wide_vec<T>(k);
+  while (v > 1 && 32 * v >= k) v /= 2;
+  return v;
+}
+
+// (vector width, dense block size) dispatch of the block sweeps
+#define KB_MB(v, MBV, CALL)                                                   \
+  do {                                                                        \
+    if ((v) == 4) { CALL(4, MBV); }                                           \
+    else if ((v) == 2) { CALL(2, MBV); }                                      \
+    else { CALL(1, MBV); }                                                    \
+  } while (0)
+#define KB_DISPATCH(v, mb, CALL)                                              \
+  do {                                                                        \
+    if ((mb) == 8) KB_MB(v, 8, CALL);                                         \
+    else if ((mb) == 16) KB_MB(v, 16, CALL);                                  \
+    else KB_MB(v, 32, CALL);                                                  \
+  } while (0)
+
+// Workgroups of one block-sweep kernel resident per CU (registers and
+// shared memory), cached per kernel.
+static unsigned wg_per_cu(const void *fn) {
+  static std::map<const void *, unsigned> cache;
+  auto it = cache.find(fn);
+  if (it != cache.end()) return it->second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kBlock, 0) != hipSuccess || nb <= 0)
+    nb = 1;
+  (void)hipGetLastError();
+  cache[fn] = (unsigned)nb;
+  return (unsigned)nb;
+}
+
+template <typename T>
+static unsigned kb_per_cu(int v, int mb, bool bwd) {
+  const void *fn = nullptr;
+#define F_(VV, MM)                                                                     \
+  fn = bwd ? reinterpret_cast<const void *>(&kb_bwd<T, WideV<T, VV>::v, MM>)         \
+           : reinterpret_cast<const void *>(&kb_fwd<T, WideV<T, VV>::v, MM, true>)
+  KB_DISPATCH(v, mb, F_);
+#undef F_
+  return wg_per_cu(fn);
+}
+
+template <typename T>
+static LevelGrid level_grid(const SysView<T> &V, int64_t k, int32_t nb, bool bwd) {
+  LevelGrid L{};
+  const bool multi = V.ic0->max_block > 1;
+  if (k <= 4) {
+    if (multi) {
+      L.kind = SW_THREAD;
+      L.g = dim3(blocks_for_rows(nb, kBlock, kCapLevel));
+    } else {
+      L.kind = SW_NARROW;
+      L.g = dim3(blocks_for_rows(nb, kBlock / kGroup, kCapLevel));
+    }
+    L.rows = (int32_t)L.g.x;
+    return L;
+  }
+  if (multi && V.mb > 0) {
+    const int v = block_vec<T>(k);
+    L.kind = SW_BLOCK;
+    L.v = v;
+    const unsigned tiles = (unsigned)((k + 64 * v - 1) / (64 * v));
+    const unsigned per_cu = kb_per_cu<T>(v, V.mb, bwd);
+    const unsigned fit = std::max(1u, (resident_blocks() / 8) * per_cu / tiles);
+    // every workgroup takes the same number of blocks
+    const int64_t iters = (nb + fit - 1) / fit;
+    int64_t x = (nb + iters - 1) / iters;
+    if (x >= 64) x = (x + 7) / 8 * 8;
+    L.g = dim3((unsigned)x, tiles, 1);
+    L.rows = (int32_t)x;
+    return L;
+  }
+  L.kind = SW_WIDE;
+  L.v = sweep_vec<T>(k, nb, multi);
+  Grid G = grid_vec<T>(nb, k, L.v, kCapLevel);
+  L.g = G.g;
+  L.rows = G.rows;
+  return L;
+}
+
 // First half of z = M^-1 r: (FUSED: r -= alpha q) then y = L^-1 r (IC(0))
 // or z = r / d (Jacobi); partials r.r -> part[0..rows) and r.M^-1.r ->
 // part[stride..].  Returns the partial rows written.
@@ -957,35 +1355,36 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
   }
   const std::vector<int32_t> &lb = V.ic0->level_blk;
   const int32_t nl = (int32_t)lb.size() - 1;
-  const bool multi = V.ic0->max_block > 1;
   int32_t off = 0;
   ProfScope ps("fwd", s);
   for (int32_t c = 0; c < nl; ++c) {
     int32_t b0 = lb[c], b1 = lb[c + 1];
     if (b1 <= b0) continue;
     double *pp = part + (int64_t)off * k;
-    if (k <= 4 && multi) {
-      dim3 g(blocks_for_rows(b1 - b0, kBlock, kCapLevel));
-#define N_(KK) kt_fwd<T, KK, FUSED><<<g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, Q, R, Z, pp, stride, ctl)
-      switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
+    const LevelGrid L = level_grid<T>(V, k, b1 - b0, false);
+    switch (L.kind) {
+      case SW_THREAD:
+#define N_(KK) kt_fwd<T, KK, FUSED><<<L.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, Q, R, Z, pp, stride, ctl)
+        switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
 #undef N_
-      off += g.x;
-      continue;
-    }
-    if (k > 4) {
-      const int v = sweep_vec<T>(k, b1 - b0, multi);
-      Grid G = grid_vec<T>(b1 - b0, k, v, kCapLevel);
-#define W_(VV) kw_fwd<T, WideV<T, VV>::v, FUSED><<<G.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, k, Q, R, Z, pp, stride, ctl)
-      if (v == 4) { W_(4); } else if (v == 2) { W_(2); } else { W_(1); }
+        break;
+      case SW_NARROW:
+#define N_(KK) kn_fwd<T, KK, FUSED><<<L.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, Q, R, Z, pp, stride, ctl)
+        switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
+#undef N_
+        break;
+      case SW_BLOCK:
+#define B_(VV, MM) kb_fwd<T, WideV<T, VV>::v, MM, FUSED><<<L.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, SWEEP_ARGS(V.lsw), alpha, k, Q, R, Z, pp, stride, ctl)
+        KB_DISPATCH(L.v, V.mb, B_);
+#undef B_
+        break;
+      case SW_WIDE:
+#define W_(VV) kw_fwd<T, WideV<T, VV>::v, FUSED><<<L.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, k, Q, R, Z, pp, stride, ctl)
+        if (L.v == 4) { W_(4); } else if (L.v == 2) { W_(2); } else { W_(1); }
 #undef W_
-      off += G.rows;
-      continue;
+        break;
     }
-    Grid G = grid_rows<T>(b1 - b0, k, kCapLevel);
-#define N_(KK) kn_fwd<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.lp, V.lcol, V.lval, V.rdiag, alpha, Q, R, Z, pp, stride, ctl)
-    switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
-#undef N_
-    off += G.rows;
+    off += L.rows;
   }
   return off;
 }
@@ -1003,44 +1402,46 @@ static void launch_bwd(const SysView<T> &V, int64_t k, const double *st, T *Z, T
   }
   const std::vector<int32_t> &lb = V.ic0->level_blk;
   const int32_t nl = (int32_t)lb.size() - 1;
-  const bool multi = V.ic0->max_block > 1;
   ProfScope ps("bwd", s);
   for (int32_t c = nl - 1; c >= 0; --c) {
     int32_t b0 = lb[c], b1 = lb[c + 1];
     if (b1 <= b0) continue;
-    if (k <= 4 && multi) {
-      dim3 g(blocks_for_rows(b1 - b0, kBlock, kCapLevel));
-#define N_(KK) kt_bwd<T, KK><<<g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
-      switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
+    const LevelGrid L = level_grid<T>(V, k, b1 - b0, true);
+    switch (L.kind) {
+      case SW_THREAD:
+#define N_(KK) kt_bwd<T, KK><<<L.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
+        switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
 #undef N_
-      continue;
-    }
-    if (k > 4) {
-      const int v = sweep_vec<T>(k, b1 - b0, multi);
-      Grid G = grid_vec<T>(b1 - b0, k, v, kCapLevel);
-#define W_(VV) kw_bwd<T, WideV<T, VV>::v><<<G.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
-      if (v == 4) { W_(4); } else if (v == 2) { W_(2); } else { W_(1); }
+        break;
+      case SW_NARROW:
+#define N_(KK) kn_bwd<T, KK><<<L.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
+        switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
+#undef N_
+        break;
+      case SW_BLOCK:
+#define B_(VV, MM) kb_bwd<T, WideV<T, VV>::v, MM><<<L.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, SWEEP_ARGS(V.usw), st, k, Z, P, D, ctl)
+        KB_DISPATCH(L.v, V.mb, B_);
+#undef B_
+        break;
+      case SW_WIDE:
+#define W_(VV) kw_bwd<T, WideV<T, VV>::v><<<L.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
+        if (L.v == 4) { W_(4); } else if (L.v == 2) { W_(2); } else { W_(1); }
 #undef W_
-      continue;
+        break;
     }
-    Grid G = grid_rows<T>(b1 - b0, k, kCapLevel);
-#define N_(KK) kn_bwd<T, KK><<<G.g, kBlock, 0, s>>>(b0, b1, V.blk, V.rowmap, V.up, V.ucol, V.uval, V.rdiag, st, k, Z, P, D, ctl)
-    switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
-#undef N_
   }
 }
 
+// Partial rows the forward pass writes (sizes the partials buffer).
 template <typename T>
 static int32_t fwd_rows(const SysView<T> &V, int64_t k) {
   if (!V.ic0) return grid_rows<T>(V.n, k, kCapSpmm).rows;
   const std::vector<int32_t> &lb = V.ic0->level_blk;
-  const bool multi = V.ic0->max_block > 1;
   int32_t off = 0;
   for (size_t c = 0; c + 1 < lb.size(); ++c) {
     int32_t nb = lb[c + 1] - lb[c];
     if (nb <= 0) continue;
-    off += (k <= 4 && multi) ? (int32_t)blocks_for_rows(nb, kBlock, kCapLevel)
-                             : grid_rows<T>(nb, k, kCapLevel).rows;
+    off += level_grid<T>(V, k, nb, false).rows;
   }
   return off;
 }
@@ -1100,6 +1501,11 @@ SysView<float> make_view<float>(cn_system_t *S, cn_ic0_t *F, SolverWork *W) {
     V.lp = F->l_ptr.get(); V.lcol = F->l_col_nat.get(); V.lval = F->l_val32.get();
     V.up = F->u_ptr.get(); V.ucol = F->u_col_nat.get(); V.uval = F->u_val32.get();
     V.rdiag = F->l_rdiag32.get();
+    V.mb = F->sweep_mb;
+    V.lsw = {F->lsw.ell_col.get(), F->lsw.ell_cnt.get(), F->lsw.ov_ptr.get(), F->lsw.ov_col.get(),
+             F->lsw.ell_val32.get(), F->lsw.ov_val32.get(), F->lsw.dense32.get()};
+    V.usw = {F->usw.ell_col.get(), F->usw.ell_cnt.get(), F->usw.ov_ptr.get(), F->usw.ov_col.get(),
+             F->usw.ell_val32.get(), F->usw.ov_val32.get(), F->usw.dense32.get()};
   }
   V.dinv = W->dinv32.get();
   return V;
@@ -1118,6 +1524,11 @@ SysView<double> make_view<double>(cn_system_t *S, cn_ic0_t *F, SolverWork *W) {
     V.lp = F->l_ptr.get(); V.lcol = F->l_col_nat.get(); V.lval = F->l_val.get();
     V.up = F->u_ptr.get(); V.ucol = F->u_col_nat.get(); V.uval = F->u_val.get();
     V.rdiag = F->l_rdiag64.get();
+    V.mb = F->sweep_mb;
+    V.lsw = {F->lsw.ell_col.get(), F->lsw.ell_cnt.get(), F->lsw.ov_ptr.get(), F->lsw.ov_col.get(),
+             F->lsw.ell_val.get(), F->lsw.ov_val.get(), F->lsw.dense.get()};
+    V.usw = {F->usw.ell_col.get(), F->usw.ell_cnt.get(), F->usw.ov_ptr.get(), F->usw.ov_col.get(),
+             F->usw.ell_val.get(), F->usw.ov_val.get(), F->usw.dense.get()};
   }
   V.dinv = W->dinv64.get();
   return V;
@@ -1133,6 +1544,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
   SysView<T> V0{};
   V0.n = n;
   V0.ic0 = F;
+  if (F) V0.mb = F->sweep_mb;
   int32_t need = std::max<int32_t>(fwd_rows<T>(V0, k), grid_rows<T>(n, k, kCapSpmm).rows);
   need = std::max<int32_t>(need, grid_rows<double>(n, k, kCapSpmm).rows);
   SolverWork *W = ensure_work(S, k, need);
@@ -1281,6 +1693,7 @@ static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
   SysView<double> V0{};
   V0.n = n;
   V0.ic0 = F;
+  if (F) V0.mb = F->sweep_mb;
   int32_t need = std::max<int32_t>(fwd_rows<double>(V0, k),
                                    grid_rows<double>(n, k, kCapSpmm).rows);
   SolverWork *W = ensure_work(S, k, need);
