@@ -26,6 +26,50 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+WORKLOAD = "capsules100k"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", f"traffic_{WORKLOAD}.json")
+# kernels whose HBM traffic is reported (name prefixes in the counter files)
+TRAFFIC_KERNELS = {
+    "spmm": "void cn::kw_spmm_dot<float, 4>",
+    "fwd": "void cn::kb_fwd<float, 4, 16, true>",
+    "bwd": "void cn::kb_bwd<float, 4, 16>",
+}
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def traffic_from(fetch_csv, write_csv, copy_mb, out=TRAFFIC_FILE):
+    """HBM bytes per dispatch from two `rocprofv3 --pmc` runs (FETCH_SIZE,
+    then WRITE_SIZE, KiB per dispatch), each scaled so that the plain
+    `copy_mb` MiB device copy reads and writes exactly copy_mb MiB."""
+    import csv
+
+    def load(path, counter):
+        per = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            per.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return per
+
+    f, w = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    copy_kib = copy_mb * 1024.0
+    cf = copy_kib / max(max(v) for k_, v in f.items() if k_.startswith(COPY_KERNEL))
+    cw = copy_kib / max(max(v) for k_, v in w.items() if k_.startswith(COPY_KERNEL))
+    res = {"workload": WORKLOAD, "copy_mib": copy_mb, "fetch_factor": cf, "write_factor": cw,
+           "kernels": {}}
+    for tag, prefix in TRAFFIC_KERNELS.items():
+        fv = [x for k_, v in f.items() if k_.startswith(prefix + "(") for x in v]
+        wv = [x for k_, v in w.items() if k_.startswith(prefix + "(") for x in v]
+        if not fv or not wv:
+            continue
+        fk, wk = sum(fv) / len(fv), sum(wv) / len(wv)
+        res["kernels"][tag] = {"kernel": prefix, "dispatches": len(fv),
+                               "fetch_kib_raw": fk, "write_kib_raw": wk,
+                               "fetch_bytes": fk * 1024 * cf, "write_bytes": wk * 1024 * cw,
+                               "bytes": fk * 1024 * cf + wk * 1024 * cw}
+    with open(out, "w") as fh:
+        json.dump(res, fh, indent=1)
+    print(json.dumps(res, indent=1))
 
 
 def main():
@@ -46,7 +90,13 @@ def main():
                     help="also run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--plain-poses", type=int, default=2,
                     help="poses solved by the plain FP64 one-solve-per-launch path")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="no GPU: turn two rocprofv3 --pmc counter files (FETCH_SIZE, "
+                         "WRITE_SIZE) of a --copy-mb run into the traffic file")
     args = ap.parse_args()
+    if args.traffic_from:
+        traffic_from(*args.traffic_from, copy_mb=args.copy_mb or 1024)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(sharding is described in DESIGN.md, not implemented)"}))
@@ -140,10 +190,34 @@ def main():
 
     solves = P * args.steps
     value = solves / elapsed
-    spmm_launches, spmm_ms = prof["spmm"]
-    spmm_avg_ms = spmm_ms / max(spmm_launches, 1)
-    spmm_bytes = 8 * nnz + 4 * (n + 1) + 8 * n * k
-    spmm_gbs = spmm_bytes / (spmm_avg_ms * 1e-3) / 1e9 if spmm_launches else 0.0
+    # ---- roofline of the three kernels of the inner loop (HIP events around
+    # every launch in the timed region).  Algorithmic bytes: the SpMM reads
+    # A (FP32 value + int32 column per nonzero, row pointers) and P and
+    # writes Q; a fused forward sweep reads R, Q and writes R, Y (4 n x k
+    # FP32 vectors), the plain one after each residual replacement reads R
+    # and writes Y (2); a backward sweep reads Y, P, D and writes Z, P, D
+    # (6).  Sweeps launch once per colour level; gathers of rows from other
+    # colours and the index tables are not counted.  Launches that find
+    # every column converged return at once and are given no bytes.
+    nk4 = 4 * n * k
+    nlev = ic0.info.nlevels
+    spmm_bytes = 8 * nnz + 4 * (n + 1) + 2 * nk4
+    kern = {
+        "spmm": ("kw_spmm_dot<float,4>", prof["spmm"][0], prof["spmm"][1],
+                 iters * spmm_bytes),
+        "fwd": ("kb_fwd<float,4,16>", prof["fwd"][0] * nlev, prof["fwd"][1],
+                (4 * iters + 2 * refines) * nk4),
+        "bwd": ("kb_bwd<float,4,16>", prof["bwd"][0] * nlev, prof["bwd"][1],
+                6 * (iters + refines) * nk4),
+    }
+    kernels = {}
+    for tag, (kname, launches, ms, nbytes) in kern.items():
+        gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        kernels[tag] = {"kernel": kname, "launches": launches, "ms": ms,
+                        "avg_ms": ms / max(launches, 1),
+                        "bytes_per_launch": nbytes / max(launches, 1),
+                        "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
+    dom = max(kernels, key=lambda t: kernels[t]["ms"])
 
     # --------------------------------------- plain FP64 one-solve-per-launch
     plain = None
@@ -188,6 +262,13 @@ def main():
                          f"order (factor outside the timing), {cit} iterations, {tce:.1f} s; "
                          f"{os.cpu_count()} host cores visible, 1 used"}
 
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            tk = json.load(fh).get("kernels", {}).get(dom)
+        if tk and k == 768 and args.ordering == "block" and args.block_size == 16:
+            traffic = tk["bytes"]
+
     line = {
         "metric": "solves/s at 1e-9 rel. residual (100k-vert cut Laplacian, 3 RHS)",
         "value": value,
@@ -206,10 +287,12 @@ def main():
                                f"{P} poses x 3 RHS (k={k}), IC(0) {args.ordering} order "
                                f"({ic0.info.nlevels} colours, {ic0.info.nblocks} blocks), "
                                f"tol {args.tol}"},
-        "roofline": {"bound": "hbm", "achieved": spmm_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": spmm_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "kw_spmm_dot<float,4>", "avg_ms": spmm_avg_ms,
-                     "launches": spmm_launches, "bytes_per_launch": spmm_bytes},
+        "roofline": {"bound": "hbm", "achieved": kernels[dom]["achieved_gbs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernels[dom]["frac"],
+                     "traffic": traffic, "kernel": kernels[dom]["kernel"],
+                     "avg_ms": kernels[dom]["avg_ms"], "launches": kernels[dom]["launches"],
+                     "bytes_per_launch": kernels[dom]["bytes_per_launch"]},
+        "kernels": kernels,
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
         "kernel_ms_per_step": {k_: v[1] / args.steps for k_, v in prof.items()},

@@ -45,8 +45,9 @@ enum {
   ST_BB, ST_RR, ST_TMP, ST_N
 };
 // control block
+// CTL_ITERS counts the iterations in which some column was still active.
 enum { CTL_ACTIVE = 0, CTL_BREAKDOWN = 1, CTL_ONE = 2, CTL_TRIGGER = 3, CTL_JUST = 4,
-       CTL_N = 8 };
+       CTL_ITERS = 5, CTL_N = 8 };
 // CTL_JUST counts columns stopped by this iteration's k_reduce_beta: the
 // backward pass that follows still has to apply their last d += alpha p.
 __device__ __forceinline__ bool bwd_idle(const int32_t *ctl) {
@@ -858,7 +859,10 @@ k_reduce_alpha(int32_t nrows, int64_t k, const double *__restrict__ part,
                int32_t *__restrict__ ctl) {
   const bool idle = ctl[CTL_ACTIVE] == 0;
   __syncthreads();  // every thread has read ctl before block 0 clears JUST
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[CTL_JUST] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl[CTL_JUST] = 0;
+    if (!idle) ctl[CTL_ITERS] += 1;
+  }
   if (idle) return;
   __shared__ double sh[4][64];
   const int64_t c = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
@@ -1583,7 +1587,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
   CN_HIP(hipGetLastError());
 
   std::vector<int32_t> act(k, 0), rst(k, 0);
-  int32_t iters = 0, updates = 0;
+  int32_t iters = 0, launched = 0, updates = 0;
   cn_status status = CN_OK;
   double relmax = 0.0;
   bool first = true;
@@ -1608,7 +1612,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
     }
     if (nact == 0) { status = CN_OK; break; }
     if (!std::isfinite(relmax)) { status = CN_BREAKDOWN; break; }
-    if (iters >= o->max_iter || updates >= o->max_refine * 8) { status = CN_MAXIT; break; }
+    if (launched >= o->max_iter || updates >= o->max_refine * 8) { status = CN_MAXIT; break; }
     // ---- per-column scale s = 1/||b|| (fixed for the solve)
     if (first) {
       for (int32_t c = 0; c < k; ++c) {
@@ -1632,7 +1636,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
     hc[CTL_ONE] = 1;
     hc[CTL_TRIGGER] = 0;
     hc[CTL_JUST] = 0;
-    CN_HIP(hipMemcpyAsync(ctl, hc, sizeof(int32_t) * 5, hipMemcpyHostToDevice, s));
+    hc[CTL_ITERS] = iters;
+    CN_HIP(hipMemcpyAsync(ctl, hc, sizeof(int32_t) * 6, hipMemcpyHostToDevice, s));
     CN_HIP(hipMemsetAsync(st + ST_ALPHA * k, 0, sizeof(double) * k, s));
     k_scale_in<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_S * k, R64, R);
     // z = M^-1 r; rz; continuing columns keep p, restarted ones take p = z
@@ -1644,7 +1649,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
     CN_HIP(hipGetLastError());
     // ---- CG iterations until every column stops or an update is due
     int32_t it = 0;
-    const int32_t budget = o->max_iter - iters;
+    const int32_t budget = o->max_iter - launched;
     const int32_t every = std::max(1, o->check_every);
     while (it < budget) {
       rows = launch_spmm_dot<T>(V, k, P, Q, p0, ctl, s);
@@ -1657,13 +1662,14 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
       launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
       ++it;
       if (it % every == 0 || it == budget) {
-        CN_HIP(hipMemcpyAsync(hc, ctl, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
+        CN_HIP(hipMemcpyAsync(hc, ctl, sizeof(int32_t) * 6, hipMemcpyDeviceToHost, s));
         CN_HIP(hipStreamSynchronize(s));
         if (hc[CTL_ACTIVE] == 0 || hc[CTL_TRIGGER]) break;
       }
     }
     CN_HIP(hipGetLastError());
-    iters += it;
+    launched += it;
+    iters = hc[CTL_ITERS];
     // ---- reliable update: x += d / s (FP64), r = b - A x (FP64)
     k_x_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_INVS * k, D, X);
     rows = launch_resid(n, V.ptr, V.col, V.val64, k, X, d_b, R64, p0, s);
