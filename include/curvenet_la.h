@@ -264,10 +264,31 @@ cn_status cn_pcg_solve(cn_system sys, cn_ic0 ic0, const cn_pcg_opts *opts,
                        int32_t k, const double *d_b, double *d_x,
                        cn_solve_info *info, void *stream);
 
+/* Adjoint of the harmonic solve A_ff x_f = -A_fc x_c.  No counterpart in
+ * the reference (the deform path HarmonicSolve.solveMulti,
+ * HarmonicSolve.lean:51-66, is only ever run forward); it is the backward
+ * pass a gradient-based caller (an autograd function) needs.  For the
+ * gradient g = dL/dx_f (n_free x k, FP64, device):
+ *   d_lambda = A_ff^-1 g, solved like cn_pcg_solve (same opts and factor,
+ *              always from x0 = 0);
+ *   d_gc     = dL/dx_c = -A_fc^T lambda (n_con x k), skipped if NULL;
+ *   d_gw     = dL/dw for every stored entry of A_ff (nnz_ff values) then of
+ *              A_fc (nnz_fc): w is the weight of the edge (i, j) the entry
+ *              couples (A_ii += w, A_jj += w, A_ij = A_ji -= w), so
+ *              dL/dw = -sum_cols (lambda_i - lambda_j)(x_i - x_j) with
+ *              lambda = 0 on constrained vertices; 0 on the diagonal.
+ *              Needs d_x (n_free x k) and d_xc (n_con x k); skipped if NULL.
+ * Returns the solve's status. */
+cn_status cn_adjoint_solve(cn_system sys, cn_ic0 ic0, const cn_pcg_opts *opts,
+                           int32_t k, const double *d_g, const double *d_x,
+                           const double *d_xc, double *d_lambda, double *d_gc,
+                           double *d_gw, cn_solve_info *info, void *stream);
+
 /* Per-kernel timing of the last cn_pcg_solve on this thread, gathered with
  * HIP events around every launch of the named kernel when enabled.
- * name: "spmm" (FP32 SpMM of the inner loop).  Returns launches timed;
- * *ms_total gets their summed duration. */
+ * name: "spmm" (FP32 SpMM of the inner loop), "fwd" / "bwd" (forward and
+ * backward preconditioner sweeps), "resid" (FP64 residual).  Returns the
+ * launches (sweeps) timed; *ms_total gets their summed duration. */
 void cn_profile_enable(int32_t on);
 int32_t cn_profile_query(const char *name, double *ms_total);
 

@@ -39,6 +39,9 @@ PINS = {
     "dense solve/LU/Cholesky": "DenseLinAlg.lean:294-489 KATs, numpy.linalg",
     "hsc_eliminate_vertex": "HSC KATs (HierarchicalSparsifyCompensate.lean:134-231), "
                             "dense Schur complement",
+    "adjoint": "central finite differences (h = 1e-6) of a quadratic loss through a dense "
+               "FP64 solve, in x_c and in every edge weight; planted sign / transpose "
+               "mistakes fail them",
 }
 
 
@@ -394,3 +397,43 @@ def hsc_degree_weight(edges, v):
     ev = _i([y for _, y, _ in e] or [0])
     ew = _d([z for _, _, z in e] or [0.0])
     return lib().orc_hsc_degree_weight(len(e), _p(eu), _p(ev), _p(ew), v)
+
+
+# ---------------------------------------------------------------- adjoint --
+def adjoint(ff, fc, g, x=None, xc=None, tol_rel=1e-14, factor=None):
+    """Adjoint of the harmonic solve A_ff x_f = -A_fc x_c (builder-defined;
+    the reference only runs HarmonicSolve.solveMulti forward,
+    HarmonicSolve.lean:51-66).  ff = (ptr, col, val) of A_ff; fc = (ptr,
+    col, val, n_con) of A_fc or None; g = dL/dx_f (n x k).
+      lam = A_ff^-1 g (FP64 PCG per column, IC(0) when `factor` is given);
+      gc  = -A_fc^T lam (n_con x k);
+      gw  = dL/dw per stored entry of A_ff then A_fc (needs x, xc), w the
+            weight of the edge the entry couples: -sum_cols (lam_i - lam_j)
+            (x_i - x_j), lam = 0 on constrained vertices, 0 on the diagonal.
+    Returns (lam, gc, gw)."""
+    ptr, col, val = (np.asarray(a) for a in ff)
+    g = np.asarray(g, dtype=np.float64).reshape(len(ptr) - 1, -1)
+    n, k = g.shape
+    lam = np.zeros_like(g)
+    for c in range(k):
+        nb = np.linalg.norm(g[:, c])
+        if nb > 0:
+            lam[:, c], _ = pcg(ptr, col, val, g[:, c], tol_rel * nb, factor=factor)
+    rows = np.repeat(np.arange(n), np.diff(ptr))
+    gc = None
+    if fc is not None:
+        fptr, fcol, fval, ncon = fc
+        frows = np.repeat(np.arange(n), np.diff(fptr))
+        gc = np.zeros((ncon, k))
+        np.add.at(gc, np.asarray(fcol), -np.asarray(fval)[:, None] * lam[frows])
+    gw = None
+    if x is not None:
+        x = np.asarray(x, dtype=np.float64).reshape(n, k)
+        d = -((lam[rows] - lam[col]) * (x[rows] - x[col])).sum(1)
+        d[rows == col] = 0.0
+        parts = [d]
+        if fc is not None:
+            xc = np.asarray(xc, dtype=np.float64).reshape(ncon, k)
+            parts.append(-(lam[frows] * (x[frows] - xc[np.asarray(fcol)])).sum(1))
+        gw = np.concatenate(parts)
+    return lam, gc, gw

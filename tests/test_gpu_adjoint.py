@@ -1,0 +1,109 @@
+"""GPU adjoint (cn_adjoint_solve) and its torch.autograd binding.
+
+Against the oracle adjoint (itself pinned by finite differences in
+test_oracle_adjoint.py) on the cylinder, by torch's gradcheck on a tiny cut
+mesh, and on the B-spline workload: a loss on the sphere's free vertices,
+differentiated in the control points of 8 closed cubic B-splines (16 control
+points each, 3 coordinates: 384 parameters), against central finite
+differences (h = 1e-6) in 32 random parameters."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _lap(mesh):
+    from tool_godot_curvenet_amd import CutLaplacian
+    return CutLaplacian.from_mesh(mesh)
+
+
+def _oracle_blocks(lap):
+    A = lap.A_ff()
+    F = lap.A_fc()
+    return (A.row_ptr, A.col_idx, A.values), (F.row_ptr, F.col_idx, F.values, lap.n_con)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_adjoint_matches_oracle(gpu, precision):
+    from tool_godot_curvenet_amd import meshgen as G
+    m = G.config_mesh("cylinder64")
+    lap = _lap(m)
+    k = 3
+    rng = np.random.default_rng(11)
+    xc = rng.uniform(-0.2, 0.2, size=(lap.n_con, k))
+    B = lap.harmonic_rhs(xc)
+    x, _ = lap.solve(B, precision="fp64", tol=1e-13)
+    X = x.cpu().numpy()
+    g = X - rng.normal(0, 0.05, size=X.shape)
+    lam, gc, gw, info = lap.adjoint(g, x=X, xc=xc, precision=precision, tol=1e-12,
+                                    weights=True)
+    assert info.status == 0
+    ff, fc = _oracle_blocks(lap)
+    lam_o, gc_o, gw_o = O.adjoint(ff, fc, g, x=X, xc=xc, tol_rel=1e-14)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(lam.cpu().numpy(), lam_o) <= 1e-10
+    assert rel(gc.cpu().numpy(), gc_o) <= 1e-10
+    assert rel(gw.cpu().numpy(), gw_o) <= 1e-10
+
+
+def test_adjoint_zero_gradient_column(gpu):
+    """A zero column of dL/dx_f gives lambda = 0 and g_c = 0 for it."""
+    from tool_godot_curvenet_amd import meshgen as G
+    lap = _lap(G.config_mesh("cylinder64"))
+    g = np.random.default_rng(2).normal(size=(lap.n_free, 2))
+    g[:, 1] = 0.0
+    lam, gc, _, info = lap.adjoint(g, tol=1e-10)
+    assert info.status == 0
+    assert np.all(lam.cpu().numpy()[:, 1] == 0) and np.all(gc.cpu().numpy()[:, 1] == 0)
+
+
+def test_autograd_gradcheck_tiny(gpu):
+    import torch
+    from tool_godot_curvenet_amd import meshgen as G
+    from tool_godot_curvenet_amd.autograd import harmonic_solve
+    lap = _lap(G.open_cylinder(8, 6, seed=3))
+    xc = torch.tensor(np.random.default_rng(1).uniform(-0.2, 0.2, (lap.n_con, 2)),
+                      dtype=torch.float64, device="cuda", requires_grad=True)
+    fn = lambda v: harmonic_solve(v, lap, tol=1e-13, precision="fp64")
+    assert torch.autograd.gradcheck(fn, (xc,), eps=1e-6, atol=1e-7, rtol=1e-6)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-11)])
+def test_bspline_control_gradient_matches_fd(gpu, precision, tol):
+    import torch
+    from tool_godot_curvenet_amd import IC0, curves, meshgen as G
+    from tool_godot_curvenet_amd.autograd import harmonic_solve
+    m = G.config_mesh("sphere45")
+    lap = _lap(m)
+    ic0 = IC0(lap)
+    Bm = torch.tensor(curves.constrained_basis(lap, m, 16), device="cuda")
+    rng = np.random.default_rng(G.BASE_SEED + 4)
+    delta = torch.tensor(rng.normal(0, 0.05, size=(lap.n_free, 3)), device="cuda")
+    ctrl0 = rng.normal(0, 0.1, size=(Bm.shape[1], 3))
+    ctrl = torch.tensor(ctrl0, device="cuda", requires_grad=True)
+    x = harmonic_solve(Bm @ ctrl, lap, ic0=ic0, tol=tol, precision=precision)
+    loss = 0.5 * ((x - delta) ** 2).sum()
+    loss.backward()
+    g_ad = ctrl.grad.cpu().numpy().ravel()
+
+    # central differences in 32 random parameters, all 64 solves in one batch
+    h = 1e-6
+    params = rng.choice(ctrl0.size, 32, replace=False)
+    cols = []
+    for p in params:
+        for s in (1.0, -1.0):
+            c = ctrl0.copy().ravel()
+            c[p] += s * h
+            cols.append(c.reshape(ctrl0.shape))
+    XC = torch.cat([Bm @ torch.tensor(c, device="cuda") for c in cols], dim=1)
+    b = lap.harmonic_rhs(XC, as_numpy=False)
+    xs, info = lap.solve(b, precision="fp64", tol=1e-13, ic0=ic0)
+    assert info.status == 0
+    xs = xs.reshape(lap.n_free, len(cols), 3)
+    losses = 0.5 * ((xs - delta[:, None, :]) ** 2).sum(dim=(0, 2)).cpu().numpy()
+    fd = (losses[0::2] - losses[1::2]) / (2 * h)
+    err = np.abs(g_ad[params] - fd)
+    scale = np.abs(g_ad).max()
+    assert np.all(err <= 1e-5 * np.maximum(np.abs(fd), 1e-2 * scale)), (err, fd)

@@ -5,7 +5,7 @@ Drop-in for the solver path of V-Sekai-fire/TOOL_godot_curvenet
 IncompleteCholesky, HarmonicSolve): hand-written HIP kernels behind the C
 ABI of ``include/curvenet_la.h``; this package is the Python host side.
 """
-from . import _lib
+from . import _lib, curves
 from .curvenet import (  # noqa: F401
     IC0,
     CutLaplacian,
@@ -19,8 +19,10 @@ from .curvenet import (  # noqa: F401
     spmm,
     spmv,
 )
+from .autograd import harmonic_solve as harmonic_solve_ad  # noqa: F401
 
 __all__ = [
     "IC0", "CutLaplacian", "CsrMatrix", "cg", "cg_with_guess", "cut_seams",
-    "from_triangles", "harmonic_solve", "project_to_vertices", "spmm", "spmv",
+    "from_triangles", "harmonic_solve", "harmonic_solve_ad", "project_to_vertices", "spmm",
+    "spmv", "curves",
 ]

@@ -88,6 +88,8 @@ _SIGS = {
     "cn_ic0_destroy": (i32, [vp]),
     "cn_pcg_default_opts": (None, [C.POINTER(PcgOpts)]),
     "cn_pcg_solve": (i32, [vp, vp, C.POINTER(PcgOpts), i32, vp, vp, C.POINTER(SolveInfo), vp]),
+    "cn_adjoint_solve": (i32, [vp, vp, C.POINTER(PcgOpts), i32, vp, vp, vp, vp, vp, vp,
+                               C.POINTER(SolveInfo), vp]),
     "cn_profile_enable": (None, [i32]),
     "cn_profile_query": (i32, [C.c_char_p, C.POINTER(f64)]),
 }

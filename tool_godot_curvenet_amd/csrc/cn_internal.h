@@ -131,6 +131,11 @@ struct cn_system_t {
   cn::DevBuf<double> ff_val, fc_val;
   cn::DevBuf<float> ff_val32, fc_val32;
   cn::DevBuf<double> diag;          // diagonal of A_ff
+  // A_fc transposed (rows = constrained columns), built on first use: the
+  // free row of each entry and its index in fc_col / fc_val, rows sorted by
+  // free row, so the values are read from fc_val and never go stale
+  cn::DevBuf<int32_t> cf_ptr, cf_row, cf_map;
+  bool cf_ready = false;
   cn::SolverWork *work = nullptr;   // solver workspace, grown on demand
   ~cn_system_t() { cn::free_solver_work(work); }
 };

@@ -197,6 +197,17 @@ class _System:
         return CsrMatrix(n, n, _np(i.d_ff_ptr, n + 1, np.int32), _np(i.d_ff_col, i.nnz_ff, np.int32),
                          _np(i.d_ff_val, i.nnz_ff, np.float64))
 
+    @staticmethod
+    def _opts(precond, precision, tol, max_iter, inner_tol, check_every, max_refine, warm):
+        o = L.PcgOpts()
+        L.load().cn_pcg_default_opts(C.byref(o))
+        o.precond = {"ic0": L.CN_PRECOND_IC0, "jacobi": L.CN_PRECOND_JACOBI}[precond]
+        o.precision = {"mixed": L.CN_PREC_MIXED, "fp64": L.CN_PREC_FP64,
+                       "fp64_plain": L.CN_PREC_FP64_PLAIN}[precision]
+        o.tol_rel, o.max_iter, o.inner_tol = tol, max_iter, inner_tol
+        o.check_every, o.max_refine, o.warm_start = check_every, max_refine, int(warm)
+        return o
+
     def solve(self, b, precond: str = "ic0", precision: str = "mixed", tol: float = 1e-9,
               x0=None, ic0: "IC0 | None" = None, max_iter: int = 20000,
               inner_tol: float = 1e-5, check_every: int = 8, max_refine: int = 30):
@@ -209,13 +220,8 @@ class _System:
         k = bt.shape[1]
         x = (_as_tensor(x0, torch.float64).reshape(self.n_free, k).clone()
              if x0 is not None else torch.zeros_like(bt))
-        o = L.PcgOpts()
-        lib.cn_pcg_default_opts(C.byref(o))
-        o.precond = {"ic0": L.CN_PRECOND_IC0, "jacobi": L.CN_PRECOND_JACOBI}[precond]
-        o.precision = {"mixed": L.CN_PREC_MIXED, "fp64": L.CN_PREC_FP64,
-                       "fp64_plain": L.CN_PREC_FP64_PLAIN}[precision]
-        o.tol_rel, o.max_iter, o.inner_tol = tol, max_iter, inner_tol
-        o.check_every, o.max_refine, o.warm_start = check_every, max_refine, int(x0 is not None)
+        o = self._opts(precond, precision, tol, max_iter, inner_tol, check_every, max_refine,
+                       x0 is not None)
         if precond == "ic0" and ic0 is None:
             ic0 = IC0(self)
         info = L.SolveInfo()
@@ -223,6 +229,45 @@ class _System:
                               C.byref(o), k, _p(bt), _p(x), C.byref(info), _stream())
         L.check(st, "pcg_solve", ok=(L.CN_OK, L.CN_MAXIT, L.CN_BREAKDOWN))
         return (x.reshape(-1) if vec else x), info
+
+    def adjoint(self, g, x=None, xc=None, precond: str = "ic0", precision: str = "mixed",
+                tol: float = 1e-9, ic0: "IC0 | None" = None, weights: bool = False,
+                max_iter: int = 20000, inner_tol: float = 1e-5, check_every: int = 8,
+                max_refine: int = 30):
+        """Adjoint of the harmonic solve (cn_adjoint_solve) for g = dL/dx_f
+        (n_free x k).  Returns (lambda, g_c, g_w, SolveInfo) as device
+        tensors: lambda = A_ff^-1 g, g_c = -A_fc^T lambda (None without
+        constrained columns), g_w = dL/dw per stored entry of A_ff then A_fc
+        (only with weights=True, which needs the forward x and x_c)."""
+        torch = _torch()
+        lib = L.load()
+        gt = _as_tensor(g, torch.float64)
+        vec = gt.dim() == 1
+        gt = gt.reshape(self.n_free, -1).contiguous()
+        k = gt.shape[1]
+        i = self.info
+        lam = torch.empty_like(gt)
+        gc = (torch.empty((i.n_con, k), dtype=torch.float64, device="cuda")
+              if i.n_con > 0 else None)
+        gw = xt = xct = None
+        if weights:
+            xt = _as_tensor(x, torch.float64).reshape(self.n_free, k).contiguous()
+            if i.n_con > 0:
+                xct = _as_tensor(xc, torch.float64).reshape(i.n_con, k).contiguous()
+            gw = torch.empty(i.nnz_ff + i.nnz_fc, dtype=torch.float64, device="cuda")
+        o = self._opts(precond, precision, tol, max_iter, inner_tol, check_every, max_refine,
+                       False)
+        if precond == "ic0" and ic0 is None:
+            ic0 = IC0(self)
+        info = L.SolveInfo()
+        st = lib.cn_adjoint_solve(self._h, ic0.handle if ic0 is not None else C.c_void_p(0),
+                                  C.byref(o), k, _p(gt), _p(xt), _p(xct), _p(lam), _p(gc),
+                                  _p(gw), C.byref(info), _stream())
+        L.check(st, "adjoint_solve", ok=(L.CN_OK, L.CN_MAXIT, L.CN_BREAKDOWN))
+        if vec:
+            lam = lam.reshape(-1)
+            gc = gc.reshape(-1) if gc is not None else None
+        return lam, gc, gw, info
 
 
 class CutLaplacian(_System):

@@ -238,6 +238,7 @@ def tilted_loops(mesh: Mesh, n_loops: int = 8, max_tilt_deg: float = 30.0,
     cut = np.unique(np.sort(np.concatenate(edges), axis=1), axis=0).astype(np.int32)
     meta = dict(mesh.meta)
     meta["loops"] = [len(x) for x in loops]
+    meta["loop_verts"] = [list(map(int, x)) for x in loops]
     return Mesh(mesh.pos, mesh.tris, cut, mesh.name + "_loops", meta)
 
 
@@ -391,6 +392,12 @@ def config_mesh(name: str) -> Mesh:
         return tilted_loops(m, 8, 30.0, seed=BASE_SEED + 1)
     if name == "capsules100k":
         return capsule_union(12, 16, 100_000, seed=BASE_SEED + 3)
+    if name in ("sphere45", "sphere224"):
+        # geodesic sphere (V = 10 nu^2 + 2) with 8 tilted latitude loops, the
+        # curves of the B-spline gradient workload
+        nu = 45 if name == "sphere45" else 224
+        m = geodesic_sphere(nu, jitter_frac=0.05, seed=BASE_SEED + 4)
+        return tilted_loops(m, 8, 30.0, seed=BASE_SEED + 5)
     raise KeyError(name)
 
 
