@@ -220,10 +220,89 @@ cn_status cn_ic0_apply(cn_ic0 f, int32_t k, const double *d_r, double *d_z,
 cn_status cn_ic0_destroy(cn_ic0 f);
 
 /* ------------------------------------------------------------------------
+ * Dense FP64 Cholesky on the matrix cores (DenseLinAlg.choleskyFactor /
+ * forwardSolve / backwardSolve / solveWithCholesky, DenseLinAlg.lean:
+ * 255-286).  Row-major, device memory.
+ * ---------------------------------------------------------------------- */
+
+/* In place: the lower triangle of A (n x n, leading dimension lda) becomes L
+ * with A = L L^T, the strict upper triangle is zeroed (the spec returns L
+ * with a zero upper triangle).  A non-positive pivot gives CN_NOT_SPD (the
+ * spec produces NaN there). */
+cn_status cn_dense_cholesky_f64(int32_t n, double *d_A, int32_t lda, void *stream);
+
+/* B (n x k, leading dimension ldb) <- (L L^T)^-1 B for the factor of
+ * cn_dense_cholesky_f64 (solveWithCholesky, every column at once). */
+cn_status cn_dense_cholesky_solve_f64(int32_t n, int32_t k, const double *d_L,
+                                      int32_t lda, double *d_B, int32_t ldb,
+                                      void *stream);
+
+/* ------------------------------------------------------------------------
+ * Hierarchical Schur preconditioner (HSC, after Krishnan-Fattal-Szeliski
+ * 2013; the per-vertex elimination rule is HSC.eliminateVertex,
+ * HierarchicalSparsifyCompensate.lean:96-125, the coarse solve the dense
+ * Cholesky above).  Level l eliminates a greedy independent set F (vertices
+ * by ascending off-diagonal count, then index): the remaining C vertices get
+ * the exact Schur complement S = A_CC - A_CF D_F^-1 A_FC; fill entries of S
+ * with |s_ij| < theta sqrt(s_ii s_jj) are dropped and compensated on the
+ * diagonals.  Levels stop at coarse_max rows; the coarse matrix is factored
+ * and inverted on the matrix cores.  Apply: r_C += R r_F level by level
+ * down, z_c = A_c^-1 r_c, z_F = D_F^-1 r_F + P z_C back up.
+ * ---------------------------------------------------------------------- */
+typedef struct cn_hsc_t *cn_hsc;
+
+enum {
+  CN_HSC_LUMP = 0,  /* a dropped s_ij is added to s_ii and s_jj: row sums
+                       (the action on constants) are kept */
+  CN_HSC_DIAG = 1   /* |s_ij| is added: S only grows, always definite */
+};
+
+typedef struct cn_hsc_opts {
+  double theta;         /* drop threshold (default 0.02)                     */
+  int32_t coarse_max;   /* stop at this many rows (default 512)              */
+  int32_t max_levels;   /* default 24                                         */
+  double min_frac;      /* stop when |F| < min_frac n (default 0.05)          */
+  int32_t compensate;   /* CN_HSC_LUMP (default) or CN_HSC_DIAG              */
+} cn_hsc_opts;
+
+typedef struct cn_hsc_info {
+  int32_t nlevels;      /* elimination levels                                */
+  int32_t n_coarse;     /* rows of the coarse matrix                         */
+  int64_t nnz;          /* stored R + P entries over all levels              */
+  const double *d_coarse_L;    /* coarse Cholesky factor (n_coarse^2, lower) */
+  const double *d_coarse_inv;  /* its inverse (n_coarse^2, symmetric)        */
+} cn_hsc_info;
+
+/* Level l of the hierarchy (device pointers, level-local row numbering):
+ * F (nF rows eliminated, ascending), C (nC kept, ascending; C[c] is row c of
+ * level l + 1), dinv = 1 / a_ff, R (nC rows; columns are level-l rows of F
+ * vertices, values -a_cf / a_ff) and P (nF rows; columns are level-(l + 1)
+ * rows, values -a_fc / a_ff), all CSR with ascending columns. */
+typedef struct cn_hsc_level {
+  int32_t n, nF, nC;
+  const int32_t *d_F, *d_C;
+  const double *d_dinv;
+  const int32_t *d_r_ptr, *d_r_col;
+  const double *d_r_val;
+  const int32_t *d_p_ptr, *d_p_col;
+  const double *d_p_val;
+} cn_hsc_level;
+
+void cn_hsc_default_opts(cn_hsc_opts *opts);
+cn_status cn_hsc_build(cn_system sys, const cn_hsc_opts *opts, cn_hsc *out,
+                       void *stream);
+cn_status cn_hsc_get_info(cn_hsc h, cn_hsc_info *info);
+cn_status cn_hsc_get_level(cn_hsc h, int32_t level, cn_hsc_level *out);
+/* z = M^-1 r (FP64), r and z n_free x k, distinct. */
+cn_status cn_hsc_apply(cn_hsc h, int32_t k, const double *d_r, double *d_z,
+                       void *stream);
+cn_status cn_hsc_destroy(cn_hsc h);
+
+/* ------------------------------------------------------------------------
  * Preconditioned conjugate gradients (SparseLinAlg.cg / cgWithGuess,
  * SparseLinAlg.lean:104-163) on A_ff, k right-hand sides in lockstep.
  * ---------------------------------------------------------------------- */
-enum { CN_PRECOND_JACOBI = 0, CN_PRECOND_IC0 = 1 };
+enum { CN_PRECOND_JACOBI = 0, CN_PRECOND_IC0 = 1, CN_PRECOND_HSC = 2 };
 enum {
   CN_PREC_FP64_PLAIN = 0, /* FP64; one kernel per BLAS-1/SpMV step, alpha/beta
                              and the stop test on the host (3 host round
@@ -263,6 +342,12 @@ void cn_pcg_default_opts(cn_pcg_opts *opts);
 cn_status cn_pcg_solve(cn_system sys, cn_ic0 ic0, const cn_pcg_opts *opts,
                        int32_t k, const double *d_b, double *d_x,
                        cn_solve_info *info, void *stream);
+
+/* The same solve preconditioned by an HSC hierarchy of this system
+ * (opts->precond is ignored). */
+cn_status cn_pcg_solve_hsc(cn_system sys, cn_hsc hsc, const cn_pcg_opts *opts,
+                           int32_t k, const double *d_b, double *d_x,
+                           cn_solve_info *info, void *stream);
 
 /* Adjoint of the harmonic solve A_ff x_f = -A_fc x_c.  No counterpart in
  * the reference (the deform path HarmonicSolve.solveMulti,

@@ -9,6 +9,7 @@ tests/test_oracle_pins.py or tests/test_oracle_kat.py.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import pathlib
 import subprocess
 
@@ -437,3 +438,143 @@ def adjoint(ff, fc, g, x=None, xc=None, tol_rel=1e-14, factor=None):
             parts.append(-(lam[frows] * (x[frows] - xc[np.asarray(fcol)])).sum(1))
         gw = np.concatenate(parts)
     return lam, gc, gw
+
+
+# -------------------------------------------------------------------- HSC --
+def hsc_build(ptr, col, val, theta=0.02, coarse_max=512, max_levels=24, min_frac=0.05,
+              comp="lump"):
+    """Hierarchical Schur elimination with sparsification (after
+    Krishnan-Fattal-Szeliski 2013; the per-vertex rule is
+    HierarchicalSparsifyCompensate.eliminateVertex,
+    HierarchicalSparsifyCompensate.lean:96-125, generalised from a graph
+    Laplacian to an SPD matrix: the pivot is the diagonal, not the degree).
+
+    Level l, matrix A (CSR, sorted columns):
+      1. F = greedy independent set, vertices visited by (off-diagonal count,
+         index) ascending; C = the rest.  Stop (A is the coarse matrix) when
+         n <= coarse_max, at max_levels, or when |F| < min_frac n.
+      2. S = A_CC - A_CF D_F^-1 A_FC exactly: starting from A_CC, for every
+         f in F ascending, for every pair (c, c') of f's neighbours
+         (ascending, c' over all of them), S[c][c'] -= a_cf a_fc' / a_ff.
+      3. Off-diagonal entries of S outside the pattern of A_CC (fill) with
+         |s_ij| < theta sqrt(s_ii s_jj) (tested on S before any change) are
+         dropped; each adds |s_ij| to s_ii (rows ascending, columns
+         ascending): S + |s|(e_i e_i^T + e_j e_j^T) - s(e_i e_j^T + e_j e_i^T)
+         >= S, so every level stays SPD.
+    `comp` = "lump" adds s_ij itself instead (s (e_i - e_j)(e_i - e_j)^T:
+    row sums, hence the action on constants, are kept; far fewer iterations
+    on Laplacians, definite while theta is small).  Drops are applied in
+    (i, j) ascending order.
+    Returns a dict of levels (F, C, dinvF, R, P as CSR) and the dense coarse
+    matrix.  Indices are level-local; C keeps ascending order."""
+    ptr = np.asarray(ptr, np.int64)
+    col = np.asarray(col, np.int64)
+    val = np.asarray(val, np.float64)
+    rows = [dict(zip(col[ptr[i]:ptr[i + 1]].tolist(), val[ptr[i]:ptr[i + 1]].tolist()))
+            for i in range(len(ptr) - 1)]
+    levels = []
+    while True:
+        n = len(rows)
+        if n <= coarse_max or len(levels) >= max_levels:
+            break
+        deg = [len(r) - (1 if i in r else 0) for i, r in enumerate(rows)]
+        order = sorted(range(n), key=lambda i: (deg[i], i))
+        isF = [False] * n
+        blocked = [False] * n
+        for v in order:
+            if not blocked[v]:
+                isF[v] = True
+                blocked[v] = True
+                for u in rows[v]:
+                    blocked[u] = True
+        F = [i for i in range(n) if isF[i]]
+        if len(F) < min_frac * n:
+            break
+        C = [i for i in range(n) if not isF[i]]
+        cidx = {c: t for t, c in enumerate(C)}
+        fidx = {f: t for t, f in enumerate(F)}
+        S = [{cidx[j]: a for j, a in rows[c].items() if not isF[j]} for c in C]
+        orig = [set(r.keys()) for r in S]
+        for f in F:
+            d = rows[f][f]
+            nb = sorted(j for j in rows[f] if j != f)
+            for c in nb:
+                acf = rows[c][f]
+                rc = S[cidx[c]]
+                for c2 in nb:
+                    t = cidx[c2]
+                    rc[t] = rc.get(t, 0.0) - acf * rows[f][c2] / d
+        drop = []
+        for i, r in enumerate(S):
+            for j in sorted(r):
+                s = r[j]
+                if j > i and j not in orig[i] and abs(s) < theta * math.sqrt(r[i] * S[j][j]):
+                    drop.append((i, j))
+        for i, j in drop:
+            s = S[i][j]
+            del S[i][j]
+            del S[j][i]
+            w = abs(s) if comp == "diag" else s
+            S[i][i] += w
+            S[j][j] += w
+        dinv = np.array([1.0 / rows[f][f] for f in F])
+        Rp, Rc, Rv = [0], [], []
+        for c in C:
+            ent = sorted((fidx[j], -a / rows[j][j]) for j, a in rows[c].items() if isF[j])
+            Rc += [e[0] for e in ent]
+            Rv += [e[1] for e in ent]
+            Rp.append(len(Rc))
+        Pp, Pc, Pv = [0], [], []
+        for f in F:
+            ent = sorted((cidx[j], -a / rows[f][f]) for j, a in rows[f].items() if j != f)
+            Pc += [e[0] for e in ent]
+            Pv += [e[1] for e in ent]
+            Pp.append(len(Pc))
+        levels.append({"n": n, "F": np.array(F, np.int32), "C": np.array(C, np.int32),
+                       "dinv": dinv,
+                       "R": (np.array(Rp, np.int32), np.array(Rc, np.int32), np.array(Rv)),
+                       "P": (np.array(Pp, np.int32), np.array(Pc, np.int32), np.array(Pv))})
+        rows = [dict(sorted(r.items())) for r in S]
+    n = len(rows)
+    dense = np.zeros((n, n))
+    for i, r in enumerate(rows):
+        for j, a in r.items():
+            dense[i, j] = a
+    return {"levels": levels, "coarse": dense}
+
+
+def hsc_apply(h, r):
+    """z = M^-1 r for the hierarchy of hsc_build (FP64): per level
+    r_C += R r_F (terms added to r_C in ascending F index) down to the coarse
+    level, z_c = A_c^-1 r_c there (dense Cholesky), then
+    z_F = dinv r_F + P z_C (terms added in ascending C index), z_C = z_c."""
+    r = np.asarray(r, np.float64)
+    vec = r.ndim == 1
+    r = r.reshape(len(r), -1)
+    rs = [r]
+    for L in h["levels"]:
+        cur = rs[-1]
+        Rp, Rc, Rv = L["R"]
+        nxt = cur[L["C"]].copy()
+        for c in range(len(L["C"])):
+            for e in range(Rp[c], Rp[c + 1]):
+                nxt[c] = nxt[c] + Rv[e] * cur[L["F"][Rc[e]]]
+        rs.append(nxt)
+    Lc = np.linalg.cholesky(h["coarse"]) if len(h["coarse"]) else None
+    z = rs[-1].copy()
+    if Lc is not None:
+        import scipy.linalg as sla
+        z = sla.cho_solve((Lc, True), rs[-1])
+    for li in range(len(h["levels"]) - 1, -1, -1):
+        L = h["levels"][li]
+        cur = rs[li]
+        Pp, Pc, Pv = L["P"]
+        zl = np.zeros_like(cur)
+        zl[L["C"]] = z
+        for f in range(len(L["F"])):
+            acc = L["dinv"][f] * cur[L["F"][f]]
+            for e in range(Pp[f], Pp[f + 1]):
+                acc = acc + Pv[e] * z[Pc[e]]
+            zl[L["F"][f]] = acc
+        z = zl
+    return z.reshape(-1) if vec else z

@@ -1,0 +1,149 @@
+"""GPU hierarchical Schur preconditioner and the dense FP64 Cholesky on the
+matrix cores, against the oracle (itself pinned in test_oracle_hsc.py)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _spd(n, seed):
+    rng = np.random.default_rng(seed)
+    M = rng.normal(size=(n, n))
+    return M @ M.T / n + np.eye(n)
+
+
+@pytest.mark.parametrize("n", [1, 31, 32, 100, 257])
+def test_dense_cholesky_matches_numpy(gpu, n):
+    from tool_godot_curvenet_amd import dense_cholesky
+    A = _spd(n, n)
+    Lg = dense_cholesky(A)
+    Ln = np.linalg.cholesky(A)
+    assert np.all(np.triu(Lg, 1) == 0)          # spec: zero upper triangle
+    assert np.abs(Lg - Ln).max() <= 1e-12 * np.abs(Ln).max()
+    assert np.abs(Lg @ Lg.T - A).max() <= 1e-12 * np.abs(A).max()
+
+
+def test_dense_cholesky_spec_kat(gpu):
+    """DenseLinAlg.lean:294-489: [[4,2],[2,3]] -> [[2,0],[1,sqrt 2]], solves
+    to (1.75, 1.5)-style values reproduced by the oracle."""
+    from tool_godot_curvenet_amd import dense_cholesky, dense_cholesky_solve
+    A = np.array([[4.0, 2.0], [2.0, 3.0]])
+    Lg = dense_cholesky(A)
+    assert np.allclose(Lg, [[2.0, 0.0], [1.0, np.sqrt(2.0)]], atol=1e-15)
+    b = np.array([10.0, 8.0])
+    assert np.allclose(dense_cholesky_solve(Lg, b), np.linalg.solve(A, b), atol=1e-14)
+
+
+def test_dense_cholesky_rejects_indefinite(gpu):
+    from tool_godot_curvenet_amd import dense_cholesky
+    from tool_godot_curvenet_amd._lib import CurvenetError
+    A = _spd(40, 3)
+    A[20, 20] = -5.0
+    with pytest.raises(CurvenetError):
+        dense_cholesky(A)
+
+
+@pytest.mark.parametrize("n,k", [(64, 1), (100, 7), (300, 64)])
+def test_dense_cholesky_solve(gpu, n, k):
+    from tool_godot_curvenet_amd import dense_cholesky, dense_cholesky_solve
+    A = _spd(n, 7 + n)
+    B = np.random.default_rng(k).normal(size=(n, k))
+    X = dense_cholesky_solve(dense_cholesky(A), B)
+    assert np.abs(A @ X - B).max() <= 1e-11 * np.abs(B).max()
+
+
+def _lap(name):
+    from tool_godot_curvenet_amd import CutLaplacian, meshgen
+    return CutLaplacian.from_mesh(meshgen.config_mesh(name))
+
+
+@pytest.mark.parametrize("name", ["cylinder64", "icosphere6"])
+@pytest.mark.parametrize("comp", ["lump", "diag"])
+def test_hierarchy_bit_exact(gpu, name, comp):
+    from tool_godot_curvenet_amd import HSC
+    lap = _lap(name)
+    A = lap.A_ff()
+    H = HSC(lap, compensate=comp)
+    ho = O.hsc_build(A.row_ptr, A.col_idx, A.values, comp=comp)
+    lv = H.levels()
+    assert len(lv) == len(ho["levels"]) and len(lv) >= 3
+    for g, o in zip(lv, ho["levels"]):
+        assert g["n"] == o["n"]
+        assert np.array_equal(g["F"], o["F"]) and np.array_equal(g["C"], o["C"])
+        assert np.array_equal(g["dinv"], o["dinv"])
+        assert np.array_equal(g["R"][0], o["R"][0])
+        assert np.array_equal(g["R"][1], o["F"][o["R"][1]])   # level rows of F vertices
+        assert np.array_equal(g["R"][2], o["R"][2])
+        for a, b in zip(g["P"], o["P"]):
+            assert np.array_equal(a, b)
+    Lc, inv = H.coarse()
+    Co = ho["coarse"]
+    assert Lc.shape == Co.shape
+    assert np.abs(Lc @ Lc.T - Co).max() <= 1e-12 * np.abs(Co).max()
+    assert np.abs(inv @ Co - np.eye(len(Co))).max() <= 1e-9
+
+
+@pytest.mark.parametrize("k", [1, 3, 8])
+def test_hsc_apply_matches_oracle(gpu, k):
+    from tool_godot_curvenet_amd import HSC
+    lap = _lap("cylinder64")
+    A = lap.A_ff()
+    H = HSC(lap)
+    ho = O.hsc_build(A.row_ptr, A.col_idx, A.values)
+    r = np.random.default_rng(k).normal(size=(lap.n_free, k))
+    zg = H.apply(r)
+    zo = O.hsc_apply(ho, r)
+    assert np.linalg.norm(zg - zo) <= 1e-11 * np.linalg.norm(zo)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp64_plain", 1e-11),
+                                           ("mixed", 2e-5)])
+@pytest.mark.parametrize("k", [3, 100])
+def test_pcg_first_step_is_hsc_direction(gpu, precision, tol, k):
+    """One PCG step from 0 gives x1 = alpha0 M^-1 b: its direction is the
+    oracle hierarchy's apply of b."""
+    from tool_godot_curvenet_amd import HSC
+    from tool_godot_curvenet_amd import meshgen as G
+    lap = _lap("cylinder64")
+    A = lap.A_ff()
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=60 + k))
+    x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, max_iter=1,
+                        hsc=HSC(lap))
+    assert info.iters == 1
+    X = x.cpu().numpy()
+    Z = O.hsc_apply(O.hsc_build(A.row_ptr, A.col_idx, A.values), B)
+    for c in range(k):
+        e = np.linalg.norm(X[:, c] / np.linalg.norm(X[:, c]) - Z[:, c] / np.linalg.norm(Z[:, c]))
+        assert e <= tol, (c, e)
+
+
+@pytest.mark.parametrize("name", ["cylinder64", "icosphere6"])
+@pytest.mark.parametrize("precision", ["mixed", "fp64", "fp64_plain"])
+@pytest.mark.parametrize("k", [3, 96])
+def test_pcg_hsc_converges(gpu, name, precision, k):
+    from tool_godot_curvenet_amd import HSC
+    from tool_godot_curvenet_amd import meshgen as G
+    lap = _lap(name)
+    A = lap.A_ff()
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=k))
+    x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, hsc=HSC(lap))
+    assert info.status == 0
+    X = x.cpu().numpy()
+    R = B - O.spmm(A.row_ptr, A.col_idx, A.values, X)
+    rr = np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
+    assert np.all(rr <= 1e-9), rr
+    assert info.iters <= 40, info.iters
+
+
+def test_hsc_beats_ic0_on_capsules(gpu):
+    """The benchmark mesh: far fewer iterations than block IC(0)."""
+    from tool_godot_curvenet_amd import HSC, IC0
+    from tool_godot_curvenet_amd import meshgen as G
+    lap = _lap("capsules100k")
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, 3, seed=5), as_numpy=False)
+    _, ih = lap.solve(B, precond="hsc", precision="mixed", tol=1e-9, hsc=HSC(lap))
+    _, ii = lap.solve(B, precond="ic0", precision="mixed", tol=1e-9, ic0=IC0(lap))
+    assert ih.status == 0 and ii.status == 0
+    assert ih.iters * 5 < ii.iters, (ih.iters, ii.iters)

@@ -7,12 +7,15 @@ ABI of ``include/curvenet_la.h``; this package is the Python host side.
 """
 from . import _lib, curves
 from .curvenet import (  # noqa: F401
+    HSC,
     IC0,
     CutLaplacian,
     CsrMatrix,
     cg,
     cg_with_guess,
     cut_seams,
+    dense_cholesky,
+    dense_cholesky_solve,
     from_triangles,
     harmonic_solve,
     project_to_vertices,
@@ -22,7 +25,7 @@ from .curvenet import (  # noqa: F401
 from .autograd import harmonic_solve as harmonic_solve_ad  # noqa: F401
 
 __all__ = [
-    "IC0", "CutLaplacian", "CsrMatrix", "cg", "cg_with_guess", "cut_seams",
+    "HSC", "dense_cholesky", "dense_cholesky_solve", "IC0", "CutLaplacian", "CsrMatrix", "cg", "cg_with_guess", "cut_seams",
     "from_triangles", "harmonic_solve", "harmonic_solve_ad", "project_to_vertices", "spmm",
     "spmv", "curves",
 ]

@@ -19,7 +19,8 @@ STATUS_NAMES = {0: "OK", 1: "MAXIT", 2: "BREAKDOWN", 3: "NOT_SPD", -1: "BAD_ARG"
                 -2: "HIP_ERROR", -3: "NO_DEVICE", -4: "OOM", -5: "NOT_MANIFOLD"}
 
 CN_ORDER_COLOR, CN_ORDER_NATURAL, CN_ORDER_BLOCK = 0, 1, 2
-CN_PRECOND_JACOBI, CN_PRECOND_IC0 = 0, 1
+CN_PRECOND_JACOBI, CN_PRECOND_IC0, CN_PRECOND_HSC = 0, 1, 2
+CN_HSC_LUMP, CN_HSC_DIAG = 0, 1
 CN_PREC_FP64_PLAIN, CN_PREC_FP64, CN_PREC_MIXED = 0, 1, 2
 
 i32 = C.c_int32
@@ -57,6 +58,22 @@ class PcgOpts(C.Structure):
                 ("warm_start", i32)]
 
 
+class HscOpts(C.Structure):
+    _fields_ = [("theta", f64), ("coarse_max", i32), ("max_levels", i32), ("min_frac", f64),
+                ("compensate", i32)]
+
+
+class HscInfo(C.Structure):
+    _fields_ = [("nlevels", i32), ("n_coarse", i32), ("nnz", i64), ("d_coarse_L", vp),
+                ("d_coarse_inv", vp)]
+
+
+class HscLevel(C.Structure):
+    _fields_ = [("n", i32), ("nF", i32), ("nC", i32), ("d_F", vp), ("d_C", vp), ("d_dinv", vp),
+                ("d_r_ptr", vp), ("d_r_col", vp), ("d_r_val", vp), ("d_p_ptr", vp),
+                ("d_p_col", vp), ("d_p_val", vp)]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("status", i32), ("iters", i32), ("refinements", i32), ("relres_max", f64)]
 
@@ -90,6 +107,16 @@ _SIGS = {
     "cn_pcg_solve": (i32, [vp, vp, C.POINTER(PcgOpts), i32, vp, vp, C.POINTER(SolveInfo), vp]),
     "cn_adjoint_solve": (i32, [vp, vp, C.POINTER(PcgOpts), i32, vp, vp, vp, vp, vp, vp,
                                C.POINTER(SolveInfo), vp]),
+    "cn_pcg_solve_hsc": (i32, [vp, vp, C.POINTER(PcgOpts), i32, vp, vp, C.POINTER(SolveInfo),
+                               vp]),
+    "cn_dense_cholesky_f64": (i32, [i32, vp, i32, vp]),
+    "cn_dense_cholesky_solve_f64": (i32, [i32, i32, vp, i32, vp, i32, vp]),
+    "cn_hsc_default_opts": (None, [C.POINTER(HscOpts)]),
+    "cn_hsc_build": (i32, [vp, C.POINTER(HscOpts), C.POINTER(vp), vp]),
+    "cn_hsc_get_info": (i32, [vp, C.POINTER(HscInfo)]),
+    "cn_hsc_get_level": (i32, [vp, i32, C.POINTER(HscLevel)]),
+    "cn_hsc_apply": (i32, [vp, i32, vp, vp, vp]),
+    "cn_hsc_destroy": (i32, [vp]),
     "cn_profile_enable": (None, [i32]),
     "cn_profile_query": (i32, [C.c_char_p, C.POINTER(f64)]),
 }

@@ -186,3 +186,25 @@ struct cn_ic0_t {
   int32_t sweep_mb = 0;
   Sweep lsw, usw;
 };
+
+// Hierarchical Schur preconditioner (see cn_hsc_build.hip).
+struct cn_hsc_t {
+  int32_t n0 = 0;  // rows of level 0 (= the system's n_free)
+  cn_hsc_opts opts{};
+  struct Level {
+    int32_t n = 0, nF = 0, nC = 0;
+    cn::DevBuf<int32_t> F, C, role;  // role[i] = f for F rows, -1 - c for C rows
+    cn::DevBuf<double> dinv;
+    cn::DevBuf<float> dinv32;
+    cn::DevBuf<int32_t> r_ptr, r_col, p_ptr, p_col;
+    cn::DevBuf<double> r_val, p_val;
+    cn::DevBuf<float> r_val32, p_val32;
+  };
+  std::vector<Level> lv;
+  int32_t nc = 0;  // coarse rows
+  cn::DevBuf<double> coarse_L, coarse_inv;
+  int64_t nnz = 0;
+  // apply workspace (levels >= 1 and the coarse result), grown on demand
+  cn::DevBuf<unsigned char> work;
+  size_t work_bytes = 0;
+};

@@ -710,6 +710,59 @@ kb_bwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
   }
 }
 
+// ---- (FUSED) r -= alpha q with partial r.r: the first half of a
+//      preconditioner applied as a whole (HSC), whose r.z comes after it ----
+template <typename T, int K, bool FUSED>
+__global__ void __launch_bounds__(kBlock)
+kn_rupd(int32_t n, const double *__restrict__ alpha, const T *__restrict__ Q,
+        T *__restrict__ R, double *__restrict__ part, const int32_t *__restrict__ ctl) {
+  if (ctl[CTL_ACTIVE] == 0) return;
+  const int g = threadIdx.x % kGroup;
+  double pd[1] = {0.0};
+  const T al = (FUSED && g < K) ? (T)alpha[g] : T(0);
+  NARROW_LOOP(row, 0, n) {
+    if (g < K) {
+      const int64_t o = row * K + g;
+      T r = R[o];
+      if (FUSED) {
+        r = fma(-al, Q[o], r);
+        R[o] = r;
+      }
+      pd[0] += (double)r * (double)r;
+    }
+  }
+  block_partials_narrow<K, 1>(pd, part, 0);
+}
+
+template <typename T, int VEC, bool FUSED>
+__global__ void __launch_bounds__(kBlock)
+kw_rupd(int32_t n, const double *__restrict__ alpha, int64_t k, const T *__restrict__ Q,
+        T *__restrict__ R, double *__restrict__ part, const int32_t *__restrict__ ctl) {
+  if (ctl[CTL_ACTIVE] == 0) return;
+  WIDE_COLS(VEC)
+  double pd[1][VEC] = {};
+  T al[VEC];
+#pragma unroll
+  for (int m = 0; m < VEC; ++m) al[m] = (FUSED && valid) ? (T)alpha[c0 + m] : T(0);
+  WIDE_LOOP(row, 0, n) {
+    if (valid) {
+      const int64_t o = (int64_t)row * k + c0;
+      T rv[VEC];
+      VecIO<T, VEC>::ld(R + o, rv);
+      if (FUSED) {
+        T qv[VEC];
+        VecIO<T, VEC>::ld(Q + o, qv);
+#pragma unroll
+        for (int m = 0; m < VEC; ++m) rv[m] = fma(-al[m], qv[m], rv[m]);
+        VecIO<T, VEC>::st(R + o, rv);
+      }
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) pd[0][m] += (double)rv[m] * (double)rv[m];
+    }
+  }
+  block_partials_wide<VEC, 1>(pd, part, k, c0, 0);
+}
+
 // ---- Jacobi (SparseLinAlg.applyJacobi, zero diagonal -> 0): z = r / d,
 //      (FUSED) r -= alpha q first; partials r.r and r.z -----------------------
 template <typename T, int K, bool FUSED>
@@ -1214,6 +1267,7 @@ struct SysView {
   const int32_t *rowmap, *blk, *lp, *lcol, *up, *ucol;
   const T *lval, *uval, *rdiag;
   const T *dinv;
+  cn_hsc_t *hsc = nullptr;  // whole-apply preconditioner (replaces ic0 / dinv)
   // block sweeps (forward: L, backward: U); mb = 0 when not available
   int32_t mb = 0;
   SweepView<T> lsw{}, usw{};
@@ -1244,6 +1298,12 @@ static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
 #undef W_
   return G.rows;
 }
+
+template <typename T>
+void hsc_apply(cn_hsc_t *H, int64_t k, const T *r0, T *z0, hipStream_t s);
+template <typename T>
+static int32_t launch_dot(int32_t n, int64_t k, const T *A, const T *B, double *part,
+                          hipStream_t s);
 
 // How one level of an IC(0) sweep is launched.
 enum SweepKind { SW_NARROW, SW_THREAD, SW_WIDE, SW_BLOCK };
@@ -1347,6 +1407,18 @@ template <typename T, bool FUSED>
 static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
                           const T *Q, T *R, T *Z, double *part, int64_t stride,
                           const int32_t *ctl, hipStream_t s) {
+  if (V.hsc) {
+    Grid G = grid_rows<T>(V.n, k, kCapSpmm);
+    ProfScope ps("fwd", s);
+#define N_(KK) kn_rupd<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(V.n, alpha, Q, R, part, ctl)
+#define W_(VV) kw_rupd<T, WideV<T, VV>::v, FUSED><<<G.g, kBlock, 0, s>>>(V.n, alpha, k, Q, R, part, ctl)
+    DISPATCH_K(k, T, N_, W_);
+#undef N_
+#undef W_
+    hsc_apply<T>(V.hsc, k, R, Z, s);
+    launch_dot<T>(V.n, k, R, Z, part + stride, s);
+    return G.rows;
+  }
   if (!V.ic0) {
     Grid G = grid_rows<T>(V.n, k, kCapSpmm);
     ProfScope ps("fwd", s);
@@ -1540,7 +1612,7 @@ SysView<double> make_view<double>(cn_system_t *S, cn_ic0_t *F, SolverWork *W) {
 
 // ========================================= fused solve (MIXED and FP64) ==
 template <typename T>
-static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
+static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_pcg_opts *o,
                              int32_t k, const double *d_b, double *d_x,
                              cn_solve_info *info, hipStream_t s) {
   const int32_t n = S->n_free;
@@ -1555,6 +1627,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
   k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
                                          W->dinv32.get());
   SysView<T> V = make_view<T>(S, F, W);
+  V.hsc = H;
   double *st = W->st.get();
   double *hs = W->h_st;
   int32_t *ctl = W->ctl.get();
@@ -1691,7 +1764,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
 }
 
 // ================================ unfused FP64 baseline, scalars on host ==
-static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
+static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_pcg_opts *o,
                              int32_t k, const double *d_b, double *d_x,
                              cn_solve_info *info, hipStream_t s) {
   const int32_t n = S->n_free;
@@ -1706,6 +1779,7 @@ static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, const cn_pcg_opts *o,
   k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
                                          W->dinv32.get());
   SysView<double> V = make_view<double>(S, F, W);
+  V.hsc = H;
   double *st = W->st.get();
   const int64_t stride = (int64_t)W->part_rows * k;
   const unsigned red = (unsigned)((k + 63) / 64);
@@ -1845,23 +1919,21 @@ int32_t cn_profile_query(const char *name, double *ms_total) {
   return it->second.first;
 }
 
-cn_status cn_pcg_solve(cn_system S, cn_ic0 F, const cn_pcg_opts *o, int32_t k,
-                       const double *d_b, double *d_x, cn_solve_info *info,
-                       void *stream) {
-  CN_API_BEGIN
+static cn_status solve_impl(cn_system S, cn_ic0 F, cn_hsc H, const cn_pcg_opts *o, int32_t k,
+                            const double *d_b, double *d_x, cn_solve_info *info, void *stream,
+                            const char *who) {
   if (!S || !o || k <= 0 || !d_b || !d_x) {
-    set_error("cn_pcg_solve: bad arguments");
-    return CN_BAD_ARG;
-  }
-  if (o->precond == CN_PRECOND_IC0 && !F) {
-    set_error("cn_pcg_solve: IC0 preconditioner needs a factor");
+    set_error("%s: bad arguments", who);
     return CN_BAD_ARG;
   }
   if (F && F->n != S->n_free) {
-    set_error("cn_pcg_solve: factor does not match the system");
+    set_error("%s: factor does not match the system", who);
     return CN_BAD_ARG;
   }
-  if (o->precond != CN_PRECOND_IC0) F = nullptr;
+  if (H && H->n0 != S->n_free) {
+    set_error("%s: hierarchy does not match the system", who);
+    return CN_BAD_ARG;
+  }
   hipStream_t s = as_stream(stream);
   if (S->n_free == 0) {
     if (info) { info->status = CN_OK; info->iters = 0; info->refinements = 0; info->relres_max = 0.0; }
@@ -1870,15 +1942,43 @@ cn_status cn_pcg_solve(cn_system S, cn_ic0 F, const cn_pcg_opts *o, int32_t k,
   prof_reset();
   cn_status st;
   switch (o->precision) {
-    case CN_PREC_FP64_PLAIN: st = plain_solve(S, F, o, k, d_b, d_x, info, s); break;
-    case CN_PREC_FP64: st = fused_solve<double>(S, F, o, k, d_b, d_x, info, s); break;
-    case CN_PREC_MIXED: st = fused_solve<float>(S, F, o, k, d_b, d_x, info, s); break;
+    case CN_PREC_FP64_PLAIN: st = plain_solve(S, F, H, o, k, d_b, d_x, info, s); break;
+    case CN_PREC_FP64: st = fused_solve<double>(S, F, H, o, k, d_b, d_x, info, s); break;
+    case CN_PREC_MIXED: st = fused_solve<float>(S, F, H, o, k, d_b, d_x, info, s); break;
     default:
-      set_error("cn_pcg_solve: unknown precision %d", o->precision);
+      set_error("%s: unknown precision %d", who, o->precision);
       return CN_BAD_ARG;
   }
   prof_collect();
   return st;
+}
+
+cn_status cn_pcg_solve(cn_system S, cn_ic0 F, const cn_pcg_opts *o, int32_t k,
+                       const double *d_b, double *d_x, cn_solve_info *info,
+                       void *stream) {
+  CN_API_BEGIN
+  if (o && o->precond == CN_PRECOND_IC0 && !F) {
+    set_error("cn_pcg_solve: IC0 preconditioner needs a factor");
+    return CN_BAD_ARG;
+  }
+  if (o && o->precond == CN_PRECOND_HSC) {
+    set_error("cn_pcg_solve: the HSC preconditioner is applied by cn_pcg_solve_hsc");
+    return CN_BAD_ARG;
+  }
+  if (o && o->precond != CN_PRECOND_IC0) F = nullptr;
+  return solve_impl(S, F, nullptr, o, k, d_b, d_x, info, stream, "cn_pcg_solve");
+  CN_API_END
+}
+
+cn_status cn_pcg_solve_hsc(cn_system S, cn_hsc H, const cn_pcg_opts *o, int32_t k,
+                           const double *d_b, double *d_x, cn_solve_info *info,
+                           void *stream) {
+  CN_API_BEGIN
+  if (!H) {
+    set_error("cn_pcg_solve_hsc: no hierarchy");
+    return CN_BAD_ARG;
+  }
+  return solve_impl(S, nullptr, H, o, k, d_b, d_x, info, stream, "cn_pcg_solve_hsc");
   CN_API_END
 }
 

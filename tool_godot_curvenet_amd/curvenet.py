@@ -210,8 +210,10 @@ class _System:
 
     def solve(self, b, precond: str = "ic0", precision: str = "mixed", tol: float = 1e-9,
               x0=None, ic0: "IC0 | None" = None, max_iter: int = 20000,
-              inner_tol: float = 1e-5, check_every: int = 8, max_refine: int = 30):
-        """Solve A_ff X = B (n x k or n).  Returns (x, SolveInfo)."""
+              inner_tol: float = 1e-5, check_every: int = 8, max_refine: int = 30,
+              hsc: "HSC | None" = None):
+        """Solve A_ff X = B (n x k or n); precond "ic0", "jacobi" or "hsc".
+        Returns (x, SolveInfo)."""
         torch = _torch()
         lib = L.load()
         bt = _as_tensor(b, torch.float64)
@@ -220,13 +222,19 @@ class _System:
         k = bt.shape[1]
         x = (_as_tensor(x0, torch.float64).reshape(self.n_free, k).clone()
              if x0 is not None else torch.zeros_like(bt))
-        o = self._opts(precond, precision, tol, max_iter, inner_tol, check_every, max_refine,
-                       x0 is not None)
-        if precond == "ic0" and ic0 is None:
-            ic0 = IC0(self)
+        o = self._opts("jacobi" if precond == "hsc" else precond, precision, tol, max_iter,
+                       inner_tol, check_every, max_refine, x0 is not None)
         info = L.SolveInfo()
-        st = lib.cn_pcg_solve(self._h, ic0.handle if ic0 is not None else C.c_void_p(0),
-                              C.byref(o), k, _p(bt), _p(x), C.byref(info), _stream())
+        if precond == "hsc":
+            if hsc is None:
+                hsc = HSC(self)
+            st = lib.cn_pcg_solve_hsc(self._h, hsc.handle, C.byref(o), k, _p(bt), _p(x),
+                                      C.byref(info), _stream())
+        else:
+            if precond == "ic0" and ic0 is None:
+                ic0 = IC0(self)
+            st = lib.cn_pcg_solve(self._h, ic0.handle if ic0 is not None else C.c_void_p(0),
+                                  C.byref(o), k, _p(bt), _p(x), C.byref(info), _stream())
         L.check(st, "pcg_solve", ok=(L.CN_OK, L.CN_MAXIT, L.CN_BREAKDOWN))
         return (x.reshape(-1) if vec else x), info
 
@@ -342,6 +350,97 @@ class CsrSystem(_System):
         L.check(L.load().cn_system_from_csr(A.rows, _p(ptr), _p(col), _p(val), C.byref(h),
                                             _stream()), "system_from_csr")
         super().__init__(h.value)
+
+
+class HSC:
+    """Hierarchical Schur preconditioner of a system's A_ff (cn_hsc_build):
+    greedy independent-set elimination with exact Schur complements,
+    sparsified fill, dense coarse inverse on the matrix cores."""
+
+    def __init__(self, system: "_System", theta: float = 0.02, coarse_max: int = 512,
+                 max_levels: int = 24, min_frac: float = 0.05, compensate: str = "lump"):
+        lib = L.load()
+        self.system = system
+        o = L.HscOpts()
+        lib.cn_hsc_default_opts(C.byref(o))
+        o.theta, o.coarse_max, o.max_levels, o.min_frac = theta, coarse_max, max_levels, min_frac
+        o.compensate = {"lump": L.CN_HSC_LUMP, "diag": L.CN_HSC_DIAG}[compensate]
+        h = C.c_void_p(0)
+        L.check(lib.cn_hsc_build(system.handle, C.byref(o), C.byref(h), _stream()), "hsc_build")
+        self._h = h
+        info = L.HscInfo()
+        L.check(lib.cn_hsc_get_info(h, C.byref(info)), "hsc_get_info")
+        self.info = info
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                L.load().cn_hsc_destroy(h)
+            except Exception:
+                pass
+            self._h = C.c_void_p(0)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def levels(self) -> list:
+        """Host copies of every level: dict with n, F, C, dinv, R = (ptr, col,
+        val) (columns are level rows of F vertices), P = (ptr, col, val)."""
+        out = []
+        for l in range(self.info.nlevels):
+            v = L.HscLevel()
+            L.check(L.load().cn_hsc_get_level(self._h, l, C.byref(v)), "hsc_get_level")
+            rp = _np(v.d_r_ptr, v.nC + 1, np.int32)
+            pp = _np(v.d_p_ptr, v.nF + 1, np.int32)
+            out.append({"n": v.n, "F": _np(v.d_F, v.nF, np.int32), "C": _np(v.d_C, v.nC, np.int32),
+                        "dinv": _np(v.d_dinv, v.nF, np.float64),
+                        "R": (rp, _np(v.d_r_col, int(rp[-1]), np.int32),
+                              _np(v.d_r_val, int(rp[-1]), np.float64)),
+                        "P": (pp, _np(v.d_p_col, int(pp[-1]), np.int32),
+                              _np(v.d_p_val, int(pp[-1]), np.float64))})
+        return out
+
+    def coarse(self):
+        """(Cholesky factor, inverse) of the coarse matrix, dense FP64."""
+        nc = self.info.n_coarse
+        return (_np(self.info.d_coarse_L, nc * nc, np.float64).reshape(nc, nc),
+                _np(self.info.d_coarse_inv, nc * nc, np.float64).reshape(nc, nc))
+
+    def apply(self, r):
+        torch = _torch()
+        rt = _as_tensor(r, torch.float64)
+        vec = rt.dim() == 1
+        rt = rt.reshape(self.system.n_free, -1).contiguous()
+        z = torch.empty_like(rt)
+        L.check(L.load().cn_hsc_apply(self._h, rt.shape[1], _p(rt), _p(z), _stream()),
+                "hsc_apply")
+        z = z.cpu().numpy()
+        return z.reshape(-1) if vec else z
+
+
+def dense_cholesky(A):
+    """Lower Cholesky factor of a dense SPD matrix (cn_dense_cholesky_f64)."""
+    torch = _torch()
+    a = _as_tensor(np.asarray(A, dtype=np.float64), torch.float64).clone()
+    n = a.shape[0]
+    L.check(L.load().cn_dense_cholesky_f64(n, _p(a), n, _stream()), "dense_cholesky_f64")
+    return a.cpu().numpy()
+
+
+def dense_cholesky_solve(Lf, B):
+    """(L L^T)^-1 B for a factor from dense_cholesky (cn_dense_cholesky_solve_f64)."""
+    torch = _torch()
+    lt = _as_tensor(np.asarray(Lf, dtype=np.float64), torch.float64)
+    b = _as_tensor(np.asarray(B, dtype=np.float64), torch.float64)
+    vec = b.dim() == 1
+    b = b.reshape(lt.shape[0], -1).contiguous().clone()
+    L.check(L.load().cn_dense_cholesky_solve_f64(lt.shape[0], b.shape[1], _p(lt), lt.shape[0],
+                                                 _p(b), b.shape[1], _stream()),
+            "dense_cholesky_solve_f64")
+    out = b.cpu().numpy()
+    return out.reshape(-1) if vec else out
 
 
 class IC0:
