@@ -79,6 +79,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--poses", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-9)
+    ap.add_argument("--precond", default="hsc", choices=["hsc", "ic0"],
+                    help="hierarchical Schur (default) or IC(0)")
+    ap.add_argument("--theta", type=float, default=0.015, help="HSC drop threshold")
     ap.add_argument("--ordering", default="block", choices=["block", "color", "natural"],
                     help="IC(0) ordering (block multicolour, point colour, natural levels)")
     ap.add_argument("--block-size", type=int, default=16)
@@ -126,7 +129,14 @@ def main():
     t0 = time.time()
     mesh = meshgen.config_mesh("capsules100k")
     lap = cn.CutLaplacian.from_mesh(mesh)
-    ic0 = cn.IC0(lap, args.ordering, block_size=args.block_size)
+    if args.precond == "hsc":
+        pre = cn.HSC(lap, theta=args.theta)
+        pre_desc = (f"HSC theta {args.theta} ({pre.info.nlevels} levels, coarse "
+                    f"{pre.info.n_coarse})")
+    else:
+        pre = cn.IC0(lap, args.ordering, block_size=args.block_size)
+        pre_desc = (f"IC(0) {args.ordering} order ({pre.info.nlevels} colours, "
+                    f"{pre.info.nblocks} blocks)")
     n, ncon = lap.n_free, lap.n_con
     nnz = lap.info.nnz_ff
     P = args.poses
@@ -140,7 +150,7 @@ def main():
 
     opts = _lib.PcgOpts()
     lib.cn_pcg_default_opts(opts)
-    opts.precond = _lib.CN_PRECOND_IC0
+    opts.precond = _lib.CN_PRECOND_IC0 if args.precond == "ic0" else _lib.CN_PRECOND_JACOBI
     opts.precision = _lib.CN_PREC_MIXED if args.precision == "mixed" else _lib.CN_PREC_FP64
     opts.tol_rel = args.tol
     stream = torch.cuda.current_stream().cuda_stream
@@ -151,7 +161,8 @@ def main():
         st = lib.cn_harmonic_rhs(lap.handle, k, C.c_void_p(Xc.data_ptr()),
                                  C.c_void_p(B.data_ptr()), sp)
         _lib.check(st, "harmonic_rhs")
-        st = lib.cn_pcg_solve(lap.handle, ic0.handle, C.byref(opts), k,
+        solve = lib.cn_pcg_solve_hsc if args.precond == "hsc" else lib.cn_pcg_solve
+        st = solve(lap.handle, pre.handle, C.byref(opts), k,
                               C.c_void_p(B.data_ptr()), C.c_void_p(X.data_ptr()),
                               C.byref(info), sp)
         _lib.check(st, "pcg_solve")
@@ -200,16 +211,17 @@ def main():
     # colours and the index tables are not counted.  Launches that find
     # every column converged return at once and are given no bytes.
     nk4 = 4 * n * k
-    nlev = ic0.info.nlevels
     spmm_bytes = 8 * nnz + 4 * (n + 1) + 2 * nk4
     kern = {
         "spmm": ("kw_spmm_dot<float,4>", prof["spmm"][0], prof["spmm"][1],
                  iters * spmm_bytes),
-        "fwd": ("kb_fwd<float,4,16>", prof["fwd"][0] * nlev, prof["fwd"][1],
-                (4 * iters + 2 * refines) * nk4),
-        "bwd": ("kb_bwd<float,4,16>", prof["bwd"][0] * nlev, prof["bwd"][1],
-                6 * (iters + refines) * nk4),
     }
+    if args.precond == "ic0":
+        nlev = pre.info.nlevels
+        kern["fwd"] = ("kb_fwd<float,4,16>", prof["fwd"][0] * nlev, prof["fwd"][1],
+                       (4 * iters + 2 * refines) * nk4)
+        kern["bwd"] = ("kb_bwd<float,4,16>", prof["bwd"][0] * nlev, prof["bwd"][1],
+                       6 * (iters + refines) * nk4)
     kernels = {}
     for tag, (kname, launches, ms, nbytes) in kern.items():
         gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
@@ -284,8 +296,7 @@ def main():
         "data": "synthetic (seeded capsule union, seeded constraint displacements)",
         "config": {"workload": "capsule union 12 capsules / 16 ring cuts, "
                                f"{mesh.nv} verts, n_free={n}, nnz={nnz}, "
-                               f"{P} poses x 3 RHS (k={k}), IC(0) {args.ordering} order "
-                               f"({ic0.info.nlevels} colours, {ic0.info.nblocks} blocks), "
+                               f"{P} poses x 3 RHS (k={k}), {pre_desc}, "
                                f"tol {args.tol}"},
         "roofline": {"bound": "hbm", "achieved": kernels[dom]["achieved_gbs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernels[dom]["frac"],

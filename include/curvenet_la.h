@@ -258,7 +258,7 @@ enum {
 };
 
 typedef struct cn_hsc_opts {
-  double theta;         /* drop threshold (default 0.02)                     */
+  double theta;         /* drop threshold (default 0.015)                    */
   int32_t coarse_max;   /* stop at this many rows (default 512)              */
   int32_t max_levels;   /* default 24                                         */
   double min_frac;      /* stop when |F| < min_frac n (default 0.05)          */

@@ -441,7 +441,7 @@ def adjoint(ff, fc, g, x=None, xc=None, tol_rel=1e-14, factor=None):
 
 
 # -------------------------------------------------------------------- HSC --
-def hsc_build(ptr, col, val, theta=0.02, coarse_max=512, max_levels=24, min_frac=0.05,
+def hsc_build(ptr, col, val, theta=0.015, coarse_max=512, max_levels=24, min_frac=0.05,
               comp="lump"):
     """Hierarchical Schur elimination with sparsification (after
     Krishnan-Fattal-Szeliski 2013; the per-vertex rule is

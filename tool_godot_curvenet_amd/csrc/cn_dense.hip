@@ -41,7 +41,7 @@ template <bool AT, bool BT, bool LOWER, typename TB, typename TC>
 __global__ void __launch_bounds__(256)
 k_gemm_f64(int m, int n, int kk, double alpha, const double *__restrict__ A, int64_t lda,
            const TB *__restrict__ B, int64_t ldb, double beta, TC *__restrict__ C,
-           int64_t ldc) {
+           int64_t ldc, const int32_t *__restrict__ crow = nullptr) {
   if (LOWER && blockIdx.x > blockIdx.y) return;
   __shared__ double As[kGT][kGK + 1];
   __shared__ double Bs[kGK][kGT + 1];
@@ -76,7 +76,7 @@ k_gemm_f64(int m, int n, int kk, double alpha, const double *__restrict__ A, int
   for (int v = 0; v < 4; ++v) {
     const int i = bi + wi + (l >> 4) + 4 * v, j = bj + wj + (l & 15);
     if (i < m && j < n && (!LOWER || j <= i)) {
-      TC *cp = C + (int64_t)i * ldc + j;
+      TC *cp = C + (int64_t)(crow ? crow[i] : i) * ldc + j;
       const double old = beta == 0.0 ? 0.0 : to_d(*cp);
       *cp = (TC)(beta * old + alpha * acc[v]);
     }
@@ -243,6 +243,19 @@ template void dense_gemm_nn<double, double>(int, int, const double *, const doub
                                             double *, int64_t, hipStream_t);
 template void dense_gemm_nn<float, float>(int, int, const double *, const float *, int64_t,
                                           float *, int64_t, hipStream_t);
+
+// Same product written to rows zrows[i] of Z.
+template <typename TB, typename TC>
+void dense_gemm_nn_rows(int m, int k, const double *M, const TB *X, int64_t ldx, TC *Z,
+                        int64_t ldz, const int32_t *zrows, hipStream_t s) {
+  k_gemm_f64<false, false, false, TB, TC><<<gemm_grid(m, k), 256, 0, s>>>(
+      m, k, m, 1.0, M, m, X, ldx, 0.0, Z, ldz, zrows);
+}
+template void dense_gemm_nn_rows<double, double>(int, int, const double *, const double *,
+                                                 int64_t, double *, int64_t, const int32_t *,
+                                                 hipStream_t);
+template void dense_gemm_nn_rows<float, float>(int, int, const double *, const float *, int64_t,
+                                               float *, int64_t, const int32_t *, hipStream_t);
 
 }  // namespace cn
 

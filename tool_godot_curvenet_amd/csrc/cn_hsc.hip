@@ -8,9 +8,14 @@
 //   up:   z_F = D_F^-1 r_F + P z_C    (P = -D_F^-1 A_FC = R^T),  z_C = z_{l+1}
 // R rows are the coarse vertices and gather their fine (F) neighbours: the
 // per-coarse restriction of SlangCodegen/HierarchicalSparsify.lean:64-93 in
-// its CSR-by-coarse form, with the Schur weights instead of 1.  Levels >= 1
-// keep r and z in one buffer: the up pass overwrites r_l with z_l (each row
-// reads only its own r).
+// its CSR-by-coarse form, with the Schur weights instead of 1.
+//
+// Every level is addressed in level-0 rows, so the whole apply works in
+// place in the output z (n0 x k): the down pass rewrites the kept rows of a
+// level (reading the eliminated ones, which it never writes), the up pass
+// writes only the eliminated rows (the kept rows already hold their z from
+// the deeper levels).  Level 0 reads the input r instead of z.  Only the
+// coarse rows are gathered into a small buffer for the GEMM.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,92 +26,92 @@
 namespace cn {
 
 template <typename TB, typename TC>
-void dense_gemm_nn(int m, int k, const double *M, const TB *X, int64_t ldx, TC *Z, int64_t ldz,
-                   hipStream_t s);
+void dense_gemm_nn_rows(int m, int k, const double *M, const TB *X, int64_t ldx, TC *Z,
+                        int64_t ldz, const int32_t *zrows, hipStream_t s);
 
 // ---- wide form: one wavefront per (row, 64 VEC column tile) ------------
+// Z[Cg[c]] = src[Cg[c]] + sum_e rval[e] src[rcol[e]]   (src = r or Z)
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
-kw_hsc_down(int32_t nC, const int32_t *__restrict__ C, const int32_t *__restrict__ rptr,
+kw_hsc_down(int32_t nC, const int32_t *__restrict__ Cg, const int32_t *__restrict__ rptr,
             const int32_t *__restrict__ rcol, const T *__restrict__ rval, int64_t k,
-            const T *in, T *__restrict__ out) {
+            const T *src, T *Z) {
   const int lane = threadIdx.x & 63;
   const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
   const bool valid = c0 < k;
   const RowIter ri = row_iter(0, nC, kWaves, threadIdx.x >> 6);
   for (int32_t row = ri.first; row < ri.end; row += ri.stride) {
     const int32_t r = uniform(row);
-    const int32_t src = uniform(C[r]);
+    const int64_t o = (int64_t)uniform(Cg[r]) * k + c0;
     T acc[VEC] = {};
-    if (valid) VecIO<T, VEC>::ld(in + (int64_t)src * k + c0, acc);
-    gather_wide<T, VEC>(uniform(rptr[r]), uniform(rptr[r + 1]), rcol, rval, in, k, c0, valid, acc);
-    if (valid) VecIO<T, VEC>::st(out + (int64_t)r * k + c0, acc);
+    if (valid) VecIO<T, VEC>::ld(src + o, acc);
+    gather_wide<T, VEC>(uniform(rptr[r]), uniform(rptr[r + 1]), rcol, rval, src, k, c0, valid, acc);
+    if (valid) VecIO<T, VEC>::st(Z + o, acc);
   }
 }
 
+// Z[Fg[f]] = dinv[f] src[Fg[f]] + sum_e pval[e] Z[pcol[e]]   (src = r or Z)
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
-kw_hsc_up(int32_t n, const int32_t *__restrict__ role, const T *__restrict__ dinv,
+kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv,
           const int32_t *__restrict__ pptr, const int32_t *__restrict__ pcol,
-          const T *__restrict__ pval, int64_t k, const T *rin, const T *zin, T *out) {
+          const T *__restrict__ pval, int64_t k, const T *src, T *Z) {
   const int lane = threadIdx.x & 63;
   const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
   const bool valid = c0 < k;
-  const RowIter ri = row_iter(0, n, kWaves, threadIdx.x >> 6);
+  const RowIter ri = row_iter(0, nF, kWaves, threadIdx.x >> 6);
   for (int32_t row = ri.first; row < ri.end; row += ri.stride) {
-    const int32_t i = uniform(row);
-    const int32_t ro = uniform(role[i]);
+    const int32_t f = uniform(row);
+    const int64_t o = (int64_t)uniform(Fg[f]) * k + c0;
+    const T d = dinv[f];
     T acc[VEC] = {};
-    if (ro < 0) {
-      if (valid) VecIO<T, VEC>::ld(zin + (int64_t)(-1 - ro) * k + c0, acc);
-    } else {
-      const T d = dinv[ro];
-      if (valid) {
-        VecIO<T, VEC>::ld(rin + (int64_t)i * k + c0, acc);
+    if (valid) {
+      VecIO<T, VEC>::ld(src + o, acc);
 #pragma unroll
-        for (int m = 0; m < VEC; ++m) acc[m] = d * acc[m];
-      }
-      gather_wide<T, VEC>(uniform(pptr[ro]), uniform(pptr[ro + 1]), pcol, pval, zin, k, c0,
-                          valid, acc);
+      for (int m = 0; m < VEC; ++m) acc[m] = d * acc[m];
     }
-    if (valid) VecIO<T, VEC>::st(out + (int64_t)i * k + c0, acc);
+    gather_wide<T, VEC>(uniform(pptr[f]), uniform(pptr[f + 1]), pcol, pval, Z, k, c0, valid, acc);
+    if (valid) VecIO<T, VEC>::st(Z + o, acc);
   }
 }
 
 // ---- scalar form: one thread per (row, column) --------------------------
 template <typename T>
-__global__ void k_hsc_down_s(int32_t nC, const int32_t *__restrict__ C,
+__global__ void k_hsc_down_s(int32_t nC, const int32_t *__restrict__ Cg,
                              const int32_t *__restrict__ rptr, const int32_t *__restrict__ rcol,
-                             const T *__restrict__ rval, int64_t k, const T *in,
-                             T *__restrict__ out) {
+                             const T *__restrict__ rval, int64_t k, const T *src, T *Z) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= (int64_t)nC * k) return;
   const int32_t r = (int32_t)(id / k);
   const int64_t c = id - (int64_t)r * k;
-  T acc = in[(int64_t)C[r] * k + c];
-  for (int32_t e = rptr[r]; e < rptr[r + 1]; ++e) acc = fma(rval[e], in[(int64_t)rcol[e] * k + c], acc);
-  out[id] = acc;
+  const int64_t o = (int64_t)Cg[r] * k + c;
+  T acc = src[o];
+  for (int32_t e = rptr[r]; e < rptr[r + 1]; ++e) acc = fma(rval[e], src[(int64_t)rcol[e] * k + c], acc);
+  Z[o] = acc;
 }
 
 template <typename T>
-__global__ void k_hsc_up_s(int32_t n, const int32_t *__restrict__ role, const T *__restrict__ dinv,
+__global__ void k_hsc_up_s(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv,
                            const int32_t *__restrict__ pptr, const int32_t *__restrict__ pcol,
-                           const T *__restrict__ pval, int64_t k, const T *rin, const T *zin,
-                           T *out) {
+                           const T *__restrict__ pval, int64_t k, const T *src, T *Z) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= (int64_t)n * k) return;
-  const int32_t i = (int32_t)(id / k);
-  const int64_t c = id - (int64_t)i * k;
-  const int32_t ro = role[i];
-  T acc;
-  if (ro < 0) {
-    acc = zin[(int64_t)(-1 - ro) * k + c];
-  } else {
-    acc = dinv[ro] * rin[id];
-    for (int32_t e = pptr[ro]; e < pptr[ro + 1]; ++e)
-      acc = fma(pval[e], zin[(int64_t)pcol[e] * k + c], acc);
-  }
-  out[id] = acc;
+  if (id >= (int64_t)nF * k) return;
+  const int32_t f = (int32_t)(id / k);
+  const int64_t c = id - (int64_t)f * k;
+  const int64_t o = (int64_t)Fg[f] * k + c;
+  T acc = dinv[f] * src[o];
+  for (int32_t e = pptr[f]; e < pptr[f + 1]; ++e) acc = fma(pval[e], Z[(int64_t)pcol[e] * k + c], acc);
+  Z[o] = acc;
+}
+
+// W[t] = src[rows[t]] (the coarse rows, gathered for the GEMM)
+template <typename T>
+__global__ void k_gather_rows(int32_t m, const int32_t *__restrict__ rows, int64_t k,
+                              const T *__restrict__ src, T *__restrict__ W) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)m * k) return;
+  const int32_t t = (int32_t)(id / k);
+  W[id] = src[(int64_t)rows[t] * k + (id - (int64_t)t * k)];
 }
 
 template <typename T>
@@ -123,12 +128,6 @@ static dim3 hsc_grid(int64_t rows, int64_t k, int v) {
   return dim3((unsigned)g, (unsigned)((k + 64 * v - 1) / (64 * v)), 1);
 }
 
-template <typename T>
-static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *in, T *out, hipStream_t s);
-template <typename T>
-static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *rin, const T *zin, T *out,
-                     hipStream_t s);
-
 template <typename T> struct LvVals;
 template <> struct LvVals<double> {
   static const double *r(const cn_hsc_t::Level &L) { return L.r_val.get(); }
@@ -142,17 +141,17 @@ template <> struct LvVals<float> {
 };
 
 template <typename T>
-static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *in, T *out, hipStream_t s) {
+static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z, hipStream_t s) {
   if (L.nC == 0) return;
   const int v = hsc_vec<T>(k);
   if (v == 0) {
     const int64_t tot = (int64_t)L.nC * k;
-    k_hsc_down_s<T><<<grid_for(tot, 256), 256, 0, s>>>(L.nC, L.C.get(), L.r_ptr.get(),
-                                                     L.r_col.get(), LvVals<T>::r(L), k, in, out);
+    k_hsc_down_s<T><<<grid_for(tot, 256), 256, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(),
+                                                     L.r_colg.get(), LvVals<T>::r(L), k, src, Z);
     return;
   }
   const dim3 g = hsc_grid(L.nC, k, v);
-#define D_(VV) kw_hsc_down<T, VV><<<g, kBlock, 0, s>>>(L.nC, L.C.get(), L.r_ptr.get(), L.r_col.get(), LvVals<T>::r(L), k, in, out)
+#define D_(VV) kw_hsc_down<T, VV><<<g, kBlock, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(), L.r_colg.get(), LvVals<T>::r(L), k, src, Z)
   if constexpr (sizeof(T) == 4) {
     if (v == 4) { D_(4); return; }
   }
@@ -161,18 +160,18 @@ static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *in, T *out,
 }
 
 template <typename T>
-static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *rin, const T *zin, T *out,
-                     hipStream_t s) {
+static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z, hipStream_t s) {
+  if (L.nF == 0) return;
   const int v = hsc_vec<T>(k);
   if (v == 0) {
-    const int64_t tot = (int64_t)L.n * k;
-    k_hsc_up_s<T><<<grid_for(tot, 256), 256, 0, s>>>(L.n, L.role.get(), LvVals<T>::d(L),
-                                                   L.p_ptr.get(), L.p_col.get(), LvVals<T>::p(L),
-                                                   k, rin, zin, out);
+    const int64_t tot = (int64_t)L.nF * k;
+    k_hsc_up_s<T><<<grid_for(tot, 256), 256, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L),
+                                                   L.p_ptr.get(), L.p_colg.get(),
+                                                   LvVals<T>::p(L), k, src, Z);
     return;
   }
-  const dim3 g = hsc_grid(L.n, k, v);
-#define U_(VV) kw_hsc_up<T, VV><<<g, kBlock, 0, s>>>(L.n, L.role.get(), LvVals<T>::d(L), L.p_ptr.get(), L.p_col.get(), LvVals<T>::p(L), k, rin, zin, out)
+  const dim3 g = hsc_grid(L.nF, k, v);
+#define U_(VV) kw_hsc_up<T, VV><<<g, kBlock, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L), L.p_ptr.get(), L.p_colg.get(), LvVals<T>::p(L), k, src, Z)
   if constexpr (sizeof(T) == 4) {
     if (v == 4) { U_(4); return; }
   }
@@ -180,37 +179,25 @@ static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *rin, const T 
 #undef U_
 }
 
-// z0 = M^-1 r0 (n0 x k); level buffers live in H->work.
+// z = M^-1 r (n0 x k), r and z distinct.
 template <typename T>
-void hsc_apply(cn_hsc_t *H, int64_t k, const T *r0, T *z0, hipStream_t s) {
+void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s) {
   const int32_t L = (int32_t)H->lv.size();
-  // buffers: levels 1..L (n_l rows; level L is the coarse input) + coarse z
-  std::vector<size_t> off(L + 2, 0);
-  size_t tot = 0;
-  for (int32_t l = 1; l <= L; ++l) {
-    off[l] = tot;
-    const int64_t nl = l < L ? H->lv[l].n : H->nc;
-    tot += (size_t)nl * k;
-  }
-  off[L + 1] = tot;
-  tot += (size_t)H->nc * k;
-  const size_t bytes = tot * sizeof(T) + 256;
+  const size_t bytes = sizeof(T) * (size_t)H->nc * k + 256;
   if (H->work_bytes < bytes) {
     H->work.alloc(bytes);
     H->work_bytes = bytes;
   }
-  T *base = reinterpret_cast<T *>(H->work.get());
-  auto buf = [&](int32_t l) { return base + off[l]; };
-  for (int32_t l = 0; l < L; ++l) level_down<T>(H->lv[l], k, l == 0 ? r0 : buf(l), buf(l + 1), s);
-  const T *rc = L == 0 ? r0 : buf(L);
-  T *zc = L == 0 ? z0 : buf(L + 1);
-  if (H->nc > 0) dense_gemm_nn<T, T>(H->nc, (int)k, H->coarse_inv.get(), rc, k, zc, k, s);
-  for (int32_t l = L - 1; l >= 0; --l) {
-    const T *zin = l == L - 1 ? zc : buf(l + 1);
-    const T *rin = l == 0 ? r0 : buf(l);
-    T *out = l == 0 ? z0 : buf(l);
-    level_up<T>(H->lv[l], k, rin, zin, out, s);
+  T *W = reinterpret_cast<T *>(H->work.get());
+  for (int32_t l = 0; l < L; ++l) level_down<T>(H->lv[l], k, l == 0 ? r : z, z, s);
+  if (H->nc > 0) {
+    const int64_t tot = (int64_t)H->nc * k;
+    k_gather_rows<T><<<grid_for(tot, 256), 256, 0, s>>>(H->nc, H->coarse_rows.get(), k,
+                                                      L == 0 ? r : z, W);
+    dense_gemm_nn_rows<T, T>(H->nc, (int)k, H->coarse_inv.get(), W, k, z, k,
+                             H->coarse_rows.get(), s);
   }
+  for (int32_t l = L - 1; l >= 0; --l) level_up<T>(H->lv[l], k, l == 0 ? r : z, z, s);
 }
 template void hsc_apply<float>(cn_hsc_t *, int64_t, const float *, float *, hipStream_t);
 template void hsc_apply<double>(cn_hsc_t *, int64_t, const double *, double *, hipStream_t);

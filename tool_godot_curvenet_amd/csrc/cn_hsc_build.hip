@@ -223,7 +223,7 @@ extern "C" {
 
 void cn_hsc_default_opts(cn_hsc_opts *o) {
   if (!o) return;
-  o->theta = 0.02;
+  o->theta = 0.015;
   o->coarse_max = 512;
   o->max_levels = 24;
   o->min_frac = 0.05;
@@ -263,6 +263,8 @@ cn_status cn_hsc_build(cn_system S, const cn_hsc_opts *opts, cn_hsc *out, void *
   try {
     H->n0 = n;
     H->opts = o;
+    std::vector<int32_t> g(n);  // level-0 row of every row of the current level
+    for (int32_t i = 0; i < n; ++i) g[i] = i;
     while (A.n() > o.coarse_max && (int32_t)H->lv.size() < o.max_levels) {
       HostLevel hl;
       HostCsr next;
@@ -272,12 +274,17 @@ cn_status cn_hsc_build(cn_system S, const cn_hsc_opts *opts, cn_hsc *out, void *
       L.n = hl.n;
       L.nF = (int32_t)hl.F.size();
       L.nC = (int32_t)hl.C.size();
-      std::vector<int32_t> role(hl.n);
-      for (int32_t f = 0; f < L.nF; ++f) role[hl.F[f]] = f;
-      for (int32_t c = 0; c < L.nC; ++c) role[hl.C[c]] = -1 - c;
+      std::vector<int32_t> Fg(L.nF), Cg(L.nC), rg(hl.r_col.size()), pg(hl.p_col.size());
+      for (int32_t f = 0; f < L.nF; ++f) Fg[f] = g[hl.F[f]];
+      for (int32_t c = 0; c < L.nC; ++c) Cg[c] = g[hl.C[c]];
+      for (size_t e = 0; e < rg.size(); ++e) rg[e] = g[hl.r_col[e]];
+      for (size_t e = 0; e < pg.size(); ++e) pg[e] = Cg[hl.p_col[e]];
       upload(L.F, hl.F, s);
       upload(L.C, hl.C, s);
-      upload(L.role, role, s);
+      upload(L.Fg, Fg, s);
+      upload(L.Cg, Cg, s);
+      upload(L.r_colg, rg, s);
+      upload(L.p_colg, pg, s);
       upload(L.dinv, hl.dinv, s);
       upload(L.dinv32, to_f32(hl.dinv), s);
       upload(L.r_ptr, hl.r_ptr, s);
@@ -290,11 +297,13 @@ cn_status cn_hsc_build(cn_system S, const cn_hsc_opts *opts, cn_hsc *out, void *
       upload(L.p_val32, to_f32(hl.p_val), s);
       H->nnz += (int64_t)hl.r_col.size() + (int64_t)hl.p_col.size();
       CN_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
+      g = std::move(Cg);
       A = std::move(next);
     }
     // coarse: dense, factored and inverted on the matrix cores
     const int32_t nc = A.n();
     H->nc = nc;
+    upload(H->coarse_rows, g, s);
     if (nc > 0) {
       std::vector<double> dense((size_t)nc * nc, 0.0);
       for (int32_t i = 0; i < nc; ++i)

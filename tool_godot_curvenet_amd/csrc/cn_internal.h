@@ -193,18 +193,23 @@ struct cn_hsc_t {
   cn_hsc_opts opts{};
   struct Level {
     int32_t n = 0, nF = 0, nC = 0;
-    cn::DevBuf<int32_t> F, C, role;  // role[i] = f for F rows, -1 - c for C rows
+    // level-local numbering (the hierarchy as built; cn_hsc_get_level)
+    cn::DevBuf<int32_t> F, C;
     cn::DevBuf<double> dinv;
     cn::DevBuf<float> dinv32;
     cn::DevBuf<int32_t> r_ptr, r_col, p_ptr, p_col;
     cn::DevBuf<double> r_val, p_val;
     cn::DevBuf<float> r_val32, p_val32;
+    // the same rows and columns as level-0 rows: the apply works in place
+    // in one n0 x k buffer
+    cn::DevBuf<int32_t> Fg, Cg, r_colg, p_colg;
   };
   std::vector<Level> lv;
-  int32_t nc = 0;  // coarse rows
+  int32_t nc = 0;                   // coarse rows
+  cn::DevBuf<int32_t> coarse_rows;  // their level-0 rows
   cn::DevBuf<double> coarse_L, coarse_inv;
   int64_t nnz = 0;
-  // apply workspace (levels >= 1 and the coarse result), grown on demand
+  // apply workspace: the gathered coarse rows (nc x k), grown on demand
   cn::DevBuf<unsigned char> work;
   size_t work_bytes = 0;
 };

@@ -46,12 +46,20 @@ enum {
 };
 // control block
 // CTL_ITERS counts the iterations in which some column was still active.
+// CTL_PAUSE stops the iteration until the host's next residual replacement:
+// set when a column's iterated residual has fallen delta^4 below the last
+// true one (far past the delta^2 trigger), so a fast-converging solve does
+// not run on in FP32 between two host checks.
 enum { CTL_ACTIVE = 0, CTL_BREAKDOWN = 1, CTL_ONE = 2, CTL_TRIGGER = 3, CTL_JUST = 4,
-       CTL_ITERS = 5, CTL_N = 8 };
-// CTL_JUST counts columns stopped by this iteration's k_reduce_beta: the
-// backward pass that follows still has to apply their last d += alpha p.
+       CTL_ITERS = 5, CTL_PAUSE = 6, CTL_N = 8 };
+__device__ __forceinline__ bool ctl_idle(const int32_t *ctl) {
+  return ctl[CTL_ACTIVE] == 0 || ctl[CTL_PAUSE] != 0;
+}
+// CTL_JUST counts columns stopped (or paused) by this iteration's
+// k_reduce_beta: the backward pass that follows still has to apply their
+// last d += alpha p.
 __device__ __forceinline__ bool bwd_idle(const int32_t *ctl) {
-  return ctl[CTL_ACTIVE] == 0 && ctl[CTL_JUST] == 0;
+  return ctl_idle(ctl) && ctl[CTL_JUST] == 0;
 }
 
 // ================================================================ kernels ==
@@ -73,7 +81,7 @@ __global__ void __launch_bounds__(kBlock)
 kn_spmm_dot(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
             const T *__restrict__ val, const T *__restrict__ P, T *__restrict__ Q,
             double *__restrict__ part, const int32_t *__restrict__ ctl) {
-  if (ctl[CTL_ACTIVE] == 0) return;
+  if (ctl_idle(ctl)) return;
   const int g = threadIdx.x % kGroup;
   double pd[1] = {0.0};
   NARROW_LOOP(row, 0, n) {
@@ -94,7 +102,7 @@ kw_spmm_dot(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restric
             const T *__restrict__ val, int64_t k, const T *__restrict__ P,
             T *__restrict__ Q, double *__restrict__ part,
             const int32_t *__restrict__ ctl) {
-  if (ctl[CTL_ACTIVE] == 0) return;
+  if (ctl_idle(ctl)) return;
   WIDE_COLS(VEC)
   double pd[1][VEC] = {};
   WIDE_LOOP(row, 0, n) {
@@ -175,7 +183,7 @@ kn_fwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
        const T *__restrict__ Q, T *__restrict__ R, T *Y, double *__restrict__ part,
        int64_t stride, const int32_t *__restrict__ ctl) {
   // one row per block: 8-lane groups, lanes stride over the row's entries
-  if (ctl[CTL_ACTIVE] == 0) return;
+  if (ctl_idle(ctl)) return;
   const int g = threadIdx.x % kGroup;
   double pd[2] = {0.0, 0.0};
   const T al = (FUSED && g < K) ? (T)alpha[g] : T(0);
@@ -229,7 +237,7 @@ kt_fwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
        const T *__restrict__ rdiag, const double *__restrict__ alpha,
        const T *__restrict__ Q, T *__restrict__ R, T *Y, double *__restrict__ part,
        int64_t stride, const int32_t *__restrict__ ctl) {
-  if (ctl[CTL_ACTIVE] == 0) return;
+  if (ctl_idle(ctl)) return;
   double pd[2][K] = {};
   T al[K];
 #pragma unroll
@@ -272,7 +280,7 @@ kw_fwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
        const T *__restrict__ rdiag, const double *__restrict__ alpha, int64_t k,
        const T *__restrict__ Q, T *__restrict__ R, T *Y, double *__restrict__ part,
        int64_t stride, const int32_t *__restrict__ ctl) {
-  if (ctl[CTL_ACTIVE] == 0) return;
+  if (ctl_idle(ctl)) return;
   WIDE_COLS(VEC)
   double pd[2][VEC] = {};
   T al[VEC];
@@ -545,7 +553,7 @@ kb_fwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
        const double *__restrict__ alpha, int64_t k, const T *__restrict__ Q,
        T *__restrict__ R, T *Y, double *__restrict__ part, int64_t stride,
        const int32_t *__restrict__ ctl) {
-  if (ctl[CTL_ACTIVE] == 0) return;
+  if (ctl_idle(ctl)) return;
   using SH = BlockShared<T, VEC, MB>;
   constexpr int TW = SH::TW, RPW = MB / kWaves, G = RPW < 4 ? RPW : 4;
   __shared__ __align__(16) unsigned char smem[SH::kBytes];
@@ -716,7 +724,7 @@ template <typename T, int K, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
 kn_rupd(int32_t n, const double *__restrict__ alpha, const T *__restrict__ Q,
         T *__restrict__ R, double *__restrict__ part, const int32_t *__restrict__ ctl) {
-  if (ctl[CTL_ACTIVE] == 0) return;
+  if (ctl_idle(ctl)) return;
   const int g = threadIdx.x % kGroup;
   double pd[1] = {0.0};
   const T al = (FUSED && g < K) ? (T)alpha[g] : T(0);
@@ -738,7 +746,7 @@ template <typename T, int VEC, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
 kw_rupd(int32_t n, const double *__restrict__ alpha, int64_t k, const T *__restrict__ Q,
         T *__restrict__ R, double *__restrict__ part, const int32_t *__restrict__ ctl) {
-  if (ctl[CTL_ACTIVE] == 0) return;
+  if (ctl_idle(ctl)) return;
   WIDE_COLS(VEC)
   double pd[1][VEC] = {};
   T al[VEC];
@@ -770,7 +778,7 @@ __global__ void __launch_bounds__(kBlock)
 kn_jac(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alpha,
        const T *__restrict__ Q, T *__restrict__ R, T *__restrict__ Z,
        double *__restrict__ part, int64_t stride, const int32_t *__restrict__ ctl) {
-  if (ctl[CTL_ACTIVE] == 0) return;
+  if (ctl_idle(ctl)) return;
   const int g = threadIdx.x % kGroup;
   double pd[2] = {0.0, 0.0};
   const T al = (FUSED && g < K) ? (T)alpha[g] : T(0);
@@ -796,7 +804,7 @@ __global__ void __launch_bounds__(kBlock)
 kw_jac(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alpha,
        int64_t k, const T *__restrict__ Q, T *__restrict__ R, T *__restrict__ Z,
        double *__restrict__ part, int64_t stride, const int32_t *__restrict__ ctl) {
-  if (ctl[CTL_ACTIVE] == 0) return;
+  if (ctl_idle(ctl)) return;
   WIDE_COLS(VEC)
   double pd[2][VEC] = {};
   T al[VEC];
@@ -910,7 +918,7 @@ __global__ void __launch_bounds__(kBlock)
 k_reduce_alpha(int32_t nrows, int64_t k, const double *__restrict__ part,
                double *__restrict__ st, int32_t *__restrict__ active,
                int32_t *__restrict__ ctl) {
-  const bool idle = ctl[CTL_ACTIVE] == 0;
+  const bool idle = ctl_idle(ctl);
   __syncthreads();  // every thread has read ctl before block 0 clears JUST
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl[CTL_JUST] = 0;
@@ -947,7 +955,7 @@ k_reduce_beta(int32_t nrows, int64_t k, const double *__restrict__ prr,
               const double *__restrict__ prz, double *__restrict__ st,
               int32_t *__restrict__ active, const int32_t *__restrict__ restart,
               int32_t *__restrict__ ctl, int mode, double delta2) {
-  if (ctl[CTL_ACTIVE] == 0) return;
+  if (ctl_idle(ctl)) return;
   __shared__ double sh[4][64];
   const int64_t c = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   double rr = col_reduce(prr, nrows, k, c, sh);
@@ -970,6 +978,10 @@ k_reduce_beta(int32_t nrows, int64_t k, const double *__restrict__ prr,
         gamma = 1.0;
         st[ST_RZ * k + c] = rz;
         if (rr < delta2 * st[ST_RREF * k + c]) atomicOr(&ctl[CTL_TRIGGER], 1);
+        if (rr < delta2 * delta2 * st[ST_RREF * k + c]) {
+          atomicOr(&ctl[CTL_PAUSE], 1);
+          atomicAdd(&ctl[CTL_JUST], 1);
+        }
       }
     }
     st[ST_BETA * k + c] = beta;
@@ -1699,6 +1711,11 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
       CN_HIP(hipMemcpyAsync(st + ST_TOL2 * k, hs + ST_TOL2 * k, sizeof(double) * 4 * k,
                             hipMemcpyHostToDevice, s));  // TOL2, RREF, S, INVS
     }
+    // Continuing columns keep p and the iterated r.z (van der Vorst-Ye
+    // reliable update): only a (re)start needs z = M^-1 r of the new r.
+    bool any_rst = false;
+    for (int32_t c = 0; c < k; ++c) any_rst = any_rst || rst[c];
+    const bool apply_m = first || any_rst;
     first = false;
     CN_HIP(hipMemcpyAsync(W->active.get(), act.data(), sizeof(int32_t) * k,
                           hipMemcpyHostToDevice, s));
@@ -1710,20 +1727,33 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     hc[CTL_TRIGGER] = 0;
     hc[CTL_JUST] = 0;
     hc[CTL_ITERS] = iters;
-    CN_HIP(hipMemcpyAsync(ctl, hc, sizeof(int32_t) * 6, hipMemcpyHostToDevice, s));
+    hc[CTL_PAUSE] = 0;
+    CN_HIP(hipMemcpyAsync(ctl, hc, sizeof(int32_t) * 7, hipMemcpyHostToDevice, s));
     CN_HIP(hipMemsetAsync(st + ST_ALPHA * k, 0, sizeof(double) * k, s));
     k_scale_in<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_S * k, R64, R);
-    // z = M^-1 r; rz; continuing columns keep p, restarted ones take p = z
-    rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * k, Q, R, Z, p1, stride, ctl, s);
-    rd = reduce_rows(W, p1, rows, k, stride, 2, s);
-    k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, rd.p + rd.stride, st,
-                                         W->active.get(), W->restart.get(), ctl, 1, delta2);
-    launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
+    if (apply_m) {
+      // z = M^-1 r; rz; continuing columns keep p, restarted ones take p = z
+      rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * k, Q, R, Z, p1, stride, ctl, s);
+      rd = reduce_rows(W, p1, rows, k, stride, 2, s);
+      k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, rd.p + rd.stride, st,
+                                           W->active.get(), W->restart.get(), ctl, 1, delta2);
+      launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
+    } else {
+      // new reference for the replacement trigger: the scaled true r.r
+      for (int32_t c = 0; c < k; ++c) {
+        const double sc = hs[ST_S * k + c];
+        hs[ST_RREF * k + c] = hs[ST_RR * k + c] * sc * sc;
+      }
+      CN_HIP(hipMemcpyAsync(st + ST_RREF * k, hs + ST_RREF * k, sizeof(double) * k,
+                            hipMemcpyHostToDevice, s));
+    }
     CN_HIP(hipGetLastError());
     // ---- CG iterations until every column stops or an update is due
     int32_t it = 0;
     const int32_t budget = o->max_iter - launched;
-    const int32_t every = std::max(1, o->check_every);
+    // a whole-apply preconditioner converges in a few iterations per
+    // residual replacement: read the flags often enough not to idle long
+    const int32_t every = std::max(1, H ? std::min(o->check_every, 2) : o->check_every);
     while (it < budget) {
       rows = launch_spmm_dot<T>(V, k, P, Q, p0, ctl, s);
       rd = reduce_rows(W, p0, rows, k, stride, 1, s);
@@ -1790,7 +1820,9 @@ static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   W->h_ctl[CTL_ACTIVE] = 1; W->h_ctl[CTL_BREAKDOWN] = 0; W->h_ctl[CTL_ONE] = 1;
   W->h_ctl[CTL_TRIGGER] = 0;
   W->h_ctl[CTL_JUST] = 0;
-  CN_HIP(hipMemcpyAsync(ctl, W->h_ctl, sizeof(int32_t) * 5, hipMemcpyHostToDevice, s));
+  W->h_ctl[CTL_ITERS] = 0;
+  W->h_ctl[CTL_PAUSE] = 0;
+  CN_HIP(hipMemcpyAsync(ctl, W->h_ctl, sizeof(int32_t) * 7, hipMemcpyHostToDevice, s));
   // preconditioner-only coefficients: alpha 0, beta 1, gamma 0 (p, d untouched)
   {
     std::vector<double> h(3 * (size_t)k);

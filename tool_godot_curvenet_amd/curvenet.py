@@ -357,7 +357,7 @@ class HSC:
     greedy independent-set elimination with exact Schur complements,
     sparsified fill, dense coarse inverse on the matrix cores."""
 
-    def __init__(self, system: "_System", theta: float = 0.02, coarse_max: int = 512,
+    def __init__(self, system: "_System", theta: float = 0.015, coarse_max: int = 512,
                  max_levels: int = 24, min_frac: float = 0.05, compensate: str = "lump"):
         lib = L.load()
         self.system = system
