@@ -33,7 +33,11 @@ TRAFFIC_KERNELS = {
     "spmm": "void cn::kw_spmm_dot<float, 4>",
     "fwd": "void cn::kb_fwd<float, 4, 16, true>",
     "bwd": "void cn::kb_bwd<float, 4, 16>",
+    "hsc_down": "void cn::kw_hsc_down<float, 4>",
+    "hsc_up": "void cn::kw_hsc_up<float, 4>",
+    "pd_update": "void cn::k_pd_update<float>",
 }
+FP64_MFMA_PEAK_TFLOPS = 78.6
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 
@@ -173,7 +177,8 @@ def main():
     torch.cuda.synchronize()
 
     lib.cn_profile_enable(1)
-    prof = {name: [0, 0.0] for name in ("spmm", "fwd", "bwd", "resid")}
+    prof = {name: [0, 0.0] for name in ("spmm", "fwd", "bwd", "resid", "hsc_down", "hsc_up",
+                                         "hsc_coarse")}
     iters, refines, relmax = 0, 0, 0.0
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -216,12 +221,46 @@ def main():
         "spmm": ("kw_spmm_dot<float,4>", prof["spmm"][0], prof["spmm"][1],
                  iters * spmm_bytes),
     }
+    coarse = None
     if args.precond == "ic0":
         nlev = pre.info.nlevels
         kern["fwd"] = ("kb_fwd<float,4,16>", prof["fwd"][0] * nlev, prof["fwd"][1],
                        (4 * iters + 2 * refines) * nk4)
         kern["bwd"] = ("kb_bwd<float,4,16>", prof["bwd"][0] * nlev, prof["bwd"][1],
                        6 * (iters + refines) * nk4)
+    else:
+        # HSC apply, per level l (n_F eliminated, n_C kept rows, k FP32 columns):
+        # down reads the kept rows and, once, the eliminated ones and writes the
+        # kept rows: 4k(2 n_C + n_F) plus R (8 B per entry, row pointers, row
+        # map); up reads and writes the eliminated rows and reads, once, the
+        # kept ones: 4k(2 n_F + n_C) plus P, the row map and 1/a_ff.
+        # k_pd_update (d += alpha p, p = z + beta p) streams 5 n x k vectors.
+        down_b = up_b = 0
+        for lv in pre.levels():
+            nF, nC = len(lv["F"]), len(lv["C"])
+            down_b += 4 * k * (2 * nC + nF) + 8 * len(lv["R"][1]) + 4 * (nC + 1) + 4 * nC
+            up_b += 4 * k * (2 * nF + nC) + 8 * len(lv["P"][1]) + 4 * (nF + 1) + 8 * nF
+        # Bytes are counted for the applies known to do work: one per active
+        # iteration and the first of each solve (a restart adds an uncounted
+        # one; launches during a pause return at once).  Time covers every
+        # launch, so the achieved rates are lower bounds.
+        nlev = max(pre.info.nlevels, 1)
+        napply = prof["hsc_down"][0]
+        nwork = iters + args.steps
+        kern["hsc_down"] = ("kw_hsc_down<float,4>", napply * nlev, prof["hsc_down"][1],
+                            nwork * down_b)
+        kern["hsc_up"] = ("kw_hsc_up<float,4>", napply * nlev, prof["hsc_up"][1],
+                          nwork * up_b)
+        kern["pd_update"] = ("k_pd_update<float>", prof["bwd"][0], prof["bwd"][1],
+                             nwork * 5 * nk4)
+        nc = pre.info.n_coarse
+        cms = prof["hsc_coarse"][1]
+        ncoarse = prof["hsc_coarse"][0]
+        tflops = 2.0 * nc * nc * k * nwork / (cms * 1e-3) / 1e12 if cms > 0 else 0.0
+        coarse = {"kernel": "k_gemm_f64 (coarse inverse x rows, FP64 MFMA)", "n_coarse": nc,
+                  "launches": ncoarse, "avg_ms": cms / max(ncoarse, 1),
+                  "achieved_tflops": tflops, "peak_tflops": FP64_MFMA_PEAK_TFLOPS,
+                  "frac": tflops / FP64_MFMA_PEAK_TFLOPS}
     kernels = {}
     for tag, (kname, launches, ms, nbytes) in kern.items():
         gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
@@ -278,7 +317,9 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
             tk = json.load(fh).get("kernels", {}).get(dom)
-        if tk and k == 768 and args.ordering == "block" and args.block_size == 16:
+        same = (args.precond == "hsc" and args.theta == 0.015) or \
+            (args.precond == "ic0" and args.ordering == "block" and args.block_size == 16)
+        if tk and k == 768 and same:
             traffic = tk["bytes"]
 
     line = {
@@ -304,6 +345,7 @@ def main():
                      "avg_ms": kernels[dom]["avg_ms"], "launches": kernels[dom]["launches"],
                      "bytes_per_launch": kernels[dom]["bytes_per_launch"]},
         "kernels": kernels,
+        "coarse_mfma": coarse,
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
         "kernel_ms_per_step": {k_: v[1] / args.steps for k_, v in prof.items()},

@@ -372,8 +372,10 @@ cn_status cn_adjoint_solve(cn_system sys, cn_ic0 ic0, const cn_pcg_opts *opts,
 /* Per-kernel timing of the last cn_pcg_solve on this thread, gathered with
  * HIP events around every launch of the named kernel when enabled.
  * name: "spmm" (FP32 SpMM of the inner loop), "fwd" / "bwd" (forward and
- * backward preconditioner sweeps), "resid" (FP64 residual).  Returns the
- * launches (sweeps) timed; *ms_total gets their summed duration. */
+ * backward preconditioner sweeps, or r -= alpha q + M^-1 r + r.z and the
+ * p / d update for HSC), "resid" (FP64 residual), "hsc_down" / "hsc_up" /
+ * "hsc_coarse" (the level passes and coarse product of every HSC apply).
+ * Returns the scopes timed; *ms_total gets their summed duration. */
 void cn_profile_enable(int32_t on);
 int32_t cn_profile_query(const char *name, double *ms_total);
 

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "cn_internal.h"
+#include "cn_kernels.h"
 
 namespace cn {
 
@@ -41,8 +42,10 @@ template <bool AT, bool BT, bool LOWER, typename TB, typename TC>
 __global__ void __launch_bounds__(256)
 k_gemm_f64(int m, int n, int kk, double alpha, const double *__restrict__ A, int64_t lda,
            const TB *__restrict__ B, int64_t ldb, double beta, TC *__restrict__ C,
-           int64_t ldc, const int32_t *__restrict__ crow = nullptr) {
+           int64_t ldc, const int32_t *__restrict__ crow = nullptr,
+           const int32_t *__restrict__ ctl = nullptr) {
   if (LOWER && blockIdx.x > blockIdx.y) return;
+  if (ctl && ctl_idle(ctl)) return;
   __shared__ double As[kGT][kGK + 1];
   __shared__ double Bs[kGK][kGT + 1];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
@@ -247,15 +250,16 @@ template void dense_gemm_nn<float, float>(int, int, const double *, const float 
 // Same product written to rows zrows[i] of Z.
 template <typename TB, typename TC>
 void dense_gemm_nn_rows(int m, int k, const double *M, const TB *X, int64_t ldx, TC *Z,
-                        int64_t ldz, const int32_t *zrows, hipStream_t s) {
+                        int64_t ldz, const int32_t *zrows, const int32_t *ctl, hipStream_t s) {
   k_gemm_f64<false, false, false, TB, TC><<<gemm_grid(m, k), 256, 0, s>>>(
-      m, k, m, 1.0, M, m, X, ldx, 0.0, Z, ldz, zrows);
+      m, k, m, 1.0, M, m, X, ldx, 0.0, Z, ldz, zrows, ctl);
 }
 template void dense_gemm_nn_rows<double, double>(int, int, const double *, const double *,
                                                  int64_t, double *, int64_t, const int32_t *,
-                                                 hipStream_t);
+                                                 const int32_t *, hipStream_t);
 template void dense_gemm_nn_rows<float, float>(int, int, const double *, const float *, int64_t,
-                                               float *, int64_t, const int32_t *, hipStream_t);
+                                               float *, int64_t, const int32_t *, const int32_t *,
+                                               hipStream_t);
 
 }  // namespace cn
 

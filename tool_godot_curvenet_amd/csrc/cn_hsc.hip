@@ -22,12 +22,13 @@
 
 #include "cn_internal.h"
 #include "cn_kernels.h"
+#include "cn_prof.h"
 
 namespace cn {
 
 template <typename TB, typename TC>
 void dense_gemm_nn_rows(int m, int k, const double *M, const TB *X, int64_t ldx, TC *Z,
-                        int64_t ldz, const int32_t *zrows, hipStream_t s);
+                        int64_t ldz, const int32_t *zrows, const int32_t *ctl, hipStream_t s);
 
 // ---- wide form: one wavefront per (row, 64 VEC column tile) ------------
 // Z[Cg[c]] = src[Cg[c]] + sum_e rval[e] src[rcol[e]]   (src = r or Z)
@@ -35,7 +36,8 @@ template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
 kw_hsc_down(int32_t nC, const int32_t *__restrict__ Cg, const int32_t *__restrict__ rptr,
             const int32_t *__restrict__ rcol, const T *__restrict__ rval, int64_t k,
-            const T *src, T *Z) {
+            const T *src, T *Z, const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
   const int lane = threadIdx.x & 63;
   const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
   const bool valid = c0 < k;
@@ -45,7 +47,9 @@ kw_hsc_down(int32_t nC, const int32_t *__restrict__ Cg, const int32_t *__restric
     const int64_t o = (int64_t)uniform(Cg[r]) * k + c0;
     T acc[VEC] = {};
     if (valid) VecIO<T, VEC>::ld(src + o, acc);
-    gather_wide<T, VEC>(uniform(rptr[r]), uniform(rptr[r + 1]), rcol, rval, src, k, c0, valid, acc);
+    // a kept row has ~2-3 eliminated neighbours: chunks of 4 gathers
+    gather_wide<T, VEC, 4>(uniform(rptr[r]), uniform(rptr[r + 1]), rcol, rval, src, k, c0, valid,
+                           acc);
     if (valid) VecIO<T, VEC>::st(Z + o, acc);
   }
 }
@@ -55,7 +59,9 @@ template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
 kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv,
           const int32_t *__restrict__ pptr, const int32_t *__restrict__ pcol,
-          const T *__restrict__ pval, int64_t k, const T *src, T *Z) {
+          const T *__restrict__ pval, int64_t k, const T *src, T *Z,
+          const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
   const int lane = threadIdx.x & 63;
   const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
   const bool valid = c0 < k;
@@ -79,7 +85,9 @@ kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv
 template <typename T>
 __global__ void k_hsc_down_s(int32_t nC, const int32_t *__restrict__ Cg,
                              const int32_t *__restrict__ rptr, const int32_t *__restrict__ rcol,
-                             const T *__restrict__ rval, int64_t k, const T *src, T *Z) {
+                             const T *__restrict__ rval, int64_t k, const T *src, T *Z,
+                             const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= (int64_t)nC * k) return;
   const int32_t r = (int32_t)(id / k);
@@ -93,7 +101,9 @@ __global__ void k_hsc_down_s(int32_t nC, const int32_t *__restrict__ Cg,
 template <typename T>
 __global__ void k_hsc_up_s(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv,
                            const int32_t *__restrict__ pptr, const int32_t *__restrict__ pcol,
-                           const T *__restrict__ pval, int64_t k, const T *src, T *Z) {
+                           const T *__restrict__ pval, int64_t k, const T *src, T *Z,
+                           const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= (int64_t)nF * k) return;
   const int32_t f = (int32_t)(id / k);
@@ -107,7 +117,9 @@ __global__ void k_hsc_up_s(int32_t nF, const int32_t *__restrict__ Fg, const T *
 // W[t] = src[rows[t]] (the coarse rows, gathered for the GEMM)
 template <typename T>
 __global__ void k_gather_rows(int32_t m, const int32_t *__restrict__ rows, int64_t k,
-                              const T *__restrict__ src, T *__restrict__ W) {
+                              const T *__restrict__ src, T *__restrict__ W,
+                              const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= (int64_t)m * k) return;
   const int32_t t = (int32_t)(id / k);
@@ -141,17 +153,18 @@ template <> struct LvVals<float> {
 };
 
 template <typename T>
-static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z, hipStream_t s) {
+static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
+                       const int32_t *ctl, hipStream_t s) {
   if (L.nC == 0) return;
   const int v = hsc_vec<T>(k);
   if (v == 0) {
     const int64_t tot = (int64_t)L.nC * k;
     k_hsc_down_s<T><<<grid_for(tot, 256), 256, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(),
-                                                     L.r_colg.get(), LvVals<T>::r(L), k, src, Z);
+                                                     L.r_colg.get(), LvVals<T>::r(L), k, src, Z, ctl);
     return;
   }
   const dim3 g = hsc_grid(L.nC, k, v);
-#define D_(VV) kw_hsc_down<T, VV><<<g, kBlock, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(), L.r_colg.get(), LvVals<T>::r(L), k, src, Z)
+#define D_(VV) kw_hsc_down<T, VV><<<g, kBlock, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(), L.r_colg.get(), LvVals<T>::r(L), k, src, Z, ctl)
   if constexpr (sizeof(T) == 4) {
     if (v == 4) { D_(4); return; }
   }
@@ -160,18 +173,19 @@ static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z, 
 }
 
 template <typename T>
-static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z, hipStream_t s) {
+static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
+                     const int32_t *ctl, hipStream_t s) {
   if (L.nF == 0) return;
   const int v = hsc_vec<T>(k);
   if (v == 0) {
     const int64_t tot = (int64_t)L.nF * k;
     k_hsc_up_s<T><<<grid_for(tot, 256), 256, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L),
                                                    L.p_ptr.get(), L.p_colg.get(),
-                                                   LvVals<T>::p(L), k, src, Z);
+                                                   LvVals<T>::p(L), k, src, Z, ctl);
     return;
   }
   const dim3 g = hsc_grid(L.nF, k, v);
-#define U_(VV) kw_hsc_up<T, VV><<<g, kBlock, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L), L.p_ptr.get(), L.p_colg.get(), LvVals<T>::p(L), k, src, Z)
+#define U_(VV) kw_hsc_up<T, VV><<<g, kBlock, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L), L.p_ptr.get(), L.p_colg.get(), LvVals<T>::p(L), k, src, Z, ctl)
   if constexpr (sizeof(T) == 4) {
     if (v == 4) { U_(4); return; }
   }
@@ -181,7 +195,7 @@ static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z, hi
 
 // z = M^-1 r (n0 x k), r and z distinct.
 template <typename T>
-void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s) {
+void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const int32_t *ctl) {
   const int32_t L = (int32_t)H->lv.size();
   const size_t bytes = sizeof(T) * (size_t)H->nc * k + 256;
   if (H->work_bytes < bytes) {
@@ -189,18 +203,27 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s) {
     H->work_bytes = bytes;
   }
   T *W = reinterpret_cast<T *>(H->work.get());
-  for (int32_t l = 0; l < L; ++l) level_down<T>(H->lv[l], k, l == 0 ? r : z, z, s);
+  {
+    ProfScope ps("hsc_down", s);
+    for (int32_t l = 0; l < L; ++l) level_down<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s);
+  }
   if (H->nc > 0) {
+    ProfScope ps("hsc_coarse", s);
     const int64_t tot = (int64_t)H->nc * k;
     k_gather_rows<T><<<grid_for(tot, 256), 256, 0, s>>>(H->nc, H->coarse_rows.get(), k,
-                                                      L == 0 ? r : z, W);
+                                                      L == 0 ? r : z, W, ctl);
     dense_gemm_nn_rows<T, T>(H->nc, (int)k, H->coarse_inv.get(), W, k, z, k,
-                             H->coarse_rows.get(), s);
+                             H->coarse_rows.get(), ctl, s);
   }
-  for (int32_t l = L - 1; l >= 0; --l) level_up<T>(H->lv[l], k, l == 0 ? r : z, z, s);
+  {
+    ProfScope ps("hsc_up", s);
+    for (int32_t l = L - 1; l >= 0; --l) level_up<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s);
+  }
 }
-template void hsc_apply<float>(cn_hsc_t *, int64_t, const float *, float *, hipStream_t);
-template void hsc_apply<double>(cn_hsc_t *, int64_t, const double *, double *, hipStream_t);
+template void hsc_apply<float>(cn_hsc_t *, int64_t, const float *, float *, hipStream_t,
+                               const int32_t *);
+template void hsc_apply<double>(cn_hsc_t *, int64_t, const double *, double *, hipStream_t,
+                                const int32_t *);
 
 }  // namespace cn
 
@@ -215,7 +238,7 @@ cn_status cn_hsc_apply(cn_hsc H, int32_t k, const double *d_r, double *d_z, void
     return CN_BAD_ARG;
   }
   hipStream_t s = as_stream(stream);
-  if (H->n0 > 0) hsc_apply<double>(H, k, d_r, d_z, s);
+  if (H->n0 > 0) hsc_apply<double>(H, k, d_r, d_z, s, nullptr);
   CN_HIP(hipGetLastError());
   CN_HIP(hipStreamSynchronize(s));
   return CN_OK;

@@ -36,6 +36,18 @@ __device__ __forceinline__ int32_t uniform(int32_t x) {
 // that slides forward, so a row's neighbours -- the x rows it gathers, a
 // few ring-lengths away in mesh order -- are still in that XCD's L2.
 constexpr int kXcd = 8;
+// Solver control block (device), read by every kernel of the PCG loop.
+// CTL_ITERS counts the iterations in which some column was still active.
+// CTL_PAUSE stops the iteration until the host's next residual replacement:
+// set when a column's iterated residual has fallen delta^4 below the last
+// true one (far past the delta^2 trigger), so a fast-converging solve does
+// not run on in FP32 between two host checks.
+enum { CTL_ACTIVE = 0, CTL_BREAKDOWN = 1, CTL_ONE = 2, CTL_TRIGGER = 3, CTL_JUST = 4,
+       CTL_ITERS = 5, CTL_PAUSE = 6, CTL_N = 8 };
+__device__ __forceinline__ bool ctl_idle(const int32_t *ctl) {
+  return ctl[CTL_ACTIVE] == 0 || ctl[CTL_PAUSE] != 0;
+}
+
 struct RowIter {
   int32_t first, end, stride;
 };

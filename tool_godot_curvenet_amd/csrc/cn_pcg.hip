@@ -36,6 +36,7 @@
 
 #include "cn_internal.h"
 #include "cn_kernels.h"
+#include "cn_prof.h"
 
 namespace cn {
 
@@ -45,16 +46,6 @@ enum {
   ST_BB, ST_RR, ST_TMP, ST_N
 };
 // control block
-// CTL_ITERS counts the iterations in which some column was still active.
-// CTL_PAUSE stops the iteration until the host's next residual replacement:
-// set when a column's iterated residual has fallen delta^4 below the last
-// true one (far past the delta^2 trigger), so a fast-converging solve does
-// not run on in FP32 between two host checks.
-enum { CTL_ACTIVE = 0, CTL_BREAKDOWN = 1, CTL_ONE = 2, CTL_TRIGGER = 3, CTL_JUST = 4,
-       CTL_ITERS = 5, CTL_PAUSE = 6, CTL_N = 8 };
-__device__ __forceinline__ bool ctl_idle(const int32_t *ctl) {
-  return ctl[CTL_ACTIVE] == 0 || ctl[CTL_PAUSE] != 0;
-}
 // CTL_JUST counts columns stopped (or paused) by this iteration's
 // k_reduce_beta: the backward pass that follows still has to apply their
 // last d += alpha p.
@@ -1061,40 +1052,7 @@ __global__ void k_dinv(int32_t n, const double *__restrict__ d, double *__restri
 }
 
 // ============================================================== profiling ==
-struct Prof {
-  bool on = false;
-  std::vector<hipEvent_t> pool;
-  struct Rec { std::string name; hipEvent_t a, b; };
-  std::vector<Rec> recs;
-  size_t used = 0;
-  std::map<std::string, std::pair<int, double>> result;
-};
-static thread_local Prof g_prof;
-
-struct ProfScope {
-  int idx = -1;
-  hipStream_t s;
-  ProfScope(const char *name, hipStream_t st) : s(st) {
-    Prof &P = g_prof;
-    if (!P.on) return;
-    if (P.used + 2 > P.pool.size()) {
-      if (P.pool.size() >= 65536) return;
-      size_t add = std::max<size_t>(256, P.pool.size());
-      for (size_t i = 0; i < add; ++i) {
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
-        P.pool.push_back(e);
-      }
-    }
-    hipEvent_t a = P.pool[P.used++], b = P.pool[P.used++];
-    P.recs.push_back({name, a, b});
-    idx = (int)P.recs.size() - 1;
-    (void)hipEventRecord(a, s);
-  }
-  ~ProfScope() {
-    if (idx >= 0) (void)hipEventRecord(g_prof.recs[idx].b, s);
-  }
-};
+thread_local Prof g_prof;
 
 static void prof_reset() {
   g_prof.recs.clear();
@@ -1312,7 +1270,7 @@ static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
 }
 
 template <typename T>
-void hsc_apply(cn_hsc_t *H, int64_t k, const T *r0, T *z0, hipStream_t s);
+void hsc_apply(cn_hsc_t *H, int64_t k, const T *r0, T *z0, hipStream_t s, const int32_t *ctl);
 template <typename T>
 static int32_t launch_dot(int32_t n, int64_t k, const T *A, const T *B, double *part,
                           hipStream_t s);
@@ -1427,7 +1385,7 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
     DISPATCH_K(k, T, N_, W_);
 #undef N_
 #undef W_
-    hsc_apply<T>(V.hsc, k, R, Z, s);
+    hsc_apply<T>(V.hsc, k, R, Z, s, ctl);
     launch_dot<T>(V.n, k, R, Z, part + stride, s);
     return G.rows;
   }
