@@ -1612,7 +1612,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   T *Q = reinterpret_cast<T *>(W->buf[4].get());
   double *X = W->xp.get(), *R64 = W->rp.get();
   const double theta = 0.5;  // iterated-residual target below tol
-  const double delta2 = sizeof(T) == 4 ? 1e-2 : 1e-8;
+  const double delta2 = sizeof(T) == 4 ? 1e-3 : 1e-8;
 
   // x0, true residual, ||b||
   if (o->warm_start)
