@@ -113,34 +113,40 @@ kw_spmm_dot(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restric
 }
 
 // ---- FP64 residual r = b - A x with partial r.r ------------------------
-template <int K>
+// R = b - A x, or (scale != nullptr) R = scale[col] (b - A x) in the
+// inner-loop type TO; partial r.r of the unscaled residual.
+template <int K, typename TO>
 __global__ void __launch_bounds__(kBlock)
 kn_resid(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
          const double *__restrict__ val, const double *__restrict__ X,
-         const double *__restrict__ B, double *__restrict__ R,
+         const double *__restrict__ B, TO *__restrict__ R, const double *__restrict__ scale,
          double *__restrict__ part) {
   const int g = threadIdx.x % kGroup;
   double pd[1] = {0.0};
+  const double sc = (scale && g < K) ? scale[g] : 1.0;
   NARROW_LOOP(row, 0, n) {
     double acc[K] = {};
     gather_narrow<double, K>(ptr[row], ptr[row + 1], g, col, val, X, acc);
     if (g < K) {
       double r = B[row * K + g] - pick<double, K>(acc, g);
-      R[row * K + g] = r;
+      R[row * K + g] = (TO)(scale ? r * sc : r);
       pd[0] += r * r;
     }
   }
   block_partials_narrow<K, 1>(pd, part, 0);
 }
 
-template <int VEC>
+template <int VEC, typename TO>
 __global__ void __launch_bounds__(kBlock)
 kw_resid(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
          const double *__restrict__ val, int64_t k, const double *__restrict__ X,
-         const double *__restrict__ B, double *__restrict__ R,
+         const double *__restrict__ B, TO *__restrict__ R, const double *__restrict__ scale,
          double *__restrict__ part) {
   WIDE_COLS(VEC)
   double pd[1][VEC] = {};
+  double sc[VEC];
+#pragma unroll
+  for (int m = 0; m < VEC; ++m) sc[m] = (scale && valid) ? scale[c0 + m] : 1.0;
   WIDE_LOOP(row, 0, n) {
     const int32_t r = uniform(row);
     const int32_t a = uniform(ptr[r]), b = uniform(ptr[r + 1]);
@@ -148,13 +154,15 @@ kw_resid(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__
     gather_wide<double, VEC>(a, b, col, val, X, k, c0, valid, acc);
     if (valid) {
       double bb[VEC];
+      TO out[VEC];
       VecIO<double, VEC>::ld(B + (int64_t)r * k + c0, bb);
 #pragma unroll
       for (int m = 0; m < VEC; ++m) {
         acc[m] = bb[m] - acc[m];
         pd[0][m] += acc[m] * acc[m];
+        out[m] = (TO)(scale ? acc[m] * sc[m] : acc[m]);
       }
-      VecIO<double, VEC>::st(R + (int64_t)r * k + c0, acc);
+      VecIO<TO, VEC>::st(R + (int64_t)r * k + c0, out);
     }
   }
   block_partials_wide<VEC, 1>(pd, part, k, c0, 0);
@@ -1256,13 +1264,15 @@ static int32_t launch_spmm_dot(const SysView<T> &V, int64_t k, const T *P, T *Q,
   return G.rows;
 }
 
+template <typename TO>
 static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
                             const double *val, int64_t k, const double *X,
-                            const double *B, double *R, double *part, hipStream_t s) {
+                            const double *B, TO *R, const double *scale, double *part,
+                            hipStream_t s) {
   Grid G = grid_rows<double>(n, k, kCapSpmm);
   ProfScope ps("resid", s);
-#define N_(KK) kn_resid<KK><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, X, B, R, part)
-#define W_(VV) kw_resid<WideV<double, VV>::v><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, k, X, B, R, part)
+#define N_(KK) kn_resid<KK, TO><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, X, B, R, scale, part)
+#define W_(VV) kw_resid<WideV<double, VV>::v, TO><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, k, X, B, R, scale, part)
   DISPATCH_K(k, double, N_, W_);
 #undef N_
 #undef W_
@@ -1622,7 +1632,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   int32_t rows = launch_dot<double>(n, k, d_b, d_b, p0, s);
   Red rd = reduce_rows(W, p0, rows, k, stride, 1, s);
   k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_BB * k);
-  rows = launch_resid(n, V.ptr, V.col, V.val64, k, X, d_b, R64, p0, s);
+  rows = launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
   rd = reduce_rows(W, p0, rows, k, stride, 1, s);
   k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
   CN_HIP(hipMemsetAsync(D, 0, sizeof(T) * total, s));
@@ -1631,6 +1641,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
 
   std::vector<int32_t> act(k, 0), rst(k, 0);
   int32_t iters = 0, launched = 0, updates = 0;
+  bool r_scaled = false;  // R already holds the scaled residual of this round
   cn_status status = CN_OK;
   double relmax = 0.0;
   bool first = true;
@@ -1688,7 +1699,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     hc[CTL_PAUSE] = 0;
     CN_HIP(hipMemcpyAsync(ctl, hc, sizeof(int32_t) * 7, hipMemcpyHostToDevice, s));
     CN_HIP(hipMemsetAsync(st + ST_ALPHA * k, 0, sizeof(double) * k, s));
-    k_scale_in<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_S * k, R64, R);
+    // after the first round the residual kernel writes the scaled R itself
+    if (!r_scaled) k_scale_in<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_S * k, R64, R);
     if (apply_m) {
       // z = M^-1 r; rz; continuing columns keep p, restarted ones take p = z
       rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * k, Q, R, Z, p1, stride, ctl, s);
@@ -1733,7 +1745,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     iters = hc[CTL_ITERS];
     // ---- reliable update: x += d / s (FP64), r = b - A x (FP64)
     k_x_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_INVS * k, D, X);
-    rows = launch_resid(n, V.ptr, V.col, V.val64, k, X, d_b, R64, p0, s);
+    rows = launch_resid<T>(n, V.ptr, V.col, V.val64, k, X, d_b, R, st + ST_S * k, p0, s);
+    r_scaled = true;
     rd = reduce_rows(W, p0, rows, k, stride, 1, s);
     k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
     CN_HIP(hipGetLastError());
@@ -1803,7 +1816,7 @@ static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   };
   if (o->warm_start) {
     CN_HIP(hipMemcpyAsync(X, d_x, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
-    launch_resid(n, V.ptr, V.col, V.val64, k, X, d_b, R, W->part.get(), s);
+    launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R, nullptr, W->part.get(), s);
   } else {
     CN_HIP(hipMemsetAsync(X, 0, sizeof(double) * total, s));
     CN_HIP(hipMemcpyAsync(R, d_b, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
@@ -1852,7 +1865,8 @@ static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
                           hipMemcpyHostToDevice, s));
     k_p_update<double><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_TMP * k, Z, P);
   }
-  int32_t rows = launch_resid(n, V.ptr, V.col, V.val64, k, X, d_b, R, W->part.get(), s);
+  int32_t rows = launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R, nullptr,
+                                      W->part.get(), s);
   Red rdf = reduce_rows(W, W->part.get(), rows, k, stride, 1, s);
   k_reduce_plain<<<red, kBlock, 0, s>>>(rdf.rows, k, rdf.p, st + ST_TMP * k);
   CN_HIP(hipMemcpyAsync(tmp.data(), st + ST_TMP * k, sizeof(double) * k,
