@@ -384,3 +384,70 @@ def test_project_to_vertices(gpu):
     q = np.random.default_rng(3).uniform(-1, 1, (200, 3))
     q[:5] = m.pos[[0, 5, 5, 77, 4000]]                 # exact hits incl. repeats
     assert np.array_equal(project_to_vertices(q, m.pos), O.project_to_vertices(q, m.pos))
+
+
+def _hub_matrix(n, seed):
+    """SPD graph Laplacian + 0.01 I: a ring with +-1, +-2 neighbours and 24
+    hub vertices joined to 10-20 random others, so some rows hold up to ~25
+    entries and most hold 5."""
+    from tool_godot_curvenet_amd import CsrMatrix
+    rng = np.random.default_rng(seed)
+    W = {}
+    for i in range(n):
+        for d in (1, 2):
+            j = (i + d) % n
+            W[(min(i, j), max(i, j))] = rng.uniform(0.5, 2.0)
+    for h in rng.choice(n, 24, replace=False):
+        for j in rng.choice(n, rng.integers(10, 21), replace=False):
+            if j != h:
+                W[(min(h, j), max(h, j))] = rng.uniform(0.5, 2.0)
+    rows = [dict() for _ in range(n)]
+    for (i, j), w in W.items():
+        rows[i][j] = -w
+        rows[j][i] = -w
+        rows[i][i] = rows[i].get(i, 0.0) + w
+        rows[j][j] = rows[j].get(j, 0.0) + w
+    ptr, col, val = [0], [], []
+    for i in range(n):
+        rows[i][i] += 0.01
+        for j in sorted(rows[i]):
+            col.append(j)
+            val.append(rows[i][j])
+        ptr.append(len(col))
+    return CsrMatrix(n, n, np.array(ptr, np.int32), np.array(col, np.int32),
+                     np.array(val, np.float64))
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64", "fp64_plain"])
+@pytest.mark.parametrize("k", [3, 100])
+def test_pcg_csr_long_rows(gpu, precision, k):
+    """Rows longer than the multi-column SpMM's 8-wide row table take its
+    overflow entries; the FP64 residual (plain CSR) checks the result and the
+    iteration count stays at the oracle's FP64 Jacobi-PCG count."""
+    from tool_godot_curvenet_amd import CsrSystem
+    A = _hub_matrix(3000, 7)
+    assert np.diff(A.row_ptr).max() > 16
+    B = np.random.default_rng(k).normal(size=(A.rows, k))
+    x, info = CsrSystem(A).solve(B, precond="jacobi", precision=precision, tol=1e-9)
+    assert info.status == 0
+    assert np.all(_relres(A, x.cpu().numpy(), B) <= 1e-9)
+    _, it = O.pcg(A.row_ptr, A.col_idx, A.values, B[:, 0], 1e-9 * np.linalg.norm(B[:, 0]))
+    # FP64 follows the oracle's recurrence; the FP32 inner loop of "mixed"
+    # costs some iterations on this poorly conditioned matrix (kappa ~ 1e4)
+    lo, hi = (0.9, 1.1) if precision != "mixed" else (0.9, 1.3)
+    assert lo * it - 3 <= info.iters <= hi * it + 3, (info.iters, it)
+
+
+def test_solve_after_set_positions(gpu):
+    """New positions change the matrix values; the next solve uses them."""
+    from tool_godot_curvenet_amd import meshgen as G
+    m, lap = _system("cylinder64")
+    B0 = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, 96, seed=3))
+    lap.solve(B0, precond="jacobi", precision="mixed", tol=1e-9)
+    rng = np.random.default_rng(11)
+    lap.set_positions(lap.pos * rng.uniform(0.8, 1.25, size=(1, 3)))
+    A = lap.A_ff()
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, 96, seed=4))
+    x, info = lap.solve(B, precond="jacobi", precision="mixed", tol=1e-9)
+    assert info.status == 0
+    assert np.all(_relres(A, x.cpu().numpy(), B) <= 1e-9)

@@ -41,16 +41,20 @@ kw_hsc_down(int32_t nC, const int32_t *__restrict__ Cg, const int32_t *__restric
   const int lane = threadIdx.x & 63;
   const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
   const bool valid = c0 < k;
+  // loads and their uses are unconditional (lanes past k read the last VEC
+  // columns), only the store is not: a load consumed under a branch makes
+  // the compiler wait for the previous row's store before the next row
+  const int64_t cl = valid ? c0 : k - VEC;
   const RowIter ri = row_iter(0, nC, kWaves, threadIdx.x >> 6);
   for (int32_t row = ri.first; row < ri.end; row += ri.stride) {
     const int32_t r = uniform(row);
-    const int64_t o = (int64_t)uniform(Cg[r]) * k + c0;
-    T acc[VEC] = {};
-    if (valid) VecIO<T, VEC>::ld(src + o, acc);
+    const int64_t g = (int64_t)uniform(Cg[r]) * k;
+    T acc[VEC];
+    VecIO<T, VEC>::ld(src + g + cl, acc);
     // a kept row has ~2-3 eliminated neighbours: chunks of 4 gathers
-    gather_wide<T, VEC, 4>(uniform(rptr[r]), uniform(rptr[r + 1]), rcol, rval, src, k, c0, valid,
+    gather_wide<T, VEC, 4>(uniform(rptr[r]), uniform(rptr[r + 1]), rcol, rval, src, k, cl, true,
                            acc);
-    if (valid) VecIO<T, VEC>::st(Z + o, acc);
+    if (valid) VecIO<T, VEC>::st(Z + g + c0, acc);
   }
 }
 
@@ -65,19 +69,18 @@ kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv
   const int lane = threadIdx.x & 63;
   const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
   const bool valid = c0 < k;
+  const int64_t cl = valid ? c0 : k - VEC;   // as in kw_hsc_down
   const RowIter ri = row_iter(0, nF, kWaves, threadIdx.x >> 6);
   for (int32_t row = ri.first; row < ri.end; row += ri.stride) {
     const int32_t f = uniform(row);
-    const int64_t o = (int64_t)uniform(Fg[f]) * k + c0;
+    const int64_t g = (int64_t)uniform(Fg[f]) * k;
     const T d = dinv[f];
-    T acc[VEC] = {};
-    if (valid) {
-      VecIO<T, VEC>::ld(src + o, acc);
+    T acc[VEC];
+    VecIO<T, VEC>::ld(src + g + cl, acc);
 #pragma unroll
-      for (int m = 0; m < VEC; ++m) acc[m] = d * acc[m];
-    }
-    gather_wide<T, VEC>(uniform(pptr[f]), uniform(pptr[f + 1]), pcol, pval, Z, k, c0, valid, acc);
-    if (valid) VecIO<T, VEC>::st(Z + o, acc);
+    for (int m = 0; m < VEC; ++m) acc[m] = d * acc[m];
+    gather_wide<T, VEC>(uniform(pptr[f]), uniform(pptr[f + 1]), pcol, pval, Z, k, cl, true, acc);
+    if (valid) VecIO<T, VEC>::st(Z + g + c0, acc);
   }
 }
 

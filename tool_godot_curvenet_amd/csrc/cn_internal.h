@@ -136,6 +136,14 @@ struct cn_system_t {
   // free row, so the values are read from fc_val and never go stale
   cn::DevBuf<int32_t> cf_ptr, cf_row, cf_map;
   bool cf_ready = false;
+  // A_ff as a fixed-width row table for the solver's multi-column SpMM:
+  // the first kEll entries of every row (short rows padded with their last
+  // column and weight 0), the rest in an overflow CSR.  The pattern is built
+  // on first use (ov_nnz = -1 until then), the values at every solve.
+  cn::DevBuf<int32_t> ell_col, ov_ptr, ov_col;
+  cn::DevBuf<float> ell_val32, ov_val32;
+  cn::DevBuf<double> ell_val64, ov_val64;
+  int64_t ov_nnz = -1;
   cn::SolverWork *work = nullptr;   // solver workspace, grown on demand
   ~cn_system_t() { cn::free_solver_work(work); }
 };

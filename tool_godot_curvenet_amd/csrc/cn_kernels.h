@@ -36,6 +36,9 @@ __device__ __forceinline__ int32_t uniform(int32_t x) {
 // that slides forward, so a row's neighbours -- the x rows it gathers, a
 // few ring-lengths away in mesh order -- are still in that XCD's L2.
 constexpr int kXcd = 8;
+// width of the solver SpMM's row table (diagonal + 6 neighbours covers a
+// regular triangle mesh)
+constexpr int kEll = 7;
 // Solver control block (device), read by every kernel of the PCG loop.
 // CTL_ITERS counts the iterations in which some column was still active.
 // CTL_PAUSE stops the iteration until the host's next residual replacement:
@@ -60,6 +63,22 @@ __device__ __forceinline__ RowIter row_iter(int32_t n0, int32_t n1, int32_t slot
   const int32_t chunk = (n1 - n0 + kXcd - 1) / kXcd;
   const int32_t cs = n0 + x * chunk;
   return RowIter{cs + lb * slots + slot, min(n1, cs + chunk), nbx * slots};
+}
+
+// row_iter with each XCD's share cut into `gens` stripes: the grid holds
+// gens * (resident blocks) blocks, generation g of an XCD's blocks walks
+// stripe g, and generations start in dispatch order.
+__device__ __forceinline__ RowIter row_iter_gen(int32_t n0, int32_t n1, int32_t slots,
+                                                int32_t slot, int32_t gens) {
+  const int32_t nb = gridDim.x;
+  if (gens <= 1 || nb % (kXcd * gens) != 0) return row_iter(n0, n1, slots, slot);
+  const int32_t x = blockIdx.x % kXcd, lb = blockIdx.x / kXcd, nbx = nb / kXcd;
+  const int32_t F = nbx / gens, gi = lb / F, lf = lb % F;
+  const int32_t chunk = (n1 - n0 + kXcd - 1) / kXcd;
+  const int32_t cs = n0 + x * chunk, ce = min(n1, cs + chunk);
+  const int32_t stripe = (chunk + gens - 1) / gens;
+  const int32_t ss = cs + gi * stripe;
+  return RowIter{ss + lf * slots + slot, min(ce, ss + stripe), F * slots};
 }
 
 template <typename T, int VEC>

@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -87,29 +88,113 @@ kn_spmm_dot(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restric
   block_partials_narrow<K, 1>(pd, part, 0);
 }
 
+// Wide form over the row table: each wave walks its rows with the next
+// row's kEll columns and weights (scalar loads) fetched while the current
+// row's gathers are in flight, so a row costs one memory round trip.  Slot 0
+// of every row is its own column, so the gather of P[r] also serves p.q.
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
-kw_spmm_dot(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-            const T *__restrict__ val, int64_t k, const T *__restrict__ P,
+kw_spmm_dot(int32_t n, const int32_t *__restrict__ ecol, const T *__restrict__ eval,
+            const int32_t *__restrict__ ovp, const int32_t *__restrict__ ovc,
+            const T *__restrict__ ovv, int64_t k, const T *__restrict__ P,
             T *__restrict__ Q, double *__restrict__ part,
-            const int32_t *__restrict__ ctl) {
+            const int32_t *__restrict__ ctl, int32_t gens) {
   if (ctl_idle(ctl)) return;
   WIDE_COLS(VEC)
   double pd[1][VEC] = {};
-  WIDE_LOOP(row, 0, n) {
-    const int32_t r = uniform(row);
-    const int32_t a = uniform(ptr[r]), b = uniform(ptr[r + 1]);
-    T acc[VEC] = {};
-    gather_wide<T, VEC>(a, b, col, val, P, k, c0, valid, acc);
-    if (valid) {
-      T p[VEC];
-      VecIO<T, VEC>::ld(P + (int64_t)r * k + c0, p);
-      VecIO<T, VEC>::st(Q + (int64_t)r * k + c0, acc);
+  const RowIter it = row_iter_gen(0, n, kWaves, threadIdx.x >> 6, gens);
+  const int32_t end = it.end, stride = uniform(it.stride);
+  int32_t r = uniform(it.first);
+  int32_t j[kEll], oa = 0, ob = 0;
+  T v[kEll];
+  if (r < end) {
 #pragma unroll
-      for (int m = 0; m < VEC; ++m) pd[0][m] += (double)p[m] * (double)acc[m];
+    for (int u = 0; u < kEll; ++u) {
+      j[u] = ecol[(int64_t)r * kEll + u];
+      v[u] = eval[(int64_t)r * kEll + u];
     }
+    if (ovp) { oa = ovp[r]; ob = ovp[r + 1]; }
+  }
+  // Loads and their uses are unconditional (lanes past k read the last VEC
+  // columns); only the store depends on `valid`.  A load consumed under a
+  // branch makes the compiler wait for the previous row's store at the top
+  // of every iteration.
+  const int64_t cl = valid ? c0 : k - VEC;
+  while (r < end) {
+    T x[kEll][VEC];
+#pragma unroll
+    for (int u = 0; u < kEll; ++u) VecIO<T, VEC>::ld(P + (int64_t)j[u] * k + cl, x[u]);
+    const int32_t rn = r + stride;
+    int32_t jn[kEll], oan = 0, obn = 0;
+    T vn[kEll];
+    if (rn < end) {
+#pragma unroll
+      for (int u = 0; u < kEll; ++u) {
+        jn[u] = ecol[(int64_t)rn * kEll + u];
+        vn[u] = eval[(int64_t)rn * kEll + u];
+      }
+      if (ovp) { oan = ovp[rn]; obn = ovp[rn + 1]; }
+    }
+    T acc[VEC] = {};
+#pragma unroll
+    for (int u = 0; u < kEll; ++u)
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) acc[m] = fma(v[u], x[u][m], acc[m]);
+    if (oa < ob) gather_wide<T, VEC, 2>(oa, ob, ovc, ovv, P, k, cl, true, acc);
+    // lanes past k accumulate too; their partials are never written
+#pragma unroll
+    for (int m = 0; m < VEC; ++m) pd[0][m] += (double)x[0][m] * (double)acc[m];
+    if (valid) VecIO<T, VEC>::st(Q + (int64_t)r * k + c0, acc);
+#pragma unroll
+    for (int u = 0; u < kEll; ++u) { j[u] = jn[u]; v[u] = vn[u]; }
+    oa = oan;
+    ob = obn;
+    r = rn;
   }
   block_partials_wide<VEC, 1>(pd, part, k, c0, 0);
+}
+
+// Row table of A_ff.  Row r lists its diagonal first (column r, weight 0
+// if the row stores none), then its other entries in column order; slots
+// past the row's length repeat its last column with weight 0, and entries
+// past kEll go to the overflow CSR in the same order.
+__global__ void k_ell_fill(int32_t n, const int32_t *__restrict__ ptr,
+                           const int32_t *__restrict__ col, const double *__restrict__ val,
+                           int32_t *__restrict__ ecol, float *__restrict__ ev32,
+                           double *__restrict__ ev64, const int32_t *__restrict__ ovp,
+                           int32_t *__restrict__ ovc, float *__restrict__ ov32,
+                           double *__restrict__ ov64) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int32_t a = ptr[r], b = ptr[r + 1];
+  double d = 0.0;
+  for (int32_t q = a; q < b; ++q)
+    if (col[q] == r) d = val[q];
+  const int64_t o = (int64_t)r * kEll;
+  ecol[o] = r;
+  ev64[o] = d;
+  ev32[o] = (float)d;
+  int32_t u = 1, last = r, ov = ovp ? ovp[r] : 0;
+  for (int32_t q = a; q < b; ++q) {
+    if (col[q] == r) continue;
+    if (u < kEll) {
+      ecol[o + u] = col[q];
+      ev64[o + u] = val[q];
+      ev32[o + u] = (float)val[q];
+      ++u;
+    } else {
+      ovc[ov] = col[q];
+      ov64[ov] = val[q];
+      ov32[ov] = (float)val[q];
+      ++ov;
+    }
+    last = col[q];
+  }
+  for (; u < kEll; ++u) {
+    ecol[o + u] = last;
+    ev64[o + u] = 0.0;
+    ev32[o + u] = 0.0f;
+  }
 }
 
 // ---- FP64 residual r = b - A x with partial r.r ------------------------
@@ -1246,18 +1331,38 @@ struct SysView {
   const T *lval, *uval, *rdiag;
   const T *dinv;
   cn_hsc_t *hsc = nullptr;  // whole-apply preconditioner (replaces ic0 / dinv)
+  // row table of A_ff (SpMM); ovp = nullptr when no row overflows it
+  const int32_t *ecol = nullptr, *ovp = nullptr, *ovc = nullptr;
+  const T *eval = nullptr, *ovv = nullptr;
   // block sweeps (forward: L, backward: U); mb = 0 when not available
   int32_t mb = 0;
   SweepView<T> lsw{}, usw{};
 };
 
+// Row stripes per XCD share in the wide SpMM: the grid holds this many
+// generations of resident blocks, so the rows in flight on an XCD stay
+// within one stripe and its L2 serves the neighbour gathers (measured: L2
+// misses 705 MB -> 431 MB per launch on the 100k capsule mesh at k = 768).
+static const unsigned kSpmmGens = 4;
+
+template <typename T>
+static Grid grid_spmm(int64_t n, int64_t k) {
+  Grid G = grid_rows<T>(n, k, kCapSpmm);
+  if (k > 4 && G.g.x % kXcd == 0) {
+    G.g.x *= kSpmmGens;
+    G.rows = (int32_t)G.g.x;
+  }
+  return G;
+}
+
 template <typename T>
 static int32_t launch_spmm_dot(const SysView<T> &V, int64_t k, const T *P, T *Q,
                                double *part, const int32_t *ctl, hipStream_t s) {
-  Grid G = grid_rows<T>(V.n, k, kCapSpmm);
+  Grid G = grid_spmm<T>(V.n, k);
+  const int32_t gens = (int32_t)(G.g.x / std::max(1u, grid_rows<T>(V.n, k, kCapSpmm).g.x));
   ProfScope ps("spmm", s);
 #define N_(KK) kn_spmm_dot<T, KK><<<G.g, kBlock, 0, s>>>(V.n, V.ptr, V.col, V.val, P, Q, part, ctl)
-#define W_(VV) kw_spmm_dot<T, WideV<T, VV>::v><<<G.g, kBlock, 0, s>>>(V.n, V.ptr, V.col, V.val, k, P, Q, part, ctl)
+#define W_(VV) kw_spmm_dot<T, WideV<T, VV>::v><<<G.g, kBlock, 0, s>>>(V.n, V.ecol, V.eval, V.ovp, V.ovc, V.ovv, k, P, Q, part, ctl, gens)
   DISPATCH_K(k, T, N_, W_);
 #undef N_
 #undef W_
@@ -1541,6 +1646,51 @@ static Red reduce_rows(SolverWork *W, const double *part, int32_t nrows, int64_t
   return Red{W->red.get(), S, ostride};
 }
 
+// Row table of A_ff for the SpMM: pattern once per system, values at every
+// solve (cn_system_set_positions changes them).
+static void ensure_ell(cn_system_t *S, hipStream_t s) {
+  const int32_t n = S->n_free;
+  if (n <= 0) return;
+  if (S->ov_nnz < 0) {
+    std::vector<int32_t> ptr((size_t)n + 1), op((size_t)n + 1, 0);
+    CN_HIP(hipMemcpyAsync(ptr.data(), S->ff_ptr.get(), sizeof(int32_t) * ((size_t)n + 1),
+                          hipMemcpyDeviceToHost, s));
+    CN_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> col((size_t)std::max(ptr[n], 1));
+    if (ptr[n] > 0) {
+      CN_HIP(hipMemcpyAsync(col.data(), S->ff_col.get(), sizeof(int32_t) * (size_t)ptr[n],
+                            hipMemcpyDeviceToHost, s));
+      CN_HIP(hipStreamSynchronize(s));
+    }
+    // slot 0 holds the diagonal whether or not the row stores one; the
+    // row's other entries fill slots 1.. and then the overflow
+    for (int32_t i = 0; i < n; ++i) {
+      int32_t off = 0;
+      for (int32_t q = ptr[i]; q < ptr[i + 1]; ++q) off += col[q] != i;
+      op[i + 1] = op[i] + std::max(0, off - (kEll - 1));
+    }
+    S->ell_col.alloc((size_t)n * kEll);
+    S->ell_val32.alloc((size_t)n * kEll);
+    S->ell_val64.alloc((size_t)n * kEll);
+    if (op[n] > 0) {
+      S->ov_ptr.alloc((size_t)n + 1);
+      S->ov_col.alloc((size_t)op[n]);
+      S->ov_val32.alloc((size_t)op[n]);
+      S->ov_val64.alloc((size_t)op[n]);
+      CN_HIP(hipMemcpyAsync(S->ov_ptr.get(), op.data(), sizeof(int32_t) * ((size_t)n + 1),
+                            hipMemcpyHostToDevice, s));
+      CN_HIP(hipStreamSynchronize(s));
+    }
+    S->ov_nnz = op[n];
+  }
+  const bool ov = S->ov_nnz > 0;
+  k_ell_fill<<<grid_for(n, 256), 256, 0, s>>>(
+      n, S->ff_ptr.get(), S->ff_col.get(), S->ff_val.get(), S->ell_col.get(),
+      S->ell_val32.get(), S->ell_val64.get(), ov ? S->ov_ptr.get() : nullptr,
+      S->ov_col.get(), S->ov_val32.get(), S->ov_val64.get());
+  CN_HIP(hipGetLastError());
+}
+
 template <typename T>
 static SysView<T> make_view(cn_system_t *S, cn_ic0_t *F, SolverWork *W);
 
@@ -1550,6 +1700,8 @@ SysView<float> make_view<float>(cn_system_t *S, cn_ic0_t *F, SolverWork *W) {
   V.n = S->n_free;
   V.ptr = S->ff_ptr.get(); V.col = S->ff_col.get(); V.val = S->ff_val32.get();
   V.val64 = S->ff_val.get();
+  V.ecol = S->ell_col.get(); V.eval = S->ell_val32.get();
+  if (S->ov_nnz > 0) { V.ovp = S->ov_ptr.get(); V.ovc = S->ov_col.get(); V.ovv = S->ov_val32.get(); }
   if (F) {
     V.ic0 = F;
     V.rowmap = F->perm.get();
@@ -1573,6 +1725,8 @@ SysView<double> make_view<double>(cn_system_t *S, cn_ic0_t *F, SolverWork *W) {
   V.n = S->n_free;
   V.ptr = S->ff_ptr.get(); V.col = S->ff_col.get(); V.val = S->ff_val.get();
   V.val64 = S->ff_val.get();
+  V.ecol = S->ell_col.get(); V.eval = S->ell_val64.get();
+  if (S->ov_nnz > 0) { V.ovp = S->ov_ptr.get(); V.ovc = S->ov_col.get(); V.ovv = S->ov_val64.get(); }
   if (F) {
     V.ic0 = F;
     V.rowmap = F->perm.get();
@@ -1601,11 +1755,12 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   V0.n = n;
   V0.ic0 = F;
   if (F) V0.mb = F->sweep_mb;
-  int32_t need = std::max<int32_t>(fwd_rows<T>(V0, k), grid_rows<T>(n, k, kCapSpmm).rows);
-  need = std::max<int32_t>(need, grid_rows<double>(n, k, kCapSpmm).rows);
+  int32_t need = std::max<int32_t>(fwd_rows<T>(V0, k), grid_spmm<T>(n, k).rows);
+  need = std::max<int32_t>(need, grid_spmm<double>(n, k).rows);
   SolverWork *W = ensure_work(S, k, need);
   k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
                                          W->dinv32.get());
+  ensure_ell(S, s);
   SysView<T> V = make_view<T>(S, F, W);
   V.hsc = H;
   double *st = W->st.get();
@@ -1774,11 +1929,11 @@ static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   V0.n = n;
   V0.ic0 = F;
   if (F) V0.mb = F->sweep_mb;
-  int32_t need = std::max<int32_t>(fwd_rows<double>(V0, k),
-                                   grid_rows<double>(n, k, kCapSpmm).rows);
+  int32_t need = std::max<int32_t>(fwd_rows<double>(V0, k), grid_spmm<double>(n, k).rows);
   SolverWork *W = ensure_work(S, k, need);
   k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
                                          W->dinv32.get());
+  ensure_ell(S, s);
   SysView<double> V = make_view<double>(S, F, W);
   V.hsc = H;
   double *st = W->st.get();
