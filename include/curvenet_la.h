@@ -299,10 +299,63 @@ cn_status cn_hsc_apply(cn_hsc h, int32_t k, const double *d_r, double *d_z,
 cn_status cn_hsc_destroy(cn_hsc h);
 
 /* ------------------------------------------------------------------------
+ * Colour-parallel symmetric Gauss-Seidel (SlangCodegen/SgsColor.lean:
+ * 87-130) on A_ff.  Rows are grouped by GraphColoring.greedyColor
+ * (GraphColoring.lean:41-56); one sweep runs the SgsColor row update over
+ * colours 0..C-1 and then C-1..0 (SgsColor.lean:12-14), one launch per
+ * colour.  Row update, column c of an n x k row-major block: d = A_ii (rows
+ * with d == 0 are left alone), s = sum of the row's entries times x in
+ * storage order from 0, x_i = (b_i - (s - d x_i)) / d, with no fused
+ * multiply-add, so the FP64 sweep is bit-identical to the spec's loop and
+ * the FP32 one to its float transcription.
+ * ---------------------------------------------------------------------- */
+typedef struct cn_sgs_t *cn_sgs;
+
+typedef struct cn_sgs_info {
+  int32_t n, ncolors;
+  const int32_t *h_color_ptr;   /* host, ncolors + 1                         */
+  const int32_t *d_color_rows;  /* rows of each colour, ascending            */
+  const int32_t *d_color;       /* colour of each row                        */
+} cn_sgs_info;
+
+/* Colour the pattern of A_ff.  The handle keeps the system: sweeps read its
+ * current values (cn_system_set_positions needs no rebuild). */
+cn_status cn_sgs_build(cn_system sys, cn_sgs *out, void *stream);
+cn_status cn_sgs_get_info(cn_sgs h, cn_sgs_info *info);
+/* `sweeps` symmetric sweeps of A_ff x = b in place on d_x (n_free x k). */
+cn_status cn_sgs_sweep_f64(cn_sgs h, int32_t k, const double *d_b, double *d_x,
+                           int32_t sweeps, void *stream);
+/* The same in FP32 (values and diagonal rounded from the FP64 matrix). */
+cn_status cn_sgs_sweep_f32(cn_sgs h, int32_t k, const float *d_b, float *d_x,
+                           int32_t sweeps, void *stream);
+/* The SGS preconditioner z = M^-1 r, M = (D+L) D^-1 (D+U) in colour order:
+ * one symmetric sweep from z = 0 (FP64). */
+cn_status cn_sgs_apply(cn_sgs h, int32_t k, const double *d_r, double *d_z, void *stream);
+cn_status cn_sgs_destroy(cn_sgs h);
+
+/* ------------------------------------------------------------------------
+ * Coarse-vertex restriction of a parent map (SlangCodegen/
+ * HierarchicalSparsify.lean:64-93): coarse[c] = sum of fine[i] over fine
+ * vertices i with parent[i] = c, added in ascending i from 0 (bit-identical
+ * to the spec's scan).  Built once into the CSR-by-coarse form the spec
+ * names as the production form; parents outside [0, n_coarse) match no
+ * coarse vertex.  Blocks are row-major (n x k).
+ * ---------------------------------------------------------------------- */
+typedef struct cn_restrict_t *cn_restrict;
+
+cn_status cn_restrict_build(int32_t n_fine, const int32_t *d_parent, int32_t n_coarse,
+                            cn_restrict *out, void *stream);
+cn_status cn_restrict_sum_f64(cn_restrict h, int32_t k, const double *d_fine,
+                              double *d_coarse, void *stream);
+cn_status cn_restrict_sum_f32(cn_restrict h, int32_t k, const float *d_fine,
+                              float *d_coarse, void *stream);
+cn_status cn_restrict_destroy(cn_restrict h);
+
+/* ------------------------------------------------------------------------
  * Preconditioned conjugate gradients (SparseLinAlg.cg / cgWithGuess,
  * SparseLinAlg.lean:104-163) on A_ff, k right-hand sides in lockstep.
  * ---------------------------------------------------------------------- */
-enum { CN_PRECOND_JACOBI = 0, CN_PRECOND_IC0 = 1, CN_PRECOND_HSC = 2 };
+enum { CN_PRECOND_JACOBI = 0, CN_PRECOND_IC0 = 1, CN_PRECOND_HSC = 2, CN_PRECOND_SGS = 3 };
 enum {
   CN_PREC_FP64_PLAIN = 0, /* FP64; one kernel per BLAS-1/SpMV step, alpha/beta
                              and the stop test on the host (3 host round
@@ -346,6 +399,12 @@ cn_status cn_pcg_solve(cn_system sys, cn_ic0 ic0, const cn_pcg_opts *opts,
 /* The same solve preconditioned by an HSC hierarchy of this system
  * (opts->precond is ignored). */
 cn_status cn_pcg_solve_hsc(cn_system sys, cn_hsc hsc, const cn_pcg_opts *opts,
+                           int32_t k, const double *d_b, double *d_x,
+                           cn_solve_info *info, void *stream);
+
+/* The same solve preconditioned by one symmetric SgsColor sweep from zero
+ * per application (cn_sgs_apply; opts->precond is ignored). */
+cn_status cn_pcg_solve_sgs(cn_system sys, cn_sgs sgs, const cn_pcg_opts *opts,
                            int32_t k, const double *d_b, double *d_x,
                            cn_solve_info *info, void *stream);
 

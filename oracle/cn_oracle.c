@@ -1018,6 +1018,91 @@ int32_t orc_greedy_color(int32_t n, const int32_t *ptr, const int32_t *col,
 }
 
 /* ======================================================================
+ * Colour-parallel symmetric Gauss-Seidel (SlangCodegen/SgsColor.lean:87-130)
+ * and the coarse-vertex restriction (SlangCodegen/HierarchicalSparsify.lean:
+ * 64-93)
+ * ==================================================================== */
+
+/* One SgsColor row update (SgsColor.lean:104-125), column c of an n x k
+ * row-major block: d = A_ii; a row with d == 0 is left alone; s_full =
+ * sum over the row's entries in storage order, from 0; x_i = (b_i - (s_full
+ * - d x_i)) / d. */
+static void sgs_row_f64(int32_t i, const int32_t *ptr, const int32_t *col, const double *val,
+                        const double *diag, int32_t k, int32_t c, const double *b, double *x) {
+  double d = diag[i];
+  if (d == 0.0) return;
+  double s = 0.0;
+  for (int32_t q = ptr[i]; q < ptr[i + 1]; ++q) s = s + val[q] * x[(size_t)col[q] * k + c];
+  double off = s - d * x[(size_t)i * k + c];
+  x[(size_t)i * k + c] = (b[(size_t)i * k + c] - off) / d;
+}
+
+static void sgs_row_f32(int32_t i, const int32_t *ptr, const int32_t *col, const float *val,
+                        const float *diag, int32_t k, int32_t c, const float *b, float *x) {
+  float d = diag[i];
+  if (d == 0.0f) return;
+  float s = 0.0f;
+  for (int32_t q = ptr[i]; q < ptr[i + 1]; ++q) s = s + val[q] * x[(size_t)col[q] * k + c];
+  float off = s - d * x[(size_t)i * k + c];
+  x[(size_t)i * k + c] = (b[(size_t)i * k + c] - off) / d;
+}
+
+/* `sweeps` symmetric sweeps: colours 0..C-1 (forward), then C-1..0
+ * (backward), the host schedule of SgsColor.lean:12-14.  Rows of one colour
+ * are visited in ascending order; the colouring makes them independent. */
+void orc_sgs_sweep_f64(int32_t n, const int32_t *ptr, const int32_t *col, const double *val,
+                       const double *diag, int32_t ncolors, const int32_t *color_ptr,
+                       const int32_t *color_rows, int32_t k, const double *b, double *x,
+                       int32_t sweeps) {
+  (void)n;
+  for (int32_t sw = 0; sw < sweeps; ++sw)
+    for (int32_t pass = 0; pass < 2 * ncolors; ++pass) {
+      int32_t c = pass < ncolors ? pass : 2 * ncolors - 1 - pass;
+      for (int32_t t = color_ptr[c]; t < color_ptr[c + 1]; ++t)
+        for (int32_t j = 0; j < k; ++j)
+          sgs_row_f64(color_rows[t], ptr, col, val, diag, k, j, b, x);
+    }
+}
+
+void orc_sgs_sweep_f32(int32_t n, const int32_t *ptr, const int32_t *col, const float *val,
+                       const float *diag, int32_t ncolors, const int32_t *color_ptr,
+                       const int32_t *color_rows, int32_t k, const float *b, float *x,
+                       int32_t sweeps) {
+  (void)n;
+  for (int32_t sw = 0; sw < sweeps; ++sw)
+    for (int32_t pass = 0; pass < 2 * ncolors; ++pass) {
+      int32_t c = pass < ncolors ? pass : 2 * ncolors - 1 - pass;
+      for (int32_t t = color_ptr[c]; t < color_ptr[c + 1]; ++t)
+        for (int32_t j = 0; j < k; ++j)
+          sgs_row_f32(color_rows[t], ptr, col, val, diag, k, j, b, x);
+    }
+}
+
+/* coarse[c] = sum over fine i with parent[i] = c of fine[i], scanning i in
+ * ascending order from 0 (HierarchicalSparsify.lean:70-84), for k columns. */
+void orc_restrict_sum_f64(int32_t n_fine, const int32_t *parent, int32_t n_coarse, int32_t k,
+                          const double *fine, double *coarse) {
+  for (int32_t c = 0; c < n_coarse; ++c)
+    for (int32_t j = 0; j < k; ++j) {
+      double acc = 0.0;
+      for (int32_t i = 0; i < n_fine; ++i)
+        if (parent[i] == c) acc = acc + fine[(size_t)i * k + j];
+      coarse[(size_t)c * k + j] = acc;
+    }
+}
+
+void orc_restrict_sum_f32(int32_t n_fine, const int32_t *parent, int32_t n_coarse, int32_t k,
+                          const float *fine, float *coarse) {
+  for (int32_t c = 0; c < n_coarse; ++c)
+    for (int32_t j = 0; j < k; ++j) {
+      float acc = 0.0f;
+      for (int32_t i = 0; i < n_fine; ++i)
+        if (parent[i] == c) acc = acc + fine[(size_t)i * k + j];
+      coarse[(size_t)c * k + j] = acc;
+    }
+}
+
+/* ======================================================================
  * Dense linear algebra (DenseLinAlg.lean)
  * ==================================================================== */
 

@@ -186,6 +186,18 @@ void orc_permute_csr(int32_t n, const int32_t *ptr, const int32_t *col,
  * number of colours. */
 int32_t orc_greedy_color(int32_t n, const int32_t *ptr, const int32_t *col,
                          int32_t *color);
+void orc_sgs_sweep_f64(int32_t n, const int32_t *ptr, const int32_t *col, const double *val,
+                       const double *diag, int32_t ncolors, const int32_t *color_ptr,
+                       const int32_t *color_rows, int32_t k, const double *b, double *x,
+                       int32_t sweeps);
+void orc_sgs_sweep_f32(int32_t n, const int32_t *ptr, const int32_t *col, const float *val,
+                       const float *diag, int32_t ncolors, const int32_t *color_ptr,
+                       const int32_t *color_rows, int32_t k, const float *b, float *x,
+                       int32_t sweeps);
+void orc_restrict_sum_f64(int32_t n_fine, const int32_t *parent, int32_t n_coarse, int32_t k,
+                          const double *fine, double *coarse);
+void orc_restrict_sum_f32(int32_t n_fine, const int32_t *parent, int32_t n_coarse, int32_t k,
+                          const float *fine, float *coarse);
 
 /* ---- Dense linear algebra (DenseLinAlg.lean) -------------------------- */
 void orc_dense_solve(int32_t n, const double *A, const double *b,

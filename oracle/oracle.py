@@ -37,6 +37,14 @@ PINS = {
     "ic0_solve": "LL^T x == b",
     "pcg": "dense GE / numpy solve",
     "greedy_color": "GraphColoring.lean:79-117 KATs, validity",
+    "sgs_sweep/sgs_apply": "equal to lexical Gauss-Seidel in colour order (the kernel's stated "
+                           "equivalence); one sweep from 0 == the textbook SGS preconditioner "
+                           "(D+L) D^-1 (D+U) solved densely; many sweeps converge to the "
+                           "dense solve; zero-diagonal rows untouched; planted mistakes fail",
+    "restrict_sum": "numpy np.add.at closed form (exact for the same order), hand KAT, "
+                    "parents outside [0, n_coarse) ignored",
+    "hsc_build/hsc_apply": "theta = 0: one level == dense Schur complement, exact hierarchy "
+                           "apply == A^-1; lump/diag compensation invariants; planted sign",
     "dense solve/LU/Cholesky": "DenseLinAlg.lean:294-489 KATs, numpy.linalg",
     "hsc_eliminate_vertex": "HSC KATs (HierarchicalSparsifyCompensate.lean:134-231), "
                             "dense Schur complement",
@@ -321,6 +329,75 @@ def greedy_color(ptr, col):
     color = np.zeros(n, np.int32)
     nc = lib().orc_greedy_color(n, _p(_i(ptr)), _p(_i(col)), _p(color))
     return color, nc
+
+
+def color_classes(color, ncolors):
+    """Rows of each colour, ascending: (color_ptr[ncolors + 1], color_rows)."""
+    color = np.asarray(color)
+    rows = np.argsort(color, kind="stable").astype(np.int32)
+    cptr = np.zeros(ncolors + 1, np.int32)
+    cptr[1:] = np.cumsum(np.bincount(color, minlength=ncolors))
+    return cptr, rows
+
+
+def csr_diag(ptr, col, val):
+    """A_ii from CSR (0 where the row stores no diagonal)."""
+    n = len(ptr) - 1
+    d = np.zeros(n)
+    for i in range(n):
+        for q in range(ptr[i], ptr[i + 1]):
+            if col[q] == i:
+                d[i] = val[q]
+    return d
+
+
+def sgs_sweep(ptr, col, val, b, x0, sweeps=1, dtype=np.float64, color=None):
+    """SgsColor (SgsColor.lean:87-130): `sweeps` symmetric colour sweeps of
+    x (n or n x k) for A x = b, greedy colours unless `color` = (color,
+    ncolors) is given.  FP64 or FP32 arithmetic (dtype)."""
+    n = len(ptr) - 1
+    if color is None:
+        color = greedy_color(ptr, col)
+    cptr, rows = color_classes(*color)
+    b2 = np.asarray(b).reshape(n, -1)
+    k = b2.shape[1]
+    if dtype == np.float32:
+        v = np.ascontiguousarray(val, np.float32)
+        d = np.ascontiguousarray(csr_diag(ptr, col, val), np.float32)
+        bb = np.ascontiguousarray(b2, np.float32)
+        x = np.ascontiguousarray(np.asarray(x0).reshape(n, k), np.float32).copy()
+        fn = lib().orc_sgs_sweep_f32
+    else:
+        v, d, bb = _d(val), _d(csr_diag(ptr, col, val)), _d(b2)
+        x = _d(np.asarray(x0).reshape(n, k)).copy()
+        fn = lib().orc_sgs_sweep_f64
+    fn(n, _p(_i(ptr)), _p(_i(col)), _p(v), _p(d), color[1], _p(cptr), _p(rows), k, _p(bb),
+       _p(x), sweeps)
+    return x.reshape(np.asarray(b).shape)
+
+
+def sgs_apply(ptr, col, val, r, color=None):
+    """SGS preconditioner: one symmetric sweep from x = 0 for b = r."""
+    return sgs_sweep(ptr, col, val, r, np.zeros_like(np.asarray(r, np.float64)), 1,
+                     color=color)
+
+
+def restrict_sum(parent, n_coarse, fine, dtype=np.float64):
+    """HierarchicalSparsify restriction (HierarchicalSparsify.lean:64-93):
+    coarse[c] = sum of fine[i] over parent[i] == c, i ascending."""
+    f = np.asarray(fine)
+    f2 = f.reshape(f.shape[0], -1)
+    k = f2.shape[1]
+    if dtype == np.float32:
+        ff = np.ascontiguousarray(f2, np.float32)
+        out = np.zeros((n_coarse, k), np.float32)
+        fn = lib().orc_restrict_sum_f32
+    else:
+        ff = _d(f2)
+        out = np.zeros((n_coarse, k))
+        fn = lib().orc_restrict_sum_f64
+    fn(len(parent), _p(_i(parent)), n_coarse, k, _p(ff), _p(out))
+    return out.reshape((n_coarse,) + f.shape[1:])
 
 
 # -------------------------------------------------------------------- dense --

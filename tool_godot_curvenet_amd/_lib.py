@@ -19,7 +19,7 @@ STATUS_NAMES = {0: "OK", 1: "MAXIT", 2: "BREAKDOWN", 3: "NOT_SPD", -1: "BAD_ARG"
                 -2: "HIP_ERROR", -3: "NO_DEVICE", -4: "OOM", -5: "NOT_MANIFOLD"}
 
 CN_ORDER_COLOR, CN_ORDER_NATURAL, CN_ORDER_BLOCK = 0, 1, 2
-CN_PRECOND_JACOBI, CN_PRECOND_IC0, CN_PRECOND_HSC = 0, 1, 2
+CN_PRECOND_JACOBI, CN_PRECOND_IC0, CN_PRECOND_HSC, CN_PRECOND_SGS = 0, 1, 2, 3
 CN_HSC_LUMP, CN_HSC_DIAG = 0, 1
 CN_PREC_FP64_PLAIN, CN_PREC_FP64, CN_PREC_MIXED = 0, 1, 2
 
@@ -74,6 +74,11 @@ class HscLevel(C.Structure):
                 ("d_p_col", vp), ("d_p_val", vp)]
 
 
+class SgsInfo(C.Structure):
+    _fields_ = [("n", i32), ("ncolors", i32), ("h_color_ptr", P_i32), ("d_color_rows", vp),
+                ("d_color", vp)]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("status", i32), ("iters", i32), ("refinements", i32), ("relres_max", f64)]
 
@@ -117,6 +122,18 @@ _SIGS = {
     "cn_hsc_get_level": (i32, [vp, i32, C.POINTER(HscLevel)]),
     "cn_hsc_apply": (i32, [vp, i32, vp, vp, vp]),
     "cn_hsc_destroy": (i32, [vp]),
+    "cn_sgs_build": (i32, [vp, C.POINTER(vp), vp]),
+    "cn_sgs_get_info": (i32, [vp, C.POINTER(SgsInfo)]),
+    "cn_sgs_sweep_f64": (i32, [vp, i32, vp, vp, i32, vp]),
+    "cn_sgs_sweep_f32": (i32, [vp, i32, vp, vp, i32, vp]),
+    "cn_sgs_apply": (i32, [vp, i32, vp, vp, vp]),
+    "cn_sgs_destroy": (i32, [vp]),
+    "cn_pcg_solve_sgs": (i32, [vp, vp, C.POINTER(PcgOpts), i32, vp, vp, C.POINTER(SolveInfo),
+                               vp]),
+    "cn_restrict_build": (i32, [i32, vp, i32, C.POINTER(vp), vp]),
+    "cn_restrict_sum_f64": (i32, [vp, i32, vp, vp, vp]),
+    "cn_restrict_sum_f32": (i32, [vp, i32, vp, vp, vp]),
+    "cn_restrict_destroy": (i32, [vp]),
     "cn_profile_enable": (None, [i32]),
     "cn_profile_query": (i32, [C.c_char_p, C.POINTER(f64)]),
 }

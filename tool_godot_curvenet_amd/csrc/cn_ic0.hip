@@ -209,9 +209,8 @@ __global__ void k_scatter_rows(int32_t n, int32_t k, const int32_t *perm,
 
 // GraphColoring.greedyColor (GraphColoring.lean:41-56): vertex order,
 // smallest colour not used by an earlier neighbour.
-static int32_t greedy_color(int32_t n, const std::vector<int32_t> &ptr,
-                            const std::vector<int32_t> &col,
-                            std::vector<int32_t> &color) {
+int32_t greedy_color(int32_t n, const std::vector<int32_t> &ptr,
+                     const std::vector<int32_t> &col, std::vector<int32_t> &color) {
   color.assign(n, 0);
   std::vector<int32_t> mark;
   int32_t ncol = 0;

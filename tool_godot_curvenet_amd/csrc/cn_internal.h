@@ -117,6 +117,14 @@ void free_solver_work(SolverWork *w);
 // ------------------------------------------------------------ handles --
 // The reduced cut-Laplacian system A_ff (free x free) and the coupling
 // block A_fc (free x constrained), plus the cut mesh they came from.
+namespace cn {
+// GraphColoring.greedyColor (GraphColoring.lean:41-56) of a CSR pattern:
+// vertex order, smallest colour not used by an earlier neighbour.  Returns
+// the number of colours.
+int32_t greedy_color(int32_t n, const std::vector<int32_t> &ptr,
+                     const std::vector<int32_t> &col, std::vector<int32_t> &color);
+}  // namespace cn
+
 struct cn_system_t {
   int32_t nv = 0, nf = 0, n_free = 0, n_con = 0;
   int64_t nnz_ff = 0, nnz_fc = 0;
@@ -146,6 +154,23 @@ struct cn_system_t {
   int64_t ov_nnz = -1;
   cn::SolverWork *work = nullptr;   // solver workspace, grown on demand
   ~cn_system_t() { cn::free_solver_work(work); }
+};
+
+// Colour classes of A_ff for the SgsColor sweeps (pattern only: the
+// values and the diagonal are read from the system at every sweep).
+struct cn_sgs_t {
+  cn_system_t *sys = nullptr;
+  int32_t n = 0, ncolors = 0;
+  std::vector<int32_t> color_ptr;   // host: rows of colour c are
+                                    // color_rows[color_ptr[c] .. color_ptr[c+1])
+  cn::DevBuf<int32_t> color_rows, color;
+};
+
+// CSR-by-coarse form of a parent map: the fine vertices of coarse vertex c,
+// ascending, are child[cptr[c] .. cptr[c+1]).
+struct cn_restrict_t {
+  int32_t n_fine = 0, n_coarse = 0;
+  cn::DevBuf<int32_t> cptr, child;
 };
 
 struct cn_ic0_t {

@@ -1330,7 +1330,8 @@ struct SysView {
   const int32_t *rowmap, *blk, *lp, *lcol, *up, *ucol;
   const T *lval, *uval, *rdiag;
   const T *dinv;
-  cn_hsc_t *hsc = nullptr;  // whole-apply preconditioner (replaces ic0 / dinv)
+  cn_hsc_t *hsc = nullptr;  // whole-apply preconditioners (replace ic0 / dinv)
+  const cn_sgs_t *sgs = nullptr;
   // row table of A_ff (SpMM); ovp = nullptr when no row overflows it
   const int32_t *ecol = nullptr, *ovp = nullptr, *ovc = nullptr;
   const T *eval = nullptr, *ovv = nullptr;
@@ -1386,6 +1387,9 @@ static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
 
 template <typename T>
 void hsc_apply(cn_hsc_t *H, int64_t k, const T *r0, T *z0, hipStream_t s, const int32_t *ctl);
+template <typename T>
+void sgs_apply(const cn_sgs_t *G, int64_t k, const T *r, T *z, hipStream_t s,
+               const int32_t *ctl);
 template <typename T>
 static int32_t launch_dot(int32_t n, int64_t k, const T *A, const T *B, double *part,
                           hipStream_t s);
@@ -1492,7 +1496,7 @@ template <typename T, bool FUSED>
 static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
                           const T *Q, T *R, T *Z, double *part, int64_t stride,
                           const int32_t *ctl, hipStream_t s) {
-  if (V.hsc) {
+  if (V.hsc || V.sgs) {
     Grid G = grid_rows<T>(V.n, k, kCapSpmm);
     ProfScope ps("fwd", s);
 #define N_(KK) kn_rupd<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(V.n, alpha, Q, R, part, ctl)
@@ -1500,7 +1504,10 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
     DISPATCH_K(k, T, N_, W_);
 #undef N_
 #undef W_
-    hsc_apply<T>(V.hsc, k, R, Z, s, ctl);
+    if (V.hsc)
+      hsc_apply<T>(V.hsc, k, R, Z, s, ctl);
+    else
+      sgs_apply<T>(V.sgs, k, R, Z, s, ctl);
     launch_dot<T>(V.n, k, R, Z, part + stride, s);
     return G.rows;
   }
@@ -1746,7 +1753,8 @@ SysView<double> make_view<double>(cn_system_t *S, cn_ic0_t *F, SolverWork *W) {
 
 // ========================================= fused solve (MIXED and FP64) ==
 template <typename T>
-static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_pcg_opts *o,
+static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_sgs_t *G,
+                             const cn_pcg_opts *o,
                              int32_t k, const double *d_b, double *d_x,
                              cn_solve_info *info, hipStream_t s) {
   const int32_t n = S->n_free;
@@ -1763,6 +1771,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   ensure_ell(S, s);
   SysView<T> V = make_view<T>(S, F, W);
   V.hsc = H;
+  V.sgs = G;
   double *st = W->st.get();
   double *hs = W->h_st;
   int32_t *ctl = W->ctl.get();
@@ -1920,7 +1929,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
 }
 
 // ================================ unfused FP64 baseline, scalars on host ==
-static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_pcg_opts *o,
+static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_sgs_t *G,
+                             const cn_pcg_opts *o,
                              int32_t k, const double *d_b, double *d_x,
                              cn_solve_info *info, hipStream_t s) {
   const int32_t n = S->n_free;
@@ -1936,6 +1946,7 @@ static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   ensure_ell(S, s);
   SysView<double> V = make_view<double>(S, F, W);
   V.hsc = H;
+  V.sgs = G;
   double *st = W->st.get();
   const int64_t stride = (int64_t)W->part_rows * k;
   const unsigned red = (unsigned)((k + 63) / 64);
@@ -2078,7 +2089,8 @@ int32_t cn_profile_query(const char *name, double *ms_total) {
   return it->second.first;
 }
 
-static cn_status solve_impl(cn_system S, cn_ic0 F, cn_hsc H, const cn_pcg_opts *o, int32_t k,
+static cn_status solve_impl(cn_system S, cn_ic0 F, cn_hsc H, cn_sgs G, const cn_pcg_opts *o,
+                            int32_t k,
                             const double *d_b, double *d_x, cn_solve_info *info, void *stream,
                             const char *who) {
   if (!S || !o || k <= 0 || !d_b || !d_x) {
@@ -2093,6 +2105,10 @@ static cn_status solve_impl(cn_system S, cn_ic0 F, cn_hsc H, const cn_pcg_opts *
     set_error("%s: hierarchy does not match the system", who);
     return CN_BAD_ARG;
   }
+  if (G && (G->sys != S || G->n != S->n_free)) {
+    set_error("%s: colouring does not belong to the system", who);
+    return CN_BAD_ARG;
+  }
   hipStream_t s = as_stream(stream);
   if (S->n_free == 0) {
     if (info) { info->status = CN_OK; info->iters = 0; info->refinements = 0; info->relres_max = 0.0; }
@@ -2101,9 +2117,9 @@ static cn_status solve_impl(cn_system S, cn_ic0 F, cn_hsc H, const cn_pcg_opts *
   prof_reset();
   cn_status st;
   switch (o->precision) {
-    case CN_PREC_FP64_PLAIN: st = plain_solve(S, F, H, o, k, d_b, d_x, info, s); break;
-    case CN_PREC_FP64: st = fused_solve<double>(S, F, H, o, k, d_b, d_x, info, s); break;
-    case CN_PREC_MIXED: st = fused_solve<float>(S, F, H, o, k, d_b, d_x, info, s); break;
+    case CN_PREC_FP64_PLAIN: st = plain_solve(S, F, H, G, o, k, d_b, d_x, info, s); break;
+    case CN_PREC_FP64: st = fused_solve<double>(S, F, H, G, o, k, d_b, d_x, info, s); break;
+    case CN_PREC_MIXED: st = fused_solve<float>(S, F, H, G, o, k, d_b, d_x, info, s); break;
     default:
       set_error("%s: unknown precision %d", who, o->precision);
       return CN_BAD_ARG;
@@ -2124,8 +2140,12 @@ cn_status cn_pcg_solve(cn_system S, cn_ic0 F, const cn_pcg_opts *o, int32_t k,
     set_error("cn_pcg_solve: the HSC preconditioner is applied by cn_pcg_solve_hsc");
     return CN_BAD_ARG;
   }
+  if (o && o->precond == CN_PRECOND_SGS) {
+    set_error("cn_pcg_solve: the SGS preconditioner is applied by cn_pcg_solve_sgs");
+    return CN_BAD_ARG;
+  }
   if (o && o->precond != CN_PRECOND_IC0) F = nullptr;
-  return solve_impl(S, F, nullptr, o, k, d_b, d_x, info, stream, "cn_pcg_solve");
+  return solve_impl(S, F, nullptr, nullptr, o, k, d_b, d_x, info, stream, "cn_pcg_solve");
   CN_API_END
 }
 
@@ -2137,7 +2157,19 @@ cn_status cn_pcg_solve_hsc(cn_system S, cn_hsc H, const cn_pcg_opts *o, int32_t 
     set_error("cn_pcg_solve_hsc: no hierarchy");
     return CN_BAD_ARG;
   }
-  return solve_impl(S, nullptr, H, o, k, d_b, d_x, info, stream, "cn_pcg_solve_hsc");
+  return solve_impl(S, nullptr, H, nullptr, o, k, d_b, d_x, info, stream, "cn_pcg_solve_hsc");
+  CN_API_END
+}
+
+cn_status cn_pcg_solve_sgs(cn_system S, cn_sgs G, const cn_pcg_opts *o, int32_t k,
+                           const double *d_b, double *d_x, cn_solve_info *info,
+                           void *stream) {
+  CN_API_BEGIN
+  if (!G) {
+    set_error("cn_pcg_solve_sgs: no colouring");
+    return CN_BAD_ARG;
+  }
+  return solve_impl(S, nullptr, nullptr, G, o, k, d_b, d_x, info, stream, "cn_pcg_solve_sgs");
   CN_API_END
 }
 

@@ -211,9 +211,9 @@ class _System:
     def solve(self, b, precond: str = "ic0", precision: str = "mixed", tol: float = 1e-9,
               x0=None, ic0: "IC0 | None" = None, max_iter: int = 20000,
               inner_tol: float = 1e-5, check_every: int = 8, max_refine: int = 30,
-              hsc: "HSC | None" = None):
-        """Solve A_ff X = B (n x k or n); precond "ic0", "jacobi" or "hsc".
-        Returns (x, SolveInfo)."""
+              hsc: "HSC | None" = None, sgs: "SGS | None" = None):
+        """Solve A_ff X = B (n x k or n); precond "ic0", "jacobi", "hsc" or
+        "sgs".  Returns (x, SolveInfo)."""
         torch = _torch()
         lib = L.load()
         bt = _as_tensor(b, torch.float64)
@@ -222,13 +222,18 @@ class _System:
         k = bt.shape[1]
         x = (_as_tensor(x0, torch.float64).reshape(self.n_free, k).clone()
              if x0 is not None else torch.zeros_like(bt))
-        o = self._opts("jacobi" if precond == "hsc" else precond, precision, tol, max_iter,
-                       inner_tol, check_every, max_refine, x0 is not None)
+        o = self._opts("jacobi" if precond in ("hsc", "sgs") else precond, precision, tol,
+                       max_iter, inner_tol, check_every, max_refine, x0 is not None)
         info = L.SolveInfo()
         if precond == "hsc":
             if hsc is None:
                 hsc = HSC(self)
             st = lib.cn_pcg_solve_hsc(self._h, hsc.handle, C.byref(o), k, _p(bt), _p(x),
+                                      C.byref(info), _stream())
+        elif precond == "sgs":
+            if sgs is None:
+                sgs = SGS(self)
+            st = lib.cn_pcg_solve_sgs(self._h, sgs.handle, C.byref(o), k, _p(bt), _p(x),
                                       C.byref(info), _stream())
         else:
             if precond == "ic0" and ic0 is None:
@@ -418,6 +423,99 @@ class HSC:
                 "hsc_apply")
         z = z.cpu().numpy()
         return z.reshape(-1) if vec else z
+
+
+class SGS:
+    """Colour-parallel symmetric Gauss-Seidel on a system's A_ff
+    (SlangCodegen/SgsColor.lean:87-130; cn_sgs_build): greedy colours,
+    sweeps over colours 0..C-1 then C-1..0."""
+
+    def __init__(self, system: "_System"):
+        self.system = system
+        h = C.c_void_p(0)
+        L.check(L.load().cn_sgs_build(system.handle, C.byref(h), _stream()), "sgs_build")
+        self._h = h
+        info = L.SgsInfo()
+        L.check(L.load().cn_sgs_get_info(h, C.byref(info)), "sgs_get_info")
+        self.n, self.ncolors = info.n, info.ncolors
+        self.color_ptr = np.array([info.h_color_ptr[c] for c in range(self.ncolors + 1)],
+                                  np.int32)
+        self.color_rows = _np(info.d_color_rows, self.n, np.int32)
+        self.color = _np(info.d_color, self.n, np.int32)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                L.load().cn_sgs_destroy(h)
+            except Exception:
+                pass
+            self._h = C.c_void_p(0)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def sweep(self, b, x0, sweeps: int = 1, dtype=np.float64):
+        """`sweeps` symmetric sweeps of A_ff x = b from x0 (n or n x k), in
+        FP64 or FP32 (dtype).  Returns x on the host."""
+        torch = _torch()
+        tt = torch.float32 if dtype == np.float32 else torch.float64
+        bt = _as_tensor(np.asarray(b, dtype=dtype), tt)
+        vec = bt.dim() == 1
+        bt = bt.reshape(self.n, -1).contiguous()
+        x = _as_tensor(np.asarray(x0, dtype=dtype), tt).reshape(self.n, -1).contiguous().clone()
+        fn = L.load().cn_sgs_sweep_f32 if dtype == np.float32 else L.load().cn_sgs_sweep_f64
+        L.check(fn(self._h, bt.shape[1], _p(bt), _p(x), sweeps, _stream()), "sgs_sweep")
+        x = x.cpu().numpy()
+        return x.reshape(-1) if vec else x
+
+    def apply(self, r):
+        """z = M^-1 r: one symmetric sweep from zero (cn_sgs_apply)."""
+        torch = _torch()
+        rt = _as_tensor(r, torch.float64)
+        vec = rt.dim() == 1
+        rt = rt.reshape(self.n, -1).contiguous()
+        z = torch.empty_like(rt)
+        L.check(L.load().cn_sgs_apply(self._h, rt.shape[1], _p(rt), _p(z), _stream()),
+                "sgs_apply")
+        z = z.cpu().numpy()
+        return z.reshape(-1) if vec else z
+
+
+class Restriction:
+    """Coarse-vertex restriction of a parent map (SlangCodegen/
+    HierarchicalSparsify.lean:64-93; cn_restrict_build): coarse[c] = sum of
+    fine[i] over parent[i] == c, in ascending i."""
+
+    def __init__(self, parent, n_coarse: int):
+        torch = _torch()
+        p = _dev(np.asarray(parent, np.int32), torch.int32)
+        self.n_fine, self.n_coarse = int(p.numel()), int(n_coarse)
+        h = C.c_void_p(0)
+        L.check(L.load().cn_restrict_build(self.n_fine, _p(p), self.n_coarse, C.byref(h),
+                                           _stream()), "restrict_build")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                L.load().cn_restrict_destroy(h)
+            except Exception:
+                pass
+            self._h = C.c_void_p(0)
+
+    def __call__(self, fine, dtype=np.float64):
+        torch = _torch()
+        tt = torch.float32 if dtype == np.float32 else torch.float64
+        f = np.asarray(fine, dtype=dtype)
+        ft = _as_tensor(f.reshape(self.n_fine, -1), tt).contiguous()
+        k = ft.shape[1]
+        out = torch.empty((self.n_coarse, k), dtype=tt, device="cuda")
+        fn = L.load().cn_restrict_sum_f32 if dtype == np.float32 else L.load().cn_restrict_sum_f64
+        L.check(fn(self._h, k, _p(ft), _p(out), _stream()), "restrict_sum")
+        return out.cpu().numpy().reshape((self.n_coarse,) + f.shape[1:])
 
 
 def dense_cholesky(A):
