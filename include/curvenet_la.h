@@ -143,6 +143,24 @@ cn_status cn_system_set_positions(cn_system sys, const double *d_pos,
                                   void *stream);
 cn_status cn_system_destroy(cn_system sys);
 
+/* Many independent systems as one batch: a block-diagonal copy of their
+ * A_ff (segment s = systems[s], rows in order).  No counterpart in the
+ * reference, which solves one mesh per call (HarmonicSolve.solveMulti,
+ * HarmonicSolve.lean:51-66); it is the many-uneven-meshes launch of the
+ * deform path.  Every solver entry point accepts a batch (IC(0) and
+ * CN_PREC_FP64_PLAIN excepted): right-hand sides and solutions are
+ * (sum of n_free) x k, segment s in rows [seg_ptr[s], seg_ptr[s+1]), and
+ * every (segment, column) pair is its own CG column -- its own alpha, beta,
+ * stop test and residual replacement -- so a segment's solution is the
+ * solution of its system.  The batch copies the values: after
+ * cn_system_set_positions on a member, rebuild it.  The right-hand side of
+ * a segment comes from cn_harmonic_rhs on its member system. */
+cn_status cn_system_batch(int32_t nsys, const cn_system *systems, cn_system *out,
+                          void *stream);
+/* Segments of a system: *nseg = 1 and h_seg_ptr = {0, n_free} for a
+ * single system.  h_seg_ptr (host, nseg + 1) lives as long as the system. */
+cn_status cn_system_segments(cn_system sys, int32_t *nseg, const int32_t **h_seg_ptr);
+
 /* HarmonicSolve.solveMulti right-hand side (HarmonicSolve.lean:35-45,57-64)
  * restricted to free rows: d_rhs[n_free*k] = -(Vt Lh C Fc), with
  * d_Xc[n_con*k] the sample values per constrained column, summed in the

@@ -14,6 +14,7 @@ from .curvenet import (  # noqa: F401
     CutLaplacian,
     CsrMatrix,
     CsrSystem,
+    BatchSystem,
     cg,
     cg_with_guess,
     cut_seams,
@@ -28,7 +29,7 @@ from .curvenet import (  # noqa: F401
 from .autograd import harmonic_solve as harmonic_solve_ad  # noqa: F401
 
 __all__ = [
-    "HSC", "SGS", "Restriction", "dense_cholesky", "dense_cholesky_solve", "IC0", "CutLaplacian", "CsrMatrix", "CsrSystem", "cg", "cg_with_guess", "cut_seams",
+    "HSC", "SGS", "Restriction", "dense_cholesky", "dense_cholesky_solve", "IC0", "CutLaplacian", "CsrMatrix", "CsrSystem", "BatchSystem", "cg", "cg_with_guess", "cut_seams",
     "from_triangles", "harmonic_solve", "harmonic_solve_ad", "project_to_vertices", "spmm",
     "spmv", "curves",
 ]

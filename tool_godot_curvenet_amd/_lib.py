@@ -98,6 +98,8 @@ _SIGS = {
     "cn_system_get_info": (i32, [vp, C.POINTER(SystemInfo)]),
     "cn_system_set_positions": (i32, [vp, vp, vp]),
     "cn_system_destroy": (i32, [vp]),
+    "cn_system_batch": (i32, [i32, C.POINTER(vp), C.POINTER(vp), vp]),
+    "cn_system_segments": (i32, [vp, P_i32, C.POINTER(P_i32)]),
     "cn_harmonic_rhs": (i32, [vp, i32, vp, vp, vp]),
     "cn_spmv_f64": (i32, [i32, vp, vp, vp, vp, vp, vp]),
     "cn_spmv_f32": (i32, [i32, vp, vp, vp, vp, vp, vp]),

@@ -124,6 +124,11 @@ namespace cn {
 int32_t greedy_color(int32_t n, const std::vector<int32_t> &ptr,
                      const std::vector<int32_t> &col, std::vector<int32_t> &color);
 }  // namespace cn
+struct cn_system_t;
+namespace cn {
+// one segment over every row of a single system, if it has no segments yet
+void ensure_segments(cn_system_t *S, hipStream_t s);
+}  // namespace cn
 
 struct cn_system_t {
   int32_t nv = 0, nf = 0, n_free = 0, n_con = 0;
@@ -152,6 +157,14 @@ struct cn_system_t {
   cn::DevBuf<float> ell_val32, ov_val32;
   cn::DevBuf<double> ell_val64, ov_val64;
   int64_t ov_nnz = -1;
+  // A batch of independent systems (cn_system_batch) is one block-diagonal
+  // A_ff: segment s owns rows [seg_ptr[s], seg_ptr[s+1]).  The solver keeps
+  // its CG scalars per (segment, column), and takes its dots over blocks of
+  // rows that never cross a segment: blocks [blk_ptr[s], blk_ptr[s+1]) of
+  // the dot grid cover segment s.  nseg = 1 for a single system.
+  int32_t nseg = 1, seg_blocks = 0;
+  std::vector<int32_t> seg_ptr;      // host, nseg + 1
+  cn::DevBuf<int32_t> d_seg_ptr, seg_of_row, blk_ptr, blk_seg;
   cn::SolverWork *work = nullptr;   // solver workspace, grown on demand
   ~cn_system_t() { cn::free_solver_work(work); }
 };

@@ -54,6 +54,18 @@ __device__ __forceinline__ bool bwd_idle(const int32_t *ctl) {
   return ctl_idle(ctl) && ctl[CTL_JUST] == 0;
 }
 
+// The CG scalars of a batch (cn_system_batch) are kept per (segment,
+// column): element (row, c) of an n x k block uses virtual column
+// seg(row) * k + c of kk = nseg * k.  A single system has of_row = nullptr
+// and kk = k.
+struct SegMap {
+  const int32_t *of_row;
+  int64_t kk;
+  __device__ __forceinline__ int64_t vc(int64_t row, int64_t k, int64_t c) const {
+    return of_row ? (int64_t)of_row[row] * k + c : c;
+  }
+};
+
 // ================================================================ kernels ==
 // rows of an XCD's contiguous eighth, interleaved over its row slots
 #define NARROW_LOOP(row, n0, n1)                                              \
@@ -806,17 +818,18 @@ kb_bwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
 //      preconditioner applied as a whole (HSC), whose r.z comes after it ----
 template <typename T, int K, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
-kn_rupd(int32_t n, const double *__restrict__ alpha, const T *__restrict__ Q,
+kn_rupd(int32_t n, const double *__restrict__ alpha, SegMap sm, const T *__restrict__ Q,
         T *__restrict__ R, double *__restrict__ part, const int32_t *__restrict__ ctl) {
   if (ctl_idle(ctl)) return;
   const int g = threadIdx.x % kGroup;
   double pd[1] = {0.0};
-  const T al = (FUSED && g < K) ? (T)alpha[g] : T(0);
+  const T al0 = (FUSED && g < K) ? (T)alpha[g] : T(0);
   NARROW_LOOP(row, 0, n) {
     if (g < K) {
       const int64_t o = row * K + g;
       T r = R[o];
       if (FUSED) {
+        const T al = sm.of_row ? (T)alpha[sm.vc(row, K, g)] : al0;
         r = fma(-al, Q[o], r);
         R[o] = r;
       }
@@ -828,8 +841,9 @@ kn_rupd(int32_t n, const double *__restrict__ alpha, const T *__restrict__ Q,
 
 template <typename T, int VEC, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
-kw_rupd(int32_t n, const double *__restrict__ alpha, int64_t k, const T *__restrict__ Q,
-        T *__restrict__ R, double *__restrict__ part, const int32_t *__restrict__ ctl) {
+kw_rupd(int32_t n, const double *__restrict__ alpha, SegMap sm, int64_t k,
+        const T *__restrict__ Q, T *__restrict__ R, double *__restrict__ part,
+        const int32_t *__restrict__ ctl) {
   if (ctl_idle(ctl)) return;
   WIDE_COLS(VEC)
   double pd[1][VEC] = {};
@@ -842,6 +856,11 @@ kw_rupd(int32_t n, const double *__restrict__ alpha, int64_t k, const T *__restr
       T rv[VEC];
       VecIO<T, VEC>::ld(R + o, rv);
       if (FUSED) {
+        if (sm.of_row) {
+          const int64_t so = sm.vc(uniform(row), k, c0);
+#pragma unroll
+          for (int m = 0; m < VEC; ++m) al[m] = (T)alpha[so + m];
+        }
         T qv[VEC];
         VecIO<T, VEC>::ld(Q + o, qv);
 #pragma unroll
@@ -1082,6 +1101,83 @@ k_reduce_plain(int32_t nrows, int64_t k, const double *__restrict__ part,
   if (threadIdx.x < 64 && c < k) out[c] = v;
 }
 
+// ---- per-segment dots of a batch -------------------------------------------
+// Blocks [blk_ptr[s], blk_ptr[s+1]) of the grid walk the rows of segment s
+// only, so every block's partial row belongs to one segment.
+__device__ __forceinline__ RowIter seg_rows(const int32_t *__restrict__ seg_ptr,
+                                            const int32_t *__restrict__ blk_ptr,
+                                            const int32_t *__restrict__ blk_seg,
+                                            int32_t slots, int32_t slot) {
+  const int32_t b = blockIdx.x, sg = blk_seg[b];
+  const int32_t lb = b - blk_ptr[sg], nb = blk_ptr[sg + 1] - blk_ptr[sg];
+  return RowIter{seg_ptr[sg] + lb * slots + slot, seg_ptr[sg + 1], nb * slots};
+}
+
+// partials of a.b (and, NP = 2, of c.d at part + stride)
+template <typename TA, typename TB, int K, int NP>
+__global__ void __launch_bounds__(kBlock)
+kn_segdot(const int32_t *__restrict__ seg_ptr, const int32_t *__restrict__ blk_ptr,
+          const int32_t *__restrict__ blk_seg, const TA *__restrict__ a,
+          const TB *__restrict__ b, const TA *__restrict__ c, const TB *__restrict__ d,
+          double *__restrict__ part, int64_t stride) {
+  const int g = threadIdx.x % kGroup;
+  double pd[NP] = {};
+  const RowIter it = seg_rows(seg_ptr, blk_ptr, blk_seg, kBlock / kGroup, threadIdx.x / kGroup);
+  for (int32_t row = it.first; row < it.end; row += it.stride)
+    if (g < K) {
+      const int64_t o = (int64_t)row * K + g;
+      pd[0] += (double)a[o] * (double)b[o];
+      if (NP == 2) pd[NP - 1] += (double)c[o] * (double)d[o];
+    }
+  block_partials_narrow<K, NP>(pd, part, stride);
+}
+
+template <typename TA, typename TB, int NP>
+__global__ void __launch_bounds__(kBlock)
+kw_segdot(const int32_t *__restrict__ seg_ptr, const int32_t *__restrict__ blk_ptr,
+          const int32_t *__restrict__ blk_seg, int64_t k, const TA *__restrict__ a,
+          const TB *__restrict__ b, const TA *__restrict__ c, const TB *__restrict__ d,
+          double *__restrict__ part, int64_t stride) {
+  WIDE_COLS(1)
+  double pd[NP][1] = {};
+  const RowIter it = seg_rows(seg_ptr, blk_ptr, blk_seg, kWaves, threadIdx.x >> 6);
+  for (int32_t row = it.first; row < it.end; row += it.stride)
+    if (valid) {
+      const int64_t o = (int64_t)row * k + c0;
+      pd[0][0] += (double)a[o] * (double)b[o];
+      if (NP == 2) pd[NP - 1][0] += (double)c[o] * (double)d[o];
+    }
+  block_partials_wide<1, NP>(pd, part, k, c0, stride);
+}
+
+// out[q * kk + s * k + c] = sum over the blocks of segment s, in order, of
+// their partial (q, c)
+__global__ void k_segsum(int32_t nseg, const int32_t *__restrict__ blk_ptr, int64_t k, int np,
+                         const double *__restrict__ part, int64_t stride,
+                         double *__restrict__ out) {
+  const int64_t kk = (int64_t)nseg * k;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)np * kk) return;
+  const int q = (int)(id / kk);
+  const int64_t v = id - q * kk;
+  const int32_t sg = (int32_t)(v / k);
+  const int64_t c = v - (int64_t)sg * k;
+  double t = 0.0;
+  for (int32_t b = blk_ptr[sg]; b < blk_ptr[sg + 1]; ++b) t += part[q * stride + (int64_t)b * k + c];
+  out[id] = t;
+}
+
+// Jacobi as a whole apply (batches): z = dinv r, zero diagonal -> 0
+template <typename T>
+__global__ void k_diag_apply(int64_t total, int64_t k, const T *__restrict__ dinv,
+                             const T *__restrict__ R, T *__restrict__ Z,
+                             const int32_t *__restrict__ ctl) {
+  if (ctl_idle(ctl)) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x)
+    Z[i] = dinv[i / k] * R[i];
+}
+
 // ---- elementwise -----------------------------------------------------------
 // plain path: P = Z + beta P
 template <typename T>
@@ -1094,35 +1190,38 @@ __global__ void k_p_update(int64_t total, int64_t k, const double *__restrict__ 
 
 // Jacobi second half: d += alpha p; p = gamma z + beta p
 template <typename T>
-__global__ void k_pd_update(int64_t total, int64_t k, const double *__restrict__ st,
+__global__ void k_pd_update(int64_t total, int64_t k, SegMap sm, const double *__restrict__ st,
                             const T *__restrict__ Z, T *__restrict__ P,
                             T *__restrict__ D, const int32_t *__restrict__ ctl) {
   if (bwd_idle(ctl)) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = i % k;
+    const int64_t row = i / k, v = sm.vc(row, k, i - row * k);
     T p = P[i];
-    D[i] = fma((T)st[ST_ALPHA * k + c], p, D[i]);
-    P[i] = fma((T)st[ST_BETA * k + c], p, (T)st[ST_GAMMA * k + c] * Z[i]);
+    D[i] = fma((T)st[ST_ALPHA * sm.kk + v], p, D[i]);
+    P[i] = fma((T)st[ST_BETA * sm.kk + v], p, (T)st[ST_GAMMA * sm.kk + v] * Z[i]);
   }
 }
 
 // Y = (T)(X * s_c)
 template <typename T>
-__global__ void k_scale_in(int64_t total, int64_t k, const double *__restrict__ s,
+__global__ void k_scale_in(int64_t total, int64_t k, SegMap sm, const double *__restrict__ s,
                            const double *__restrict__ X, T *__restrict__ Y) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x)
-    Y[i] = (T)(X[i] * s[i % k]);
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = i / k;
+    Y[i] = (T)(X[i] * s[sm.vc(row, k, i - row * k)]);
+  }
 }
 
 // X += invs_c * D; D = 0
 template <typename T>
-__global__ void k_x_update(int64_t total, int64_t k, const double *__restrict__ invs,
+__global__ void k_x_update(int64_t total, int64_t k, SegMap sm, const double *__restrict__ invs,
                            T *__restrict__ D, double *__restrict__ X) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
-    X[i] = fma(invs[i % k], (double)D[i], X[i]);
+    const int64_t row = i / k;
+    X[i] = fma(invs[sm.vc(row, k, i - row * k)], (double)D[i], X[i]);
     D[i] = T(0);
   }
 }
@@ -1172,17 +1271,19 @@ static void prof_collect() {
 // ============================================================== workspace ==
 struct SolverWork {
   int32_t n = 0, k = 0, part_rows = 0;
+  int64_t nvc = 0;                    // scalar columns (nseg x k for a batch)
+  DevBuf<double> seg;                 // 2 x nvc per-segment dot sums
   DevBuf<double> xp, rp;              // FP64 n x k: x, true residual
   DevBuf<double> buf[5];              // inner D, R, Z, P, Q (8 bytes/elem)
   DevBuf<double> part;                // 3 x part_rows x k
   DevBuf<double> red;                 // 2 x kRedRows x k, first-stage sums
-  DevBuf<double> st;                  // ST_N x k per-column scalars
-  DevBuf<int32_t> active, restart;    // k
+  DevBuf<double> st;                  // ST_N x nvc per-column scalars
+  DevBuf<int32_t> active, restart;    // nvc
   DevBuf<int32_t> ctl;                // CTL_N ints
   DevBuf<double> dinv64;
   DevBuf<float> dinv32;
   int32_t *h_ctl = nullptr;           // pinned
-  double *h_st = nullptr;             // pinned, ST_N x k
+  double *h_st = nullptr;             // pinned, ST_N x nvc
   ~SolverWork() {
     if (h_ctl) (void)hipHostFree(h_ctl);
     if (h_st) (void)hipHostFree(h_st);
@@ -1194,29 +1295,34 @@ void free_solver_work(SolverWork *w) { delete w; }
 static SolverWork *ensure_work(cn_system_t *S, int32_t k, int32_t part_rows) {
   SolverWork *W = S->work;
   const int32_t n = S->n_free;
-  if (W && W->n == n && W->k >= k && W->part_rows >= part_rows) return W;
+  const int64_t nvc_need = (int64_t)std::max(S->nseg, 1) * k;
+  if (W && W->n == n && W->k >= k && W->part_rows >= part_rows && W->nvc >= nvc_need)
+    return W;
   int32_t kk = std::max(k, W ? W->k : 0);
   int32_t pr = std::max(part_rows, W ? W->part_rows : 0);
+  const int64_t nvc = std::max<int64_t>(std::max<int64_t>(nvc_need, kk), W ? W->nvc : 0);
   delete W;
   S->work = nullptr;
   W = new SolverWork();
   W->n = n;
   W->k = kk;
   W->part_rows = pr;
+  W->nvc = nvc;
+  W->seg.alloc((size_t)2 * nvc + 4);
   size_t nk = (size_t)n * kk + 4;
   W->xp.alloc(nk);
   W->rp.alloc(nk);
   for (auto &b : W->buf) b.alloc(nk);
   W->part.alloc((size_t)3 * pr * kk + 4);
   W->red.alloc((size_t)2 * kRedRows * kk + 4);
-  W->st.alloc((size_t)ST_N * kk + 4);
-  W->active.alloc((size_t)kk + 4);
-  W->restart.alloc((size_t)kk + 4);
+  W->st.alloc((size_t)ST_N * nvc + 4);
+  W->active.alloc((size_t)nvc + 4);
+  W->restart.alloc((size_t)nvc + 4);
   W->ctl.alloc(CTL_N);
   W->dinv64.alloc((size_t)n + 1);
   W->dinv32.alloc((size_t)n + 1);
   CN_HIP(hipHostMalloc((void **)&W->h_ctl, sizeof(int32_t) * CTL_N, hipHostMallocDefault));
-  CN_HIP(hipHostMalloc((void **)&W->h_st, sizeof(double) * ((size_t)ST_N * kk + 4),
+  CN_HIP(hipHostMalloc((void **)&W->h_st, sizeof(double) * ((size_t)ST_N * nvc + 4),
                        hipHostMallocDefault));
   S->work = W;
   return W;
@@ -1332,6 +1438,8 @@ struct SysView {
   const T *dinv;
   cn_hsc_t *hsc = nullptr;  // whole-apply preconditioners (replace ic0 / dinv)
   const cn_sgs_t *sgs = nullptr;
+  // per-(segment, column) scalars of a batch (of_row = nullptr: one system)
+  SegMap sm{nullptr, 0};
   // row table of A_ff (SpMM); ovp = nullptr when no row overflows it
   const int32_t *ecol = nullptr, *ovp = nullptr, *ovc = nullptr;
   const T *eval = nullptr, *ovv = nullptr;
@@ -1496,19 +1604,25 @@ template <typename T, bool FUSED>
 static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
                           const T *Q, T *R, T *Z, double *part, int64_t stride,
                           const int32_t *ctl, hipStream_t s) {
-  if (V.hsc || V.sgs) {
+  if (V.hsc || V.sgs || V.sm.of_row) {
     Grid G = grid_rows<T>(V.n, k, kCapSpmm);
     ProfScope ps("fwd", s);
-#define N_(KK) kn_rupd<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(V.n, alpha, Q, R, part, ctl)
-#define W_(VV) kw_rupd<T, WideV<T, VV>::v, FUSED><<<G.g, kBlock, 0, s>>>(V.n, alpha, k, Q, R, part, ctl)
+#define N_(KK) kn_rupd<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(V.n, alpha, V.sm, Q, R, part, ctl)
+#define W_(VV) kw_rupd<T, WideV<T, VV>::v, FUSED><<<G.g, kBlock, 0, s>>>(V.n, alpha, V.sm, k, Q, R, part, ctl)
     DISPATCH_K(k, T, N_, W_);
 #undef N_
 #undef W_
-    if (V.hsc)
+    if (V.hsc) {
       hsc_apply<T>(V.hsc, k, R, Z, s, ctl);
-    else
+    } else if (V.sgs) {
       sgs_apply<T>(V.sgs, k, R, Z, s, ctl);
-    launch_dot<T>(V.n, k, R, Z, part + stride, s);
+    } else {
+      const int64_t total = (int64_t)V.n * k;
+      k_diag_apply<T><<<(unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192)),
+                        256, 0, s>>>(total, k, V.dinv, R, Z, ctl);
+    }
+    // a batch takes its per-segment dots separately (launch_segdot)
+    if (!V.sm.of_row) launch_dot<T>(V.n, k, R, Z, part + stride, s);
     return G.rows;
   }
   if (!V.ic0) {
@@ -1565,7 +1679,7 @@ static void launch_bwd(const SysView<T> &V, int64_t k, const double *st, T *Z, T
     ProfScope ps("bwd", s);
     const int64_t total = (int64_t)V.n * k;
     unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192));
-    k_pd_update<T><<<g, 256, 0, s>>>(total, k, st, Z, P, D, ctl);
+    k_pd_update<T><<<g, 256, 0, s>>>(total, k, V.sm, st, Z, P, D, ctl);
     return;
   }
   const std::vector<int32_t> &lb = V.ic0->level_blk;
@@ -1624,6 +1738,36 @@ static int32_t launch_dot(int32_t n, int64_t k, const T *A, const T *B, double *
 #undef N_
 #undef W_
   return G.rows;
+}
+
+// Per-(segment, column) dots a.b (and c.d when c != nullptr) of a batch
+// into W->seg (then W->seg + nvc): one partial row per segment-aligned
+// block, summed per segment in block order.
+template <typename TA, typename TB>
+static void launch_segdot(const cn_system_t *S, SolverWork *W, int64_t k, const TA *a,
+                          const TB *b, const TA *c, const TB *d, hipStream_t s) {
+  const int np = c ? 2 : 1;
+  double *part = W->part.get();
+  const int64_t stride = (int64_t)W->part_rows * k;
+  const dim3 g((unsigned)S->seg_blocks, k <= 4 ? 1u : (unsigned)((k + 63) / 64), 1);
+  const int32_t *sp = S->d_seg_ptr.get(), *bp = S->blk_ptr.get(), *bs = S->blk_seg.get();
+  if (k <= 4) {
+#define N_(KK)                                                                              \
+  if (np == 2)                                                                              \
+    kn_segdot<TA, TB, KK, 2><<<g, kBlock, 0, s>>>(sp, bp, bs, a, b, c, d, part, stride);    \
+  else                                                                                      \
+    kn_segdot<TA, TB, KK, 1><<<g, kBlock, 0, s>>>(sp, bp, bs, a, b, c, d, part, stride);
+    switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
+#undef N_
+  } else if (np == 2) {
+    kw_segdot<TA, TB, 2><<<g, kBlock, 0, s>>>(sp, bp, bs, k, a, b, c, d, part, stride);
+  } else {
+    kw_segdot<TA, TB, 1><<<g, kBlock, 0, s>>>(sp, bp, bs, k, a, b, c, d, part, stride);
+  }
+  const int64_t nvc = (int64_t)S->nseg * k;
+  k_segsum<<<grid_for(np * nvc, 256), 256, 0, s>>>(S->nseg, bp, k, np, part, stride,
+                                                    W->seg.get());
+  CN_HIP(hipGetLastError());
 }
 
 static unsigned ew_blocks(int64_t total) {
@@ -1763,8 +1907,13 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   V0.n = n;
   V0.ic0 = F;
   if (F) V0.mb = F->sweep_mb;
+  ensure_segments(S, s);
+  // a batch keeps its scalars per (segment, column): kk virtual columns
+  const bool batch = S->nseg > 1;
+  const int64_t kk = (int64_t)S->nseg * k;
   int32_t need = std::max<int32_t>(fwd_rows<T>(V0, k), grid_spmm<T>(n, k).rows);
   need = std::max<int32_t>(need, grid_spmm<double>(n, k).rows);
+  need = std::max<int32_t>(need, S->seg_blocks);
   SolverWork *W = ensure_work(S, k, need);
   k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
                                          W->dinv32.get());
@@ -1772,13 +1921,15 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   SysView<T> V = make_view<T>(S, F, W);
   V.hsc = H;
   V.sgs = G;
+  V.sm = SegMap{batch ? S->seg_of_row.get() : nullptr, kk};
   double *st = W->st.get();
+  double *sg0 = W->seg.get(), *sg1 = sg0 + kk;  // per-segment dot sums
   double *hs = W->h_st;
   int32_t *ctl = W->ctl.get();
   int32_t *hc = W->h_ctl;
   const int64_t stride = (int64_t)W->part_rows * k;
   double *p0 = W->part.get(), *p1 = p0 + stride;
-  const unsigned red = (unsigned)((k + 63) / 64);
+  const unsigned red = (unsigned)((kk + 63) / 64);
   T *D = reinterpret_cast<T *>(W->buf[0].get());
   T *R = reinterpret_cast<T *>(W->buf[1].get());
   T *Z = reinterpret_cast<T *>(W->buf[2].get());
@@ -1793,17 +1944,27 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     CN_HIP(hipMemcpyAsync(X, d_x, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
   else
     CN_HIP(hipMemsetAsync(X, 0, sizeof(double) * total, s));
-  int32_t rows = launch_dot<double>(n, k, d_b, d_b, p0, s);
-  Red rd = reduce_rows(W, p0, rows, k, stride, 1, s);
-  k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_BB * k);
-  rows = launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
-  rd = reduce_rows(W, p0, rows, k, stride, 1, s);
-  k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
+  int32_t rows;
+  Red rd;
+  if (batch) {
+    launch_segdot<double, double>(S, W, k, d_b, d_b, nullptr, nullptr, s);
+    k_reduce_plain<<<red, kBlock, 0, s>>>(1, kk, sg0, st + ST_BB * kk);
+    launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
+    launch_segdot<double, double>(S, W, k, R64, R64, nullptr, nullptr, s);
+    k_reduce_plain<<<red, kBlock, 0, s>>>(1, kk, sg0, st + ST_RR * kk);
+  } else {
+    rows = launch_dot<double>(n, k, d_b, d_b, p0, s);
+    rd = reduce_rows(W, p0, rows, k, stride, 1, s);
+    k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_BB * k);
+    rows = launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
+    rd = reduce_rows(W, p0, rows, k, stride, 1, s);
+    k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
+  }
   CN_HIP(hipMemsetAsync(D, 0, sizeof(T) * total, s));
   CN_HIP(hipMemsetAsync(P, 0, sizeof(T) * total, s));
   CN_HIP(hipGetLastError());
 
-  std::vector<int32_t> act(k, 0), rst(k, 0);
+  std::vector<int32_t> act(kk, 0), rst(kk, 0);
   int32_t iters = 0, launched = 0, updates = 0;
   bool r_scaled = false;  // R already holds the scaled residual of this round
   cn_status status = CN_OK;
@@ -1811,13 +1972,13 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   bool first = true;
   for (;;) {
     // ---- read the true FP64 residual, decide which columns continue
-    CN_HIP(hipMemcpyAsync(hs + ST_BB * k, st + ST_BB * k, sizeof(double) * 2 * k,
+    CN_HIP(hipMemcpyAsync(hs + ST_BB * kk, st + ST_BB * kk, sizeof(double) * 2 * kk,
                           hipMemcpyDeviceToHost, s));  // BB and RR adjacent
     CN_HIP(hipStreamSynchronize(s));
     relmax = 0.0;
     int32_t nact = 0;
-    for (int32_t c = 0; c < k; ++c) {
-      double bb = hs[ST_BB * k + c], rr = hs[ST_RR * k + c];
+    for (int64_t c = 0; c < kk; ++c) {
+      double bb = hs[ST_BB * kk + c], rr = hs[ST_RR * kk + c];
       double rel = bb > 0.0 ? std::sqrt(rr / bb) : (rr > 0.0 ? INFINITY : 0.0);
       if (!std::isfinite(rel)) rel = INFINITY;
       relmax = std::max(relmax, rel);
@@ -1833,26 +1994,26 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     if (launched >= o->max_iter || updates >= o->max_refine * 8) { status = CN_MAXIT; break; }
     // ---- per-column scale s = 1/||b|| (fixed for the solve)
     if (first) {
-      for (int32_t c = 0; c < k; ++c) {
-        double bb = hs[ST_BB * k + c];
-        hs[ST_S * k + c] = bb > 0.0 ? 1.0 / std::sqrt(bb) : 0.0;
-        hs[ST_INVS * k + c] = bb > 0.0 ? std::sqrt(bb) : 0.0;
+      for (int64_t c = 0; c < kk; ++c) {
+        double bb = hs[ST_BB * kk + c];
+        hs[ST_S * kk + c] = bb > 0.0 ? 1.0 / std::sqrt(bb) : 0.0;
+        hs[ST_INVS * kk + c] = bb > 0.0 ? std::sqrt(bb) : 0.0;
         double t = theta * o->tol_rel;
-        hs[ST_TOL2 * k + c] = t * t;
-        hs[ST_RREF * k + c] = 0.0;
+        hs[ST_TOL2 * kk + c] = t * t;
+        hs[ST_RREF * kk + c] = 0.0;
       }
-      CN_HIP(hipMemcpyAsync(st + ST_TOL2 * k, hs + ST_TOL2 * k, sizeof(double) * 4 * k,
+      CN_HIP(hipMemcpyAsync(st + ST_TOL2 * kk, hs + ST_TOL2 * kk, sizeof(double) * 4 * kk,
                             hipMemcpyHostToDevice, s));  // TOL2, RREF, S, INVS
     }
     // Continuing columns keep p and the iterated r.z (van der Vorst-Ye
     // reliable update): only a (re)start needs z = M^-1 r of the new r.
     bool any_rst = false;
-    for (int32_t c = 0; c < k; ++c) any_rst = any_rst || rst[c];
+    for (int64_t c = 0; c < kk; ++c) any_rst = any_rst || rst[c];
     const bool apply_m = first || any_rst;
     first = false;
-    CN_HIP(hipMemcpyAsync(W->active.get(), act.data(), sizeof(int32_t) * k,
+    CN_HIP(hipMemcpyAsync(W->active.get(), act.data(), sizeof(int32_t) * kk,
                           hipMemcpyHostToDevice, s));
-    CN_HIP(hipMemcpyAsync(W->restart.get(), rst.data(), sizeof(int32_t) * k,
+    CN_HIP(hipMemcpyAsync(W->restart.get(), rst.data(), sizeof(int32_t) * kk,
                           hipMemcpyHostToDevice, s));
     hc[CTL_ACTIVE] = nact;
     hc[CTL_BREAKDOWN] = 0;
@@ -1862,23 +2023,30 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     hc[CTL_ITERS] = iters;
     hc[CTL_PAUSE] = 0;
     CN_HIP(hipMemcpyAsync(ctl, hc, sizeof(int32_t) * 7, hipMemcpyHostToDevice, s));
-    CN_HIP(hipMemsetAsync(st + ST_ALPHA * k, 0, sizeof(double) * k, s));
+    CN_HIP(hipMemsetAsync(st + ST_ALPHA * kk, 0, sizeof(double) * kk, s));
     // after the first round the residual kernel writes the scaled R itself
-    if (!r_scaled) k_scale_in<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_S * k, R64, R);
+    if (!r_scaled)
+      k_scale_in<T><<<ew_blocks(total), 256, 0, s>>>(total, k, V.sm, st + ST_S * kk, R64, R);
     if (apply_m) {
       // z = M^-1 r; rz; continuing columns keep p, restarted ones take p = z
-      rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * k, Q, R, Z, p1, stride, ctl, s);
-      rd = reduce_rows(W, p1, rows, k, stride, 2, s);
-      k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, rd.p + rd.stride, st,
-                                           W->active.get(), W->restart.get(), ctl, 1, delta2);
+      rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, s);
+      if (batch) {
+        launch_segdot<T, T>(S, W, k, R, R, R, Z, s);
+        k_reduce_beta<<<red, kBlock, 0, s>>>(1, kk, sg0, sg1, st, W->active.get(),
+                                             W->restart.get(), ctl, 1, delta2);
+      } else {
+        rd = reduce_rows(W, p1, rows, k, stride, 2, s);
+        k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, rd.p + rd.stride, st,
+                                             W->active.get(), W->restart.get(), ctl, 1, delta2);
+      }
       launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
     } else {
       // new reference for the replacement trigger: the scaled true r.r
-      for (int32_t c = 0; c < k; ++c) {
-        const double sc = hs[ST_S * k + c];
-        hs[ST_RREF * k + c] = hs[ST_RR * k + c] * sc * sc;
+      for (int64_t c = 0; c < kk; ++c) {
+        const double sc = hs[ST_S * kk + c];
+        hs[ST_RREF * kk + c] = hs[ST_RR * kk + c] * sc * sc;
       }
-      CN_HIP(hipMemcpyAsync(st + ST_RREF * k, hs + ST_RREF * k, sizeof(double) * k,
+      CN_HIP(hipMemcpyAsync(st + ST_RREF * kk, hs + ST_RREF * kk, sizeof(double) * kk,
                             hipMemcpyHostToDevice, s));
     }
     CN_HIP(hipGetLastError());
@@ -1890,12 +2058,23 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     const int32_t every = std::max(1, H ? std::min(o->check_every, 2) : o->check_every);
     while (it < budget) {
       rows = launch_spmm_dot<T>(V, k, P, Q, p0, ctl, s);
-      rd = reduce_rows(W, p0, rows, k, stride, 1, s);
-      k_reduce_alpha<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st, W->active.get(), ctl);
-      rows = launch_fwd<T, true>(V, k, st + ST_ALPHA * k, Q, R, Z, p1, stride, ctl, s);
-      rd = reduce_rows(W, p1, rows, k, stride, 2, s);
-      k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, rd.p + rd.stride, st,
-                                           W->active.get(), W->restart.get(), ctl, 0, delta2);
+      if (batch) {
+        launch_segdot<T, T>(S, W, k, P, Q, nullptr, nullptr, s);
+        k_reduce_alpha<<<red, kBlock, 0, s>>>(1, kk, sg0, st, W->active.get(), ctl);
+      } else {
+        rd = reduce_rows(W, p0, rows, k, stride, 1, s);
+        k_reduce_alpha<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st, W->active.get(), ctl);
+      }
+      rows = launch_fwd<T, true>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, s);
+      if (batch) {
+        launch_segdot<T, T>(S, W, k, R, R, R, Z, s);
+        k_reduce_beta<<<red, kBlock, 0, s>>>(1, kk, sg0, sg1, st, W->active.get(),
+                                             W->restart.get(), ctl, 0, delta2);
+      } else {
+        rd = reduce_rows(W, p1, rows, k, stride, 2, s);
+        k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, rd.p + rd.stride, st,
+                                             W->active.get(), W->restart.get(), ctl, 0, delta2);
+      }
       launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
       ++it;
       if (it % every == 0 || it == budget) {
@@ -1908,11 +2087,18 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     launched += it;
     iters = hc[CTL_ITERS];
     // ---- reliable update: x += d / s (FP64), r = b - A x (FP64)
-    k_x_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, st + ST_INVS * k, D, X);
-    rows = launch_resid<T>(n, V.ptr, V.col, V.val64, k, X, d_b, R, st + ST_S * k, p0, s);
-    r_scaled = true;
-    rd = reduce_rows(W, p0, rows, k, stride, 1, s);
-    k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
+    k_x_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, V.sm, st + ST_INVS * kk, D, X);
+    if (batch) {
+      // FP64 residual, per-segment r.r, scaled into R at the top of the loop
+      launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
+      launch_segdot<double, double>(S, W, k, R64, R64, nullptr, nullptr, s);
+      k_reduce_plain<<<red, kBlock, 0, s>>>(1, kk, sg0, st + ST_RR * kk);
+    } else {
+      rows = launch_resid<T>(n, V.ptr, V.col, V.val64, k, X, d_b, R, st + ST_S * k, p0, s);
+      r_scaled = true;
+      rd = reduce_rows(W, p0, rows, k, stride, 1, s);
+      k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
+    }
     CN_HIP(hipGetLastError());
     ++updates;
     if (it == 0) { status = CN_MAXIT; break; }
@@ -1947,6 +2133,7 @@ static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   SysView<double> V = make_view<double>(S, F, W);
   V.hsc = H;
   V.sgs = G;
+  V.sm = SegMap{nullptr, k};
   double *st = W->st.get();
   const int64_t stride = (int64_t)W->part_rows * k;
   const unsigned red = (unsigned)((k + 63) / 64);
@@ -2107,6 +2294,11 @@ static cn_status solve_impl(cn_system S, cn_ic0 F, cn_hsc H, cn_sgs G, const cn_
   }
   if (G && (G->sys != S || G->n != S->n_free)) {
     set_error("%s: colouring does not belong to the system", who);
+    return CN_BAD_ARG;
+  }
+  if (S->nseg > 1 && (F || o->precision == CN_PREC_FP64_PLAIN)) {
+    set_error("%s: a batch of systems solves with Jacobi, HSC or SGS in CN_PREC_FP64 or "
+              "CN_PREC_MIXED", who);
     return CN_BAD_ARG;
   }
   hipStream_t s = as_stream(stream);

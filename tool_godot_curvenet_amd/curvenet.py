@@ -357,6 +357,40 @@ class CsrSystem(_System):
         super().__init__(h.value)
 
 
+class BatchSystem(_System):
+    """Independent systems solved together (cn_system_batch): a
+    block-diagonal copy of their A_ff.  Right-hand sides and solutions stack
+    the members' rows in order; every (member, column) pair follows its own
+    CG recurrence, so a member's rows of the solution solve its system."""
+
+    def __init__(self, systems):
+        lib = L.load()
+        hs = (C.c_void_p * len(systems))(*[m.handle.value for m in systems])
+        h = C.c_void_p(0)
+        L.check(lib.cn_system_batch(len(systems), C.cast(hs, C.POINTER(C.c_void_p)), C.byref(h),
+                                    _stream()), "system_batch")
+        super().__init__(h.value)
+        nseg = C.c_int32(0)
+        sp = L.P_i32()
+        L.check(lib.cn_system_segments(self._h, C.byref(nseg), C.byref(sp)), "system_segments")
+        self.seg_ptr = np.array([sp[i] for i in range(nseg.value + 1)], np.int64)
+
+    @property
+    def nseg(self) -> int:
+        return len(self.seg_ptr) - 1
+
+    def stack(self, blocks):
+        """Concatenate per-member n_i x k blocks (numpy or device tensors)."""
+        torch = _torch()
+        if all(isinstance(b, torch.Tensor) for b in blocks):
+            return torch.cat([b.reshape(b.shape[0], -1) for b in blocks], dim=0).contiguous()
+        return np.concatenate([np.asarray(b).reshape(len(b), -1) for b in blocks], axis=0)
+
+    def split(self, x):
+        """The members' row blocks of a stacked block."""
+        return [x[self.seg_ptr[i]:self.seg_ptr[i + 1]] for i in range(self.nseg)]
+
+
 class HSC:
     """Hierarchical Schur preconditioner of a system's A_ff (cn_hsc_build):
     greedy independent-set elimination with exact Schur complements,
