@@ -97,6 +97,10 @@ def main():
                     help="also run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--plain-poses", type=int, default=2,
                     help="poses solved by the plain FP64 one-solve-per-launch path")
+    ap.add_argument("--workload", default="capsules100k", choices=["capsules100k", "meshes64"],
+                    help="capsules100k: 256 poses x 3 RHS on the 100k capsule union (the "
+                         "headline); meshes64: 64 uneven meshes x 3 RHS, batched vs one solve "
+                         "per mesh")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="no GPU: turn two rocprofv3 --pmc counter files (FETCH_SIZE, "
                          "WRITE_SIZE) of a --copy-mb run into the traffic file")
@@ -118,6 +122,10 @@ def main():
     assert torch.cuda.is_available(), "no GPU"
     lib = _lib.load()
     dev = torch.device("cuda:0")
+
+    if args.workload == "meshes64":
+        meshes64_bench(args)
+        return
 
     if args.copy_mb > 0:
         # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters
@@ -356,6 +364,126 @@ def main():
     }
     if plain:
         line["speedup_vs_plain_fp64"] = value / plain["solves_per_s"]
+    print(json.dumps(line))
+
+
+def meshes64_bench(args):
+    """64 capsule unions of log-uniform size (2k-200k vertices), 3 RHS
+    each, solved to 1e-9: all of them as one batch (one launch per solver
+    step covers every mesh, HSC on the block-diagonal system) against one
+    solve per mesh (each with its own HSC) on the same inputs."""
+    import numpy as np
+    import scipy.sparse as sps
+    import torch
+
+    import tool_godot_curvenet_amd as cn
+    from tool_godot_curvenet_amd import _lib, meshgen
+    import ctypes as C
+
+    assert torch.cuda.is_available(), "no GPU"
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    meshes = meshgen.meshes64()
+    print(f"meshes64: {len(meshes)} meshes, {sum(m.nv for m in meshes)} vertices generated "
+          f"in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    laps = [cn.CutLaplacian.from_mesh(m) for m in meshes]
+    batch = cn.BatchSystem(laps)
+    hb = cn.HSC(batch, theta=args.theta, max_levels=48)
+    hs = [cn.HSC(l, theta=args.theta) for l in laps]
+    print(f"meshes64: systems and hierarchies built at {time.time() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    k = 3
+    N = batch.n_free
+    rng = np.random.default_rng(meshgen.BASE_SEED + 40)
+    Xc = [torch.as_tensor(rng.uniform(-0.2, 0.2, size=(l.n_con, k)), device=dev) for l in laps]
+    Bb = torch.empty((N, k), dtype=torch.float64, device=dev)
+    Xb = torch.zeros_like(Bb)
+    Bs = [torch.empty((l.n_free, k), dtype=torch.float64, device=dev) for l in laps]
+    Xs = [torch.zeros_like(b) for b in Bs]
+    seg = batch.seg_ptr
+    sp = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    opts = _lib.PcgOpts()
+    lib.cn_pcg_default_opts(opts)
+    opts.precision = _lib.CN_PREC_MIXED if args.precision == "mixed" else _lib.CN_PREC_FP64
+    opts.tol_rel = args.tol
+    info = _lib.SolveInfo()
+
+    def rhs_batch():
+        for i, l in enumerate(laps):
+            _lib.check(lib.cn_harmonic_rhs(l.handle, k, C.c_void_p(Xc[i].data_ptr()),
+                                           C.c_void_p(Bb[seg[i]:seg[i + 1]].data_ptr()), sp),
+                       "harmonic_rhs")
+
+    def step_batch():
+        rhs_batch()
+        _lib.check(lib.cn_pcg_solve_hsc(batch.handle, hb.handle, C.byref(opts), k,
+                                        C.c_void_p(Bb.data_ptr()), C.c_void_p(Xb.data_ptr()),
+                                        C.byref(info), sp), "pcg_solve_hsc")
+        return info.iters
+
+    def step_each():
+        it = 0
+        for i, l in enumerate(laps):
+            _lib.check(lib.cn_harmonic_rhs(l.handle, k, C.c_void_p(Xc[i].data_ptr()),
+                                           C.c_void_p(Bs[i].data_ptr()), sp), "harmonic_rhs")
+            _lib.check(lib.cn_pcg_solve_hsc(l.handle, hs[i].handle, C.byref(opts), k,
+                                            C.c_void_p(Bs[i].data_ptr()),
+                                            C.c_void_p(Xs[i].data_ptr()), C.byref(info), sp),
+                       "pcg_solve_hsc")
+            it = max(it, info.iters)
+        return it
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        its = [fn() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, its
+
+    eb, itb = timed(step_batch)
+    print(f"meshes64: batched {len(laps) * args.steps / eb:.1f} solves/s", file=sys.stderr,
+          flush=True)
+    ee, ite = timed(step_each)
+    # FP64 residual of every mesh, both paths, on the CPU (not timed)
+    worst = [0.0, 0.0]
+    Xbh, Bbh = Xb.cpu().numpy(), Bb.cpu().numpy()
+    for i, l in enumerate(laps):
+        A = l.A_ff()
+        M = sps.csr_matrix((A.values, A.col_idx, A.row_ptr), shape=(l.n_free, l.n_free))
+        for j, (x, b) in enumerate(((Xbh[seg[i]:seg[i + 1]], Bbh[seg[i]:seg[i + 1]]),
+                                    (Xs[i].cpu().numpy(), Bs[i].cpu().numpy()))):
+            r = np.linalg.norm(b - M @ x, axis=0) / np.linalg.norm(b, axis=0)
+            worst[j] = max(worst[j], float(r.max()))
+    value = len(laps) * args.steps / eb
+    each = len(laps) * args.steps / ee
+    sizes = [l.n_free for l in laps]
+    line = {
+        "metric": "solves/s at 1e-9 rel. residual (64 uneven meshes, 3 RHS each)",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * eb / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 refinement" if args.precision == "mixed" else "f64",
+        "data": "synthetic (64 seeded capsule unions, seeded constraint displacements)",
+        "config": {"workload": f"64 capsule unions, n_free {min(sizes)}-{max(sizes)} "
+                               f"(sum {N}), nnz {batch.info.nnz_ff}, 3 RHS each, HSC theta "
+                               f"{args.theta} ({hb.info.nlevels} levels, coarse "
+                               f"{hb.info.n_coarse}), tol {args.tol}"},
+        "one_solve_per_mesh": {"solves_per_s": each, "ms_per_step": 1e3 * ee / args.steps,
+                               "max_iters": max(ite)},
+        "speedup_vs_one_solve_per_mesh": value / each,
+        "solve": {"inner_iters_per_step": sum(itb) / len(itb),
+                  "relres_max_batched": worst[0], "relres_max_per_mesh": worst[1],
+                  "setup_s": time.time() - t0 - eb - ee},
+    }
     print(json.dumps(line))
 
 

@@ -84,37 +84,85 @@ kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv
   }
 }
 
-// ---- scalar form: one thread per (row, column) --------------------------
-template <typename T>
-__global__ void k_hsc_down_s(int32_t nC, const int32_t *__restrict__ Cg,
-                             const int32_t *__restrict__ rptr, const int32_t *__restrict__ rcol,
-                             const T *__restrict__ rval, int64_t k, const T *src, T *Z,
-                             const int32_t *__restrict__ ctl) {
+// ---- narrow form (k <= 4): one thread per row, its K columns in registers
+// (the row's index entries are read once for all columns) ---------------
+template <typename T, int K>
+__global__ void __launch_bounds__(256)
+k_hsc_down_n(int32_t nC, const int32_t *__restrict__ Cg, const int32_t *__restrict__ rptr,
+             const int32_t *__restrict__ rcol, const T *__restrict__ rval, const T *src, T *Z,
+             const int32_t *__restrict__ ctl) {
   if (ctl && ctl_idle(ctl)) return;
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= (int64_t)nC * k) return;
-  const int32_t r = (int32_t)(id / k);
-  const int64_t c = id - (int64_t)r * k;
-  const int64_t o = (int64_t)Cg[r] * k + c;
-  T acc = src[o];
-  for (int32_t e = rptr[r]; e < rptr[r + 1]; ++e) acc = fma(rval[e], src[(int64_t)rcol[e] * k + c], acc);
-  Z[o] = acc;
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nC) return;
+  const int64_t o = (int64_t)Cg[r] * K;
+  T acc[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) acc[c] = src[o + c];
+  const int32_t e1 = rptr[r + 1];
+  for (int32_t e = rptr[r]; e < e1; ++e) {
+    const T w = rval[e];
+    const int64_t q = (int64_t)rcol[e] * K;
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] = fma(w, src[q + c], acc[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < K; ++c) Z[o + c] = acc[c];
 }
 
-template <typename T>
-__global__ void k_hsc_up_s(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv,
-                           const int32_t *__restrict__ pptr, const int32_t *__restrict__ pcol,
-                           const T *__restrict__ pval, int64_t k, const T *src, T *Z,
-                           const int32_t *__restrict__ ctl) {
+template <typename T, int K>
+__global__ void __launch_bounds__(256)
+k_hsc_up_n(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv,
+           const int32_t *__restrict__ pptr, const int32_t *__restrict__ pcol,
+           const T *__restrict__ pval, const T *src, T *Z, const int32_t *__restrict__ ctl) {
   if (ctl && ctl_idle(ctl)) return;
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= (int64_t)nF * k) return;
-  const int32_t f = (int32_t)(id / k);
-  const int64_t c = id - (int64_t)f * k;
-  const int64_t o = (int64_t)Fg[f] * k + c;
-  T acc = dinv[f] * src[o];
-  for (int32_t e = pptr[f]; e < pptr[f + 1]; ++e) acc = fma(pval[e], Z[(int64_t)pcol[e] * k + c], acc);
-  Z[o] = acc;
+  const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nF) return;
+  const int64_t o = (int64_t)Fg[f] * K;
+  const T d = dinv[f];
+  T acc[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) acc[c] = d * src[o + c];
+  const int32_t e1 = pptr[f + 1];
+  for (int32_t e = pptr[f]; e < e1; ++e) {
+    const T w = pval[e];
+    const int64_t q = (int64_t)pcol[e] * K;
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] = fma(w, Z[q + c], acc[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < K; ++c) Z[o + c] = acc[c];
+}
+
+// z[rows[i]] = sum_j inv[i][j] W[j] for K <= 4 columns: wavefront i of the
+// grid takes coarse row i, lane l the entries j = l, l + 64, ... (FP64),
+// and the lane sums meet in a fixed butterfly
+template <typename T, int K>
+__global__ void __launch_bounds__(kBlock)
+k_coarse_gemv(int32_t nc, const double *__restrict__ inv, const T *__restrict__ W,
+              const int32_t *__restrict__ rows, T *__restrict__ z,
+              const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
+  const int32_t i = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= nc) return;
+  double acc[K] = {};
+  const double *a = inv + (int64_t)i * nc;
+  for (int32_t j = lane; j < nc; j += 64) {
+    const double m = a[j];
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] += m * (double)W[(int64_t)j * K + c];
+  }
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+  }
+  if (lane < K) {
+    double v = acc[0];
+#pragma unroll
+    for (int c = 1; c < K; ++c) v = lane == c ? acc[c] : v;
+    z[(int64_t)rows[i] * K + lane] = (T)v;
+  }
 }
 
 // W[t] = src[rows[t]] (the coarse rows, gathered for the GEMM)
@@ -161,9 +209,9 @@ static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
   if (L.nC == 0) return;
   const int v = hsc_vec<T>(k);
   if (v == 0) {
-    const int64_t tot = (int64_t)L.nC * k;
-    k_hsc_down_s<T><<<grid_for(tot, 256), 256, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(),
-                                                     L.r_colg.get(), LvVals<T>::r(L), k, src, Z, ctl);
+#define N_(KK) k_hsc_down_n<T, KK><<<grid_for(L.nC, 256), 256, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(), L.r_colg.get(), LvVals<T>::r(L), src, Z, ctl)
+    switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
+#undef N_
     return;
   }
   const dim3 g = hsc_grid(L.nC, k, v);
@@ -181,10 +229,9 @@ static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
   if (L.nF == 0) return;
   const int v = hsc_vec<T>(k);
   if (v == 0) {
-    const int64_t tot = (int64_t)L.nF * k;
-    k_hsc_up_s<T><<<grid_for(tot, 256), 256, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L),
-                                                   L.p_ptr.get(), L.p_colg.get(),
-                                                   LvVals<T>::p(L), k, src, Z, ctl);
+#define N_(KK) k_hsc_up_n<T, KK><<<grid_for(L.nF, 256), 256, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L), L.p_ptr.get(), L.p_colg.get(), LvVals<T>::p(L), src, Z, ctl)
+    switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
+#undef N_
     return;
   }
   const dim3 g = hsc_grid(L.nF, k, v);
@@ -215,8 +262,17 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
     const int64_t tot = (int64_t)H->nc * k;
     k_gather_rows<T><<<grid_for(tot, 256), 256, 0, s>>>(H->nc, H->coarse_rows.get(), k,
                                                       L == 0 ? r : z, W, ctl);
-    dense_gemm_nn_rows<T, T>(H->nc, (int)k, H->coarse_inv.get(), W, k, z, k,
-                             H->coarse_rows.get(), ctl, s);
+    if (k <= 4) {
+      // a few columns: one wavefront per coarse row beats padded MFMA tiles
+#define G_(KK)                                                                            \
+  k_coarse_gemv<T, KK><<<(unsigned)((H->nc + kWaves - 1) / kWaves), kBlock, 0, s>>>(      \
+      H->nc, H->coarse_inv.get(), W, H->coarse_rows.get(), z, ctl)
+      switch (k) { case 1: G_(1); break; case 2: G_(2); break; case 3: G_(3); break; default: G_(4); }
+#undef G_
+    } else {
+      dense_gemm_nn_rows<T, T>(H->nc, (int)k, H->coarse_inv.get(), W, k, z, k,
+                               H->coarse_rows.get(), ctl, s);
+    }
   }
   {
     ProfScope ps("hsc_up", s);

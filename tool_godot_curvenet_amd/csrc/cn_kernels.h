@@ -232,6 +232,26 @@ __device__ __forceinline__ void block_partials_narrow(const double (&p)[NP],
   }
 }
 
+// Per-thread partial reduction (thread t holds K columns) for the
+// thread-per-block kernels: fixed-order sum over the block's 256 threads.
+template <int K, int NP>
+__device__ __forceinline__ void block_partials_thread(const double (&p)[NP][K],
+                                                      double *__restrict__ out,
+                                                      int64_t stride_np) {
+  __shared__ double sh[NP][K][kBlock];
+#pragma unroll
+  for (int q = 0; q < NP; ++q)
+#pragma unroll
+    for (int c = 0; c < K; ++c) sh[q][c][threadIdx.x] = p[q][c];
+  __syncthreads();
+  if (threadIdx.x < NP * K) {
+    const int q = threadIdx.x / K, c = threadIdx.x % K;
+    double t = 0.0;
+    for (int i = 0; i < kBlock; ++i) t += sh[q][c][i];
+    out[q * stride_np + (int64_t)blockIdx.x * K + c] = t;
+  }
+}
+
 // register-array element by a lane-varying index (K <= 4)
 template <typename T, int K>
 __device__ __forceinline__ T pick(const T (&a)[K], int g) {

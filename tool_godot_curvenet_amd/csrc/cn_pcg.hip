@@ -80,24 +80,52 @@ struct SegMap {
   const bool valid = c0 < k;
 
 // ---- SpMM with partial p.q -------------------------------------------
+// Narrow form (k <= 4): one thread per row of the row table, its K columns
+// in registers; slot 0 is the row itself, so its gather also serves p.q.
 template <typename T, int K>
 __global__ void __launch_bounds__(kBlock)
-kn_spmm_dot(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-            const T *__restrict__ val, const T *__restrict__ P, T *__restrict__ Q,
+kn_spmm_dot(int32_t n, const int32_t *__restrict__ ecol, const T *__restrict__ eval,
+            const int32_t *__restrict__ ovp, const int32_t *__restrict__ ovc,
+            const T *__restrict__ ovv, const T *__restrict__ P, T *__restrict__ Q,
             double *__restrict__ part, const int32_t *__restrict__ ctl) {
   if (ctl_idle(ctl)) return;
-  const int g = threadIdx.x % kGroup;
-  double pd[1] = {0.0};
-  NARROW_LOOP(row, 0, n) {
+  double pd[1][K] = {};
+  const RowIter it = row_iter(0, n, kBlock, threadIdx.x);
+  for (int32_t r = it.first; r < it.end; r += it.stride) {
+    const int64_t e0 = (int64_t)r * kEll;
+    int32_t j[kEll];
+    T w[kEll];
+#pragma unroll
+    for (int u = 0; u < kEll; ++u) {
+      j[u] = ecol[e0 + u];
+      w[u] = eval[e0 + u];
+    }
+    T x[kEll][K];
+#pragma unroll
+    for (int u = 0; u < kEll; ++u)
+#pragma unroll
+      for (int c = 0; c < K; ++c) x[u][c] = P[(int64_t)j[u] * K + c];
     T acc[K] = {};
-    gather_narrow<T, K>(ptr[row], ptr[row + 1], g, col, val, P, acc);
-    if (g < K) {
-      T q = pick<T, K>(acc, g);
-      Q[row * K + g] = q;
-      pd[0] += (double)P[row * K + g] * (double)q;
+#pragma unroll
+    for (int u = 0; u < kEll; ++u)
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[c] = fma(w[u], x[u][c], acc[c]);
+    if (ovp) {
+      const int32_t e1 = ovp[r + 1];
+      for (int32_t e = ovp[r]; e < e1; ++e) {
+        const T v = ovv[e];
+        const int64_t q = (int64_t)ovc[e] * K;
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc[c] = fma(v, P[q + c], acc[c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      Q[(int64_t)r * K + c] = acc[c];
+      pd[0][c] += (double)x[0][c] * (double)acc[c];
     }
   }
-  block_partials_narrow<K, 1>(pd, part, 0);
+  block_partials_thread<K, 1>(pd, part, 0);
 }
 
 // Wide form over the row table: each wave walks its rows with the next
@@ -302,26 +330,6 @@ kn_fwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
     }
   }
   block_partials_narrow<K, 2>(pd, part, stride);
-}
-
-// Per-thread partial reduction (thread t holds K columns) for the
-// thread-per-block kernels: fixed-order sum over the block's 256 threads.
-template <int K, int NP>
-__device__ __forceinline__ void block_partials_thread(const double (&p)[NP][K],
-                                                      double *__restrict__ out,
-                                                      int64_t stride_np) {
-  __shared__ double sh[NP][K][kBlock];
-#pragma unroll
-  for (int q = 0; q < NP; ++q)
-#pragma unroll
-    for (int c = 0; c < K; ++c) sh[q][c][threadIdx.x] = p[q][c];
-  __syncthreads();
-  if (threadIdx.x < NP * K) {
-    const int q = threadIdx.x / K, c = threadIdx.x % K;
-    double t = 0.0;
-    for (int i = 0; i < kBlock; ++i) t += sh[q][c][i];
-    out[q * stride_np + (int64_t)blockIdx.x * K + c] = t;
-  }
 }
 
 // multi-row blocks, k <= 4: one thread per block, all K columns
@@ -1150,21 +1158,22 @@ kw_segdot(const int32_t *__restrict__ seg_ptr, const int32_t *__restrict__ blk_p
   block_partials_wide<1, NP>(pd, part, k, c0, stride);
 }
 
-// out[q * kk + s * k + c] = sum over the blocks of segment s, in order, of
-// their partial (q, c)
-__global__ void k_segsum(int32_t nseg, const int32_t *__restrict__ blk_ptr, int64_t k, int np,
-                         const double *__restrict__ part, int64_t stride,
-                         double *__restrict__ out) {
-  const int64_t kk = (int64_t)nseg * k;
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= (int64_t)np * kk) return;
-  const int q = (int)(id / kk);
-  const int64_t v = id - q * kk;
-  const int32_t sg = (int32_t)(v / k);
-  const int64_t c = v - (int64_t)sg * k;
-  double t = 0.0;
-  for (int32_t b = blk_ptr[sg]; b < blk_ptr[sg + 1]; ++b) t += part[q * stride + (int64_t)b * k + c];
-  out[id] = t;
+// out[q * kk + s * k + c] = sum over the blocks of segment s of their
+// partial (q, c): one wavefront per (segment, quantity), lane l takes
+// blocks l, l + 64, ..., and the lane sums meet in a fixed butterfly
+__global__ void __launch_bounds__(64)
+k_segsum(int32_t nseg, const int32_t *__restrict__ blk_ptr, int64_t k,
+         const double *__restrict__ part, int64_t stride, double *__restrict__ out) {
+  const int32_t sg = blockIdx.x;
+  const int q = blockIdx.y, lane = threadIdx.x;
+  const int32_t b0 = blk_ptr[sg], b1 = blk_ptr[sg + 1];
+  for (int64_t c = 0; c < k; ++c) {
+    double t = 0.0;
+    for (int32_t b = b0 + lane; b < b1; b += 64) t += part[q * stride + (int64_t)b * k + c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (lane == 0) out[(int64_t)q * nseg * k + (int64_t)sg * k + c] = t;
+  }
 }
 
 // Jacobi as a whole apply (batches): z = dinv r, zero diagonal -> 0
@@ -1456,6 +1465,12 @@ static const unsigned kSpmmGens = 4;
 
 template <typename T>
 static Grid grid_spmm(int64_t n, int64_t k) {
+  if (k <= 4) {  // one thread per row
+    Grid G;
+    G.g = dim3(blocks_for_rows(n, kBlock, kCapSpmm), 1, 1);
+    G.rows = (int32_t)G.g.x;
+    return G;
+  }
   Grid G = grid_rows<T>(n, k, kCapSpmm);
   if (k > 4 && G.g.x % kXcd == 0) {
     G.g.x *= kSpmmGens;
@@ -1470,7 +1485,7 @@ static int32_t launch_spmm_dot(const SysView<T> &V, int64_t k, const T *P, T *Q,
   Grid G = grid_spmm<T>(V.n, k);
   const int32_t gens = (int32_t)(G.g.x / std::max(1u, grid_rows<T>(V.n, k, kCapSpmm).g.x));
   ProfScope ps("spmm", s);
-#define N_(KK) kn_spmm_dot<T, KK><<<G.g, kBlock, 0, s>>>(V.n, V.ptr, V.col, V.val, P, Q, part, ctl)
+#define N_(KK) kn_spmm_dot<T, KK><<<G.g, kBlock, 0, s>>>(V.n, V.ecol, V.eval, V.ovp, V.ovc, V.ovv, P, Q, part, ctl)
 #define W_(VV) kw_spmm_dot<T, WideV<T, VV>::v><<<G.g, kBlock, 0, s>>>(V.n, V.ecol, V.eval, V.ovp, V.ovc, V.ovv, k, P, Q, part, ctl, gens)
   DISPATCH_K(k, T, N_, W_);
 #undef N_
@@ -1764,9 +1779,8 @@ static void launch_segdot(const cn_system_t *S, SolverWork *W, int64_t k, const 
   } else {
     kw_segdot<TA, TB, 1><<<g, kBlock, 0, s>>>(sp, bp, bs, k, a, b, c, d, part, stride);
   }
-  const int64_t nvc = (int64_t)S->nseg * k;
-  k_segsum<<<grid_for(np * nvc, 256), 256, 0, s>>>(S->nseg, bp, k, np, part, stride,
-                                                    W->seg.get());
+  k_segsum<<<dim3((unsigned)S->nseg, (unsigned)np, 1), 64, 0, s>>>(S->nseg, bp, k, part,
+                                                                    stride, W->seg.get());
   CN_HIP(hipGetLastError());
 }
 
@@ -1786,6 +1800,9 @@ struct Red {
 };
 static Red reduce_rows(SolverWork *W, const double *part, int32_t nrows, int64_t k,
                        int64_t stride, int np, hipStream_t s) {
+  // every partial-writing grid is sized into part_rows when the workspace
+  // is set up; a grid past it would have written out of the buffer
+  if (nrows > W->part_rows) throw HipError{hipErrorInvalidValue, "partial rows > workspace"};
   if (nrows <= kRedRows) return Red{part, nrows, stride};
   int32_t S = kRedRows;
   int32_t chunk = (nrows + S - 1) / S;
@@ -1911,8 +1928,12 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   // a batch keeps its scalars per (segment, column): kk virtual columns
   const bool batch = S->nseg > 1;
   const int64_t kk = (int64_t)S->nseg * k;
+  // partial rows: the widest of every row grid that writes partials (the
+  // SpMM's own, the row kernels' -- rupd, dot, residual -- and the sweeps')
   int32_t need = std::max<int32_t>(fwd_rows<T>(V0, k), grid_spmm<T>(n, k).rows);
   need = std::max<int32_t>(need, grid_spmm<double>(n, k).rows);
+  need = std::max<int32_t>(need, grid_rows<T>(n, k, kCapSpmm).rows);
+  need = std::max<int32_t>(need, grid_rows<double>(n, k, kCapSpmm).rows);
   need = std::max<int32_t>(need, S->seg_blocks);
   SolverWork *W = ensure_work(S, k, need);
   k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
@@ -2126,6 +2147,7 @@ static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   V0.ic0 = F;
   if (F) V0.mb = F->sweep_mb;
   int32_t need = std::max<int32_t>(fwd_rows<double>(V0, k), grid_spmm<double>(n, k).rows);
+  need = std::max<int32_t>(need, grid_rows<double>(n, k, kCapSpmm).rows);
   SolverWork *W = ensure_work(S, k, need);
   k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
                                          W->dinv32.get());

@@ -401,6 +401,20 @@ def config_mesh(name: str) -> Mesh:
     raise KeyError(name)
 
 
+def meshes64(n: int = 64, lo: int = 2_000, hi: int = 200_000,
+             seed: int = BASE_SEED + 6) -> list:
+    """Many uneven meshes (the C3b workload): n capsule unions whose vertex
+    counts are exp(U(ln lo, ln hi)); a union of t vertices has
+    clip(round(t / 9000), 1, 12) capsules and a third more ring cuts."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(lo), math.log(hi), size=n))
+    out = []
+    for i, t in enumerate(sizes):
+        caps = int(np.clip(round(t / 9000), 1, 12))
+        out.append(capsule_union(caps, caps + 1 + caps // 3, int(t), seed=seed + 1 + i))
+    return out
+
+
 def seeded_boundary(n_con: int, k: int = 3, amp: float = 0.2, seed: int = BASE_SEED + 10) -> np.ndarray:
     """Seeded boundary displacements U(-amp, amp), n_con x k."""
     rng = np.random.default_rng(seed)
