@@ -446,6 +446,13 @@ cn_status cn_adjoint_solve(cn_system sys, cn_ic0 ic0, const cn_pcg_opts *opts,
                            const double *d_xc, double *d_lambda, double *d_gc,
                            double *d_gw, cn_solve_info *info, void *stream);
 
+/* cn_adjoint_solve with lambda solved by the HSC-preconditioned PCG
+ * (cn_pcg_solve_hsc, same hierarchy as the forward solve). */
+cn_status cn_adjoint_solve_hsc(cn_system sys, cn_hsc hsc, const cn_pcg_opts *opts,
+                               int32_t k, const double *d_g, const double *d_x,
+                               const double *d_xc, double *d_lambda, double *d_gc,
+                               double *d_gw, cn_solve_info *info, void *stream);
+
 /* Per-kernel timing of the last cn_pcg_solve on this thread, gathered with
  * HIP events around every launch of the named kernel when enabled.
  * name: "spmm" (FP32 SpMM of the inner loop), "fwd" / "bwd" (forward and

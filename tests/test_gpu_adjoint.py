@@ -107,3 +107,44 @@ def test_bspline_control_gradient_matches_fd(gpu, precision, tol):
     err = np.abs(g_ad[params] - fd)
     scale = np.abs(g_ad).max()
     assert np.all(err <= 1e-5 * np.maximum(np.abs(fd), 1e-2 * scale)), (err, fd)
+
+
+def test_bspline_gradient_sphere224_hsc(gpu):
+    """The large gradient workload: geodesic sphere nu = 224 (501,762
+    vertices, 8 loops x 16 control points), HSC in the forward solve, the
+    adjoint and the finite differences; 32 parameters against central
+    differences (h = 1e-6)."""
+    import torch
+    from tool_godot_curvenet_amd import HSC, curves, meshgen as G
+    from tool_godot_curvenet_amd.autograd import harmonic_solve
+    m = G.config_mesh("sphere224")
+    lap = _lap(m)
+    assert m.nv == 501_762
+    hsc = HSC(lap)
+    Bm = torch.tensor(curves.constrained_basis(lap, m, 16), device="cuda")
+    rng = np.random.default_rng(G.BASE_SEED + 7)
+    delta = torch.tensor(rng.normal(0, 0.05, size=(lap.n_free, 3)), device="cuda")
+    ctrl0 = rng.normal(0, 0.1, size=(Bm.shape[1], 3))
+    ctrl = torch.tensor(ctrl0, device="cuda", requires_grad=True)
+    x = harmonic_solve(Bm @ ctrl, lap, tol=1e-12, precision="fp64", hsc=hsc)
+    loss = 0.5 * ((x - delta) ** 2).sum()
+    loss.backward()
+    g_ad = ctrl.grad.cpu().numpy().ravel()
+    h = 1e-6
+    params = rng.choice(ctrl0.size, 32, replace=False)
+    cols = []
+    for p in params:
+        for s in (1.0, -1.0):
+            c = ctrl0.copy().ravel()
+            c[p] += s * h
+            cols.append(c.reshape(ctrl0.shape))
+    XC = torch.cat([Bm @ torch.tensor(c, device="cuda") for c in cols], dim=1)
+    b = lap.harmonic_rhs(XC, as_numpy=False)
+    xs, info = lap.solve(b, precond="hsc", precision="fp64", tol=1e-13, hsc=hsc)
+    assert info.status == 0
+    xs = xs.reshape(lap.n_free, len(cols), 3)
+    losses = 0.5 * ((xs - delta[:, None, :]) ** 2).sum(dim=(0, 2)).cpu().numpy()
+    fd = (losses[0::2] - losses[1::2]) / (2 * h)
+    err = np.abs(g_ad[params] - fd)
+    scale = np.abs(g_ad).max()
+    assert np.all(err <= 1e-5 * np.maximum(np.abs(fd), 1e-2 * scale)), (err, fd)

@@ -114,6 +114,8 @@ _SIGS = {
     "cn_pcg_solve": (i32, [vp, vp, C.POINTER(PcgOpts), i32, vp, vp, C.POINTER(SolveInfo), vp]),
     "cn_adjoint_solve": (i32, [vp, vp, C.POINTER(PcgOpts), i32, vp, vp, vp, vp, vp, vp,
                                C.POINTER(SolveInfo), vp]),
+    "cn_adjoint_solve_hsc": (i32, [vp, vp, C.POINTER(PcgOpts), i32, vp, vp, vp, vp, vp, vp,
+                                   C.POINTER(SolveInfo), vp]),
     "cn_pcg_solve_hsc": (i32, [vp, vp, C.POINTER(PcgOpts), i32, vp, vp, C.POINTER(SolveInfo),
                                vp]),
     "cn_dense_cholesky_f64": (i32, [i32, vp, i32, vp]),

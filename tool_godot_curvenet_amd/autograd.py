@@ -2,15 +2,16 @@
 
 `harmonic_solve(xc, lap)` returns x_f with A_ff x_f = -A_fc x_c (the
 HarmonicSolve RHS of HarmonicSolve.lean:35-66, solved by the library's
-PCG); its backward pass is the library's adjoint (cn_adjoint_solve):
-lambda = A_ff^-1 dL/dx_f with the same IC(0) factor, dL/dx_c = -A_fc^T lambda.
+PCG); its backward pass is the library's adjoint (cn_adjoint_solve, or
+cn_adjoint_solve_hsc): lambda = A_ff^-1 dL/dx_f with the same preconditioner
+(IC(0) factor or HSC hierarchy), dL/dx_c = -A_fc^T lambda.
 Composes with any torch code in front (for example a B-spline basis mapping
 control points to x_c) and behind (a loss on x_f).
 """
 from __future__ import annotations
 
 from . import _lib as L
-from .curvenet import IC0, _p, _stream, _torch
+from .curvenet import HSC, IC0, _p, _stream, _torch
 
 
 def _function():
@@ -18,23 +19,28 @@ def _function():
 
     class HarmonicSolveFn(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, xc, lap, ic0, tol, precision):
+        def forward(ctx, xc, lap, pre, tol, precision):
             xc = xc.detach().to(device="cuda", dtype=torch.float64).contiguous()
             k = xc.shape[1]
             b = torch.empty((lap.n_free, k), dtype=torch.float64, device="cuda")
             L.check(L.load().cn_harmonic_rhs(lap.handle, k, _p(xc), _p(b), _stream()),
                     "harmonic_rhs")
-            x, info = lap.solve(b, precond="ic0", precision=precision, tol=tol, ic0=ic0)
+            kw = {"hsc": pre} if isinstance(pre, HSC) else {"ic0": pre}
+            pc = "hsc" if isinstance(pre, HSC) else "ic0"
+            x, info = lap.solve(b, precond=pc, precision=precision, tol=tol, **kw)
             if info.status != L.CN_OK:
                 raise L.CurvenetError(info.status, "harmonic_solve forward")
-            ctx.lap, ctx.ic0, ctx.tol, ctx.precision = lap, ic0, tol, precision
+            ctx.lap, ctx.pre, ctx.tol, ctx.precision = lap, pre, tol, precision
             return x
 
         @staticmethod
         def backward(ctx, gx):
             gx = gx.detach().to(dtype=torch.float64).contiguous()
-            _, gc, _, info = ctx.lap.adjoint(gx, precond="ic0", precision=ctx.precision,
-                                             tol=ctx.tol, ic0=ctx.ic0)
+            pre = ctx.pre
+            kw = {"hsc": pre} if isinstance(pre, HSC) else {"ic0": pre}
+            pc = "hsc" if isinstance(pre, HSC) else "ic0"
+            _, gc, _, info = ctx.lap.adjoint(gx, precond=pc, precision=ctx.precision,
+                                             tol=ctx.tol, **kw)
             if info.status != L.CN_OK:
                 raise L.CurvenetError(info.status, "harmonic_solve backward")
             return gc, None, None, None, None
@@ -46,12 +52,13 @@ _FN = None
 
 
 def harmonic_solve(xc, lap, ic0: IC0 | None = None, tol: float = 1e-9,
-                   precision: str = "mixed"):
+                   precision: str = "mixed", hsc: HSC | None = None):
     """x_f (n_free x k, FP64, on the GPU) from constrained values xc
-    (n_con x k torch tensor), differentiable in xc."""
+    (n_con x k torch tensor), differentiable in xc.  Preconditioned by the
+    HSC hierarchy `hsc` when given, else by the IC(0) factor `ic0` (built
+    when None)."""
     global _FN
     if _FN is None:
         _FN = _function()
-    if ic0 is None:
-        ic0 = IC0(lap)
-    return _FN.apply(xc, lap, ic0, tol, precision)
+    pre = hsc if hsc is not None else (ic0 if ic0 is not None else IC0(lap))
+    return _FN.apply(xc, lap, pre, tol, precision)

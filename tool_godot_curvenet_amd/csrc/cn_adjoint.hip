@@ -129,28 +129,11 @@ static void ensure_cf(cn_system_t *S, hipStream_t s) {
 
 }  // namespace cn
 
-using namespace cn;
-
-extern "C" {
-
-cn_status cn_adjoint_solve(cn_system S, cn_ic0 F, const cn_pcg_opts *o, int32_t k,
-                           const double *d_g, const double *d_x, const double *d_xc,
-                           double *d_lambda, double *d_gc, double *d_gw,
-                           cn_solve_info *info, void *stream) {
-  CN_API_BEGIN
-  if (!S || !o || k <= 0 || !d_g || !d_lambda) {
-    set_error("cn_adjoint_solve: bad arguments");
-    return CN_BAD_ARG;
-  }
-  if (d_gw && (!d_x || (S->n_con > 0 && !d_xc))) {
-    set_error("cn_adjoint_solve: the weight gradient needs x (and x_c)");
-    return CN_BAD_ARG;
-  }
-  hipStream_t s = as_stream(stream);
-  cn_pcg_opts oo = *o;
-  oo.warm_start = 0;
-  const cn_status st = cn_pcg_solve(S, F, &oo, k, d_g, d_lambda, info, stream);
-  if (st != CN_OK && st != CN_MAXIT) return st;
+namespace cn {
+// The adjoint after its solve: d_lambda holds A_ff^-1 g (status st).
+static cn_status adjoint_gradients(cn_system_t *S, int32_t k, const double *d_x,
+                                   const double *d_xc, const double *d_lambda, double *d_gc,
+                                   double *d_gw, hipStream_t s) {
   if (d_gc && S->n_con > 0) {
     ensure_cf(S, s);
     const int64_t tot = (int64_t)S->n_con * k;
@@ -166,6 +149,56 @@ cn_status cn_adjoint_solve(cn_system S, cn_ic0 F, const cn_pcg_opts *o, int32_t 
   }
   CN_HIP(hipGetLastError());
   CN_HIP(hipStreamSynchronize(s));
+  return CN_OK;
+}
+
+static bool adjoint_args(cn_system S, const cn_pcg_opts *o, int32_t k, const double *d_g,
+                         const double *d_x, const double *d_xc, const double *d_lambda,
+                         const double *d_gw, const char *who) {
+  if (!S || !o || k <= 0 || !d_g || !d_lambda) {
+    set_error("%s: bad arguments", who);
+    return false;
+  }
+  if (d_gw && (!d_x || (S->n_con > 0 && !d_xc))) {
+    set_error("%s: the weight gradient needs x (and x_c)", who);
+    return false;
+  }
+  return true;
+}
+}  // namespace cn
+
+using namespace cn;
+
+extern "C" {
+
+cn_status cn_adjoint_solve(cn_system S, cn_ic0 F, const cn_pcg_opts *o, int32_t k,
+                           const double *d_g, const double *d_x, const double *d_xc,
+                           double *d_lambda, double *d_gc, double *d_gw,
+                           cn_solve_info *info, void *stream) {
+  CN_API_BEGIN
+  if (!adjoint_args(S, o, k, d_g, d_x, d_xc, d_lambda, d_gw, "cn_adjoint_solve"))
+    return CN_BAD_ARG;
+  cn_pcg_opts oo = *o;
+  oo.warm_start = 0;
+  const cn_status st = cn_pcg_solve(S, F, &oo, k, d_g, d_lambda, info, stream);
+  if (st != CN_OK && st != CN_MAXIT) return st;
+  adjoint_gradients(S, k, d_x, d_xc, d_lambda, d_gc, d_gw, as_stream(stream));
+  return st;
+  CN_API_END
+}
+
+cn_status cn_adjoint_solve_hsc(cn_system S, cn_hsc H, const cn_pcg_opts *o, int32_t k,
+                               const double *d_g, const double *d_x, const double *d_xc,
+                               double *d_lambda, double *d_gc, double *d_gw,
+                               cn_solve_info *info, void *stream) {
+  CN_API_BEGIN
+  if (!adjoint_args(S, o, k, d_g, d_x, d_xc, d_lambda, d_gw, "cn_adjoint_solve_hsc"))
+    return CN_BAD_ARG;
+  cn_pcg_opts oo = *o;
+  oo.warm_start = 0;
+  const cn_status st = cn_pcg_solve_hsc(S, H, &oo, k, d_g, d_lambda, info, stream);
+  if (st != CN_OK && st != CN_MAXIT) return st;
+  adjoint_gradients(S, k, d_x, d_xc, d_lambda, d_gc, d_gw, as_stream(stream));
   return st;
   CN_API_END
 }

@@ -246,8 +246,9 @@ class _System:
     def adjoint(self, g, x=None, xc=None, precond: str = "ic0", precision: str = "mixed",
                 tol: float = 1e-9, ic0: "IC0 | None" = None, weights: bool = False,
                 max_iter: int = 20000, inner_tol: float = 1e-5, check_every: int = 8,
-                max_refine: int = 30):
-        """Adjoint of the harmonic solve (cn_adjoint_solve) for g = dL/dx_f
+                max_refine: int = 30, hsc: "HSC | None" = None):
+        """Adjoint of the harmonic solve (cn_adjoint_solve, or
+        cn_adjoint_solve_hsc with precond "hsc") for g = dL/dx_f
         (n_free x k).  Returns (lambda, g_c, g_w, SolveInfo) as device
         tensors: lambda = A_ff^-1 g, g_c = -A_fc^T lambda (None without
         constrained columns), g_w = dL/dw per stored entry of A_ff then A_fc
@@ -268,14 +269,21 @@ class _System:
             if i.n_con > 0:
                 xct = _as_tensor(xc, torch.float64).reshape(i.n_con, k).contiguous()
             gw = torch.empty(i.nnz_ff + i.nnz_fc, dtype=torch.float64, device="cuda")
-        o = self._opts(precond, precision, tol, max_iter, inner_tol, check_every, max_refine,
-                       False)
-        if precond == "ic0" and ic0 is None:
-            ic0 = IC0(self)
+        o = self._opts("jacobi" if precond == "hsc" else precond, precision, tol, max_iter,
+                       inner_tol, check_every, max_refine, False)
         info = L.SolveInfo()
-        st = lib.cn_adjoint_solve(self._h, ic0.handle if ic0 is not None else C.c_void_p(0),
-                                  C.byref(o), k, _p(gt), _p(xt), _p(xct), _p(lam), _p(gc),
-                                  _p(gw), C.byref(info), _stream())
+        if precond == "hsc":
+            if hsc is None:
+                hsc = HSC(self)
+            st = lib.cn_adjoint_solve_hsc(self._h, hsc.handle, C.byref(o), k, _p(gt), _p(xt),
+                                          _p(xct), _p(lam), _p(gc), _p(gw), C.byref(info),
+                                          _stream())
+        else:
+            if precond == "ic0" and ic0 is None:
+                ic0 = IC0(self)
+            st = lib.cn_adjoint_solve(self._h, ic0.handle if ic0 is not None else C.c_void_p(0),
+                                      C.byref(o), k, _p(gt), _p(xt), _p(xct), _p(lam), _p(gc),
+                                      _p(gw), C.byref(info), _stream())
         L.check(st, "adjoint_solve", ok=(L.CN_OK, L.CN_MAXIT, L.CN_BREAKDOWN))
         if vec:
             lam = lam.reshape(-1)

@@ -224,7 +224,15 @@ def tilted_loops(mesh: Mesh, n_loops: int = 8, max_tilt_deg: float = 30.0,
         path: list[int] = []
         for k in range(len(way)):
             a, b = way[k], way[(k + 1) % len(way)]
-            _, pred = dijkstra(g, indices=a, return_predecessors=True)
+            # Dijkstra settles vertices in order of distance, so a search cut
+            # off past a few chord lengths (a mesh path between neighbouring
+            # waypoints is barely longer than the chord) gives the same
+            # predecessors along the path to b; the full search is the
+            # fallback when b was not reached
+            lim = 4.0 * float(np.linalg.norm(pos[a] - pos[b])) + 1e-12
+            _, pred = dijkstra(g, indices=a, return_predecessors=True, limit=lim)
+            if pred[b] < 0 and a != b:
+                _, pred = dijkstra(g, indices=a, return_predecessors=True)
             seg = [b]
             while seg[-1] != a:
                 seg.append(int(pred[seg[-1]]))
