@@ -97,10 +97,12 @@ def main():
                     help="also run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--plain-poses", type=int, default=2,
                     help="poses solved by the plain FP64 one-solve-per-launch path")
-    ap.add_argument("--workload", default="capsules100k", choices=["capsules100k", "meshes64"],
+    ap.add_argument("--workload", default="capsules100k",
+                    choices=["capsules100k", "meshes64", "icosphere9"],
                     help="capsules100k: 256 poses x 3 RHS on the 100k capsule union (the "
                          "headline); meshes64: 64 uneven meshes x 3 RHS, batched vs one solve "
-                         "per mesh")
+                         "per mesh; icosphere9: 16 poses x 3 RHS forward + adjoint on the "
+                         "2.6M-vertex icosphere with 64 loops")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="no GPU: turn two rocprofv3 --pmc counter files (FETCH_SIZE, "
                          "WRITE_SIZE) of a --copy-mb run into the traffic file")
@@ -125,6 +127,9 @@ def main():
 
     if args.workload == "meshes64":
         meshes64_bench(args)
+        return
+    if args.workload == "icosphere9":
+        icosphere9_bench(args)
         return
 
     if args.copy_mb > 0:
@@ -483,6 +488,100 @@ def meshes64_bench(args):
         "solve": {"inner_iters_per_step": sum(itb) / len(itb),
                   "relres_max_batched": worst[0], "relres_max_per_mesh": worst[1],
                   "setup_s": time.time() - t0 - eb - ee},
+    }
+    print(json.dumps(line))
+
+
+def icosphere9_bench(args):
+    """The large-mesh workload: level-9 icosphere (2,621,442 vertices, 64
+    tilted loops), 16 poses x 3 RHS; one step = harmonic RHS, forward solve
+    to 1e-9 and the adjoint of a quadratic loss (lambda and dL/dx_c), both
+    HSC-preconditioned, FP32 inner loop with FP64 refinement."""
+    import numpy as np
+    import scipy.sparse as sps
+    import torch
+
+    import tool_godot_curvenet_amd as cn
+    from tool_godot_curvenet_amd import _lib, meshgen
+    import ctypes as C
+
+    assert torch.cuda.is_available(), "no GPU"
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    mesh = meshgen.config_mesh("icosphere9")
+    print(f"icosphere9: {mesh.nv} vertices, {len(mesh.meta['loops'])} loops in "
+          f"{time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    lap = cn.CutLaplacian.from_mesh(mesh)
+    hsc = cn.HSC(lap, theta=args.theta)
+    print(f"icosphere9: system and hierarchy at {time.time() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    P = 16
+    k = 3 * P
+    n, ncon = lap.n_free, lap.n_con
+    rng = np.random.default_rng(meshgen.BASE_SEED + 50)
+    Xc = torch.as_tensor(rng.uniform(-0.2, 0.2, size=(ncon, k)), device=dev)
+    target = torch.as_tensor(rng.normal(0, 0.05, size=(n, k)), device=dev)
+    B = torch.empty((n, k), dtype=torch.float64, device=dev)
+    X = torch.zeros_like(B)
+    G = torch.empty_like(B)
+    LAM = torch.empty_like(B)
+    GC = torch.empty((ncon, k), dtype=torch.float64, device=dev)
+    sp = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    opts = _lib.PcgOpts()
+    lib.cn_pcg_default_opts(opts)
+    opts.precision = _lib.CN_PREC_MIXED if args.precision == "mixed" else _lib.CN_PREC_FP64
+    opts.tol_rel = args.tol
+    fi, ai = _lib.SolveInfo(), _lib.SolveInfo()
+    tf = [0.0]
+
+    def step():
+        _lib.check(lib.cn_harmonic_rhs(lap.handle, k, C.c_void_p(Xc.data_ptr()),
+                                       C.c_void_p(B.data_ptr()), sp), "harmonic_rhs")
+        _lib.check(lib.cn_pcg_solve_hsc(lap.handle, hsc.handle, C.byref(opts), k,
+                                        C.c_void_p(B.data_ptr()), C.c_void_p(X.data_ptr()),
+                                        C.byref(fi), sp), "pcg_solve_hsc")
+        torch.sub(X, target, out=G)              # dL/dx for L = |x - target|^2 / 2
+        _lib.check(lib.cn_adjoint_solve_hsc(lap.handle, hsc.handle, C.byref(opts), k,
+                                            C.c_void_p(G.data_ptr()), None, None,
+                                            C.c_void_p(LAM.data_ptr()),
+                                            C.c_void_p(GC.data_ptr()), None, C.byref(ai), sp),
+                   "adjoint_solve_hsc")
+        return fi.iters, ai.iters
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    its = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    # FP64 residuals of the last step's forward and adjoint solves (not timed)
+    A = lap.A_ff()
+    M = sps.csr_matrix((A.values, A.col_idx, A.row_ptr), shape=(n, n))
+    Xh, Bh, Gh, Lh = X.cpu().numpy(), B.cpu().numpy(), G.cpu().numpy(), LAM.cpu().numpy()
+    rf = float((np.linalg.norm(Bh - M @ Xh, axis=0) / np.linalg.norm(Bh, axis=0)).max())
+    ra = float((np.linalg.norm(Gh - M @ Lh, axis=0) / np.linalg.norm(Gh, axis=0)).max())
+    line = {
+        "metric": "poses/s, forward + adjoint at 1e-9 (2.6M-vert icosphere, 64 loops, 3 RHS)",
+        "value": P * args.steps / el,
+        "unit": "poses/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 refinement" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded icosphere, seeded constraint displacements and targets)",
+        "config": {"workload": f"icosphere level 9 (nu 512), {mesh.nv} verts, 64 tilted loops, "
+                               f"n_free={n}, n_con={ncon}, nnz={lap.info.nnz_ff}, {P} poses "
+                               f"x 3 RHS, HSC theta {args.theta} ({hsc.info.nlevels} levels, "
+                               f"coarse {hsc.info.n_coarse}), tol {args.tol}"},
+        "solve": {"forward_iters": [f for f, _ in its], "adjoint_iters": [a for _, a in its],
+                  "relres_forward": rf, "relres_adjoint": ra,
+                  "setup_s": time.time() - t0 - el},
     }
     print(json.dumps(line))
 

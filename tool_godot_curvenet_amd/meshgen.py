@@ -188,16 +188,29 @@ def loop_edges(loop: list[int]) -> np.ndarray:
     return np.array([(loop[a], loop[(a + 1) % m]) for a in range(m)], dtype=np.int32)
 
 
+def _nearest_vertex(tree, pos, p) -> int:
+    """Vertex nearest to p by squared distance ((pos - p)**2).sum(1), the
+    lowest index among ties (SurfaceProjection's strict < scan): the tree
+    proposes every vertex within a hair of its nearest distance, and the
+    exact rule picks among those."""
+    d0, _ = tree.query(p)
+    cand = np.array(sorted(tree.query_ball_point(p, d0 * (1 + 1e-9) + 1e-15)), dtype=np.int64)
+    d = ((pos[cand] - p[None]) ** 2).sum(1)
+    return int(cand[int(np.argmin(d))])
+
+
 def tilted_loops(mesh: Mesh, n_loops: int = 8, max_tilt_deg: float = 30.0,
                  n_way: int = 32, seed: int = BASE_SEED + 1) -> Mesh:
     """Latitude loops of random tilt, snapped to the mesh: waypoints on the
     tilted circle go to their nearest vertex (strict <, lowest index), and
     consecutive waypoints are joined by shortest edge paths."""
     from scipy.sparse.csgraph import dijkstra
+    from scipy.spatial import cKDTree
     rng = np.random.default_rng(seed)
     pos = mesh.pos
     rad = float(np.linalg.norm(pos, axis=1).mean())
     g = _edge_graph(pos, mesh.tris)
+    tree = cKDTree(pos)
     lats = np.sort(rng.uniform(-0.8, 0.8, size=n_loops))
     edges = []
     loops = []
@@ -214,10 +227,7 @@ def tilted_loops(mesh: Mesh, n_loops: int = 8, max_tilt_deg: float = 30.0,
         rr = math.sqrt(max(rad * rad - h * h, 1e-12))
         ang = np.linspace(0, 2 * math.pi, n_way, endpoint=False)
         pts = h * nrm[None] + rr * (np.cos(ang)[:, None] * u[None] + np.sin(ang)[:, None] * v[None])
-        way = []
-        for p in pts:
-            d = ((pos - p[None]) ** 2).sum(1)
-            way.append(int(np.argmin(d)))   # argmin = lowest index among ties
+        way = [_nearest_vertex(tree, pos, p) for p in pts]
         way = [w for k, w in enumerate(way) if k == 0 or w != way[k - 1]]
         if way[0] == way[-1]:
             way = way[:-1]
@@ -229,13 +239,26 @@ def tilted_loops(mesh: Mesh, n_loops: int = 8, max_tilt_deg: float = 30.0,
             # waypoints is barely longer than the chord) gives the same
             # predecessors along the path to b; the full search is the
             # fallback when b was not reached
-            lim = 4.0 * float(np.linalg.norm(pos[a] - pos[b])) + 1e-12
-            _, pred = dijkstra(g, indices=a, return_predecessors=True, limit=lim)
-            if pred[b] < 0 and a != b:
-                _, pred = dijkstra(g, indices=a, return_predecessors=True)
+            # graph distance is never shorter than Euclidean distance, so the
+            # cut-off search stays inside the ball of radius lim around a:
+            # it runs on that ball's subgraph (vertices kept in ascending
+            # order).  An edge path between neighbouring waypoints is ~1.1x
+            # the chord on these meshes; a longer one takes the fallback.
+            lim = 1.5 * float(np.linalg.norm(pos[a] - pos[b])) + 1e-12
+            ball = np.array(sorted(tree.query_ball_point(pos[a], lim * (1 + 1e-9))), dtype=np.int64)
+            sub = g[ball][:, ball]
+            la, lb = int(np.searchsorted(ball, a)), int(np.searchsorted(ball, b))
             seg = [b]
-            while seg[-1] != a:
-                seg.append(int(pred[seg[-1]]))
+            if lb < len(ball) and ball[lb] == b:
+                _, pred = dijkstra(sub, indices=la, return_predecessors=True, limit=lim)
+                if pred[lb] >= 0 or a == b:
+                    while seg[-1] != a:
+                        seg.append(int(ball[pred[int(np.searchsorted(ball, seg[-1]))]]))
+            if seg[-1] != a:
+                _, pred = dijkstra(g, indices=a, return_predecessors=True)
+                seg = [b]
+                while seg[-1] != a:
+                    seg.append(int(pred[seg[-1]]))
             seg.reverse()
             path.extend(seg[:-1])
         loop = _simplify_loop(path)
@@ -406,6 +429,10 @@ def config_mesh(name: str) -> Mesh:
         nu = 45 if name == "sphere45" else 224
         m = geodesic_sphere(nu, jitter_frac=0.05, seed=BASE_SEED + 4)
         return tilted_loops(m, 8, 30.0, seed=BASE_SEED + 5)
+    if name == "icosphere9":
+        # level-9 icosphere (nu = 512, V = 2,621,442) with 64 tilted loops
+        m = geodesic_sphere(512, jitter_frac=0.05, seed=BASE_SEED + 8)
+        return tilted_loops(m, 64, 30.0, n_way=128, seed=BASE_SEED + 9)
     raise KeyError(name)
 
 
