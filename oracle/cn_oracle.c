@@ -9,6 +9,7 @@
 #include "cn_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -904,6 +905,63 @@ int32_t orc_pcg(int32_t n, const int32_t *ptr, const int32_t *col,
                 const double *x0, int32_t max_iter, double tol_abs, double *x) {
   return pcg_core(n, ptr, col, val, precond, l_ptr, l_col, l_val, b, x0,
                   max_iter, tol_abs, x);
+}
+
+/* Many independent right-hand sides (columns of a row-major n x k block),
+ * each solved by orc_pcg with tolerance tol_rel * ||b_c||, spread over
+ * `nthreads` POSIX threads (column c on thread c mod nthreads).  Every
+ * column's arithmetic is exactly the single-column solve's.  Returns the
+ * total iteration count. */
+typedef struct {
+  int32_t n, k, c0, step, precond, max_iter;
+  const int32_t *ptr, *col, *l_ptr, *l_col;
+  const double *val, *l_val, *B;
+  double tol_rel, *X;
+  int64_t iters;
+} pcg_job;
+
+static void *pcg_worker(void *arg) {
+  pcg_job *J = (pcg_job *)arg;
+  double *b = (double *)malloc(sizeof(double) * (size_t)J->n + 8);
+  double *x = (double *)malloc(sizeof(double) * (size_t)J->n + 8);
+  for (int32_t c = J->c0; c < J->k; c += J->step) {
+    double bb = 0.0;
+    for (int32_t i = 0; i < J->n; ++i) {
+      b[i] = J->B[(size_t)i * J->k + c];
+      bb = bb + b[i] * b[i];
+    }
+    J->iters += pcg_core(J->n, J->ptr, J->col, J->val, J->precond, J->l_ptr, J->l_col,
+                         J->l_val, b, NULL, J->max_iter, J->tol_rel * sqrt(bb), x);
+    for (int32_t i = 0; i < J->n; ++i) J->X[(size_t)i * J->k + c] = x[i];
+  }
+  free(b);
+  free(x);
+  return NULL;
+}
+
+int64_t orc_pcg_many(int32_t n, const int32_t *ptr, const int32_t *col, const double *val,
+                     int32_t precond, const int32_t *l_ptr, const int32_t *l_col,
+                     const double *l_val, int32_t k, const double *B, int32_t max_iter,
+                     double tol_rel, double *X, int32_t nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > k) nthreads = k;
+  pcg_job *jobs = (pcg_job *)calloc((size_t)nthreads, sizeof(pcg_job));
+  pthread_t *th = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+  for (int32_t t = 0; t < nthreads; ++t) {
+    pcg_job J = {n, k, t, nthreads, precond, max_iter, ptr, col, l_ptr, l_col,
+                 val, l_val, B, tol_rel, X, 0};
+    jobs[t] = J;
+  }
+  for (int32_t t = 1; t < nthreads; ++t) pthread_create(&th[t], NULL, pcg_worker, &jobs[t]);
+  pcg_worker(&jobs[0]);
+  int64_t it = jobs[0].iters;
+  for (int32_t t = 1; t < nthreads; ++t) {
+    pthread_join(th[t], NULL);
+    it += jobs[t].iters;
+  }
+  free(jobs);
+  free(th);
+  return it;
 }
 
 /* ======================================================================

@@ -175,6 +175,10 @@ int32_t orc_pcg(int32_t n, const int32_t *ptr, const int32_t *col,
                 const double *val, int32_t precond, const int32_t *l_ptr,
                 const int32_t *l_col, const double *l_val, const double *b,
                 const double *x0, int32_t max_iter, double tol_abs, double *x);
+int64_t orc_pcg_many(int32_t n, const int32_t *ptr, const int32_t *col, const double *val,
+                     int32_t precond, const int32_t *l_ptr, const int32_t *l_col,
+                     const double *l_val, int32_t k, const double *B, int32_t max_iter,
+                     double tol_rel, double *X, int32_t nthreads);
 
 /* Symmetric permutation B = P A P^T with perm[new] = old; columns sorted. */
 void orc_permute_csr(int32_t n, const int32_t *ptr, const int32_t *col,

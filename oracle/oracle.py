@@ -36,6 +36,7 @@ PINS = {
                   "full-pattern IC(0) == dense Cholesky",
     "ic0_solve": "LL^T x == b",
     "pcg": "dense GE / numpy solve",
+    "pcg_many": "column for column equal to pcg (the same arithmetic on each thread)",
     "greedy_color": "GraphColoring.lean:79-117 KATs, validity",
     "sgs_sweep/sgs_apply": "equal to lexical Gauss-Seidel in colour order (the kernel's stated "
                            "equivalence); one sweep from 0 == the textbook SGS preconditioner "
@@ -74,6 +75,7 @@ def lib():
         _lib.orc_dot.restype = f64
         _lib.orc_hsc_degree_weight.restype = f64
         _lib.orc_hsc_edge_weight.restype = f64
+        _lib.orc_pcg_many.restype = i64
         for name in ("orc_he_from_triangles_spec", "orc_he_from_triangles", "orc_he_manifold",
                      "orc_subdivide_edge", "orc_split_face", "orc_insert_crack", "orc_cut_edges",
                      "orc_cut_laplacian_csr", "orc_cg", "orc_ic0_factor", "orc_pcg",
@@ -312,6 +314,25 @@ def pcg(ptr, col, val, b, tol_abs, max_iter=100000, factor=None, x0=None):
                        _p(_d(b)), _p(_d(x0)) if x0 is not None else None, max_iter, C.c_double(tol_abs),
                        _p(x))
     return x, it
+
+
+def pcg_many(ptr, col, val, B, tol_rel, factor=None, threads=1, max_iter=100000):
+    """Every column of B (n x k) solved by `pcg` to tol_rel * ||b_c||, the
+    columns spread over `threads` threads.  Returns (X, total iterations)."""
+    n = len(ptr) - 1
+    B = np.ascontiguousarray(B, dtype=np.float64).reshape(n, -1)
+    X = np.zeros_like(B)
+    if factor is None:
+        lp = lc = lv = None
+        pc = 0
+    else:
+        lp, lc, lv = (_i(factor[0]), _i(factor[1]), _d(factor[2]))
+        pc = 1
+    it = lib().orc_pcg_many(n, _p(_i(ptr)), _p(_i(col)), _p(_d(val)), pc,
+                            _p(lp) if pc else None, _p(lc) if pc else None,
+                            _p(lv) if pc else None, B.shape[1], _p(B), max_iter,
+                            C.c_double(tol_rel), _p(X), threads)
+    return X, int(it)
 
 
 def permute_csr(ptr, col, val, perm):

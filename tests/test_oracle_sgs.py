@@ -106,3 +106,21 @@ def test_restrict_sum_kat_and_closed_form():
     want32 = np.zeros((40, 5), np.float32)
     np.add.at(want32, parent[ok], fine[ok].astype(np.float32))
     assert np.array_equal(f32, want32)
+
+
+def test_pcg_many_equals_single_column_solves(cyl):
+    """The threaded CPU baseline solves every column with exactly the
+    single-column PCG's arithmetic."""
+    ptr, col, val, A = cyl
+    B = np.random.default_rng(6).normal(size=(len(A), 5))
+    fac = O.ic0_factor(ptr, col, val)
+    X, it = O.pcg_many(ptr, col, val, B, 1e-10, factor=fac, threads=3)
+    total = 0
+    for c in range(5):
+        bb = 0.0
+        for v in B[:, c]:
+            bb = bb + v * v
+        x, i = O.pcg(ptr, col, val, B[:, c], 1e-10 * bb ** 0.5, factor=fac)
+        assert np.array_equal(X[:, c], x)
+        total += i
+    assert it == total
