@@ -131,6 +131,25 @@ typedef struct cn_system_info {
 cn_status cn_system_create(int32_t nv, int32_t nf, const int32_t *d_tris,
                            const uint8_t *d_constrained, const double *d_pos,
                            cn_system *out, void *stream);
+
+/* Triangle Laplacian of the assembly.  CN_LAP_COT is the embedded cot of
+ * PolygonLaplacian.triangleCotLaplacian (PolygonLaplacian.lean:56-71), what
+ * CutMeshLaplacian assembles (CutMeshLaplacian.lean:64); CN_LAP_ROBUST is
+ * RobustLaplacian.robustTriangleCotLaplacian (RobustLaplacian.lean:136-140):
+ * edge lengths mollified per triangle by the smallest eps >= 0 giving every
+ * corner a + b > c + delta, cot by the law of cosines over Heron's area --
+ * finite on near-degenerate triangles. */
+enum { CN_LAP_COT = 0, CN_LAP_ROBUST = 1 };
+typedef struct cn_system_opts {
+  int32_t laplacian;    /* CN_LAP_*                        */
+  double delta;         /* CN_LAP_ROBUST margin (>= 0)     */
+} cn_system_opts;
+void cn_system_default_opts(cn_system_opts *opts);
+/* cn_system_create with the triangle Laplacian chosen by opts; the choice
+ * holds for cn_system_set_positions too. */
+cn_status cn_system_create_ex(int32_t nv, int32_t nf, const int32_t *d_tris,
+                              const uint8_t *d_constrained, const double *d_pos,
+                              const cn_system_opts *opts, cn_system *out, void *stream);
 /* A system holding a caller-assembled SPD matrix as its A_ff (no mesh, no
  * A_fc): the `SparseMatrixCSR` argument of SparseLinAlg.cg
  * (SparseLinAlg.lean:35-41, 104).  Copies its inputs. */

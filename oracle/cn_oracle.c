@@ -417,13 +417,71 @@ void orc_triangle_cot_laplacian(const double *a, const double *b,
   m[8] = w_ac + w_bc;
 }
 
+/* RobustLaplacian (RobustLaplacian.lean:58-140): Sharp & Crane intrinsic
+ * mollification.  Edge lengths from the embedding (:63-70), the smallest
+ * eps >= 0 that makes every corner satisfy a + b > c + delta (:83-89, per
+ * triangle, as the code does), cot from the mollified lengths by the law of
+ * cosines over Heron's area (:106-109), the 3x3 block as
+ * triangleCotLaplacian (:114-131). */
+static double max_f(double a, double b) { return a <= b ? b : a; }
+
+static double edge_len(const double *u, const double *v) {
+  double dx = u[0] - v[0], dy = u[1] - v[1], dz = u[2] - v[2];
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+double orc_mollify_triangle(double delta, double eA, double eB, double eC) {
+  double ra = delta + eA - eB - eC;
+  double rb = delta + eB - eA - eC;
+  double rc = delta + eC - eA - eB;
+  return max_f(0.0, max_f(ra, max_f(rb, rc)));
+}
+
+double orc_cot_from_lengths(double eo, double e1, double e2) {
+  double s = (eo + e1 + e2) * 0.5;
+  double area = sqrt(s * (s - eo) * (s - e1) * (s - e2));
+  return (e1 * e1 + e2 * e2 - eo * eo) / (4.0 * area);
+}
+
+void orc_robust_triangle_cot_laplacian(double delta, const double *a, const double *b,
+                                       const double *c, double *m) {
+  double eA = edge_len(b, c), eB = edge_len(a, c), eC = edge_len(a, b);
+  double eps = orc_mollify_triangle(delta, eA, eB, eC);
+  eA = eA + eps;
+  eB = eB + eps;
+  eC = eC + eps;
+  double cotA = orc_cot_from_lengths(eA, eB, eC);
+  double cotB = orc_cot_from_lengths(eB, eA, eC);
+  double cotC = orc_cot_from_lengths(eC, eA, eB);
+  double w_bc = cotA * 0.5, w_ac = cotB * 0.5, w_ab = cotC * 0.5;
+  m[1] = -w_ab; m[3] = -w_ab;
+  m[2] = -w_ac; m[6] = -w_ac;
+  m[5] = -w_bc; m[7] = -w_bc;
+  m[0] = w_ab + w_ac;
+  m[4] = w_ab + w_bc;
+  m[8] = w_ac + w_bc;
+}
+
+/* Which triangle Laplacian the assembly uses: 0 = embedded cot
+ * (triangleCotLaplacian), 1 = robust with margin delta.  A setting of the
+ * checker, so that the assembly functions keep their spec signatures. */
+static int g_lap_kind = 0;
+static double g_lap_delta = 0.0;
+void orc_set_laplacian(int32_t kind, double delta) {
+  g_lap_kind = kind;
+  g_lap_delta = delta;
+}
+
 /* polygonCotLaplacian (PolygonLaplacian.lean:77-93): fan from vertex 0. */
 void orc_polygon_cot_laplacian(int32_t n, const double *poly, double *m) {
   for (int32_t i = 0; i < n * n; ++i) m[i] = 0.0;
   if (n < 3) return;
   double tri[9];
   for (int32_t i = 1; i < n - 1; ++i) {
-    orc_triangle_cot_laplacian(poly, poly + 3 * i, poly + 3 * (i + 1), tri);
+    if (g_lap_kind == 1)
+      orc_robust_triangle_cot_laplacian(g_lap_delta, poly, poly + 3 * i, poly + 3 * (i + 1), tri);
+    else
+      orc_triangle_cot_laplacian(poly, poly + 3 * i, poly + 3 * (i + 1), tri);
     int32_t idx[3] = {0, i, i + 1};
     for (int32_t li = 0; li < 3; ++li)
       for (int32_t lj = 0; lj < 3; ++lj)

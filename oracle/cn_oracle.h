@@ -85,6 +85,11 @@ double orc_cot_at_vertex(const double *apex, const double *o1,
 /* triangleCotLaplacian (:56-71): m[9] row-major. */
 void orc_triangle_cot_laplacian(const double *a, const double *b,
                                 const double *c, double *m);
+double orc_mollify_triangle(double delta, double eA, double eB, double eC);
+double orc_cot_from_lengths(double eo, double e1, double e2);
+void orc_robust_triangle_cot_laplacian(double delta, const double *a, const double *b,
+                                       const double *c, double *m);
+void orc_set_laplacian(int32_t kind, double delta);
 /* polygonCotLaplacian (:77-93): poly[3*n], m[n*n]. */
 void orc_polygon_cot_laplacian(int32_t n, const double *poly, double *m);
 

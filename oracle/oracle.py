@@ -27,6 +27,11 @@ PINS = {
     "cut_edges": "cylinder seam closed form (64 copies at nv+j), Euler/partition invariants, "
                  "brute-force wedge recount",
     "cot/triangle/polygon Laplacian": "closed forms (equilateral -1/(2 sqrt 3)), row sums, symmetry",
+    "robust triangle Laplacian": "RobustLaplacian.lean:160-290 KATs (cot 60/30/90 degrees, "
+                                 "mollification margins, equilateral == embedded, finite on a "
+                                 "collinear triangle); equal to the embedded cot on random "
+                                 "non-degenerate triangles with delta = 0 (law of cosines + "
+                                 "Heron vs cross product)",
     "assemble_vtlhv_dense": "CutMeshLaplacian.lean:101-179 KATs, linear precision",
     "cut_laplacian_csr": "bit-identical to the dense spec transcription on small cut meshes",
     "harmonic_rhs_csr": "bit-identical to the dense spec transcription; HarmonicSolve KATs",
@@ -76,6 +81,12 @@ def lib():
         _lib.orc_hsc_degree_weight.restype = f64
         _lib.orc_hsc_edge_weight.restype = f64
         _lib.orc_pcg_many.restype = i64
+        _lib.orc_mollify_triangle.restype = f64
+        _lib.orc_mollify_triangle.argtypes = [f64, f64, f64, f64]
+        _lib.orc_cot_from_lengths.restype = f64
+        _lib.orc_cot_from_lengths.argtypes = [f64, f64, f64]
+        _lib.orc_set_laplacian.argtypes = [i32, f64]
+        _lib.orc_robust_triangle_cot_laplacian.argtypes = [f64, P, P, P, P]
         for name in ("orc_he_from_triangles_spec", "orc_he_from_triangles", "orc_he_manifold",
                      "orc_subdivide_edge", "orc_split_face", "orc_insert_crack", "orc_cut_edges",
                      "orc_cut_laplacian_csr", "orc_cg", "orc_ic0_factor", "orc_pcg",
@@ -204,6 +215,39 @@ def harmonic_rhs_dense(he, col_of_vertex, pos, Fc):
                                  _p(_i(he["face"])), _p(_i(col_of_vertex)), _p(_d(pos)), k,
                                  _p(Fc), _p(out))
     return out.reshape(nv, k)
+
+
+def mollify_triangle(delta, eA, eB, eC):
+    """RobustLaplacian.mollifyTriangle (RobustLaplacian.lean:83-89)."""
+    return lib().orc_mollify_triangle(delta, eA, eB, eC)
+
+
+def cot_from_lengths(eo, e1, e2):
+    """RobustLaplacian.cotFromLengths (RobustLaplacian.lean:106-109)."""
+    return lib().orc_cot_from_lengths(eo, e1, e2)
+
+
+def robust_triangle_cot_laplacian(delta, a, b, c):
+    """RobustLaplacian.robustTriangleCotLaplacian (RobustLaplacian.lean:136-140)."""
+    m = np.zeros(9)
+    lib().orc_robust_triangle_cot_laplacian(delta, _p(_d(a)), _p(_d(b)), _p(_d(c)), _p(m))
+    return m.reshape(3, 3)
+
+
+class laplacian:
+    """Context: the oracle assembles with the robust triangle Laplacian
+    (margin delta) inside the block, the embedded cot outside."""
+
+    def __init__(self, kind="robust", delta=0.0):
+        self.kind, self.delta = kind, delta
+
+    def __enter__(self):
+        lib().orc_set_laplacian(1 if self.kind == "robust" else 0, self.delta)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_laplacian(0, 0.0)
+        return False
 
 
 def cut_laplacian_csr(he, constrained, pos):

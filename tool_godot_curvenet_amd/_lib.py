@@ -40,6 +40,13 @@ class SystemInfo(C.Structure):
                 ("d_he_next", vp), ("d_he_face", vp)]
 
 
+CN_LAP_COT, CN_LAP_ROBUST = 0, 1
+
+
+class SystemOpts(C.Structure):
+    _fields_ = [("laplacian", i32), ("delta", f64)]
+
+
 class Ic0Opts(C.Structure):
     _fields_ = [("ordering", i32), ("block_size", i32), ("allow_shift", i32)]
 
@@ -94,6 +101,9 @@ _SIGS = {
     "cn_cut_seams": (i32, [i32, i32, vp, i32, vp, vp, vp, vp, P_i32, vp]),
     "cn_project_to_vertices": (i32, [i32, vp, i32, vp, vp, vp]),
     "cn_system_create": (i32, [i32, i32, vp, vp, vp, C.POINTER(vp), vp]),
+    "cn_system_default_opts": (None, [C.POINTER(SystemOpts)]),
+    "cn_system_create_ex": (i32, [i32, i32, vp, vp, vp, C.POINTER(SystemOpts), C.POINTER(vp),
+                                  vp]),
     "cn_system_from_csr": (i32, [i32, vp, vp, vp, C.POINTER(vp), vp]),
     "cn_system_get_info": (i32, [vp, C.POINTER(SystemInfo)]),
     "cn_system_set_positions": (i32, [vp, vp, vp]),

@@ -58,6 +58,50 @@ __global__ void k_face_cot(int32_t nf, const int32_t *tris, const double *pos,
   o[6] = -w_ac;       o[7] = -w_bc;       o[8] = w_ac + w_bc;
 }
 
+// RobustLaplacian.robustTriangleCotLaplacian (RobustLaplacian.lean:136-140)
+// on the same corners as k_face_cot: edge lengths (:63-70), the per-triangle
+// mollification eps = max(0, max over corners of delta + opposite - the two
+// adjacent) (:83-89), cot by the law of cosines over Heron's area
+// (:106-109), the 3x3 block of triangleCotLaplacian (:114-131).
+__device__ __forceinline__ double max_f(double a, double b) { return a <= b ? b : a; }
+__device__ __forceinline__ double edge_len(const double *u, const double *v) {
+  double dx = u[0] - v[0], dy = u[1] - v[1], dz = u[2] - v[2];
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+__device__ __forceinline__ double cot_from_lengths(double eo, double e1, double e2) {
+  double s = (eo + e1 + e2) * 0.5;
+  double area = sqrt(s * (s - eo) * (s - e1) * (s - e2));
+  return (e1 * e1 + e2 * e2 - eo * eo) / (4.0 * area);
+}
+
+__global__ void k_face_robust(int32_t nf, const int32_t *tris, const double *pos,
+                              double delta, double *m) {
+  int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  double a[3], b[3], c[3];
+  int32_t ia = tris[3 * f + 1], ib = tris[3 * f + 2], ic = tris[3 * f];
+  for (int d = 0; d < 3; ++d) {
+    a[d] = pos[3 * ia + d];
+    b[d] = pos[3 * ib + d];
+    c[d] = pos[3 * ic + d];
+  }
+  double eA = edge_len(b, c), eB = edge_len(a, c), eC = edge_len(a, b);
+  double ra = delta + eA - eB - eC;
+  double rb = delta + eB - eA - eC;
+  double rc = delta + eC - eA - eB;
+  double eps = max_f(0.0, max_f(ra, max_f(rb, rc)));
+  eA = eA + eps;
+  eB = eB + eps;
+  eC = eC + eps;
+  double cotA = cot_from_lengths(eA, eB, eC), cotB = cot_from_lengths(eB, eA, eC),
+         cotC = cot_from_lengths(eC, eA, eB);
+  double w_bc = cotA * 0.5, w_ac = cotB * 0.5, w_ab = cotC * 0.5;
+  double *o = m + 9 * (int64_t)f;
+  o[0] = w_ab + w_ac; o[1] = -w_ab;       o[2] = -w_ac;
+  o[3] = -w_ab;       o[4] = w_ab + w_bc; o[5] = -w_bc;
+  o[6] = -w_ac;       o[7] = -w_bc;       o[8] = w_ac + w_bc;
+}
+
 __global__ void k_flags(int32_t nv, const uint8_t *constrained, int32_t *fr,
                         int32_t *co) {
   int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -228,7 +272,10 @@ __global__ void k_harmonic_rhs(int32_t n_free, int32_t k,
 }
 
 static void assemble_values(cn_system_t *S, hipStream_t s) {
-  if (S->nf > 0)
+  if (S->nf > 0 && S->lap_kind == CN_LAP_ROBUST)
+    k_face_robust<<<grid_for(S->nf, 256), 256, 0, s>>>(S->nf, S->tris.get(), S->pos.get(),
+                                                      S->lap_delta, S->face_m.get());
+  else if (S->nf > 0)
     k_face_cot<<<grid_for(S->nf, 256), 256, 0, s>>>(S->nf, S->tris.get(),
                                                    S->pos.get(),
                                                    S->face_m.get());
@@ -259,12 +306,14 @@ using namespace cn;
 
 extern "C" {
 
-cn_status cn_system_create(int32_t nv, int32_t nf, const int32_t *d_tris,
-                           const uint8_t *d_constrained, const double *d_pos,
-                           cn_system *out, void *stream) {
+static cn_status system_create(int32_t nv, int32_t nf, const int32_t *d_tris,
+                               const uint8_t *d_constrained, const double *d_pos,
+                               int32_t lap_kind, double lap_delta, cn_system *out,
+                               void *stream) {
   CN_API_BEGIN
   if (!out || nv <= 0 || nf < 0 || !d_constrained || !d_pos || (nf > 0 && !d_tris) ||
-      nf > INT_MAX / 6) {
+      nf > INT_MAX / 6 || (lap_kind != CN_LAP_COT && lap_kind != CN_LAP_ROBUST) ||
+      !(lap_delta >= 0.0)) {
     set_error("cn_system_create: bad arguments");
     return CN_BAD_ARG;
   }
@@ -274,6 +323,8 @@ cn_status cn_system_create(int32_t nv, int32_t nf, const int32_t *d_tris,
   try {
     S->nv = nv;
     S->nf = nf;
+    S->lap_kind = lap_kind;
+    S->lap_delta = lap_delta;
     S->tris.alloc((size_t)3 * nf + 1);
     S->constrained.alloc((size_t)nv);
     S->pos.alloc((size_t)3 * nv);
@@ -359,6 +410,29 @@ cn_status cn_system_create(int32_t nv, int32_t nf, const int32_t *d_tris,
   *out = S;
   return CN_OK;
   CN_API_END
+}
+
+cn_status cn_system_create(int32_t nv, int32_t nf, const int32_t *d_tris,
+                           const uint8_t *d_constrained, const double *d_pos,
+                           cn_system *out, void *stream) {
+  return system_create(nv, nf, d_tris, d_constrained, d_pos, CN_LAP_COT, 0.0, out, stream);
+}
+
+void cn_system_default_opts(cn_system_opts *opts) {
+  if (!opts) return;
+  opts->laplacian = CN_LAP_COT;
+  opts->delta = 0.0;
+}
+
+cn_status cn_system_create_ex(int32_t nv, int32_t nf, const int32_t *d_tris,
+                              const uint8_t *d_constrained, const double *d_pos,
+                              const cn_system_opts *opts, cn_system *out, void *stream) {
+  if (!opts) {
+    set_error("cn_system_create_ex: no options");
+    return CN_BAD_ARG;
+  }
+  return system_create(nv, nf, d_tris, d_constrained, d_pos, opts->laplacian, opts->delta, out,
+                       stream);
 }
 
 cn_status cn_system_from_csr(int32_t n, const int32_t *d_ptr, const int32_t *d_col,

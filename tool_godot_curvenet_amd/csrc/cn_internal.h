@@ -132,6 +132,8 @@ void ensure_segments(cn_system_t *S, hipStream_t s);
 
 struct cn_system_t {
   int32_t nv = 0, nf = 0, n_free = 0, n_con = 0;
+  int32_t lap_kind = 0;             // CN_LAP_COT or CN_LAP_ROBUST
+  double lap_delta = 0.0;           // robust margin
   int64_t nnz_ff = 0, nnz_fc = 0;
   cn::DevBuf<int32_t> tris;         // cut-mesh triangles (3 nf)
   cn::DevBuf<uint8_t> constrained;  // nv

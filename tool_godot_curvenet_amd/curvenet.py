@@ -295,7 +295,9 @@ class CutLaplacian(_System):
     """VtLhV of a cut triangle mesh (CutMeshLaplacian.assembleVtLhV), stored
     as the free block A_ff and the coupling block A_fc."""
 
-    def __init__(self, pos, tris, constrained):
+    def __init__(self, pos, tris, constrained, laplacian: str = "cot", delta: float = 0.0):
+        """laplacian "cot" (the embedded cot, PolygonLaplacian) or "robust"
+        (RobustLaplacian, mollified edge lengths with margin delta)."""
         torch = _torch()
         lib = L.load()
         self.pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
@@ -305,16 +307,21 @@ class CutLaplacian(_System):
         c = _dev(self.constrained, torch.uint8)
         p = _dev(self.pos, torch.float64)
         h = C.c_void_p(0)
-        L.check(lib.cn_system_create(self.pos.shape[0], self.tris.shape[0], _p(t), _p(c), _p(p),
-                                     C.byref(h), _stream()), "system_create")
+        o = L.SystemOpts()
+        lib.cn_system_default_opts(C.byref(o))
+        o.laplacian = {"cot": L.CN_LAP_COT, "robust": L.CN_LAP_ROBUST}[laplacian]
+        o.delta = delta
+        L.check(lib.cn_system_create_ex(self.pos.shape[0], self.tris.shape[0], _p(t), _p(c),
+                                        _p(p), C.byref(o), C.byref(h), _stream()),
+                "system_create")
         super().__init__(h.value)
 
     @classmethod
-    def from_mesh(cls, mesh) -> "CutLaplacian":
+    def from_mesh(cls, mesh, laplacian: str = "cot", delta: float = 0.0) -> "CutLaplacian":
         """Cut a generated mesh along its cut edges and build the system."""
         nt, origin, con, nv2 = cut_seams(mesh.nv, mesh.tris, mesh.cut_edges)
         pos = mesh.pos[origin]
-        obj = cls(pos, nt, con)
+        obj = cls(pos, nt, con, laplacian=laplacian, delta=delta)
         obj.origin = origin
         return obj
 
