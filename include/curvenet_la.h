@@ -336,6 +336,42 @@ cn_status cn_hsc_apply(cn_hsc h, int32_t k, const double *d_r, double *d_z,
 cn_status cn_hsc_destroy(cn_hsc h);
 
 /* ------------------------------------------------------------------------
+ * The deformation solve (DeformSolve.solveDeformation, DeformSolve.lean:
+ * 110-135) on this system: stage 1 solves the deformation gradients F
+ * (9 columns a pose: cn_harmonic_rhs of F_c, then a solve), stage 2 the
+ * positions (3 columns a pose) with the right-hand side below, then a
+ * solve.  Poses are column blocks: F is n x 9P (pose p in columns
+ * 9p..9p+8, a 3x3 row-major), positions n x 3P.
+ * ---------------------------------------------------------------------- */
+
+/* Stage-2 right-hand side -V^T Lh (C X_c - y_h) (DeformSolve.lean:118-133)
+ * for the free rows, with the bridge folded in: f_h = F of the halfedge's
+ * target (computeFh, :38-52; d_Ffree for free vertices, d_Fc for
+ * constrained ones), F_f = the mean over the face loop (averageOverFaces,
+ * :56-69), y_h = X_target . F_f^T (computeYh, :74-96).  Summed in the
+ * assembly's order, bit-identical to the dense spec form. */
+cn_status cn_deform_rhs(cn_system sys, int32_t P, const double *d_Ffree,
+                        const double *d_Fc, const double *d_Xc, double *d_rhs,
+                        void *stream);
+
+/* Curve inputs of the solve, one thread per evaluation, the spec's
+ * operation order (FP64):
+ *   cn_bezier3: EndToEndExample.bezier3 (EndToEndExample.lean:52-60),
+ *     d_ctrl ncurves x 4 x 3, d_t nt, d_out ncurves x nt x 3;
+ *   cn_segment_gradients: ScaledFrames.isolatedSegmentGradient
+ *     (ScaledFrames.lean:90-95), F = (l / l_rest) R(t_rest, t), n x 3 points
+ *     in, n x 9 out;
+ *   cn_deformation_gradients: ScaledFrames.deformationGradient
+ *     (ScaledFrames.lean:147-150), F = posed inv(rest), n x 9 each. */
+cn_status cn_bezier3(int32_t ncurves, const double *d_ctrl, int32_t nt, const double *d_t,
+                     double *d_out, void *stream);
+cn_status cn_segment_gradients(int32_t n, const double *d_rest_p, const double *d_rest_q,
+                               const double *d_posed_p, const double *d_posed_q, double *d_F,
+                               void *stream);
+cn_status cn_deformation_gradients(int32_t n, const double *d_rest, const double *d_posed,
+                                   double *d_F, void *stream);
+
+/* ------------------------------------------------------------------------
  * Colour-parallel symmetric Gauss-Seidel (SlangCodegen/SgsColor.lean:
  * 87-130) on A_ff.  Rows are grouped by GraphColoring.greedyColor
  * (GraphColoring.lean:41-56); one sweep runs the SgsColor row update over

@@ -804,6 +804,213 @@ void orc_harmonic_rhs_csr(int32_t nv, int32_t nh, const int32_t *target,
 }
 
 /* ======================================================================
+ * Deformation solve (DeformSolve.lean) and its curve inputs
+ * (ScaledFrames.lean, EndToEndExample.lean)
+ * ==================================================================== */
+
+/* EndToEndExample.bezier3 (EndToEndExample.lean:52-60). */
+void orc_bezier3(const double *P, double t, double *out) {
+  double u = 1.0 - t;
+  double c0 = u * u * u, c1 = 3.0 * u * u * t, c2 = 3.0 * u * t * t, c3 = t * t * t;
+  for (int d = 0; d < 3; ++d)
+    out[d] = c0 * P[d] + c1 * P[3 + d] + c2 * P[6 + d] + c3 * P[9 + d];
+}
+
+/* ScaledFrames.tangentLength (ScaledFrames.lean:39-44). */
+static double tangent_length(const double *p, const double *q, double *t) {
+  double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
+  double l = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  double inv = l == 0.0 ? 0.0 : 1.0 / l;
+  for (int i = 0; i < 3; ++i) t[i] = d[i] * inv;
+  return l;
+}
+
+/* mat3Mul (ScaledFrames.lean:57-64): row i . column j, three terms left to
+ * right. */
+static void mat3_mul(const double *a, const double *b, double *c) {
+  for (int k = 0; k < 9; ++k) {
+    int i = k / 3, j = k % 3;
+    c[k] = a[i * 3] * b[j] + a[i * 3 + 1] * b[3 + j] + a[i * 3 + 2] * b[6 + j];
+  }
+}
+
+/* smallestRotation (ScaledFrames.lean:74-86): R = I + K + K^2 / (1 + cos). */
+static void smallest_rotation(const double *f, const double *t, double *R) {
+  double cx = f[1] * t[2] - f[2] * t[1];
+  double cy = f[2] * t[0] - f[0] * t[2];
+  double cz = f[0] * t[1] - f[1] * t[0];
+  double cosT = f[0] * t[0] + f[1] * t[1] + f[2] * t[2];
+  double K[9] = {0.0, -cz, cy, cz, 0.0, -cx, -cy, cx, 0.0}, K2[9];
+  mat3_mul(K, K, K2);
+  double denom = 1.0 + cosT;
+  double inv = denom == 0.0 ? 0.0 : 1.0 / denom;
+  static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  for (int k = 0; k < 9; ++k) R[k] = (I[k] + K[k]) + inv * K2[k];
+}
+
+/* isolatedSegmentGradient (ScaledFrames.lean:90-95): F = (l/l0) R(t0, t). */
+void orc_isolated_segment_gradient(const double *rp, const double *rq, const double *pp,
+                                   const double *pq, double *F) {
+  double t0[3], t1[3], R[9];
+  double l0 = tangent_length(rp, rq, t0);
+  double l1 = tangent_length(pp, pq, t1);
+  double r = l0 == 0.0 ? 0.0 : l1 / l0;
+  smallest_rotation(t0, t1, R);
+  for (int k = 0; k < 9; ++k) F[k] = r * R[k];
+}
+
+/* mat3Det / mat3Inv / deformationGradient (ScaledFrames.lean:108-150). */
+double orc_mat3_det(const double *m) {
+  double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], h = m[7],
+         i = m[8];
+  return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+}
+
+void orc_mat3_inv(const double *m, double *o) {
+  double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], h = m[7],
+         i = m[8];
+  double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+  double inv = det == 0.0 ? 0.0 : 1.0 / det;
+  o[0] = inv * (e * i - f * h); o[1] = inv * (c * h - b * i); o[2] = inv * (b * f - c * e);
+  o[3] = inv * (f * g - d * i); o[4] = inv * (a * i - c * g); o[5] = inv * (c * d - a * f);
+  o[6] = inv * (d * h - e * g); o[7] = inv * (b * g - a * h); o[8] = inv * (a * e - b * d);
+}
+
+void orc_deformation_gradient(const double *rest, const double *posed, double *F) {
+  double ri[9];
+  orc_mat3_inv(rest, ri);
+  mat3_mul(posed, ri, F);
+}
+
+/* Per-face average of a per-vertex 9-column F over the face loop
+ * (averageOverFaces, DeformSolve.lean:56-69): sum from 0 in loop order,
+ * times 1/n. */
+static void face_average(const face_tables *T, const int32_t *target, int32_t nf,
+                         const double *F, int32_t k, double *Ff) {
+  for (int32_t f = 0; f < nf; ++f) {
+    int32_t a = T->loop_ptr[f], n = T->loop_ptr[f + 1] - a;
+    for (int32_t j = 0; j < k; ++j) Ff[(size_t)f * k + j] = 0.0;
+    if (n == 0) continue;
+    double invN = 1.0 / (double)n;
+    for (int32_t j = 0; j < k; ++j) {
+      double s = 0.0;
+      for (int32_t i = 0; i < n; ++i) s = s + F[(size_t)target[T->loop_he[a + i]] * k + j];
+      Ff[(size_t)f * k + j] = s * invN;
+    }
+  }
+}
+
+/* computeYh (DeformSolve.lean:74-96): y_h = pos[target] . row of F_f,
+ * one 9-column block of Ff per pose. */
+static double yh_entry(const double *pos, const double *Ff9, int32_t i) {
+  return pos[0] * Ff9[3 * i] + pos[1] * Ff9[3 * i + 1] + pos[2] * Ff9[3 * i + 2];
+}
+
+/* The stage-2 right-hand side -V^T Lh (C Xc - y_h) in the dense spec form
+ * (DeformSolve.lean:118-133), P poses: F is nv x 9P (per vertex; samples
+ * hold their F_c row, i.e. computeFh folded in), Xc nc x 3P. */
+void orc_deform_rhs_dense(int32_t nv, int32_t nf, int32_t nh, const int32_t *target,
+                          const int32_t *next, const int32_t *face,
+                          const int32_t *col_of_vertex, const double *pos, int32_t P,
+                          const double *F, const double *Xc, double *rhs) {
+  face_tables T;
+  build_face_tables(nf, nh, target, next, face, pos, &T);
+  const int32_t k9 = 9 * P, k3 = 3 * P;
+  double *Ff = (double *)calloc((size_t)nf * k9 + 1, sizeof(double));
+  face_average(&T, target, nf, F, k9, Ff);
+  double *L = (double *)calloc((size_t)nh * nh, sizeof(double));
+  for (int32_t f = 0; f < nf; ++f) {
+    int32_t a = T.loop_ptr[f], n = T.loop_ptr[f + 1] - a;
+    if (n < 3) continue;
+    const double *Lf = T.mat + T.mat_ptr[f];
+    for (int32_t li = 0; li < n; ++li)
+      for (int32_t lj = 0; lj < n; ++lj) {
+        int32_t gi = T.loop_he[a + li], gj = T.loop_he[a + lj];
+        L[(size_t)gi * nh + gj] = L[(size_t)gi * nh + gj] + Lf[li * n + lj];
+      }
+  }
+  double *diff = (double *)calloc((size_t)nh * k3 + 1, sizeof(double));
+  for (int32_t h = 0; h < nh; ++h) {
+    int32_t c = col_of_vertex[target[h]], f = face[h];
+    for (int32_t j = 0; j < k3; ++j) {
+      double cx = c >= 0 ? Xc[(size_t)c * k3 + j] : 0.0;
+      double y = f >= 0 ? yh_entry(pos + 3 * target[h], Ff + (size_t)f * k9 + 9 * (j / 3), j % 3)
+                        : 0.0;
+      diff[(size_t)h * k3 + j] = cx - y;
+    }
+  }
+  double *LD = (double *)calloc((size_t)nh * k3 + 1, sizeof(double));
+  for (int32_t i = 0; i < nh; ++i)
+    for (int32_t j = 0; j < k3; ++j) {
+      double t = 0.0;
+      for (int32_t p = 0; p < nh; ++p) t = t + L[(size_t)i * nh + p] * diff[(size_t)p * k3 + j];
+      LD[(size_t)i * k3 + j] = t;
+    }
+  for (int32_t i = 0; i < nv; ++i)
+    for (int32_t j = 0; j < k3; ++j) {
+      double t = 0.0;
+      for (int32_t p = 0; p < nh; ++p) {
+        double vt = (col_of_vertex[target[p]] < 0 && target[p] == i) ? 1.0 : 0.0;
+        t = t + vt * LD[(size_t)p * k3 + j];
+      }
+      rhs[(size_t)i * k3 + j] = -t;
+    }
+  free(Ff); free(L); free(diff); free(LD);
+  free_face_tables(&T);
+}
+
+/* The same right-hand side in the sparse assembly order of
+ * orc_harmonic_rhs_csr (incoming halfedges of a free vertex ascending, the
+ * face's halfedges ascending), for the free rows; F split into its free
+ * rows (Ffree, nfree x 9P) and constrained rows (Fc, nc x 9P). */
+void orc_deform_rhs_csr(int32_t nv, int32_t nh, const int32_t *target, const int32_t *next,
+                        const int32_t *face, const int32_t *free_of, const int32_t *con_of,
+                        const double *pos, int32_t P, const double *Ffree, const double *Fc,
+                        const double *Xc, double *rhs) {
+  int32_t nf = 0;
+  for (int32_t h = 0; h < nh; ++h) if (face[h] + 1 > nf) nf = face[h] + 1;
+  face_tables T;
+  build_face_tables(nf, nh, target, next, face, pos, &T);
+  const int32_t k9 = 9 * P, k3 = 3 * P;
+  double *F = (double *)calloc((size_t)nv * k9 + 1, sizeof(double));
+  for (int32_t v = 0; v < nv; ++v) {
+    const double *src = free_of[v] >= 0 ? Ffree + (size_t)free_of[v] * k9
+                                        : (con_of[v] >= 0 ? Fc + (size_t)con_of[v] * k9 : NULL);
+    if (src) memcpy(F + (size_t)v * k9, src, sizeof(double) * (size_t)k9);
+  }
+  double *Ff = (double *)calloc((size_t)nf * k9 + 1, sizeof(double));
+  face_average(&T, target, nf, F, k9, Ff);
+  int32_t *iptr, *ihe;
+  incoming_lists(nv, nh, target, &iptr, &ihe);
+  int32_t *sorted = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nh + 1));
+  for (int32_t v = 0; v < nv; ++v) {
+    int32_t i = free_of[v];
+    if (i < 0) continue;
+    for (int32_t j = 0; j < k3; ++j) {
+      double s = 0.0;
+      for (int32_t a = iptr[v]; a < iptr[v + 1]; ++a) {
+        int32_t p = ihe[a], f = face[p];
+        if (f < 0) continue;
+        int32_t b = T.loop_ptr[f], n = T.loop_ptr[f + 1] - b;
+        memcpy(sorted, T.loop_he + b, sizeof(int32_t) * (size_t)n);
+        qsort(sorted, (size_t)n, sizeof(int32_t), cmp_i32);
+        double tsum = 0.0;
+        for (int32_t e = 0; e < n; ++e) {
+          int32_t q = sorted[e], cc = con_of[target[q]];
+          double cx = cc >= 0 ? Xc[(size_t)cc * k3 + j] : 0.0;
+          double y = yh_entry(pos + 3 * target[q], Ff + (size_t)f * k9 + 9 * (j / 3), j % 3);
+          tsum = tsum + lh_entry(&T, face, p, q) * (cx - y);
+        }
+        s = s + tsum;
+      }
+      rhs[(size_t)i * k3 + j] = -s;
+    }
+  }
+  free(sorted); free(iptr); free(ihe); free(F); free(Ff);
+  free_face_tables(&T);
+}
+
+/* ======================================================================
  * Sparse linear algebra (SparseLinAlg.lean)
  * ==================================================================== */
 

@@ -134,6 +134,21 @@ void orc_harmonic_rhs_csr(int32_t nv, int32_t nh, const int32_t *target,
                           const int32_t *free_of, const int32_t *con_of,
                           const double *pos, int32_t k, const double *Xc,
                           double *rhs);
+void orc_bezier3(const double *P, double t, double *out);
+void orc_isolated_segment_gradient(const double *rp, const double *rq, const double *pp,
+                                   const double *pq, double *F);
+double orc_mat3_det(const double *m);
+void orc_mat3_inv(const double *m, double *o);
+void orc_deformation_gradient(const double *rest, const double *posed, double *F);
+void orc_deform_rhs_dense(int32_t nv, int32_t nf, int32_t nh, const int32_t *target,
+                          const int32_t *next, const int32_t *face,
+                          const int32_t *col_of_vertex, const double *pos, int32_t P,
+                          const double *F, const double *Xc, double *rhs);
+void orc_deform_rhs_csr(int32_t nv, int32_t nh, const int32_t *target, const int32_t *next,
+                        const int32_t *face, const int32_t *free_of, const int32_t *con_of,
+                        const double *pos, int32_t P, const double *Ffree, const double *Fc,
+                        const double *Xc, double *rhs);
+
 
 /* ---- Sparse linear algebra (SparseLinAlg.lean) ------------------------ */
 

@@ -35,6 +35,12 @@ PINS = {
     "assemble_vtlhv_dense": "CutMeshLaplacian.lean:101-179 KATs, linear precision",
     "cut_laplacian_csr": "bit-identical to the dense spec transcription on small cut meshes",
     "harmonic_rhs_csr": "bit-identical to the dense spec transcription; HarmonicSolve KATs",
+    "solve_deformation_dense": "DeformSolve.lean:164-253 KATs (translation, rotation, scale on "
+                               "triangleWithSample); EndToEndExample Bezier-driven translation",
+    "deform_rhs_csr": "bit-identical to the dense spec transcription (deform_rhs_dense); a "
+                      "rigid motion is reproduced exactly by the two-stage solve",
+    "bezier3/isolated_segment_gradient/deformation_gradient/mat3_inv": "ScaledFrames.lean and "
+        "EndToEndExample.lean KATs (identity, rotation, scale, rotate+scale, det, inverse)",
     "spmv/spmm": "identity KAT; numpy reference",
     "cg": "SparseLinAlg.lean:171-287 KATs; dense GE",
     "ic0_factor": "LL^T == A on the kept pattern; IncompleteCholesky.lean:128-186 KATs; "
@@ -81,6 +87,7 @@ def lib():
         _lib.orc_hsc_degree_weight.restype = f64
         _lib.orc_hsc_edge_weight.restype = f64
         _lib.orc_pcg_many.restype = i64
+        _lib.orc_mat3_det.restype = f64
         _lib.orc_mollify_triangle.restype = f64
         _lib.orc_mollify_triangle.argtypes = [f64, f64, f64, f64]
         _lib.orc_cot_from_lengths.restype = f64
@@ -271,6 +278,72 @@ def cut_laplacian_csr(he, constrained, pos):
     return {"n_free": nfree, "free_of": free_of, "con_of": con_of,
             "A_ff": (ffp[:nfree + 1].copy(), ffc[:a].copy(), ffv[:a].copy()),
             "A_fc": (fcp[:nfree + 1].copy(), fcc[:b].copy(), fcv[:b].copy())}
+
+
+def bezier3(P, t):
+    """EndToEndExample.bezier3 (EndToEndExample.lean:52-60); P is 4 x 3."""
+    out = np.zeros(3)
+    lib().orc_bezier3(_p(_d(P)), C.c_double(t), _p(out))
+    return out
+
+
+def isolated_segment_gradient(rest_p, rest_q, posed_p, posed_q):
+    """ScaledFrames.isolatedSegmentGradient (ScaledFrames.lean:90-95), 3 x 3."""
+    F = np.zeros(9)
+    lib().orc_isolated_segment_gradient(_p(_d(rest_p)), _p(_d(rest_q)), _p(_d(posed_p)),
+                                        _p(_d(posed_q)), _p(F))
+    return F.reshape(3, 3)
+
+
+def deformation_gradient(rest, posed):
+    """ScaledFrames.deformationGradient (ScaledFrames.lean:147-150)."""
+    F = np.zeros(9)
+    lib().orc_deformation_gradient(_p(_d(rest)), _p(_d(posed)), _p(F))
+    return F.reshape(3, 3)
+
+
+def mat3_inv(m):
+    o = np.zeros(9)
+    lib().orc_mat3_inv(_p(_d(m)), _p(o))
+    return o.reshape(3, 3)
+
+
+def deform_rhs_dense(he, col_of_vertex, pos, F, Xc):
+    """Stage-2 RHS -V^T Lh (C Xc - y_h) (DeformSolve.lean:118-133), dense
+    spec form; F per vertex (nv x 9P, samples hold F_c), Xc nc x 3P."""
+    F, Xc = _d(np.asarray(F)), _d(np.asarray(Xc))
+    P = F.shape[1] // 9
+    out = np.zeros(he["nv"] * 3 * P)
+    lib().orc_deform_rhs_dense(he["nv"], he["nf"], he["nh"], _p(_i(he["target"])),
+                               _p(_i(he["next"])), _p(_i(he["face"])), _p(_i(col_of_vertex)),
+                               _p(_d(pos)), P, _p(F), _p(Xc), _p(out))
+    return out.reshape(he["nv"], 3 * P)
+
+
+def deform_rhs_csr(he, free_of, con_of, pos, Ffree, Fc, Xc):
+    """The stage-2 RHS in the sparse assembly order, free rows only."""
+    Ffree, Fc, Xc = _d(np.asarray(Ffree)), _d(np.asarray(Fc)), _d(np.asarray(Xc))
+    P = Ffree.shape[1] // 9
+    nfree = int((np.asarray(free_of) >= 0).sum())
+    out = np.zeros(nfree * 3 * P)
+    lib().orc_deform_rhs_csr(he["nv"], he["nh"], _p(_i(he["target"])), _p(_i(he["next"])),
+                             _p(_i(he["face"])), _p(_i(free_of)), _p(_i(con_of)), _p(_d(pos)),
+                             P, _p(Ffree), _p(Fc), _p(Xc), _p(out))
+    return out.reshape(nfree, 3 * P)
+
+
+def solve_deformation_dense(he, is_sample, pos, Fc, Xc):
+    """DeformSolve.solveDeformation (DeformSolve.lean:110-135), the dense
+    transcription: stage 1 (k = 9), computeFh, averageOverFaces, computeYh,
+    stage 2 (k = 3), each solve by DenseLinAlg.solveMulti.  One pose; the
+    sample rows of the result are 0 (the caller overlays them)."""
+    is_sample = np.asarray(is_sample, np.uint8)
+    col = np.where(is_sample > 0, np.cumsum(is_sample) - 1, -1).astype(np.int32)
+    lhs = assemble_vtlhv_dense(he, is_sample, pos)
+    Fv = dense_solve_multi(lhs, harmonic_rhs_dense(he, col, pos, Fc))
+    F = Fv.copy()
+    F[is_sample > 0] = np.asarray(Fc).reshape(-1, 9)      # computeFh: samples take F_c
+    return dense_solve_multi(lhs, deform_rhs_dense(he, col, pos, F, Xc))
 
 
 def harmonic_rhs_csr(he, free_of, con_of, pos, Xc):

@@ -17,19 +17,23 @@ from .curvenet import (  # noqa: F401
     BatchSystem,
     cg,
     cg_with_guess,
+    bezier3,
     cut_seams,
+    deform_solve,
+    deformation_gradients,
     dense_cholesky,
     dense_cholesky_solve,
     from_triangles,
     harmonic_solve,
     project_to_vertices,
+    segment_gradients,
     spmm,
     spmv,
 )
 from .autograd import harmonic_solve as harmonic_solve_ad  # noqa: F401
 
 __all__ = [
-    "HSC", "SGS", "Restriction", "dense_cholesky", "dense_cholesky_solve", "IC0", "CutLaplacian", "CsrMatrix", "CsrSystem", "BatchSystem", "cg", "cg_with_guess", "cut_seams",
+    "HSC", "SGS", "Restriction", "dense_cholesky", "dense_cholesky_solve", "IC0", "CutLaplacian", "CsrMatrix", "CsrSystem", "BatchSystem", "cg", "cg_with_guess", "cut_seams", "deform_solve", "bezier3", "segment_gradients", "deformation_gradients",
     "from_triangles", "harmonic_solve", "harmonic_solve_ad", "project_to_vertices", "spmm",
     "spmv", "curves",
 ]

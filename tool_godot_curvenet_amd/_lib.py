@@ -148,6 +148,10 @@ _SIGS = {
     "cn_restrict_sum_f64": (i32, [vp, i32, vp, vp, vp]),
     "cn_restrict_sum_f32": (i32, [vp, i32, vp, vp, vp]),
     "cn_restrict_destroy": (i32, [vp]),
+    "cn_deform_rhs": (i32, [vp, i32, vp, vp, vp, vp, vp]),
+    "cn_bezier3": (i32, [i32, vp, i32, vp, vp, vp]),
+    "cn_segment_gradients": (i32, [i32, vp, vp, vp, vp, vp, vp]),
+    "cn_deformation_gradients": (i32, [i32, vp, vp, vp, vp]),
     "cn_profile_enable": (None, [i32]),
     "cn_profile_query": (i32, [C.c_char_p, C.POINTER(f64)]),
 }

@@ -167,6 +167,7 @@ struct cn_system_t {
   int32_t nseg = 1, seg_blocks = 0;
   std::vector<int32_t> seg_ptr;      // host, nseg + 1
   cn::DevBuf<int32_t> d_seg_ptr, seg_of_row, blk_ptr, blk_seg;
+  cn::DevBuf<double> deform_ff;     // per-face F of cn_deform_rhs, grown on demand
   cn::SolverWork *work = nullptr;   // solver workspace, grown on demand
   ~cn_system_t() { cn::free_solver_work(work); }
 };

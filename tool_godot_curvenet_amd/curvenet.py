@@ -372,6 +372,73 @@ class CsrSystem(_System):
         super().__init__(h.value)
 
 
+def deform_solve(lap: "CutLaplacian", Fc, Xc, precond: str = "hsc", precision: str = "mixed",
+                 tol: float = 1e-9, hsc: "HSC | None" = None, ic0: "IC0 | None" = None):
+    """DeformSolve.solveDeformation (DeformSolve.lean:110-135) for P poses:
+    Fc (n_con x 9P, a row-major 3x3 per pose), Xc (n_con x 3P).  Stage 1
+    solves the free vertices' deformation gradients, stage 2 their
+    positions.  Returns (x_free n_free x 3P, x_all nv x 3P with the
+    constrained vertices overlaid (DeformSolve.lean:108-109), info1, info2)
+    as device tensors."""
+    torch = _torch()
+    lib = L.load()
+    fc = _as_tensor(Fc, torch.float64).reshape(lap.n_con, -1).contiguous()
+    xc = _as_tensor(Xc, torch.float64).reshape(lap.n_con, -1).contiguous()
+    P = fc.shape[1] // 9
+    if fc.shape[1] != 9 * P or xc.shape[1] != 3 * P or P == 0:
+        raise ValueError("Fc must be n_con x 9P and Xc n_con x 3P")
+    kw = {"hsc": hsc} if precond == "hsc" else ({"ic0": ic0} if precond == "ic0" else {})
+    b1 = lap.harmonic_rhs(fc, as_numpy=False)
+    fv, info1 = lap.solve(b1, precond=precond, precision=precision, tol=tol, **kw)
+    b2 = torch.empty((lap.n_free, 3 * P), dtype=torch.float64, device="cuda")
+    L.check(lib.cn_deform_rhs(lap.handle, P, _p(fv), _p(fc), _p(xc), _p(b2), _stream()),
+            "deform_rhs")
+    xv, info2 = lap.solve(b2, precond=precond, precision=precision, tol=tol, **kw)
+    free_of, con_of = lap.maps()
+    fo = torch.as_tensor(free_of, device="cuda").long()
+    co = torch.as_tensor(con_of, device="cuda").long()
+    xa = torch.zeros((len(free_of), 3 * P), dtype=torch.float64, device="cuda")
+    xa[fo >= 0] = xv[fo[fo >= 0]]
+    xa[co >= 0] = xc[co[co >= 0]]
+    return xv, xa, info1, info2
+
+
+def bezier3(ctrl, t):
+    """EndToEndExample.bezier3 on the GPU: ctrl (ncurves x 4 x 3), t (nt)
+    -> ncurves x nt x 3 (cn_bezier3)."""
+    torch = _torch()
+    c = _as_tensor(np.asarray(ctrl, np.float64).reshape(-1, 12), torch.float64).contiguous()
+    tv = _as_tensor(np.asarray(t, np.float64).reshape(-1), torch.float64).contiguous()
+    out = torch.empty((c.shape[0], tv.shape[0], 3), dtype=torch.float64, device="cuda")
+    L.check(L.load().cn_bezier3(c.shape[0], _p(c), tv.shape[0], _p(tv), _p(out), _stream()),
+            "bezier3")
+    return out.cpu().numpy()
+
+
+def segment_gradients(rest_p, rest_q, posed_p, posed_q):
+    """ScaledFrames.isolatedSegmentGradient per segment (cn_segment_gradients):
+    n x 3 points each -> n x 3 x 3."""
+    torch = _torch()
+    a = [_as_tensor(np.asarray(v, np.float64).reshape(-1, 3), torch.float64).contiguous()
+         for v in (rest_p, rest_q, posed_p, posed_q)]
+    n = a[0].shape[0]
+    F = torch.empty((n, 9), dtype=torch.float64, device="cuda")
+    L.check(L.load().cn_segment_gradients(n, *[_p(v) for v in a], _p(F), _stream()),
+            "segment_gradients")
+    return F.cpu().numpy().reshape(n, 3, 3)
+
+
+def deformation_gradients(rest, posed):
+    """ScaledFrames.deformationGradient per frame pair: n x 3 x 3 each."""
+    torch = _torch()
+    r = _as_tensor(np.asarray(rest, np.float64).reshape(-1, 9), torch.float64).contiguous()
+    p = _as_tensor(np.asarray(posed, np.float64).reshape(-1, 9), torch.float64).contiguous()
+    F = torch.empty_like(r)
+    L.check(L.load().cn_deformation_gradients(r.shape[0], _p(r), _p(p), _p(F), _stream()),
+            "deformation_gradients")
+    return F.cpu().numpy().reshape(-1, 3, 3)
+
+
 class BatchSystem(_System):
     """Independent systems solved together (cn_system_batch): a
     block-diagonal copy of their A_ff.  Right-hand sides and solutions stack
