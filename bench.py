@@ -30,11 +30,11 @@ WORKLOAD = "capsules100k"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", f"traffic_{WORKLOAD}.json")
 # kernels whose HBM traffic is reported (name prefixes in the counter files)
 TRAFFIC_KERNELS = {
-    "spmm": "void cn::kw_spmm_dot<float, 4>",
+    "spmm": "void cn::kw_spmm_dot<float, 4, 64>",
     "fwd": "void cn::kb_fwd<float, 4, 16, true>",
     "bwd": "void cn::kb_bwd<float, 4, 16>",
-    "hsc_down": "void cn::kw_hsc_down<float, 4>",
-    "hsc_up": "void cn::kw_hsc_up<float, 4>",
+    "hsc_down": "void cn::kw_hsc_down<float, 4, 64>",
+    "hsc_up": "void cn::kw_hsc_up<float, 4, 64>",
     "pd_update": "void cn::k_pd_update<float>",
 }
 FP64_MFMA_PEAK_TFLOPS = 78.6
@@ -242,7 +242,7 @@ def main():
     nk4 = 4 * n * k
     spmm_bytes = 8 * nnz + 4 * (n + 1) + 2 * nk4
     kern = {
-        "spmm": ("kw_spmm_dot<float,4>", prof["spmm"][0], prof["spmm"][1],
+        "spmm": ("kw_spmm_dot<float,4,64>", prof["spmm"][0], prof["spmm"][1],
                  iters * spmm_bytes),
     }
     coarse = None
@@ -271,9 +271,9 @@ def main():
         nlev = max(pre.info.nlevels, 1)
         napply = prof["hsc_down"][0]
         nwork = iters + args.steps
-        kern["hsc_down"] = ("kw_hsc_down<float,4>", napply * nlev, prof["hsc_down"][1],
+        kern["hsc_down"] = ("kw_hsc_down<float,4,64>", napply * nlev, prof["hsc_down"][1],
                             nwork * down_b)
-        kern["hsc_up"] = ("kw_hsc_up<float,4>", napply * nlev, prof["hsc_up"][1],
+        kern["hsc_up"] = ("kw_hsc_up<float,4,64>", napply * nlev, prof["hsc_up"][1],
                           nwork * up_b)
         kern["pd_update"] = ("k_pd_update<float>", prof["bwd"][0], prof["bwd"][1],
                              nwork * 5 * nk4)

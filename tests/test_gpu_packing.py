@@ -1,0 +1,116 @@
+"""Moderate column counts (5 <= k <= 190), where the wide row kernels pack
+16 or 32 lanes per row: a few PCG iterations against the same iterations in
+numpy over the oracle's SpMM and hierarchy, and converged solves.
+
+The lanes per row follow from k: FP32 columns k = 5, 9, 12, 48, 144 pack 16
+lanes per row, k = 27, 100, 190 pack 32 (190 in three column tiles), and
+k = 200 keeps 64; FP64 columns k = 5, 12, 144 pack 16, k = 27, 48, 190 pack 32.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KS = [5, 12, 27, 48, 144, 190, 200]
+
+
+def _lap(name):
+    from tool_godot_curvenet_amd import CutLaplacian, meshgen
+    return CutLaplacian.from_mesh(meshgen.config_mesh(name))
+
+
+def _pcg_steps(A, B, apply_m, iters):
+    """`iters` steps of preconditioned CG from 0 (SparseLinAlg.lean:104-163
+    with M^-1 applied to the residual), every column at once."""
+    X = np.zeros_like(B)
+    R = B.copy()
+    Z = apply_m(R)
+    P = Z.copy()
+    rz = np.sum(R * Z, axis=0)
+    for _ in range(iters):
+        Q = O.spmm(A.row_ptr, A.col_idx, A.values, P)
+        al = rz / np.sum(P * Q, axis=0)
+        X += al * P
+        R -= al * Q
+        Z = apply_m(R)
+        rz1 = np.sum(R * Z, axis=0)
+        P = Z + (rz1 / rz) * P
+        rz = rz1
+    return X
+
+
+@pytest.mark.parametrize("precond", ["hsc", "jacobi"])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("mixed", 2e-4)])
+@pytest.mark.parametrize("k", KS)
+def test_pcg_steps_match_numpy(gpu, precond, precision, tol, k):
+    from tool_godot_curvenet_amd import HSC
+    from tool_godot_curvenet_amd import meshgen as G
+    lap = _lap("cylinder64")
+    A = lap.A_ff()
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=300 + k))
+    if precond == "hsc":
+        h = O.hsc_build(A.row_ptr, A.col_idx, A.values)
+        apply_m = lambda R: O.hsc_apply(h, R)  # noqa: E731
+        extra = {"hsc": HSC(lap)}
+    else:
+        d = O.csr_diag(A.row_ptr, A.col_idx, A.values)
+        dinv = np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 0.0)
+        apply_m = lambda R: R * dinv[:, None]  # noqa: E731
+        extra = {}
+    x, info = lap.solve(B, precond=precond, precision=precision, tol=1e-14, max_iter=3,
+                        **extra)
+    assert info.iters == 3, info.iters
+    X = x.cpu().numpy()
+    Xo = _pcg_steps(A, B, apply_m, 3)
+    err = np.linalg.norm(X - Xo, axis=0) / np.linalg.norm(Xo, axis=0)
+    assert np.all(err <= tol), (err.max(), int(err.argmax()))
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("k", [9, 27, 48, 144])
+def test_pcg_hsc_converges_moderate_k(gpu, precision, k):
+    from tool_godot_curvenet_amd import HSC
+    from tool_godot_curvenet_amd import meshgen as G
+    lap = _lap("icosphere6")
+    A = lap.A_ff()
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=400 + k))
+    x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, hsc=HSC(lap))
+    assert info.status == 0
+    X = x.cpu().numpy()
+    R = B - O.spmm(A.row_ptr, A.col_idx, A.values, X)
+    rr = np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
+    assert np.all(rr <= 1e-9), rr.max()
+
+
+@pytest.mark.parametrize("k", [5, 27, 144])
+def test_hsc_apply_moderate_k(gpu, k):
+    from tool_godot_curvenet_amd import HSC
+    lap = _lap("icosphere6")
+    A = lap.A_ff()
+    H = HSC(lap)
+    ho = O.hsc_build(A.row_ptr, A.col_idx, A.values)
+    r = np.random.default_rng(k).normal(size=(lap.n_free, k))
+    zg = H.apply(r)
+    zo = O.hsc_apply(ho, r)
+    assert np.linalg.norm(zg - zo) <= 1e-11 * np.linalg.norm(zo)
+
+
+def test_warm_start_moderate_k(gpu):
+    """A warm start runs the FP64 residual kernel on a nonzero x (k = 27:
+    32 lanes per row)."""
+    from tool_godot_curvenet_amd import HSC
+    from tool_godot_curvenet_amd import meshgen as G
+    lap = _lap("cylinder64")
+    A = lap.A_ff()
+    k = 27
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=77))
+    H = HSC(lap)
+    x1, _ = lap.solve(B, precond="hsc", precision="fp64", tol=1e-4, hsc=H)
+    x, info = lap.solve(B, precond="hsc", precision="mixed", tol=1e-10, hsc=H, x0=x1)
+    assert info.status == 0
+    X = x.cpu().numpy()
+    R = B - O.spmm(A.row_ptr, A.col_idx, A.values, X)
+    rr = np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
+    assert np.all(rr <= 1e-10), rr.max()

@@ -30,56 +30,57 @@ template <typename TB, typename TC>
 void dense_gemm_nn_rows(int m, int k, const double *M, const TB *X, int64_t ldx, TC *Z,
                         int64_t ldz, const int32_t *zrows, const int32_t *ctl, hipStream_t s);
 
-// ---- wide form: one wavefront per (row, 64 VEC column tile) ------------
+// ---- wide form: LPR lanes per (row, LPR VEC column tile), 64 / LPR rows
+// per wavefront -----------------------------------------------------------
 // Z[Cg[c]] = src[Cg[c]] + sum_e rval[e] src[rcol[e]]   (src = r or Z)
-template <typename T, int VEC>
+template <typename T, int VEC, int LPR>
 __global__ void __launch_bounds__(kBlock)
 kw_hsc_down(int32_t nC, const int32_t *__restrict__ Cg, const int32_t *__restrict__ rptr,
             const int32_t *__restrict__ rcol, const T *__restrict__ rval, int64_t k,
             const T *src, T *Z, const int32_t *__restrict__ ctl) {
   if (ctl && ctl_idle(ctl)) return;
-  const int lane = threadIdx.x & 63;
-  const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
+  using P = Pack<LPR>;
+  const int64_t c0 = P::col0(VEC);
   const bool valid = c0 < k;
   // loads and their uses are unconditional (lanes past k read the last VEC
   // columns), only the store is not: a load consumed under a branch makes
   // the compiler wait for the previous row's store before the next row
   const int64_t cl = valid ? c0 : k - VEC;
-  const RowIter ri = row_iter(0, nC, kWaves, threadIdx.x >> 6);
+  const RowIter ri = row_iter(0, nC, P::kSlots, P::slot());
   for (int32_t row = ri.first; row < ri.end; row += ri.stride) {
-    const int32_t r = uniform(row);
-    const int64_t g = (int64_t)uniform(Cg[r]) * k;
+    const int32_t r = P::u(row);
+    const int64_t g = (int64_t)P::u(Cg[r]) * k;
     T acc[VEC];
     VecIO<T, VEC>::ld(src + g + cl, acc);
     // a kept row has ~2-3 eliminated neighbours: chunks of 4 gathers
-    gather_wide<T, VEC, 4>(uniform(rptr[r]), uniform(rptr[r + 1]), rcol, rval, src, k, cl, true,
+    gather_wide<T, VEC, 4>(P::u(rptr[r]), P::u(rptr[r + 1]), rcol, rval, src, k, cl, true,
                            acc);
     if (valid) VecIO<T, VEC>::st(Z + g + c0, acc);
   }
 }
 
 // Z[Fg[f]] = dinv[f] src[Fg[f]] + sum_e pval[e] Z[pcol[e]]   (src = r or Z)
-template <typename T, int VEC>
+template <typename T, int VEC, int LPR>
 __global__ void __launch_bounds__(kBlock)
 kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv,
           const int32_t *__restrict__ pptr, const int32_t *__restrict__ pcol,
           const T *__restrict__ pval, int64_t k, const T *src, T *Z,
           const int32_t *__restrict__ ctl) {
   if (ctl && ctl_idle(ctl)) return;
-  const int lane = threadIdx.x & 63;
-  const int64_t c0 = (int64_t)blockIdx.y * 64 * VEC + lane * VEC;
+  using P = Pack<LPR>;
+  const int64_t c0 = P::col0(VEC);
   const bool valid = c0 < k;
   const int64_t cl = valid ? c0 : k - VEC;   // as in kw_hsc_down
-  const RowIter ri = row_iter(0, nF, kWaves, threadIdx.x >> 6);
+  const RowIter ri = row_iter(0, nF, P::kSlots, P::slot());
   for (int32_t row = ri.first; row < ri.end; row += ri.stride) {
-    const int32_t f = uniform(row);
-    const int64_t g = (int64_t)uniform(Fg[f]) * k;
+    const int32_t f = P::u(row);
+    const int64_t g = (int64_t)P::u(Fg[f]) * k;
     const T d = dinv[f];
     T acc[VEC];
     VecIO<T, VEC>::ld(src + g + cl, acc);
 #pragma unroll
     for (int m = 0; m < VEC; ++m) acc[m] = d * acc[m];
-    gather_wide<T, VEC>(uniform(pptr[f]), uniform(pptr[f + 1]), pcol, pval, Z, k, cl, true, acc);
+    gather_wide<T, VEC>(P::u(pptr[f]), P::u(pptr[f + 1]), pcol, pval, Z, k, cl, true, acc);
     if (valid) VecIO<T, VEC>::st(Z + g + c0, acc);
   }
 }
@@ -184,12 +185,21 @@ static int hsc_vec(int64_t k) {
   return (k % 2 == 0) ? 2 : 1;
 }
 
-static dim3 hsc_grid(int64_t rows, int64_t k, int v) {
-  int64_t g = (rows + kWaves - 1) / kWaves;
+static dim3 hsc_grid(int64_t rows, int64_t k, int v, int lpr) {
+  const int slots = kWaves * (64 / lpr);
+  int64_t g = (rows + slots - 1) / slots;
   g = std::max<int64_t>(1, std::min<int64_t>(g, 1024));
   if (g >= 64) g -= g % 8;
-  return dim3((unsigned)g, (unsigned)((k + 64 * v - 1) / (64 * v)), 1);
+  return dim3((unsigned)g, (unsigned)((k + (int64_t)lpr * v - 1) / ((int64_t)lpr * v)), 1);
 }
+
+// launch WIDE(V, L) for the lanes per row L of vector width V
+#define HSC_LPR(lpr, WIDE, V)                                                 \
+  do {                                                                        \
+    if ((lpr) == 16) { WIDE(V, 16); }                                         \
+    else if ((lpr) == 32) { WIDE(V, 32); }                                    \
+    else { WIDE(V, 64); }                                                     \
+  } while (0)
 
 template <typename T> struct LvVals;
 template <> struct LvVals<double> {
@@ -214,12 +224,13 @@ static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
 #undef N_
     return;
   }
-  const dim3 g = hsc_grid(L.nC, k, v);
-#define D_(VV) kw_hsc_down<T, VV><<<g, kBlock, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(), L.r_colg.get(), LvVals<T>::r(L), k, src, Z, ctl)
+  const int lpr = pack_lpr((k + v - 1) / v);
+  const dim3 g = hsc_grid(L.nC, k, v, lpr);
+#define D_(VV, LL) kw_hsc_down<T, VV, LL><<<g, kBlock, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(), L.r_colg.get(), LvVals<T>::r(L), k, src, Z, ctl)
   if constexpr (sizeof(T) == 4) {
-    if (v == 4) { D_(4); return; }
+    if (v == 4) { HSC_LPR(lpr, D_, 4); return; }
   }
-  if (v >= 2) { D_(2); } else { D_(1); }
+  if (v >= 2) { HSC_LPR(lpr, D_, 2); } else { HSC_LPR(lpr, D_, 1); }
 #undef D_
 }
 
@@ -234,12 +245,13 @@ static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
 #undef N_
     return;
   }
-  const dim3 g = hsc_grid(L.nF, k, v);
-#define U_(VV) kw_hsc_up<T, VV><<<g, kBlock, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L), L.p_ptr.get(), L.p_colg.get(), LvVals<T>::p(L), k, src, Z, ctl)
+  const int lpr = pack_lpr((k + v - 1) / v);
+  const dim3 g = hsc_grid(L.nF, k, v, lpr);
+#define U_(VV, LL) kw_hsc_up<T, VV, LL><<<g, kBlock, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L), L.p_ptr.get(), L.p_colg.get(), LvVals<T>::p(L), k, src, Z, ctl)
   if constexpr (sizeof(T) == 4) {
-    if (v == 4) { U_(4); return; }
+    if (v == 4) { HSC_LPR(lpr, U_, 4); return; }
   }
-  if (v >= 2) { U_(2); } else { U_(1); }
+  if (v >= 2) { HSC_LPR(lpr, U_, 2); } else { HSC_LPR(lpr, U_, 1); }
 #undef U_
 }
 

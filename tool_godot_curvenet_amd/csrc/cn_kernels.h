@@ -172,22 +172,60 @@ __device__ __forceinline__ void gather_narrow(int32_t a, int32_t b, int g,
   }
 }
 
+// Lane packing of the wide mapping: LPR lanes (16, 32 or 64) own one row's
+// column tile, so a wavefront walks 64 / LPR rows at once.  A moderate k
+// (k / VEC of 9..32 lanes) would otherwise leave most of a 64-lane tile
+// idle.  With LPR < 64 a row index is no longer wave-uniform and row data
+// is read with vector loads; LPR = 64 is the one-row-per-wave form.
+template <int LPR>
+struct Pack {
+  static_assert(LPR == 16 || LPR == 32 || LPR == 64, "lanes per row");
+  static constexpr int G = 64 / LPR;        // rows per wavefront
+  static constexpr int kSlots = kWaves * G;  // rows per block
+  // row slot of this thread: the wave's row groups are consecutive slots
+  static __device__ __forceinline__ int32_t slot() {
+    return (int32_t)((threadIdx.x >> 6) * G + (threadIdx.x & 63) / LPR);
+  }
+  static __device__ __forceinline__ int64_t col0(int vec) {
+    return (int64_t)blockIdx.y * LPR * vec + (int64_t)(threadIdx.x & (LPR - 1)) * vec;
+  }
+  // wave-uniform only when the wave holds a single row
+  static __device__ __forceinline__ int32_t u(int32_t x) {
+    if constexpr (LPR == 64) return uniform(x);
+    else return x;
+  }
+};
+
+// Lanes per row for a row's column work of `lanes` lanes: of 64, 32 and 16,
+// the width that leaves the fewest lanes of its column tiles idle (ties go
+// to the widest, whose fewer tiles re-read each row's index data less often).
+inline int pack_lpr(int64_t lanes) {
+  int best = 64;
+  int64_t idle = (lanes + 63) / 64 * 64 - lanes;
+  for (int l : {32, 16}) {
+    const int64_t w = (lanes + l - 1) / l * l - lanes;
+    if (w < idle) { best = l; idle = w; }
+  }
+  return best;
+}
+
 // Block reduction of per-lane FP64 partials for the wide mapping:
-// part[lane*VEC+m] of each wave summed over the 4 waves (fixed order).
+// part[(lane % LPR)*VEC+m] of each row group summed over the 4 waves and
+// the wave's 64 / LPR row groups (waves outer, groups inner: a fixed order).
 // `sh` holds kWaves * NP * 64 * VEC doubles of shared memory.
-template <int VEC, int NP>
+template <int VEC, int NP, int LPR = 64>
 __device__ __forceinline__ void block_partials_wide_in(double (&p)[NP][VEC],
                                                        double *__restrict__ out,
                                                        int64_t k, int64_t c0,
                                                        int64_t stride_np, double *sh) {
-  constexpr int TW = 64 * VEC;
+  constexpr int TW = 64 * VEC, G = 64 / LPR;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < NP; ++q)
 #pragma unroll
     for (int m = 0; m < VEC; ++m) sh[(w * NP + q) * TW + lane * VEC + m] = p[q][m];
   __syncthreads();
-  if (w == 0) {
+  if (w == 0 && lane < LPR) {
 #pragma unroll
     for (int q = 0; q < NP; ++q)
 #pragma unroll
@@ -195,21 +233,23 @@ __device__ __forceinline__ void block_partials_wide_in(double (&p)[NP][VEC],
         int64_t c = c0 + m;
         const int o = q * TW + lane * VEC + m;
         double s = sh[o];
-        s += sh[NP * TW + o];
-        s += sh[2 * NP * TW + o];
-        s += sh[3 * NP * TW + o];
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ++ww)
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+            if (ww + g > 0) s += sh[ww * NP * TW + g * LPR * VEC + o];
         if (c < k) out[q * stride_np + (int64_t)blockIdx.x * k + c] = s;
       }
   }
 }
 
-template <int VEC, int NP>
+template <int VEC, int NP, int LPR = 64>
 __device__ __forceinline__ void block_partials_wide(double (&p)[NP][VEC],
                                                     double *__restrict__ out,
                                                     int64_t k, int64_t c0,
                                                     int64_t stride_np) {
   __shared__ double sh[kWaves * NP * 64 * VEC];
-  block_partials_wide_in<VEC, NP>(p, out, k, c0, stride_np, sh);
+  block_partials_wide_in<VEC, NP, LPR>(p, out, k, c0, stride_np, sh);
 }
 
 // narrow: lane g < K of every 8-lane group holds the partials of column g;

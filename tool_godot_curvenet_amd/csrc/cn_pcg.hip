@@ -78,6 +78,13 @@ struct SegMap {
   const int lane = threadIdx.x & 63;                                          \
   const int64_t c0 = (int64_t)blockIdx.y * 64 * (VEC) + lane * (VEC);         \
   const bool valid = c0 < k;
+// packed wide mapping: LPR lanes per row, 64 / LPR rows per wavefront
+#define WIDE_LOOP_P(row, n0, n1, LPR)                                         \
+  const RowIter ri_ = row_iter((n0), (n1), Pack<LPR>::kSlots, Pack<LPR>::slot()); \
+  for (int32_t row = ri_.first; row < ri_.end; row += ri_.stride)
+#define WIDE_COLS_P(VEC, LPR)                                                 \
+  const int64_t c0 = Pack<LPR>::col0(VEC);                                    \
+  const bool valid = c0 < k;
 
 // ---- SpMM with partial p.q -------------------------------------------
 // Narrow form (k <= 4): one thread per row of the row table, its K columns
@@ -132,7 +139,7 @@ kn_spmm_dot(int32_t n, const int32_t *__restrict__ ecol, const T *__restrict__ e
 // row's kEll columns and weights (scalar loads) fetched while the current
 // row's gathers are in flight, so a row costs one memory round trip.  Slot 0
 // of every row is its own column, so the gather of P[r] also serves p.q.
-template <typename T, int VEC>
+template <typename T, int VEC, int LPR>
 __global__ void __launch_bounds__(kBlock)
 kw_spmm_dot(int32_t n, const int32_t *__restrict__ ecol, const T *__restrict__ eval,
             const int32_t *__restrict__ ovp, const int32_t *__restrict__ ovc,
@@ -140,11 +147,11 @@ kw_spmm_dot(int32_t n, const int32_t *__restrict__ ecol, const T *__restrict__ e
             T *__restrict__ Q, double *__restrict__ part,
             const int32_t *__restrict__ ctl, int32_t gens) {
   if (ctl_idle(ctl)) return;
-  WIDE_COLS(VEC)
+  WIDE_COLS_P(VEC, LPR)
   double pd[1][VEC] = {};
-  const RowIter it = row_iter_gen(0, n, kWaves, threadIdx.x >> 6, gens);
+  const RowIter it = row_iter_gen(0, n, Pack<LPR>::kSlots, Pack<LPR>::slot(), gens);
   const int32_t end = it.end, stride = uniform(it.stride);
-  int32_t r = uniform(it.first);
+  int32_t r = Pack<LPR>::u(it.first);
   int32_t j[kEll], oa = 0, ob = 0;
   T v[kEll];
   if (r < end) {
@@ -191,7 +198,7 @@ kw_spmm_dot(int32_t n, const int32_t *__restrict__ ecol, const T *__restrict__ e
     ob = obn;
     r = rn;
   }
-  block_partials_wide<VEC, 1>(pd, part, k, c0, 0);
+  block_partials_wide<VEC, 1, LPR>(pd, part, k, c0, 0);
 }
 
 // Row table of A_ff.  Row r lists its diagonal first (column r, weight 0
@@ -261,20 +268,20 @@ kn_resid(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__
   block_partials_narrow<K, 1>(pd, part, 0);
 }
 
-template <int VEC, typename TO>
+template <int VEC, typename TO, int LPR>
 __global__ void __launch_bounds__(kBlock)
 kw_resid(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
          const double *__restrict__ val, int64_t k, const double *__restrict__ X,
          const double *__restrict__ B, TO *__restrict__ R, const double *__restrict__ scale,
          double *__restrict__ part) {
-  WIDE_COLS(VEC)
+  WIDE_COLS_P(VEC, LPR)
   double pd[1][VEC] = {};
   double sc[VEC];
 #pragma unroll
   for (int m = 0; m < VEC; ++m) sc[m] = (scale && valid) ? scale[c0 + m] : 1.0;
-  WIDE_LOOP(row, 0, n) {
-    const int32_t r = uniform(row);
-    const int32_t a = uniform(ptr[r]), b = uniform(ptr[r + 1]);
+  WIDE_LOOP_P(row, 0, n, LPR) {
+    const int32_t r = Pack<LPR>::u(row);
+    const int32_t a = Pack<LPR>::u(ptr[r]), b = Pack<LPR>::u(ptr[r + 1]);
     double acc[VEC] = {};
     gather_wide<double, VEC>(a, b, col, val, X, k, c0, valid, acc);
     if (valid) {
@@ -290,7 +297,7 @@ kw_resid(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__
       VecIO<TO, VEC>::st(R + (int64_t)r * k + c0, out);
     }
   }
-  block_partials_wide<VEC, 1>(pd, part, k, c0, 0);
+  block_partials_wide<VEC, 1, LPR>(pd, part, k, c0, 0);
 }
 
 // ---- IC(0) forward pass over the blocks [b0, b1) of one level:
@@ -847,25 +854,25 @@ kn_rupd(int32_t n, const double *__restrict__ alpha, SegMap sm, const T *__restr
   block_partials_narrow<K, 1>(pd, part, 0);
 }
 
-template <typename T, int VEC, bool FUSED>
+template <typename T, int VEC, bool FUSED, int LPR>
 __global__ void __launch_bounds__(kBlock)
 kw_rupd(int32_t n, const double *__restrict__ alpha, SegMap sm, int64_t k,
         const T *__restrict__ Q, T *__restrict__ R, double *__restrict__ part,
         const int32_t *__restrict__ ctl) {
   if (ctl_idle(ctl)) return;
-  WIDE_COLS(VEC)
+  WIDE_COLS_P(VEC, LPR)
   double pd[1][VEC] = {};
   T al[VEC];
 #pragma unroll
   for (int m = 0; m < VEC; ++m) al[m] = (FUSED && valid) ? (T)alpha[c0 + m] : T(0);
-  WIDE_LOOP(row, 0, n) {
+  WIDE_LOOP_P(row, 0, n, LPR) {
     if (valid) {
       const int64_t o = (int64_t)row * k + c0;
       T rv[VEC];
       VecIO<T, VEC>::ld(R + o, rv);
       if (FUSED) {
         if (sm.of_row) {
-          const int64_t so = sm.vc(uniform(row), k, c0);
+          const int64_t so = sm.vc(Pack<LPR>::u(row), k, c0);
 #pragma unroll
           for (int m = 0; m < VEC; ++m) al[m] = (T)alpha[so + m];
         }
@@ -879,7 +886,7 @@ kw_rupd(int32_t n, const double *__restrict__ alpha, SegMap sm, int64_t k,
       for (int m = 0; m < VEC; ++m) pd[0][m] += (double)rv[m] * (double)rv[m];
     }
   }
-  block_partials_wide<VEC, 1>(pd, part, k, c0, 0);
+  block_partials_wide<VEC, 1, LPR>(pd, part, k, c0, 0);
 }
 
 // ---- Jacobi (SparseLinAlg.applyJacobi, zero diagonal -> 0): z = r / d,
@@ -910,18 +917,18 @@ kn_jac(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alpha,
   block_partials_narrow<K, 2>(pd, part, stride);
 }
 
-template <typename T, int VEC, bool FUSED>
+template <typename T, int VEC, bool FUSED, int LPR>
 __global__ void __launch_bounds__(kBlock)
 kw_jac(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alpha,
        int64_t k, const T *__restrict__ Q, T *__restrict__ R, T *__restrict__ Z,
        double *__restrict__ part, int64_t stride, const int32_t *__restrict__ ctl) {
   if (ctl_idle(ctl)) return;
-  WIDE_COLS(VEC)
+  WIDE_COLS_P(VEC, LPR)
   double pd[2][VEC] = {};
   T al[VEC];
 #pragma unroll
   for (int m = 0; m < VEC; ++m) al[m] = (FUSED && valid) ? (T)alpha[c0 + m] : T(0);
-  WIDE_LOOP(row, 0, n) {
+  WIDE_LOOP_P(row, 0, n, LPR) {
     if (valid) {
       const int64_t o = (int64_t)row * k + c0;
       T rv[VEC], zv[VEC];
@@ -943,7 +950,7 @@ kw_jac(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alpha,
       VecIO<T, VEC>::st(Z + o, zv);
     }
   }
-  block_partials_wide<VEC, 2>(pd, part, k, c0, stride);
+  block_partials_wide<VEC, 2, LPR>(pd, part, k, c0, stride);
 }
 
 // ---- generic two-vector column dot: partial a.b -------------------------
@@ -959,13 +966,13 @@ kn_dot(int32_t n, const T *__restrict__ A, const T *__restrict__ B,
   block_partials_narrow<K, 1>(pd, part, 0);
 }
 
-template <typename T, int VEC>
+template <typename T, int VEC, int LPR>
 __global__ void __launch_bounds__(kBlock)
 kw_dot(int32_t n, int64_t k, const T *__restrict__ A, const T *__restrict__ B,
        double *__restrict__ part) {
-  WIDE_COLS(VEC)
+  WIDE_COLS_P(VEC, LPR)
   double pd[1][VEC] = {};
-  WIDE_LOOP(row, 0, n) {
+  WIDE_LOOP_P(row, 0, n, LPR) {
     if (valid) {
       T a[VEC], b[VEC];
       VecIO<T, VEC>::ld(A + (int64_t)row * k + c0, a);
@@ -974,7 +981,7 @@ kw_dot(int32_t n, int64_t k, const T *__restrict__ A, const T *__restrict__ B,
       for (int m = 0; m < VEC; ++m) pd[0][m] += (double)a[m] * (double)b[m];
     }
   }
-  block_partials_wide<VEC, 1>(pd, part, k, c0, 0);
+  block_partials_wide<VEC, 1, LPR>(pd, part, k, c0, 0);
 }
 
 // ---- column reductions of the [rows][k] partial buffers (fixed order) ----
@@ -1351,6 +1358,7 @@ static unsigned blocks_for_rows(int64_t rows, int rows_per_block, unsigned cap) 
 struct Grid {
   dim3 g;
   int32_t rows;  // partial rows written (= g.x)
+  int lpr = 64;  // lanes per row of the wide mapping
 };
 
 template <typename T>
@@ -1375,16 +1383,23 @@ static unsigned resident_blocks() {
   return cached;
 }
 
+// lanes per row of the packed wide mapping
+template <typename T>
+static int wide_lpr(int64_t k) {
+  return pack_lpr((k + wide_vec<T>(k) - 1) / wide_vec<T>(k));
+}
+
 template <typename T>
 static Grid grid_rows(int64_t rows, int64_t k, unsigned cap) {
   Grid G;
   if (k <= 4) {
     G.g = dim3(blocks_for_rows(rows, kBlock / kGroup, cap), 1, 1);
   } else {
-    int v = wide_vec<T>(k);
-    const unsigned tiles = (unsigned)((k + 64 * v - 1) / (64 * v));
+    const int v = wide_vec<T>(k), l = wide_lpr<T>(k);
+    const unsigned tiles = (unsigned)((k + (int64_t)l * v - 1) / ((int64_t)l * v));
     const unsigned fit = std::max(8u, resident_blocks() / tiles);
-    G.g = dim3(blocks_for_rows(rows, kWaves, std::min(cap, fit)), tiles, 1);
+    G.g = dim3(blocks_for_rows(rows, kWaves * (64 / l), std::min(cap, fit)), tiles, 1);
+    G.lpr = l;
   }
   G.rows = (int32_t)G.g.x;
   return G;
@@ -1412,8 +1427,15 @@ static Grid grid_vec(int64_t rows, int64_t k, int v, unsigned cap) {
 static const unsigned kCapSpmm = 1024;
 static const unsigned kCapLevel = 512;
 
-// dispatch: NARROW(K) for k <= 4, WIDE(V) with V the vector width
-#define DISPATCH_K(k, T, NARROW, WIDE)                                        \
+// dispatch: NARROW(K) for k <= 4, WIDE(V, L) with V the vector width and
+// L the lanes per row
+#define WIDE_LPR(lpr, WIDE, V)                                                \
+  do {                                                                        \
+    if ((lpr) == 16) { WIDE(V, 16); }                                         \
+    else if ((lpr) == 32) { WIDE(V, 32); }                                    \
+    else { WIDE(V, 64); }                                                     \
+  } while (0)
+#define DISPATCH_K(k, T, lpr, NARROW, WIDE)                                   \
   do {                                                                        \
     if ((k) <= 4) {                                                           \
       switch (k) {                                                            \
@@ -1424,9 +1446,9 @@ static const unsigned kCapLevel = 512;
       }                                                                       \
     } else {                                                                  \
       int v_ = wide_vec<T>(k);                                                \
-      if (v_ == 4) { WIDE(4); }                                               \
-      else if (v_ == 2) { WIDE(2); }                                          \
-      else { WIDE(1); }                                                       \
+      if (v_ == 4) { WIDE_LPR(lpr, WIDE, 4); }                                \
+      else if (v_ == 2) { WIDE_LPR(lpr, WIDE, 2); }                           \
+      else { WIDE_LPR(lpr, WIDE, 1); }                                        \
     }                                                                         \
   } while (0)
 
@@ -1486,8 +1508,8 @@ static int32_t launch_spmm_dot(const SysView<T> &V, int64_t k, const T *P, T *Q,
   const int32_t gens = (int32_t)(G.g.x / std::max(1u, grid_rows<T>(V.n, k, kCapSpmm).g.x));
   ProfScope ps("spmm", s);
 #define N_(KK) kn_spmm_dot<T, KK><<<G.g, kBlock, 0, s>>>(V.n, V.ecol, V.eval, V.ovp, V.ovc, V.ovv, P, Q, part, ctl)
-#define W_(VV) kw_spmm_dot<T, WideV<T, VV>::v><<<G.g, kBlock, 0, s>>>(V.n, V.ecol, V.eval, V.ovp, V.ovc, V.ovv, k, P, Q, part, ctl, gens)
-  DISPATCH_K(k, T, N_, W_);
+#define W_(VV, LL) kw_spmm_dot<T, WideV<T, VV>::v, LL><<<G.g, kBlock, 0, s>>>(V.n, V.ecol, V.eval, V.ovp, V.ovc, V.ovv, k, P, Q, part, ctl, gens)
+  DISPATCH_K(k, T, G.lpr, N_, W_);
 #undef N_
 #undef W_
   return G.rows;
@@ -1501,8 +1523,8 @@ static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
   Grid G = grid_rows<double>(n, k, kCapSpmm);
   ProfScope ps("resid", s);
 #define N_(KK) kn_resid<KK, TO><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, X, B, R, scale, part)
-#define W_(VV) kw_resid<WideV<double, VV>::v, TO><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, k, X, B, R, scale, part)
-  DISPATCH_K(k, double, N_, W_);
+#define W_(VV, LL) kw_resid<WideV<double, VV>::v, TO, LL><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, k, X, B, R, scale, part)
+  DISPATCH_K(k, double, G.lpr, N_, W_);
 #undef N_
 #undef W_
   return G.rows;
@@ -1623,8 +1645,8 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
     Grid G = grid_rows<T>(V.n, k, kCapSpmm);
     ProfScope ps("fwd", s);
 #define N_(KK) kn_rupd<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(V.n, alpha, V.sm, Q, R, part, ctl)
-#define W_(VV) kw_rupd<T, WideV<T, VV>::v, FUSED><<<G.g, kBlock, 0, s>>>(V.n, alpha, V.sm, k, Q, R, part, ctl)
-    DISPATCH_K(k, T, N_, W_);
+#define W_(VV, LL) kw_rupd<T, WideV<T, VV>::v, FUSED, LL><<<G.g, kBlock, 0, s>>>(V.n, alpha, V.sm, k, Q, R, part, ctl)
+    DISPATCH_K(k, T, G.lpr, N_, W_);
 #undef N_
 #undef W_
     if (V.hsc) {
@@ -1644,8 +1666,8 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
     Grid G = grid_rows<T>(V.n, k, kCapSpmm);
     ProfScope ps("fwd", s);
 #define N_(KK) kn_jac<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(V.n, V.dinv, alpha, Q, R, Z, part, stride, ctl)
-#define W_(VV) kw_jac<T, WideV<T, VV>::v, FUSED><<<G.g, kBlock, 0, s>>>(V.n, V.dinv, alpha, k, Q, R, Z, part, stride, ctl)
-    DISPATCH_K(k, T, N_, W_);
+#define W_(VV, LL) kw_jac<T, WideV<T, VV>::v, FUSED, LL><<<G.g, kBlock, 0, s>>>(V.n, V.dinv, alpha, k, Q, R, Z, part, stride, ctl)
+    DISPATCH_K(k, T, G.lpr, N_, W_);
 #undef N_
 #undef W_
     return G.rows;
@@ -1748,8 +1770,8 @@ static int32_t launch_dot(int32_t n, int64_t k, const T *A, const T *B, double *
                           hipStream_t s) {
   Grid G = grid_rows<T>(n, k, kCapSpmm);
 #define N_(KK) kn_dot<T, KK><<<G.g, kBlock, 0, s>>>(n, A, B, part)
-#define W_(VV) kw_dot<T, WideV<T, VV>::v><<<G.g, kBlock, 0, s>>>(n, k, A, B, part)
-  DISPATCH_K(k, T, N_, W_);
+#define W_(VV, LL) kw_dot<T, WideV<T, VV>::v, LL><<<G.g, kBlock, 0, s>>>(n, k, A, B, part)
+  DISPATCH_K(k, T, G.lpr, N_, W_);
 #undef N_
 #undef W_
   return G.rows;
