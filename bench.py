@@ -109,11 +109,13 @@ def main():
     ap.add_argument("--plain-poses", type=int, default=2,
                     help="poses solved by the plain FP64 one-solve-per-launch path")
     ap.add_argument("--workload", default="capsules100k",
-                    choices=["capsules100k", "meshes64", "icosphere9"],
+                    choices=["capsules100k", "meshes64", "icosphere9", "ddm"],
                     help="capsules100k: 256 poses x 3 RHS on the 100k capsule union (the "
                          "headline); meshes64: 64 uneven meshes x 3 RHS, batched vs one solve "
                          "per mesh; icosphere9: 16 poses x 3 RHS forward + adjoint on the "
-                         "2.6M-vertex icosphere with 64 loops")
+                         "2.6M-vertex icosphere with 64 loops; ddm: Direct Delta Mush bind "
+                         "(harvest of every handle, smoothing, top-8) and the FP32 skinning "
+                         "matvec over 256 poses on the capsule union")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="no GPU: turn two rocprofv3 --pmc counter files (FETCH_SIZE, "
                          "WRITE_SIZE) of a --copy-mb run into the traffic file")
@@ -141,6 +143,9 @@ def main():
         return
     if args.workload == "icosphere9":
         icosphere9_bench(args)
+        return
+    if args.workload == "ddm":
+        ddm_bench(args)
         return
 
     if args.copy_mb > 0:
@@ -603,6 +608,151 @@ def icosphere9_bench(args):
         "solve": {"forward_iters": [f for f, _ in its], "adjoint_iters": [a for _, a in its],
                   "relres_forward": rf, "relres_adjoint": ra,
                   "setup_s": time.time() - t0 - el},
+    }
+    print(json.dumps(line))
+
+
+def ddm_bench(args):
+    """Direct Delta Mush on the 100k capsule union: the bind (unit-perturbation
+    harvest of every handle with the HSC-preconditioned multi-RHS PCG, two
+    smoothing passes, top-8 sparsification), then the runtime skinning matvec
+    (FP32, 8 influences a vertex) over 256 poses of per-handle transforms --
+    one step is one matvec launch over all poses."""
+    import numpy as np
+    import torch
+
+    import oracle as O
+    import tool_godot_curvenet_amd as cn
+    from tool_godot_curvenet_amd import ddm, meshgen
+
+    assert torch.cuda.is_available(), "no GPU"
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    mesh = meshgen.config_mesh("capsules100k")
+    lap = cn.CutLaplacian.from_mesh(mesh)
+    hsc = cn.HSC(lap, theta=args.theta)
+    nv, H, K, P = lap.info.nv, lap.n_con, 8, args.poses
+    print(f"ddm: system and hierarchy at {time.time() - t0:.1f} s (nv {nv}, handles {H})",
+          file=sys.stderr, flush=True)
+
+    def bind_once():
+        ts = [time.perf_counter()]
+        W, infos = ddm.bind_harvest(lap, hsc=hsc, block=768, precision=args.precision,
+                                    tol=args.tol)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter())
+        ap, aj = ddm.vertex_adjacency(nv, lap.tris)
+        Ws = ddm.smooth_weights(W, ap, aj, 2, 0.5)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter())
+        del W
+        idx, w = ddm.sparsify_topk(Ws, K)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter())
+        pou = float((Ws.sum(dim=1) - 1.0).abs().max().item())
+        return idx, w, infos, np.diff(ts), pou
+
+    bind_once()                                         # warm-up (allocations, code load)
+    idx, w, infos, tb, pou = bind_once()
+    print(f"ddm: bind {tb.sum():.3f} s (harvest {tb[0]:.3f}, smooth {tb[1]:.3f}, "
+          f"top-K {tb[2]:.3f})", file=sys.stderr, flush=True)
+
+    # runtime: per-handle rigid transforms for P poses (FP32)
+    rng = np.random.default_rng(meshgen.BASE_SEED + 60)
+    ang = rng.uniform(-0.3, 0.3, size=(P, H, 3))
+    T = np.zeros((P, H, 4, 4))
+    for a in range(3):
+        c, s_ = np.cos(ang[..., a]), np.sin(ang[..., a])
+        R = np.zeros((P, H, 3, 3))
+        i, j = [(1, 2), (0, 2), (0, 1)][a]
+        R[..., a, a] = 1.0
+        R[..., i, i], R[..., j, j], R[..., i, j], R[..., j, i] = c, c, -s_, s_
+        T[..., :3, :3] = R if a == 0 else T[..., :3, :3] @ R
+    T[..., :3, 3] = rng.normal(0, 0.05, size=(P, H, 3))
+    T[..., 3, 3] = 1.0
+    Tg = torch.as_tensor(T.reshape(P, H, 16), dtype=torch.float32, device=dev)
+    rest = torch.as_tensor(lap.pos, dtype=torch.float32, device=dev)
+    start = torch.arange(0, (nv + 1) * K, K, dtype=torch.int32, device=dev)
+    idxg = idx.reshape(-1).contiguous()
+    wg = w.reshape(-1).to(torch.float32).contiguous()
+    out = torch.empty((P, nv, 3), dtype=torch.float32, device=dev)
+    # distinct handles of each 256-vertex tile (the kernel stages them in LDS)
+    ih = idxg.cpu().numpy().reshape(nv, K)
+    tile_nu = np.array([len(np.unique(ih[i:i + 256])) for i in range(0, nv, 256)])
+    from tool_godot_curvenet_amd import _lib
+    from tool_godot_curvenet_amd.curvenet import _p, _stream
+    lib = _lib.load()
+
+    def step():
+        _lib.check(lib.cn_lbs_matvec_f32(nv, H, P, _p(rest), _p(Tg), _p(start), _p(idxg),
+                                         _p(wg), _p(out), _stream()), "lbs_matvec_f32")
+
+    for _ in range(args.warmup):
+        step()
+    steps = args.steps * 20                             # one launch is ~0.1 ms
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t1 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    ms = e0.elapsed_time(e1) / steps
+    # algorithmic bytes of one launch: rest, row pointers, influences, output,
+    # and every pose's transform table once
+    alg = nv * (12 + 4 + 8 * K) + P * nv * 12 + P * H * 64
+    # check one pose against the oracle (FP64 on the FP32 inputs)
+    p = P // 3
+    ref = O.lbs_matvec(lap.pos.astype(np.float32).astype(np.float64),
+                       T.reshape(P, H, 16)[p].astype(np.float32).astype(np.float64),
+                       start.cpu().numpy(), idxg.cpu().numpy(),
+                       wg.cpu().numpy().astype(np.float64))
+    err = float(np.abs(out[p].cpu().numpy() - ref).max())
+    cpu = None
+    if not args.no_cpu:
+        cp = 32
+        tc = time.perf_counter()
+        for q in range(cp):
+            O.lbs_matvec(lap.pos, T.reshape(P, H, 16)[q], start.cpu().numpy(),
+                         idxg.cpu().numpy(), w.reshape(-1).cpu().numpy())
+        tcs = time.perf_counter() - tc
+        cpu = {"value": cp * nv / tcs, "unit": "vertex-poses/s", "cores": 1, "kind": "port",
+               "sample": f"oracle lbs_matvec (FP64, one thread) over {cp} poses of the "
+                         f"same influences and transforms ({tcs:.2f} s)",
+               "cpu": _cpu_model()}
+    line = {
+        "metric": "vertex-poses/s, DDM skinning matvec (100k capsule union, 8 influences)",
+        "value": P * nv / (ms * 1e-3),
+        "unit": "vertex-poses/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (capsule union, harvested weights, seeded per-handle rigid "
+                "transforms)",
+        "config": {"workload": f"capsule union {nv} verts, {H} handles, top-{K} influences, "
+                               f"{P} poses; bind: harvest (HSC PCG, {args.precision}, tol "
+                               f"{args.tol}, blocks of 768 handles), 2 smoothing passes "
+                               f"(omega 0.5), top-{K}"},
+        "roofline": {"bound": "hbm", "achieved": alg / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_lbs<float>", "avg_ms": ms,
+                     "bytes_per_launch": alg},
+        "cpu_baseline": cpu,
+        "bind": {"seconds": float(tb.sum()), "harvest_s": float(tb[0]),
+                 "smooth_s": float(tb[1]), "topk_s": float(tb[2]),
+                 "handles_per_s": H / float(tb[0]),
+                 "iters": [i.iters for i in infos], "partition_of_unity_err": pou},
+        "check": {"pose": p, "max_abs_err_vs_oracle": err},
+        "tiles": {"count": int(len(tile_nu)), "handles_mean": float(tile_nu.mean()),
+                  "handles_max": int(tile_nu.max())},
+        "wall_s": el,
     }
     print(json.dumps(line))
 

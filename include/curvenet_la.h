@@ -372,6 +372,54 @@ cn_status cn_deformation_gradients(int32_t n, const double *d_rest, const double
                                    double *d_F, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Direct Delta Mush.
+ *
+ * Runtime: DirectDeltaMush.lbsMatvec (DirectDeltaMush.lean:85-98) over every
+ * vertex, in the buffers of the SlangCodegen/DirectDeltaMush.lean:33-87
+ * shader: d_rest nv x 3, d_T P x nt x 16 (4x4 row-major, rows 0-2 used),
+ * influences CSR d_start[nv + 1], d_idx, d_w; d_out P x nv x 3.  A handle
+ * index outside [0, nt) reads the zero matrix.  The f64 form keeps the
+ * spec's operation order (bit-exact); the f32 form is the shader's float
+ * arithmetic with fused multiply-adds.
+ * ---------------------------------------------------------------------- */
+cn_status cn_lbs_matvec_f64(int32_t nv, int32_t nt, int32_t P, const double *d_rest,
+                            const double *d_T, const int32_t *d_start, const int32_t *d_idx,
+                            const double *d_w, double *d_out, void *stream);
+cn_status cn_lbs_matvec_f32(int32_t nv, int32_t nt, int32_t P, const float *d_rest,
+                            const float *d_T, const int32_t *d_start, const int32_t *d_idx,
+                            const float *d_w, float *d_out, void *stream);
+
+/* DirectDeltaMushBind.smoothWeights (DirectDeltaMushBind.lean:76-97): nu
+ * passes over the n x nh row-major d_W, in place, with d_tmp (same size,
+ * distinct) as the other buffer.  Adjacency CSR d_adj_ptr[n + 1], d_adj in
+ * the caller's neighbour order; a neighbour outside [0, n) adds 0.
+ * Bit-exact against the spec. */
+cn_status cn_smooth_weights_f64(int32_t n, int32_t nh, const int32_t *d_adj_ptr,
+                                const int32_t *d_adj, double *d_W, double *d_tmp, int32_t nu,
+                                double omega, void *stream);
+
+/* DirectDeltaMushBind.sparsifyTopK (DirectDeltaMushBind.lean:107-136):
+ * take = min(K, nh) (index, weight) pairs per row into d_idx / d_w
+ * (n x take), in the spec's selection order, renormalised by their sum when
+ * it is nonzero.  take <= 64. */
+cn_status cn_sparsify_topk_f64(int32_t n, int32_t nh, const double *d_W, int32_t K,
+                               int32_t *d_idx, double *d_w, void *stream);
+
+/* Bind harvest (DirectDeltaMushBind.lean:9-12; the sparse multi-RHS form of
+ * Features/KnownUnknowns/SparseBindHarvest.lean:3-5).  Handle h is
+ * constrained column h; its weights are the harmonic solution for the unit
+ * constrained vector e_h.
+ *   cn_bind_rhs: d_rhs n_free x nb, column c = -A_fc e_{h0+c}, equal to
+ *     cn_harmonic_rhs of the unit block;
+ *   cn_bind_weights: writes columns [h0, h0 + nb) of the nv x ldw vertex
+ *     weight matrix d_W: free vertices take their row of the solution d_x
+ *     (n_free x nb), sample vertices 1 on their own column, else 0.
+ * Single systems only (not batches). */
+cn_status cn_bind_rhs(cn_system sys, int32_t h0, int32_t nb, double *d_rhs, void *stream);
+cn_status cn_bind_weights(cn_system sys, int32_t h0, int32_t nb, const double *d_x,
+                          double *d_W, int64_t ldw, void *stream);
+
+/* ------------------------------------------------------------------------
  * Colour-parallel symmetric Gauss-Seidel (SlangCodegen/SgsColor.lean:
  * 87-130) on A_ff.  Rows are grouped by GraphColoring.greedyColor
  * (GraphColoring.lean:41-56); one sweep runs the SgsColor row update over

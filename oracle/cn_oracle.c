@@ -1618,3 +1618,102 @@ int32_t orc_hsc_eliminate_vertex(int32_t ne, const int32_t *eu,
   free(nb); free(nw);
   return m;
 }
+
+/* ======================================================================
+ * Direct Delta Mush: runtime matvec and bind-time weight post-processing
+ * ==================================================================== */
+
+/* DirectDeltaMush.applyTransform4x4 (DirectDeltaMush.lean:69-73): rows 0-2
+ * of T (x, y, z, 1), each summed left to right. */
+void orc_apply_transform4x4(const double *T, double x, double y, double z, double *out) {
+  for (int r = 0; r < 3; ++r) out[r] = T[4 * r] * x + T[4 * r + 1] * y + T[4 * r + 2] * z + T[4 * r + 3];
+}
+
+/* DirectDeltaMush.lbsMatvec (DirectDeltaMush.lean:85-98), one vertex per CSR
+ * row of influences, accumulated from 0 in list order.  A handle index past
+ * the transform array reads the default (empty) matrix, whose entries are 0. */
+void orc_lbs_matvec(int32_t nv, const double *rest, const double *T, int32_t nt,
+                    const int32_t *start, const int32_t *idx, const double *w, double *out) {
+  static const double zero16[16] = {0};
+  for (int32_t v = 0; v < nv; ++v) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int32_t q = start[v]; q < start[v + 1]; ++q) {
+      const double *M = (idx[q] >= 0 && idx[q] < nt) ? T + 16 * (int64_t)idx[q] : zero16;
+      double t[3];
+      orc_apply_transform4x4(M, rest[3 * v], rest[3 * v + 1], rest[3 * v + 2], t);
+      for (int d = 0; d < 3; ++d) acc[d] = acc[d] + w[q] * t[d];
+    }
+    for (int d = 0; d < 3; ++d) out[3 * v + d] = acc[d];
+  }
+}
+
+/* DirectDeltaMushBind.smoothWeights (DirectDeltaMushBind.lean:76-97): nu
+ * passes of w <- (1 - omega) w + omega * (sum of neighbour weights / count),
+ * the neighbour sum from 0 in adjacency order (0 for a vertex without
+ * neighbours; a neighbour index outside [0, n) reads 0). */
+void orc_smooth_weights(int32_t n, int32_t nh, const int32_t *adj_ptr, const int32_t *adj,
+                        const double *W, int32_t nu, double omega, double *out) {
+  const size_t sz = (size_t)n * (size_t)nh;
+  double *cur = (double *)malloc(sizeof(double) * (sz ? sz : 1));
+  double *nxt = (double *)malloc(sizeof(double) * (sz ? sz : 1));
+  memcpy(cur, W, sizeof(double) * sz);
+  for (int32_t pass = 0; pass < nu; ++pass) {
+    for (int32_t v = 0; v < n; ++v) {
+      const int32_t a = adj_ptr[v], b = adj_ptr[v + 1], count = b - a;
+      for (int32_t i = 0; i < nh; ++i) {
+        double nb = 0.0;
+        if (count > 0) {
+          for (int32_t q = a; q < b; ++q) {
+            const int32_t u = adj[q];
+            nb = nb + ((u >= 0 && u < n) ? cur[(size_t)u * nh + i] : 0.0);
+          }
+          nb = nb / (double)count;
+        }
+        nxt[(size_t)v * nh + i] = (1.0 - omega) * cur[(size_t)v * nh + i] + omega * nb;
+      }
+    }
+    double *t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  memcpy(out, cur, sizeof(double) * sz);
+  free(cur);
+  free(nxt);
+}
+
+/* DirectDeltaMushBind.sparsifyTopK (DirectDeltaMushBind.lean:107-136):
+ * selection sort of the row's (index, weight) pairs for the first
+ * take = min(K, nh) positions -- the first position of the largest |w|
+ * (strict >) is swapped into place -- then the kept weights are divided by
+ * their sum (accumulated from 0 in kept order) when that sum is not 0. */
+void orc_sparsify_topk(int32_t n, int32_t nh, const double *W, int32_t K, int32_t *idx,
+                       double *w) {
+  const int32_t take = K < nh ? K : nh;
+  int32_t *pi = (int32_t *)malloc(sizeof(int32_t) * (nh ? nh : 1));
+  double *pw = (double *)malloc(sizeof(double) * (nh ? nh : 1));
+  for (int32_t v = 0; v < n; ++v) {
+    for (int32_t i = 0; i < nh; ++i) {
+      pi[i] = i;
+      pw[i] = W[(size_t)v * nh + i];
+    }
+    for (int32_t k = 0; k < take; ++k) {
+      int32_t best = k;
+      for (int32_t j = k + 1; j < nh; ++j)
+        if (fabs(pw[j]) > fabs(pw[best])) best = j;
+      const int32_t ti = pi[k];
+      const double tw = pw[k];
+      pi[k] = pi[best];
+      pw[k] = pw[best];
+      pi[best] = ti;
+      pw[best] = tw;
+    }
+    double sum = 0.0;
+    for (int32_t k = 0; k < take; ++k) sum = sum + pw[k];
+    for (int32_t k = 0; k < take; ++k) {
+      idx[(size_t)v * take + k] = pi[k];
+      w[(size_t)v * take + k] = sum != 0.0 ? pw[k] / sum : pw[k];
+    }
+  }
+  free(pi);
+  free(pw);
+}

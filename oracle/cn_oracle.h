@@ -248,6 +248,21 @@ int32_t orc_hsc_eliminate_vertex(int32_t ne, const int32_t *eu,
                                  int32_t v, int32_t *ou, int32_t *ov,
                                  double *ow);
 
+/* ---- Direct Delta Mush (DirectDeltaMush.lean, DirectDeltaMushBind.lean) --
+ * Transforms are 4x4 row-major (16 doubles each).  Influences are CSR:
+ * start[nv+1], idx[], w[].  Weight matrices are dense row-major n x nh. */
+void orc_apply_transform4x4(const double *T, double x, double y, double z,
+                            double *out);                          /* :69-73 */
+void orc_lbs_matvec(int32_t nv, const double *rest, const double *T, int32_t nt,
+                    const int32_t *start, const int32_t *idx, const double *w,
+                    double *out);                                  /* :85-98 */
+void orc_smooth_weights(int32_t n, int32_t nh, const int32_t *adj_ptr,
+                        const int32_t *adj, const double *W, int32_t nu, double omega,
+                        double *out);                              /* Bind :76-97 */
+/* take = min(K, nh) pairs per row: idx[n*take], w[n*take] */
+void orc_sparsify_topk(int32_t n, int32_t nh, const double *W, int32_t K, int32_t *idx,
+                       double *w);                                 /* Bind :107-136 */
+
 #ifdef __cplusplus
 }
 #endif

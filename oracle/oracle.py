@@ -60,6 +60,16 @@ PINS = {
     "dense solve/LU/Cholesky": "DenseLinAlg.lean:294-489 KATs, numpy.linalg",
     "hsc_eliminate_vertex": "HSC KATs (HierarchicalSparsifyCompensate.lean:134-231), "
                             "dense Schur complement",
+    "apply_transform4x4/lbs_matvec": "DirectDeltaMush.lean:115-165 and "
+        "Features/KnownKnowns/DirectDeltaMush.lean KATs (identity, translation, blends, empty "
+        "influences); rotation about an axis; linear in the weights; planted mistakes fail",
+    "smooth_weights": "DirectDeltaMushBind.lean:169-197 KATs (nu = 0 identity, half weights "
+                      "fixed, partition of unity); dense matrix form "
+                      "((1-omega) I + omega D^-1 A)^nu on random graphs; planted mistakes fail",
+    "sparsify_topk": "DirectDeltaMushBind.lean:199-262 KATs (top-2 of [0.4 .. 0.1] = 4/7, 3/7; "
+                     "K = width; smooth-then-sparsify partition of unity); a literal Python "
+                     "selection sort with swaps on tie-heavy rows; top-K set == numpy "
+                     "argsort on distinct values",
     "adjoint": "central finite differences (h = 1e-6) of a quadratic loss through a dense "
                "FP64 solve, in x_c and in every edge weight; planted sign / transpose "
                "mistakes fail them",
@@ -278,6 +288,49 @@ def cut_laplacian_csr(he, constrained, pos):
     return {"n_free": nfree, "free_of": free_of, "con_of": con_of,
             "A_ff": (ffp[:nfree + 1].copy(), ffc[:a].copy(), ffv[:a].copy()),
             "A_fc": (fcp[:nfree + 1].copy(), fcc[:b].copy(), fcv[:b].copy())}
+
+
+def apply_transform4x4(T, x, y, z):
+    """DirectDeltaMush.applyTransform4x4 (DirectDeltaMush.lean:69-73); T is 16
+    row-major values."""
+    out = np.zeros(3)
+    lib().orc_apply_transform4x4(_p(_d(T)), C.c_double(x), C.c_double(y), C.c_double(z),
+                                 _p(out))
+    return out
+
+
+def lbs_matvec(rest, T, start, idx, w):
+    """DirectDeltaMush.lbsMatvec (DirectDeltaMush.lean:85-98) for every vertex:
+    rest nv x 3, T nt x 16, influences CSR (start, idx, w)."""
+    rest = _d(rest).reshape(-1, 3)
+    T = _d(T).reshape(-1, 16)
+    out = np.zeros_like(rest)
+    lib().orc_lbs_matvec(len(rest), _p(rest), _p(T), len(T), _p(_i(start)), _p(_i(idx)),
+                         _p(_d(w)), _p(out))
+    return out
+
+
+def smooth_weights(W, adj_ptr, adj, nu, omega):
+    """DirectDeltaMushBind.smoothWeights (DirectDeltaMushBind.lean:76-97);
+    adjacency as CSR (adj_ptr, adj), neighbour order kept."""
+    W = _d(W)
+    n, nh = W.shape
+    out = np.zeros_like(W)
+    lib().orc_smooth_weights(n, nh, _p(_i(adj_ptr)), _p(_i(adj)), _p(W), int(nu),
+                             C.c_double(omega), _p(out))
+    return out
+
+
+def sparsify_topk(W, K):
+    """DirectDeltaMushBind.sparsifyTopK (DirectDeltaMushBind.lean:107-136):
+    (idx, w), each n x min(K, nh)."""
+    W = _d(W)
+    n, nh = W.shape
+    take = min(K, nh)
+    idx = np.zeros((n, take), np.int32)
+    w = np.zeros((n, take))
+    lib().orc_sparsify_topk(n, nh, _p(W), int(K), _p(idx), _p(w))
+    return idx, w
 
 
 def bezier3(P, t):

@@ -152,6 +152,12 @@ _SIGS = {
     "cn_bezier3": (i32, [i32, vp, i32, vp, vp, vp]),
     "cn_segment_gradients": (i32, [i32, vp, vp, vp, vp, vp, vp]),
     "cn_deformation_gradients": (i32, [i32, vp, vp, vp, vp]),
+    "cn_lbs_matvec_f64": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]),
+    "cn_lbs_matvec_f32": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]),
+    "cn_smooth_weights_f64": (i32, [i32, i32, vp, vp, vp, vp, i32, f64, vp]),
+    "cn_sparsify_topk_f64": (i32, [i32, i32, vp, i32, vp, vp, vp]),
+    "cn_bind_rhs": (i32, [vp, i32, i32, vp, vp]),
+    "cn_bind_weights": (i32, [vp, i32, i32, vp, vp, i64, vp]),
     "cn_profile_enable": (None, [i32]),
     "cn_profile_query": (i32, [C.c_char_p, C.POINTER(f64)]),
 }

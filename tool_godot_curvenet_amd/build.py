@@ -26,7 +26,7 @@ CSRC = HERE / "csrc"
 LIB = HERE / "libcurvenet_la.so"
 OBJ = HERE / "_build"
 
-EXACT = {"cn_topology.hip", "cn_assembly.hip", "cn_ic0.hip", "cn_spmv.hip", "cn_hsc_build.hip", "cn_sgs.hip", "cn_deform.hip"}
+EXACT = {"cn_topology.hip", "cn_assembly.hip", "cn_ic0.hip", "cn_spmv.hip", "cn_hsc_build.hip", "cn_sgs.hip", "cn_deform.hip", "cn_ddm.hip"}
 
 
 def _hipcc() -> str:
