@@ -657,6 +657,32 @@ def ddm_bench(args):
     print(f"ddm: bind {tb.sum():.3f} s (harvest {tb[0]:.3f}, smooth {tb[1]:.3f}, "
           f"top-K {tb[2]:.3f})", file=sys.stderr, flush=True)
 
+    # the bind-cache sidecar: hierarchy + influences written once, read back
+    # by a later session instead of rebuilding
+    import tempfile
+    from tool_godot_curvenet_amd import formats
+    th = time.perf_counter()
+    hsc2 = cn.HSC(lap, theta=args.theta)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - th
+    start0 = torch.arange(0, (nv + 1) * K, K, dtype=torch.int32, device=dev)
+    with tempfile.TemporaryDirectory() as td:
+        side = os.path.join(td, "bind.safetensors")
+        tw = time.perf_counter()
+        formats.save_sidecar(side, lap, hsc=hsc2, influences=(start0, idx.reshape(-1), w.reshape(-1)))
+        t_write = time.perf_counter() - tw
+        side_bytes = os.path.getsize(side)
+        tl = time.perf_counter()
+        hsc3 = formats.load_hsc(side, lap)
+        formats.load_influences(side, lap)
+        torch.cuda.synchronize()
+        t_load = time.perf_counter() - tl
+    r_chk = torch.randn(lap.n_free, 3, dtype=torch.float64, device=dev)
+    same = bool(np.array_equal(hsc2.apply(r_chk), hsc3.apply(r_chk)))
+    del hsc2, hsc3
+    print(f"ddm: sidecar {side_bytes / 1e6:.1f} MB, write {t_write:.3f} s, load {t_load:.3f} s, "
+          f"HSC build {t_build:.3f} s", file=sys.stderr, flush=True)
+
     # runtime: per-handle rigid transforms for P poses (FP32)
     rng = np.random.default_rng(meshgen.BASE_SEED + 60)
     ang = rng.uniform(-0.3, 0.3, size=(P, H, 3))
@@ -750,6 +776,8 @@ def ddm_bench(args):
                  "handles_per_s": H / float(tb[0]),
                  "iters": [i.iters for i in infos], "partition_of_unity_err": pou},
         "check": {"pose": p, "max_abs_err_vs_oracle": err},
+        "sidecar": {"bytes": side_bytes, "write_s": t_write, "load_s": t_load,
+                    "hsc_build_s": t_build, "apply_identical": same},
         "tiles": {"count": int(len(tile_nu)), "handles_mean": float(tile_nu.mean()),
                   "handles_max": int(tile_nu.max())},
         "wall_s": el,

@@ -325,6 +325,18 @@ typedef struct cn_hsc_level {
   const double *d_p_val;
 } cn_hsc_level;
 
+/* One level of a persisted hierarchy (host pointers, the layout of
+ * cn_hsc_level; r_col has r_ptr[nC] entries, p_col p_ptr[nF]). */
+typedef struct cn_hsc_level_in {
+  int32_t n, nF, nC;
+  const int32_t *F, *C;
+  const double *dinv;
+  const int32_t *r_ptr, *r_col;
+  const double *r_val;
+  const int32_t *p_ptr, *p_col;
+  const double *p_val;
+} cn_hsc_level_in;
+
 void cn_hsc_default_opts(cn_hsc_opts *opts);
 cn_status cn_hsc_build(cn_system sys, const cn_hsc_opts *opts, cn_hsc *out,
                        void *stream);
@@ -334,6 +346,19 @@ cn_status cn_hsc_get_level(cn_hsc h, int32_t level, cn_hsc_level *out);
 cn_status cn_hsc_apply(cn_hsc h, int32_t k, const double *d_r, double *d_z,
                        void *stream);
 cn_status cn_hsc_destroy(cn_hsc h);
+/* A hierarchy of `sys` from persisted arrays (a sidecar written from
+ * cn_hsc_get_level / cn_hsc_get_info): no elimination and no coarse
+ * factorization run, so applies are bit-identical to the saved hierarchy's.
+ * Checks on the host that level 0 has n_free rows, every level's F and C
+ * split its rows in ascending order, R / P are CSR of that level, and
+ * n_coarse is what the levels leave; CN_BAD_ARG otherwise.  Whether the
+ * arrays belong to this system's values is the caller's to check (the
+ * sidecar stores a fingerprint).  This is the persisted-cache form of
+ * Features/KnownUnknowns/BindCachePersistence.lean:3-5. */
+cn_status cn_hsc_import(cn_system sys, const cn_hsc_opts *opts, int32_t nlevels,
+                        const cn_hsc_level_in *levels, int32_t n_coarse,
+                        const double *coarse_L, const double *coarse_inv, cn_hsc *out,
+                        void *stream);
 
 /* ------------------------------------------------------------------------
  * The deformation solve (DeformSolve.solveDeformation, DeformSolve.lean:
@@ -555,6 +580,48 @@ cn_status cn_adjoint_solve_hsc(cn_system sys, cn_hsc hsc, const cn_pcg_opts *opt
                                int32_t k, const double *d_g, const double *d_x,
                                const double *d_xc, double *d_lambda, double *d_gc,
                                double *d_gw, cn_solve_info *info, void *stream);
+
+/* ------------------------------------------------------------------------
+ * safetensors container (host code; LeanSafetensors Header.lean,
+ * Reader.lean, Writer.lean).  Used for mesh ingest and for the hierarchy
+ * sidecar.  dtype codes follow Header.lean:36-41: F32 F64 F16 BF16 U8 U16
+ * U32 U64 I8 I16 I32 I64 BOOL = 0..12.
+ *   cn_st_parse / cn_st_read_file: SafeTensors.parse + validate
+ *     (Reader.lean:33-61).  Tensors and metadata come back in key order (the
+ *     reference decodes JSON objects into key-sorted maps; a repeated key
+ *     keeps its last value); non-string metadata values as their JSON text.
+ *   cn_st_find: SafeTensors.get (:64-69), index or -1;
+ *   cn_st_metadata_get: metadataAt (:72-74), NULL when absent;
+ *   cn_st_encode / cn_st_write_file: Writer.layout + encode
+ *     (Writer.lean:20-58), tensors contiguous in the given order,
+ *     "__metadata__" first, header space-padded to an 8-byte boundary.
+ *     cn_st_encode with buf = NULL returns the size only.
+ * A cn_st owns its bytes; pointers it hands out live until cn_st_free.
+ * ---------------------------------------------------------------------- */
+typedef struct cn_st_t *cn_st;
+typedef struct cn_st_tensor {
+  const char *name;
+  int32_t dtype;
+  int32_t ndim;
+  const int64_t *shape;
+  const void *data;
+  int64_t nbytes;
+} cn_st_tensor;
+
+cn_status cn_st_parse(const void *bytes, int64_t size, cn_st *out);
+cn_status cn_st_read_file(const char *path, cn_st *out);
+int32_t cn_st_count(cn_st st);
+cn_status cn_st_tensor_at(cn_st st, int32_t i, cn_st_tensor *out);
+int32_t cn_st_find(cn_st st, const char *name);
+int32_t cn_st_metadata_count(cn_st st);
+cn_status cn_st_metadata_at(cn_st st, int32_t i, const char **key, const char **value);
+const char *cn_st_metadata_get(cn_st st, const char *key);
+void cn_st_free(cn_st st);
+cn_status cn_st_encode(int32_t n, const cn_st_tensor *tensors, int32_t nmeta,
+                       const char *const *keys, const char *const *values, void *buf,
+                       int64_t cap, int64_t *size);
+cn_status cn_st_write_file(const char *path, int32_t n, const cn_st_tensor *tensors,
+                           int32_t nmeta, const char *const *keys, const char *const *values);
 
 /* Per-kernel timing of the last cn_pcg_solve on this thread, gathered with
  * HIP events around every launch of the named kernel when enabled.

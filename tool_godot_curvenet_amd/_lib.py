@@ -81,6 +81,17 @@ class HscLevel(C.Structure):
                 ("d_p_col", vp), ("d_p_val", vp)]
 
 
+class HscLevelIn(C.Structure):
+    _fields_ = [("n", i32), ("nF", i32), ("nC", i32), ("F", vp), ("C", vp), ("dinv", vp),
+                ("r_ptr", vp), ("r_col", vp), ("r_val", vp), ("p_ptr", vp), ("p_col", vp),
+                ("p_val", vp)]
+
+
+class StTensor(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("dtype", i32), ("ndim", i32),
+                ("shape", C.POINTER(i64)), ("data", vp), ("nbytes", i64)]
+
+
 class SgsInfo(C.Structure):
     _fields_ = [("n", i32), ("ncolors", i32), ("h_color_ptr", P_i32), ("d_color_rows", vp),
                 ("d_color", vp)]
@@ -152,6 +163,21 @@ _SIGS = {
     "cn_bezier3": (i32, [i32, vp, i32, vp, vp, vp]),
     "cn_segment_gradients": (i32, [i32, vp, vp, vp, vp, vp, vp]),
     "cn_deformation_gradients": (i32, [i32, vp, vp, vp, vp]),
+    "cn_hsc_import": (i32, [vp, C.POINTER(HscOpts), i32, C.POINTER(HscLevelIn), i32, vp, vp,
+                            C.POINTER(vp), vp]),
+    "cn_st_parse": (i32, [vp, i64, C.POINTER(vp)]),
+    "cn_st_read_file": (i32, [C.c_char_p, C.POINTER(vp)]),
+    "cn_st_count": (i32, [vp]),
+    "cn_st_tensor_at": (i32, [vp, i32, C.POINTER(StTensor)]),
+    "cn_st_find": (i32, [vp, C.c_char_p]),
+    "cn_st_metadata_count": (i32, [vp]),
+    "cn_st_metadata_at": (i32, [vp, i32, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p)]),
+    "cn_st_metadata_get": (C.c_char_p, [vp, C.c_char_p]),
+    "cn_st_free": (None, [vp]),
+    "cn_st_encode": (i32, [i32, C.POINTER(StTensor), i32, C.POINTER(C.c_char_p),
+                           C.POINTER(C.c_char_p), vp, i64, C.POINTER(i64)]),
+    "cn_st_write_file": (i32, [C.c_char_p, i32, C.POINTER(StTensor), i32,
+                               C.POINTER(C.c_char_p), C.POINTER(C.c_char_p)]),
     "cn_lbs_matvec_f64": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]),
     "cn_lbs_matvec_f32": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]),
     "cn_smooth_weights_f64": (i32, [i32, i32, vp, vp, vp, vp, i32, f64, vp]),

@@ -1,7 +1,8 @@
 """Build the gfx950 shared library ``libcurvenet_la.so`` in-tree.
 
-Every ``csrc/*.hip`` file is compiled with hipcc for gfx950 only and linked
-into one shared object that exports the C ABI of ``include/curvenet_la.h``.
+Every ``csrc/*.hip`` file is compiled with hipcc for gfx950 only (host-only
+``csrc/*.cpp`` files with g++) and linked into one shared object that exports
+the C ABI of ``include/curvenet_la.h``.
 
 Flags:
   * ``-Xclang -target-feature -Xclang -packed-fp32-ops`` keeps the compiler
@@ -54,7 +55,11 @@ def _flags(name: str) -> list[str]:
 
 def _compile(src: pathlib.Path) -> pathlib.Path:
     out = OBJ / (src.stem + ".o")
-    cmd = [_hipcc(), *_flags(src.name), "-c", str(src), "-o", str(out)]
+    if src.suffix == ".cpp":  # host-only code
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{HERE.parent / 'include'}",
+               "-c", str(src), "-o", str(out)]
+    else:
+        cmd = [_hipcc(), *_flags(src.name), "-c", str(src), "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr}")
@@ -64,7 +69,7 @@ def _compile(src: pathlib.Path) -> pathlib.Path:
 
 def build(verbose: bool = False) -> pathlib.Path:
     OBJ.mkdir(exist_ok=True)
-    srcs = sorted(CSRC.glob("*.hip"))
+    srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")), 8)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, srcs))

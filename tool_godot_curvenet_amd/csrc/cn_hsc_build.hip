@@ -334,6 +334,135 @@ cn_status cn_hsc_build(cn_system S, const cn_hsc_opts *opts, cn_hsc *out, void *
   CN_API_END
 }
 
+// A hierarchy from its persisted arrays (cn_hsc_import): the same device
+// layout as cn_hsc_build gives, the coarse factor and inverse taken as
+// stored, every index checked on the host first.
+cn_status cn_hsc_import(cn_system S, const cn_hsc_opts *opts, int32_t nlevels,
+                        const cn_hsc_level_in *levels, int32_t n_coarse,
+                        const double *coarse_L, const double *coarse_inv, cn_hsc *out,
+                        void *stream) {
+  CN_API_BEGIN
+  if (!S || !out || nlevels < 0 || (nlevels > 0 && !levels) || n_coarse < 0 ||
+      (n_coarse > 0 && (!coarse_L || !coarse_inv))) {
+    set_error("cn_hsc_import: bad arguments");
+    return CN_BAD_ARG;
+  }
+  // structure: level 0 has n_free rows, level l + 1 the C rows of level l;
+  // F and C split [0, n) in ascending order; R columns are F rows, P
+  // columns level-(l + 1) rows; CSR pointers start at 0 and never decrease
+  int32_t n = S->n_free;
+  for (int32_t l = 0; l < nlevels; ++l) {
+    const cn_hsc_level_in &L = levels[l];
+    if (L.n != n || L.nF < 0 || L.nC < 0 || (int64_t)L.nF + L.nC != n ||
+        (L.nF > 0 && (!L.F || !L.dinv || !L.p_ptr)) || (L.nC > 0 && !L.C) || !L.r_ptr ||
+        !L.p_ptr) {
+      set_error("cn_hsc_import: level %d has inconsistent sizes", l);
+      return CN_BAD_ARG;
+    }
+    std::vector<int8_t> kind((size_t)n, 0);  // 1 = F, 2 = C
+    for (int32_t f = 0; f < L.nF; ++f) {
+      const int32_t v = L.F[f];
+      if (v < 0 || v >= n || kind[v] || (f > 0 && v <= L.F[f - 1])) {
+        set_error("cn_hsc_import: level %d: F is not an ascending set of rows", l);
+        return CN_BAD_ARG;
+      }
+      kind[v] = 1;
+    }
+    for (int32_t c = 0; c < L.nC; ++c) {
+      const int32_t v = L.C[c];
+      if (v < 0 || v >= n || kind[v] || (c > 0 && v <= L.C[c - 1])) {
+        set_error("cn_hsc_import: level %d: C is not an ascending set of rows outside F", l);
+        return CN_BAD_ARG;
+      }
+      kind[v] = 2;
+    }
+    auto check_csr = [&](const int32_t *ptr, int32_t rows, const int32_t *col, int32_t lim,
+                         bool f_cols) {
+      if (ptr[0] != 0) return false;
+      for (int32_t r = 0; r < rows; ++r)
+        if (ptr[r + 1] < ptr[r]) return false;
+      if (ptr[rows] > 0 && !col) return false;
+      for (int32_t e = 0; e < ptr[rows]; ++e) {
+        if (col[e] < 0 || col[e] >= lim) return false;
+        if (f_cols && kind[col[e]] != 1) return false;
+      }
+      return true;
+    };
+    if (!check_csr(L.r_ptr, L.nC, L.r_col, n, true) ||
+        !check_csr(L.p_ptr, L.nF, L.p_col, L.nC, false) ||
+        (L.r_ptr[L.nC] > 0 && !L.r_val) || (L.p_ptr[L.nF] > 0 && !L.p_val)) {
+      set_error("cn_hsc_import: level %d: R or P is not a valid CSR of this level", l);
+      return CN_BAD_ARG;
+    }
+    n = L.nC;
+  }
+  if (n != n_coarse) {
+    set_error("cn_hsc_import: %d coarse rows given, the levels leave %d", n_coarse, n);
+    return CN_BAD_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  cn_hsc_t *H = new cn_hsc_t();
+  try {
+    H->n0 = S->n_free;
+    cn_hsc_default_opts(&H->opts);
+    if (opts) H->opts = *opts;
+    std::vector<int32_t> g((size_t)S->n_free);
+    for (int32_t i = 0; i < S->n_free; ++i) g[i] = i;
+    for (int32_t l = 0; l < nlevels; ++l) {
+      const cn_hsc_level_in &in = levels[l];
+      H->lv.emplace_back();
+      cn_hsc_t::Level &L = H->lv.back();
+      L.n = in.n;
+      L.nF = in.nF;
+      L.nC = in.nC;
+      const int32_t nr = in.r_ptr[in.nC], np = in.p_ptr[in.nF];
+      std::vector<int32_t> F(in.F, in.F + in.nF), Cv(in.C, in.C + in.nC);
+      std::vector<int32_t> rp(in.r_ptr, in.r_ptr + in.nC + 1), rc(in.r_col, in.r_col + nr);
+      std::vector<int32_t> pp(in.p_ptr, in.p_ptr + in.nF + 1), pc(in.p_col, in.p_col + np);
+      std::vector<double> dv(in.dinv, in.dinv + in.nF), rv(in.r_val, in.r_val + nr),
+          pv(in.p_val, in.p_val + np);
+      std::vector<int32_t> Fg(in.nF), Cg(in.nC), rg(nr), pg(np);
+      for (int32_t f = 0; f < in.nF; ++f) Fg[f] = g[F[f]];
+      for (int32_t c = 0; c < in.nC; ++c) Cg[c] = g[Cv[c]];
+      for (int32_t e = 0; e < nr; ++e) rg[e] = g[rc[e]];
+      for (int32_t e = 0; e < np; ++e) pg[e] = Cg[pc[e]];
+      upload(L.F, F, s);
+      upload(L.C, Cv, s);
+      upload(L.Fg, Fg, s);
+      upload(L.Cg, Cg, s);
+      upload(L.r_colg, rg, s);
+      upload(L.p_colg, pg, s);
+      upload(L.dinv, dv, s);
+      upload(L.dinv32, to_f32(dv), s);
+      upload(L.r_ptr, rp, s);
+      upload(L.r_col, rc, s);
+      upload(L.r_val, rv, s);
+      upload(L.r_val32, to_f32(rv), s);
+      upload(L.p_ptr, pp, s);
+      upload(L.p_col, pc, s);
+      upload(L.p_val, pv, s);
+      upload(L.p_val32, to_f32(pv), s);
+      H->nnz += (int64_t)nr + np;
+      CN_HIP(hipStreamSynchronize(s));
+      g = std::move(Cg);
+    }
+    H->nc = n_coarse;
+    upload(H->coarse_rows, g, s);
+    if (n_coarse > 0) {
+      const size_t nn = (size_t)n_coarse * n_coarse;
+      upload(H->coarse_L, std::vector<double>(coarse_L, coarse_L + nn), s);
+      upload(H->coarse_inv, std::vector<double>(coarse_inv, coarse_inv + nn), s);
+    }
+    CN_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    delete H;
+    throw;
+  }
+  *out = H;
+  return CN_OK;
+  CN_API_END
+}
+
 cn_status cn_hsc_get_info(cn_hsc H, cn_hsc_info *info) {
   if (!H || !info) return CN_BAD_ARG;
   info->nlevels = (int32_t)H->lv.size();

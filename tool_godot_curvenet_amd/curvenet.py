@@ -488,10 +488,39 @@ class HSC:
         o.compensate = {"lump": L.CN_HSC_LUMP, "diag": L.CN_HSC_DIAG}[compensate]
         h = C.c_void_p(0)
         L.check(lib.cn_hsc_build(system.handle, C.byref(o), C.byref(h), _stream()), "hsc_build")
+        self._adopt(h, o)
+
+    def _adopt(self, h, o):
         self._h = h
+        self.opts = o
         info = L.HscInfo()
-        L.check(lib.cn_hsc_get_info(h, C.byref(info)), "hsc_get_info")
+        L.check(L.load().cn_hsc_get_info(h, C.byref(info)), "hsc_get_info")
         self.info = info
+
+    @classmethod
+    def from_import(cls, system: "_System", opts, levels, nlevels: int, n_coarse: int,
+                    coarse_L: np.ndarray, coarse_inv: np.ndarray) -> "HSC":
+        """A hierarchy rebuilt from persisted host arrays (cn_hsc_import)."""
+        self = cls.__new__(cls)
+        self.system = system
+        h = C.c_void_p(0)
+        L.check(L.load().cn_hsc_import(system.handle, C.byref(opts), nlevels, levels, n_coarse,
+                                       coarse_L.ctypes.data if n_coarse else None,
+                                       coarse_inv.ctypes.data if n_coarse else None,
+                                       C.byref(h), _stream()), "hsc_import")
+        self._adopt(h, opts)
+        return self
+
+    def save(self, path) -> None:
+        """Write the hierarchy to a safetensors sidecar (formats.save_sidecar)."""
+        from . import formats
+        formats.save_sidecar(path, self.system, hsc=self)
+
+    @staticmethod
+    def load(system: "_System", path) -> "HSC":
+        """The hierarchy of a sidecar written for `system` (formats.load_hsc)."""
+        from . import formats
+        return formats.load_hsc(path, system)
 
     def __del__(self):
         h = getattr(self, "_h", None)
