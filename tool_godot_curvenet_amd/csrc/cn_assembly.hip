@@ -15,6 +15,7 @@
 #include <climits>
 
 #include "cn_internal.h"
+#include "cn_kernels.h"
 
 #pragma clang fp contract(off)
 
@@ -247,28 +248,130 @@ __global__ void k_diag(int32_t n, const int32_t *ptr, const int32_t *col,
   d[i] = x;
 }
 
-// RHS of HarmonicSolve.solveMulti on the free rows (spec summation order).
-__global__ void k_harmonic_rhs(int32_t n_free, int32_t k,
-                               const int32_t *free_vertex, const int32_t *tris,
-                               const int32_t *iptr, const int32_t *ihe,
-                               const double *m, const int32_t *con_of,
-                               const double *Xc, double *rhs) {
-  int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= (int64_t)n_free * k) return;
-  int32_t r = (int32_t)(id / k), c = (int32_t)(id - (int64_t)r * k);
-  int32_t v = free_vertex[r];
-  double s = 0.0;
-  for (int32_t a = iptr[v]; a < iptr[v + 1]; ++a) {
-    int32_t p = ihe[a], f = p / 3, lp = p - 3 * f;
-    const double *mf = m + 9 * (int64_t)f + 3 * lp;
-    double t = 0.0;
-    for (int lq = 0; lq < 3; ++lq) {
-      int32_t cc = con_of[tgt3(tris, 3 * f + lq)];
-      if (cc >= 0) t = t + (0.0 + mf[lq]) * Xc[(int64_t)cc * k + c];
+// RHS of HarmonicSolve.solveMulti on the free rows (spec summation order):
+// for free row r of vertex v, s = sum over v's incoming interior halfedges
+// p (ascending) of t_p, t_p = sum over the three halfedges q of p's face of
+// (0 + Lh[p][q]) Xc[col of q] (constrained targets only); rhs = -s.
+// Computed over the right-hand side table: one row's column tile per
+// LPR lanes, the row's groups and entries read once for all of its columns.
+// s = sum over groups of (sum over entries of (0 + m) x), from 0 in order;
+// the groups and entries the table leaves out would add exact zeros.
+template <int VEC, int LPR>
+__global__ void __launch_bounds__(kBlock)
+k_harmonic_rhs_w(int32_t n_free, int64_t k, const int32_t *__restrict__ gptr,
+                 const int32_t *__restrict__ eptr, const int32_t *__restrict__ cc,
+                 const int32_t *__restrict__ midx, const double *__restrict__ m,
+                 const double *__restrict__ Xc, double *__restrict__ rhs) {
+  using P = Pack<LPR>;
+  const int64_t c0 = P::col0(VEC);
+  const bool valid = c0 < k;
+  const int64_t cl = valid ? c0 : k - VEC;
+  const RowIter ri = row_iter(0, n_free, P::kSlots, P::slot());
+  for (int32_t r0 = ri.first; r0 < ri.end; r0 += ri.stride) {
+    const int32_t r = P::u(r0);
+    double acc[VEC];
+#pragma unroll
+    for (int q = 0; q < VEC; ++q) acc[q] = 0.0;
+    const int32_t g1 = P::u(gptr[r + 1]);
+    for (int32_t g = P::u(gptr[r]); g < g1; ++g) {
+      double t[VEC];
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) t[q] = 0.0;
+      const int32_t e1 = P::u(eptr[g + 1]);
+      for (int32_t e = P::u(eptr[g]); e < e1; ++e) {
+        const double w = 0.0 + m[midx[e]];
+        double x[VEC];
+        VecIO<double, VEC>::ld(Xc + (int64_t)cc[e] * k + cl, x);
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) t[q] = t[q] + w * x[q];
+      }
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) acc[q] = acc[q] + t[q];
     }
-    s = s + t;
+    if (valid) {
+      double o[VEC];
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) o[q] = -acc[q];
+      VecIO<double, VEC>::st(rhs + (int64_t)r * k + c0, o);
+    }
   }
-  rhs[id] = -s;
+}
+
+// the right-hand side table of a system (host build from the incidence
+// lists, once per system: the pattern does not depend on positions)
+static void ensure_rhs_table(cn_system_t *S, hipStream_t s) {
+  if (S->rhs_ready) return;
+  const int32_t n = S->n_free;
+  std::vector<int32_t> fv(n), tris((size_t)3 * S->nf), iptr((size_t)S->nv + 1), con(S->nv);
+  CN_HIP(hipMemcpyAsync(fv.data(), S->free_vertex.get(), sizeof(int32_t) * n,
+                        hipMemcpyDeviceToHost, s));
+  CN_HIP(hipMemcpyAsync(tris.data(), S->tris.get(), sizeof(int32_t) * tris.size(),
+                        hipMemcpyDeviceToHost, s));
+  CN_HIP(hipMemcpyAsync(iptr.data(), S->inc.ptr.get(), sizeof(int32_t) * iptr.size(),
+                        hipMemcpyDeviceToHost, s));
+  CN_HIP(hipMemcpyAsync(con.data(), S->con_of.get(), sizeof(int32_t) * S->nv,
+                        hipMemcpyDeviceToHost, s));
+  CN_HIP(hipStreamSynchronize(s));
+  std::vector<int32_t> ihe(iptr[S->nv]);
+  if (!ihe.empty())
+    CN_HIP(hipMemcpy(ihe.data(), S->inc.he.get(), sizeof(int32_t) * ihe.size(),
+                     hipMemcpyDeviceToHost));
+  std::vector<int32_t> gptr(1, 0), eptr(1, 0), cc, midx;
+  gptr.reserve((size_t)n + 1);
+  for (int32_t r = 0; r < n; ++r) {
+    const int32_t v = fv[r];
+    for (int32_t a = iptr[v]; a < iptr[v + 1]; ++a) {
+      const int32_t p = ihe[a], f = p / 3, lp = p - 3 * f;
+      const size_t before = cc.size();
+      for (int lq = 0; lq < 3; ++lq) {
+        const int32_t h = 3 * f + lq, l = h - 3 * f;
+        const int32_t c = con[tris[3 * (size_t)f + (l == 2 ? 0 : l + 1)]];
+        if (c >= 0) {
+          cc.push_back(c);
+          midx.push_back(9 * f + 3 * lp + lq);
+        }
+      }
+      if (cc.size() > before) eptr.push_back((int32_t)cc.size());
+    }
+    gptr.push_back((int32_t)eptr.size() - 1);
+  }
+  auto up = [&](DevBuf<int32_t> &d, const std::vector<int32_t> &h) {
+    d.alloc(std::max<size_t>(h.size(), 1));
+    if (!h.empty())
+      CN_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(int32_t) * h.size(),
+                            hipMemcpyHostToDevice, s));
+  };
+  up(S->rhs_gptr, gptr);
+  up(S->rhs_eptr, eptr);
+  up(S->rhs_cc, cc);
+  up(S->rhs_midx, midx);
+  CN_HIP(hipStreamSynchronize(s));
+  S->rhs_ready = true;
+}
+
+static void launch_harmonic_rhs(cn_system_t *S, int32_t k, const double *Xc, double *rhs,
+                                hipStream_t s) {
+  const int v = (k % 2 == 0) ? 2 : 1;
+  const int lpr = pack_lpr((k + v - 1) / v);
+  const int slots = kWaves * (64 / lpr);
+  int64_t g = ((int64_t)S->n_free + slots - 1) / slots;
+  g = std::max<int64_t>(1, std::min<int64_t>(g, 2048));
+  if (g >= 64) g -= g % 8;
+  const dim3 grid((unsigned)g, (unsigned)((k + (int64_t)lpr * v - 1) / ((int64_t)lpr * v)), 1);
+#define H_(VV, LL)                                                                       \
+  k_harmonic_rhs_w<VV, LL><<<grid, kBlock, 0, s>>>(S->n_free, k, S->rhs_gptr.get(),    \
+                                                    S->rhs_eptr.get(), S->rhs_cc.get(),  \
+                                                    S->rhs_midx.get(), S->face_m.get(),  \
+                                                    Xc, rhs)
+#define H_L(VV)                                                                          \
+  do {                                                                                   \
+    if (lpr == 16) H_(VV, 16);                                                           \
+    else if (lpr == 32) H_(VV, 32);                                                      \
+    else H_(VV, 64);                                                                     \
+  } while (0)
+  if (v == 2) H_L(2); else H_L(1);
+#undef H_L
+#undef H_
 }
 
 static void assemble_values(cn_system_t *S, hipStream_t s) {
@@ -542,10 +645,10 @@ cn_status cn_harmonic_rhs(cn_system S, int32_t k, const double *d_Xc,
   }
   hipStream_t s = as_stream(stream);
   int64_t tot = (int64_t)S->n_free * k;
-  if (tot > 0)
-    k_harmonic_rhs<<<grid_for(tot, 256), 256, 0, s>>>(
-        S->n_free, k, S->free_vertex.get(), S->tris.get(), S->inc.ptr.get(),
-        S->inc.he.get(), S->face_m.get(), S->con_of.get(), d_Xc, d_rhs);
+  if (tot > 0) {
+    ensure_rhs_table(S, s);
+    launch_harmonic_rhs(S, k, d_Xc, d_rhs, s);
+  }
   CN_HIP(hipGetLastError());
   return CN_OK;
   CN_API_END

@@ -151,6 +151,13 @@ struct cn_system_t {
   // free row, so the values are read from fc_val and never go stale
   cn::DevBuf<int32_t> cf_ptr, cf_row, cf_map;
   bool cf_ready = false;
+  // harmonic right-hand side table, built on first use: for every free row
+  // the groups (incoming interior halfedges, ascending) that touch a
+  // constrained vertex, each with its (face_m index, constrained column)
+  // entries in face order -- the spec's summation order without the terms
+  // that are structurally zero
+  cn::DevBuf<int32_t> rhs_gptr, rhs_eptr, rhs_cc, rhs_midx;
+  bool rhs_ready = false;
   // A_ff as a fixed-width row table for the solver's multi-column SpMM:
   // the first kEll entries of every row (short rows padded with their last
   // column and weight 0), the rest in an overflow CSR.  The pattern is built

@@ -1,6 +1,10 @@
 // cn_prof.h -- per-kernel timing of a solve with HIP events (enabled by
 // cn_profile_enable): a ProfScope records an event pair on the stream around
-// the launches it encloses; cn_profile_query sums them by name.
+// the launches it encloses; cn_profile_query sums them by name.  The events
+// are timing-only (read after the solve's final stream synchronization), so
+// they are created without the system-scope fence: a fenced event writes
+// back and invalidates the caches at every record, which slows the kernels
+// after it and so the solve being measured.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -32,7 +36,7 @@ struct ProfScope {
       size_t add = std::max<size_t>(256, P.pool.size());
       for (size_t i = 0; i < add; ++i) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return;
         P.pool.push_back(e);
       }
     }
