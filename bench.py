@@ -207,7 +207,7 @@ def main():
 
     lib.cn_profile_enable(1)
     prof = {name: [0, 0.0] for name in ("spmm", "fwd", "bwd", "resid", "hsc_down", "hsc_up",
-                                         "hsc_coarse")}
+                                         "hsc_coarse", "hsc_tail")}
     iters, refines, relmax = 0, 0, 0.0
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -264,32 +264,51 @@ def main():
         # map); up reads and writes the eliminated rows and reads, once, the
         # kept ones: 4k(2 n_F + n_C) plus P, the row map and 1/a_ff.
         # k_pd_update (d += alpha p, p = z + beta p) streams 5 n x k vectors.
-        down_b = up_b = 0
-        for lv in pre.levels():
+        # levels [l0, L) and the coarse solve run in the fused tail launch
+        l0 = lib.cn_hsc_tail_level(pre.handle, k, 1 if args.precision == "mixed" else 0)
+        down_b = up_b = tail_b = 0
+        for li, lv in enumerate(pre.levels()):
             nF, nC = len(lv["F"]), len(lv["C"])
-            down_b += 4 * k * (2 * nC + nF) + 8 * len(lv["R"][1]) + 4 * (nC + 1) + 4 * nC
-            up_b += 4 * k * (2 * nF + nC) + 8 * len(lv["P"][1]) + 4 * (nF + 1) + 8 * nF
+            db = 4 * k * (2 * nC + nF) + 8 * len(lv["R"][1]) + 4 * (nC + 1) + 4 * nC
+            ub = 4 * k * (2 * nF + nC) + 8 * len(lv["P"][1]) + 4 * (nF + 1) + 8 * nF
+            if 0 <= l0 <= li:
+                tail_b += db + ub
+            else:
+                down_b += db
+                up_b += ub
         # Bytes are counted for the applies known to do work: one per active
         # iteration and the first of each solve (a restart adds an uncounted
         # one; launches during a pause return at once).  Time covers every
         # launch, so the achieved rates are lower bounds.
-        nlev = max(pre.info.nlevels, 1)
+        nlev = max(pre.info.nlevels if l0 < 0 else l0, 1)
         napply = prof["hsc_down"][0]
         nwork = iters + args.steps
         kern["hsc_down"] = ("kw_hsc_down<float,4,64>", napply * nlev, prof["hsc_down"][1],
                             nwork * down_b)
         kern["hsc_up"] = ("kw_hsc_up<float,4,64>", napply * nlev, prof["hsc_up"][1],
                           nwork * up_b)
+        if l0 >= 0:
+            # the tail: its levels' down and up bytes, the coarse rows read and
+            # written, and the FP64 coarse inverse once per apply
+            ncr = pre.info.n_coarse
+            tail_b += 8 * k * ncr + 8 * ncr * ncr
+            kern["hsc_tail"] = ("k_hsc_tail<float,4>", prof["hsc_tail"][0], prof["hsc_tail"][1],
+                                nwork * tail_b)
         kern["pd_update"] = ("k_pd_update<float>", prof["bwd"][0], prof["bwd"][1],
                              nwork * 5 * nk4)
         nc = pre.info.n_coarse
         cms = prof["hsc_coarse"][1]
         ncoarse = prof["hsc_coarse"][0]
         tflops = 2.0 * nc * nc * k * nwork / (cms * 1e-3) / 1e12 if cms > 0 else 0.0
-        coarse = {"kernel": "k_gemm_f64 (coarse inverse x rows, FP64 MFMA)", "n_coarse": nc,
-                  "launches": ncoarse, "avg_ms": cms / max(ncoarse, 1),
-                  "achieved_tflops": tflops, "peak_tflops": FP64_MFMA_PEAK_TFLOPS,
-                  "frac": tflops / FP64_MFMA_PEAK_TFLOPS}
+        if l0 >= 0:
+            coarse = {"kernel": "coarse solve inside k_hsc_tail (FP64 on the vector units, "
+                                "one workgroup per column slice)", "n_coarse": nc,
+                      "tail_from_level": int(l0), "levels": int(pre.info.nlevels)}
+        else:
+            coarse = {"kernel": "k_gemm_f64 (coarse inverse x rows, FP64 MFMA)",
+                      "n_coarse": nc, "launches": ncoarse, "avg_ms": cms / max(ncoarse, 1),
+                      "achieved_tflops": tflops, "peak_tflops": FP64_MFMA_PEAK_TFLOPS,
+                      "frac": tflops / FP64_MFMA_PEAK_TFLOPS}
     kernels = {}
     for tag, (kname, launches, ms, nbytes) in kern.items():
         gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0

@@ -342,6 +342,11 @@ cn_status cn_hsc_build(cn_system sys, const cn_hsc_opts *opts, cn_hsc *out,
                        void *stream);
 cn_status cn_hsc_get_info(cn_hsc h, cn_hsc_info *info);
 cn_status cn_hsc_get_level(cn_hsc h, int32_t level, cn_hsc_level *out);
+/* Levels an apply of k columns runs in the fused tail kernel: the first
+ * such level (levels [level, nlevels) and the coarse solve run in one
+ * launch, one workgroup per column slice), or -1 when every level runs its
+ * own kernels.  fp32 = 1 for the mixed-precision inner loop, 0 for FP64. */
+int32_t cn_hsc_tail_level(cn_hsc h, int32_t k, int32_t fp32);
 /* z = M^-1 r (FP64), r and z n_free x k, distinct. */
 cn_status cn_hsc_apply(cn_hsc h, int32_t k, const double *d_r, double *d_z,
                        void *stream);

@@ -147,3 +147,45 @@ def test_hsc_beats_ic0_on_capsules(gpu):
     _, ii = lap.solve(B, precond="ic0", precision="mixed", tol=1e-9, ic0=IC0(lap))
     assert ih.status == 0 and ii.status == 0
     assert ih.iters * 5 < ii.iters, (ih.iters, ii.iters)
+
+
+# Applies with many columns run the small levels and the coarse solve in one
+# fused launch (one workgroup per column slice): at least 128 slices, i.e.
+# k >= 256 in FP64 and k >= 512 in FP32.  On cylinder64 every level is small
+# (the whole apply is the fused launch); on icosphere6 the top levels keep
+# their own kernels.
+@pytest.mark.parametrize("name", ["cylinder64", "icosphere6"])
+@pytest.mark.parametrize("k", [256, 301])
+def test_hsc_apply_fused_tail(gpu, name, k):
+    from tool_godot_curvenet_amd import HSC
+    lap = _lap(name)
+    A = lap.A_ff()
+    H = HSC(lap)
+    assert H.tail_level(k, fp32=False) >= 0
+    ho = O.hsc_build(A.row_ptr, A.col_idx, A.values)
+    r = np.random.default_rng(k).normal(size=(lap.n_free, k))
+    zg = H.apply(r)
+    zo = O.hsc_apply(ho, r)
+    assert np.linalg.norm(zg - zo) <= 1e-11 * np.linalg.norm(zo)
+
+
+@pytest.mark.parametrize("name", ["cylinder64", "icosphere6"])
+@pytest.mark.parametrize("precision,tol", [("mixed", 2e-5), ("fp64", 1e-11)])
+def test_pcg_fused_tail_direction_and_convergence(gpu, name, precision, tol):
+    from tool_godot_curvenet_amd import HSC
+    from tool_godot_curvenet_amd import meshgen as G
+    k = 512
+    lap = _lap(name)
+    A = lap.A_ff()
+    H = HSC(lap)
+    assert H.tail_level(k, fp32=precision == "mixed") >= 0
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=700))
+    x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, max_iter=1, hsc=H)
+    X = x.cpu().numpy()
+    Z = O.hsc_apply(O.hsc_build(A.row_ptr, A.col_idx, A.values), B)
+    e = np.linalg.norm(X / np.linalg.norm(X, axis=0) - Z / np.linalg.norm(Z, axis=0), axis=0)
+    assert np.all(e <= tol), e.max()
+    x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, hsc=H)
+    assert info.status == 0
+    R = B - O.spmm(A.row_ptr, A.col_idx, A.values, x.cpu().numpy())
+    assert np.all(np.linalg.norm(R, axis=0) <= 1e-9 * np.linalg.norm(B, axis=0))

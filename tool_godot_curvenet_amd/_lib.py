@@ -163,6 +163,7 @@ _SIGS = {
     "cn_bezier3": (i32, [i32, vp, i32, vp, vp, vp]),
     "cn_segment_gradients": (i32, [i32, vp, vp, vp, vp, vp, vp]),
     "cn_deformation_gradients": (i32, [i32, vp, vp, vp, vp]),
+    "cn_hsc_tail_level": (i32, [vp, i32, i32]),
     "cn_hsc_import": (i32, [vp, C.POINTER(HscOpts), i32, C.POINTER(HscLevelIn), i32, vp, vp,
                             C.POINTER(vp), vp]),
     "cn_st_parse": (i32, [vp, i64, C.POINTER(vp)]),

@@ -256,6 +256,159 @@ static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
 }
 
 // z = M^-1 r (n0 x k), r and z distinct.
+// ---- fused tail: the small levels and the coarse solve in one launch ----
+// The apply is independent per column, so one workgroup can own a slice of
+// VEC columns and run every small level's down pass, the coarse product and
+// the up passes for it, with a workgroup barrier between levels instead of
+// a kernel boundary (the per-level kernels of levels of a few thousand rows
+// are bound by launch and memory latency, not bandwidth).  Slices are dealt
+// to XCDs in contiguous column ranges, so an XCD's L2 holds its columns of
+// the small levels' rows.  Arithmetic per row is the level kernels': the
+// row's own value, then fma over its R (or P) entries in order; the coarse
+// product sums the symmetric inverse's column in FP64 in order.
+template <typename T>
+struct TailLevel {
+  int32_t nC, nF;
+  const int32_t *Cg, *r_ptr, *r_colg;
+  const T *r_val;
+  const int32_t *Fg;
+  const T *dinv;
+  const int32_t *p_ptr, *p_colg;
+  const T *p_val;
+};
+
+constexpr int kTailBlock = 1024;
+// Levels of at most this many rows go to the tail.  A workgroup walks a
+// level's rows 1024 at a time, each a chain of three dependent loads, so the
+// tail pays off only where a level is a round or two: measured on the 100k
+// capsule union at k = 768, thresholds 0 / 1500 / 3000 / 6000 / 12288 rows
+// gave tails of 36 / 64 / 104 / 185 / 389 us an apply against 51 / 88 / 120 /
+// 155 / 187 us for the same levels and coarse solve run as separate kernels.
+constexpr int32_t kTailRows = 1536;
+constexpr int32_t kTailCoarse = 1024; // coarse rows the tail's LDS holds
+constexpr int kTailSlices = 128;      // fewest column slices (workgroups) worth a tail
+
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kTailBlock)
+k_hsc_tail(int32_t l0, int32_t l1, const TailLevel<T> *__restrict__ lv, int32_t nc,
+           const int32_t *__restrict__ crow, const double *__restrict__ cinv, int64_t k,
+           const T *r, T *Z, const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
+  __shared__ double sw[kTailCoarse * VEC];
+  const int32_t nsl = (int32_t)(k / VEC), b = blockIdx.x;
+  const int32_t slice = nsl % kXcd == 0 ? (b % kXcd) * (nsl / kXcd) + b / kXcd : b;
+  const int64_t c0 = (int64_t)slice * VEC;
+  const int t = threadIdx.x;
+  for (int32_t l = l0; l < l1; ++l) {
+    const TailLevel<T> L = lv[l];
+    const T *src = l == 0 ? r : Z;
+    for (int32_t c = t; c < L.nC; c += kTailBlock) {
+      const int64_t g = (int64_t)L.Cg[c] * k + c0;
+      T acc[VEC];
+      VecIO<T, VEC>::ld(src + g, acc);
+      for (int32_t e = L.r_ptr[c]; e < L.r_ptr[c + 1]; ++e) {
+        T x[VEC];
+        VecIO<T, VEC>::ld(src + (int64_t)L.r_colg[e] * k + c0, x);
+        const T w = L.r_val[e];
+#pragma unroll
+        for (int m = 0; m < VEC; ++m) acc[m] = fma(w, x[m], acc[m]);
+      }
+      VecIO<T, VEC>::st(Z + g, acc);
+    }
+    __syncthreads();
+  }
+  // coarse: z_c = A_c^-1 z_c on the coarse rows (the inverse is symmetric,
+  // so row i is read as column i: coalesced over i)
+  const T *csrc = l1 == 0 ? r : Z;
+  for (int32_t j = t; j < nc; j += kTailBlock) {
+    T x[VEC];
+    VecIO<T, VEC>::ld(csrc + (int64_t)crow[j] * k + c0, x);
+#pragma unroll
+    for (int m = 0; m < VEC; ++m) sw[j * VEC + m] = (double)x[m];
+  }
+  __syncthreads();
+  for (int32_t i = t; i < nc; i += kTailBlock) {
+    double acc[VEC] = {};
+    int32_t j = 0;
+    // 8 column entries in flight at a time (in order, as one sum)
+    for (; j + 8 <= nc; j += 8) {
+      double a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = cinv[(int64_t)(j + u) * nc + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int m = 0; m < VEC; ++m) acc[m] = fma(a[u], sw[(j + u) * VEC + m], acc[m]);
+    }
+    for (; j < nc; ++j) {
+      const double a = cinv[(int64_t)j * nc + i];
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) acc[m] = fma(a, sw[j * VEC + m], acc[m]);
+    }
+    T o[VEC];
+#pragma unroll
+    for (int m = 0; m < VEC; ++m) o[m] = (T)acc[m];
+    VecIO<T, VEC>::st(Z + (int64_t)crow[i] * k + c0, o);
+  }
+  __syncthreads();
+  for (int32_t l = l1 - 1; l >= l0; --l) {
+    const TailLevel<T> L = lv[l];
+    const T *src = l == 0 ? r : Z;
+    for (int32_t f = t; f < L.nF; f += kTailBlock) {
+      const int64_t g = (int64_t)L.Fg[f] * k + c0;
+      T acc[VEC];
+      VecIO<T, VEC>::ld(src + g, acc);
+      const T d = L.dinv[f];
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) acc[m] = d * acc[m];
+      for (int32_t e = L.p_ptr[f]; e < L.p_ptr[f + 1]; ++e) {
+        T x[VEC];
+        VecIO<T, VEC>::ld(Z + (int64_t)L.p_colg[e] * k + c0, x);
+        const T w = L.p_val[e];
+#pragma unroll
+        for (int m = 0; m < VEC; ++m) acc[m] = fma(w, x[m], acc[m]);
+      }
+      VecIO<T, VEC>::st(Z + g, acc);
+    }
+    __syncthreads();
+  }
+}
+
+// First level the fused tail takes for an apply with k columns (levels
+// [l0, L) and the coarse solve), or -1 when the per-level kernels run.
+template <typename T>
+static int32_t tail_level(const cn_hsc_t *H, int64_t k) {
+  const int v = hsc_vec<T>(k);
+  if (v == 0 || k / v < kTailSlices || H->nc > kTailCoarse) return -1;
+  const int32_t L = (int32_t)H->lv.size();
+  int32_t l0 = L;
+  while (l0 > 0 && H->lv[l0 - 1].n <= kTailRows) --l0;
+  return l0;
+}
+
+template <typename T> struct TailIdx;
+template <> struct TailIdx<float> { static constexpr int i = 0; };
+template <> struct TailIdx<double> { static constexpr int i = 1; };
+
+template <typename T>
+static const TailLevel<T> *tail_desc(cn_hsc_t *H, hipStream_t s) {
+  DevBuf<unsigned char> &d = H->tail_desc[TailIdx<T>::i];
+  if (!d.get()) {
+    std::vector<TailLevel<T>> h(H->lv.size() + 1);
+    for (size_t l = 0; l < H->lv.size(); ++l) {
+      const cn_hsc_t::Level &L = H->lv[l];
+      h[l] = TailLevel<T>{L.nC, L.nF, L.Cg.get(), L.r_ptr.get(), L.r_colg.get(),
+                          LvVals<T>::r(L), L.Fg.get(), LvVals<T>::d(L), L.p_ptr.get(),
+                          L.p_colg.get(), LvVals<T>::p(L)};
+    }
+    d.alloc(sizeof(TailLevel<T>) * h.size());
+    CN_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(TailLevel<T>) * h.size(),
+                          hipMemcpyHostToDevice, s));
+    CN_HIP(hipStreamSynchronize(s));
+  }
+  return reinterpret_cast<const TailLevel<T> *>(d.get());
+}
+
 template <typename T>
 void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const int32_t *ctl) {
   const int32_t L = (int32_t)H->lv.size();
@@ -265,6 +418,31 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
     H->work_bytes = bytes;
   }
   T *W = reinterpret_cast<T *>(H->work.get());
+  const int32_t l0 = tail_level<T>(H, k);
+  if (l0 >= 0) {
+    const TailLevel<T> *desc = tail_desc<T>(H, s);
+    {
+      ProfScope ps("hsc_down", s);
+      for (int32_t l = 0; l < l0; ++l) level_down<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s);
+    }
+    {
+      ProfScope ps("hsc_tail", s);
+      const int v = hsc_vec<T>(k);
+      const unsigned g = (unsigned)(k / v);
+#define T_(VV) k_hsc_tail<T, VV><<<g, kTailBlock, 0, s>>>(l0, L, desc, H->nc, H->coarse_rows.get(), H->coarse_inv.get(), k, r, z, ctl)
+      if constexpr (sizeof(T) == 4) {
+        if (v == 4) T_(4); else if (v == 2) T_(2); else T_(1);
+      } else {
+        if (v == 2) T_(2); else T_(1);
+      }
+#undef T_
+    }
+    {
+      ProfScope ps("hsc_up", s);
+      for (int32_t l = l0 - 1; l >= 0; --l) level_up<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s);
+    }
+    return;
+  }
   {
     ProfScope ps("hsc_down", s);
     for (int32_t l = 0; l < L; ++l) level_down<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s);
@@ -301,6 +479,11 @@ template void hsc_apply<double>(cn_hsc_t *, int64_t, const double *, double *, h
 using namespace cn;
 
 extern "C" {
+
+int32_t cn_hsc_tail_level(cn_hsc H, int32_t k, int32_t fp32) {
+  if (!H || k <= 0) return -1;
+  return fp32 ? tail_level<float>(H, k) : tail_level<double>(H, k);
+}
 
 cn_status cn_hsc_apply(cn_hsc H, int32_t k, const double *d_r, double *d_z, void *stream) {
   CN_API_BEGIN

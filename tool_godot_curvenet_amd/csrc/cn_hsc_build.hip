@@ -400,6 +400,14 @@ cn_status cn_hsc_import(cn_system S, const cn_hsc_opts *opts, int32_t nlevels,
     set_error("cn_hsc_import: %d coarse rows given, the levels leave %d", n_coarse, n);
     return CN_BAD_ARG;
   }
+  // the apply reads the coarse inverse by columns: it must be symmetric, as
+  // cn_hsc_build leaves it
+  for (int32_t i = 0; i < n_coarse; ++i)
+    for (int32_t j = 0; j < i; ++j)
+      if (!(coarse_inv[(size_t)i * n_coarse + j] == coarse_inv[(size_t)j * n_coarse + i])) {
+        set_error("cn_hsc_import: the coarse inverse is not symmetric");
+        return CN_BAD_ARG;
+      }
   hipStream_t s = as_stream(stream);
   cn_hsc_t *H = new cn_hsc_t();
   try {

@@ -268,4 +268,7 @@ struct cn_hsc_t {
   // apply workspace: the gathered coarse rows (nc x k), grown on demand
   cn::DevBuf<unsigned char> work;
   size_t work_bytes = 0;
+  // level descriptors of the fused tail (the small levels and the coarse
+  // solve in one launch), FP32 and FP64 forms, built on first use
+  cn::DevBuf<unsigned char> tail_desc[2];
 };

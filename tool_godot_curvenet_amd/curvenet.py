@@ -511,6 +511,11 @@ class HSC:
         self._adopt(h, opts)
         return self
 
+    def tail_level(self, k: int, fp32: bool = True) -> int:
+        """First level an apply of k columns runs in the fused tail launch
+        (cn_hsc_tail_level), -1 when every level runs its own kernels."""
+        return int(L.load().cn_hsc_tail_level(self._h, int(k), 1 if fp32 else 0))
+
     def save(self, path) -> None:
         """Write the hierarchy to a safetensors sidecar (formats.save_sidecar)."""
         from . import formats
