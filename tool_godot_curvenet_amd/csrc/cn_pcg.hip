@@ -1603,14 +1603,17 @@ static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
   return G.rows;
 }
 
-// the fused reliable update + residual for k > 4 (kw_resid_x); false when k
-// is narrow (the caller runs the two kernels)
+// the fused reliable update + residual for k >= 256 (kw_resid_x); false
+// otherwise (the caller runs the two kernels)
 template <typename TO>
 static bool launch_resid_x(int32_t n, const int32_t *ptr, const int32_t *col,
                            const double *val, int64_t k, const double *X, const TO *D,
                            const double *invs, double *X2, TO *D2, const double *B, TO *R,
                            const double *scale, double *part, hipStream_t s, int32_t *rows) {
-  if (k <= 4) return false;
+  // measured: at k = 768 the pass matches the two kernels (818 vs 823 us on
+  // the 100k capsule union); at k = 48 (2.46M rows) the two kernels are ahead
+  // by 1 % of the whole step, so narrower blocks keep them
+  if (k < 256) return false;
   Grid G = grid_rows<double>(n, k, kCapSpmm);
   ProfScope ps("resid", s);
 #define N_(KK) (void)0
