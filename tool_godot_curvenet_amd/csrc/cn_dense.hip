@@ -38,32 +38,38 @@ __device__ __forceinline__ double to_d(T v) { return (double)v; }
 // op(B)(t, j) = BT ? B[j * ldb + t] : B[t * ldb + j]
 // LOWER: only tiles on or below the diagonal, only entries j <= i written.
 // Block 256 threads = 4 waves; wave w owns the 16x16 subtile (w / 2, w % 2).
+// C = alpha A B + beta C.  With part != nullptr (split K), block z sums the
+// k-range [z kchunk, (z + 1) kchunk) and writes alpha times its sum to
+// part[z][i][j] instead of C.
 template <bool AT, bool BT, bool LOWER, typename TB, typename TC>
 __global__ void __launch_bounds__(256)
 k_gemm_f64(int m, int n, int kk, double alpha, const double *__restrict__ A, int64_t lda,
            const TB *__restrict__ B, int64_t ldb, double beta, TC *__restrict__ C,
            int64_t ldc, const int32_t *__restrict__ crow = nullptr,
-           const int32_t *__restrict__ ctl = nullptr) {
+           const int32_t *__restrict__ ctl = nullptr, double *__restrict__ part = nullptr,
+           int kchunk = 0) {
   if (LOWER && blockIdx.x > blockIdx.y) return;
   if (ctl && ctl_idle(ctl)) return;
+  const int kbeg = part ? (int)blockIdx.z * kchunk : 0;
+  const int kend = part ? min(kk, kbeg + kchunk) : kk;
   __shared__ double As[kGT][kGK + 1];
   __shared__ double Bs[kGK][kGT + 1];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
   const int bi = blockIdx.y * kGT, bj = blockIdx.x * kGT;
   const int wi = (wave >> 1) * 16, wj = (wave & 1) * 16;
   v4d acc = {0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < kk; k0 += kGK) {
+  for (int k0 = kbeg; k0 < kend; k0 += kGK) {
     // stage A (32 x 16) and B (16 x 32): two elements of each per thread
     for (int e = tid; e < kGT * kGK; e += 256) {
       const int r = e / kGK, t = e % kGK;
       const int gi = bi + r, gt = k0 + t;
       double a = 0.0;
-      if (gi < m && gt < kk) a = AT ? A[(int64_t)gt * lda + gi] : A[(int64_t)gi * lda + gt];
+      if (gi < m && gt < kend) a = AT ? A[(int64_t)gt * lda + gi] : A[(int64_t)gi * lda + gt];
       As[r][t] = a;
       const int t2 = e / kGT, c = e % kGT;
       const int gt2 = k0 + t2, gj = bj + c;
       double b = 0.0;
-      if (gj < n && gt2 < kk) b = to_d(BT ? B[(int64_t)gj * ldb + gt2] : B[(int64_t)gt2 * ldb + gj]);
+      if (gj < n && gt2 < kend) b = to_d(BT ? B[(int64_t)gj * ldb + gt2] : B[(int64_t)gt2 * ldb + gj]);
       Bs[t2][c] = b;
     }
     __syncthreads();
@@ -78,6 +84,10 @@ k_gemm_f64(int m, int n, int kk, double alpha, const double *__restrict__ A, int
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int i = bi + wi + (l >> 4) + 4 * v, j = bj + wj + (l & 15);
+    if (part) {
+      if (i < m && j < n) part[((int64_t)blockIdx.z * m + i) * n + j] = alpha * acc[v];
+      continue;
+    }
     if (i < m && j < n && (!LOWER || j <= i)) {
       TC *cp = C + (int64_t)(crow ? crow[i] : i) * ldc + j;
       const double old = beta == 0.0 ? 0.0 : to_d(*cp);
@@ -248,18 +258,61 @@ template void dense_gemm_nn<float, float>(int, int, const double *, const float 
                                           float *, int64_t, hipStream_t);
 
 // Same product written to rows zrows[i] of Z.
+// Split of the K range for an m x m by m x k product: a grid of a few
+// hundred 32 x 32 tiles walks K one 16-wide step at a time, each step a
+// global load, an LDS round and a barrier; with few tiles (moderate k) the
+// K walk is split over up to 16 blocks a tile and the partial sums are
+// added in order by k_gemm_reduce.
+static int gemm_splits(int m, int k) {
+  const int tiles = ((m + kGT - 1) / kGT) * ((k + kGT - 1) / kGT);
+  int sp = 1;
+  while (sp < 16 && tiles * sp < 1024 && m / (2 * sp) >= 4 * kGK) sp *= 2;
+  return sp;
+}
+
+size_t dense_gemm_rows_work(int m, int k) {
+  const int sp = gemm_splits(m, k);
+  return sp > 1 ? (size_t)sp * m * k : 0;
+}
+
+template <typename TC>
+__global__ void k_gemm_reduce(int m, int n, int sp, const double *__restrict__ part,
+                              TC *__restrict__ C, int64_t ldc,
+                              const int32_t *__restrict__ crow,
+                              const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)m * n) return;
+  const int i = (int)(id / n), j = (int)(id - (int64_t)i * n);
+  double s = part[id];
+  for (int z = 1; z < sp; ++z) s += part[(int64_t)z * m * n + id];
+  C[(int64_t)(crow ? crow[i] : i) * ldc + j] = (TC)s;
+}
+
 template <typename TB, typename TC>
 void dense_gemm_nn_rows(int m, int k, const double *M, const TB *X, int64_t ldx, TC *Z,
-                        int64_t ldz, const int32_t *zrows, const int32_t *ctl, hipStream_t s) {
-  k_gemm_f64<false, false, false, TB, TC><<<gemm_grid(m, k), 256, 0, s>>>(
-      m, k, m, 1.0, M, m, X, ldx, 0.0, Z, ldz, zrows, ctl);
+                        int64_t ldz, const int32_t *zrows, const int32_t *ctl, double *work,
+                        hipStream_t s) {
+  const int sp = work ? gemm_splits(m, k) : 1;
+  if (sp <= 1) {
+    k_gemm_f64<false, false, false, TB, TC><<<gemm_grid(m, k), 256, 0, s>>>(
+        m, k, m, 1.0, M, m, X, ldx, 0.0, Z, ldz, zrows, ctl);
+    return;
+  }
+  const int chunk = ((m + sp - 1) / sp + kGK - 1) / kGK * kGK;
+  dim3 g = gemm_grid(m, k);
+  g.z = (unsigned)sp;
+  k_gemm_f64<false, false, false, TB, TC><<<g, 256, 0, s>>>(
+      m, k, m, 1.0, M, m, X, ldx, 0.0, Z, ldz, zrows, ctl, work, chunk);
+  k_gemm_reduce<TC><<<grid_for((int64_t)m * k, 256), 256, 0, s>>>(m, k, sp, work, Z, ldz,
+                                                                  zrows, ctl);
 }
 template void dense_gemm_nn_rows<double, double>(int, int, const double *, const double *,
                                                  int64_t, double *, int64_t, const int32_t *,
-                                                 const int32_t *, hipStream_t);
+                                                 const int32_t *, double *, hipStream_t);
 template void dense_gemm_nn_rows<float, float>(int, int, const double *, const float *, int64_t,
                                                float *, int64_t, const int32_t *, const int32_t *,
-                                               hipStream_t);
+                                               double *, hipStream_t);
 
 }  // namespace cn
 

@@ -26,9 +26,11 @@
 
 namespace cn {
 
+size_t dense_gemm_rows_work(int m, int k);
 template <typename TB, typename TC>
 void dense_gemm_nn_rows(int m, int k, const double *M, const TB *X, int64_t ldx, TC *Z,
-                        int64_t ldz, const int32_t *zrows, const int32_t *ctl, hipStream_t s);
+                        int64_t ldz, const int32_t *zrows, const int32_t *ctl, double *work,
+                        hipStream_t s);
 
 // ---- wide form: LPR lanes per (row, LPR VEC column tile), 64 / LPR rows
 // per wavefront -----------------------------------------------------------
@@ -419,7 +421,9 @@ static const TailLevel<T> *tail_desc(cn_hsc_t *H, hipStream_t s) {
 template <typename T>
 void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const int32_t *ctl) {
   const int32_t L = (int32_t)H->lv.size();
-  const size_t bytes = sizeof(T) * (size_t)H->nc * k + 256;
+  // the gathered coarse rows, then (256-aligned) the split-K partial sums
+  const size_t wbytes = (sizeof(T) * (size_t)H->nc * k + 255) / 256 * 256;
+  const size_t bytes = wbytes + sizeof(double) * dense_gemm_rows_work(H->nc, (int)k) + 256;
   if (H->work_bytes < bytes) {
     H->work.alloc(bytes);
     H->work_bytes = bytes;
@@ -468,7 +472,8 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
 #undef G_
     } else {
       dense_gemm_nn_rows<T, T>(H->nc, (int)k, H->coarse_inv.get(), W, k, z, k,
-                               H->coarse_rows.get(), ctl, s);
+                               H->coarse_rows.get(), ctl,
+                               reinterpret_cast<double *>(H->work.get() + wbytes), s);
     }
   }
   {
