@@ -35,7 +35,7 @@ TRAFFIC_KERNELS = {
     "bwd": "void cn::kb_bwd<float, 4, 16>",
     "hsc_down": "void cn::kw_hsc_down<float, 4, 64>",
     "hsc_up": "void cn::kw_hsc_up<float, 4, 64>",
-    "pd_update": "void cn::k_pd_update<float>",
+    "pd_update": "void cn::kw_pd_update<float, 4, 64>",
 }
 FP64_MFMA_PEAK_TFLOPS = 78.6
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
@@ -263,7 +263,7 @@ def main():
         # kept rows: 4k(2 n_C + n_F) plus R (8 B per entry, row pointers, row
         # map); up reads and writes the eliminated rows and reads, once, the
         # kept ones: 4k(2 n_F + n_C) plus P, the row map and 1/a_ff.
-        # k_pd_update (d += alpha p, p = z + beta p) streams 5 n x k vectors.
+        # kw_pd_update (d += alpha p, p = z + beta p) streams 5 n x k vectors.
         # levels [l0, L) and the coarse solve run in the fused tail launch
         l0 = lib.cn_hsc_tail_level(pre.handle, k, 1 if args.precision == "mixed" else 0)
         down_b = up_b = tail_b = 0
@@ -294,7 +294,7 @@ def main():
             tail_b += 8 * k * ncr + 8 * ncr * ncr
             kern["hsc_tail"] = ("k_hsc_tail<float,4>", prof["hsc_tail"][0], prof["hsc_tail"][1],
                                 nwork * tail_b)
-        kern["pd_update"] = ("k_pd_update<float>", prof["bwd"][0], prof["bwd"][1],
+        kern["pd_update"] = ("kw_pd_update<float,4,64>", prof["bwd"][0], prof["bwd"][1],
                              nwork * 5 * nk4)
         nc = pre.info.n_coarse
         cms = prof["hsc_coarse"][1]

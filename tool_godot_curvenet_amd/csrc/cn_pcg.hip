@@ -1290,6 +1290,50 @@ __global__ void k_pd_update(int64_t total, int64_t k, SegMap sm, const double *_
   }
 }
 
+// Wide form of k_pd_update (same arithmetic): LPR lanes per row, VEC
+// columns a lane, the column scalars held in registers across rows (per row
+// only for a batch, whose scalars depend on the row's system).
+template <typename T, int VEC, int LPR>
+__global__ void __launch_bounds__(kBlock)
+kw_pd_update(int32_t n, int64_t k, SegMap sm, const double *__restrict__ st,
+             const T *__restrict__ Z, T *__restrict__ P, T *__restrict__ D,
+             const int32_t *__restrict__ ctl) {
+  if (bwd_idle(ctl)) return;
+  WIDE_COLS_P(VEC, LPR)
+  T al[VEC], be[VEC], ga[VEC];
+#pragma unroll
+  for (int m = 0; m < VEC; ++m) {
+    al[m] = valid ? (T)st[ST_ALPHA * sm.kk + c0 + m] : T(0);
+    be[m] = valid ? (T)st[ST_BETA * sm.kk + c0 + m] : T(0);
+    ga[m] = valid ? (T)st[ST_GAMMA * sm.kk + c0 + m] : T(0);
+  }
+  WIDE_LOOP_P(row, 0, n, LPR) {
+    if (valid) {
+      if (sm.of_row) {
+        const int64_t so = sm.vc(Pack<LPR>::u(row), k, c0);
+#pragma unroll
+        for (int m = 0; m < VEC; ++m) {
+          al[m] = (T)st[ST_ALPHA * sm.kk + so + m];
+          be[m] = (T)st[ST_BETA * sm.kk + so + m];
+          ga[m] = (T)st[ST_GAMMA * sm.kk + so + m];
+        }
+      }
+      const int64_t o = (int64_t)row * k + c0;
+      T pv[VEC], dv[VEC], zv[VEC];
+      VecIO<T, VEC>::ld(P + o, pv);
+      VecIO<T, VEC>::ld(D + o, dv);
+      VecIO<T, VEC>::ld(Z + o, zv);
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) {
+        dv[m] = fma(al[m], pv[m], dv[m]);
+        pv[m] = fma(be[m], pv[m], ga[m] * zv[m]);
+      }
+      VecIO<T, VEC>::st(D + o, dv);
+      VecIO<T, VEC>::st(P + o, pv);
+    }
+  }
+}
+
 // Y = (T)(X * s_c)
 template <typename T>
 __global__ void k_scale_in(int64_t total, int64_t k, SegMap sm, const double *__restrict__ s,
@@ -1811,7 +1855,12 @@ static void launch_bwd(const SysView<T> &V, int64_t k, const double *st, T *Z, T
     ProfScope ps("bwd", s);
     const int64_t total = (int64_t)V.n * k;
     unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192));
-    k_pd_update<T><<<g, 256, 0, s>>>(total, k, V.sm, st, Z, P, D, ctl);
+    const Grid G = grid_rows<T>(V.n, k, kCapSpmm);
+#define N_(KK) k_pd_update<T><<<g, 256, 0, s>>>(total, k, V.sm, st, Z, P, D, ctl)
+#define W_(VV, LL) kw_pd_update<T, WideV<T, VV>::v, LL><<<G.g, kBlock, 0, s>>>(V.n, k, V.sm, st, Z, P, D, ctl)
+    DISPATCH_K(k, T, G.lpr, N_, W_);
+#undef N_
+#undef W_
     return;
   }
   const std::vector<int32_t> &lb = V.ic0->level_blk;
