@@ -85,7 +85,9 @@ def test_hierarchy_bit_exact(gpu, name, comp):
     assert np.abs(inv @ Co - np.eye(len(Co))).max() <= 1e-9
 
 
-@pytest.mark.parametrize("k", [1, 3, 8])
+# k <= 4 takes the coarse GEMV; k = 5, 8 and 48 take the coarse GEMM with its
+# K range split over 4 blocks a tile (391 coarse rows, 13 or 26 tiles).
+@pytest.mark.parametrize("k", [1, 3, 5, 8, 48])
 def test_hsc_apply_matches_oracle(gpu, k):
     from tool_godot_curvenet_amd import HSC
     lap = _lap("cylinder64")

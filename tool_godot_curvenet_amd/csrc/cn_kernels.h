@@ -113,6 +113,18 @@ struct VecIO<double, 2> {
     *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
   }
 };
+template <>
+struct VecIO<double, 4> {
+  static __device__ __forceinline__ void ld(const double *p, double (&v)[4]) {
+    double2 t = *reinterpret_cast<const double2 *>(p);
+    double2 u = *reinterpret_cast<const double2 *>(p + 2);
+    v[0] = t.x; v[1] = t.y; v[2] = u.x; v[3] = u.y;
+  }
+  static __device__ __forceinline__ void st(double *p, const double (&v)[4]) {
+    *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2 *>(p + 2) = make_double2(v[2], v[3]);
+  }
+};
 template <typename T>
 struct VecIO<T, 1> {
   static __device__ __forceinline__ void ld(const T *p, T (&v)[1]) { v[0] = *p; }

@@ -1357,6 +1357,65 @@ __global__ void k_x_update(int64_t total, int64_t k, SegMap sm, const double *__
   }
 }
 
+// Wide forms of k_scale_in and k_x_update (same arithmetic), mapped like
+// kw_pd_update.
+template <typename T, int VEC, int LPR>
+__global__ void __launch_bounds__(kBlock)
+kw_scale_in(int32_t n, int64_t k, SegMap sm, const double *__restrict__ sc,
+            const double *__restrict__ X, T *__restrict__ Y) {
+  WIDE_COLS_P(VEC, LPR)
+  double s[VEC];
+#pragma unroll
+  for (int m = 0; m < VEC; ++m) s[m] = valid ? sc[c0 + m] : 0.0;
+  WIDE_LOOP_P(row, 0, n, LPR) {
+    if (valid) {
+      if (sm.of_row) {
+        const int64_t so = sm.vc(Pack<LPR>::u(row), k, c0);
+#pragma unroll
+        for (int m = 0; m < VEC; ++m) s[m] = sc[so + m];
+      }
+      const int64_t o = (int64_t)row * k + c0;
+      double xv[VEC];
+      T yv[VEC];
+      VecIO<double, VEC>::ld(X + o, xv);
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) yv[m] = (T)(xv[m] * s[m]);
+      VecIO<T, VEC>::st(Y + o, yv);
+    }
+  }
+}
+
+template <typename T, int VEC, int LPR>
+__global__ void __launch_bounds__(kBlock)
+kw_x_update(int32_t n, int64_t k, SegMap sm, const double *__restrict__ invs,
+            T *__restrict__ D, double *__restrict__ X) {
+  WIDE_COLS_P(VEC, LPR)
+  double iv[VEC];
+#pragma unroll
+  for (int m = 0; m < VEC; ++m) iv[m] = valid ? invs[c0 + m] : 0.0;
+  WIDE_LOOP_P(row, 0, n, LPR) {
+    if (valid) {
+      if (sm.of_row) {
+        const int64_t so = sm.vc(Pack<LPR>::u(row), k, c0);
+#pragma unroll
+        for (int m = 0; m < VEC; ++m) iv[m] = invs[so + m];
+      }
+      const int64_t o = (int64_t)row * k + c0;
+      double xv[VEC];
+      T dv[VEC];
+      VecIO<double, VEC>::ld(X + o, xv);
+      VecIO<T, VEC>::ld(D + o, dv);
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) {
+        xv[m] = fma(iv[m], (double)dv[m], xv[m]);
+        dv[m] = T(0);
+      }
+      VecIO<double, VEC>::st(X + o, xv);
+      VecIO<T, VEC>::st(D + o, dv);
+    }
+  }
+}
+
 template <typename T>
 __global__ void k_axpy(int64_t total, int64_t k, const double *__restrict__ a,
                        double sign, const T *__restrict__ x, T *__restrict__ y) {
@@ -1957,6 +2016,31 @@ static unsigned ew_blocks(int64_t total) {
   return (unsigned)g;
 }
 
+// Y = (T)(X * s_c) and X += invs_c D, D = 0 over n x k
+template <typename T>
+static void launch_scale_in(const SysView<T> &V, int64_t k, const double *sc, const double *X,
+                            T *Y, hipStream_t s) {
+  const int64_t total = (int64_t)V.n * k;
+  const Grid G = grid_rows<T>(V.n, k, kCapSpmm);
+#define N_(KK) k_scale_in<T><<<ew_blocks(total), 256, 0, s>>>(total, k, V.sm, sc, X, Y)
+#define W_(VV, LL) kw_scale_in<T, WideV<T, VV>::v, LL><<<G.g, kBlock, 0, s>>>(V.n, k, V.sm, sc, X, Y)
+  DISPATCH_K(k, T, G.lpr, N_, W_);
+#undef N_
+#undef W_
+}
+
+template <typename T>
+static void launch_x_update(const SysView<T> &V, int64_t k, const double *invs, T *D,
+                            double *X, hipStream_t s) {
+  const int64_t total = (int64_t)V.n * k;
+  const Grid G = grid_rows<T>(V.n, k, kCapSpmm);
+#define N_(KK) k_x_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, V.sm, invs, D, X)
+#define W_(VV, LL) kw_x_update<T, WideV<T, VV>::v, LL><<<G.g, kBlock, 0, s>>>(V.n, k, V.sm, invs, D, X)
+  DISPATCH_K(k, T, G.lpr, N_, W_);
+#undef N_
+#undef W_
+}
+
 // Two-stage column reduction: returns the buffer and row count the final
 // k_reduce_* kernel should read (stride between quantities = rows * k).
 struct Red {
@@ -2217,7 +2301,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     CN_HIP(hipMemsetAsync(st + ST_ALPHA * kk, 0, sizeof(double) * kk, s));
     // after the first round the residual kernel writes the scaled R itself
     if (!r_scaled)
-      k_scale_in<T><<<ew_blocks(total), 256, 0, s>>>(total, k, V.sm, st + ST_S * kk, R64, R);
+      launch_scale_in<T>(V, k, st + ST_S * kk, R64, R, s);
     if (apply_m) {
       // z = M^-1 r; rz; continuing columns keep p, restarted ones take p = z
       rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, s);
@@ -2296,7 +2380,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
         continue;
       }
     }
-    k_x_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, V.sm, st + ST_INVS * kk, D, X);
+    launch_x_update<T>(V, k, st + ST_INVS * kk, D, X, s);
     if (batch) {
       // FP64 residual, per-segment r.r, scaled into R at the top of the loop
       launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
