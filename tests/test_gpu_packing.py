@@ -5,6 +5,8 @@ numpy over the oracle's SpMM and hierarchy, and converged solves.
 The lanes per row follow from k: FP32 columns k = 5, 9, 12, 48, 144 pack 16
 lanes per row, k = 27, 100, 190 pack 32 (190 in three column tiles), and
 k = 200 keeps 64; FP64 columns k = 5, 12, 144 pack 16, k = 27, 48, 190 pack 32.
+The HSC level passes pack 12 lanes per row (5 rows a wavefront) where that
+leaves fewer lanes idle: k = 5, 12, 48, 144 in FP32 and in FP64.
 """
 import numpy as np
 import pytest
@@ -84,7 +86,7 @@ def test_pcg_hsc_converges_moderate_k(gpu, precision, k):
     assert np.all(rr <= 1e-9), rr.max()
 
 
-@pytest.mark.parametrize("k", [5, 27, 144])
+@pytest.mark.parametrize("k", [5, 12, 27, 48, 144])
 def test_hsc_apply_moderate_k(gpu, k):
     from tool_godot_curvenet_amd import HSC
     lap = _lap("icosphere6")

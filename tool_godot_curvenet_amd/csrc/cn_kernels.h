@@ -189,17 +189,24 @@ __device__ __forceinline__ void gather_narrow(int32_t a, int32_t b, int g,
 // (k / VEC of 9..32 lanes) would otherwise leave most of a 64-lane tile
 // idle.  With LPR < 64 a row index is no longer wave-uniform and row data
 // is read with vector loads; LPR = 64 is the one-row-per-wave form.
+// LPR = 12 (5 rows a wavefront, lanes 60-63 idle) serves only kernels
+// without block partials (the HSC level passes): its idle lanes take the
+// wave's last row slot and a column past every k, so they load in bounds
+// and store nothing.
 template <int LPR>
 struct Pack {
-  static_assert(LPR == 16 || LPR == 32 || LPR == 64, "lanes per row");
+  static_assert(LPR == 12 || LPR == 16 || LPR == 32 || LPR == 64, "lanes per row");
   static constexpr int G = 64 / LPR;        // rows per wavefront
   static constexpr int kSlots = kWaves * G;  // rows per block
   // row slot of this thread: the wave's row groups are consecutive slots
   static __device__ __forceinline__ int32_t slot() {
-    return (int32_t)((threadIdx.x >> 6) * G + (threadIdx.x & 63) / LPR);
+    const int g = (threadIdx.x & 63) / LPR;
+    return (int32_t)((threadIdx.x >> 6) * G + (g < G ? g : G - 1));
   }
   static __device__ __forceinline__ int64_t col0(int vec) {
-    return (int64_t)blockIdx.y * LPR * vec + (int64_t)(threadIdx.x & (LPR - 1)) * vec;
+    const int lane = threadIdx.x & 63;
+    if (G * LPR < 64 && lane >= G * LPR) return INT64_MAX / 2;
+    return (int64_t)blockIdx.y * LPR * vec + (int64_t)(lane % LPR) * vec;
   }
   // wave-uniform only when the wave holds a single row
   static __device__ __forceinline__ int32_t u(int32_t x) {
@@ -230,6 +237,7 @@ __device__ __forceinline__ void block_partials_wide_in(double (&p)[NP][VEC],
                                                        double *__restrict__ out,
                                                        int64_t k, int64_t c0,
                                                        int64_t stride_np, double *sh) {
+  static_assert(64 % LPR == 0, "partials need whole row groups");
   constexpr int TW = 64 * VEC, G = 64 / LPR;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
