@@ -1,12 +1,10 @@
-"""Moderate column counts (5 <= k <= 190), where the wide row kernels pack
-16 or 32 lanes per row: a few PCG iterations against the same iterations in
-numpy over the oracle's SpMM and hierarchy, and converged solves.
+"""Moderate column counts (5 <= k <= 200), where the wide row kernels pack
+12, 16 or 32 lanes per row: a few PCG iterations against the same iterations
+in numpy over the oracle's SpMM and hierarchy, and converged solves.
 
-The lanes per row follow from k: FP32 columns k = 5, 9, 12, 48, 144 pack 16
-lanes per row, k = 27, 100, 190 pack 32 (190 in three column tiles), and
-k = 200 keeps 64; FP64 columns k = 5, 12, 144 pack 16, k = 27, 48, 190 pack 32.
-The HSC level passes pack 12 lanes per row (5 rows a wavefront) where that
-leaves fewer lanes idle: k = 5, 12, 48, 144 in FP32 and in FP64.
+The lanes per row follow from k: k = 5, 12, 48, 144 pack 12 lanes per row (5
+rows a wavefront, 4 lanes idle) in FP32 and FP64, k = 27 and 190 pack 32 (190
+in three column tiles), k = 200 keeps 64 in FP32 and packs 16 in FP64.
 """
 import numpy as np
 import pytest

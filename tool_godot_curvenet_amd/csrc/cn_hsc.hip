@@ -202,17 +202,6 @@ static dim3 hsc_grid(int64_t rows, int64_t k, int v, int lpr) {
   return dim3((unsigned)g, (unsigned)((k + (int64_t)lpr * v - 1) / ((int64_t)lpr * v)), 1);
 }
 
-// Lanes per row of the level passes: pack_lpr's choice l, or 12 (5 rows a
-// wavefront, 4 lanes idle) when that puts more of the lanes to work.  A row
-// of `lanes` lanes of work uses lanes / (tiles * l) of its lanes at width l
-// and 5 lanes / (64 tiles) at width 12: for k = 48 in FP32 (12 lanes of 4
-// columns) 0.75 at 16 against 0.94 at 12.
-static int hsc_lpr(int64_t lanes) {
-  const int l = pack_lpr(lanes);
-  const int64_t tl = (lanes + l - 1) / l, t12 = (lanes + 11) / 12;
-  return 5 * tl * l > 64 * t12 ? 12 : l;
-}
-
 // launch WIDE(V, L) for the lanes per row L of vector width V
 #define HSC_LPR(lpr, WIDE, V)                                                 \
   do {                                                                        \
@@ -245,7 +234,7 @@ static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
 #undef N_
     return;
   }
-  const int lpr = hsc_lpr((k + v - 1) / v);
+  const int lpr = pack_lpr12((k + v - 1) / v);
   const dim3 g = hsc_grid(L.nC, k, v, lpr);
 #define D_(VV, LL) kw_hsc_down<T, VV, LL><<<g, kBlock, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(), L.r_colg.get(), LvVals<T>::r(L), k, src, Z, ctl)
   if constexpr (sizeof(T) == 4) {
@@ -266,7 +255,7 @@ static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
 #undef N_
     return;
   }
-  const int lpr = hsc_lpr((k + v - 1) / v);
+  const int lpr = pack_lpr12((k + v - 1) / v);
   const dim3 g = hsc_grid(L.nF, k, v, lpr);
 #define U_(VV, LL) kw_hsc_up<T, VV, LL><<<g, kBlock, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L), L.p_ptr.get(), L.p_colg.get(), LvVals<T>::p(L), k, src, Z, ctl)
   if constexpr (sizeof(T) == 4) {

@@ -189,10 +189,11 @@ __device__ __forceinline__ void gather_narrow(int32_t a, int32_t b, int g,
 // (k / VEC of 9..32 lanes) would otherwise leave most of a 64-lane tile
 // idle.  With LPR < 64 a row index is no longer wave-uniform and row data
 // is read with vector loads; LPR = 64 is the one-row-per-wave form.
-// LPR = 12 (5 rows a wavefront, lanes 60-63 idle) serves only kernels
-// without block partials (the HSC level passes): its idle lanes take the
-// wave's last row slot and a column past every k, so they load in bounds
-// and store nothing.
+// LPR = 12: 5 rows a wavefront, lanes 60-63 idle.  The idle lanes take the
+// wave's last row slot and a column past every k, so in a kernel that
+// guards its column accesses with `valid` (or clamps them to k - VEC) they
+// load in bounds and store nothing, and block_partials_wide leaves their
+// partials out.
 template <int LPR>
 struct Pack {
   static_assert(LPR == 12 || LPR == 16 || LPR == 32 || LPR == 64, "lanes per row");
@@ -228,16 +229,26 @@ inline int pack_lpr(int64_t lanes) {
   return best;
 }
 
+// pack_lpr's width l, or 12 when that puts more of the lanes to work: a row
+// of `lanes` lanes of work uses lanes / (tiles * l) of its lanes at width l
+// and 5 lanes / (64 tiles) at width 12 (for 12 lanes, k = 48 in FP32: 0.75
+// at 16, 0.94 at 12).
+inline int pack_lpr12(int64_t lanes) {
+  const int l = pack_lpr(lanes);
+  const int64_t tl = (lanes + l - 1) / l, t12 = (lanes + 11) / 12;
+  return 5 * tl * l > 64 * t12 ? 12 : l;
+}
+
 // Block reduction of per-lane FP64 partials for the wide mapping:
 // part[(lane % LPR)*VEC+m] of each row group summed over the 4 waves and
-// the wave's 64 / LPR row groups (waves outer, groups inner: a fixed order).
+// the wave's 64 / LPR row groups (waves outer, groups inner: a fixed order;
+// for LPR = 12 the idle lanes 60-63 form no group).
 // `sh` holds kWaves * NP * 64 * VEC doubles of shared memory.
 template <int VEC, int NP, int LPR = 64>
 __device__ __forceinline__ void block_partials_wide_in(double (&p)[NP][VEC],
                                                        double *__restrict__ out,
                                                        int64_t k, int64_t c0,
                                                        int64_t stride_np, double *sh) {
-  static_assert(64 % LPR == 0, "partials need whole row groups");
   constexpr int TW = 64 * VEC, G = 64 / LPR;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll

@@ -1562,7 +1562,7 @@ static unsigned resident_blocks() {
 // lanes per row of the packed wide mapping
 template <typename T>
 static int wide_lpr(int64_t k) {
-  return pack_lpr((k + wide_vec<T>(k) - 1) / wide_vec<T>(k));
+  return pack_lpr12((k + wide_vec<T>(k) - 1) / wide_vec<T>(k));
 }
 
 template <typename T>
@@ -1607,7 +1607,8 @@ static const unsigned kCapLevel = 512;
 // L the lanes per row
 #define WIDE_LPR(lpr, WIDE, V)                                                \
   do {                                                                        \
-    if ((lpr) == 16) { WIDE(V, 16); }                                         \
+    if ((lpr) == 12) { WIDE(V, 12); }                                         \
+    else if ((lpr) == 16) { WIDE(V, 16); }                                    \
     else if ((lpr) == 32) { WIDE(V, 32); }                                    \
     else { WIDE(V, 64); }                                                     \
   } while (0)
