@@ -2,9 +2,10 @@
 12, 16 or 32 lanes per row: a few PCG iterations against the same iterations
 in numpy over the oracle's SpMM and hierarchy, and converged solves.
 
-The lanes per row follow from k: k = 5, 12, 48, 144 pack 12 lanes per row (5
-rows a wavefront, 4 lanes idle) in FP32 and FP64, k = 27 and 190 pack 32 (190
-in three column tiles), k = 200 keeps 64 in FP32 and packs 16 in FP64.
+The lanes per row follow from k: k = 5, 12, 48, 144 in FP32 and k = 5, 12 in
+FP64 pack 12 lanes per row (5 rows a wavefront, 4 lanes idle); k = 27 and 190
+pack 32 (190 in three column tiles), and so does k = 48 in FP64; k = 144 and
+200 in FP64 pack 16; k = 200 in FP32 keeps 64.
 """
 import numpy as np
 import pytest

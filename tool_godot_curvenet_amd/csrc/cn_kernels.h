@@ -229,14 +229,17 @@ inline int pack_lpr(int64_t lanes) {
   return best;
 }
 
-// pack_lpr's width l, or 12 when that puts more of the lanes to work: a row
-// of `lanes` lanes of work uses lanes / (tiles * l) of its lanes at width l
-// and 5 lanes / (64 tiles) at width 12 (for 12 lanes, k = 48 in FP32: 0.75
-// at 16, 0.94 at 12).
+// pack_lpr's width l, or 12 when that puts more of the lanes to work
+// without more column tiles: a row of `lanes` lanes of work uses
+// lanes / (tiles * l) of its lanes at width l and 5 lanes / (64 tiles) at
+// width 12 (for 12 lanes, k = 48 in FP32: 0.75 at 16, 0.94 at 12).  More
+// tiles re-read each row's index data and start more row chains: 24 lanes
+// (k = 48 in FP64) in two 12-lane tiles ran the FP64 residual 15 % slower
+// than in one 32-lane tile.
 inline int pack_lpr12(int64_t lanes) {
   const int l = pack_lpr(lanes);
   const int64_t tl = (lanes + l - 1) / l, t12 = (lanes + 11) / 12;
-  return 5 * tl * l > 64 * t12 ? 12 : l;
+  return t12 <= tl && 5 * tl * l > 64 * t12 ? 12 : l;
 }
 
 // Block reduction of per-lane FP64 partials for the wide mapping:
