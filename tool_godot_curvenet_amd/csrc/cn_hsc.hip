@@ -89,6 +89,36 @@ kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv
 
 // ---- narrow form (k <= 4): one thread per row, its K columns in registers
 // (the row's index entries are read once for all columns) ---------------
+// acc[c] += sum_{e in [a,b)} val[e] X[col[e] K + c], CH entries at a time:
+// a chunk's indices, then all its gathers, are issued back to back; the
+// tail of a chunk re-reads the last entry with weight 0 (fma(0, x, acc) is
+// acc), so the sum is the in-order one of the plain loop.
+template <typename T, int K, int CH>
+__device__ __forceinline__ void gather_rows_n(int32_t a, int32_t b,
+                                              const int32_t *__restrict__ col,
+                                              const T *__restrict__ val, const T *X,
+                                              T (&acc)[K]) {
+  for (int32_t e = a; e < b; e += CH) {
+    int32_t j[CH];
+    T v[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int32_t q = min(e + u, b - 1);
+      j[u] = col[q];
+      v[u] = (e + u < b) ? val[q] : T(0);
+    }
+    T x[CH][K];
+#pragma unroll
+    for (int u = 0; u < CH; ++u)
+#pragma unroll
+      for (int c = 0; c < K; ++c) x[u][c] = X[(int64_t)j[u] * K + c];
+#pragma unroll
+    for (int u = 0; u < CH; ++u)
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[c] = fma(v[u], x[u][c], acc[c]);
+  }
+}
+
 template <typename T, int K>
 __global__ void __launch_bounds__(256)
 k_hsc_down_n(int32_t nC, const int32_t *__restrict__ Cg, const int32_t *__restrict__ rptr,
@@ -101,13 +131,7 @@ k_hsc_down_n(int32_t nC, const int32_t *__restrict__ Cg, const int32_t *__restri
   T acc[K];
 #pragma unroll
   for (int c = 0; c < K; ++c) acc[c] = src[o + c];
-  const int32_t e1 = rptr[r + 1];
-  for (int32_t e = rptr[r]; e < e1; ++e) {
-    const T w = rval[e];
-    const int64_t q = (int64_t)rcol[e] * K;
-#pragma unroll
-    for (int c = 0; c < K; ++c) acc[c] = fma(w, src[q + c], acc[c]);
-  }
+  gather_rows_n<T, K, 4>(rptr[r], rptr[r + 1], rcol, rval, src, acc);
 #pragma unroll
   for (int c = 0; c < K; ++c) Z[o + c] = acc[c];
 }
@@ -125,13 +149,7 @@ k_hsc_up_n(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ din
   T acc[K];
 #pragma unroll
   for (int c = 0; c < K; ++c) acc[c] = d * src[o + c];
-  const int32_t e1 = pptr[f + 1];
-  for (int32_t e = pptr[f]; e < e1; ++e) {
-    const T w = pval[e];
-    const int64_t q = (int64_t)pcol[e] * K;
-#pragma unroll
-    for (int c = 0; c < K; ++c) acc[c] = fma(w, Z[q + c], acc[c]);
-  }
+  gather_rows_n<T, K, 8>(pptr[f], pptr[f + 1], pcol, pval, Z, acc);
 #pragma unroll
   for (int c = 0; c < K; ++c) Z[o + c] = acc[c];
 }
