@@ -901,28 +901,66 @@ kb_bwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
 }
 
 // ---- (FUSED) r -= alpha q with partial r.r: the first half of a
-//      preconditioner applied as a whole (HSC), whose r.z comes after it ----
+//      preconditioner applied as a whole (HSC), whose r.z comes after it.
+// Flat form for k <= 4 (kf_rupd), wide form for k > 4 (kw_rupd).
+// ---- flat form of the narrow streaming kernels (k <= 4): the n x K array
+// is walked as one flat range, lane after lane, so a wavefront reads 64
+// consecutive values; a block of kFlat(K) threads (a multiple of K) and a
+// grid stride that is a multiple of K keep each thread on one column.  The
+// grid is the narrow row grid's block count, so the partial rows match.
+__host__ __device__ constexpr int kFlat(int K) { return K == 3 ? 768 : 1024; }
+
+// per-block column sums of the threads' partials, in thread order
+template <int K, int NP>
+__device__ __forceinline__ void flat_partials(const double (&p)[NP], double *__restrict__ out,
+                                              int64_t stride_np) {
+  constexpr int B = kFlat(K);
+  __shared__ double sh[NP][B];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) sh[q][threadIdx.x] = p[q];
+  __syncthreads();
+  if (threadIdx.x < NP * K) {
+    const int q = threadIdx.x / K, c = threadIdx.x % K;
+    double s = 0.0;
+    for (int t = c; t < B; t += K) s += sh[q][t];
+    out[q * stride_np + (int64_t)blockIdx.x * K + c] = s;
+  }
+}
+
+// (FUSED) r -= alpha q; partial r.r
 template <typename T, int K, bool FUSED>
-__global__ void __launch_bounds__(kBlock)
-kn_rupd(int32_t n, const double *__restrict__ alpha, SegMap sm, const T *__restrict__ Q,
+__global__ void __launch_bounds__(1024)
+kf_rupd(int32_t n, const double *__restrict__ alpha, SegMap sm, const T *__restrict__ Q,
         T *__restrict__ R, double *__restrict__ part, const int32_t *__restrict__ ctl) {
   if (ctl_idle(ctl)) return;
-  const int g = threadIdx.x % kGroup;
+  constexpr int B = kFlat(K);
+  const int c = threadIdx.x % K;
+  const int64_t total = (int64_t)n * K, S = (int64_t)gridDim.x * B;
+  int64_t i = (int64_t)blockIdx.x * B + threadIdx.x, row = i / K;
+  const T al0 = FUSED ? (T)alpha[c] : T(0);
   double pd[1] = {0.0};
-  const T al0 = (FUSED && g < K) ? (T)alpha[g] : T(0);
-  NARROW_LOOP(row, 0, n) {
-    if (g < K) {
-      const int64_t o = row * K + g;
-      T r = R[o];
-      if (FUSED) {
-        const T al = sm.of_row ? (T)alpha[sm.vc(row, K, g)] : al0;
-        r = fma(-al, Q[o], r);
-        R[o] = r;
-      }
-      pd[0] += (double)r * (double)r;
+  for (; i < total; i += S, row += S / K) {
+    T r = R[i];
+    if (FUSED) {
+      const T al = sm.of_row ? (T)alpha[sm.vc(row, K, c)] : al0;
+      r = fma(-al, Q[i], r);
+      R[i] = r;
     }
+    pd[0] += (double)r * (double)r;
   }
-  block_partials_narrow<K, 1>(pd, part, 0);
+  flat_partials<K, 1>(pd, part, 0);
+}
+
+// partial a.b (k <= 4)
+template <typename T, int K>
+__global__ void __launch_bounds__(1024)
+kf_dot(int32_t n, const T *__restrict__ A, const T *__restrict__ Bv, double *__restrict__ part) {
+  constexpr int B = kFlat(K);
+  const int64_t total = (int64_t)n * K, S = (int64_t)gridDim.x * B;
+  double pd[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * B + threadIdx.x; i < total; i += S)
+    pd[0] += (double)A[i] * (double)Bv[i];
+  flat_partials<K, 1>(pd, part, 0);
 }
 
 template <typename T, int VEC, bool FUSED, int LPR>
@@ -1025,18 +1063,6 @@ kw_jac(int32_t n, const T *__restrict__ dinv, const double *__restrict__ alpha,
 }
 
 // ---- generic two-vector column dot: partial a.b -------------------------
-template <typename T, int K>
-__global__ void __launch_bounds__(kBlock)
-kn_dot(int32_t n, const T *__restrict__ A, const T *__restrict__ B,
-       double *__restrict__ part) {
-  const int g = threadIdx.x % kGroup;
-  double pd[1] = {0.0};
-  NARROW_LOOP(row, 0, n) {
-    if (g < K) pd[0] += (double)A[row * K + g] * (double)B[row * K + g];
-  }
-  block_partials_narrow<K, 1>(pd, part, 0);
-}
-
 template <typename T, int VEC, int LPR>
 __global__ void __launch_bounds__(kBlock)
 kw_dot(int32_t n, int64_t k, const T *__restrict__ A, const T *__restrict__ B,
@@ -1843,7 +1869,7 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
   if (V.hsc || V.sgs || V.sm.of_row) {
     Grid G = grid_rows<T>(V.n, k, kCapSpmm);
     ProfScope ps("fwd", s);
-#define N_(KK) kn_rupd<T, KK, FUSED><<<G.g, kBlock, 0, s>>>(V.n, alpha, V.sm, Q, R, part, ctl)
+#define N_(KK) kf_rupd<T, KK, FUSED><<<G.g, kFlat(KK), 0, s>>>(V.n, alpha, V.sm, Q, R, part, ctl)
 #define W_(VV, LL) kw_rupd<T, WideV<T, VV>::v, FUSED, LL><<<G.g, kBlock, 0, s>>>(V.n, alpha, V.sm, k, Q, R, part, ctl)
     DISPATCH_K(k, T, G.lpr, N_, W_);
 #undef N_
@@ -1973,7 +1999,7 @@ template <typename T>
 static int32_t launch_dot(int32_t n, int64_t k, const T *A, const T *B, double *part,
                           hipStream_t s) {
   Grid G = grid_rows<T>(n, k, kCapSpmm);
-#define N_(KK) kn_dot<T, KK><<<G.g, kBlock, 0, s>>>(n, A, B, part)
+#define N_(KK) kf_dot<T, KK><<<G.g, kFlat(KK), 0, s>>>(n, A, B, part)
 #define W_(VV, LL) kw_dot<T, WideV<T, VV>::v, LL><<<G.g, kBlock, 0, s>>>(n, k, A, B, part)
   DISPATCH_K(k, T, G.lpr, N_, W_);
 #undef N_
