@@ -1225,23 +1225,25 @@ __device__ __forceinline__ RowIter seg_rows(const int32_t *__restrict__ seg_ptr,
   return RowIter{seg_ptr[sg] + lb * slots + slot, seg_ptr[sg + 1], nb * slots};
 }
 
-// partials of a.b (and, NP = 2, of c.d at part + stride)
+// partials of a.b (and, NP = 2, of c.d at part + stride).  Flat form for
+// k <= 4: the block's share of its segment's n_s x K range, walked lane
+// after lane with a stride of (blocks of the segment) x kFlat(K).
 template <typename TA, typename TB, int K, int NP>
-__global__ void __launch_bounds__(kBlock)
-kn_segdot(const int32_t *__restrict__ seg_ptr, const int32_t *__restrict__ blk_ptr,
+__global__ void __launch_bounds__(1024)
+kf_segdot(const int32_t *__restrict__ seg_ptr, const int32_t *__restrict__ blk_ptr,
           const int32_t *__restrict__ blk_seg, const TA *__restrict__ a,
           const TB *__restrict__ b, const TA *__restrict__ c, const TB *__restrict__ d,
           double *__restrict__ part, int64_t stride) {
-  const int g = threadIdx.x % kGroup;
+  constexpr int B = kFlat(K);
+  const int32_t sg = blk_seg[blockIdx.x];
+  const int32_t lb = blockIdx.x - blk_ptr[sg], nb = blk_ptr[sg + 1] - blk_ptr[sg];
+  const int64_t end = (int64_t)seg_ptr[sg + 1] * K, S = (int64_t)nb * B;
   double pd[NP] = {};
-  const RowIter it = seg_rows(seg_ptr, blk_ptr, blk_seg, kBlock / kGroup, threadIdx.x / kGroup);
-  for (int32_t row = it.first; row < it.end; row += it.stride)
-    if (g < K) {
-      const int64_t o = (int64_t)row * K + g;
-      pd[0] += (double)a[o] * (double)b[o];
-      if (NP == 2) pd[NP - 1] += (double)c[o] * (double)d[o];
-    }
-  block_partials_narrow<K, NP>(pd, part, stride);
+  for (int64_t i = (int64_t)seg_ptr[sg] * K + (int64_t)lb * B + threadIdx.x; i < end; i += S) {
+    pd[0] += (double)a[i] * (double)b[i];
+    if (NP == 2) pd[NP - 1] += (double)c[i] * (double)d[i];
+  }
+  flat_partials<K, NP>(pd, part, stride);
 }
 
 template <typename TA, typename TB, int NP>
@@ -1301,22 +1303,34 @@ __global__ void k_p_update(int64_t total, int64_t k, const double *__restrict__ 
     P[i] = fma((T)beta[i % k], P[i], Z[i]);
 }
 
-// Jacobi second half: d += alpha p; p = gamma z + beta p
-template <typename T>
-__global__ void k_pd_update(int64_t total, int64_t k, SegMap sm, const double *__restrict__ st,
-                            const T *__restrict__ Z, T *__restrict__ P,
-                            T *__restrict__ D, const int32_t *__restrict__ ctl) {
+// d += alpha p; p = gamma z + beta p (gamma 0 and beta 1 freeze a finished
+// column).  Flat form for k <= 4: each thread on one column, the row tracked
+// across the grid stride.
+template <typename T, int K>
+__global__ void __launch_bounds__(1024)
+kf_pd_update(int32_t n, SegMap sm, const double *__restrict__ st, const T *__restrict__ Z,
+             T *__restrict__ P, T *__restrict__ D, const int32_t *__restrict__ ctl) {
   if (bwd_idle(ctl)) return;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t row = i / k, v = sm.vc(row, k, i - row * k);
-    T p = P[i];
-    D[i] = fma((T)st[ST_ALPHA * sm.kk + v], p, D[i]);
-    P[i] = fma((T)st[ST_BETA * sm.kk + v], p, (T)st[ST_GAMMA * sm.kk + v] * Z[i]);
+  constexpr int B = kFlat(K);
+  const int c = threadIdx.x % K;
+  const int64_t total = (int64_t)n * K, S = (int64_t)gridDim.x * B;
+  int64_t i = (int64_t)blockIdx.x * B + threadIdx.x, row = i / K;
+  T al = (T)st[ST_ALPHA * sm.kk + c], be = (T)st[ST_BETA * sm.kk + c],
+    ga = (T)st[ST_GAMMA * sm.kk + c];
+  for (; i < total; i += S, row += S / K) {
+    if (sm.of_row) {
+      const int64_t v = sm.vc(row, K, c);
+      al = (T)st[ST_ALPHA * sm.kk + v];
+      be = (T)st[ST_BETA * sm.kk + v];
+      ga = (T)st[ST_GAMMA * sm.kk + v];
+    }
+    const T p = P[i];
+    D[i] = fma(al, p, D[i]);
+    P[i] = fma(be, p, ga * Z[i]);
   }
 }
 
-// Wide form of k_pd_update (same arithmetic): LPR lanes per row, VEC
+// Wide form of the update (same arithmetic) for k > 4: LPR lanes per row, VEC
 // columns a lane, the column scalars held in registers across rows (per row
 // only for a batch, whose scalars depend on the row's system).
 template <typename T, int VEC, int LPR>
@@ -1939,10 +1953,8 @@ static void launch_bwd(const SysView<T> &V, int64_t k, const double *st, T *Z, T
                        T *D, const int32_t *ctl, hipStream_t s) {
   if (!V.ic0) {
     ProfScope ps("bwd", s);
-    const int64_t total = (int64_t)V.n * k;
-    unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192));
     const Grid G = grid_rows<T>(V.n, k, kCapSpmm);
-#define N_(KK) k_pd_update<T><<<g, 256, 0, s>>>(total, k, V.sm, st, Z, P, D, ctl)
+#define N_(KK) kf_pd_update<T, KK><<<G.g, kFlat(KK), 0, s>>>(V.n, V.sm, st, Z, P, D, ctl)
 #define W_(VV, LL) kw_pd_update<T, WideV<T, VV>::v, LL><<<G.g, kBlock, 0, s>>>(V.n, k, V.sm, st, Z, P, D, ctl)
     DISPATCH_K(k, T, G.lpr, N_, W_);
 #undef N_
@@ -2021,9 +2033,9 @@ static void launch_segdot(const cn_system_t *S, SolverWork *W, int64_t k, const 
   if (k <= 4) {
 #define N_(KK)                                                                              \
   if (np == 2)                                                                              \
-    kn_segdot<TA, TB, KK, 2><<<g, kBlock, 0, s>>>(sp, bp, bs, a, b, c, d, part, stride);    \
+    kf_segdot<TA, TB, KK, 2><<<g, kFlat(KK), 0, s>>>(sp, bp, bs, a, b, c, d, part, stride); \
   else                                                                                      \
-    kn_segdot<TA, TB, KK, 1><<<g, kBlock, 0, s>>>(sp, bp, bs, a, b, c, d, part, stride);
+    kf_segdot<TA, TB, KK, 1><<<g, kFlat(KK), 0, s>>>(sp, bp, bs, a, b, c, d, part, stride);
     switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
 #undef N_
   } else if (np == 2) {
