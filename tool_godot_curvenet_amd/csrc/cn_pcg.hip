@@ -910,19 +910,32 @@ kb_bwd(int32_t b0, int32_t b1, const int32_t *__restrict__ blk_ptr,
 // grid is the narrow row grid's block count, so the partial rows match.
 __host__ __device__ constexpr int kFlat(int K) { return K == 3 ? 768 : 1024; }
 
-// per-block column sums of the threads' partials, in thread order
+// per-block column sums of the threads' partials in a fixed two-stage
+// order: runs of 16 threads of a column, then the runs in order (a serial
+// walk over the 256-1024 threads of a column would cost microseconds of
+// LDS latency a launch)
 template <int K, int NP>
 __device__ __forceinline__ void flat_partials(const double (&p)[NP], double *__restrict__ out,
                                               int64_t stride_np) {
-  constexpr int B = kFlat(K);
+  constexpr int B = kFlat(K), M = B / K, R = 16, M2 = M / R;
   __shared__ double sh[NP][B];
+  __shared__ double s2[NP][K * M2];
 #pragma unroll
   for (int q = 0; q < NP; ++q) sh[q][threadIdx.x] = p[q];
+  __syncthreads();
+  if (threadIdx.x < NP * K * M2) {
+    const int q = threadIdx.x / (K * M2), r = threadIdx.x % (K * M2);
+    const int c = r % K, m2 = r / K;
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < R; ++u) s += sh[q][c + K * (m2 * R + u)];
+    s2[q][r] = s;
+  }
   __syncthreads();
   if (threadIdx.x < NP * K) {
     const int q = threadIdx.x / K, c = threadIdx.x % K;
     double s = 0.0;
-    for (int t = c; t < B; t += K) s += sh[q][t];
+    for (int m2 = 0; m2 < M2; ++m2) s += s2[q][c + K * m2];
     out[q * stride_np + (int64_t)blockIdx.x * K + c] = s;
   }
 }
