@@ -50,7 +50,7 @@ def test_batch_is_block_diagonal(gpu, batch):
 
 
 @pytest.mark.parametrize("precond", ["jacobi", "sgs"])
-@pytest.mark.parametrize("k", [3, 40])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 40])
 def test_two_step_iterates_match_members(gpu, batch, precond, k):
     """After two FP64 iterations every member's rows equal its own solve's
     iterate (alpha, beta per member and column: shared scalars would give
