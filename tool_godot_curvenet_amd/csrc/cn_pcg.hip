@@ -2273,18 +2273,25 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     CN_HIP(hipMemsetAsync(X, 0, sizeof(double) * total, s));
   int32_t rows;
   Red rd;
+  // the FP64 residual the first round scales into R: from x0 = 0 it is b
+  // itself (b - A 0 = b exactly), so no SpMM, and ||r|| is ||b||
+  const double *rsrc = o->warm_start ? R64 : d_b;
   if (batch) {
     launch_segdot<double, double>(S, W, k, d_b, d_b, nullptr, nullptr, s);
     k_reduce_plain<<<red, kBlock, 0, s>>>(1, kk, sg0, st + ST_BB * kk);
-    launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
-    launch_segdot<double, double>(S, W, k, R64, R64, nullptr, nullptr, s);
+    if (o->warm_start) {
+      launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
+      launch_segdot<double, double>(S, W, k, R64, R64, nullptr, nullptr, s);
+    }
     k_reduce_plain<<<red, kBlock, 0, s>>>(1, kk, sg0, st + ST_RR * kk);
   } else {
     rows = launch_dot<double>(n, k, d_b, d_b, p0, s);
     rd = reduce_rows(W, p0, rows, k, stride, 1, s);
     k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_BB * k);
-    rows = launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
-    rd = reduce_rows(W, p0, rows, k, stride, 1, s);
+    if (o->warm_start) {
+      rows = launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
+      rd = reduce_rows(W, p0, rows, k, stride, 1, s);
+    }
     k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
   }
   CN_HIP(hipMemsetAsync(D, 0, sizeof(T) * total, s));
@@ -2353,7 +2360,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     CN_HIP(hipMemsetAsync(st + ST_ALPHA * kk, 0, sizeof(double) * kk, s));
     // after the first round the residual kernel writes the scaled R itself
     if (!r_scaled)
-      launch_scale_in<T>(V, k, st + ST_S * kk, R64, R, s);
+      launch_scale_in<T>(V, k, st + ST_S * kk, rsrc, R, s);
     if (apply_m) {
       // z = M^-1 r; rz; continuing columns keep p, restarted ones take p = z
       rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, s);
@@ -2436,6 +2443,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     if (batch) {
       // FP64 residual, per-segment r.r, scaled into R at the top of the loop
       launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
+      rsrc = R64;
       launch_segdot<double, double>(S, W, k, R64, R64, nullptr, nullptr, s);
       k_reduce_plain<<<red, kBlock, 0, s>>>(1, kk, sg0, st + ST_RR * kk);
     } else {
