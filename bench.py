@@ -124,8 +124,8 @@ def main():
         traffic_from(*args.traffic_from, copy_mb=args.copy_mb or 1024)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(sharding is described in DESIGN.md, not implemented)"}))
+        print(json.dumps({"error": "the solver runs on one GPU; --gpus must be 1 "
+                          "(sharding across GPUs is not implemented)"}))
         sys.exit(2)
 
     import numpy as np
