@@ -300,77 +300,6 @@ kw_resid(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__
   block_partials_wide<VEC, 1, LPR>(pd, part, k, c0, 0);
 }
 
-// Reliable update and residual in one pass (single systems): with
-// X' = X + invs_c D (FP64 fma), writes X' into X2, zeros into D2 and
-// R = scale_c (b - A X') into R, partial r.r of the unscaled residual.
-// Neighbour values of X' are formed from X and D as they are gathered, so
-// nothing is read that this launch writes; the result is bit-identical to
-// k_x_update followed by kw_resid (same fma order, same chunking).
-template <int VEC, typename TO, int LPR>
-__global__ void __launch_bounds__(kBlock)
-kw_resid_x(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-           const double *__restrict__ val, int64_t k, const double *__restrict__ X,
-           const TO *__restrict__ D, const double *__restrict__ invs, double *__restrict__ X2,
-           TO *__restrict__ D2, const double *__restrict__ B, TO *__restrict__ R,
-           const double *__restrict__ scale, double *__restrict__ part) {
-  constexpr int CH = 8;
-  WIDE_COLS_P(VEC, LPR)
-  const int64_t cl = valid ? c0 : k - VEC;
-  double pd[1][VEC] = {};
-  double sc[VEC], iv[VEC];
-#pragma unroll
-  for (int m = 0; m < VEC; ++m) {
-    sc[m] = valid ? scale[c0 + m] : 1.0;
-    iv[m] = valid ? invs[c0 + m] : 0.0;
-  }
-  WIDE_LOOP_P(row, 0, n, LPR) {
-    const int32_t r = Pack<LPR>::u(row);
-    const int32_t a = Pack<LPR>::u(ptr[r]), b = Pack<LPR>::u(ptr[r + 1]);
-    double acc[VEC] = {};
-    for (int32_t e = a; e < b; e += CH) {
-      int32_t j[CH];
-      double v[CH];
-#pragma unroll
-      for (int u = 0; u < CH; ++u) {
-        const int32_t q = min(e + u, b - 1);
-        j[u] = col[q];
-        v[u] = (e + u < b) ? val[q] : 0.0;
-      }
-      double x[CH][VEC];
-      TO d[CH][VEC];
-#pragma unroll
-      for (int u = 0; u < CH; ++u) {
-        VecIO<double, VEC>::ld(X + (int64_t)j[u] * k + cl, x[u]);
-        VecIO<TO, VEC>::ld(D + (int64_t)j[u] * k + cl, d[u]);
-      }
-#pragma unroll
-      for (int u = 0; u < CH; ++u)
-#pragma unroll
-        for (int m = 0; m < VEC; ++m)
-          acc[m] = fma(v[u], fma(iv[m], (double)d[u][m], x[u][m]), acc[m]);
-    }
-    double xo[VEC], bb[VEC], xn[VEC];
-    TO dd[VEC], out[VEC], zero[VEC];
-    VecIO<double, VEC>::ld(X + (int64_t)r * k + cl, xo);
-    VecIO<TO, VEC>::ld(D + (int64_t)r * k + cl, dd);
-    VecIO<double, VEC>::ld(B + (int64_t)r * k + cl, bb);
-#pragma unroll
-    for (int m = 0; m < VEC; ++m) {
-      xn[m] = fma(iv[m], (double)dd[m], xo[m]);
-      const double res = bb[m] - acc[m];
-      pd[0][m] += res * res;
-      out[m] = (TO)(res * sc[m]);
-      zero[m] = TO(0);
-    }
-    if (valid) {
-      VecIO<double, VEC>::st(X2 + (int64_t)r * k + c0, xn);
-      VecIO<TO, VEC>::st(D2 + (int64_t)r * k + c0, zero);
-      VecIO<TO, VEC>::st(R + (int64_t)r * k + c0, out);
-    }
-  }
-  block_partials_wide<VEC, 1, LPR>(pd, part, k, c0, 0);
-}
-
 // ---- IC(0) forward pass over the blocks [b0, b1) of one level:
 //      (FUSED) r -= alpha q;  y = (r - sum L y) / L_ii;  partials r.r, y.y
 // A block's rows are walked in order by the same lanes that own the
@@ -1518,7 +1447,6 @@ struct SolverWork {
   DevBuf<double> seg;                 // 2 x nvc per-segment dot sums
   DevBuf<double> xp, rp;              // FP64 n x k: x, true residual
   DevBuf<double> buf[5];              // inner D, R, Z, P, Q (8 bytes/elem)
-  DevBuf<double> dd;                  // the other D of the fused reliable update
   DevBuf<double> part;                // 3 x part_rows x k
   DevBuf<double> red;                 // 2 x kRedRows x k, first-stage sums
   DevBuf<double> st;                  // ST_N x nvc per-column scalars
@@ -1557,7 +1485,6 @@ static SolverWork *ensure_work(cn_system_t *S, int32_t k, int32_t part_rows) {
   W->xp.alloc(nk);
   W->rp.alloc(nk);
   for (auto &b : W->buf) b.alloc(nk);
-  W->dd.alloc(nk);
   W->part.alloc((size_t)3 * pr * kk + 4);
   W->red.alloc((size_t)2 * kRedRows * kk + 4);
   W->st.alloc((size_t)ST_N * nvc + 4);
@@ -1758,28 +1685,6 @@ static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
 #undef N_
 #undef W_
   return G.rows;
-}
-
-// the fused reliable update + residual for k >= 256 (kw_resid_x); false
-// otherwise (the caller runs the two kernels)
-template <typename TO>
-static bool launch_resid_x(int32_t n, const int32_t *ptr, const int32_t *col,
-                           const double *val, int64_t k, const double *X, const TO *D,
-                           const double *invs, double *X2, TO *D2, const double *B, TO *R,
-                           const double *scale, double *part, hipStream_t s, int32_t *rows) {
-  // measured: at k = 768 the pass matches the two kernels (818 vs 823 us on
-  // the 100k capsule union); at k = 48 (2.46M rows) the two kernels are ahead
-  // by 1 % of the whole step, so narrower blocks keep them
-  if (k < 256) return false;
-  Grid G = grid_rows<double>(n, k, kCapSpmm);
-  ProfScope ps("resid", s);
-#define N_(KK) (void)0
-#define W_(VV, LL) kw_resid_x<WideV<double, VV>::v, TO, LL><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, k, X, D, invs, X2, D2, B, R, scale, part)
-  DISPATCH_K(k, double, G.lpr, N_, W_);
-#undef N_
-#undef W_
-  *rows = G.rows;
-  return true;
 }
 
 template <typename T>
@@ -2259,10 +2164,6 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   T *P = reinterpret_cast<T *>(W->buf[3].get());
   T *Q = reinterpret_cast<T *>(W->buf[4].get());
   double *X = W->xp.get(), *R64 = W->rp.get();
-  // the fused reliable update alternates X between xp and rp and D between
-  // buf[0] and dd
-  double *const X0 = X;
-  T *const D0 = D, *const D1 = reinterpret_cast<T *>(W->dd.get());
   const double theta = 0.5;  // iterated-residual target below tol
   const double delta2 = sizeof(T) == 4 ? 1e-3 : 1e-8;
 
@@ -2420,25 +2321,9 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     CN_HIP(hipGetLastError());
     launched += it;
     iters = hc[CTL_ITERS];
-    // ---- reliable update: x += d / s (FP64), r = b - A x (FP64)
-    // single systems with k > 4: one pass into the other X and D buffers
-    // (the FP64 residual buffer is free after the first round)
-    if (!batch) {
-      double *X2 = X == X0 ? R64 : X0;
-      T *D2 = D == D0 ? D1 : D0;
-      if (launch_resid_x<T>(n, V.ptr, V.col, V.val64, k, X, D, st + ST_INVS * k, X2, D2, d_b,
-                            R, st + ST_S * k, p0, s, &rows)) {
-        X = X2;
-        D = D2;
-        r_scaled = true;
-        rd = reduce_rows(W, p0, rows, k, stride, 1, s);
-        k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
-        CN_HIP(hipGetLastError());
-        ++updates;
-        if (it == 0) { status = CN_MAXIT; break; }
-        continue;
-      }
-    }
+    // ---- reliable update: x += d / s (FP64), d = 0; r = b - A x (FP64).
+    // (One pass that forms x + d / s for the gathered neighbours as well was
+    // measured 0.4 % slower a step at k = 768 than these two kernels.)
     launch_x_update<T>(V, k, st + ST_INVS * kk, D, X, s);
     if (batch) {
       // FP64 residual, per-segment r.r, scaled into R at the top of the loop
