@@ -2163,15 +2163,19 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   T *Z = reinterpret_cast<T *>(W->buf[2].get());
   T *P = reinterpret_cast<T *>(W->buf[3].get());
   T *Q = reinterpret_cast<T *>(W->buf[4].get());
-  double *X = W->xp.get(), *R64 = W->rp.get();
+  // x accumulates in the caller's d_x (no copy in or out) unless d_x is d_b,
+  // which is read until the end
+  double *X = d_x != d_b ? d_x : W->xp.get(), *R64 = W->rp.get();
   const double theta = 0.5;  // iterated-residual target below tol
   const double delta2 = sizeof(T) == 4 ? 1e-3 : 1e-8;
 
   // x0, true residual, ||b||
-  if (o->warm_start)
-    CN_HIP(hipMemcpyAsync(X, d_x, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
-  else
+  if (o->warm_start) {
+    if (X != d_x)
+      CN_HIP(hipMemcpyAsync(X, d_x, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
+  } else {
     CN_HIP(hipMemsetAsync(X, 0, sizeof(double) * total, s));
+  }
   int32_t rows;
   Red rd;
   // the FP64 residual the first round scales into R: from x0 = 0 it is b
@@ -2341,7 +2345,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     ++updates;
     if (it == 0) { status = CN_MAXIT; break; }
   }
-  CN_HIP(hipMemcpyAsync(d_x, X, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
+  if (X != d_x)
+    CN_HIP(hipMemcpyAsync(d_x, X, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
   CN_HIP(hipStreamSynchronize(s));
   if (info) {
     info->status = status;
