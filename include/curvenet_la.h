@@ -540,7 +540,8 @@ typedef struct cn_solve_info {
 /* Fill opts with defaults (IC0, MIXED, tol 1e-9, 20000 iters, inner 1e-5). */
 void cn_pcg_default_opts(cn_pcg_opts *opts);
 
-/* Solve A_ff X = B for k columns.  d_b / d_x are n_free x k FP64.
+/* Solve A_ff X = B for k columns.  d_b / d_x are n_free x k FP64; d_x
+ * holds the iterate during the solve and may be d_b itself (in place).
  * `ic0` may be NULL for Jacobi.  Promoted (sample) rows are not part of
  * A_ff; the caller overlays their values (DeformSolve.lean:108-109). */
 cn_status cn_pcg_solve(cn_system sys, cn_ic0 ic0, const cn_pcg_opts *opts,
