@@ -209,7 +209,8 @@ static int hsc_vec(int64_t k) {
 // 1024 a tile).  A row is a chain of dependent loads, so the passes need
 // many waves in flight: raising the cap from 1024 blocks a tile took the
 // down pass from 0.57 to 0.62 of peak at k = 768 (3 tiles) and the
-// icosphere9 step from 54.8 to 61.4 poses/s at k = 48 (1 tile).
+// step of the 2.46M-row icosphere from 54.8 to 61.4 poses/s at k = 48.  Totals of
+// 8192, 32768 and 65536 blocks measured within 0.3 % of 16384.
 static dim3 hsc_grid(int64_t rows, int64_t k, int v, int lpr) {
   const int slots = kWaves * (64 / lpr);
   int64_t g = (rows + slots - 1) / slots;
