@@ -111,7 +111,7 @@ def main():
     ap.add_argument("--workload", default="capsules100k",
                     choices=["capsules100k", "meshes64", "icosphere9", "ddm"],
                     help="capsules100k: 256 poses x 3 RHS on the 100k capsule union (the "
-                         "headline); meshes64: 64 uneven meshes x 3 RHS, batched vs one solve "
+                         "default); meshes64: 64 uneven meshes x 3 RHS, batched vs one solve "
                          "per mesh; icosphere9: 16 poses x 3 RHS forward + adjoint on the "
                          "2.6M-vertex icosphere with 64 loops; ddm: Direct Delta Mush bind "
                          "(harvest of every handle, smoothing, top-8) and the FP32 skinning "
