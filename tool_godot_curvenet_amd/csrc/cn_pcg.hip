@@ -247,25 +247,69 @@ __global__ void k_ell_fill(int32_t n, const int32_t *__restrict__ ptr,
 // ---- FP64 residual r = b - A x with partial r.r ------------------------
 // R = b - A x, or (scale != nullptr) R = scale[col] (b - A x) in the
 // inner-loop type TO; partial r.r of the unscaled residual.
+// Thread form (k <= 4): one thread per row, its K columns in registers; a
+// chunk of 8 entries' indices, then all their gathers, are issued at once
+// (the tail of a chunk re-reads the last entry with weight 0), so each
+// column's sum is the in-order one.  Block partials: runs of 16 threads,
+// then the 16 runs, in a fixed order.
 template <int K, typename TO>
 __global__ void __launch_bounds__(kBlock)
-kn_resid(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+kt_resid(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
          const double *__restrict__ val, const double *__restrict__ X,
          const double *__restrict__ B, TO *__restrict__ R, const double *__restrict__ scale,
          double *__restrict__ part) {
-  const int g = threadIdx.x % kGroup;
-  double pd[1] = {0.0};
-  const double sc = (scale && g < K) ? scale[g] : 1.0;
-  NARROW_LOOP(row, 0, n) {
+  constexpr int CH = 8;
+  double pd[K] = {};
+  double sc[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) sc[c] = scale ? scale[c] : 1.0;
+  for (int32_t row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock) {
     double acc[K] = {};
-    gather_narrow<double, K>(ptr[row], ptr[row + 1], g, col, val, X, acc);
-    if (g < K) {
-      double r = B[row * K + g] - pick<double, K>(acc, g);
-      R[row * K + g] = (TO)(scale ? r * sc : r);
-      pd[0] += r * r;
+    const int32_t a = ptr[row], b = ptr[row + 1];
+    for (int32_t e = a; e < b; e += CH) {
+      int32_t j[CH];
+      double v[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int32_t q = min(e + u, b - 1);
+        j[u] = col[q];
+        v[u] = (e + u < b) ? val[q] : 0.0;
+      }
+      double x[CH][K];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+#pragma unroll
+        for (int c = 0; c < K; ++c) x[u][c] = X[(int64_t)j[u] * K + c];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc[c] = fma(v[u], x[u][c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const double r = B[(int64_t)row * K + c] - acc[c];
+      R[(int64_t)row * K + c] = (TO)(scale ? r * sc[c] : r);
+      pd[c] += r * r;
     }
   }
-  block_partials_narrow<K, 1>(pd, part, 0);
+  __shared__ double sh[K][kBlock];
+  __shared__ double s2[K][kBlock / 16];
+#pragma unroll
+  for (int c = 0; c < K; ++c) sh[c][threadIdx.x] = pd[c];
+  __syncthreads();
+  if (threadIdx.x < K * (kBlock / 16)) {
+    const int c = threadIdx.x / (kBlock / 16), m = threadIdx.x % (kBlock / 16);
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) t += sh[c][m * 16 + u];
+    s2[c][m] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double t = 0.0;
+    for (int m = 0; m < kBlock / 16; ++m) t += s2[threadIdx.x][m];
+    part[(int64_t)blockIdx.x * K + threadIdx.x] = t;
+  }
 }
 
 template <int VEC, typename TO, int LPR>
@@ -1679,12 +1723,13 @@ static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
                             hipStream_t s) {
   Grid G = grid_rows<double>(n, k, kCapSpmm);
   ProfScope ps("resid", s);
-#define N_(KK) kn_resid<KK, TO><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, X, B, R, scale, part)
+  const unsigned gt = (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, G.g.x);
+#define N_(KK) kt_resid<KK, TO><<<std::max(1u, gt), kBlock, 0, s>>>(n, ptr, col, val, X, B, R, scale, part)
 #define W_(VV, LL) kw_resid<WideV<double, VV>::v, TO, LL><<<G.g, kBlock, 0, s>>>(n, ptr, col, val, k, X, B, R, scale, part)
   DISPATCH_K(k, double, G.lpr, N_, W_);
 #undef N_
 #undef W_
-  return G.rows;
+  return k <= 4 ? (int32_t)std::max(1u, gt) : G.rows;
 }
 
 template <typename T>
