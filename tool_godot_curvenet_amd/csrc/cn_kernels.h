@@ -307,21 +307,34 @@ __device__ __forceinline__ void block_partials_narrow(const double (&p)[NP],
 }
 
 // Per-thread partial reduction (thread t holds K columns) for the
-// thread-per-block kernels: fixed-order sum over the block's 256 threads.
+// thread-per-row kernels: fixed-order sum over the block's 256 threads.
 template <int K, int NP>
 __device__ __forceinline__ void block_partials_thread(const double (&p)[NP][K],
                                                       double *__restrict__ out,
                                                       int64_t stride_np) {
+  // fixed order in two stages: runs of 16 threads, then the 16 runs (a
+  // serial walk over 256 threads costs microseconds of LDS latency)
+  constexpr int RUNS = kBlock / 16;
   __shared__ double sh[NP][K][kBlock];
+  __shared__ double s2[NP][K][RUNS];
 #pragma unroll
   for (int q = 0; q < NP; ++q)
 #pragma unroll
     for (int c = 0; c < K; ++c) sh[q][c][threadIdx.x] = p[q][c];
   __syncthreads();
+  if (threadIdx.x < NP * K * RUNS) {
+    const int qc = threadIdx.x / RUNS, m = threadIdx.x % RUNS;
+    const int q = qc / K, c = qc % K;
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) t += sh[q][c][m * 16 + u];
+    s2[q][c][m] = t;
+  }
+  __syncthreads();
   if (threadIdx.x < NP * K) {
     const int q = threadIdx.x / K, c = threadIdx.x % K;
     double t = 0.0;
-    for (int i = 0; i < kBlock; ++i) t += sh[q][c][i];
+    for (int m = 0; m < RUNS; ++m) t += s2[q][c][m];
     out[q * stride_np + (int64_t)blockIdx.x * K + c] = t;
   }
 }
