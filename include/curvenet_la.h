@@ -102,6 +102,21 @@ cn_status cn_project_to_vertices(int32_t nv, const double *d_pos, int32_t nq,
                                  const double *d_q, int32_t *d_index,
                                  void *stream);
 
+/* SurfaceProjection.promoteVertexSamples (SurfaceProjection.lean:64-80):
+ * the column-assignment pass over projected knots.  Walking the knots in
+ * order, the first knot that lands on a vertex gives that vertex the next
+ * sample column; later knots on the same vertex share it; a knot with
+ * index < 0 (no vertex) gets -1.
+ *   d_index[nq]: vertex per knot (cn_project_to_vertices' output);
+ *   d_knot_to_col[nq]: column per knot (out);
+ *   d_vertex_to_col[nv]: column per vertex, -1 where no knot landed (out);
+ *   *h_ncols: number of columns.
+ * An index >= nv is CN_BAD_ARG (the spec's `vertexToCol[v]!` would panic).
+ * Synchronous. */
+cn_status cn_promote_vertex_samples(int32_t nv, int32_t nq, const int32_t *d_index,
+                                    int32_t *d_knot_to_col, int32_t *d_vertex_to_col,
+                                    int32_t *h_ncols, void *stream);
+
 /* ------------------------------------------------------------------------
  * Cut Laplacian system (L2): VtLhV of CutMeshLaplacian.assembleVtLhV
  * (CutMeshLaplacian.lean:56-93), stored sparse and split into the free
@@ -410,7 +425,10 @@ cn_status cn_deformation_gradients(int32_t n, const double *d_rest, const double
  * influences CSR d_start[nv + 1], d_idx, d_w; d_out P x nv x 3.  A handle
  * index outside [0, nt) reads the zero matrix.  The f64 form keeps the
  * spec's operation order (bit-exact); the f32 form is the shader's float
- * arithmetic with fused multiply-adds.
+ * arithmetic with fused multiply-adds.  The caller guarantees a consistent
+ * CSR (d_start[0] == 0, non-decreasing, d_start[nv] == entries of d_idx and
+ * d_w): the kernels read d_idx/d_w at every position in [start[v],
+ * start[v+1]) and do not re-check it (the Python layer does, on the host).
  * ---------------------------------------------------------------------- */
 cn_status cn_lbs_matvec_f64(int32_t nv, int32_t nt, int32_t P, const double *d_rest,
                             const double *d_T, const int32_t *d_start, const int32_t *d_idx,

@@ -26,29 +26,29 @@ PINS = {
     "subdivide_edge/split_face/insert_crack": "CutAlgorithm.lean:269-507 KATs",
     "cut_edges": "cylinder seam closed form (64 copies at nv+j), Euler/partition invariants, "
                  "brute-force wedge recount",
-    "cot/triangle/polygon Laplacian": "closed forms (equilateral -1/(2 sqrt 3)), row sums, symmetry",
-    "robust triangle Laplacian": "RobustLaplacian.lean:160-290 KATs (cot 60/30/90 degrees, "
+    "cot_at_vertex/triangle_cot_laplacian/polygon_cot_laplacian": "closed forms (equilateral -1/(2 sqrt 3)), row sums, symmetry",
+    "mollify_triangle/cot_from_lengths/robust_triangle_cot_laplacian": "RobustLaplacian.lean:160-290 KATs (cot 60/30/90 degrees, "
                                  "mollification margins, equilateral == embedded, finite on a "
                                  "collinear triangle); equal to the embedded cot on random "
                                  "non-degenerate triangles with delta = 0 (law of cosines + "
                                  "Heron vs cross product)",
-    "assemble_vtlhv_dense": "CutMeshLaplacian.lean:101-179 KATs, linear precision",
+    "assemble_vtlhv_dense/harmonic_rhs_dense": "CutMeshLaplacian.lean:101-179 KATs, linear precision",
     "cut_laplacian_csr": "bit-identical to the dense spec transcription on small cut meshes",
     "harmonic_rhs_csr": "bit-identical to the dense spec transcription; HarmonicSolve KATs",
     "solve_deformation_dense": "DeformSolve.lean:164-253 KATs (translation, rotation, scale on "
                                "triangleWithSample); EndToEndExample Bezier-driven translation",
-    "deform_rhs_csr": "bit-identical to the dense spec transcription (deform_rhs_dense); a "
+    "deform_rhs_csr/deform_rhs_dense": "bit-identical to the dense spec transcription (deform_rhs_dense); a "
                       "rigid motion is reproduced exactly by the two-stage solve",
     "bezier3/isolated_segment_gradient/deformation_gradient/mat3_inv": "ScaledFrames.lean and "
         "EndToEndExample.lean KATs (identity, rotation, scale, rotate+scale, det, inverse)",
-    "spmv/spmm": "identity KAT; numpy reference",
+    "spmv/spmm/dot/csr_diag": "identity KAT; numpy reference",
     "cg": "SparseLinAlg.lean:171-287 KATs; dense GE",
-    "ic0_factor": "LL^T == A on the kept pattern; IncompleteCholesky.lean:128-186 KATs; "
+    "ic0_factor/ic0_factor3/permute_csr": "LL^T == A on the kept pattern; IncompleteCholesky.lean:128-186 KATs; "
                   "full-pattern IC(0) == dense Cholesky",
     "ic0_solve": "LL^T x == b",
     "pcg": "dense GE / numpy solve",
     "pcg_many": "column for column equal to pcg (the same arithmetic on each thread)",
-    "greedy_color": "GraphColoring.lean:79-117 KATs, validity",
+    "greedy_color/color_classes": "GraphColoring.lean:79-117 KATs, validity",
     "sgs_sweep/sgs_apply": "equal to lexical Gauss-Seidel in colour order (the kernel's stated "
                            "equivalence); one sweep from 0 == the textbook SGS preconditioner "
                            "(D+L) D^-1 (D+U) solved densely; many sweeps converge to the "
@@ -57,8 +57,8 @@ PINS = {
                     "parents outside [0, n_coarse) ignored",
     "hsc_build/hsc_apply": "theta = 0: one level == dense Schur complement, exact hierarchy "
                            "apply == A^-1; lump/diag compensation invariants; planted sign",
-    "dense solve/LU/Cholesky": "DenseLinAlg.lean:294-489 KATs, numpy.linalg",
-    "hsc_eliminate_vertex": "HSC KATs (HierarchicalSparsifyCompensate.lean:134-231), "
+    "dense_solve/dense_solve_multi/lu_factor/lu_solve/cholesky_factor/cholesky_solve": "DenseLinAlg.lean:294-489 KATs, numpy.linalg",
+    "hsc_eliminate_vertex/hsc_edge_weight/hsc_degree_weight": "HSC KATs (HierarchicalSparsifyCompensate.lean:134-231), "
                             "dense Schur complement",
     "apply_transform4x4/lbs_matvec": "DirectDeltaMush.lean:115-165 and "
         "Features/KnownKnowns/DirectDeltaMush.lean KATs (identity, translation, blends, empty "
@@ -70,6 +70,15 @@ PINS = {
                      "K = width; smooth-then-sparsify partition of unity); a literal Python "
                      "selection sort with swaps on tie-heavy rows; top-K set == numpy "
                      "argsort on distinct values",
+    "project_to_vertices/projected_positions": "SurfaceProjection.lean:99-130 KATs (knot at "
+        "a vertex, nearest of three, all kinds vertex, empty knots, empty mesh -> -1 and the "
+        "knot's own position); numpy first-argmin of the squared distances on random clouds "
+        "with planted duplicate vertices and equidistant knots (lowest index wins); a planted "
+        "<= (last index wins) fails",
+    "promote_vertex_samples": "SurfaceProjection.lean:132-157 KATs (shared column, encounter "
+        "order, order matters, repeated knot); closed form on random projections: columns "
+        "are the ranks of np.unique(return_index) first occurrences; planted 'column = "
+        "vertex order' fails",
     "adjoint": "central finite differences (h = 1e-6) of a quadratic loss through a dense "
                "FP64 solve, in x_c and in every edge weight; planted sign / transpose "
                "mistakes fail them",
@@ -186,12 +195,53 @@ def project_to_vertices(knots, pos):
     for i, k in enumerate(knots):
         best, bi = 1e308, -1
         d = pos - k[None]
-        dd = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+        dd = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]).tolist()
         for v in range(pos.shape[0]):
             if dd[v] < best:
                 best, bi = dd[v], v
         out[i] = bi
     return out
+
+
+def projected_positions(knots, pos, index):
+    """The `position` field of SurfaceProjection.projectToVertices
+    (SurfaceProjection.lean:58): the chosen mesh vertex, or the knot itself
+    when no vertex was found (empty mesh, index -1)."""
+    knots = _d(np.asarray(knots).reshape(-1, 3))
+    pos = _d(np.asarray(pos).reshape(-1, 3))
+    index = np.asarray(index, np.int64)
+    out = knots.copy()
+    ok = index >= 0
+    out[ok] = pos[index[ok]]
+    return out
+
+
+def promote_vertex_samples(vertex_count, index, kinds=None):
+    """SurfaceProjection.promoteVertexSamples (SurfaceProjection.lean:64-80),
+    direct transcription: walking the projections in order, the first knot
+    that lands on a vertex gives that vertex the next column; later knots on
+    the same vertex share it; knots with no vertex (meshIndex < 0) or of a
+    non-vertex kind keep -1.  `kinds` (0 = vertex) defaults to all vertex,
+    the only kind projectToVertices produces.  Returns (knot_to_col,
+    vertex_to_col, ncols).  An index >= vertex_count is rejected (the spec's
+    `vertexToCol[v]!` would panic there)."""
+    index = [int(v) for v in np.asarray(index).reshape(-1)]
+    kinds = [0] * len(index) if kinds is None else [int(t) for t in kinds]
+    vertex_to_col = [-1] * int(vertex_count)
+    knot_to_col = [-1] * len(index)
+    next_col = 0
+    for i, v in enumerate(index):
+        if kinds[i] == 0 and v >= 0:
+            if v >= vertex_count:
+                raise IndexError(f"knot {i} projects to vertex {v} >= {vertex_count}")
+            cur = vertex_to_col[v]
+            if cur < 0:
+                vertex_to_col[v] = next_col
+                knot_to_col[i] = next_col
+                next_col += 1
+            else:
+                knot_to_col[i] = cur
+    return (np.asarray(knot_to_col, np.int32), np.asarray(vertex_to_col, np.int32), next_col)
 
 
 # ----------------------------------------------------------------- Laplacian --

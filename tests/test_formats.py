@@ -204,3 +204,73 @@ def test_reference_mesh_like_vector():
     assert pos.shape == (4, 3) and tris.shape == (2, 3)
     assert st.metadata["n_verts"] == "4" and st.metadata["n_tris"] == "2"
     assert np.allclose(st.get("weights").sum(axis=1), 1.0)
+
+
+class _FakeSystem:
+    """Enough of a solver system for the sidecar checks (no device): a
+    4-row CSR and the vertex count."""
+
+    def __init__(self):
+        from tool_godot_curvenet_amd.curvenet import CsrMatrix
+        self._A = CsrMatrix(4, 4, np.array([0, 1, 2, 3, 4], np.int32),
+                            np.arange(4, dtype=np.int32), np.full(4, 2.0))
+        self.n_free = 4
+
+        class _Info:
+            nv = 4
+        self.info = _Info()
+
+    def A_ff(self):
+        return self._A
+
+
+def _sidecar(tmp_path, system, tensors, levels=0):
+    F = _F()
+    meta = {"format": F.SIDECAR_FORMAT, "fingerprint": F.system_fingerprint(system),
+            "n_free": "4", "hsc_levels": str(levels), "hsc_theta": "0.015",
+            "hsc_coarse_max": "512", "hsc_max_levels": "24", "hsc_min_frac": "0.05",
+            "hsc_compensate": "0"}
+    path = tmp_path / "side.safetensors"
+    F.write_safetensors(str(path), tensors, meta)
+    return str(path)
+
+
+@pytest.mark.parametrize("bad", ["coarse_L", "coarse_inv"])
+@pytest.mark.parametrize("dtype", [np.float32, np.int32, np.uint8])
+def test_load_hsc_rejects_non_f64_coarse(tmp_path, bad, dtype):
+    """cn_hsc_import reads n_coarse^2 doubles from each coarse matrix: any
+    other dtype is refused before the library is called."""
+    sysm = _FakeSystem()
+    t = {"hsc.coarse_L": np.eye(4), "hsc.coarse_inv": np.eye(4)}
+    t["hsc." + bad] = np.eye(4).astype(dtype)
+    with pytest.raises(ValueError, match="F64"):
+        _F().load_hsc(_sidecar(tmp_path, sysm, t), sysm)
+
+
+@pytest.mark.parametrize("start,nidx,msg", [
+    ([0, 1, 2, 3], 3, "entries"),                 # nv + 1 = 5 expected
+    ([1, 1, 2, 3, 4], 4, "begin at 0"),
+    ([0, 2, 1, 3, 4], 4, "never decrease"),
+    ([0, 1, 2, 3, 5], 4, "differ"),
+])
+def test_influences_inconsistent_csr_rejected(tmp_path, start, nidx, msg):
+    F = _F()
+    with pytest.raises(ValueError, match=msg):
+        F.check_influences(4, np.array(start, np.int32), np.zeros(nidx, np.int32),
+                           np.zeros(nidx))
+    sysm = _FakeSystem()
+    t = {"ddm.start": np.array(start, np.int32), "ddm.idx": np.zeros(nidx, np.int32),
+         "ddm.w": np.zeros(nidx)}
+    with pytest.raises(ValueError, match=msg):
+        F.load_influences(_sidecar(tmp_path, sysm, t), sysm)
+
+
+def test_influences_dtypes_checked(tmp_path):
+    sysm = _FakeSystem()
+    t = {"ddm.start": np.arange(5, dtype=np.int64), "ddm.idx": np.zeros(4, np.int32),
+         "ddm.w": np.zeros(4)}
+    with pytest.raises(ValueError, match="I32"):
+        _F().load_influences(_sidecar(tmp_path, sysm, t), sysm)
+    t["ddm.start"] = np.arange(5, dtype=np.int32)
+    s, i, w = _F().load_influences(_sidecar(tmp_path, sysm, t), sysm)
+    assert list(s) == [0, 1, 2, 3, 4]

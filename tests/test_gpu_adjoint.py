@@ -70,6 +70,27 @@ def test_autograd_gradcheck_tiny(gpu):
     assert torch.autograd.gradcheck(fn, (xc,), eps=1e-6, atol=1e-7, rtol=1e-6)
 
 
+@pytest.mark.parametrize("shape", ["cpu2d", "cpu1d", "cuda1d"])
+def test_autograd_cpu_leaf_and_vector(gpu, shape):
+    """A CPU leaf gets its gradient on the CPU, and a 1-D xc (one column)
+    is a vector in and out, like CutLaplacian.solve."""
+    import torch
+    from tool_godot_curvenet_amd import meshgen as G
+    from tool_godot_curvenet_amd.autograd import harmonic_solve
+    lap = _lap(G.open_cylinder(8, 6, seed=3))
+    cols = 2 if shape == "cpu2d" else None
+    v = np.random.default_rng(2).uniform(-0.2, 0.2, (lap.n_con,) if cols is None
+                                         else (lap.n_con, cols))
+    dev = "cpu" if shape.startswith("cpu") else "cuda"
+    xc = torch.tensor(v, dtype=torch.float64, device=dev, requires_grad=True)
+    fn = lambda u: harmonic_solve(u, lap, tol=1e-13, precision="fp64")
+    out = fn(xc)
+    assert out.dim() == xc.dim()
+    out.pow(2).sum().backward()
+    assert xc.grad.device == xc.device and xc.grad.shape == xc.shape
+    assert torch.autograd.gradcheck(fn, (xc,), eps=1e-6, atol=1e-7, rtol=1e-6)
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-11)])
 def test_bspline_control_gradient_matches_fd(gpu, precision, tol):
     import torch

@@ -386,6 +386,85 @@ def test_project_to_vertices(gpu):
     assert np.array_equal(project_to_vertices(q, m.pos), O.project_to_vertices(q, m.pos))
 
 
+_TRI_MESH = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+
+
+@pytest.mark.parametrize("knots,want,cols", [
+    ([[0.0, 0.0, 0.0]], [0], [0]),                                   # SurfaceProjection.lean:99-101
+    ([[0.9, 0.0, 0.0]], [1], [0]),                                   # :105-107
+    ([[0.0, 0.9, 0.0]], [2], [0]),                                   # :110-112
+    ([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]], [0, 1, 2], [0, 1, 2]),  # :115-120, 141-144
+    ([[0.0, 0.0, 0.0], [0.01, 0.0, 0.0]], [0, 0], [0, 0]),           # :133-137
+    ([[0.0, 1.0, 0.0], [0.0, 0.0, 0.0]], [2, 0], [0, 1]),            # :148-151
+    ([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 0.0]], [0, 1, 0], [0, 1, 0]),  # :154-157
+])
+def test_projection_spec_kats(gpu, knots, want, cols):
+    from tool_godot_curvenet_amd import project_to_vertices, promote_vertex_samples
+    idx = project_to_vertices(knots, _TRI_MESH)
+    assert list(idx) == want
+    kc, vc, nc = promote_vertex_samples(3, idx)
+    assert list(kc) == cols and nc == max(cols) + 1
+    okc, ovc, onc = O.promote_vertex_samples(3, idx)
+    assert np.array_equal(kc, okc) and np.array_equal(vc, ovc) and nc == onc
+
+
+def test_projection_empty(gpu):                              # SurfaceProjection.lean:123-130
+    from tool_godot_curvenet_amd import project_to_vertices, promote_vertex_samples
+    assert project_to_vertices(np.zeros((0, 3)), _TRI_MESH).shape == (0,)
+    idx = project_to_vertices([[0.0, 0.0, 0.0]], np.zeros((0, 3)))
+    assert list(idx) == [-1]
+    kc, vc, nc = promote_vertex_samples(0, idx)
+    assert list(kc) == [-1] and vc.shape == (0,) and nc == 0
+    kc, vc, nc = promote_vertex_samples(4, np.zeros(0, np.int32))
+    assert kc.shape == (0,) and list(vc) == [-1] * 4 and nc == 0
+
+
+def test_projection_ties(gpu):
+    """Duplicated vertices and equidistant knots: the lowest index wins."""
+    from tool_godot_curvenet_amd import project_to_vertices
+    rng = np.random.default_rng(11)
+    pos = rng.uniform(-1, 1, (300, 3))
+    pos[200:250] = pos[10:60]
+    g = np.array([[0.0, 0.0, 0.0], [0.5, 0.0, 0.0], [0.0, 0.5, 0.0], [0.5, 0.5, 0.0]]) + 3.0
+    pos = np.concatenate([pos, g])
+    q = np.concatenate([rng.uniform(-1, 1, (100, 3)), pos[10:60],
+                        [[3.25, 3.0, 3.0], [3.0, 3.25, 3.0], [3.25, 3.25, 3.0]]])
+    got = project_to_vertices(q, pos)
+    assert np.array_equal(got, O.project_to_vertices(q, pos))
+    assert (got[100:150] == np.arange(10, 60)).all() and list(got[-3:]) == [300] * 3
+
+
+def test_projection_promotion_c2_knots(gpu):
+    """The 8 x 32 loop knots of the level-6 icosphere (the C2 curves):
+    projection and column assignment bit-exact against the oracle."""
+    from tool_godot_curvenet_amd import meshgen, project_to_vertices, promote_vertex_samples
+    m = meshgen.geodesic_sphere(64, jitter_frac=0.05, seed=meshgen.BASE_SEED + 2)
+    knots = np.concatenate(meshgen.loop_knots(m, 8, 30.0, 32, seed=meshgen.BASE_SEED + 1))
+    idx = project_to_vertices(knots, m.pos)
+    assert np.array_equal(idx, O.project_to_vertices(knots, m.pos))
+    kc, vc, nc = promote_vertex_samples(m.nv, idx)
+    okc, ovc, onc = O.promote_vertex_samples(m.nv, idx)
+    assert np.array_equal(kc, okc) and np.array_equal(vc, ovc) and nc == onc
+    assert nc == len(np.unique(idx))
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_promote_random(gpu, seed):
+    from tool_godot_curvenet_amd import promote_vertex_samples
+    rng = np.random.default_rng(seed)
+    nv = int(rng.integers(1, 5000))
+    idx = rng.integers(-1, nv, int(rng.integers(1, 20000))).astype(np.int32)
+    kc, vc, nc = promote_vertex_samples(nv, idx)
+    okc, ovc, onc = O.promote_vertex_samples(nv, idx)
+    assert np.array_equal(kc, okc) and np.array_equal(vc, ovc) and nc == onc
+
+
+def test_promote_rejects_out_of_range(gpu):
+    from tool_godot_curvenet_amd import promote_vertex_samples
+    with pytest.raises(RuntimeError):
+        promote_vertex_samples(3, [0, 1, 3])
+
+
 def _hub_matrix(n, seed):
     """SPD graph Laplacian + 0.01 I: a ring with +-1, +-2 neighbours and 24
     hub vertices joined to 10-20 random others, so some rows hold up to ~25
@@ -436,6 +515,40 @@ def test_pcg_csr_long_rows(gpu, precision, k):
     # costs some iterations on this poorly conditioned matrix (kappa ~ 1e4)
     lo, hi = (0.9, 1.1) if precision != "mixed" else (0.9, 1.3)
     assert lo * it - 3 <= info.iters <= hi * it + 3, (info.iters, it)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64", "fp64_plain"])
+def test_jacobi_zero_diagonal(gpu, precision):
+    """Jacobi.lean:7-10: a zero diagonal entry gives 0, not 1/0.  A 1-D
+    Laplacian (+0.1 I) with rows 5 and 17 emptied to an explicit zero
+    diagonal and zero right-hand side: Jacobi-PCG keeps those rows at 0
+    (z, p and x stay 0 there; 1/0 would make them inf/NaN) and solves the
+    rest to the FP64 tolerance."""
+    from tool_godot_curvenet_amd import CsrMatrix, CsrSystem
+    n, zero = 40, (5, 17)
+    ptr, col, val = [0], [], []
+    for i in range(n):
+        if i in zero:
+            col.append(i)
+            val.append(0.0)
+        else:
+            for j, w in ((i - 1, -1.0), (i, 2.1), (i + 1, -1.0)):
+                if 0 <= j < n and j not in zero:
+                    col.append(j)
+                    val.append(w)
+        ptr.append(len(col))
+    A = CsrMatrix(n, n, np.array(ptr, np.int32), np.array(col, np.int32),
+                  np.array(val, np.float64))
+    B = np.random.default_rng(4).normal(size=(n, 3))
+    B[list(zero)] = 0.0
+    x, info = CsrSystem(A).solve(B, precond="jacobi", precision=precision, tol=1e-9)
+    X = x.cpu().numpy()
+    assert info.status == 0 and np.all(np.isfinite(X))
+    assert np.all(X[list(zero)] == 0.0)
+    keep = [i for i in range(n) if i not in zero]
+    D = A.dense()[np.ix_(keep, keep)]
+    ref = np.linalg.solve(D, B[keep])
+    assert np.abs(X[keep] - ref).max() <= 1e-8 * np.abs(ref).max()
 
 
 def test_solve_after_set_positions(gpu):

@@ -278,3 +278,49 @@ def test_greedy_color(name):                            # GraphColoring.lean:84-
     for i, row in enumerate(adj):
         for j in row:
             assert color[i] != color[j]
+
+
+# ------------------------------------------------- SurfaceProjection.lean --
+_ORIGIN, _XU, _YU = (0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0)
+_TRI_MESH = np.array([_ORIGIN, _XU, _YU])
+
+
+def test_projection_knot_at_vertex():                   # SurfaceProjection.lean:99-101
+    r = O.project_to_vertices([_ORIGIN], _TRI_MESH)
+    assert r.shape == (1,) and r[0] == 0
+
+
+@pytest.mark.parametrize("knot,want", [((0.9, 0.0, 0.0), 1), ((0.0, 0.9, 0.0), 2)])
+def test_projection_nearest(knot, want):                # SurfaceProjection.lean:105-112
+    assert O.project_to_vertices([knot], _TRI_MESH)[0] == want
+
+
+def test_projection_all_vertex_kind():                  # SurfaceProjection.lean:115-120
+    # projectToVertices produces only the vertex kind; every knot gets a
+    # vertex (meshIndex >= 0), so every projection is of that kind
+    r = O.project_to_vertices([_ORIGIN, _XU, _YU], _TRI_MESH)
+    assert r.shape == (3,) and (r >= 0).all() and list(r) == [0, 1, 2]
+
+
+def test_projection_empty_knots():                      # SurfaceProjection.lean:123-125
+    assert O.project_to_vertices(np.zeros((0, 3)), _TRI_MESH).shape == (0,)
+
+
+def test_projection_empty_mesh():                       # SurfaceProjection.lean:128-130
+    r = O.project_to_vertices([_ORIGIN], np.zeros((0, 3)))
+    assert r[0] == -1
+    # the position falls back to the knot itself (SurfaceProjection.lean:58)
+    assert np.array_equal(O.projected_positions([(0.3, 0.2, 0.1)], np.zeros((0, 3)), r),
+                          [[0.3, 0.2, 0.1]])
+    assert np.array_equal(O.projected_positions([(0.9, 0.0, 0.0)], _TRI_MESH, [1]), [_XU])
+
+
+@pytest.mark.parametrize("knots,cols", [
+    ([_ORIGIN, (0.01, 0.0, 0.0)], [0, 0]),              # SurfaceProjection.lean:133-137
+    ([_XU, _YU, _ORIGIN], [0, 1, 2]),                   # SurfaceProjection.lean:141-144
+    ([_YU, _ORIGIN], [0, 1]),                           # SurfaceProjection.lean:148-151
+    ([_ORIGIN, _XU, _ORIGIN], [0, 1, 0]),               # SurfaceProjection.lean:154-157
+])
+def test_promote_vertex_samples(knots, cols):
+    kc, vc, nc = O.promote_vertex_samples(3, O.project_to_vertices(knots, _TRI_MESH))
+    assert list(kc) == cols and nc == max(cols) + 1

@@ -357,3 +357,95 @@ def test_dense_against_numpy():
     assert np.allclose(X, np.linalg.solve(A, B), atol=1e-12)
     a, piv = O.lu_factor(A)
     assert np.allclose(O.lu_solve(a, piv, B[:, 0]), np.linalg.solve(A, B[:, 0]), atol=1e-12)
+
+
+# ------------------------------------------------------- knot projection --
+def _first_argmin(knots, pos):
+    """numpy restatement: the lowest index among the minima of the squared
+    distances (np.argmin returns the first minimum)."""
+    d = pos[None, :, :] - knots[:, None, :]
+    dd = d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1] + d[..., 2] * d[..., 2]
+    return np.argmin(dd, axis=1).astype(np.int32)
+
+
+def _cloud_with_ties(seed):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-1, 1, (60, 3))
+    pos[40:50] = pos[5:15]                  # duplicated vertices: exact distance ties
+    knots = np.concatenate([rng.uniform(-1, 1, (40, 3)), pos[5:15] + 0.0])
+    # knots equidistant from two vertices: the midpoints of vertex pairs on
+    # an axis-aligned grid, where both squared distances round identically
+    g = np.array([[0.0, 0.0, 0.0], [0.5, 0.0, 0.0], [0.0, 0.5, 0.0], [0.5, 0.5, 0.0]])
+    pos = np.concatenate([pos, g + 3.0])
+    knots = np.concatenate([knots, [[3.25, 3.0, 3.0], [3.0, 3.25, 3.0], [3.25, 3.25, 3.0]]])
+    return knots, pos
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_projection_first_argmin(seed):
+    knots, pos = _cloud_with_ties(seed)
+    got = O.project_to_vertices(knots, pos)
+    assert np.array_equal(got, _first_argmin(knots, pos))
+    # the duplicated vertices and the equidistant knots resolve to the lower index
+    assert (got[40:50] == np.arange(5, 15)).all()
+    assert list(got[-3:]) == [60, 60, 60]
+
+
+def test_projection_planted_last_index_fails():
+    knots, pos = _cloud_with_ties(0)
+    # a "<=" scan (last minimum wins) is the plausible mistake
+    d = pos[None] - knots[:, None]
+    dd = (d * d).sum(-1)
+    last = (dd.shape[1] - 1 - np.argmin(dd[:, ::-1], axis=1)).astype(np.int32)
+    assert not np.array_equal(O.project_to_vertices(knots, pos), last)
+
+
+def _promote_closed_form(nv, idx):
+    """Columns = rank of each vertex's first occurrence among the valid knots."""
+    idx = np.asarray(idx)
+    valid = np.flatnonzero(idx >= 0)
+    kc = np.full(idx.shape[0], -1, np.int32)
+    vc = np.full(nv, -1, np.int32)
+    if valid.size:
+        verts, first = np.unique(idx[valid], return_index=True)
+        order = np.argsort(valid[first], kind="stable")       # encounter order
+        vc[verts[order]] = np.arange(order.size)
+        kc[valid] = vc[idx[valid]]
+    return kc, vc, int((vc >= 0).sum())
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_promote_closed_form(seed):
+    rng = np.random.default_rng(seed)
+    nv = 50
+    idx = rng.integers(-1, nv, 300).astype(np.int32)
+    kc, vc, nc = O.promote_vertex_samples(nv, idx)
+    ekc, evc, enc = _promote_closed_form(nv, idx)
+    assert np.array_equal(kc, ekc) and np.array_equal(vc, evc) and nc == enc
+
+
+def test_promote_planted_vertex_order_fails():
+    idx = np.array([7, 3, 7, 1, 3], np.int32)
+    kc, _, _ = O.promote_vertex_samples(8, idx)
+    assert list(kc) == [0, 1, 0, 2, 1]
+    # columns numbered by vertex index instead of encounter order
+    wrong = np.searchsorted(np.unique(idx), idx)
+    assert not np.array_equal(kc, wrong)
+
+
+def test_promote_rejects_out_of_range():
+    with pytest.raises(IndexError):
+        O.promote_vertex_samples(3, [0, 3])
+
+
+def test_every_oracle_function_is_pinned():
+    import inspect
+    import re
+    import oracle.oracle as OO
+    tokens = set()
+    for key in OO.PINS:
+        tokens |= set(re.findall(r"[A-Za-z0-9_]+", key))
+    infra = {"build", "lib"}
+    fns = [n for n, f in inspect.getmembers(OO, inspect.isfunction)
+           if not n.startswith("_") and f.__module__ == OO.__name__ and n not in infra]
+    assert [f for f in fns if f not in tokens] == []

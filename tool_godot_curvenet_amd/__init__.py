@@ -26,6 +26,7 @@ from .curvenet import (  # noqa: F401
     from_triangles,
     harmonic_solve,
     project_to_vertices,
+    promote_vertex_samples,
     segment_gradients,
     spmm,
     spmv,
@@ -34,6 +35,6 @@ from .autograd import harmonic_solve as harmonic_solve_ad  # noqa: F401
 
 __all__ = [
     "HSC", "SGS", "Restriction", "dense_cholesky", "dense_cholesky_solve", "IC0", "CutLaplacian", "CsrMatrix", "CsrSystem", "BatchSystem", "cg", "cg_with_guess", "cut_seams", "deform_solve", "bezier3", "segment_gradients", "deformation_gradients",
-    "from_triangles", "harmonic_solve", "harmonic_solve_ad", "project_to_vertices", "spmm",
+    "from_triangles", "harmonic_solve", "harmonic_solve_ad", "project_to_vertices", "promote_vertex_samples", "spmm",
     "spmv", "curves", "ddm", "formats",
 ]

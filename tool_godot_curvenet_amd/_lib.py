@@ -111,6 +111,7 @@ _SIGS = {
     "cn_cut_insert_crack": (i32, [P_i32, P_i32, P_i32, i32, vp, vp, vp, vp]),
     "cn_cut_seams": (i32, [i32, i32, vp, i32, vp, vp, vp, vp, P_i32, vp]),
     "cn_project_to_vertices": (i32, [i32, vp, i32, vp, vp, vp]),
+    "cn_promote_vertex_samples": (i32, [i32, i32, vp, vp, vp, vp, vp]),
     "cn_system_create": (i32, [i32, i32, vp, vp, vp, C.POINTER(vp), vp]),
     "cn_system_default_opts": (None, [C.POINTER(SystemOpts)]),
     "cn_system_create_ex": (i32, [i32, i32, vp, vp, vp, C.POINTER(SystemOpts), C.POINTER(vp),

@@ -20,7 +20,12 @@ def _function():
     class HarmonicSolveFn(torch.autograd.Function):
         @staticmethod
         def forward(ctx, xc, lap, pre, tol, precision):
-            xc = xc.detach().to(device="cuda", dtype=torch.float64).contiguous()
+            # the gradient goes back in xc's own device, dtype and shape
+            ctx.in_device, ctx.in_dtype, ctx.in_shape = xc.device, xc.dtype, xc.shape
+            if xc.dim() not in (1, 2) or xc.shape[0] != lap.n_con:
+                raise ValueError(f"xc must be n_con ({lap.n_con}) or n_con x k")
+            xc = xc.detach().to(device="cuda", dtype=torch.float64).reshape(lap.n_con, -1)
+            xc = xc.contiguous()
             k = xc.shape[1]
             b = torch.empty((lap.n_free, k), dtype=torch.float64, device="cuda")
             L.check(L.load().cn_harmonic_rhs(lap.handle, k, _p(xc), _p(b), _stream()),
@@ -31,11 +36,12 @@ def _function():
             if info.status != L.CN_OK:
                 raise L.CurvenetError(info.status, "harmonic_solve forward")
             ctx.lap, ctx.pre, ctx.tol, ctx.precision = lap, pre, tol, precision
-            return x
+            return x.reshape(-1) if len(ctx.in_shape) == 1 else x
 
         @staticmethod
         def backward(ctx, gx):
-            gx = gx.detach().to(dtype=torch.float64).contiguous()
+            gx = gx.detach().to(device="cuda", dtype=torch.float64).reshape(ctx.lap.n_free, -1)
+            gx = gx.contiguous()
             pre = ctx.pre
             kw = {"hsc": pre} if isinstance(pre, HSC) else {"ic0": pre}
             pc = "hsc" if isinstance(pre, HSC) else "ic0"
@@ -43,6 +49,7 @@ def _function():
                                              tol=ctx.tol, **kw)
             if info.status != L.CN_OK:
                 raise L.CurvenetError(info.status, "harmonic_solve backward")
+            gc = gc.to(device=ctx.in_device, dtype=ctx.in_dtype).reshape(ctx.in_shape)
             return gc, None, None, None, None
 
     return HarmonicSolveFn

@@ -406,6 +406,54 @@ __global__ void k_project(int32_t nv, const double *pos, int32_t nq,
   out[i] = bi;
 }
 
+
+// SurfaceProjection.promoteVertexSamples (SurfaceProjection.lean:64-80).
+// The spec walks the knots in order; here the same columns come from
+// (1) the first knot of every vertex (atomicMin of the knot index),
+// (2) an exclusive scan of the "first knot" flags -- a first knot's column is
+// the number of first knots before it, which is the spec's running
+// `nextCol` -- and (3) every other knot copying its vertex's first column.
+__global__ void k_pv_init(int32_t nv, int32_t *vtc) {
+  int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nv) vtc[v] = INT_MAX;
+}
+
+__global__ void k_pv_first(int32_t nq, const int32_t *idx, int32_t nv, int32_t *vtc,
+                           int32_t *bad) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  int32_t v = idx[i];
+  if (v >= nv) *bad = 1;  // every writer stores 1
+  else if (v >= 0) atomicMin(&vtc[v], i);
+}
+
+__global__ void k_pv_flag(int32_t nq, const int32_t *idx, int32_t nv, const int32_t *vtc,
+                          int32_t *ktc) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  int32_t v = idx[i];
+  ktc[i] = (v >= 0 && v < nv && vtc[v] == i) ? 1 : 0;
+}
+
+// after the scan a first knot holds its column; the others take their
+// vertex's (only non-first knots are written, only first knots are read)
+__global__ void k_pv_share(int32_t nq, const int32_t *idx, int32_t nv, const int32_t *vtc,
+                           int32_t *ktc) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  int32_t v = idx[i];
+  if (v < 0 || v >= nv) { ktc[i] = -1; return; }
+  int32_t f = vtc[v];
+  if (f != i) ktc[i] = ktc[f];
+}
+
+__global__ void k_pv_vertex_col(int32_t nv, int32_t *vtc, const int32_t *ktc) {
+  int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  int32_t f = vtc[v];
+  vtc[v] = (f == INT_MAX) ? -1 : ktc[f];
+}
+
 }  // namespace cn
 
 using namespace cn;
@@ -542,6 +590,50 @@ cn_status cn_project_to_vertices(int32_t nv, const double *d_pos, int32_t nq,
     k_project<<<grid_for(nq, 64), 64, 0, s>>>(nv, d_pos, nq, d_q, d_index);
   CN_HIP(hipGetLastError());
   CN_HIP(hipStreamSynchronize(s));
+  return CN_OK;
+  CN_API_END
+}
+
+cn_status cn_promote_vertex_samples(int32_t nv, int32_t nq, const int32_t *d_index,
+                                    int32_t *d_knot_to_col, int32_t *d_vertex_to_col,
+                                    int32_t *h_ncols, void *stream) {
+  CN_API_BEGIN
+  if (nv < 0 || nq < 0 || !h_ncols || (nq > 0 && (!d_index || !d_knot_to_col)) ||
+      (nv > 0 && !d_vertex_to_col)) {
+    set_error("cn_promote_vertex_samples: bad arguments");
+    return CN_BAD_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  *h_ncols = 0;
+  if (nv > 0) k_pv_init<<<grid_for(nv, 256), 256, 0, s>>>(nv, d_vertex_to_col);
+  CN_HIP(hipGetLastError());
+  if (nq > 0) {
+    DevBuf<int32_t> bad(1);
+    CN_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), s));
+    k_pv_first<<<grid_for(nq, 256), 256, 0, s>>>(nq, d_index, nv, d_vertex_to_col, bad.get());
+    k_pv_flag<<<grid_for(nq, 256), 256, 0, s>>>(nq, d_index, nv, d_vertex_to_col,
+                                                d_knot_to_col);
+    CN_HIP(hipGetLastError());
+    int64_t ncols = exclusive_scan_i32(d_knot_to_col, nq, s);  // synchronises
+    k_pv_share<<<grid_for(nq, 256), 256, 0, s>>>(nq, d_index, nv, d_vertex_to_col,
+                                                 d_knot_to_col);
+    CN_HIP(hipGetLastError());
+    if (nv > 0)
+      k_pv_vertex_col<<<grid_for(nv, 256), 256, 0, s>>>(nv, d_vertex_to_col, d_knot_to_col);
+    CN_HIP(hipGetLastError());
+    int32_t hbad = 0;
+    CN_HIP(hipMemcpyAsync(&hbad, bad.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CN_HIP(hipStreamSynchronize(s));
+    if (hbad) {
+      set_error("cn_promote_vertex_samples: a knot projects to a vertex index >= nv");
+      return CN_BAD_ARG;
+    }
+    *h_ncols = (int32_t)ncols;
+  } else if (nv > 0) {
+    k_pv_vertex_col<<<grid_for(nv, 256), 256, 0, s>>>(nv, d_vertex_to_col, nullptr);
+    CN_HIP(hipGetLastError());
+    CN_HIP(hipStreamSynchronize(s));
+  }
   return CN_OK;
   CN_API_END
 }

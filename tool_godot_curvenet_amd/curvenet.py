@@ -8,6 +8,7 @@ definitions:
 ``from_triangles``         HalfedgeBuilder.fromTriangles (HalfedgeBuilder.lean:34-90)
 ``cut_seams``              seam duplication (cn_cut_seams; builder-defined rule)
 ``project_to_vertices``    SurfaceProjection.projectToVertices (:46-64)
+``promote_vertex_samples`` SurfaceProjection.promoteVertexSamples (:64-80)
 ``CutLaplacian``           CutMeshLaplacian.assembleVtLhV (:86-93), split A_ff / A_fc
 ``CutLaplacian.harmonic_rhs``  HarmonicSolve.solveMulti RHS (:35-45, 57-64)
 ``spmv`` / ``spmm``        SparseLinAlg.spmv (:44-53) / SpmvMulti
@@ -107,6 +108,23 @@ def project_to_vertices(knots, pos) -> np.ndarray:
     L.check(lib.cn_project_to_vertices(p.shape[0], _p(p), q.shape[0], _p(q), _p(out),
                                        _stream()), "project_to_vertices")
     return out.cpu().numpy()
+
+
+def promote_vertex_samples(vertex_count: int, index):
+    """SurfaceProjection.promoteVertexSamples (SurfaceProjection.lean:64-80)
+    on the device: the sample column of every knot, in encounter order.
+    Returns (knot_to_col, vertex_to_col, ncols); -1 marks knots without a
+    vertex and vertices no knot landed on."""
+    torch = _torch()
+    lib = L.load()
+    idx = _dev(np.asarray(index, dtype=np.int32).reshape(-1), torch.int32)
+    kc = torch.empty(idx.shape[0], dtype=torch.int32, device="cuda")
+    vc = torch.empty(int(vertex_count), dtype=torch.int32, device="cuda")
+    nc = C.c_int32(0)
+    L.check(lib.cn_promote_vertex_samples(int(vertex_count), idx.shape[0], _p(idx), _p(kc),
+                                          _p(vc), C.byref(nc), _stream()),
+            "promote_vertex_samples")
+    return kc.cpu().numpy(), vc.cpu().numpy(), nc.value
 
 
 # ----------------------------------------------------------------- matrices --

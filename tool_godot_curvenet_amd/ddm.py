@@ -53,8 +53,10 @@ def lbs_matvec(transforms, start, idx, w, rest, precision: str = "f64"):
     st = _as_tensor(start, torch.int32).reshape(-1).contiguous()
     ix = _as_tensor(idx, torch.int32).reshape(-1).contiguous()
     wt = _as_tensor(w, dt).reshape(-1).contiguous()
-    if st.numel() != nv + 1:
-        raise ValueError("start must have nv + 1 entries")
+    # the kernel reads idx[q], w[q] for q in [start[v], start[v+1]): the CSR
+    # is checked on the host before any launch
+    from .formats import check_influences
+    check_influences(nv, st.cpu().numpy(), ix, wt)
     out = torch.empty((P, nv, 3), dtype=dt, device="cuda")
     fn = L.load().cn_lbs_matvec_f64 if precision == "f64" else L.load().cn_lbs_matvec_f32
     L.check(fn(nv, nt, P, _p(r), _p(T), _p(st), _p(ix), _p(wt), _p(out), _stream()),

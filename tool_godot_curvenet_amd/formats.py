@@ -266,9 +266,14 @@ def load_hsc(path, system):
         for k in ("F", "C", "dinv", "r_ptr", "r_col", "r_val", "p_ptr", "p_col", "p_val"):
             setattr(x, k, arrs[k].ctypes.data if arrs[k].size else None)
     Lc, inv = st.get("hsc.coarse_L"), st.get("hsc.coarse_inv")
-    nc = 0 if Lc is None else Lc.shape[0]
-    if Lc is None or inv is None or Lc.shape != (nc, nc) or inv.shape != (nc, nc):
+    if Lc is None or inv is None or Lc.ndim != 2:
         raise ValueError("sidecar: coarse factor missing or not square")
+    nc = Lc.shape[0]
+    if Lc.shape != (nc, nc) or inv.shape != (nc, nc):
+        raise ValueError("sidecar: coarse factor missing or not square")
+    # cn_hsc_import reads n_coarse^2 doubles from each
+    if Lc.dtype != np.float64 or inv.dtype != np.float64:
+        raise ValueError("sidecar: hsc.coarse_L / hsc.coarse_inv must be F64")
     Lc, inv = np.ascontiguousarray(Lc), np.ascontiguousarray(inv)
     o = L.HscOpts()
     L.load().cn_hsc_default_opts(C.byref(o))
@@ -284,4 +289,28 @@ def load_influences(path, system):
     """The skinning influences (start, idx, w) stored in a sidecar."""
     st = read_safetensors(path)
     _check_sidecar(st, system)
-    return st.get("ddm.start"), st.get("ddm.idx"), st.get("ddm.w")
+    start, idx, w = st.get("ddm.start"), st.get("ddm.idx"), st.get("ddm.w")
+    if start is None or idx is None or w is None:
+        raise ValueError("sidecar: ddm.start / ddm.idx / ddm.w missing")
+    if start.dtype != np.int32 or idx.dtype != np.int32 or w.dtype not in (np.float64,
+                                                                          np.float32):
+        raise ValueError("sidecar: ddm.start and ddm.idx must be I32, ddm.w F64 or F32")
+    start, idx, w = start.reshape(-1), idx.reshape(-1), w.reshape(-1)
+    nv = system.info.nv
+    check_influences(nv, start, idx, w)
+    return start, idx, w
+
+
+def check_influences(nv: int, start, idx, w):
+    """The influence CSR must be consistent before a kernel walks it:
+    start has nv + 1 entries, start[0] == 0, never decreases, and
+    start[nv] == len(idx) == len(w)."""
+    start = np.asarray(start).reshape(-1)
+    if start.shape[0] != nv + 1:
+        raise ValueError(f"influences: start has {start.shape[0]} entries, expected {nv + 1}")
+    if start[0] != 0 or np.any(np.diff(start.astype(np.int64)) < 0):
+        raise ValueError("influences: start must begin at 0 and never decrease")
+    def count(a):
+        return int(a.numel()) if hasattr(a, "numel") else int(np.asarray(a).size)
+    if int(start[-1]) != count(idx) or int(start[-1]) != count(w):
+        raise ValueError("influences: start[nv], len(idx) and len(w) differ")

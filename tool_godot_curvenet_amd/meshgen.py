@@ -199,21 +199,15 @@ def _nearest_vertex(tree, pos, p) -> int:
     return int(cand[int(np.argmin(d))])
 
 
-def tilted_loops(mesh: Mesh, n_loops: int = 8, max_tilt_deg: float = 30.0,
-                 n_way: int = 32, seed: int = BASE_SEED + 1) -> Mesh:
-    """Latitude loops of random tilt, snapped to the mesh: waypoints on the
-    tilted circle go to their nearest vertex (strict <, lowest index), and
-    consecutive waypoints are joined by shortest edge paths."""
-    from scipy.sparse.csgraph import dijkstra
-    from scipy.spatial import cKDTree
+def loop_knots(mesh: Mesh, n_loops: int = 8, max_tilt_deg: float = 30.0,
+               n_way: int = 32, seed: int = BASE_SEED + 1) -> list:
+    """The curve knots of `tilted_loops`: for each of n_loops seeded latitude
+    circles of random tilt, n_way points on the circle (n_way x 3)."""
     rng = np.random.default_rng(seed)
     pos = mesh.pos
     rad = float(np.linalg.norm(pos, axis=1).mean())
-    g = _edge_graph(pos, mesh.tris)
-    tree = cKDTree(pos)
     lats = np.sort(rng.uniform(-0.8, 0.8, size=n_loops))
-    edges = []
-    loops = []
+    out = []
     for L in range(n_loops):
         tilt = math.radians(rng.uniform(0, max_tilt_deg))
         az = rng.uniform(0, 2 * math.pi)
@@ -226,7 +220,24 @@ def tilted_loops(mesh: Mesh, n_loops: int = 8, max_tilt_deg: float = 30.0,
         h = lats[L] * rad
         rr = math.sqrt(max(rad * rad - h * h, 1e-12))
         ang = np.linspace(0, 2 * math.pi, n_way, endpoint=False)
-        pts = h * nrm[None] + rr * (np.cos(ang)[:, None] * u[None] + np.sin(ang)[:, None] * v[None])
+        out.append(h * nrm[None] + rr * (np.cos(ang)[:, None] * u[None]
+                                         + np.sin(ang)[:, None] * v[None]))
+    return out
+
+
+def tilted_loops(mesh: Mesh, n_loops: int = 8, max_tilt_deg: float = 30.0,
+                 n_way: int = 32, seed: int = BASE_SEED + 1) -> Mesh:
+    """Latitude loops of random tilt, snapped to the mesh: waypoints on the
+    tilted circle go to their nearest vertex (strict <, lowest index), and
+    consecutive waypoints are joined by shortest edge paths."""
+    from scipy.sparse.csgraph import dijkstra
+    from scipy.spatial import cKDTree
+    pos = mesh.pos
+    g = _edge_graph(pos, mesh.tris)
+    tree = cKDTree(pos)
+    edges = []
+    loops = []
+    for pts in loop_knots(mesh, n_loops, max_tilt_deg, n_way, seed):
         way = [_nearest_vertex(tree, pos, p) for p in pts]
         way = [w for k, w in enumerate(way) if k == 0 or w != way[k - 1]]
         if way[0] == way[-1]:
