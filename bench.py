@@ -106,7 +106,7 @@ def main():
     ap.add_argument("--cpu-poses", type=int, default=8, help="poses in the CPU baseline sample")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
-    ap.add_argument("--plain-poses", type=int, default=2,
+    ap.add_argument("--plain-poses", type=int, default=8,
                     help="poses solved by the plain FP64 one-solve-per-launch path")
     ap.add_argument("--workload", default="capsules100k",
                     choices=["capsules100k", "meshes64", "icosphere9", "ddm"],
@@ -319,29 +319,43 @@ def main():
     dom = max(kernels, key=lambda t: kernels[t]["ms"])
 
     # --------------------------------------- plain FP64 one-solve-per-launch
+    # The unfused FP64 path (one launch per SpMV / dot / axpy / preconditioner
+    # step, alpha and beta on the host), one pose (3 RHS) per solve, with the
+    # same HSC hierarchy as the headline (the same-preconditioner baseline),
+    # with block IC(0), and with Jacobi.
     plain = None
     if args.plain_poses > 0:
-        popts = _lib.PcgOpts()
-        lib.cn_pcg_default_opts(popts)
-        popts.precond = _lib.CN_PRECOND_JACOBI
-        popts.precision = _lib.CN_PREC_FP64_PLAIN
-        popts.tol_rel = args.tol
-        popts.max_iter = 200000
-        pinfo = _lib.SolveInfo()
         Bp = B[:, :3].contiguous()
         Xp = torch.zeros_like(Bp)
-        torch.cuda.synchronize()
-        tp = time.perf_counter()
-        pit = 0
-        for _ in range(args.plain_poses):
-            lib.cn_pcg_solve(lap.handle, None, C.byref(popts), 3, C.c_void_p(Bp.data_ptr()),
-                             C.c_void_p(Xp.data_ptr()), C.byref(pinfo), sp)
-            pit += pinfo.iters
-        torch.cuda.synchronize()
-        tpe = time.perf_counter() - tp
-        plain = {"solves_per_s": args.plain_poses / tpe, "precond": "jacobi",
-                 "iters_per_solve": pit / args.plain_poses,
-                 "relres_max": pinfo.relres_max, "status": pinfo.status}
+        plain = {}
+        pres = [("hsc", pre if args.precond == "hsc" else cn.HSC(lap, theta=args.theta)),
+                ("ic0", pre if args.precond == "ic0" else cn.IC0(lap, "block", block_size=16)),
+                ("jacobi", None)]
+        for pname, pobj in pres:
+            popts = _lib.PcgOpts()
+            lib.cn_pcg_default_opts(popts)
+            popts.precond = {"ic0": _lib.CN_PRECOND_IC0}.get(pname, _lib.CN_PRECOND_JACOBI)
+            popts.precision = _lib.CN_PREC_FP64_PLAIN
+            popts.tol_rel = args.tol
+            popts.max_iter = 200000
+            pinfo = _lib.SolveInfo()
+            fn = lib.cn_pcg_solve_hsc if pname == "hsc" else lib.cn_pcg_solve
+            hdl = pobj.handle if pobj is not None else None
+            npz = args.plain_poses if pname != "jacobi" else min(args.plain_poses, 2)
+            fn(lap.handle, hdl, C.byref(popts), 3, C.c_void_p(Bp.data_ptr()),
+               C.c_void_p(Xp.data_ptr()), C.byref(pinfo), sp)          # warm-up
+            torch.cuda.synchronize()
+            tp = time.perf_counter()
+            pit = 0
+            for _ in range(npz):
+                fn(lap.handle, hdl, C.byref(popts), 3, C.c_void_p(Bp.data_ptr()),
+                   C.c_void_p(Xp.data_ptr()), C.byref(pinfo), sp)
+                pit += pinfo.iters
+            torch.cuda.synchronize()
+            tpe = time.perf_counter() - tp
+            plain[pname] = {"solves_per_s": npz / tpe, "iters_per_solve": pit / npz,
+                            "relres_max": pinfo.relres_max, "status": pinfo.status,
+                            "poses": npz}
 
     # ---------------------------------------------------------- CPU leg ----
     cpu = None
@@ -370,6 +384,19 @@ def main():
                "single_thread": {"value": npose / tce, "unit": "solves/s", "cores": 1,
                                  "sample": f"{npose} poses x 3 RHS, {cit} iterations, "
                                            f"{tce:.1f} s"}}
+
+    # ---- k = 1 FP32 SpMV of the same matrix (cn_spmv_f32), HIP events on
+    # the launch stream.  Algorithmic bytes: 8 nnz (FP32 value + int32
+    # column) + 4 (n + 1) row pointers + 4 n (x) + 4 n (y).  The 5.9 MB
+    # working set stays in the 256 MB Infinity Cache, so this is an on-chip
+    # rate, not an HBM one (the level-9 icosphere workload gives the HBM one).
+    spmv = spmv_k1(lib, lap.A_ff(), sp, reps=200)
+
+    # ---- C1: the reference's own CPU solver (DenseLinAlg.solveMulti: dense
+    # Gaussian elimination with partial pivoting, re-run per column) on the
+    # 64 x 64 cut cylinder, one column timed, beside the GPU solve of the
+    # same system (IC(0)-PCG to 1e-10) and the difference of the two.
+    c1 = None if args.no_cpu else c1_dense_vs_gpu(lib, sp)
 
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -403,6 +430,8 @@ def main():
                      "avg_ms": kernels[dom]["avg_ms"], "launches": kernels[dom]["launches"],
                      "bytes_per_launch": kernels[dom]["bytes_per_launch"]},
         "kernels": kernels,
+        "spmv_k1": spmv,
+        "c1_dense_ge": c1,
         "coarse_mfma": coarse,
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
@@ -413,8 +442,77 @@ def main():
                   "setup_s": setup_s},
     }
     if plain:
-        line["speedup_vs_plain_fp64"] = value / plain["solves_per_s"]
+        # the same preconditioner as the headline on both sides
+        line["speedup_vs_plain_fp64"] = value / plain[args.precond]["solves_per_s"]
+        line["speedup_vs_plain_fp64_jacobi"] = value / plain["jacobi"]["solves_per_s"]
     print(json.dumps(line))
+
+
+def spmv_k1(lib, A, sp, reps=200):
+    """Time cn_spmv_f32 on CSR A (FP32 values) over `reps` launches with HIP
+    events on the launch stream; achieved GB/s of algorithmic bytes."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+    dev = torch.device("cuda:0")
+    n, nnz = A.rows, A.nnz
+    ptr = torch.as_tensor(A.row_ptr.astype(np.int32), device=dev)
+    col = torch.as_tensor(A.col_idx.astype(np.int32), device=dev)
+    val = torch.as_tensor(A.values.astype(np.float32), device=dev)
+    x = torch.as_tensor(np.random.default_rng(1).normal(size=n).astype(np.float32), device=dev)
+    y = torch.empty_like(x)
+    args_ = (n, C.c_void_p(ptr.data_ptr()), C.c_void_p(col.data_ptr()),
+             C.c_void_p(val.data_ptr()), C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), sp)
+    for _ in range(10):
+        lib.cn_spmv_f32(*args_)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        lib.cn_spmv_f32(*args_)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    import scipy.sparse as sps
+    ref = sps.csr_matrix((A.values.astype(np.float32).astype(np.float64), A.col_idx,
+                          A.row_ptr), shape=(n, n)) @ x.cpu().numpy().astype(np.float64)
+    err = float(np.abs(y.cpu().numpy() - ref).max() / max(np.abs(ref).max(), 1e-300))
+    alg = 8 * nnz + 4 * (n + 1) + 8 * n
+    gbs = alg / (ms * 1e-3) / 1e9
+    return {"kernel": "kn_spmm<float,1> (cn_spmv_f32)", "n": n, "nnz": nnz, "avg_ms": ms,
+            "bytes_per_launch": alg, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+            "max_rel_err_vs_fp64": err}
+
+
+def c1_dense_vs_gpu(lib, sp):
+    import numpy as np
+    import torch
+
+    import oracle as O
+    import tool_godot_curvenet_amd as cn
+    from tool_godot_curvenet_amd import meshgen
+    lap = cn.CutLaplacian.from_mesh(meshgen.config_mesh("cylinder64"))
+    A = lap.A_ff()
+    B = lap.harmonic_rhs(meshgen.seeded_boundary(lap.n_con, 3))
+    ic0 = cn.IC0(lap)
+    lap.solve(B, precond="ic0", precision="mixed", tol=1e-10, ic0=ic0)      # warm-up
+    torch.cuda.synchronize()
+    reps = 20
+    t = time.perf_counter()
+    for _ in range(reps):
+        x, info = lap.solve(B, precond="ic0", precision="mixed", tol=1e-10, ic0=ic0)
+    torch.cuda.synchronize()
+    gpu_s = (time.perf_counter() - t) / reps
+    D = A.dense()
+    tc = time.perf_counter()
+    xd = O.dense_solve(D, B[:, 0])
+    col_s = time.perf_counter() - tc
+    X = x.cpu().numpy()
+    diff = float(np.abs(X[:, 0] - xd).max() / np.abs(xd).max())
+    return {"n": A.rows, "cpu_kind": "port (oracle dense GE, one thread)",
+            "cpu_seconds_per_column": col_s, "cpu_solves_per_s": 1.0 / (3 * col_s),
+            "gpu_solves_per_s": 1.0 / gpu_s, "gpu_iters": info.iters,
+            "gpu_vs_dense_max_rel_diff": diff, "cpu": _cpu_model()}
 
 
 def meshes64_bench(args):
@@ -594,6 +692,7 @@ def icosphere9_bench(args):
                    "adjoint_solve_hsc")
         return fi.iters, ai.iters
 
+    free0, total = torch.cuda.mem_get_info()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -601,8 +700,13 @@ def icosphere9_bench(args):
     its = [step() for _ in range(args.steps)]
     torch.cuda.synchronize()
     el = time.perf_counter() - t1
+    # device memory in use (every allocation of the process, the library's
+    # system, hierarchy and solver workspace included) after the timed steps
+    used = total - torch.cuda.mem_get_info()[0]
     # FP64 residuals of the last step's forward and adjoint solves (not timed)
     A = lap.A_ff()
+    # k = 1 FP32 SpMV of the 2.46M-row matrix: 176 MB of matrix and vectors
+    spmv = spmv_k1(lib, A, sp, reps=50)
     M = sps.csr_matrix((A.values, A.col_idx, A.row_ptr), shape=(n, n))
     Xh, Bh, Gh, Lh = X.cpu().numpy(), B.cpu().numpy(), G.cpu().numpy(), LAM.cpu().numpy()
     rf = float((np.linalg.norm(Bh - M @ Xh, axis=0) / np.linalg.norm(Bh, axis=0)).max())
@@ -624,6 +728,9 @@ def icosphere9_bench(args):
                                f"n_free={n}, n_con={ncon}, nnz={lap.info.nnz_ff}, {P} poses "
                                f"x 3 RHS, HSC theta {args.theta} ({hsc.info.nlevels} levels, "
                                f"coarse {hsc.info.n_coarse}), tol {args.tol}"},
+        "spmv_k1": spmv,
+        "device_memory": {"in_use_after_steps_bytes": int(used), "total_bytes": int(total),
+                          "in_use_before_steps_bytes": int(total - free0)},
         "solve": {"forward_iters": [f for f, _ in its], "adjoint_iters": [a for _, a in its],
                   "relres_forward": rf, "relres_adjoint": ra,
                   "setup_s": time.time() - t0 - el},

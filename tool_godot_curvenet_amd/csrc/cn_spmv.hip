@@ -3,8 +3,10 @@
 // FP64 forms reproduce SparseLinAlg.spmv exactly (SparseLinAlg.lean:44-53):
 // each output is summed left to right from 0.0 with separate multiply and
 // add (this file is built with -ffp-contract=off).  One thread per
-// (row, column); neighbouring threads read neighbouring rows, so the val/col
-// streams are consumed at full cache-line granularity.
+// (row, column), walking its row's entries one at a time: a wavefront's
+// val/col loads of one step land in 64 different rows (a strided pattern;
+// the lines are used in full only across the row's later steps, through the
+// cache).  This form is for exactness, not speed.
 //
 // FP32 forms are the solver's kernels (fused multiply-add, group or
 // wavefront per row, see cn_kernels.h).
