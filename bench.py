@@ -206,8 +206,8 @@ def main():
     torch.cuda.synchronize()
 
     lib.cn_profile_enable(1)
-    prof = {name: [0, 0.0] for name in ("spmm", "fwd", "bwd", "resid", "hsc_down", "hsc_up",
-                                         "hsc_coarse", "hsc_tail")}
+    prof = {name: [0, 0.0] for name in ("spmm", "spmm_lds", "fwd", "bwd", "resid", "hsc_down",
+                                         "hsc_up", "hsc_coarse", "hsc_tail")}
     iters, refines, relmax = 0, 0, 0.0
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -246,9 +246,11 @@ def main():
     # every column converged return at once and are given no bytes.
     nk4 = 4 * n * k
     spmm_bytes = 8 * nnz + 4 * (n + 1) + 2 * nk4
+    # the SpMM runs as the LDS-staged kernel or the row-table one
+    sp_tag = "spmm_lds" if prof["spmm_lds"][0] > 0 else "spmm"
+    sp_name = "kl_spmm_dot<float,6>" if sp_tag == "spmm_lds" else "kw_spmm_dot<float,4,64>"
     kern = {
-        "spmm": ("kw_spmm_dot<float,4,64>", prof["spmm"][0], prof["spmm"][1],
-                 iters * spmm_bytes),
+        "spmm": (sp_name, prof[sp_tag][0], prof[sp_tag][1], iters * spmm_bytes),
     }
     coarse = None
     if args.precond == "ic0":

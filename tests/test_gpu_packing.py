@@ -115,3 +115,31 @@ def test_warm_start_moderate_k(gpu):
     R = B - O.spmm(A.row_ptr, A.col_idx, A.values, X)
     rr = np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
     assert np.all(rr <= 1e-10), rr.max()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
+@pytest.mark.parametrize("name,k", [("capsules100k", 768), ("icosphere6", 96), ("cylinder64", 8)])
+def test_spmm_forms_agree(gpu, monkeypatch, precision, tol, name, k):
+    """The LDS-staged SpMM and the row-table SpMM compute the same products
+    in the same order (only the p.q partial sums are grouped differently):
+    two PCG iterations agree to rounding."""
+    from tool_godot_curvenet_amd import CutLaplacian, HSC, meshgen
+    lap = CutLaplacian.from_mesh(meshgen.config_mesh(name))
+    B = lap.harmonic_rhs(meshgen.seeded_boundary(lap.n_con, k, seed=k))
+    h = HSC(lap)
+    import ctypes
+    from tool_godot_curvenet_amd import _lib
+    lib = _lib.load()
+    xs = {}
+    for form in ("row", "lds"):
+        monkeypatch.setenv("CN_SPMM", form)
+        lib.cn_profile_enable(1)
+        x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, max_iter=2, hsc=h)
+        ms = ctypes.c_double(0.0)
+        used = lib.cn_profile_query(b"spmm_lds", ctypes.byref(ms))
+        lib.cn_profile_enable(0)
+        assert (used > 0) == (form == "lds"), (form, used)
+        assert info.iters == 2
+        xs[form] = x.cpu().numpy()
+    d = np.abs(xs["row"] - xs["lds"]).max() / np.abs(xs["row"]).max()
+    assert d <= tol, d

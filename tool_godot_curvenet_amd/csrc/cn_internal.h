@@ -166,6 +166,16 @@ struct cn_system_t {
   cn::DevBuf<float> ell_val32, ov_val32;
   cn::DevBuf<double> ell_val64, ov_val64;
   int64_t ov_nnz = -1;
+  // A_ff as the 8-slot window row table of the LDS-staged SpMM, for ranges
+  // of win_rpr rows: a slot holds the neighbour's row, or -(f + 1) for far
+  // row f of its range (far rows: win_far_rows[win_far_ptr[x] ..]), and the
+  // CSR position its weight is read from (-1: weight 0).  win_ok = 1 when
+  // built and usable, -1 when the matrix does not fit the window.
+  cn::DevBuf<int32_t> win_col, win_q, win_far_ptr, win_far_rows;
+  // the table packed per 32-row chunk for the kernel (columns, then FP32 or
+  // FP64 weights), refilled from the values at every solve
+  cn::DevBuf<unsigned char> win_pk32, win_pk64;
+  int32_t win_rpr = 0, win_ok = 0;
   // A batch of independent systems (cn_system_batch) is one block-diagonal
   // A_ff: segment s owns rows [seg_ptr[s], seg_ptr[s+1]).  The solver keeps
   // its CG scalars per (segment, column), and takes its dots over blocks of

@@ -201,6 +201,193 @@ kw_spmm_dot(int32_t n, const int32_t *__restrict__ ecol, const T *__restrict__ e
   block_partials_wide<VEC, 1, LPR>(pd, part, k, c0, 0);
 }
 
+// ---- LDS-staged SpMM (wide k) -------------------------------------------
+// A block owns one contiguous range of rows and one 256-byte column tile
+// (64 FP32 / 32 FP64 columns, 16 lanes a row, 32 rows a step).  It streams
+// the tile's rows of P through a ring of 12 chunks of 32 rows in LDS, so a
+// row is read from memory once per launch however many rows gather it: the
+// ring holds the 5 chunks behind and the 5 ahead of the chunk being
+// computed (every neighbour within 160 rows of its row), and the loads of
+// the next D chunks are in flight in registers.  Neighbours outside that
+// window ("far" rows, a few per range on banded meshes) are read into a
+// small LDS cache once at the start.
+// The row table has 8 slots a row (diagonal first, then the row's entries
+// in column order, padding with the last column and weight 0).  A slot
+// holds the LDS row the neighbour sits in while its row is computed: ring
+// slot (chunk mod 12) * 32 + row in chunk, or 12 * 32 + f for far row f.
+// That is fixed by the range, so the kernel does no index arithmetic per
+// entry.  The FMAs run in the order of the 7-slot table plus its overflow,
+// so Q is the same as kw_spmm_dot's.
+// Index data (LDS rows, weights) is staged through LDS with the same
+// pipeline, so the only vector-memory operations of a step are the
+// prefetches and the Q store and no wait drains the pipeline.
+constexpr int kLdsRows = 32;    // rows per chunk (512 threads, 16 lanes a row)
+constexpr int kLdsSlots = 12;   // ring of chunks
+constexpr int kLdsBack = 5;     // chunks kept behind the computed one
+constexpr int kLdsAhead = 5;    // chunks landed ahead of it
+constexpr int kLdsFar = 160;    // far rows a range may cache
+constexpr int kWinEll = 8;      // slots of the window row table
+constexpr int kWinEnt = kLdsRows * kWinEll;  // index entries of a chunk
+static_assert(kLdsSlots >= kLdsBack + kLdsAhead + 2, "ring too small for the window");
+
+// bytes of a chunk's packed index block: kWinEnt int32 columns, then
+// kWinEnt weights
+template <typename T>
+constexpr int win_block_bytes() { return kWinEnt * (4 + (int)sizeof(T)); }
+
+// Ranges alternate direction: even ranges walk their rows up, odd ones
+// down, so two neighbouring ranges reach their common boundary at the same
+// time (both at the start or both at the end) and the rows either side
+// read as halo are fetched once.  A range's walking position p is row
+// rs + p (up) or re - 1 - p (down); the table is laid out by (range,
+// position).  Blocks are mapped to (range, tile) pairs in contiguous runs
+// per XCD (workgroups go round-robin over the 8 XCDs), so the tiles of one
+// range share an XCD's L2 for the index data.
+// Every load of the main loop is unconditional (rows clamped into the
+// matrix, chunks clamped into the table): a load issued under a branch makes
+// the compiler wait for all outstanding loads at the next use of any of
+// them, which would drain the pipeline every step.  Clamped rows land in
+// ring rows no neighbour index maps to.
+template <typename T, int D>
+__global__ void __launch_bounds__(512)
+kl_spmm_dot(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
+            const int32_t *__restrict__ far_ptr, const int32_t *__restrict__ far_rows,
+            int64_t k, const T *__restrict__ P, T *__restrict__ Q, double *__restrict__ part,
+            const int32_t *__restrict__ ctl) {
+  if (ctl_idle(ctl)) return;
+  constexpr int VEC = 16 / sizeof(T);
+  constexpr int W = 16 * VEC;          // columns of a tile (256 bytes)
+  constexpr int CH = kLdsRows * W;     // elements of a chunk
+  constexpr int IB = win_block_bytes<T>();
+  // index words a thread stages (one T-sized word each; threads past the
+  // block re-load its last word)
+  using IW = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
+  constexpr int NIW = IB / (int)sizeof(IW);
+  static_assert(NIW <= 512, "index block larger than one word a thread");
+  // (range, tile) of this block
+  const int32_t R = (n + rpr - 1) / rpr;
+  const int32_t tiles = (int32_t)((k + W - 1) / W);
+  const int32_t pairs = R * tiles, per = (pairs + kXcd - 1) / kXcd;
+  const int32_t pair = (int32_t)(blockIdx.x % kXcd) * per + (int32_t)(blockIdx.x / kXcd);
+  if (pair >= pairs) return;
+  const int32_t x = pair / tiles, y = pair - x * tiles;
+  // the ring of chunks, then the far rows
+  __shared__ __align__(16) T ring[kLdsSlots * CH + kLdsFar * W];
+  T *const farc = ring + kLdsSlots * CH;
+  __shared__ __align__(16) IW stage[2][NIW];
+  const int tid = threadIdx.x;
+  const int rsub = tid >> 4;
+  const int lq = (tid & 15) * VEC;
+  const int64_t c0 = (int64_t)y * W + lq;
+  const bool valid = c0 < k;
+  const int64_t cl = valid ? c0 : k - VEC;   // in-bounds loads for lanes past k
+  const int32_t rs = x * rpr;
+  const int32_t re = min(n, rs + rpr);
+  const bool down = (x & 1) != 0;
+  const int32_t len = re - rs;
+  const int32_t nch = (len + kLdsRows - 1) / kLdsRows;
+  const int32_t cbase = x * (rpr / kLdsRows);                // first index block
+  const int32_t clast = R * (rpr / kLdsRows) - 1;            // last index block
+  const int iw = min(tid, NIW - 1);
+  auto row_of = [&](int32_t p) { return down ? re - 1 - p : rs + p; };
+  // far rows of the range, once
+  {
+    const int32_t fa = far_ptr[x], fb = far_ptr[x + 1];
+    for (int32_t f = fa + rsub; f < fb; f += kLdsRows) {
+      T v[VEC];
+      VecIO<T, VEC>::ld(P + (int64_t)far_rows[f] * k + cl, v);
+      VecIO<T, VEC>::st(farc + (f - fa) * W + lq, v);
+    }
+  }
+  T pb[D][VEC];
+  IW ib[D];
+  auto issue = [&](int32_t t, T (&v)[VEC], IW &w) {
+    const int32_t g = t + kLdsAhead + D;                 // P chunk
+    const int32_t row = min(max(row_of(g * kLdsRows + rsub), 0), n - 1);
+    VecIO<T, VEC>::ld(P + (int64_t)row * k + cl, v);
+    const int32_t gi = min(max(cbase + t + D, 0), clast);  // index chunk
+    w = reinterpret_cast<const IW *>(wpk + (int64_t)gi * IB)[iw];
+  };
+  auto land = [&](int32_t t, const T (&v)[VEC], IW w) {
+    const int32_t g = t + kLdsAhead;
+    VecIO<T, VEC>::st(ring + ((g + kLdsSlots) % kLdsSlots) * CH + rsub * W + lq, v);
+    stage[t & 1][iw] = w;
+  };
+  double pd[VEC] = {};
+  auto compute = [&](int32_t t) {
+    const int32_t p = t * kLdsRows + rsub;
+    if (p >= len) return;
+    const int32_t row = row_of(p);
+    const int32_t *jc = reinterpret_cast<const int32_t *>(&stage[t & 1][0]) + rsub * kWinEll;
+    const T *wv = reinterpret_cast<const T *>(reinterpret_cast<const unsigned char *>(
+                      &stage[t & 1][0]) + 4 * kWinEnt) + rsub * kWinEll;
+    T acc[VEC] = {}, x0[VEC];
+#pragma unroll
+    for (int u = 0; u < kWinEll; ++u) {
+      T v[VEC];
+      VecIO<T, VEC>::ld(ring + jc[u] * W + lq, v);
+      const T w = wv[u];
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) acc[m] = fma(w, v[m], acc[m]);
+      if (u == 0) {
+#pragma unroll
+        for (int m = 0; m < VEC; ++m) x0[m] = v[m];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < VEC; ++m) pd[m] += (double)x0[m] * (double)acc[m];
+    if (valid) VecIO<T, VEC>::st(Q + (int64_t)row * k + c0, acc);
+  };
+  const int32_t t0 = -kLdsBack - kLdsAhead;
+  // prologue: the first D steps' loads
+#pragma unroll
+  for (int u = 0; u < D; ++u) issue(t0 - D + u, pb[u], ib[u]);
+  __syncthreads();  // the far cache
+  // steps t0 .. nch - 1, rounded up to whole groups of D: every group runs
+  // all its steps (the ones past nch compute nothing), so the compiler sees
+  // the same loads in flight on every path and waits only for the oldest
+  const int32_t groups = (nch - t0 + D - 1) / D;
+  for (int32_t gi = 0; gi < groups; ++gi) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int32_t t = t0 + gi * D + u;
+      land(t, pb[u], ib[u]);
+      issue(t, pb[u], ib[u]);
+      __syncthreads();
+      if (t >= 0 && t < nch) compute(t);
+    }
+  }
+  // block partials of p.q: the 32 row slots of each column, in order
+  __syncthreads();
+  double *sh = reinterpret_cast<double *>(ring);
+#pragma unroll
+  for (int m = 0; m < VEC; ++m) sh[rsub * W + lq + m] = pd[m];
+  __syncthreads();
+  if (tid < W) {
+    double sum = 0.0;
+    for (int q = 0; q < kLdsRows; ++q) sum += sh[q * W + tid];
+    const int64_t c = (int64_t)y * W + tid;
+    if (c < k) part[(int64_t)x * k + c] = sum;
+  }
+}
+
+// The packed index blocks of the window row table from the CSR values:
+// block c holds the LDS rows of its 32 walking positions' kWinEll slots,
+// then their weights (q < 0: weight 0).
+template <typename T>
+__global__ void k_win_pack(int64_t total, const int32_t *__restrict__ wcol,
+                           const int32_t *__restrict__ wq, const double *__restrict__ val,
+                           unsigned char *__restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int64_t c = e / kWinEnt;
+  const int i = (int)(e - c * kWinEnt);
+  const int32_t q = wq[e];
+  unsigned char *b = out + c * win_block_bytes<T>();
+  reinterpret_cast<int32_t *>(b)[i] = wcol[e];
+  reinterpret_cast<T *>(b + 4 * kWinEnt)[i] = q >= 0 ? (T)val[q] : T(0);
+}
+
 // Row table of A_ff.  Row r lists its diagonal first (column r, weight 0
 // if the row stores none), then its other entries in column order; slots
 // past the row's length repeat its last column with weight 0, and entries
@@ -1675,6 +1862,10 @@ struct SysView {
   // row table of A_ff (SpMM); ovp = nullptr when no row overflows it
   const int32_t *ecol = nullptr, *ovp = nullptr, *ovc = nullptr;
   const T *eval = nullptr, *ovv = nullptr;
+  // window row table of the LDS-staged SpMM; rpr = 0 when not in use
+  int32_t rpr = 0;
+  const unsigned char *wpk = nullptr;
+  const int32_t *far_ptr = nullptr, *far_rows = nullptr;
   // block sweeps (forward: L, backward: U); mb = 0 when not available
   int32_t mb = 0;
   SweepView<T> lsw{}, usw{};
@@ -1702,9 +1893,37 @@ static Grid grid_spmm(int64_t n, int64_t k) {
   return G;
 }
 
+// Rows per range of the LDS-staged SpMM: one block per CU over the
+// (range, column tile) grid, ranges a whole number of 32-row chunks.
+template <typename T>
+static int32_t win_rpr(int64_t n, int64_t k) {
+  const int64_t W = 256 / sizeof(T);
+  const int64_t tiles = (k + W - 1) / W;
+  const int64_t cus = std::max<int64_t>(1, resident_blocks() / 8);
+  const int64_t R = std::max<int64_t>(1, cus / tiles);
+  const int64_t per = (n + R - 1) / R;
+  return (int32_t)std::max<int64_t>(kLdsRows, (per + kLdsRows - 1) / kLdsRows * kLdsRows);
+}
+
+// the LDS-staged SpMM applies to wide k whose rows are 16-byte aligned
+template <typename T>
+static bool win_applies(int64_t n, int64_t k) {
+  const int vec = 16 / (int)sizeof(T);
+  return k > 4 && k % vec == 0 && n >= 4 * kLdsRows;
+}
+
 template <typename T>
 static int32_t launch_spmm_dot(const SysView<T> &V, int64_t k, const T *P, T *Q,
                                double *part, const int32_t *ctl, hipStream_t s) {
+  if (V.rpr > 0 && win_applies<T>(V.n, k)) {
+    const int64_t W = 256 / sizeof(T);
+    const int64_t R = (V.n + V.rpr - 1) / V.rpr, pairs = R * ((k + W - 1) / W);
+    const dim3 g((unsigned)(kXcd * ((pairs + kXcd - 1) / kXcd)));
+    ProfScope ps("spmm_lds", s);
+    kl_spmm_dot<T, 6><<<g, 512, 0, s>>>(V.n, V.rpr, V.wpk, V.far_ptr, V.far_rows, k, P, Q,
+                                        part, ctl);
+    return (int32_t)R;
+  }
   Grid G = grid_spmm<T>(V.n, k);
   const int32_t gens = (int32_t)(G.g.x / std::max(1u, grid_rows<T>(V.n, k, kCapSpmm).g.x));
   ProfScope ps("spmm", s);
@@ -2111,6 +2330,140 @@ static void ensure_ell(cn_system_t *S, hipStream_t s) {
   CN_HIP(hipGetLastError());
 }
 
+// Window row table for ranges of rpr rows (see kl_spmm_dot); refills the
+// weights on every call.  Returns false when the matrix does not fit the
+// window (a row of more than kWinEll entries, or more than kLdsFar far rows
+// in a range).
+static bool ensure_win(cn_system_t *S, int32_t rpr, hipStream_t s) {
+  const int32_t n = S->n_free;
+  if (n <= 0 || rpr <= 0) return false;
+  if (S->win_rpr != rpr) {
+    S->win_rpr = rpr;
+    S->win_ok = -1;
+    std::vector<int32_t> ptr((size_t)n + 1);
+    CN_HIP(hipMemcpyAsync(ptr.data(), S->ff_ptr.get(), sizeof(int32_t) * ((size_t)n + 1),
+                          hipMemcpyDeviceToHost, s));
+    CN_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> col((size_t)std::max(ptr[n], 1));
+    if (ptr[n] > 0) {
+      CN_HIP(hipMemcpyAsync(col.data(), S->ff_col.get(), sizeof(int32_t) * (size_t)ptr[n],
+                            hipMemcpyDeviceToHost, s));
+      CN_HIP(hipStreamSynchronize(s));
+    }
+    const int32_t R = (n + rpr - 1) / rpr;
+    // entries by (range, walking position); positions past a range's rows
+    // point at LDS row 0 with weight 0
+    std::vector<int32_t> wc((size_t)R * rpr * kWinEll, 0), wq((size_t)R * rpr * kWinEll, -1);
+    std::vector<int32_t> fp(R + 1, 0), fr;
+    bool ok = true;
+    for (int32_t x = 0; x < R && ok; ++x) {
+      const int32_t rs = x * rpr, re = std::min(n, rs + rpr);
+      const bool down = (x & 1) != 0;
+      auto pos = [&](int32_t j) { return down ? re - 1 - j : j - rs; };  // walking position
+      auto chunk = [&](int32_t j) {  // floor(pos / kLdsRows)
+        const int32_t d = pos(j);
+        return d >= 0 ? d / kLdsRows : -((-d + kLdsRows - 1) / kLdsRows);
+      };
+      std::vector<int32_t> far;  // this range's far rows, sorted
+      for (int32_t r = rs; r < re; ++r) {
+        const int32_t tr = chunk(r);
+        for (int32_t q = ptr[r]; q < ptr[r + 1]; ++q) {
+          const int32_t gj = chunk(col[q]);
+          if (gj < tr - kLdsBack || gj > tr + kLdsAhead) far.push_back(col[q]);
+        }
+      }
+      std::sort(far.begin(), far.end());
+      far.erase(std::unique(far.begin(), far.end()), far.end());
+      if ((int32_t)far.size() > kLdsFar) { ok = false; break; }
+      for (int32_t r = rs; r < re && ok; ++r) {
+        const int32_t tr = chunk(r);
+        // the LDS row of neighbour j while row r is computed
+        auto enc = [&](int32_t j) {
+          const int32_t gj = chunk(j);
+          if (gj >= tr - kLdsBack && gj <= tr + kLdsAhead)
+            return ((gj + kLdsSlots) % kLdsSlots) * kLdsRows + (pos(j) & (kLdsRows - 1));
+          const int32_t f = (int32_t)(std::lower_bound(far.begin(), far.end(), j) - far.begin());
+          return kLdsSlots * kLdsRows + f;
+        };
+        const size_t o = ((size_t)x * rpr + (size_t)pos(r)) * kWinEll;
+        int32_t *c = &wc[o], *qq = &wq[o];
+        int32_t dq = -1, u = 1, last = r;
+        for (int32_t q = ptr[r]; q < ptr[r + 1]; ++q) {
+          if (col[q] == r) { dq = q; continue; }
+          if (u >= kWinEll) { ok = false; break; }
+          c[u] = enc(col[q]);
+          qq[u] = q;
+          last = col[q];
+          ++u;
+        }
+        c[0] = enc(r);
+        qq[0] = dq;
+        for (; u < kWinEll; ++u) { c[u] = enc(last); qq[u] = -1; }
+      }
+      fp[x + 1] = fp[x] + (int32_t)far.size();
+      fr.insert(fr.end(), far.begin(), far.end());
+    }
+    if (!ok) return false;
+    S->win_col.alloc(wc.size());
+    S->win_q.alloc(wq.size());
+    S->win_pk32.release();
+    S->win_pk64.release();
+    S->win_far_ptr.alloc(fp.size());
+    S->win_far_rows.alloc(std::max<size_t>(fr.size(), 1));
+    CN_HIP(hipMemcpyAsync(S->win_col.get(), wc.data(), sizeof(int32_t) * wc.size(),
+                          hipMemcpyHostToDevice, s));
+    CN_HIP(hipMemcpyAsync(S->win_q.get(), wq.data(), sizeof(int32_t) * wq.size(),
+                          hipMemcpyHostToDevice, s));
+    CN_HIP(hipMemcpyAsync(S->win_far_ptr.get(), fp.data(), sizeof(int32_t) * fp.size(),
+                          hipMemcpyHostToDevice, s));
+    if (!fr.empty())
+      CN_HIP(hipMemcpyAsync(S->win_far_rows.get(), fr.data(), sizeof(int32_t) * fr.size(),
+                            hipMemcpyHostToDevice, s));
+    CN_HIP(hipStreamSynchronize(s));
+    S->win_ok = 1;
+  }
+  if (S->win_ok != 1) return false;
+  const int64_t total = (int64_t)S->win_col.n;  // (R * rpr) positions x kWinEll
+  const int64_t nblocks = total / kWinEnt;
+  if (!S->win_pk32.get()) {
+    S->win_pk32.alloc((size_t)nblocks * win_block_bytes<float>());
+    S->win_pk64.alloc((size_t)nblocks * win_block_bytes<double>());
+  }
+  k_win_pack<float><<<grid_for(total, 256), 256, 0, s>>>(total, S->win_col.get(),
+                                                        S->win_q.get(), S->ff_val.get(),
+                                                        S->win_pk32.get());
+  k_win_pack<double><<<grid_for(total, 256), 256, 0, s>>>(total, S->win_col.get(),
+                                                         S->win_q.get(), S->ff_val.get(),
+                                                         S->win_pk64.get());
+  CN_HIP(hipGetLastError());
+  return true;
+}
+
+// which SpMM form the solver runs: the LDS-staged one unless CN_SPMM=row
+static bool spmm_lds_wanted() {
+  const char *e = std::getenv("CN_SPMM");
+  return !(e && std::strcmp(e, "row") == 0);
+}
+
+template <typename T>
+static void use_win(cn_system_t *S, SysView<T> &V, int64_t k, hipStream_t s);
+template <>
+void use_win<float>(cn_system_t *S, SysView<float> &V, int64_t k, hipStream_t s) {
+  if (!spmm_lds_wanted() || !win_applies<float>(S->n_free, k)) return;
+  if (!ensure_win(S, win_rpr<float>(S->n_free, k), s)) return;
+  V.rpr = S->win_rpr;
+  V.wpk = S->win_pk32.get();
+  V.far_ptr = S->win_far_ptr.get(); V.far_rows = S->win_far_rows.get();
+}
+template <>
+void use_win<double>(cn_system_t *S, SysView<double> &V, int64_t k, hipStream_t s) {
+  if (!spmm_lds_wanted() || !win_applies<double>(S->n_free, k)) return;
+  if (!ensure_win(S, win_rpr<double>(S->n_free, k), s)) return;
+  V.rpr = S->win_rpr;
+  V.wpk = S->win_pk64.get();
+  V.far_ptr = S->win_far_ptr.get(); V.far_rows = S->win_far_rows.get();
+}
+
 template <typename T>
 static SysView<T> make_view(cn_system_t *S, cn_ic0_t *F, SolverWork *W);
 
@@ -2192,6 +2545,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
                                          W->dinv32.get());
   ensure_ell(S, s);
   SysView<T> V = make_view<T>(S, F, W);
+  use_win<T>(S, V, k, s);
   V.hsc = H;
   V.sgs = G;
   V.sm = SegMap{batch ? S->seg_of_row.get() : nullptr, kk};
