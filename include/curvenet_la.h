@@ -188,7 +188,10 @@ cn_status cn_system_destroy(cn_system sys);
  * stop test and residual replacement -- so a segment's solution is the
  * solution of its system.  The batch copies the values: after
  * cn_system_set_positions on a member, rebuild it.  The right-hand side of
- * a segment comes from cn_harmonic_rhs on its member system. */
+ * a segment comes from cn_harmonic_rhs on its member system, or every
+ * segment's at once from cn_harmonic_rhs on the batch, which takes the
+ * members' constrained values stacked (sum of n_con) x k (one launch;
+ * the same bits as the per-member calls). */
 cn_status cn_system_batch(int32_t nsys, const cn_system *systems, cn_system *out,
                           void *stream);
 /* Segments of a system: *nseg = 1 and h_seg_ptr = {0, n_free} for a
@@ -546,6 +549,12 @@ typedef struct cn_pcg_opts {
   int32_t max_refine;     /* MIXED: refinement steps cap                      */
   int32_t check_every;    /* device-side stop flags are read every n iters    */
   int32_t warm_start;     /* 1: d_x holds x0 (cgWithGuess), 0: x0 = 0 (cg)    */
+  int32_t graph;          /* 1: the fused solvers run the iterations between
+                             two reads of the stop flags as one captured
+                             hipGraph, replayed (captured once per system,
+                             preconditioner and k); 0: launch each kernel.
+                             The results are the same bits either way.
+                             Not used while cn_profile_enable is on.        */
 } cn_pcg_opts;
 
 typedef struct cn_solve_info {
@@ -555,7 +564,8 @@ typedef struct cn_solve_info {
   double relres_max;      /* max over columns of ||b-Ax||/||b|| (FP64)        */
 } cn_solve_info;
 
-/* Fill opts with defaults (IC0, MIXED, tol 1e-9, 20000 iters, inner 1e-5). */
+/* Fill opts with defaults (IC0, MIXED, tol 1e-9, 20000 iters, inner 1e-5,
+ * graph replay on). */
 void cn_pcg_default_opts(cn_pcg_opts *opts);
 
 /* Solve A_ff X = B for k columns.  d_b / d_x are n_free x k FP64; d_x

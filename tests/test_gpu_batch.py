@@ -92,3 +92,18 @@ def test_batch_rejects_ic0_and_plain(gpu, batch):
         B.solve(b, precond="jacobi", precision="fp64_plain")
     with pytest.raises((CurvenetError, Exception)):
         B.solve(b, precond="ic0", precision="mixed")
+
+
+def test_batch_harmonic_rhs_one_launch(gpu):
+    """cn_harmonic_rhs on a batch builds every member's right-hand side from
+    the stacked constrained values: the same bits as the member calls."""
+    from tool_godot_curvenet_amd import BatchSystem, CutLaplacian, meshgen
+    laps = [CutLaplacian.from_mesh(meshgen.capsule_union(c, c + 2, t, seed=31 + c))
+            for c, t in ((1, 2500), (2, 7000), (1, 4000))]
+    batch = BatchSystem(laps)
+    rng = np.random.default_rng(2)
+    for k in (1, 3, 8):
+        Xc = [rng.uniform(-0.2, 0.2, (l.n_con, k)) for l in laps]
+        got = batch.harmonic_rhs(Xc)
+        want = batch.stack([l.harmonic_rhs(x) for l, x in zip(laps, Xc)])
+        assert np.array_equal(got, want), k

@@ -143,3 +143,44 @@ def test_spmm_forms_agree(gpu, monkeypatch, precision, tol, name, k):
         xs[form] = x.cpu().numpy()
     d = np.abs(xs["row"] - xs["lds"]).max() / np.abs(xs["row"]).max()
     assert d <= tol, d
+
+
+@pytest.mark.parametrize("name,k,precond,precision", [
+    ("capsules100k", 768, "hsc", "mixed"),
+    ("icosphere6", 48, "hsc", "fp64"),
+    ("icosphere6", 3, "hsc", "mixed"),
+    ("cylinder64", 3, "ic0", "mixed"),
+    ("cylinder64", 5, "jacobi", "fp64"),
+])
+def test_graph_replay_bit_identical(gpu, name, k, precond, precision):
+    """The inner iterations replayed from a captured hipGraph run the same
+    kernels in the same order as direct launches: the same bits, also on
+    a second solve that replays the cached graph."""
+    from tool_godot_curvenet_amd import HSC, IC0, CutLaplacian, meshgen
+    lap = CutLaplacian.from_mesh(meshgen.config_mesh(name))
+    B = lap.harmonic_rhs(meshgen.seeded_boundary(lap.n_con, k, seed=3 * k))
+    kw = {"hsc": HSC(lap)} if precond == "hsc" else ({"ic0": IC0(lap)} if precond == "ic0"
+                                                     else {})
+    x0, i0 = lap.solve(B, precond=precond, precision=precision, tol=1e-9, graph=False, **kw)
+    for rep in range(2):
+        x1, i1 = lap.solve(B, precond=precond, precision=precision, tol=1e-9, graph=True, **kw)
+        assert i1.status == 0 and i1.iters == i0.iters
+        assert np.array_equal(x0.cpu().numpy(), x1.cpu().numpy()), rep
+    A = lap.A_ff()
+    R = B - O.spmm(A.row_ptr, A.col_idx, A.values, x1.cpu().numpy())
+    assert np.all(np.linalg.norm(R, axis=0) <= 1e-9 * np.linalg.norm(B, axis=0))
+
+
+def test_graph_replay_batch(gpu):
+    """A batch of systems (per-member scalars): replayed equals direct."""
+    from tool_godot_curvenet_amd import HSC, BatchSystem, CutLaplacian, meshgen
+    laps = [CutLaplacian.from_mesh(meshgen.capsule_union(c, c + 2, t, seed=11 + c))
+            for c, t in ((1, 3000), (2, 9000), (3, 20000))]
+    batch = BatchSystem(laps)
+    rng = np.random.default_rng(7)
+    B = batch.stack([l.harmonic_rhs(rng.uniform(-0.2, 0.2, (l.n_con, 3))) for l in laps])
+    h = HSC(batch, max_levels=48)
+    x0, i0 = batch.solve(B, precond="hsc", precision="mixed", tol=1e-9, hsc=h, graph=False)
+    x1, i1 = batch.solve(B, precond="hsc", precision="mixed", tol=1e-9, hsc=h, graph=True)
+    assert i0.status == 0 and i1.iters == i0.iters
+    assert np.array_equal(x0.cpu().numpy(), x1.cpu().numpy())

@@ -62,7 +62,7 @@ class Ic0Info(C.Structure):
 class PcgOpts(C.Structure):
     _fields_ = [("precond", i32), ("precision", i32), ("tol_rel", f64), ("max_iter", i32),
                 ("inner_tol", f64), ("max_refine", i32), ("check_every", i32),
-                ("warm_start", i32)]
+                ("warm_start", i32), ("graph", i32)]
 
 
 class HscOpts(C.Structure):

@@ -349,6 +349,65 @@ static void ensure_rhs_table(cn_system_t *S, hipStream_t s) {
   S->rhs_ready = true;
 }
 
+void batch_rhs_table(cn_system_t *B, const cn_system_t *const *members, int32_t nsys,
+                     hipStream_t s) {
+  std::vector<int32_t> gptr(1, 0), eptr(1, 0), cc, midx;
+  int64_t faces = 0;
+  for (int32_t i = 0; i < nsys; ++i) faces += members[i]->nf;
+  if (9 * faces > INT32_MAX) throw HipError{hipErrorInvalidValue, "batch face table > int32"};
+  B->face_m.alloc((size_t)std::max<int64_t>(9 * faces, 1));
+  int32_t con0 = 0;
+  int64_t face0 = 0;
+  for (int32_t i = 0; i < nsys; ++i) {
+    cn_system_t *A = const_cast<cn_system_t *>(members[i]);
+    const int32_t n = A->n_free;
+    if (n > 0 && A->nv > 0) ensure_rhs_table(A, s);
+    std::vector<int32_t> g((size_t)n + 1, 0);
+    if (n > 0 && A->rhs_ready)
+      CN_HIP(hipMemcpyAsync(g.data(), A->rhs_gptr.get(), sizeof(int32_t) * g.size(),
+                            hipMemcpyDeviceToHost, s));
+    CN_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> e((size_t)g[n] + 1, 0);
+    if (g[n] > 0)
+      CN_HIP(hipMemcpyAsync(e.data(), A->rhs_eptr.get(), sizeof(int32_t) * e.size(),
+                            hipMemcpyDeviceToHost, s));
+    CN_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> c((size_t)e[g[n]]), m((size_t)e[g[n]]);
+    if (!c.empty()) {
+      CN_HIP(hipMemcpyAsync(c.data(), A->rhs_cc.get(), sizeof(int32_t) * c.size(),
+                            hipMemcpyDeviceToHost, s));
+      CN_HIP(hipMemcpyAsync(m.data(), A->rhs_midx.get(), sizeof(int32_t) * m.size(),
+                            hipMemcpyDeviceToHost, s));
+    }
+    if (A->nf > 0)
+      CN_HIP(hipMemcpyAsync(B->face_m.get() + 9 * face0, A->face_m.get(),
+                            sizeof(double) * 9 * (size_t)A->nf, hipMemcpyDeviceToDevice, s));
+    CN_HIP(hipStreamSynchronize(s));
+    const int32_t gbase = (int32_t)eptr.size() - 1, ebase = (int32_t)cc.size();
+    for (int32_t r = 0; r < n; ++r) gptr.push_back(gbase + g[r + 1]);
+    for (int32_t q = 1; q <= g[n]; ++q) eptr.push_back(ebase + e[q]);
+    for (size_t q = 0; q < c.size(); ++q) {
+      cc.push_back(con0 + c[q]);
+      midx.push_back((int32_t)(9 * face0) + m[q]);
+    }
+    con0 += A->n_con;
+    face0 += A->nf;
+  }
+  auto up = [&](DevBuf<int32_t> &d, const std::vector<int32_t> &h) {
+    d.alloc(std::max<size_t>(h.size(), 1));
+    if (!h.empty())
+      CN_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(int32_t) * h.size(),
+                            hipMemcpyHostToDevice, s));
+  };
+  up(B->rhs_gptr, gptr);
+  up(B->rhs_eptr, eptr);
+  up(B->rhs_cc, cc);
+  up(B->rhs_midx, midx);
+  CN_HIP(hipStreamSynchronize(s));
+  B->rhs_ncon = con0;
+  B->rhs_ready = true;
+}
+
 static void launch_harmonic_rhs(cn_system_t *S, int32_t k, const double *Xc, double *rhs,
                                 hipStream_t s) {
   const int v = (k % 2 == 0) ? 2 : 1;
@@ -639,14 +698,19 @@ cn_status cn_system_destroy(cn_system S) {
 cn_status cn_harmonic_rhs(cn_system S, int32_t k, const double *d_Xc,
                           double *d_rhs, void *stream) {
   CN_API_BEGIN
-  if (!S || k <= 0 || (S->n_con > 0 && !d_Xc) || !d_rhs || (S->nv == 0 && S->n_free > 0)) {
+  // a batch (cn_system_batch) takes its members' constrained values
+  // stacked, through the table built with it
+  const bool batch = S && S->nseg > 1;
+  const int32_t ncon = S ? (batch ? S->rhs_ncon : S->n_con) : 0;
+  if (!S || k <= 0 || (ncon > 0 && !d_Xc) || !d_rhs ||
+      (!batch && S->nv == 0 && S->n_free > 0) || (batch && !S->rhs_ready)) {
     set_error("cn_harmonic_rhs: bad arguments");
     return CN_BAD_ARG;
   }
   hipStream_t s = as_stream(stream);
   int64_t tot = (int64_t)S->n_free * k;
   if (tot > 0) {
-    ensure_rhs_table(S, s);
+    if (!batch) ensure_rhs_table(S, s);
     launch_harmonic_rhs(S, k, d_Xc, d_rhs, s);
   }
   CN_HIP(hipGetLastError());

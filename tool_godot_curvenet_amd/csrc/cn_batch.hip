@@ -159,6 +159,7 @@ cn_status cn_system_batch(int32_t nsys, const cn_system *systems, cn_system *out
                           hipMemcpyHostToDevice, s));
     CN_HIP(hipGetLastError());
     CN_HIP(hipStreamSynchronize(s));
+    batch_rhs_table(B, systems, nsys, s);
   } catch (...) {
     delete B;
     throw;

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -112,6 +113,14 @@ void build_outgoing(int32_t nv, int32_t nf, const int32_t *d_tris, Buckets &b,
 struct SolverWork;
 void free_solver_work(SolverWork *w);
 
+// A serial number for every preconditioner handle, never reused: cached
+// captured graphs name the handles they were captured with by serial, since
+// a new handle may get the address of a destroyed one.
+inline uint64_t next_serial() {
+  static std::atomic<uint64_t> c{1};
+  return c.fetch_add(1);
+}
+
 }  // namespace cn
 
 // ------------------------------------------------------------ handles --
@@ -128,6 +137,10 @@ struct cn_system_t;
 namespace cn {
 // one segment over every row of a single system, if it has no segments yet
 void ensure_segments(cn_system_t *S, hipStream_t s);
+// the right-hand side table of batch B from its members' (cn_harmonic_rhs
+// over the batch then builds every member's right-hand side in one launch)
+void batch_rhs_table(cn_system_t *B, const cn_system_t *const *members, int32_t nsys,
+                     hipStream_t s);
 }  // namespace cn
 
 struct cn_system_t {
@@ -158,6 +171,10 @@ struct cn_system_t {
   // that are structurally zero
   cn::DevBuf<int32_t> rhs_gptr, rhs_eptr, rhs_cc, rhs_midx;
   bool rhs_ready = false;
+  // a batch's right-hand side table is its members' tables end to end (rows,
+  // constrained columns and face blocks offset per member, face blocks
+  // copied into face_m): the constrained columns it takes, stacked
+  int32_t rhs_ncon = 0;
   // A_ff as a fixed-width row table for the solver's multi-column SpMM:
   // the first kEll entries of every row (short rows padded with their last
   // column and weight 0), the rest in an overflow CSR.  The pattern is built
@@ -192,6 +209,7 @@ struct cn_system_t {
 // Colour classes of A_ff for the SgsColor sweeps (pattern only: the
 // values and the diagonal are read from the system at every sweep).
 struct cn_sgs_t {
+  uint64_t serial = cn::next_serial();
   cn_system_t *sys = nullptr;
   int32_t n = 0, ncolors = 0;
   std::vector<int32_t> color_ptr;   // host: rows of colour c are
@@ -207,6 +225,7 @@ struct cn_restrict_t {
 };
 
 struct cn_ic0_t {
+  uint64_t serial = cn::next_serial();
   int32_t n = 0, ncolors = 0, ordering = 0, block_size = 1, max_block = 1;
   double shift = 0.0;
   // Rows are renumbered into levels (colours), each level into blocks, each
@@ -255,6 +274,7 @@ struct cn_ic0_t {
 
 // Hierarchical Schur preconditioner (see cn_hsc_build.hip).
 struct cn_hsc_t {
+  uint64_t serial = cn::next_serial();
   int32_t n0 = 0;  // rows of level 0 (= the system's n_free)
   cn_hsc_opts opts{};
   struct Level {

@@ -1687,7 +1687,15 @@ struct SolverWork {
   DevBuf<float> dinv32;
   int32_t *h_ctl = nullptr;           // pinned
   double *h_st = nullptr;             // pinned, ST_N x nvc
+  // the captured block of inner iterations (fused solve), replayed while
+  // the key -- everything its launches depend on -- is unchanged; captured
+  // on its own stream (a graph cannot be captured on the legacy stream)
+  hipStream_t cap = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> gkey;
   ~SolverWork() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (cap) (void)hipStreamDestroy(cap);
     if (h_ctl) (void)hipHostFree(h_ctl);
     if (h_st) (void)hipHostFree(h_st);
   }
@@ -2694,27 +2702,64 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     // a whole-apply preconditioner converges in a few iterations per
     // residual replacement: read the flags often enough not to idle long
     const int32_t every = std::max(1, H ? std::min(o->check_every, 2) : o->check_every);
-    while (it < budget) {
-      rows = launch_spmm_dot<T>(V, k, P, Q, p0, ctl, s);
+    // one CG iteration: every launch reads its scalars and flags from the
+    // device, so the same launches serve every iteration
+    auto iterate = [&](hipStream_t q) {
+      int32_t rw = launch_spmm_dot<T>(V, k, P, Q, p0, ctl, q);
       if (batch) {
-        launch_segdot<T, T>(S, W, k, P, Q, nullptr, nullptr, s);
-        k_reduce_alpha<<<red, kBlock, 0, s>>>(1, kk, sg0, st, W->active.get(), ctl);
+        launch_segdot<T, T>(S, W, k, P, Q, nullptr, nullptr, q);
+        k_reduce_alpha<<<red, kBlock, 0, q>>>(1, kk, sg0, st, W->active.get(), ctl);
       } else {
-        rd = reduce_rows(W, p0, rows, k, stride, 1, s);
-        k_reduce_alpha<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st, W->active.get(), ctl);
+        Red r1 = reduce_rows(W, p0, rw, k, stride, 1, q);
+        k_reduce_alpha<<<red, kBlock, 0, q>>>(r1.rows, k, r1.p, st, W->active.get(), ctl);
       }
-      rows = launch_fwd<T, true>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, s);
+      rw = launch_fwd<T, true>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, q);
       if (batch) {
-        launch_segdot<T, T>(S, W, k, R, R, R, Z, s);
-        k_reduce_beta<<<red, kBlock, 0, s>>>(1, kk, sg0, sg1, st, W->active.get(),
+        launch_segdot<T, T>(S, W, k, R, R, R, Z, q);
+        k_reduce_beta<<<red, kBlock, 0, q>>>(1, kk, sg0, sg1, st, W->active.get(),
                                              W->restart.get(), ctl, 0, delta2);
       } else {
-        rd = reduce_rows(W, p1, rows, k, stride, 2, s);
-        k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, rd.p + rd.stride, st,
+        Red r2 = reduce_rows(W, p1, rw, k, stride, 2, q);
+        k_reduce_beta<<<red, kBlock, 0, q>>>(r2.rows, k, r2.p, r2.p + r2.stride, st,
                                              W->active.get(), W->restart.get(), ctl, 0, delta2);
       }
-      launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
-      ++it;
+      launch_bwd<T>(V, k, st, Z, P, D, ctl, q);
+    };
+    const bool use_graph = o->graph && !g_prof.on && every > 1;
+    if (use_graph) {
+      // everything the captured launches depend on besides W's own buffers
+      const std::vector<uint64_t> key = {
+          (uint64_t)k, sizeof(T), (uint64_t)every, (uint64_t)kk,
+          H ? H->serial : 0, H ? (uint64_t)(uintptr_t)H->work.get() : 0,
+          F ? F->serial : 0, G ? G->serial : 0, (uint64_t)V.rpr,
+          (uint64_t)(uintptr_t)V.wpk, (uint64_t)(uintptr_t)V.ecol};
+      if (!W->gexec || W->gkey != key) {
+        if (W->gexec) { CN_HIP(hipGraphExecDestroy(W->gexec)); W->gexec = nullptr; }
+        if (!W->cap) CN_HIP(hipStreamCreateWithFlags(&W->cap, hipStreamNonBlocking));
+        CN_HIP(hipStreamBeginCapture(W->cap, hipStreamCaptureModeThreadLocal));
+        hipGraph_t gr = nullptr;
+        try {
+          for (int32_t e = 0; e < every; ++e) iterate(W->cap);
+        } catch (...) {
+          (void)hipStreamEndCapture(W->cap, &gr);
+          if (gr) (void)hipGraphDestroy(gr);
+          throw;
+        }
+        CN_HIP(hipStreamEndCapture(W->cap, &gr));
+        const hipError_t ie = hipGraphInstantiate(&W->gexec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        CN_HIP(ie);
+        W->gkey = key;
+      }
+    }
+    while (it < budget) {
+      if (use_graph && budget - it >= every) {
+        CN_HIP(hipGraphLaunch(W->gexec, s));
+        it += every;
+      } else {
+        iterate(s);
+        ++it;
+      }
       if (it % every == 0 || it == budget) {
         CN_HIP(hipMemcpyAsync(hc, ctl, sizeof(int32_t) * 6, hipMemcpyDeviceToHost, s));
         CN_HIP(hipStreamSynchronize(s));
@@ -2902,6 +2947,7 @@ void cn_pcg_default_opts(cn_pcg_opts *o) {
   o->max_refine = 30;
   o->check_every = 8;
   o->warm_start = 0;
+  o->graph = 1;
 }
 
 void cn_profile_enable(int32_t on) {

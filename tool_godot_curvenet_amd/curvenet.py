@@ -216,9 +216,11 @@ class _System:
                          _np(i.d_ff_val, i.nnz_ff, np.float64))
 
     @staticmethod
-    def _opts(precond, precision, tol, max_iter, inner_tol, check_every, max_refine, warm):
+    def _opts(precond, precision, tol, max_iter, inner_tol, check_every, max_refine, warm,
+              graph=True):
         o = L.PcgOpts()
         L.load().cn_pcg_default_opts(C.byref(o))
+        o.graph = int(bool(graph))
         o.precond = {"ic0": L.CN_PRECOND_IC0, "jacobi": L.CN_PRECOND_JACOBI}[precond]
         o.precision = {"mixed": L.CN_PREC_MIXED, "fp64": L.CN_PREC_FP64,
                        "fp64_plain": L.CN_PREC_FP64_PLAIN}[precision]
@@ -229,9 +231,10 @@ class _System:
     def solve(self, b, precond: str = "ic0", precision: str = "mixed", tol: float = 1e-9,
               x0=None, ic0: "IC0 | None" = None, max_iter: int = 20000,
               inner_tol: float = 1e-5, check_every: int = 8, max_refine: int = 30,
-              hsc: "HSC | None" = None, sgs: "SGS | None" = None):
+              hsc: "HSC | None" = None, sgs: "SGS | None" = None, graph: bool = True):
         """Solve A_ff X = B (n x k or n); precond "ic0", "jacobi", "hsc" or
-        "sgs".  Returns (x, SolveInfo)."""
+        "sgs".  graph: replay the inner iterations as a captured hipGraph
+        (the same results as direct launches).  Returns (x, SolveInfo)."""
         torch = _torch()
         lib = L.load()
         bt = _as_tensor(b, torch.float64)
@@ -241,7 +244,7 @@ class _System:
         x = (_as_tensor(x0, torch.float64).reshape(self.n_free, k).clone()
              if x0 is not None else torch.zeros_like(bt))
         o = self._opts("jacobi" if precond in ("hsc", "sgs") else precond, precision, tol,
-                       max_iter, inner_tol, check_every, max_refine, x0 is not None)
+                       max_iter, inner_tol, check_every, max_refine, x0 is not None, graph)
         info = L.SolveInfo()
         if precond == "hsc":
             if hsc is None:
@@ -478,6 +481,20 @@ class BatchSystem(_System):
     @property
     def nseg(self) -> int:
         return len(self.seg_ptr) - 1
+
+    def harmonic_rhs(self, Xcs, as_numpy: bool = True):
+        """Every member's HarmonicSolve right-hand side in one launch: Xcs is
+        a list of the members' n_con_i x k constrained values (or those
+        stacked); returns the stacked (sum of n_free) x k block."""
+        torch = _torch()
+        lib = L.load()
+        x = _as_tensor(self.stack(Xcs) if isinstance(Xcs, (list, tuple)) else Xcs,
+                       torch.float64)
+        x = x.reshape(x.shape[0], -1).contiguous()
+        k = x.shape[1]
+        out = torch.empty((self.n_free, k), dtype=torch.float64, device="cuda")
+        L.check(lib.cn_harmonic_rhs(self._h, k, _p(x), _p(out), _stream()), "harmonic_rhs")
+        return out.cpu().numpy() if as_numpy else out
 
     def stack(self, blocks):
         """Concatenate per-member n_i x k blocks (numpy or device tensors)."""
