@@ -201,6 +201,11 @@ def main():
         _lib.check(st, "pcg_solve")
 
     info = _lib.SolveInfo()
+    # launch shapes: the autotuner times full solves of this workload's
+    # right-hand sides for each candidate (setup, outside the timed region)
+    _lib.check(lib.cn_harmonic_rhs(lap.handle, k, C.c_void_p(Xc.data_ptr()),
+                                   C.c_void_p(B.data_ptr()), sp), "harmonic_rhs")
+    tune = tune_for(lap, B, args, pre)
     for _ in range(args.warmup):
         step(info)
     torch.cuda.synchronize()
@@ -432,6 +437,7 @@ def main():
                      "avg_ms": kernels[dom]["avg_ms"], "launches": kernels[dom]["launches"],
                      "bytes_per_launch": kernels[dom]["bytes_per_launch"]},
         "kernels": kernels,
+        "tune": tune,
         "spmv_k1": spmv,
         "c1_dense_ge": c1,
         "coarse_mfma": coarse,
@@ -448,6 +454,23 @@ def main():
         line["speedup_vs_plain_fp64"] = value / plain[args.precond]["solves_per_s"]
         line["speedup_vs_plain_fp64_jacobi"] = value / plain["jacobi"]["solves_per_s"]
     print(json.dumps(line))
+
+
+def tune_for(system, B, args, pre, precond=None):
+    """Autotune the solver's launch shapes on this workload (no cache: the
+    box is timed as it is) and report the choice beside the built-in one."""
+    from tool_godot_curvenet_amd import autotune
+    precond = precond or args.precond
+    kw = {"hsc": pre} if precond == "hsc" else {"ic0": pre}
+    r = autotune.autotune(system, B, precond=precond, precision=args.precision, tol=args.tol,
+                          use_cache=False, **kw)
+    built_in = next((c["ms"] for c in r["candidates"]
+                     if c["tune"] == {"spmm_form": 2, "spmm_gens": 0, "level_cap": 0,
+                                      "tail_rows": 0}), None)
+    return {"chosen": r["tune"], "solve_ms": r["ms"], "built_in_solve_ms": built_in,
+            "candidates": len(r["candidates"]),
+            "how": "coordinate search over SpMM form / stripes, HSC level-pass grid cap, "
+                   "fused-tail size; full solves of this workload, best of 2"}
 
 
 def spmv_k1(lib, A, sp, reps=200):
@@ -607,6 +630,9 @@ def meshes64_bench(args):
         torch.cuda.synchronize()
         return time.perf_counter() - t, its
 
+    _lib.check(lib.cn_harmonic_rhs(batch.handle, k, C.c_void_p(Xcs.data_ptr()),
+                                   C.c_void_p(Bb.data_ptr()), sp), "harmonic_rhs")
+    tune = tune_for(batch, Bb, args, hb, precond="hsc")
     ed, _ = timed(step_batch_direct)
     eb, itb = timed(step_batch)
     print(f"meshes64: batched {len(laps) * args.steps / eb:.1f} solves/s (direct launches "
@@ -645,6 +671,7 @@ def meshes64_bench(args):
         "one_solve_per_mesh": {"solves_per_s": each, "ms_per_step": 1e3 * ee / args.steps,
                                "max_iters": max(ite)},
         "speedup_vs_one_solve_per_mesh": value / each,
+        "tune": tune,
         "batched_direct_launches": {"solves_per_s": len(laps) * args.steps / ed,
                                     "ms_per_step": 1e3 * ed / args.steps,
                                     "what": "64 right-hand side launches, no graph replay"},
@@ -712,6 +739,9 @@ def icosphere9_bench(args):
                    "adjoint_solve_hsc")
         return fi.iters, ai.iters
 
+    _lib.check(lib.cn_harmonic_rhs(lap.handle, k, C.c_void_p(Xc.data_ptr()),
+                                   C.c_void_p(B.data_ptr()), sp), "harmonic_rhs")
+    tune = tune_for(lap, B, args, hsc, precond="hsc")
     free0, total = torch.cuda.mem_get_info()
     for _ in range(args.warmup):
         step()
@@ -749,6 +779,7 @@ def icosphere9_bench(args):
                                f"x 3 RHS, HSC theta {args.theta} ({hsc.info.nlevels} levels, "
                                f"coarse {hsc.info.n_coarse}), tol {args.tol}"},
         "spmv_k1": spmv,
+        "tune": tune,
         "device_memory": {"in_use_after_steps_bytes": int(used), "total_bytes": int(total),
                           "in_use_before_steps_bytes": int(total - free0)},
         "solve": {"forward_iters": [f for f, _ in its], "adjoint_iters": [a for _, a in its],

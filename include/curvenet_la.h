@@ -194,6 +194,30 @@ cn_status cn_system_destroy(cn_system sys);
  * the same bits as the per-member calls). */
 cn_status cn_system_batch(int32_t nsys, const cn_system *systems, cn_system *out,
                           void *stream);
+/* Launch shapes of a system's solver kernels.  No counterpart in the
+ * reference: the autotuner (tool_godot_curvenet_amd/autotune.py) times
+ * candidates per (system size, k, precision, preconditioner) and keeps the
+ * fastest in a JSON cache.  0 in a field keeps the built-in choice; the
+ * results are the same to rounding whatever the shapes (the SpMM forms and
+ * the tail compute the same products; only partial sums are grouped
+ * differently).
+ *   spmm_form  1: row-table SpMM; 2 (or 0): LDS-staged SpMM where it
+ *              applies (wide k, 16-byte rows, a banded matrix);
+ *   spmm_gens  row-table SpMM: row stripes per XCD share, 1, 2 or 4 (0: 4);
+ *   level_cap  HSC level passes: grid cap over all column tiles, 1024 to
+ *              65536 blocks (0: 16384);
+ *   tail_rows  HSC: levels of at most this many rows (and the coarse solve)
+ *              run in one fused launch when the apply has enough columns,
+ *              up to 16384 (0: 1536; -1: no fused tail). */
+typedef struct cn_tune {
+  int32_t spmm_form;
+  int32_t spmm_gens;
+  int32_t level_cap;
+  int32_t tail_rows;
+} cn_tune;
+cn_status cn_system_set_tune(cn_system sys, const cn_tune *tune);
+cn_status cn_system_get_tune(cn_system sys, cn_tune *tune);
+
 /* Segments of a system: *nseg = 1 and h_seg_ptr = {0, n_free} for a
  * single system.  h_seg_ptr (host, nseg + 1) lives as long as the system. */
 cn_status cn_system_segments(cn_system sys, int32_t *nseg, const int32_t **h_seg_ptr);

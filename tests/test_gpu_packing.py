@@ -119,7 +119,7 @@ def test_warm_start_moderate_k(gpu):
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
 @pytest.mark.parametrize("name,k", [("capsules100k", 768), ("icosphere6", 96), ("cylinder64", 8)])
-def test_spmm_forms_agree(gpu, monkeypatch, precision, tol, name, k):
+def test_spmm_forms_agree(gpu, precision, tol, name, k):
     """The LDS-staged SpMM and the row-table SpMM compute the same products
     in the same order (only the p.q partial sums are grouped differently):
     two PCG iterations agree to rounding."""
@@ -132,7 +132,7 @@ def test_spmm_forms_agree(gpu, monkeypatch, precision, tol, name, k):
     lib = _lib.load()
     xs = {}
     for form in ("row", "lds"):
-        monkeypatch.setenv("CN_SPMM", form)
+        lap.set_tune({"spmm_form": 1 if form == "row" else 2})
         lib.cn_profile_enable(1)
         x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, max_iter=2, hsc=h)
         ms = ctypes.c_double(0.0)
@@ -184,3 +184,27 @@ def test_graph_replay_batch(gpu):
     x1, i1 = batch.solve(B, precond="hsc", precision="mixed", tol=1e-9, hsc=h, graph=True)
     assert i0.status == 0 and i1.iters == i0.iters
     assert np.array_equal(x0.cpu().numpy(), x1.cpu().numpy())
+
+
+def test_autotune_choice_converges_and_caches(gpu, tmp_path):
+    """The autotuner tries the launch shapes, keeps one whose solve reaches
+    the tolerance, stores it, and a second call takes it from the cache."""
+    import torch
+    from tool_godot_curvenet_amd import HSC, CutLaplacian, autotune, meshgen
+    lap = CutLaplacian.from_mesh(meshgen.config_mesh("icosphere6"))
+    B = torch.as_tensor(lap.harmonic_rhs(meshgen.seeded_boundary(lap.n_con, 64, seed=9)),
+                        device="cuda")
+    h = HSC(lap)
+    path = str(tmp_path / "tune.json")
+    r = autotune.autotune(lap, B, precond="hsc", precision="mixed", hsc=h, path=path, reps=1)
+    assert not r["cached"] and len(r["candidates"]) >= 4
+    assert lap.get_tune() == r["tune"]
+    x, info = lap.solve(B, precond="hsc", precision="mixed", tol=1e-9, hsc=h)
+    assert info.status == 0
+    A = lap.A_ff()
+    Bh = B.cpu().numpy()
+    R = Bh - O.spmm(A.row_ptr, A.col_idx, A.values, x.cpu().numpy())
+    assert np.all(np.linalg.norm(R, axis=0) <= 1e-9 * np.linalg.norm(Bh, axis=0))
+    lap.set_tune(None)
+    r2 = autotune.autotune(lap, B, precond="hsc", precision="mixed", hsc=h, path=path)
+    assert r2["cached"] and r2["tune"] == r["tune"] and lap.get_tune() == r["tune"]

@@ -65,6 +65,10 @@ class PcgOpts(C.Structure):
                 ("warm_start", i32), ("graph", i32)]
 
 
+class Tune(C.Structure):
+    _fields_ = [("spmm_form", i32), ("spmm_gens", i32), ("level_cap", i32), ("tail_rows", i32)]
+
+
 class HscOpts(C.Structure):
     _fields_ = [("theta", f64), ("coarse_max", i32), ("max_levels", i32), ("min_frac", f64),
                 ("compensate", i32)]
@@ -122,6 +126,8 @@ _SIGS = {
     "cn_system_destroy": (i32, [vp]),
     "cn_system_batch": (i32, [i32, C.POINTER(vp), C.POINTER(vp), vp]),
     "cn_system_segments": (i32, [vp, P_i32, C.POINTER(P_i32)]),
+    "cn_system_set_tune": (i32, [vp, C.POINTER(Tune)]),
+    "cn_system_get_tune": (i32, [vp, C.POINTER(Tune)]),
     "cn_harmonic_rhs": (i32, [vp, i32, vp, vp, vp]),
     "cn_spmv_f64": (i32, [i32, vp, vp, vp, vp, vp, vp]),
     "cn_spmv_f32": (i32, [i32, vp, vp, vp, vp, vp, vp]),

@@ -211,11 +211,13 @@ static int hsc_vec(int64_t k) {
 // down pass from 0.57 to 0.62 of peak at k = 768 (3 tiles) and the
 // step of the 2.46M-row icosphere from 54.8 to 61.4 poses/s at k = 48.  Totals of
 // 8192, 32768 and 65536 blocks measured within 0.3 % of 16384.
-static dim3 hsc_grid(int64_t rows, int64_t k, int v, int lpr) {
+static dim3 hsc_grid(int64_t rows, int64_t k, int v, int lpr, int32_t level_cap) {
   const int slots = kWaves * (64 / lpr);
   int64_t g = (rows + slots - 1) / slots;
   const int64_t tiles = (k + (int64_t)lpr * v - 1) / ((int64_t)lpr * v);
-  const int64_t cap = std::max<int64_t>(1024, 16384 / tiles);
+  const int64_t total = level_cap > 0 ? std::min<int64_t>(std::max(level_cap, 1024), 65536)
+                                      : 16384;
+  const int64_t cap = std::max<int64_t>(1024, total / tiles);
   g = std::max<int64_t>(1, std::min<int64_t>(g, cap));
   if (g >= 64) g -= g % 8;
   return dim3((unsigned)g, (unsigned)((k + (int64_t)lpr * v - 1) / ((int64_t)lpr * v)), 1);
@@ -244,7 +246,8 @@ template <> struct LvVals<float> {
 
 template <typename T>
 static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
-                       const int32_t *ctl, hipStream_t s) {
+                       const int32_t *ctl, hipStream_t s,
+                       int32_t level_cap) {
   if (L.nC == 0) return;
   const int v = hsc_vec<T>(k);
   if (v == 0) {
@@ -254,7 +257,7 @@ static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
     return;
   }
   const int lpr = pack_lpr12((k + v - 1) / v);
-  const dim3 g = hsc_grid(L.nC, k, v, lpr);
+  const dim3 g = hsc_grid(L.nC, k, v, lpr, level_cap);
 #define D_(VV, LL) kw_hsc_down<T, VV, LL><<<g, kBlock, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(), L.r_colg.get(), LvVals<T>::r(L), k, src, Z, ctl)
   if constexpr (sizeof(T) == 4) {
     if (v == 4) { HSC_LPR(lpr, D_, 4); return; }
@@ -265,7 +268,8 @@ static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
 
 template <typename T>
 static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
-                     const int32_t *ctl, hipStream_t s) {
+                     const int32_t *ctl, hipStream_t s,
+                     int32_t level_cap) {
   if (L.nF == 0) return;
   const int v = hsc_vec<T>(k);
   if (v == 0) {
@@ -275,7 +279,7 @@ static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
     return;
   }
   const int lpr = pack_lpr12((k + v - 1) / v);
-  const dim3 g = hsc_grid(L.nF, k, v, lpr);
+  const dim3 g = hsc_grid(L.nF, k, v, lpr, level_cap);
 #define U_(VV, LL) kw_hsc_up<T, VV, LL><<<g, kBlock, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L), L.p_ptr.get(), L.p_colg.get(), LvVals<T>::p(L), k, src, Z, ctl)
   if constexpr (sizeof(T) == 4) {
     if (v == 4) { HSC_LPR(lpr, U_, 4); return; }
@@ -406,12 +410,13 @@ k_hsc_tail(int32_t l0, int32_t l1, const TailLevel<T> *__restrict__ lv, int32_t 
 // First level the fused tail takes for an apply with k columns (levels
 // [l0, L) and the coarse solve), or -1 when the per-level kernels run.
 template <typename T>
-static int32_t tail_level(const cn_hsc_t *H, int64_t k) {
+static int32_t tail_level(const cn_hsc_t *H, int64_t k, int32_t tail_rows = 0) {
   const int v = hsc_vec<T>(k);
-  if (v == 0 || k / v < kTailSlices || H->nc > kTailCoarse) return -1;
+  if (tail_rows < 0 || v == 0 || k / v < kTailSlices || H->nc > kTailCoarse) return -1;
+  const int32_t rows = tail_rows > 0 ? std::min(tail_rows, 16384) : kTailRows;
   const int32_t L = (int32_t)H->lv.size();
   int32_t l0 = L;
-  while (l0 > 0 && H->lv[l0 - 1].n <= kTailRows) --l0;
+  while (l0 > 0 && H->lv[l0 - 1].n <= rows) --l0;
   return l0;
 }
 
@@ -439,7 +444,8 @@ static const TailLevel<T> *tail_desc(cn_hsc_t *H, hipStream_t s) {
 }
 
 template <typename T>
-void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const int32_t *ctl) {
+void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const int32_t *ctl,
+               const HscShape &sh) {
   const int32_t L = (int32_t)H->lv.size();
   // the gathered coarse rows, then (256-aligned) the split-K partial sums
   const size_t wbytes = (sizeof(T) * (size_t)H->nc * k + 255) / 256 * 256;
@@ -449,12 +455,13 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
     H->work_bytes = bytes;
   }
   T *W = reinterpret_cast<T *>(H->work.get());
-  const int32_t l0 = tail_level<T>(H, k);
+  const int32_t l0 = tail_level<T>(H, k, sh.tail_rows);
   if (l0 >= 0) {
     const TailLevel<T> *desc = tail_desc<T>(H, s);
     {
       ProfScope ps("hsc_down", s);
-      for (int32_t l = 0; l < l0; ++l) level_down<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s);
+      for (int32_t l = 0; l < l0; ++l)
+        level_down<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s, sh.level_cap);
     }
     {
       ProfScope ps("hsc_tail", s);
@@ -470,13 +477,15 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
     }
     {
       ProfScope ps("hsc_up", s);
-      for (int32_t l = l0 - 1; l >= 0; --l) level_up<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s);
+      for (int32_t l = l0 - 1; l >= 0; --l)
+        level_up<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s, sh.level_cap);
     }
     return;
   }
   {
     ProfScope ps("hsc_down", s);
-    for (int32_t l = 0; l < L; ++l) level_down<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s);
+    for (int32_t l = 0; l < L; ++l)
+      level_down<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s, sh.level_cap);
   }
   if (H->nc > 0) {
     ProfScope ps("hsc_coarse", s);
@@ -498,13 +507,14 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
   }
   {
     ProfScope ps("hsc_up", s);
-    for (int32_t l = L - 1; l >= 0; --l) level_up<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s);
+    for (int32_t l = L - 1; l >= 0; --l)
+      level_up<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s, sh.level_cap);
   }
 }
 template void hsc_apply<float>(cn_hsc_t *, int64_t, const float *, float *, hipStream_t,
-                               const int32_t *);
+                               const int32_t *, const HscShape &);
 template void hsc_apply<double>(cn_hsc_t *, int64_t, const double *, double *, hipStream_t,
-                                const int32_t *);
+                                const int32_t *, const HscShape &);
 
 }  // namespace cn
 
@@ -524,7 +534,7 @@ cn_status cn_hsc_apply(cn_hsc H, int32_t k, const double *d_r, double *d_z, void
     return CN_BAD_ARG;
   }
   hipStream_t s = as_stream(stream);
-  if (H->n0 > 0) hsc_apply<double>(H, k, d_r, d_z, s, nullptr);
+  if (H->n0 > 0) hsc_apply<double>(H, k, d_r, d_z, s, nullptr, HscShape{});
   CN_HIP(hipGetLastError());
   CN_HIP(hipStreamSynchronize(s));
   return CN_OK;

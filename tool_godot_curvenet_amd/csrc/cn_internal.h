@@ -85,6 +85,12 @@ inline unsigned grid_for(int64_t n, int block) {
 int64_t exclusive_scan_i32(int32_t *d, int64_t n, hipStream_t s);
 int64_t exclusive_scan_i64(int64_t *d, int64_t n, hipStream_t s);
 
+// HSC apply launch shapes (from a system's cn_tune; 0 = built-in)
+struct HscShape {
+  int32_t level_cap = 0;
+  int32_t tail_rows = 0;
+};
+
 // ----------------------------------------------------------- topology --
 // Device halfedge mesh in SoA form (the spec's Halfedge record,
 // Halfedge.lean:30-35, with -1 for `none`).
@@ -193,6 +199,7 @@ struct cn_system_t {
   // FP64 weights), refilled from the values at every solve
   cn::DevBuf<unsigned char> win_pk32, win_pk64;
   int32_t win_rpr = 0, win_ok = 0;
+  cn_tune tune{};                   // launch shapes (cn_system_set_tune); 0 = built-in
   // A batch of independent systems (cn_system_batch) is one block-diagonal
   // A_ff: segment s owns rows [seg_ptr[s], seg_ptr[s+1]).  The solver keeps
   // its CG scalars per (segment, column), and takes its dots over blocks of

@@ -1872,6 +1872,8 @@ struct SysView {
   const T *eval = nullptr, *ovv = nullptr;
   // window row table of the LDS-staged SpMM; rpr = 0 when not in use
   int32_t rpr = 0;
+  int32_t gens = 4;  // row stripes per XCD share of the row-table SpMM
+  HscShape hsh{};    // HSC apply launch shapes
   const unsigned char *wpk = nullptr;
   const int32_t *far_ptr = nullptr, *far_rows = nullptr;
   // block sweeps (forward: L, backward: U); mb = 0 when not available
@@ -1879,14 +1881,16 @@ struct SysView {
   SweepView<T> lsw{}, usw{};
 };
 
-// Row stripes per XCD share in the wide SpMM: the grid holds this many
-// generations of resident blocks, so the rows in flight on an XCD stay
-// within one stripe and its L2 serves the neighbour gathers (measured: L2
-// misses 705 MB -> 431 MB per launch on the 100k capsule mesh at k = 768).
+// Row stripes per XCD share in the wide row-table SpMM (1, 2 or 4; 4 by
+// default): the grid holds this many generations of resident blocks, so the
+// rows in flight on an XCD stay within one stripe and its L2 serves the
+// neighbour gathers (measured: L2 misses 705 MB -> 431 MB per launch on the
+// 100k capsule mesh at k = 768).  4 is also the most the partial-sum
+// buffers are sized for.
 static const unsigned kSpmmGens = 4;
 
 template <typename T>
-static Grid grid_spmm(int64_t n, int64_t k) {
+static Grid grid_spmm(int64_t n, int64_t k, unsigned gens = kSpmmGens) {
   if (k <= 4) {  // one thread per row
     Grid G;
     G.g = dim3(blocks_for_rows(n, kBlock, kCapSpmm), 1, 1);
@@ -1895,7 +1899,7 @@ static Grid grid_spmm(int64_t n, int64_t k) {
   }
   Grid G = grid_rows<T>(n, k, kCapSpmm);
   if (k > 4 && G.g.x % kXcd == 0) {
-    G.g.x *= kSpmmGens;
+    G.g.x *= std::max(1u, std::min(gens, kSpmmGens));
     G.rows = (int32_t)G.g.x;
   }
   return G;
@@ -1932,7 +1936,7 @@ static int32_t launch_spmm_dot(const SysView<T> &V, int64_t k, const T *P, T *Q,
                                         part, ctl);
     return (int32_t)R;
   }
-  Grid G = grid_spmm<T>(V.n, k);
+  Grid G = grid_spmm<T>(V.n, k, (unsigned)V.gens);
   const int32_t gens = (int32_t)(G.g.x / std::max(1u, grid_rows<T>(V.n, k, kCapSpmm).g.x));
   ProfScope ps("spmm", s);
 #define N_(KK) kn_spmm_dot<T, KK><<<G.g, kBlock, 0, s>>>(V.n, V.ecol, V.eval, V.ovp, V.ovc, V.ovv, P, Q, part, ctl)
@@ -1960,7 +1964,8 @@ static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
 }
 
 template <typename T>
-void hsc_apply(cn_hsc_t *H, int64_t k, const T *r0, T *z0, hipStream_t s, const int32_t *ctl);
+void hsc_apply(cn_hsc_t *H, int64_t k, const T *r0, T *z0, hipStream_t s, const int32_t *ctl,
+               const HscShape &sh);
 template <typename T>
 void sgs_apply(const cn_sgs_t *G, int64_t k, const T *r, T *z, hipStream_t s,
                const int32_t *ctl);
@@ -2079,7 +2084,7 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
 #undef N_
 #undef W_
     if (V.hsc) {
-      hsc_apply<T>(V.hsc, k, R, Z, s, ctl);
+      hsc_apply<T>(V.hsc, k, R, Z, s, ctl, V.hsh);
     } else if (V.sgs) {
       sgs_apply<T>(V.sgs, k, R, Z, s, ctl);
     } else {
@@ -2447,17 +2452,20 @@ static bool ensure_win(cn_system_t *S, int32_t rpr, hipStream_t s) {
   return true;
 }
 
-// which SpMM form the solver runs: the LDS-staged one unless CN_SPMM=row
-static bool spmm_lds_wanted() {
-  const char *e = std::getenv("CN_SPMM");
-  return !(e && std::strcmp(e, "row") == 0);
-}
-
+// The system's launch shapes into the view; the LDS-staged SpMM unless the
+// tune asks for the row table (or it does not apply).
 template <typename T>
 static void use_win(cn_system_t *S, SysView<T> &V, int64_t k, hipStream_t s);
+template <typename T>
+static void use_shapes(cn_system_t *S, SysView<T> &V, int64_t k, hipStream_t s) {
+  V.gens = S->tune.spmm_gens > 0 ? S->tune.spmm_gens : (int32_t)kSpmmGens;
+  V.hsh.level_cap = S->tune.level_cap;
+  V.hsh.tail_rows = S->tune.tail_rows;
+  if (S->tune.spmm_form != 1) use_win<T>(S, V, k, s);
+}
 template <>
 void use_win<float>(cn_system_t *S, SysView<float> &V, int64_t k, hipStream_t s) {
-  if (!spmm_lds_wanted() || !win_applies<float>(S->n_free, k)) return;
+  if (!win_applies<float>(S->n_free, k)) return;
   if (!ensure_win(S, win_rpr<float>(S->n_free, k), s)) return;
   V.rpr = S->win_rpr;
   V.wpk = S->win_pk32.get();
@@ -2465,7 +2473,7 @@ void use_win<float>(cn_system_t *S, SysView<float> &V, int64_t k, hipStream_t s)
 }
 template <>
 void use_win<double>(cn_system_t *S, SysView<double> &V, int64_t k, hipStream_t s) {
-  if (!spmm_lds_wanted() || !win_applies<double>(S->n_free, k)) return;
+  if (!win_applies<double>(S->n_free, k)) return;
   if (!ensure_win(S, win_rpr<double>(S->n_free, k), s)) return;
   V.rpr = S->win_rpr;
   V.wpk = S->win_pk64.get();
@@ -2553,7 +2561,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
                                          W->dinv32.get());
   ensure_ell(S, s);
   SysView<T> V = make_view<T>(S, F, W);
-  use_win<T>(S, V, k, s);
+  use_shapes<T>(S, V, k, s);
   V.hsc = H;
   V.sgs = G;
   V.sm = SegMap{batch ? S->seg_of_row.get() : nullptr, kk};
@@ -2732,7 +2740,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
           (uint64_t)k, sizeof(T), (uint64_t)every, (uint64_t)kk,
           H ? H->serial : 0, H ? (uint64_t)(uintptr_t)H->work.get() : 0,
           F ? F->serial : 0, G ? G->serial : 0, (uint64_t)V.rpr,
-          (uint64_t)(uintptr_t)V.wpk, (uint64_t)(uintptr_t)V.ecol};
+          (uint64_t)(uintptr_t)V.wpk, (uint64_t)(uintptr_t)V.ecol, (uint64_t)V.gens,
+          (uint64_t)(uint32_t)V.hsh.level_cap, (uint64_t)(uint32_t)V.hsh.tail_rows};
       if (!W->gexec || W->gkey != key) {
         if (W->gexec) { CN_HIP(hipGraphExecDestroy(W->gexec)); W->gexec = nullptr; }
         if (!W->cap) CN_HIP(hipStreamCreateWithFlags(&W->cap, hipStreamNonBlocking));
@@ -2936,6 +2945,26 @@ using namespace cn;
 
 // ============================================================ entry points ==
 extern "C" {
+
+cn_status cn_system_set_tune(cn_system S, const cn_tune *t) {
+  if (!S || !t || t->spmm_form < 0 || t->spmm_form > 2 || t->spmm_gens < 0 ||
+      t->spmm_gens > (int32_t)kSpmmGens || t->spmm_gens == 3 || t->level_cap < 0 ||
+      t->level_cap > 65536 || t->tail_rows < -1 || t->tail_rows > 16384) {
+    set_error("cn_system_set_tune: bad arguments");
+    return CN_BAD_ARG;
+  }
+  S->tune = *t;
+  return CN_OK;
+}
+
+cn_status cn_system_get_tune(cn_system S, cn_tune *t) {
+  if (!S || !t) {
+    set_error("cn_system_get_tune: bad arguments");
+    return CN_BAD_ARG;
+  }
+  *t = S->tune;
+  return CN_OK;
+}
 
 void cn_pcg_default_opts(cn_pcg_opts *o) {
   if (!o) return;

@@ -209,6 +209,22 @@ class _System:
     def n_free(self) -> int:
         return self.info.n_free
 
+    TUNE_FIELDS = ("spmm_form", "spmm_gens", "level_cap", "tail_rows")
+
+    def set_tune(self, tune: dict | None):
+        """Launch shapes of the solver kernels (cn_system_set_tune): a dict of
+        spmm_form, spmm_gens, level_cap, tail_rows; missing keys (or None)
+        keep the built-in choice."""
+        t = L.Tune()
+        for f in self.TUNE_FIELDS:
+            setattr(t, f, int((tune or {}).get(f, 0)))
+        L.check(L.load().cn_system_set_tune(self._h, C.byref(t)), "system_set_tune")
+
+    def get_tune(self) -> dict:
+        t = L.Tune()
+        L.check(L.load().cn_system_get_tune(self._h, C.byref(t)), "system_get_tune")
+        return {f: getattr(t, f) for f in self.TUNE_FIELDS}
+
     def A_ff(self) -> CsrMatrix:
         i = self.info
         n = i.n_free
