@@ -392,6 +392,20 @@ int32_t cn_hsc_tail_level(cn_hsc h, int32_t k, int32_t fp32);
 /* z = M^-1 r (FP64), r and z n_free x k, distinct. */
 cn_status cn_hsc_apply(cn_hsc h, int32_t k, const double *d_r, double *d_z,
                        void *stream);
+/* cn_hsc_apply that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels and results as cn_hsc_apply, with the apply's workspace (the
+ * gathered coarse rows and the coarse product's split-K partial sums) in a
+ * fixed device array of CN_HSC_SCRATCH_BYTES per GPU, held until the
+ * process ends, instead of the hierarchy's own buffer.  An apply whose
+ * workspace exceeds it (about n_coarse x k x 8 bytes, more with split-K
+ * partials) is CN_BAD_ARG.  Every call on one GPU shares that array:
+ * calls that overlap on a GPU (other streams, other threads) race, so
+ * serialise them.  Stream-ordered (no synchronisation). */
+#define CN_HSC_SCRATCH_BYTES (16u << 20)
+cn_status cn_hsc_apply_noalloc(cn_hsc h, int32_t k, const double *d_r, double *d_z,
+                               void *stream);
 cn_status cn_hsc_destroy(cn_hsc h);
 /* A hierarchy of `sys` from persisted arrays (a sidecar written from
  * cn_hsc_get_level / cn_hsc_get_info): no elimination and no coarse

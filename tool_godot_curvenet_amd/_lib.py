@@ -153,6 +153,7 @@ _SIGS = {
     "cn_hsc_get_info": (i32, [vp, C.POINTER(HscInfo)]),
     "cn_hsc_get_level": (i32, [vp, i32, C.POINTER(HscLevel)]),
     "cn_hsc_apply": (i32, [vp, i32, vp, vp, vp]),
+    "cn_hsc_apply_noalloc": (i32, [vp, i32, vp, vp, vp]),
     "cn_hsc_destroy": (i32, [vp]),
     "cn_sgs_build": (i32, [vp, C.POINTER(vp), vp]),
     "cn_sgs_get_info": (i32, [vp, C.POINTER(SgsInfo)]),

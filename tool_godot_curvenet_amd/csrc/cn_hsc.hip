@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "cn_internal.h"
 #include "cn_kernels.h"
@@ -443,18 +444,32 @@ static const TailLevel<T> *tail_desc(cn_hsc_t *H, hipStream_t s) {
   return reinterpret_cast<const TailLevel<T> *>(d.get());
 }
 
+// Workspace bytes of an apply of k columns: the gathered coarse rows, then
+// (256-aligned) the split-K partial sums of the coarse product.
+template <typename T>
+static size_t apply_work_bytes(const cn_hsc_t *H, int64_t k, size_t *wbytes_out = nullptr) {
+  const size_t wbytes = (sizeof(T) * (size_t)H->nc * k + 255) / 256 * 256;
+  if (wbytes_out) *wbytes_out = wbytes;
+  return wbytes + sizeof(double) * dense_gemm_rows_work(H->nc, (int)k) + 256;
+}
+
+// ext_work: a caller's workspace of at least apply_work_bytes (the
+// hierarchy's own buffer, grown on demand, otherwise)
 template <typename T>
 void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const int32_t *ctl,
-               const HscShape &sh) {
+               const HscShape &sh, unsigned char *ext_work = nullptr) {
   const int32_t L = (int32_t)H->lv.size();
-  // the gathered coarse rows, then (256-aligned) the split-K partial sums
-  const size_t wbytes = (sizeof(T) * (size_t)H->nc * k + 255) / 256 * 256;
-  const size_t bytes = wbytes + sizeof(double) * dense_gemm_rows_work(H->nc, (int)k) + 256;
-  if (H->work_bytes < bytes) {
-    H->work.alloc(bytes);
-    H->work_bytes = bytes;
+  size_t wbytes = 0;
+  const size_t bytes = apply_work_bytes<T>(H, k, &wbytes);
+  unsigned char *work = ext_work;
+  if (!work) {
+    if (H->work_bytes < bytes) {
+      H->work.alloc(bytes);
+      H->work_bytes = bytes;
+    }
+    work = H->work.get();
   }
-  T *W = reinterpret_cast<T *>(H->work.get());
+  T *W = reinterpret_cast<T *>(work);
   const int32_t l0 = tail_level<T>(H, k, sh.tail_rows);
   if (l0 >= 0) {
     const TailLevel<T> *desc = tail_desc<T>(H, s);
@@ -502,7 +517,7 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
     } else {
       dense_gemm_nn_rows<T, T>(H->nc, (int)k, H->coarse_inv.get(), W, k, z, k,
                                H->coarse_rows.get(), ctl,
-                               reinterpret_cast<double *>(H->work.get() + wbytes), s);
+                               reinterpret_cast<double *>(work + wbytes), s);
     }
   }
   {
@@ -512,9 +527,51 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
   }
 }
 template void hsc_apply<float>(cn_hsc_t *, int64_t, const float *, float *, hipStream_t,
-                               const int32_t *, const HscShape &);
+                               const int32_t *, const HscShape &, unsigned char *);
 template void hsc_apply<double>(cn_hsc_t *, int64_t, const double *, double *, hipStream_t,
-                                const int32_t *, const HscShape &);
+                                const int32_t *, const HscShape &, unsigned char *);
+
+// ---- workspace of the apply that allocates nothing (cn_hsc_apply_noalloc)
+// A fixed array in device memory, one copy per GPU the library runs on,
+// held until the process ends.  Its host-side name is not a device
+// address: kernels get the address hipGetSymbolAddress returns for the
+// current device, looked up once per device (at hierarchy build, or at the
+// first call on a device) and kept.
+static __device__ __align__(256) unsigned char scratch[CN_HSC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// this device's copy of `scratch` (nullptr and the error set on failure)
+unsigned char *hsc_scratch() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("hsc scratch: no current device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hsc scratch: hipGetSymbolAddress failed (%d: %s)", (int)e,
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+// Everything an apply reads besides its workspace, built at set-up: the
+// fused tail's level descriptors (FP32 and FP64) and this device's scratch
+// address, so cn_hsc_apply_noalloc never allocates.
+void hsc_prepare(cn_hsc_t *H, hipStream_t s) {
+  (void)tail_desc<float>(H, s);
+  (void)tail_desc<double>(H, s);
+  (void)hsc_scratch();
+}
 
 }  // namespace cn
 
@@ -525,6 +582,33 @@ extern "C" {
 int32_t cn_hsc_tail_level(cn_hsc H, int32_t k, int32_t fp32) {
   if (!H || k <= 0) return -1;
   return fp32 ? tail_level<float>(H, k) : tail_level<double>(H, k);
+}
+
+cn_status cn_hsc_apply_noalloc(cn_hsc H, int32_t k, const double *d_r, double *d_z,
+                               void *stream) {
+  CN_API_BEGIN
+  if (!H || k <= 0 || !d_r || !d_z || d_r == d_z) {
+    set_error("cn_hsc_apply_noalloc: bad arguments");
+    return CN_BAD_ARG;
+  }
+  if (H->n0 == 0) return CN_OK;
+  const size_t need = apply_work_bytes<double>(H, k);
+  if (need > CN_HSC_SCRATCH_BYTES) {
+    set_error("cn_hsc_apply_noalloc: %d columns need %zu bytes of workspace, more than the "
+              "%u-byte scratch (coarse rows %d); use cn_hsc_apply or fewer columns",
+              (int)k, need, (unsigned)CN_HSC_SCRATCH_BYTES, (int)H->nc);
+    return CN_BAD_ARG;
+  }
+  if (!H->tail_desc[1].get()) {
+    set_error("cn_hsc_apply_noalloc: the hierarchy's tail descriptors are missing");
+    return CN_BAD_ARG;
+  }
+  unsigned char *w = hsc_scratch();
+  if (!w) return CN_HIP_ERROR;
+  hsc_apply<double>(H, k, d_r, d_z, as_stream(stream), nullptr, HscShape{}, w);
+  CN_HIP(hipGetLastError());
+  return CN_OK;
+  CN_API_END
 }
 
 cn_status cn_hsc_apply(cn_hsc H, int32_t k, const double *d_r, double *d_z, void *stream) {

@@ -325,6 +325,7 @@ cn_status cn_hsc_build(cn_system S, const cn_hsc_opts *opts, cn_hsc *out, void *
       CN_HIP(hipGetLastError());
       CN_HIP(hipStreamSynchronize(s));
     }
+    hsc_prepare(H, s);
   } catch (...) {
     delete H;
     throw;
@@ -462,6 +463,7 @@ cn_status cn_hsc_import(cn_system S, const cn_hsc_opts *opts, int32_t nlevels,
       upload(H->coarse_inv, std::vector<double>(coarse_inv, coarse_inv + nn), s);
     }
     CN_HIP(hipStreamSynchronize(s));
+    hsc_prepare(H, s);
   } catch (...) {
     delete H;
     throw;

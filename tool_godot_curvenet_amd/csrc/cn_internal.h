@@ -309,3 +309,9 @@ struct cn_hsc_t {
   // solve in one launch), FP32 and FP64 forms, built on first use
   cn::DevBuf<unsigned char> tail_desc[2];
 };
+
+namespace cn {
+// the tail descriptors and this device's scratch address of a hierarchy,
+// built at set-up (cn_hsc_build, cn_hsc_import)
+void hsc_prepare(cn_hsc_t *H, hipStream_t s);
+}  // namespace cn

@@ -1965,7 +1965,7 @@ static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
 
 template <typename T>
 void hsc_apply(cn_hsc_t *H, int64_t k, const T *r0, T *z0, hipStream_t s, const int32_t *ctl,
-               const HscShape &sh);
+               const HscShape &sh, unsigned char *ext_work = nullptr);
 template <typename T>
 void sgs_apply(const cn_sgs_t *G, int64_t k, const T *r, T *z, hipStream_t s,
                const int32_t *ctl);
