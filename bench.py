@@ -30,7 +30,8 @@ WORKLOAD = "capsules100k"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", f"traffic_{WORKLOAD}.json")
 # kernels whose HBM traffic is reported (name prefixes in the counter files)
 TRAFFIC_KERNELS = {
-    "spmm": "void cn::kw_spmm_dot<float, 4, 64>",
+    "spmm": "void cn::kl_spmm_dot<float, 6>",
+    "spmm_row": "void cn::kw_spmm_dot<float, 4, 64>",
     "fwd": "void cn::kb_fwd<float, 4, 16, true>",
     "bwd": "void cn::kb_bwd<float, 4, 16>",
     "hsc_down": "void cn::kw_hsc_down<float, 4, 64>",
