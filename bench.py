@@ -505,7 +505,7 @@ def spmv_k1(lib, A, sp, reps=200):
     err = float(np.abs(y.cpu().numpy() - ref).max() / max(np.abs(ref).max(), 1e-300))
     alg = 8 * nnz + 4 * (n + 1) + 8 * n
     gbs = alg / (ms * 1e-3) / 1e9
-    return {"kernel": "kn_spmm<float,1> (cn_spmv_f32)", "n": n, "nnz": nnz, "avg_ms": ms,
+    return {"kernel": "kt_spmm<float,1> (cn_spmv_f32)", "n": n, "nnz": nnz, "avg_ms": ms,
             "bytes_per_launch": alg, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
             "max_rel_err_vs_fp64": err}
 

@@ -564,3 +564,35 @@ def test_solve_after_set_positions(gpu):
     x, info = lap.solve(B, precond="jacobi", precision="mixed", tol=1e-9)
     assert info.status == 0
     assert np.all(_relres(A, x.cpu().numpy(), B) <= 1e-9)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_spmm_f32_long_rows_and_grid_stride(gpu, k):
+    """Thread-per-row SpMM (k <= 4): rows longer than its 8-entry chunk (hub
+    rows of up to ~25 entries), empty rows, and a matrix of more rows than
+    one grid of threads (the grid-stride walk)."""
+    import scipy.sparse as sps
+    from tool_godot_curvenet_amd import CsrMatrix, spmm
+    A = _hub_matrix(3000, 11)
+    p = A.row_ptr.copy()
+    # empty rows 100 and 101: drop their entries
+    keep = np.ones(len(A.col_idx), bool)
+    for r in (100, 101):
+        keep[p[r]:p[r + 1]] = False
+    rows = np.repeat(np.arange(A.rows), np.diff(p))[keep]
+    M = sps.csr_matrix((A.values[keep], (rows, A.col_idx[keep])), shape=(A.rows, A.rows))
+    n2 = 2_200_000
+    T = sps.diags([np.full(n2 - 1, -1.0), np.full(n2, 2.5), np.full(n2 - 1, -1.0)],
+                  [-1, 0, 1], format="csr")
+    for Ms in (M, T):
+        Ms.sort_indices()
+        C = CsrMatrix(Ms.shape[0], Ms.shape[0], Ms.indptr.astype(np.int32),
+                      Ms.indices.astype(np.int32), Ms.data)
+        X = np.random.default_rng(k).standard_normal((C.rows, k)).astype(np.float32)
+        got = spmm(C, X, "f32").astype(np.float64)
+        M32 = sps.csr_matrix((Ms.data.astype(np.float32).astype(np.float64), Ms.indices,
+                              Ms.indptr), shape=Ms.shape)
+        want = M32 @ X.astype(np.float64)
+        absA = abs(M32) @ np.abs(X).astype(np.float64)
+        assert np.all(np.abs(got - want) <= 1e-5 * absA + 1e-30)
+        assert np.all(got[100:102] == 0.0) if Ms is M else True

@@ -8,12 +8,14 @@
 // the lines are used in full only across the row's later steps, through the
 // cache).  This form is for exactness, not speed.
 //
-// FP32 forms are the solver's kernels (fused multiply-add, group or
-// wavefront per row, see cn_kernels.h).
+// FP32 forms use fused multiply-adds in entry order: a thread per row for
+// k <= 4, a wavefront per (row, column tile) above (see cn_kernels.h).
 #include <hip/hip_runtime.h>
 
 #include "cn_internal.h"
 #include "cn_kernels.h"
+
+#include <algorithm>
 
 #pragma clang fp contract(off)
 
@@ -31,18 +33,55 @@ __global__ void k_spmm_exact(int32_t rows, int32_t k, const int32_t *ptr,
   Y[id] = s;
 }
 
+// Thread-per-row form (k <= 4): one thread owns a row and its K columns.
+// The row's entries are taken 8 at a time: the chunk's columns and weights,
+// then all its gathers, are issued back to back (the tail of a chunk
+// re-reads the row's last entry with weight 0, so every load is
+// unconditional and the sum is the plain in-order one); the row pointers of
+// the thread's next row are loaded while the current row's gathers are in
+// flight.  A wavefront reads 64 consecutive rows' index data, which covers
+// whole cache lines across its 8 loads.
 template <typename T, int K>
-__global__ void __launch_bounds__(kBlock) kn_spmm(int32_t n, const int32_t *ptr,
-                                                  const int32_t *col, const T *val,
-                                                  const T *X, T *Y) {
-  const int grp = threadIdx.x / kGroup, g = threadIdx.x % kGroup;
-  for (int64_t row = (int64_t)blockIdx.x * (kBlock / kGroup) + grp; row < n;
-       row += (int64_t)gridDim.x * (kBlock / kGroup)) {
+__global__ void __launch_bounds__(kBlock) kt_spmm(int32_t n, const int32_t *__restrict__ ptr,
+                                                  const int32_t *__restrict__ col,
+                                                  const T *__restrict__ val,
+                                                  const T *__restrict__ X, T *__restrict__ Y) {
+  constexpr int CH = 8;
+  const int32_t stride = (int32_t)(gridDim.x * blockDim.x);
+  int32_t r = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= n) return;
+  int32_t a = ptr[r], b = ptr[r + 1];
+  while (true) {
+    const int32_t rn = r + stride;
+    const bool more = rn < n;
+    int32_t an = 0, bn = 0;
+    if (more) { an = ptr[rn]; bn = ptr[rn + 1]; }
     T acc[K] = {};
-    gather_narrow<T, K>(ptr[row], ptr[row + 1], g, col, val, X, acc);
+    for (int32_t e = a; e < b; e += CH) {
+      int32_t j[CH];
+      T v[CH];
 #pragma unroll
-    for (int c = 0; c < K; ++c)
-      if (g == c) Y[row * K + c] = acc[c];
+      for (int u = 0; u < CH; ++u) {
+        const int32_t q = min(e + u, b - 1);
+        j[u] = col[q];
+        v[u] = (e + u < b) ? val[q] : T(0);
+      }
+      T x[CH][K];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+#pragma unroll
+        for (int c = 0; c < K; ++c) x[u][c] = X[(int64_t)j[u] * K + c];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc[c] = fma(v[u], x[u][c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) Y[(int64_t)r * K + c] = acc[c];
+    if (!more) break;
+    r = rn;
+    a = an;
+    b = bn;
   }
 }
 
@@ -75,12 +114,14 @@ static void launch_spmm_fast(int32_t n, const int32_t *ptr, const int32_t *col,
                              hipStream_t s) {
   if (n <= 0) return;
   if (k <= 4) {
-    unsigned gx = row_blocks(n, kBlock / kGroup);
+    // a thread per row, one wave of resident blocks at most (the threads
+    // walk on by the grid stride)
+    const unsigned gx = (unsigned)std::min<int64_t>(((int64_t)n + kBlock - 1) / kBlock, 8192);
     switch (k) {
-      case 1: kn_spmm<T, 1><<<gx, kBlock, 0, s>>>(n, ptr, col, val, X, Y); break;
-      case 2: kn_spmm<T, 2><<<gx, kBlock, 0, s>>>(n, ptr, col, val, X, Y); break;
-      case 3: kn_spmm<T, 3><<<gx, kBlock, 0, s>>>(n, ptr, col, val, X, Y); break;
-      default: kn_spmm<T, 4><<<gx, kBlock, 0, s>>>(n, ptr, col, val, X, Y); break;
+      case 1: kt_spmm<T, 1><<<gx, kBlock, 0, s>>>(n, ptr, col, val, X, Y); break;
+      case 2: kt_spmm<T, 2><<<gx, kBlock, 0, s>>>(n, ptr, col, val, X, Y); break;
+      case 3: kt_spmm<T, 3><<<gx, kBlock, 0, s>>>(n, ptr, col, val, X, Y); break;
+      default: kt_spmm<T, 4><<<gx, kBlock, 0, s>>>(n, ptr, col, val, X, Y); break;
     }
   } else {
     unsigned gx = row_blocks(n, kWaves);
