@@ -30,7 +30,9 @@ WORKLOAD = "capsules100k"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", f"traffic_{WORKLOAD}.json")
 # kernels whose HBM traffic is reported (name prefixes in the counter files)
 TRAFFIC_KERNELS = {
-    "spmm": "void cn::kl_spmm_dot<float, 6>",
+    "spmm_dir": "void cn::kl_spmm<float, 6, 1>",
+    "spmm_upd": "void cn::kl_spmm<float, 6, 2>",
+    "spmm": "void cn::kl_spmm<float, 6, 0>",
     "spmm_row": "void cn::kw_spmm_dot<float, 4, 64>",
     "fwd": "void cn::kb_fwd<float, 4, 16, true>",
     "bwd": "void cn::kb_bwd<float, 4, 16>",
@@ -212,8 +214,9 @@ def main():
     torch.cuda.synchronize()
 
     lib.cn_profile_enable(1)
-    prof = {name: [0, 0.0] for name in ("spmm", "spmm_lds", "fwd", "bwd", "resid", "hsc_down",
-                                         "hsc_up", "hsc_coarse", "hsc_tail")}
+    prof = {name: [0, 0.0] for name in ("spmm", "spmm_lds", "spmm_dir", "spmm_upd", "fwd",
+                                         "bwd", "resid", "hsc_down", "hsc_up", "hsc_coarse",
+                                         "hsc_tail")}
     iters, refines, relmax = 0, 0, 0.0
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -252,12 +255,28 @@ def main():
     # every column converged return at once and are given no bytes.
     nk4 = 4 * n * k
     spmm_bytes = 8 * nnz + 4 * (n + 1) + 2 * nk4
-    # the SpMM runs as the LDS-staged kernel or the row-table one
-    sp_tag = "spmm_lds" if prof["spmm_lds"][0] > 0 else "spmm"
-    sp_name = "kl_spmm_dot<float,6>" if sp_tag == "spmm_lds" else "kw_spmm_dot<float,4,64>"
-    kern = {
-        "spmm": (sp_name, prof[sp_tag][0], prof[sp_tag][1], iters * spmm_bytes),
-    }
+    # The SpMM runs as the fused pair of LDS-staged products (KL_DIR: the
+    # direction update p = beta p + gamma z with q = A p and p.q -- reads
+    # P and Z, writes P; KL_UPD: q = A p again with r -= alpha q, d += alpha
+    # p and r.r -- reads P, R, D, writes R, D), or as one product with Q
+    # stored (LDS-staged or row-table) followed by kw_rupd and kw_pd_update.
+    fused = prof["spmm_dir"][0] > 0
+    if fused:
+        dir_bytes = 8 * nnz + 4 * (n + 1) + 3 * nk4
+        upd_bytes = 8 * nnz + 4 * (n + 1) + 5 * nk4
+        kern = {
+            "spmm_dir": ("kl_spmm<float,6,KL_DIR>", prof["spmm_dir"][0], prof["spmm_dir"][1],
+                         iters * dir_bytes),
+            "spmm_upd": ("kl_spmm<float,6,KL_UPD>", prof["spmm_upd"][0], prof["spmm_upd"][1],
+                         iters * upd_bytes),
+        }
+    else:
+        sp_tag = "spmm_lds" if prof["spmm_lds"][0] > 0 else "spmm"
+        sp_name = "kl_spmm<float,6,KL_PLAIN>" if sp_tag == "spmm_lds" else \
+            "kw_spmm_dot<float,4,64>"
+        kern = {
+            "spmm": (sp_name, prof[sp_tag][0], prof[sp_tag][1], iters * spmm_bytes),
+        }
     coarse = None
     if args.precond == "ic0":
         nlev = pre.info.nlevels
@@ -302,8 +321,12 @@ def main():
             tail_b += 8 * k * ncr + 8 * ncr * ncr
             kern["hsc_tail"] = ("k_hsc_tail<float,4>", prof["hsc_tail"][0], prof["hsc_tail"][1],
                                 nwork * tail_b)
-        kern["pd_update"] = ("kw_pd_update<float,4,64>", prof["bwd"][0], prof["bwd"][1],
-                             nwork * 5 * nk4)
+        # kw_pd_update: every iteration's direction update, or with the fused
+        # products only the deferred update before a round that re-applies
+        # the preconditioner
+        nbwd = prof["bwd"][0]
+        kern["pd_update"] = ("kw_pd_update<float,4,64>", nbwd, prof["bwd"][1],
+                             (nbwd if fused else nwork) * 5 * nk4)
         nc = pre.info.n_coarse
         cms = prof["hsc_coarse"][1]
         ncoarse = prof["hsc_coarse"][0]

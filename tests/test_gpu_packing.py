@@ -136,7 +136,9 @@ def test_spmm_forms_agree(gpu, precision, tol, name, k):
         lib.cn_profile_enable(1)
         x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, max_iter=2, hsc=h)
         ms = ctypes.c_double(0.0)
-        used = lib.cn_profile_query(b"spmm_lds", ctypes.byref(ms))
+        # the LDS-staged product alone, or the fused pair of them
+        used = lib.cn_profile_query(b"spmm_lds", ctypes.byref(ms)) + \
+            lib.cn_profile_query(b"spmm_dir", ctypes.byref(ms))
         lib.cn_profile_enable(0)
         assert (used > 0) == (form == "lds"), (form, used)
         assert info.iters == 2
@@ -208,3 +210,47 @@ def test_autotune_choice_converges_and_caches(gpu, tmp_path):
     lap.set_tune(None)
     r2 = autotune.autotune(lap, B, precond="hsc", precision="mixed", hsc=h, path=path)
     assert r2["cached"] and r2["tune"] == r["tune"] and lap.get_tune() == r["tune"]
+
+
+@pytest.mark.parametrize("precond,precision,tol", [("hsc", "fp64", 1e-12), ("hsc", "mixed", 1e-5),
+                                                   ("sgs", "fp64", 1e-12), ("sgs", "mixed", 1e-5)])
+def test_fused_iteration_equals_unfused(gpu, precond, precision, tol):
+    """The fused iteration (direction update on the product, residual and
+    iterate updates on a second product) computes the unfused path's
+    vectors with the same arithmetic; only the partial sums of p.q and r.r
+    are grouped differently.  Three iterations agree to rounding, and both
+    solves converge in the same number of iterations (within one)."""
+    from tool_godot_curvenet_amd import HSC, SGS, CutLaplacian, meshgen
+    lap = CutLaplacian.from_mesh(meshgen.config_mesh("icosphere6"))
+    k = 64
+    B = lap.harmonic_rhs(meshgen.seeded_boundary(lap.n_con, k, seed=21))
+    kw = {"hsc": HSC(lap)} if precond == "hsc" else {"sgs": SGS(lap)}
+    out = {}
+    for form in (1, 2):
+        lap.set_tune({"spmm_form": form})
+        x3, i3 = lap.solve(B, precond=precond, precision=precision, tol=1e-9, max_iter=3, **kw)
+        xf, inf = lap.solve(B, precond=precond, precision=precision, tol=1e-9, **kw)
+        out[form] = (x3.cpu().numpy(), xf.cpu().numpy(), i3, inf)
+    lap.set_tune(None)
+    a, b = out[1], out[2]
+    assert np.abs(a[0] - b[0]).max() <= tol * np.abs(a[0]).max()
+    assert a[3].status == 0 and b[3].status == 0 and abs(a[3].iters - b[3].iters) <= 1
+    A = lap.A_ff()
+    R = B - O.spmm(A.row_ptr, A.col_idx, A.values, b[1])
+    assert np.all(np.linalg.norm(R, axis=0) <= 1e-9 * np.linalg.norm(B, axis=0))
+
+
+def test_fused_graph_odd_block(gpu):
+    """SGS polls every check_every iterations: with 3 the captured block
+    flips the two P buffers; replays from either parity equal direct
+    launches bit for bit."""
+    from tool_godot_curvenet_amd import SGS, CutLaplacian, meshgen
+    lap = CutLaplacian.from_mesh(meshgen.config_mesh("icosphere6"))
+    B = lap.harmonic_rhs(meshgen.seeded_boundary(lap.n_con, 32, seed=23))
+    g = SGS(lap)
+    x0, i0 = lap.solve(B, precond="sgs", precision="mixed", tol=1e-9, sgs=g, check_every=3,
+                       graph=False)
+    for _ in range(2):
+        x1, i1 = lap.solve(B, precond="sgs", precision="mixed", tol=1e-9, sgs=g, check_every=3,
+                           graph=True)
+        assert i1.iters == i0.iters and np.array_equal(x0.cpu().numpy(), x1.cpu().numpy())

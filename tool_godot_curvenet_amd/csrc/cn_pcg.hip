@@ -248,12 +248,32 @@ constexpr int win_block_bytes() { return kWinEnt * (4 + (int)sizeof(T)); }
 // the compiler wait for all outstanding loads at the next use of any of
 // them, which would drain the pipeline every step.  Clamped rows land in
 // ring rows no neighbour index maps to.
-template <typename T, int D>
+// Three forms share the pipeline (MODE):
+//  KL_PLAIN  Q = A P, partial p.q;
+//  KL_DIR    the direction update folded into the product: every row that
+//            lands in LDS is p = beta p_old + gamma z (the arithmetic of
+//            kw_pd_update), the block's own rows are written to Pout (a
+//            buffer other than Pin, so a block's halo rows read p_old even
+//            after their owner has written p_new), then q = A p with the
+//            partial p.q and no Q stored;
+//  KL_UPD    q = A p recomputed in the same order (so the same bits as
+//            KL_DIR's), then r -= alpha q and d += alpha p on the block's
+//            own rows (kw_rupd's and kw_pd_update's arithmetic), partial r.r.
+// With KL_DIR then KL_UPD an iteration's SpMM, residual and iterate updates
+// and direction update move P in, P out, then P, R, D in and R, D out:
+// 8 n x k passes instead of the 10 of the SpMM, kw_rupd and kw_pd_update.
+// The own-row R and D values KL_UPD needs are loaded D steps ahead in
+// registers (the thread that loads a row is the one that updates it).
+enum { KL_PLAIN = 0, KL_DIR = 1, KL_UPD = 2 };
+
+template <typename T, int D, int MODE>
 __global__ void __launch_bounds__(512)
-kl_spmm_dot(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
-            const int32_t *__restrict__ far_ptr, const int32_t *__restrict__ far_rows,
-            int64_t k, const T *__restrict__ P, T *__restrict__ Q, double *__restrict__ part,
-            const int32_t *__restrict__ ctl) {
+kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
+        const int32_t *__restrict__ far_ptr, const int32_t *__restrict__ far_rows, int64_t k,
+        const T *__restrict__ P, const T *__restrict__ Z, T *__restrict__ Pout,
+        T *__restrict__ Q, T *__restrict__ Rv, T *__restrict__ Dv,
+        const double *__restrict__ st, double *__restrict__ part,
+        const int32_t *__restrict__ ctl) {
   if (ctl_idle(ctl)) return;
   constexpr int VEC = 16 / sizeof(T);
   constexpr int W = 16 * VEC;          // columns of a tile (256 bytes)
@@ -290,31 +310,64 @@ kl_spmm_dot(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
   const int32_t clast = R * (rpr / kLdsRows) - 1;            // last index block
   const int iw = min(tid, NIW - 1);
   auto row_of = [&](int32_t p) { return down ? re - 1 - p : rs + p; };
+  // the column scalars (single system: slot c of k)
+  T al[VEC], be[VEC], ga[VEC];
+#pragma unroll
+  for (int m = 0; m < VEC; ++m) {
+    al[m] = (MODE == KL_UPD && valid) ? (T)st[ST_ALPHA * k + c0 + m] : T(0);
+    be[m] = (MODE == KL_DIR && valid) ? (T)st[ST_BETA * k + c0 + m] : T(0);
+    ga[m] = (MODE == KL_DIR && valid) ? (T)st[ST_GAMMA * k + c0 + m] : T(0);
+  }
+  auto dir = [&](T (&v)[VEC], const T (&zv)[VEC]) {
+#pragma unroll
+    for (int m = 0; m < VEC; ++m) v[m] = fma(be[m], v[m], ga[m] * zv[m]);
+  };
   // far rows of the range, once
   {
     const int32_t fa = far_ptr[x], fb = far_ptr[x + 1];
     for (int32_t f = fa + rsub; f < fb; f += kLdsRows) {
       T v[VEC];
       VecIO<T, VEC>::ld(P + (int64_t)far_rows[f] * k + cl, v);
+      if constexpr (MODE == KL_DIR) {
+        T zv[VEC];
+        VecIO<T, VEC>::ld(Z + (int64_t)far_rows[f] * k + cl, zv);
+        dir(v, zv);
+      }
       VecIO<T, VEC>::st(farc + (f - fa) * W + lq, v);
     }
   }
-  T pb[D][VEC];
+  constexpr int DZ = MODE == KL_DIR ? D : 1;   // z of the landing rows
+  constexpr int DR = MODE == KL_UPD ? D : 1;   // own-row r and d
+  T pb[D][VEC], zb[DZ][VEC], rb[DR][VEC], db[DR][VEC];
   IW ib[D];
-  auto issue = [&](int32_t t, T (&v)[VEC], IW &w) {
+  auto issue = [&](int32_t t, int u) {
     const int32_t g = t + kLdsAhead + D;                 // P chunk
     const int32_t row = min(max(row_of(g * kLdsRows + rsub), 0), n - 1);
-    VecIO<T, VEC>::ld(P + (int64_t)row * k + cl, v);
+    VecIO<T, VEC>::ld(P + (int64_t)row * k + cl, pb[u]);
+    if constexpr (MODE == KL_DIR) VecIO<T, VEC>::ld(Z + (int64_t)row * k + cl, zb[u]);
     const int32_t gi = min(max(cbase + t + D, 0), clast);  // index chunk
-    w = reinterpret_cast<const IW *>(wpk + (int64_t)gi * IB)[iw];
+    ib[u] = reinterpret_cast<const IW *>(wpk + (int64_t)gi * IB)[iw];
   };
-  auto land = [&](int32_t t, const T (&v)[VEC], IW w) {
+  // own-row r and d of compute chunk t + D, after compute(t) read the buffer
+  auto issue_rd = [&](int32_t t, int u) {
+    const int32_t row = min(max(row_of((t + D) * kLdsRows + rsub), 0), n - 1);
+    VecIO<T, VEC>::ld(Rv + (int64_t)row * k + cl, rb[u]);
+    VecIO<T, VEC>::ld(Dv + (int64_t)row * k + cl, db[u]);
+  };
+  auto land = [&](int32_t t, int u) {
     const int32_t g = t + kLdsAhead;
-    VecIO<T, VEC>::st(ring + ((g + kLdsSlots) % kLdsSlots) * CH + rsub * W + lq, v);
-    stage[t & 1][iw] = w;
+    if constexpr (MODE == KL_DIR) {
+      dir(pb[u], zb[u]);
+      // the block's own rows go out once (halo rows belong to another range)
+      const int32_t p = g * kLdsRows + rsub;
+      if (valid && p >= 0 && p < len)
+        VecIO<T, VEC>::st(Pout + (int64_t)row_of(p) * k + c0, pb[u]);
+    }
+    VecIO<T, VEC>::st(ring + ((g + kLdsSlots) % kLdsSlots) * CH + rsub * W + lq, pb[u]);
+    stage[t & 1][iw] = ib[u];
   };
   double pd[VEC] = {};
-  auto compute = [&](int32_t t) {
+  auto compute = [&](int32_t t, int u) {
     const int32_t p = t * kLdsRows + rsub;
     if (p >= len) return;
     const int32_t row = row_of(p);
@@ -323,25 +376,39 @@ kl_spmm_dot(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
                       &stage[t & 1][0]) + 4 * kWinEnt) + rsub * kWinEll;
     T acc[VEC] = {}, x0[VEC];
 #pragma unroll
-    for (int u = 0; u < kWinEll; ++u) {
+    for (int e = 0; e < kWinEll; ++e) {
       T v[VEC];
-      VecIO<T, VEC>::ld(ring + jc[u] * W + lq, v);
-      const T w = wv[u];
+      VecIO<T, VEC>::ld(ring + jc[e] * W + lq, v);
+      const T w = wv[e];
 #pragma unroll
       for (int m = 0; m < VEC; ++m) acc[m] = fma(w, v[m], acc[m]);
-      if (u == 0) {
+      if (e == 0) {
 #pragma unroll
         for (int m = 0; m < VEC; ++m) x0[m] = v[m];
       }
     }
+    if constexpr (MODE == KL_UPD) {
+      T rv[VEC], dv[VEC];
 #pragma unroll
-    for (int m = 0; m < VEC; ++m) pd[m] += (double)x0[m] * (double)acc[m];
-    if (valid) VecIO<T, VEC>::st(Q + (int64_t)row * k + c0, acc);
+      for (int m = 0; m < VEC; ++m) {
+        rv[m] = fma(-al[m], acc[m], rb[u][m]);
+        dv[m] = fma(al[m], x0[m], db[u][m]);
+        pd[m] += (double)rv[m] * (double)rv[m];
+      }
+      if (valid) {
+        VecIO<T, VEC>::st(Rv + (int64_t)row * k + c0, rv);
+        VecIO<T, VEC>::st(Dv + (int64_t)row * k + c0, dv);
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) pd[m] += (double)x0[m] * (double)acc[m];
+      if (MODE == KL_PLAIN && valid) VecIO<T, VEC>::st(Q + (int64_t)row * k + c0, acc);
+    }
   };
   const int32_t t0 = -kLdsBack - kLdsAhead;
   // prologue: the first D steps' loads
 #pragma unroll
-  for (int u = 0; u < D; ++u) issue(t0 - D + u, pb[u], ib[u]);
+  for (int u = 0; u < D; ++u) issue(t0 - D + u, u);
   __syncthreads();  // the far cache
   // steps t0 .. nch - 1, rounded up to whole groups of D: every group runs
   // all its steps (the ones past nch compute nothing), so the compiler sees
@@ -351,13 +418,15 @@ kl_spmm_dot(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
 #pragma unroll
     for (int u = 0; u < D; ++u) {
       const int32_t t = t0 + gi * D + u;
-      land(t, pb[u], ib[u]);
-      issue(t, pb[u], ib[u]);
+      land(t, u);
+      issue(t, u);
       __syncthreads();
-      if (t >= 0 && t < nch) compute(t);
+      if (t >= 0 && t < nch) compute(t, u);
+      if constexpr (MODE == KL_UPD) issue_rd(t, u);
     }
   }
-  // block partials of p.q: the 32 row slots of each column, in order
+  // block partials (p.q, or r.r for KL_UPD): the 32 row slots of each
+  // column, in order
   __syncthreads();
   double *sh = reinterpret_cast<double *>(ring);
 #pragma unroll
@@ -1339,7 +1408,7 @@ k_reduce_alpha(int32_t nrows, int64_t k, const double *__restrict__ part,
 //          continuing columns keep p (beta 1, gamma 0), restarted ones take
 //          p = z (beta 0, gamma 1)
 __global__ void __launch_bounds__(kBlock)
-k_reduce_beta(int32_t nrows, int64_t k, const double *__restrict__ prr,
+k_reduce_beta(int32_t nrows, int32_t nrows_rz, int64_t k, const double *__restrict__ prr,
               const double *__restrict__ prz, double *__restrict__ st,
               int32_t *__restrict__ active, const int32_t *__restrict__ restart,
               int32_t *__restrict__ ctl, int mode, double delta2) {
@@ -1347,7 +1416,7 @@ k_reduce_beta(int32_t nrows, int64_t k, const double *__restrict__ prr,
   __shared__ double sh[4][64];
   const int64_t c = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   double rr = col_reduce(prr, nrows, k, c, sh);
-  double rz = col_reduce(prz, nrows, k, c, sh);
+  double rz = col_reduce(prz, nrows_rz, k, c, sh);
   if (threadIdx.x < 64 && c < k) {
     double beta = 1.0, gamma = 0.0;
     if (active[c]) {
@@ -1688,13 +1757,13 @@ struct SolverWork {
   int32_t *h_ctl = nullptr;           // pinned
   double *h_st = nullptr;             // pinned, ST_N x nvc
   // the captured block of inner iterations (fused solve), replayed while
-  // the key -- everything its launches depend on -- is unchanged; captured
-  // on its own stream (a graph cannot be captured on the legacy stream)
+  // the key -- everything its launches depend on -- is unchanged (a few
+  // keys kept); captured on its own stream (a graph cannot be captured on
+  // the legacy stream)
   hipStream_t cap = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  std::vector<uint64_t> gkey;
+  std::vector<std::pair<std::vector<uint64_t>, hipGraphExec_t>> graphs;
   ~SolverWork() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
+    for (auto &e : graphs) (void)hipGraphExecDestroy(e.second);
     if (cap) (void)hipStreamDestroy(cap);
     if (h_ctl) (void)hipHostFree(h_ctl);
     if (h_st) (void)hipHostFree(h_st);
@@ -1932,8 +2001,9 @@ static int32_t launch_spmm_dot(const SysView<T> &V, int64_t k, const T *P, T *Q,
     const int64_t R = (V.n + V.rpr - 1) / V.rpr, pairs = R * ((k + W - 1) / W);
     const dim3 g((unsigned)(kXcd * ((pairs + kXcd - 1) / kXcd)));
     ProfScope ps("spmm_lds", s);
-    kl_spmm_dot<T, 6><<<g, 512, 0, s>>>(V.n, V.rpr, V.wpk, V.far_ptr, V.far_rows, k, P, Q,
-                                        part, ctl);
+    kl_spmm<T, 6, KL_PLAIN><<<g, 512, 0, s>>>(V.n, V.rpr, V.wpk, V.far_ptr, V.far_rows, k, P,
+                                              nullptr, nullptr, Q, nullptr, nullptr, nullptr,
+                                              part, ctl);
     return (int32_t)R;
   }
   Grid G = grid_spmm<T>(V.n, k, (unsigned)V.gens);
@@ -1945,6 +2015,22 @@ static int32_t launch_spmm_dot(const SysView<T> &V, int64_t k, const T *P, T *Q,
 #undef N_
 #undef W_
   return G.rows;
+}
+
+// The direction update + SpMM (KL_DIR: Pin -> Pout, partial p.q) and the
+// SpMM + residual and iterate update (KL_UPD: partial r.r) of the fused
+// iteration; partial rows written = row ranges.
+template <typename T, int MODE>
+static int32_t launch_kl(const SysView<T> &V, int64_t k, const T *P, const T *Z, T *Pout,
+                         T *R, T *D, const double *st, double *part, const int32_t *ctl,
+                         hipStream_t s) {
+  const int64_t W = 256 / sizeof(T);
+  const int64_t nr = (V.n + V.rpr - 1) / V.rpr, pairs = nr * ((k + W - 1) / W);
+  const dim3 g((unsigned)(kXcd * ((pairs + kXcd - 1) / kXcd)));
+  ProfScope ps(MODE == KL_DIR ? "spmm_dir" : "spmm_upd", s);
+  kl_spmm<T, 6, MODE><<<g, 512, 0, s>>>(V.n, V.rpr, V.wpk, V.far_ptr, V.far_rows, k, P, Z,
+                                        Pout, nullptr, R, D, st, part, ctl);
+  return (int32_t)nr;
 }
 
 template <typename TO>
@@ -2071,16 +2157,18 @@ static LevelGrid level_grid(const SysView<T> &V, int64_t k, int32_t nb, bool bwd
 // First half of z = M^-1 r: (FUSED: r -= alpha q) then y = L^-1 r (IC(0))
 // or z = r / d (Jacobi); partials r.r -> part[0..rows) and r.M^-1.r ->
 // part[stride..].  Returns the partial rows written.
+// no_rupd: the residual update and its r.r partials were done already (the
+// LDS-staged KL_UPD product); only the preconditioner and r.z run.
 template <typename T, bool FUSED>
 static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
                           const T *Q, T *R, T *Z, double *part, int64_t stride,
-                          const int32_t *ctl, hipStream_t s) {
+                          const int32_t *ctl, hipStream_t s, bool no_rupd = false) {
   if (V.hsc || V.sgs || V.sm.of_row) {
     Grid G = grid_rows<T>(V.n, k, kCapSpmm);
     ProfScope ps("fwd", s);
 #define N_(KK) kf_rupd<T, KK, FUSED><<<G.g, kFlat(KK), 0, s>>>(V.n, alpha, V.sm, Q, R, part, ctl)
 #define W_(VV, LL) kw_rupd<T, WideV<T, VV>::v, FUSED, LL><<<G.g, kBlock, 0, s>>>(V.n, alpha, V.sm, k, Q, R, part, ctl)
-    DISPATCH_K(k, T, G.lpr, N_, W_);
+    if (!no_rupd) DISPATCH_K(k, T, G.lpr, N_, W_);
 #undef N_
 #undef W_
     if (V.hsc) {
@@ -2282,8 +2370,10 @@ struct Red {
   int32_t rows;
   int64_t stride;
 };
+// slot 1 writes the first-stage sums to the second half of the buffer (a
+// second reduction whose result is needed together with slot 0's)
 static Red reduce_rows(SolverWork *W, const double *part, int32_t nrows, int64_t k,
-                       int64_t stride, int np, hipStream_t s) {
+                       int64_t stride, int np, hipStream_t s, int slot = 0) {
   // every partial-writing grid is sized into part_rows when the workspace
   // is set up; a grid past it would have written out of the buffer
   if (nrows > W->part_rows) throw HipError{hipErrorInvalidValue, "partial rows > workspace"};
@@ -2293,9 +2383,9 @@ static Red reduce_rows(SolverWork *W, const double *part, int32_t nrows, int64_t
   S = (nrows + chunk - 1) / chunk;
   dim3 g((unsigned)((k + 63) / 64), (unsigned)S);
   const int64_t ostride = (int64_t)kRedRows * k;
-  k_reduce_stage1<<<g, kBlock, 0, s>>>(nrows, chunk, k, part, stride, np, W->red.get(),
-                                       ostride);
-  return Red{W->red.get(), S, ostride};
+  double *out = W->red.get() + (int64_t)slot * ostride;
+  k_reduce_stage1<<<g, kBlock, 0, s>>>(nrows, chunk, k, part, stride, np, out, ostride);
+  return Red{out, S, ostride};
 }
 
 // Row table of A_ff for the SpMM: pattern once per system, values at every
@@ -2578,6 +2668,18 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   T *Z = reinterpret_cast<T *>(W->buf[2].get());
   T *P = reinterpret_cast<T *>(W->buf[3].get());
   T *Q = reinterpret_cast<T *>(W->buf[4].get());
+  // The fused iteration (a single system preconditioned by HSC or SGS, with
+  // the LDS-staged SpMM): the direction update rides on the product
+  // (KL_DIR) and the residual and iterate updates on a second product
+  // (KL_UPD), so there is no Q; its buffer holds the other of two P buffers
+  // (KL_DIR reads p_old from one and writes p_new to the other).  The
+  // direction update of an iteration is applied by the next iteration's
+  // KL_DIR ("pending"), or, before a round that re-applies the
+  // preconditioner, by a plain update (the round's new r.z overwrites beta).
+  const bool klf = !batch && !F && (H || G) && V.rpr > 0 && win_applies<T>(n, k);
+  T *Pb[2] = {P, Q};
+  int cur = 0;
+  bool pending = false;
   // x accumulates in the caller's d_x (no copy in or out) unless d_x is d_b,
   // which is read until the end
   double *X = d_x != d_b ? d_x : W->xp.get(), *R64 = W->rp.get();
@@ -2682,18 +2784,23 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     if (!r_scaled)
       launch_scale_in<T>(V, k, st + ST_S * kk, rsrc, R, s);
     if (apply_m) {
+      // the deferred direction update first (alpha is 0 here: no d update),
+      // while z and beta are still the last iteration's
+      if (klf && pending) launch_bwd<T>(V, k, st, Z, Pb[cur], D, ctl, s);
+      pending = false;
       // z = M^-1 r; rz; continuing columns keep p, restarted ones take p = z
       rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, s);
       if (batch) {
         launch_segdot<T, T>(S, W, k, R, R, R, Z, s);
-        k_reduce_beta<<<red, kBlock, 0, s>>>(1, kk, sg0, sg1, st, W->active.get(),
+        k_reduce_beta<<<red, kBlock, 0, s>>>(1, 1, kk, sg0, sg1, st, W->active.get(),
                                              W->restart.get(), ctl, 1, delta2);
       } else {
         rd = reduce_rows(W, p1, rows, k, stride, 2, s);
-        k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, rd.p + rd.stride, st,
+        k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, rd.rows, k, rd.p, rd.p + rd.stride, st,
                                              W->active.get(), W->restart.get(), ctl, 1, delta2);
       }
-      launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
+      if (klf) pending = true;   // p = gamma z + beta p in the first KL_DIR
+      else launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
     } else {
       // new reference for the replacement trigger: the scaled true r.r
       for (int64_t c = 0; c < kk; ++c) {
@@ -2713,6 +2820,22 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     // one CG iteration: every launch reads its scalars and flags from the
     // device, so the same launches serve every iteration
     auto iterate = [&](hipStream_t q) {
+      if (klf) {
+        const int32_t r0 = launch_kl<T, KL_DIR>(V, k, Pb[cur], Z, Pb[cur ^ 1], nullptr,
+                                                nullptr, st, p0, ctl, q);
+        Red a1 = reduce_rows(W, p0, r0, k, stride, 1, q);
+        k_reduce_alpha<<<red, kBlock, 0, q>>>(a1.rows, k, a1.p, st, W->active.get(), ctl);
+        const int32_t r1 = launch_kl<T, KL_UPD>(V, k, Pb[cur ^ 1], nullptr, nullptr, R, D, st,
+                                                p1, ctl, q);
+        const int32_t r2 = launch_fwd<T, true>(V, k, st + ST_ALPHA * kk, nullptr, R, Z, p1,
+                                               stride, ctl, q, true);
+        Red b1 = reduce_rows(W, p1, r1, k, stride, 1, q, 0);
+        Red b2 = reduce_rows(W, p1 + stride, r2, k, stride, 1, q, 1);
+        k_reduce_beta<<<red, kBlock, 0, q>>>(b1.rows, b2.rows, k, b1.p, b2.p, st,
+                                             W->active.get(), W->restart.get(), ctl, 0, delta2);
+        cur ^= 1;
+        return;
+      }
       int32_t rw = launch_spmm_dot<T>(V, k, P, Q, p0, ctl, q);
       if (batch) {
         launch_segdot<T, T>(S, W, k, P, Q, nullptr, nullptr, q);
@@ -2724,50 +2847,63 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
       rw = launch_fwd<T, true>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, q);
       if (batch) {
         launch_segdot<T, T>(S, W, k, R, R, R, Z, q);
-        k_reduce_beta<<<red, kBlock, 0, q>>>(1, kk, sg0, sg1, st, W->active.get(),
+        k_reduce_beta<<<red, kBlock, 0, q>>>(1, 1, kk, sg0, sg1, st, W->active.get(),
                                              W->restart.get(), ctl, 0, delta2);
       } else {
         Red r2 = reduce_rows(W, p1, rw, k, stride, 2, q);
-        k_reduce_beta<<<red, kBlock, 0, q>>>(r2.rows, k, r2.p, r2.p + r2.stride, st,
+        k_reduce_beta<<<red, kBlock, 0, q>>>(r2.rows, r2.rows, k, r2.p, r2.p + r2.stride, st,
                                              W->active.get(), W->restart.get(), ctl, 0, delta2);
       }
       launch_bwd<T>(V, k, st, Z, P, D, ctl, q);
     };
     const bool use_graph = o->graph && !g_prof.on && every > 1;
-    if (use_graph) {
-      // everything the captured launches depend on besides W's own buffers
+    // the captured block for the current P parity: everything its launches
+    // depend on besides W's own buffers is in the key
+    auto block_graph = [&]() -> hipGraphExec_t {
       const std::vector<uint64_t> key = {
           (uint64_t)k, sizeof(T), (uint64_t)every, (uint64_t)kk,
           H ? H->serial : 0, H ? (uint64_t)(uintptr_t)H->work.get() : 0,
           F ? F->serial : 0, G ? G->serial : 0, (uint64_t)V.rpr,
           (uint64_t)(uintptr_t)V.wpk, (uint64_t)(uintptr_t)V.ecol, (uint64_t)V.gens,
-          (uint64_t)(uint32_t)V.hsh.level_cap, (uint64_t)(uint32_t)V.hsh.tail_rows};
-      if (!W->gexec || W->gkey != key) {
-        if (W->gexec) { CN_HIP(hipGraphExecDestroy(W->gexec)); W->gexec = nullptr; }
-        if (!W->cap) CN_HIP(hipStreamCreateWithFlags(&W->cap, hipStreamNonBlocking));
-        CN_HIP(hipStreamBeginCapture(W->cap, hipStreamCaptureModeThreadLocal));
-        hipGraph_t gr = nullptr;
-        try {
-          for (int32_t e = 0; e < every; ++e) iterate(W->cap);
-        } catch (...) {
-          (void)hipStreamEndCapture(W->cap, &gr);
-          if (gr) (void)hipGraphDestroy(gr);
-          throw;
-        }
-        CN_HIP(hipStreamEndCapture(W->cap, &gr));
-        const hipError_t ie = hipGraphInstantiate(&W->gexec, gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        CN_HIP(ie);
-        W->gkey = key;
+          (uint64_t)(uint32_t)V.hsh.level_cap, (uint64_t)(uint32_t)V.hsh.tail_rows,
+          (uint64_t)klf, (uint64_t)cur};
+      for (auto &e : W->graphs)
+        if (e.first == key) return e.second;
+      if (W->graphs.size() >= 4) {
+        CN_HIP(hipGraphExecDestroy(W->graphs.front().second));
+        W->graphs.erase(W->graphs.begin());
       }
-    }
+      if (!W->cap) CN_HIP(hipStreamCreateWithFlags(&W->cap, hipStreamNonBlocking));
+      CN_HIP(hipStreamBeginCapture(W->cap, hipStreamCaptureModeThreadLocal));
+      hipGraph_t gr = nullptr;
+      const int cur0 = cur;
+      try {
+        for (int32_t e = 0; e < every; ++e) iterate(W->cap);
+      } catch (...) {
+        cur = cur0;
+        (void)hipStreamEndCapture(W->cap, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        throw;
+      }
+      cur = cur0;  // captured, not run
+      CN_HIP(hipStreamEndCapture(W->cap, &gr));
+      hipGraphExec_t ex = nullptr;
+      const hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      CN_HIP(ie);
+      W->graphs.emplace_back(key, ex);
+      return ex;
+    };
     while (it < budget) {
       if (use_graph && budget - it >= every) {
-        CN_HIP(hipGraphLaunch(W->gexec, s));
+        CN_HIP(hipGraphLaunch(block_graph(), s));
+        if (klf && (every & 1)) cur ^= 1;
         it += every;
+        if (klf) pending = true;
       } else {
         iterate(s);
         ++it;
+        if (klf) pending = true;
       }
       if (it % every == 0 || it == budget) {
         CN_HIP(hipMemcpyAsync(hc, ctl, sizeof(int32_t) * 6, hipMemcpyDeviceToHost, s));
