@@ -112,13 +112,16 @@ def main():
     ap.add_argument("--plain-poses", type=int, default=8,
                     help="poses solved by the plain FP64 one-solve-per-launch path")
     ap.add_argument("--workload", default="capsules100k",
-                    choices=["capsules100k", "meshes64", "icosphere9", "ddm"],
+                    choices=["capsules100k", "meshes64", "icosphere9", "ddm", "setup"],
                     help="capsules100k: 256 poses x 3 RHS on the 100k capsule union (the "
                          "default); meshes64: 64 uneven meshes x 3 RHS, batched vs one solve "
                          "per mesh; icosphere9: 16 poses x 3 RHS forward + adjoint on the "
                          "2.6M-vertex icosphere with 64 loops; ddm: Direct Delta Mush bind "
                          "(harvest of every handle, smoothing, top-8) and the FP32 skinning "
-                         "matvec over 256 poses on the capsule union")
+                         "matvec over 256 poses on the capsule union; setup: the "
+                         "set-up stages (seam cut, system build, hierarchy, RHS table) "
+                         "and the re-assembly kernels on the capsule union and the "
+                         "icosphere")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="no GPU: turn two rocprofv3 --pmc counter files (FETCH_SIZE, "
                          "WRITE_SIZE) of a --copy-mb run into the traffic file")
@@ -149,6 +152,9 @@ def main():
         return
     if args.workload == "ddm":
         ddm_bench(args)
+        return
+    if args.workload == "setup":
+        setup_bench(args)
         return
 
     if args.copy_mb > 0:
@@ -811,6 +817,100 @@ def icosphere9_bench(args):
                   "setup_s": time.time() - t0 - el},
     }
     print(json.dumps(line))
+
+
+def setup_bench(args):
+    """The set-up path of a mesh, stage by stage: the seam cut (halfedges and
+    seam duplication), the system build (halfedges, incoming lists, CSR
+    pattern, assembly plan, values), the HSC hierarchy, the harmonic RHS
+    table; then the re-assembly of new positions (cn_system_set_positions),
+    kernel by kernel with HIP events, against its algorithmic bytes.  One
+    JSON line per mesh."""
+    import numpy as np
+    import torch
+
+    import tool_godot_curvenet_amd as cn
+    from tool_godot_curvenet_amd import _lib, meshgen
+
+    assert torch.cuda.is_available(), "no GPU"
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    peak = 8000.0
+    for name, reps in (("capsules100k", 20), ("icosphere9", 10)):
+        t = time.perf_counter()
+        mesh = meshgen.config_mesh(name)
+        t_mesh = time.perf_counter() - t
+        print(f"setup {name}: mesh generated in {t_mesh:.1f} s", file=sys.stderr, flush=True)
+
+        def timed(fn):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            return r, time.perf_counter() - t0
+
+        (nt, origin, con, nv2), t_cut = timed(
+            lambda: cn.cut_seams(mesh.nv, mesh.tris, mesh.cut_edges))
+        pos = np.ascontiguousarray(mesh.pos[origin])
+        lap, t_sys = timed(lambda: cn.CutLaplacian(pos, nt, con))
+        hsc, t_hsc = timed(lambda: cn.HSC(lap, theta=args.theta))
+        n, ncon, nv, nf = lap.n_free, lap.n_con, lap.info.nv, lap.info.nf
+        nnz_ff, nnz_fc = lap.info.nnz_ff, lap.info.nnz_fc
+        rng = np.random.default_rng(meshgen.BASE_SEED + 60)
+        Xc = torch.as_tensor(rng.uniform(-0.2, 0.2, size=(ncon, 3)), device=dev)
+        _, t_rhs_first = timed(lambda: lap.harmonic_rhs(Xc, as_numpy=False))
+        _, t_rhs = timed(lambda: lap.harmonic_rhs(Xc, as_numpy=False))
+        print(f"setup {name}: cut {t_cut:.2f} s, system {t_sys:.2f} s, hierarchy "
+              f"{t_hsc:.2f} s", file=sys.stderr, flush=True)
+        # re-assembly of the same positions: the values must not change
+        a0 = lap.A_ff().values.copy()
+        pos_d = torch.as_tensor(pos, device=dev)
+        import ctypes as C
+        sp = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        names = ("asm_face", "asm_values", "asm_diag")
+        ms = {k: 0.0 for k in names}
+        lib.cn_profile_enable(1)
+        wall = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _lib.check(lib.cn_system_set_positions(lap.handle, C.c_void_p(pos_d.data_ptr()),
+                                                   sp), "set_positions")
+            wall.append(time.perf_counter() - t0)
+            for k in names:
+                v = C.c_double(0.0)
+                lib.cn_profile_query(k.encode(), C.byref(v))
+                ms[k] += v.value
+        lib.cn_profile_enable(0)
+        same = bool(np.array_equal(lap.A_ff().values, a0))
+        # terms of the assembly plan: 3 per face corner at a free vertex
+        nterms = int(3 * np.count_nonzero(con[nt.reshape(-1)] == 0))
+        tot = nnz_ff + nnz_fc
+        by = {"asm_face": 12 * nf + 24 * nv + 72 * nf,
+              "asm_values": 4 * (tot + 1) + 4 * nterms + 72 * nf + 12 * tot,
+              "asm_diag": 20 * n}
+        kern = {}
+        for k in names:
+            avg = ms[k] / reps
+            gbs = by[k] / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
+            kern[k] = {"avg_ms": avg, "bytes_per_launch": by[k], "achieved_gbs": gbs,
+                       "frac": gbs / peak}
+        line = {
+            "metric": f"set-up stages and re-assembly ({name})",
+            "workload": name,
+            "nv": nv, "nf": nf, "n_free": n, "n_con": ncon, "nnz_ff": nnz_ff,
+            "nnz_fc": nnz_fc, "assembly_terms": nterms,
+            "stages_s": {"mesh_generation_host": t_mesh, "cut_seams": t_cut,
+                         "system_build": t_sys, "hsc_build": t_hsc,
+                         "hsc_levels": hsc.info.nlevels, "hsc_coarse": hsc.info.n_coarse,
+                         "harmonic_rhs_first_call": t_rhs_first, "harmonic_rhs": t_rhs},
+            "reassembly": {"reps": reps, "wall_ms_median": 1e3 * float(np.median(wall)),
+                           "kernels": kern, "values_unchanged": same},
+            "device": torch.cuda.get_device_name(0),
+        }
+        print(json.dumps(line), flush=True)
+        del lap, hsc, pos_d, Xc
+        torch.cuda.empty_cache()
 
 
 def ddm_bench(args):

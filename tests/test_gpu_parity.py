@@ -131,6 +131,31 @@ def test_cut_laplacian_bit_exact(gpu, name):
         assert np.array_equal(he_g[k], h[k])
 
 
+@pytest.mark.parametrize("name", ["cylinder64", "icosphere6"])
+def test_reassembly_bit_exact(gpu, name):
+    """New positions through set_positions (the assembly plan built with the
+    system, summed again): A_ff, A_fc and the FP32 mirror equal the oracle's
+    assembly at those positions and a freshly built system's."""
+    from tool_godot_curvenet_amd import CutLaplacian
+    from tool_godot_curvenet_amd.curvenet import _np
+    m, lap = _system(name)
+    rng = np.random.default_rng(21)
+    pos2 = lap.pos + rng.uniform(-0.01, 0.01, size=lap.pos.shape)
+    lap.set_positions(pos2)
+    h = O.he_from_triangles(pos2.shape[0], lap.tris)
+    o = O.cut_laplacian_csr(h, lap.constrained, pos2)
+    A, B = lap.A_ff(), lap.A_fc()
+    for got, want in zip((A.row_ptr, A.col_idx, A.values), o["A_ff"]):
+        assert np.array_equal(got, want)
+    for got, want in zip((B.row_ptr, B.col_idx, B.values), o["A_fc"]):
+        assert np.array_equal(got, want)
+    v32 = _np(lap.info.d_ff_val32, lap.info.nnz_ff, np.float32)
+    assert np.array_equal(v32, A.values.astype(np.float32))
+    fresh = CutLaplacian(pos2, lap.tris, lap.constrained)
+    assert np.array_equal(fresh.A_ff().values, A.values)
+    assert np.array_equal(fresh.A_fc().values, B.values)
+
+
 def test_cut_laplacian_matches_dense_spec(gpu):
     """Small cut patch: GPU CSR == the literal dense assembleVtLhV."""
     from tool_godot_curvenet_amd import CutLaplacian, cut_seams

@@ -27,7 +27,7 @@ CSRC = HERE / "csrc"
 LIB = HERE / "libcurvenet_la.so"
 OBJ = HERE / "_build"
 
-EXACT = {"cn_topology.hip", "cn_assembly.hip", "cn_ic0.hip", "cn_spmv.hip", "cn_hsc_build.hip", "cn_sgs.hip", "cn_deform.hip", "cn_ddm.hip"}
+EXACT = {"cn_topology.hip", "cn_assembly.hip", "cn_ic0.hip", "cn_spmv.hip", "cn_hsc_build.hip", "cn_hsc_host.cpp", "cn_sgs.hip", "cn_deform.hip", "cn_ddm.hip"}
 
 
 def _hipcc() -> str:
@@ -56,8 +56,10 @@ def _flags(name: str) -> list[str]:
 def _compile(src: pathlib.Path) -> pathlib.Path:
     out = OBJ / (src.stem + ".o")
     if src.suffix == ".cpp":  # host-only code
-        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{HERE.parent / 'include'}",
-               "-c", str(src), "-o", str(out)]
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall",
+               f"-I{HERE.parent / 'include'}", "-c", str(src), "-o", str(out)]
+        if src.name in EXACT:
+            cmd.insert(1, "-ffp-contract=off")
     else:
         cmd = [_hipcc(), *_flags(src.name), "-c", str(src), "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -76,7 +78,7 @@ def build(verbose: bool = False) -> pathlib.Path:
     tmp = LIB.with_suffix(".so.tmp")
     # --no-undefined: a declaration that matches no definition fails here,
     # not at load time
-    cmd = [_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined",
+    cmd = [_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-Wl,--no-undefined",
            "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -7,15 +7,17 @@
 // Layout: per face 9 doubles (the triangle cot Laplacian of the face loop
 // that starts at halfedge 3f, whose targets are t1, t2, t0); per vertex the
 // ascending list of incoming interior halfedges; per free row the sorted
-// column list.  A row's values are a segmented reduction over its incoming
-// halfedges in ascending order -- one thread per row walks them, so each
-// entry is summed in exactly the spec's order without atomics.
+// column list.  The values are a segmented reduction: every CSR entry is
+// the sum of its terms (face_m entries) in the spec's order, one lane per
+// entry over a plan built once from the pattern, so each entry is summed in
+// exactly the spec's order without atomics.
 #include <hip/hip_runtime.h>
 
 #include <climits>
 
 #include "cn_internal.h"
 #include "cn_kernels.h"
+#include "cn_prof.h"
 
 #pragma clang fp contract(off)
 
@@ -194,43 +196,109 @@ __device__ __forceinline__ int32_t find_col(const int32_t *col, int32_t a,
   return a;
 }
 
-// Row values: for p in inc(v) ascending, for each distinct target j of p's
-// face: t = sum over q (ascending) targeting j of Lh[p][q]; entry(j) += t.
-__global__ void k_row_values(int32_t n_free, const int32_t *free_vertex,
-                             const int32_t *tris, const int32_t *iptr,
-                             const int32_t *ihe, const double *m,
-                             const uint8_t *constrained,
-                             const int32_t *free_of, const int32_t *con_of,
-                             const int32_t *ff_ptr, const int32_t *ff_col,
-                             double *ff_val, const int32_t *fc_ptr,
-                             const int32_t *fc_col, double *fc_val) {
-  int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+// Row values (the spec's order): for p in inc(v) ascending, for each
+// distinct target j of p's face: t = sum over q (ascending) targeting j of
+// Lh[p][q], from 0; entry(j) += t, from 0.  Each CSR entry is a segment of
+// that walk: the face_m positions of its terms in that order, a term that
+// starts a new t flagged in bit 31.  The plan depends only on the pattern,
+// so it is built once per system (two passes of the walk, one thread per
+// row: counts, then the terms); every assembly then sums each segment on
+// one lane.
+constexpr uint32_t kGroupStart = 0x80000000u;
+
+template <bool FILL>
+__global__ void k_asm_plan(int32_t n_free, const int32_t *__restrict__ free_vertex,
+                           const int32_t *__restrict__ tris, const int32_t *__restrict__ iptr,
+                           const int32_t *__restrict__ ihe,
+                           const uint8_t *__restrict__ constrained,
+                           const int32_t *__restrict__ free_of,
+                           const int32_t *__restrict__ con_of,
+                           const int32_t *__restrict__ ff_ptr,
+                           const int32_t *__restrict__ ff_col,
+                           const int32_t *__restrict__ fc_ptr,
+                           const int32_t *__restrict__ fc_col, int64_t nnz_ff, int32_t *cnt,
+                           const int32_t *__restrict__ term_ptr, uint32_t *terms,
+                           int32_t *diag_pos) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_free) return;
-  int32_t v = free_vertex[r];
-  int32_t a0 = ff_ptr[r], a1 = ff_ptr[r + 1], b0 = fc_ptr[r], b1 = fc_ptr[r + 1];
-  for (int32_t i = a0; i < a1; ++i) ff_val[i] = 0.0;
-  for (int32_t i = b0; i < b1; ++i) fc_val[i] = 0.0;
+  const int32_t v = free_vertex[r];
+  const int32_t a0 = ff_ptr[r], a1 = ff_ptr[r + 1], b0 = fc_ptr[r], b1 = fc_ptr[r + 1];
+  // cnt: the count pass counts every entry's terms, the fill pass uses it
+  // as the entry's write cursor; only this row's thread touches its entries
+  for (int32_t i = a0; i < a1; ++i) cnt[i] = FILL ? term_ptr[i] : 0;
+  for (int32_t i = b0; i < b1; ++i) cnt[nnz_ff + i] = FILL ? term_ptr[nnz_ff + i] : 0;
+  if (FILL) {
+    const int32_t d = find_col(ff_col, a0, a1, r);
+    diag_pos[r] = (d < a1 && ff_col[d] == r) ? d : -1;
+  }
   for (int32_t a = iptr[v]; a < iptr[v + 1]; ++a) {
-    int32_t p = ihe[a], f = p / 3, lp = p - 3 * f;
-    const double *mf = m + 9 * (int64_t)f + 3 * lp;
-    int32_t tj[3] = {tgt3(tris, 3 * f), tgt3(tris, 3 * f + 1), tgt3(tris, 3 * f + 2)};
+    const int32_t p = ihe[a], f = p / 3, lp = p - 3 * f;
+    const int32_t tj[3] = {tgt3(tris, 3 * f), tgt3(tris, 3 * f + 1), tgt3(tris, 3 * f + 2)};
     for (int lq = 0; lq < 3; ++lq) {
-      int32_t j = tj[lq];
+      const int32_t j = tj[lq];
       bool seen = false;
       for (int e = 0; e < lq; ++e) seen |= (tj[e] == j);
       if (seen) continue;
-      double t = 0.0;
-      for (int e = lq; e < 3; ++e)
-        if (tj[e] == j) t = t + (0.0 + mf[e]);
+      int64_t pos;
       if (constrained[j]) {
-        int32_t pos = find_col(fc_col, b0, b1, con_of[j]);
-        if (pos < b1 && fc_col[pos] == con_of[j]) fc_val[pos] = fc_val[pos] + t;
+        const int32_t q = find_col(fc_col, b0, b1, con_of[j]);
+        if (!(q < b1 && fc_col[q] == con_of[j])) continue;
+        pos = nnz_ff + q;
       } else {
-        int32_t pos = find_col(ff_col, a0, a1, free_of[j]);
-        if (pos < a1 && ff_col[pos] == free_of[j]) ff_val[pos] = ff_val[pos] + t;
+        const int32_t q = find_col(ff_col, a0, a1, free_of[j]);
+        if (!(q < a1 && ff_col[q] == free_of[j])) continue;
+        pos = q;
+      }
+      for (int e = lq; e < 3; ++e) {
+        if (tj[e] != j) continue;
+        if (FILL)
+          terms[cnt[pos]++] = (uint32_t)(9 * f + 3 * lp + e) | (e == lq ? kGroupStart : 0u);
+        else
+          ++cnt[pos];
       }
     }
   }
+}
+
+// Every entry of A_ff and A_fc (A_fc's after A_ff's): the in-order sum of
+// its segment, written in FP64 and FP32.  Lanes take consecutive entries,
+// so the plan is read and the values written in whole cache lines; the
+// face_m terms of a row's entries come from the few faces around it.
+__global__ void __launch_bounds__(256)
+k_asm_values(int64_t nnz_tot, int64_t nnz_ff, const int32_t *__restrict__ term_ptr,
+             const uint32_t *__restrict__ terms, const double *__restrict__ m,
+             double *__restrict__ ff_val, float *__restrict__ ff_val32,
+             double *__restrict__ fc_val, float *__restrict__ fc_val32) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz_tot) return;
+  double acc = 0.0, t = 0.0;
+  bool any = false;
+  const int32_t q1 = term_ptr[e + 1];
+  for (int32_t q = term_ptr[e]; q < q1; ++q) {
+    const uint32_t u = terms[q];
+    if ((u & kGroupStart) && any) {
+      acc = acc + t;
+      t = 0.0;
+    }
+    t = t + (0.0 + m[u & ~kGroupStart]);
+    any = true;
+  }
+  if (any) acc = acc + t;
+  if (e < nnz_ff) {
+    ff_val[e] = acc;
+    ff_val32[e] = (float)acc;
+  } else {
+    fc_val[e - nnz_ff] = acc;
+    fc_val32[e - nnz_ff] = (float)acc;
+  }
+}
+
+__global__ void k_diag_at(int32_t n, const int32_t *__restrict__ diag_pos,
+                          const double *__restrict__ val, double *__restrict__ d) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t q = diag_pos[i];
+  d[i] = q >= 0 ? val[q] : 0.0;
 }
 
 __global__ void k_mirror(int64_t n, const double *x, float *y) {
@@ -433,31 +501,61 @@ static void launch_harmonic_rhs(cn_system_t *S, int32_t k, const double *Xc, dou
 #undef H_
 }
 
+// the assembly plan of a system (once: it depends only on the pattern)
+static void build_asm_plan(cn_system_t *S, hipStream_t s) {
+  const int64_t tot = S->nnz_ff + S->nnz_fc;
+  if ((int64_t)9 * S->nf >= (int64_t)kGroupStart)
+    throw HipError{hipErrorInvalidValue, "face table exceeds 2^31 entries"};
+  S->asm_ptr.alloc((size_t)tot + 1);
+  S->diag_pos.alloc((size_t)S->n_free + 1);
+  // zeroed whole: the count pass leaves the last slot, which the scan reads
+  CN_HIP(hipMemsetAsync(S->asm_ptr.get(), 0, sizeof(int32_t) * ((size_t)tot + 1), s));
+  if (S->n_free == 0 || tot == 0) {
+    S->asm_terms.alloc(1);
+    return;
+  }
+  // at most 9 terms per face, so the int32 scan cannot overflow
+  const dim3 g(grid_for(S->n_free, 128)), b(128);
+  k_asm_plan<false><<<g, b, 0, s>>>(
+      S->n_free, S->free_vertex.get(), S->tris.get(), S->inc.ptr.get(), S->inc.he.get(),
+      S->constrained.get(), S->free_of.get(), S->con_of.get(), S->ff_ptr.get(),
+      S->ff_col.get(), S->fc_ptr.get(), S->fc_col.get(), S->nnz_ff, S->asm_ptr.get(), nullptr,
+      nullptr, nullptr);
+  CN_HIP(hipGetLastError());
+  const int64_t nterms = exclusive_scan_i32(S->asm_ptr.get(), tot + 1, s);
+  S->asm_terms.alloc((size_t)nterms + 1);
+  DevBuf<int32_t> cursor((size_t)tot + 1);
+  k_asm_plan<true><<<g, b, 0, s>>>(
+      S->n_free, S->free_vertex.get(), S->tris.get(), S->inc.ptr.get(), S->inc.he.get(),
+      S->constrained.get(), S->free_of.get(), S->con_of.get(), S->ff_ptr.get(),
+      S->ff_col.get(), S->fc_ptr.get(), S->fc_col.get(), S->nnz_ff, cursor.get(),
+      S->asm_ptr.get(), S->asm_terms.get(), S->diag_pos.get());
+  CN_HIP(hipGetLastError());
+  CN_HIP(hipStreamSynchronize(s));  // cursor goes out of scope
+}
+
 static void assemble_values(cn_system_t *S, hipStream_t s) {
-  if (S->nf > 0 && S->lap_kind == CN_LAP_ROBUST)
-    k_face_robust<<<grid_for(S->nf, 256), 256, 0, s>>>(S->nf, S->tris.get(), S->pos.get(),
-                                                      S->lap_delta, S->face_m.get());
-  else if (S->nf > 0)
-    k_face_cot<<<grid_for(S->nf, 256), 256, 0, s>>>(S->nf, S->tris.get(),
-                                                   S->pos.get(),
-                                                   S->face_m.get());
-  if (S->n_free > 0)
-    k_row_values<<<grid_for(S->n_free, 128), 128, 0, s>>>(
-        S->n_free, S->free_vertex.get(), S->tris.get(), S->inc.ptr.get(),
-        S->inc.he.get(), S->face_m.get(), S->constrained.get(),
-        S->free_of.get(), S->con_of.get(), S->ff_ptr.get(), S->ff_col.get(),
-        S->ff_val.get(), S->fc_ptr.get(), S->fc_col.get(), S->fc_val.get());
-  if (S->nnz_ff > 0)
-    k_mirror<<<grid_for(S->nnz_ff, 256), 256, 0, s>>>(S->nnz_ff, S->ff_val.get(),
-                                                     S->ff_val32.get());
-  if (S->nnz_fc > 0)
-    k_mirror<<<grid_for(S->nnz_fc, 256), 256, 0, s>>>(S->nnz_fc, S->fc_val.get(),
-                                                     S->fc_val32.get());
-  if (S->n_free > 0)
-    k_diag<<<grid_for(S->n_free, 256), 256, 0, s>>>(S->n_free, S->ff_ptr.get(),
-                                                   S->ff_col.get(),
-                                                   S->ff_val.get(),
-                                                   S->diag.get());
+  if (S->nf > 0) {
+    ProfScope pf("asm_face", s);
+    if (S->lap_kind == CN_LAP_ROBUST)
+      k_face_robust<<<grid_for(S->nf, 256), 256, 0, s>>>(S->nf, S->tris.get(), S->pos.get(),
+                                                        S->lap_delta, S->face_m.get());
+    else
+      k_face_cot<<<grid_for(S->nf, 256), 256, 0, s>>>(S->nf, S->tris.get(), S->pos.get(),
+                                                     S->face_m.get());
+  }
+  const int64_t tot = S->nnz_ff + S->nnz_fc;
+  if (tot > 0) {
+    ProfScope pf("asm_values", s);
+    k_asm_values<<<grid_for(tot, 256), 256, 0, s>>>(
+        tot, S->nnz_ff, S->asm_ptr.get(), S->asm_terms.get(), S->face_m.get(),
+        S->ff_val.get(), S->ff_val32.get(), S->fc_val.get(), S->fc_val32.get());
+  }
+  if (S->n_free > 0) {
+    ProfScope pf("asm_diag", s);
+    k_diag_at<<<grid_for(S->n_free, 256), 256, 0, s>>>(S->n_free, S->diag_pos.get(),
+                                                       S->ff_val.get(), S->diag.get());
+  }
   CN_HIP(hipGetLastError());
   CN_HIP(hipStreamSynchronize(s));
 }
@@ -564,6 +662,7 @@ static cn_status system_create(int32_t nv, int32_t nf, const int32_t *d_tris,
           S->constrained.get(), S->free_of.get(), S->con_of.get(),
           S->ff_ptr.get(), S->fc_ptr.get(), S->ff_col.get(), S->fc_col.get());
     CN_HIP(hipGetLastError());
+    build_asm_plan(S, s);
     assemble_values(S, s);
   } catch (...) {
     delete S;
@@ -685,7 +784,9 @@ cn_status cn_system_set_positions(cn_system S, const double *d_pos, void *stream
   hipStream_t s = as_stream(stream);
   CN_HIP(hipMemcpyAsync(S->pos.get(), d_pos, sizeof(double) * 3 * (size_t)S->nv,
                         hipMemcpyDeviceToDevice, s));
+  prof_reset();
   assemble_values(S, s);
+  prof_collect();
   return CN_OK;
   CN_API_END
 }

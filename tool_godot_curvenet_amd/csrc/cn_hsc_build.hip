@@ -1,22 +1,16 @@
 // cn_hsc_build.hip -- construction of the hierarchical Schur preconditioner
-// (host C++, FP64, built with -ffp-contract=off so the hierarchy equals the
-// oracle's restatement bit for bit).
-//
-// After Krishnan, Fattal, Szeliski 2013 (HSC).  The per-vertex step is the
-// spec's HSC.eliminateVertex (HierarchicalSparsifyCompensate.lean:96-125):
-// eliminating v adds w_vi w_vj / deg(v) to every pair of v's neighbours.
-// On an SPD matrix the pivot is the diagonal a_vv (degree plus the weight to
-// constrained vertices) and the update is s_ij -= a_iv a_vj / a_vv.  A level
-// eliminates a whole independent set F at once, which is exact because F's
-// rows do not couple: S = A_CC - A_CF D_F^-1 A_FC.  Fill is then sparsified
-// (drop small new entries, compensate on the diagonal) to keep the next
-// level as sparse as the mesh.
+// (FP64, built with -ffp-contract=off so the hierarchy equals the oracle's
+// restatement bit for bit).  Each level is built on host threads
+// (cn_hsc_host.cpp: independent set, exact Schur complement, sparsified
+// fill with diagonal compensation) and uploaded; the dense coarse matrix is
+// factored and inverted on the GPU.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
+#include "cn_hsc_host.h"
 #include "cn_internal.h"
 
 #pragma clang fp contract(off)
@@ -27,172 +21,15 @@ int dense_cholesky(int n, double *A, int64_t lda, hipStream_t s);
 void dense_cholesky_solve(int n, int k, const double *L, int64_t lda, double *B, int64_t ldb,
                           hipStream_t s);
 
-struct HostCsr {
-  std::vector<int32_t> ptr, col;
-  std::vector<double> val;
-  int32_t n() const { return (int32_t)ptr.size() - 1; }
-  double at(int32_t i, int32_t j) const {  // entry (i, j); rows sorted by column
-    auto b = col.begin() + ptr[i], e = col.begin() + ptr[i + 1];
-    auto it = std::lower_bound(b, e, j);
-    return (it != e && *it == j) ? val[it - col.begin()] : 0.0;
-  }
-};
-
-struct HostLevel {
-  int32_t n = 0;
-  std::vector<int32_t> F, C;
-  std::vector<double> dinv;
-  std::vector<int32_t> r_ptr, r_col, p_ptr, p_col;  // r_col: level-l rows of F vertices
-  std::vector<double> r_val, p_val;
-};
-
-// One level: returns false when the level should not be built (A is the
-// coarse matrix then).  On success `next` is the sparsified Schur complement.
-static bool build_level(const HostCsr &A, const cn_hsc_opts &o, HostLevel &L, HostCsr &next) {
-  const int32_t n = A.n();
-  std::vector<int32_t> deg(n);
-  for (int32_t i = 0; i < n; ++i) {
-    int32_t d = 0;
-    for (int32_t e = A.ptr[i]; e < A.ptr[i + 1]; ++e) d += A.col[e] != i;
-    deg[i] = d;
-  }
-  std::vector<int32_t> order(n);
-  for (int32_t i = 0; i < n; ++i) order[i] = i;
-  std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-    return deg[a] != deg[b] ? deg[a] < deg[b] : a < b;
-  });
-  std::vector<char> isF(n, 0), blocked(n, 0);
-  for (int32_t v : order) {
-    if (blocked[v]) continue;
-    isF[v] = 1;
-    blocked[v] = 1;
-    for (int32_t e = A.ptr[v]; e < A.ptr[v + 1]; ++e) blocked[A.col[e]] = 1;
-  }
-  std::vector<int32_t> F, C, cidx(n, -1), fidx(n, -1);
-  for (int32_t i = 0; i < n; ++i) {
-    if (isF[i]) { fidx[i] = (int32_t)F.size(); F.push_back(i); }
-    else { cidx[i] = (int32_t)C.size(); C.push_back(i); }
-  }
-  if ((double)F.size() < o.min_frac * n || F.empty()) return false;
-  const int32_t nC = (int32_t)C.size();
-
-  // S, row by row.  Row c starts as A_CC's row; then for every F neighbour f
-  // of c (ascending), for every neighbour c2 of f (ascending):
-  // S[c][c2] -= (a_cf * a_fc2) / a_ff.  Each entry sees its terms in
-  // ascending f, the order of the f-major loop of the restatement.
-  std::vector<double> acc(nC, 0.0);
-  std::vector<char> touched(nC, 0), orig(nC, 0);
-  std::vector<int32_t> list;
-  std::vector<std::vector<int32_t>> scol(nC), sorig(nC);
-  std::vector<std::vector<double>> sval(nC);
-  for (int32_t t = 0; t < nC; ++t) {
-    const int32_t c = C[t];
-    list.clear();
-    for (int32_t e = A.ptr[c]; e < A.ptr[c + 1]; ++e) {
-      const int32_t j = A.col[e];
-      if (isF[j]) continue;
-      const int32_t u = cidx[j];
-      acc[u] = A.val[e];
-      touched[u] = 1;
-      orig[u] = 1;
-      list.push_back(u);
-    }
-    for (int32_t e = A.ptr[c]; e < A.ptr[c + 1]; ++e) {
-      const int32_t f = A.col[e];
-      if (!isF[f]) continue;
-      const double acf = A.val[e];
-      const double d = A.at(f, f);
-      for (int32_t q = A.ptr[f]; q < A.ptr[f + 1]; ++q) {
-        const int32_t c2 = A.col[q];
-        if (c2 == f) continue;
-        const int32_t u = cidx[c2];
-        if (!touched[u]) { touched[u] = 1; acc[u] = 0.0; list.push_back(u); }
-        acc[u] = acc[u] - acf * A.val[q] / d;
-      }
-    }
-    std::sort(list.begin(), list.end());
-    scol[t] = list;
-    sval[t].resize(list.size());
-    sorig[t].resize(list.size());
-    for (size_t m = 0; m < list.size(); ++m) {
-      sval[t][m] = acc[list[m]];
-      sorig[t][m] = orig[list[m]];
-      touched[list[m]] = 0;
-      orig[list[m]] = 0;
-    }
-  }
-  // diagonal of S before any compensation
-  std::vector<double> sdiag(nC, 0.0);
-  for (int32_t t = 0; t < nC; ++t)
-    for (size_t m = 0; m < scol[t].size(); ++m)
-      if (scol[t][m] == t) sdiag[t] = sval[t][m];
-  // drops: fill entries (i < j) small against sqrt(s_ii s_jj), (i, j) ascending
-  std::vector<std::vector<char>> dropped(nC);
-  for (int32_t t = 0; t < nC; ++t) dropped[t].assign(scol[t].size(), 0);
-  std::vector<double> ndiag = sdiag;
-  for (int32_t i = 0; i < nC; ++i) {
-    for (size_t m = 0; m < scol[i].size(); ++m) {
-      const int32_t j = scol[i][m];
-      if (j <= i || sorig[i][m]) continue;
-      const double s = sval[i][m];
-      if (!(std::fabs(s) < o.theta * std::sqrt(sdiag[i] * sdiag[j]))) continue;
-      const double w = o.compensate == CN_HSC_DIAG ? std::fabs(s) : s;
-      ndiag[i] = ndiag[i] + w;
-      ndiag[j] = ndiag[j] + w;
-      dropped[i][m] = 1;
-      auto it = std::lower_bound(scol[j].begin(), scol[j].end(), i);
-      dropped[j][it - scol[j].begin()] = 1;
-    }
-  }
-  next.ptr.assign(1, 0);
-  next.col.clear();
-  next.val.clear();
-  for (int32_t t = 0; t < nC; ++t) {
-    for (size_t m = 0; m < scol[t].size(); ++m) {
-      if (dropped[t][m]) continue;
-      next.col.push_back(scol[t][m]);
-      next.val.push_back(scol[t][m] == t ? ndiag[t] : sval[t][m]);
-    }
-    next.ptr.push_back((int32_t)next.col.size());
-  }
-
-  L.n = n;
-  L.F = F;
-  L.C = C;
-  L.dinv.resize(F.size());
-  for (size_t f = 0; f < F.size(); ++f) L.dinv[f] = 1.0 / A.at(F[f], F[f]);
-  L.r_ptr.assign(1, 0);
-  for (int32_t c : C) {
-    for (int32_t e = A.ptr[c]; e < A.ptr[c + 1]; ++e) {
-      const int32_t j = A.col[e];
-      if (!isF[j]) continue;
-      L.r_col.push_back(j);
-      L.r_val.push_back(-A.val[e] / A.at(j, j));
-    }
-    L.r_ptr.push_back((int32_t)L.r_col.size());
-  }
-  L.p_ptr.assign(1, 0);
-  for (int32_t f : F) {
-    const double d = A.at(f, f);
-    for (int32_t e = A.ptr[f]; e < A.ptr[f + 1]; ++e) {
-      const int32_t j = A.col[e];
-      if (j == f) continue;
-      L.p_col.push_back(cidx[j]);
-      L.p_val.push_back(-A.val[e] / d);
-    }
-    L.p_ptr.push_back((int32_t)L.p_col.size());
-  }
-  return true;
-}
-
-template <typename T>
-static void upload(DevBuf<T> &d, const std::vector<T> &h, hipStream_t s) {
+template <typename T, typename Al>
+static void upload(DevBuf<T> &d, const std::vector<T, Al> &h, hipStream_t s) {
   d.alloc(h.size() + 1);
   if (!h.empty())
     CN_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
 }
 
-static std::vector<float> to_f32(const std::vector<double> &v) {
+template <typename Al>
+static std::vector<float> to_f32(const std::vector<double, Al> &v) {
   std::vector<float> r(v.size());
   for (size_t i = 0; i < v.size(); ++i) r[i] = (float)v[i];
   return r;
@@ -268,7 +105,7 @@ cn_status cn_hsc_build(cn_system S, const cn_hsc_opts *opts, cn_hsc *out, void *
     while (A.n() > o.coarse_max && (int32_t)H->lv.size() < o.max_levels) {
       HostLevel hl;
       HostCsr next;
-      if (!build_level(A, o, hl, next)) break;
+      if (!hsc_build_level(A, o, hl, next)) break;
       H->lv.emplace_back();
       cn_hsc_t::Level &L = H->lv.back();
       L.n = hl.n;

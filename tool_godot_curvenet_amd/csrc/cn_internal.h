@@ -165,6 +165,11 @@ struct cn_system_t {
   cn::DevBuf<double> ff_val, fc_val;
   cn::DevBuf<float> ff_val32, fc_val32;
   cn::DevBuf<double> diag;          // diagonal of A_ff
+  // assembly plan: for every entry of A_ff then A_fc, the face_m positions
+  // of its terms in summation order (bit 31: the term starts a new partial
+  // sum), and the CSR position of each free row's diagonal (-1: none)
+  cn::DevBuf<int32_t> asm_ptr, diag_pos;
+  cn::DevBuf<uint32_t> asm_terms;
   // A_fc transposed (rows = constrained columns), built on first use: the
   // free row of each entry and its index in fc_col / fc_val, rows sorted by
   // free row, so the values are read from fc_val and never go stale

@@ -1718,13 +1718,13 @@ __global__ void k_dinv(int32_t n, const double *__restrict__ d, double *__restri
 // ============================================================== profiling ==
 thread_local Prof g_prof;
 
-static void prof_reset() {
+void prof_reset() {
   g_prof.recs.clear();
   g_prof.used = 0;
   g_prof.result.clear();
 }
 
-static void prof_collect() {
+void prof_collect() {
   Prof &P = g_prof;
   if (!P.on) return;
   for (auto &r : P.recs) {

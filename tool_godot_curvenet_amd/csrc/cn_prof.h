@@ -1,4 +1,4 @@
-// cn_prof.h -- per-kernel timing of a solve with HIP events (enabled by
+// cn_prof.h -- per-kernel timing of a solve or an assembly with HIP events (enabled by
 // cn_profile_enable): a ProfScope records an event pair on the stream around
 // the launches it encloses; cn_profile_query sums them by name.  The events
 // are timing-only (read after the solve's final stream synchronization), so
@@ -24,6 +24,10 @@ struct Prof {
   std::map<std::string, std::pair<int, double>> result;
 };
 extern thread_local Prof g_prof;
+// clear the records (at the start of a timed entry point) / sum the records
+// by name into the results cn_profile_query reads (after its last launch)
+void prof_reset();
+void prof_collect();
 
 struct ProfScope {
   int idx = -1;
