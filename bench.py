@@ -36,7 +36,7 @@ TRAFFIC_KERNELS = {
     "spmm_row": "void cn::kw_spmm_dot<float, 4, 64>",
     "fwd": "void cn::kb_fwd<float, 4, 16, true>",
     "bwd": "void cn::kb_bwd<float, 4, 16>",
-    "hsc_down": "void cn::kw_hsc_down<float, 4, 64>",
+    "hsc_down": "void cn::kw_hsc_chain<float, 4, 64>",
     "hsc_up": "void cn::kw_hsc_up<float, 4, 64>",
     "pd_update": "void cn::kw_pd_update<float, 4, 64>",
 }
@@ -291,24 +291,26 @@ def main():
         kern["bwd"] = ("kb_bwd<float,4,16>", prof["bwd"][0] * nlev, prof["bwd"][1],
                        6 * (iters + refines) * nk4)
     else:
-        # HSC apply, per level l (n_F eliminated, n_C kept rows, k FP32 columns):
-        # down reads the kept rows and, once, the eliminated ones and writes the
-        # kept rows: 4k(2 n_C + n_F) plus R (8 B per entry, row pointers, row
-        # map); up reads and writes the eliminated rows and reads, once, the
-        # kept ones: 4k(2 n_F + n_C) plus P, the row map and 1/a_ff.
+        # HSC apply: the down sweep as down_sweep_bytes counts it (each row
+        # read and written once, distinct gathered rows once per launch); up,
+        # per level l (n_F eliminated, n_C kept rows, k FP32 columns), reads
+        # and writes the eliminated rows and reads, once, the kept ones:
+        # 4k(2 n_F + n_C) plus P, the row map and 1/a_ff.
         # kw_pd_update (d += alpha p, p = z + beta p) streams 5 n x k vectors.
         # levels [l0, L) and the coarse solve run in the fused tail launch
         l0 = lib.cn_hsc_tail_level(pre.handle, k, 1 if args.precision == "mixed" else 0)
+        levels = pre.levels()
         down_b = up_b = tail_b = 0
-        for li, lv in enumerate(pre.levels()):
+        for li, lv in enumerate(levels):
             nF, nC = len(lv["F"]), len(lv["C"])
             db = 4 * k * (2 * nC + nF) + 8 * len(lv["R"][1]) + 4 * (nC + 1) + 4 * nC
             ub = 4 * k * (2 * nF + nC) + 8 * len(lv["P"][1]) + 4 * (nF + 1) + 8 * nF
             if 0 <= l0 <= li:
                 tail_b += db + ub
             else:
-                down_b += db
                 up_b += ub
+        down_b = down_sweep_bytes(levels, n, k, l0 if l0 >= 0 else len(levels),
+                                  lap.get_tune()["chain_levels"] or 3)
         # Bytes are counted for the applies known to do work: one per active
         # iteration and the first of each solve (a restart adds an uncounted
         # one; launches during a pause return at once).  Time covers every
@@ -316,7 +318,7 @@ def main():
         nlev = max(pre.info.nlevels if l0 < 0 else l0, 1)
         napply = prof["hsc_down"][0]
         nwork = iters + args.steps
-        kern["hsc_down"] = ("kw_hsc_down<float,4,64>", napply * nlev, prof["hsc_down"][1],
+        kern["hsc_down"] = ("kw_hsc_chain<float,4,64>", napply * nlev, prof["hsc_down"][1],
                             nwork * down_b)
         kern["hsc_up"] = ("kw_hsc_up<float,4,64>", napply * nlev, prof["hsc_up"][1],
                           nwork * up_b)
@@ -484,6 +486,45 @@ def main():
         line["speedup_vs_plain_fp64"] = value / plain[args.precond]["solves_per_s"]
         line["speedup_vs_plain_fp64_jacobi"] = value / plain["jacobi"]["solves_per_s"]
     print(json.dumps(line))
+
+
+def down_sweep_bytes(levels, n0, k, t, G=3):
+    """Algorithmic bytes of one HSC down sweep up to level t in groups of G
+    levels (kw_hsc_chain): in group [a, e), launch m (a < m < e) takes the
+    rows eliminated at level m, the last launch the rows of level e, each
+    from its level-a value through the R entries of levels a .. m-1; each
+    row is read and written once (4k bytes each way), every distinct row
+    its chain gathers is read once per launch, each chain entry costs its
+    column and FP32 weight (8 B), plus the row list, segment pointer and
+    three segment bounds (20 B a row)."""
+    import numpy as np
+    L = len(levels)
+    if t <= 0 or L == 0:
+        return 0
+    g = np.arange(n0, dtype=np.int64)           # level-l row -> level-0 row
+    cls = np.full(n0, L, np.int64)              # level each row is eliminated at
+    ent_l, ent_v, ent_u = [], [], []
+    for l, lv in enumerate(levels):
+        F, C = lv["F"].astype(np.int64), lv["C"].astype(np.int64)
+        cls[g[F]] = l
+        rp, rc, _ = lv["R"]
+        rows = np.repeat(np.arange(len(C)), np.diff(rp))
+        ent_l.append(np.full(len(rc), l, np.int64))
+        ent_v.append(g[C][rows])
+        ent_u.append(g[rc.astype(np.int64)])
+        g = g[C]
+    el, ev, eu = (np.concatenate(a) for a in (ent_l, ent_v, ent_u))
+    total, a = 0, 0
+    while a < t:
+        e = min(a + G, t)
+        for m in range(a + 1, e + 1):
+            rows = (cls == m) if m < e else (cls >= e)
+            nrows = int(rows.sum())
+            sel = (el >= a) & (el < m) & rows[ev]
+            distinct = int(np.unique(eu[sel]).size)
+            total += 4 * k * (2 * nrows + distinct) + 8 * int(sel.sum()) + 20 * nrows
+        a = e
+    return total
 
 
 def tune_for(system, B, args, pre, precond=None):

@@ -208,12 +208,16 @@ cn_status cn_system_batch(int32_t nsys, const cn_system *systems, cn_system *out
  *              65536 blocks (0: 16384);
  *   tail_rows  HSC: levels of at most this many rows (and the coarse solve)
  *              run in one fused launch when the apply has enough columns,
- *              up to 16384 (0: 1536; -1: no fused tail). */
+ *              up to 16384 (0: 1536; -1: no fused tail);
+ *   chain_levels  HSC down sweep (k > 4): levels one group of launches
+ *              chains through, each row written once a group, 1 to 24
+ *              (0: 3; 1: one in-place pass per level). */
 typedef struct cn_tune {
   int32_t spmm_form;
   int32_t spmm_gens;
   int32_t level_cap;
   int32_t tail_rows;
+  int32_t chain_levels;
 } cn_tune;
 cn_status cn_system_set_tune(cn_system sys, const cn_tune *tune);
 cn_status cn_system_get_tune(cn_system sys, cn_tune *tune);

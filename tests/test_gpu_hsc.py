@@ -100,6 +100,49 @@ def test_hsc_apply_matches_oracle(gpu, k):
     assert np.linalg.norm(zg - zo) <= 1e-11 * np.linalg.norm(zo)
 
 
+# k > 4: the down sweep runs one launch per level of elimination, each row
+# summing its restriction chain through every level it is kept through; the
+# 40k-row icosphere has 12 levels, so chains of up to 12 segments, with
+# (k = 512) and without the fused tail
+@pytest.mark.parametrize("k", [8, 48, 97, 512])
+def test_hsc_apply_chains_match_oracle(gpu, k):
+    from tool_godot_curvenet_amd import HSC
+    lap = _lap("icosphere6")
+    A = lap.A_ff()
+    H = HSC(lap)
+    assert H.info.nlevels >= 8
+    ho = O.hsc_build(A.row_ptr, A.col_idx, A.values)
+    r = np.random.default_rng(k).normal(size=(lap.n_free, k))
+    zg = H.apply(r)
+    zo = O.hsc_apply(ho, r)
+    assert np.linalg.norm(zg - zo) <= 1e-11 * np.linalg.norm(zo)
+
+
+# groups of 1 (the in-place level pass), 2, 4 and all levels: one PCG step
+# from 0 follows the oracle hierarchy's apply, and the solve converges
+@pytest.mark.parametrize("g", [1, 2, 4, 24])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", 2e-5)])
+def test_chain_levels_direction_and_convergence(gpu, g, precision, tol):
+    from tool_godot_curvenet_amd import HSC
+    from tool_godot_curvenet_amd import meshgen as G
+    k = 64
+    lap = _lap("icosphere6")
+    A = lap.A_ff()
+    H = HSC(lap)
+    lap.set_tune({"chain_levels": g})
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=90 + g))
+    x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, max_iter=1, hsc=H)
+    assert info.iters == 1
+    X = x.cpu().numpy()
+    Z = O.hsc_apply(O.hsc_build(A.row_ptr, A.col_idx, A.values), B)
+    e = np.linalg.norm(X / np.linalg.norm(X, axis=0) - Z / np.linalg.norm(Z, axis=0), axis=0)
+    assert np.all(e <= tol), e.max()
+    x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, hsc=H)
+    assert info.status == 0
+    R = B - O.spmm(A.row_ptr, A.col_idx, A.values, x.cpu().numpy())
+    assert np.all(np.linalg.norm(R, axis=0) <= 1e-9 * np.linalg.norm(B, axis=0))
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp64_plain", 1e-11),
                                            ("mixed", 2e-5)])
 @pytest.mark.parametrize("k", [3, 100])

@@ -12,11 +12,14 @@ sides k and the precision:
   dependent loads: more waves in flight hide more latency, up to the point
   where they only add scheduling);
 * ``tail_rows``: the size below which HSC levels run in the fused tail
-  launch (one workgroup per column slice walks them with barriers).
+  launch (one workgroup per column slice walks them with barriers);
+* ``chain_levels``: the levels one group of down-sweep launches chains
+  through (each row is written once a group; longer groups write less and
+  gather rows of lower levels again).
 
 ``autotune`` times full solves of the caller's system and right-hand side
 for a short list of candidates (coordinate search: the SpMM form first, then
-the level cap, then the tail size), keeps the fastest, and records it in a
+the level cap, the tail size, the chain length), keeps the fastest, and records it in a
 JSON file keyed by device, rows, nonzeros, k, precision and preconditioner.
 ``lookup`` reads that file, ``apply`` sets a choice on a system.  The shapes
 change no result beyond rounding (the SpMM forms compute the same products;
@@ -36,8 +39,8 @@ import time
 
 from . import _lib as L
 
-FIELDS = ("spmm_form", "spmm_gens", "level_cap", "tail_rows")
-DEFAULT = {"spmm_form": 0, "spmm_gens": 0, "level_cap": 0, "tail_rows": 0}
+FIELDS = ("spmm_form", "spmm_gens", "level_cap", "tail_rows", "chain_levels")
+DEFAULT = {"spmm_form": 0, "spmm_gens": 0, "level_cap": 0, "tail_rows": 0, "chain_levels": 0}
 
 
 def cache_path() -> str:
@@ -167,6 +170,12 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
                 t = run({**cur, "tail_rows": rows})
                 if t < best:
                     best, cur = t, {**cur, "tail_rows": rows}
+        # 4: the down sweep's chain length (the wide form, k > 4)
+        if k > 4:
+            for g in (2, 4):
+                t = run({**cur, "chain_levels": g})
+                if t < best:
+                    best, cur = t, {**cur, "chain_levels": g}
     system.set_tune(cur)
     rec = {"tune": cur, "ms": 1e3 * best, "candidates": tried,
            "when": time.strftime("%Y-%m-%dT%H:%M:%S")}

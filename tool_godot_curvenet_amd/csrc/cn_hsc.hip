@@ -35,33 +35,6 @@ void dense_gemm_nn_rows(int m, int k, const double *M, const TB *X, int64_t ldx,
 
 // ---- wide form: LPR lanes per (row, LPR VEC column tile), 64 / LPR rows
 // per wavefront -----------------------------------------------------------
-// Z[Cg[c]] = src[Cg[c]] + sum_e rval[e] src[rcol[e]]   (src = r or Z)
-template <typename T, int VEC, int LPR>
-__global__ void __launch_bounds__(kBlock)
-kw_hsc_down(int32_t nC, const int32_t *__restrict__ Cg, const int32_t *__restrict__ rptr,
-            const int32_t *__restrict__ rcol, const T *__restrict__ rval, int64_t k,
-            const T *src, T *Z, const int32_t *__restrict__ ctl) {
-  if (ctl && ctl_idle(ctl)) return;
-  using P = Pack<LPR>;
-  const int64_t c0 = P::col0(VEC);
-  const bool valid = c0 < k;
-  // loads and their uses are unconditional (lanes past k read the last VEC
-  // columns), only the store is not: a load consumed under a branch makes
-  // the compiler wait for the previous row's store before the next row
-  const int64_t cl = valid ? c0 : k - VEC;
-  const RowIter ri = row_iter(0, nC, P::kSlots, P::slot());
-  for (int32_t row = ri.first; row < ri.end; row += ri.stride) {
-    const int32_t r = P::u(row);
-    const int64_t g = (int64_t)P::u(Cg[r]) * k;
-    T acc[VEC];
-    VecIO<T, VEC>::ld(src + g + cl, acc);
-    // a kept row has ~2-3 eliminated neighbours: chunks of 4 gathers
-    gather_wide<T, VEC, 4>(P::u(rptr[r]), P::u(rptr[r + 1]), rcol, rval, src, k, cl, true,
-                           acc);
-    if (valid) VecIO<T, VEC>::st(Z + g + c0, acc);
-  }
-}
-
 // Z[Fg[f]] = dinv[f] src[Fg[f]] + sum_e pval[e] Z[pcol[e]]   (src = r or Z)
 template <typename T, int VEC, int LPR>
 __global__ void __launch_bounds__(kBlock)
@@ -73,7 +46,10 @@ kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv
   using P = Pack<LPR>;
   const int64_t c0 = P::col0(VEC);
   const bool valid = c0 < k;
-  const int64_t cl = valid ? c0 : k - VEC;   // as in kw_hsc_down
+  // loads and their uses are unconditional (lanes past k read the last VEC
+  // columns), only the store is not: a load consumed under a branch makes
+  // the compiler wait for the previous row's store before the next row
+  const int64_t cl = valid ? c0 : k - VEC;
   const RowIter ri = row_iter(0, nF, P::kSlots, P::slot());
   for (int32_t row = ri.first; row < ri.end; row += ri.stride) {
     const int32_t f = P::u(row);
@@ -85,6 +61,50 @@ kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv
     for (int m = 0; m < VEC; ++m) acc[m] = d * acc[m];
     gather_wide<T, VEC>(P::u(pptr[f]), P::u(pptr[f + 1]), pcol, pval, Z, k, cl, true, acc);
     if (valid) VecIO<T, VEC>::st(Z + g + c0, acc);
+  }
+}
+
+// ---- the down sweep (wide form) -----------------------------------------
+// Level l's pass r_{l+1}[c] = r_l[c] + (R_l r_l[F_l])[c] rewrites every kept
+// row at every level.  Instead the levels go in groups [a, e): within a
+// group, a row eliminated at level m (a < m < e) is taken straight from its
+// value at level a to r_m[v] = r_a[v] + sum_{a<=l<m} (R_l r_l[F_l])[v], and
+// the rows of level e to r_e.  Each such chain gathers only rows eliminated
+// at levels a .. m-1, whose values are final when its launch runs, so a
+// group is one launch per level of elimination and writes each row once;
+// longer groups write less but gather rows of lower levels again in every
+// launch (on the 100k capsule union at k = 768, groups of 1 / 2 / 3 / 4 /
+// 17 levels have 2.17 / 1.53 / 1.44 / 1.48 / 2.05 GB of algorithmic traffic
+// a sweep: rows read and written, distinct gathered rows once a launch).  r_a is r at
+// a = 0 and Z otherwise; a group of one level is the in-place level pass.
+// The sum is the level passes' -- r_a[v], then the R entries level by level,
+// each in order -- with the weight-0 padding of the gathers only at the end
+// of the two source segments.
+template <typename T, int VEC, int LPR>
+__global__ void __launch_bounds__(kBlock)
+kw_hsc_chain(int32_t nrows, const int32_t *__restrict__ rows, int32_t a, int32_t m,
+             const int32_t *__restrict__ seg_ptr, const int32_t *__restrict__ seg,
+             const int32_t *__restrict__ ecol, const T *__restrict__ eval, int64_t k,
+             const T *srcA, T *Z, const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
+  using P = Pack<LPR>;
+  const int64_t c0 = P::col0(VEC);
+  const bool valid = c0 < k;
+  // loads and their uses are unconditional (lanes past k read the last VEC
+  // columns), only the store is not
+  const int64_t cl = valid ? c0 : k - VEC;
+  const RowIter ri = row_iter(0, nrows, P::kSlots, P::slot());
+  for (int32_t i = ri.first; i < ri.end; i += ri.stride) {
+    const int32_t v = P::u(rows[P::u(i)]);
+    const int32_t sp = P::u(seg_ptr[v]);
+    const int32_t e0 = P::u(seg[sp + a]), e1 = P::u(seg[sp + a + 1]), e2 = P::u(seg[sp + m]);
+    T acc[VEC];
+    VecIO<T, VEC>::ld(srcA + (int64_t)v * k + cl, acc);
+    // level a (a kept row has ~2-3 eliminated neighbours: chunks of 4),
+    // then levels a+1 .. m-1
+    gather_wide<T, VEC, 4>(e0, e1, ecol, eval, srcA, k, cl, true, acc);
+    gather_wide<T, VEC, 8>(e1, e2, ecol, eval, Z, k, cl, true, acc);
+    if (valid) VecIO<T, VEC>::st(Z + (int64_t)v * k + c0, acc);
   }
 }
 
@@ -245,26 +265,14 @@ template <> struct LvVals<float> {
   static const float *d(const cn_hsc_t::Level &L) { return L.dinv32.get(); }
 };
 
+// narrow form (k <= 4): level l's in-place pass over its kept rows
 template <typename T>
-static void level_down(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
-                       const int32_t *ctl, hipStream_t s,
-                       int32_t level_cap) {
+static void level_down_n(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
+                         const int32_t *ctl, hipStream_t s) {
   if (L.nC == 0) return;
-  const int v = hsc_vec<T>(k);
-  if (v == 0) {
 #define N_(KK) k_hsc_down_n<T, KK><<<grid_for(L.nC, 256), 256, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(), L.r_colg.get(), LvVals<T>::r(L), src, Z, ctl)
-    switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
+  switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
 #undef N_
-    return;
-  }
-  const int lpr = pack_lpr12((k + v - 1) / v);
-  const dim3 g = hsc_grid(L.nC, k, v, lpr, level_cap);
-#define D_(VV, LL) kw_hsc_down<T, VV, LL><<<g, kBlock, 0, s>>>(L.nC, L.Cg.get(), L.r_ptr.get(), L.r_colg.get(), LvVals<T>::r(L), k, src, Z, ctl)
-  if constexpr (sizeof(T) == 4) {
-    if (v == 4) { HSC_LPR(lpr, D_, 4); return; }
-  }
-  if (v >= 2) { HSC_LPR(lpr, D_, 2); } else { HSC_LPR(lpr, D_, 1); }
-#undef D_
 }
 
 template <typename T>
@@ -287,6 +295,52 @@ static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
   }
   if (v >= 2) { HSC_LPR(lpr, U_, 2); } else { HSC_LPR(lpr, U_, 1); }
 #undef U_
+}
+
+// the rows `rows` (level-0 numbers) from level a to the ends of their
+// chains at level m
+template <typename T>
+static void level_chain(const cn_hsc_t *H, const int32_t *rows, int32_t nrows, int32_t a,
+                        int32_t m, int64_t k, const T *srcA, T *Z, const int32_t *ctl,
+                        hipStream_t s, int32_t level_cap) {
+  if (nrows == 0) return;
+  const int v = hsc_vec<T>(k);
+  const int lpr = pack_lpr12((k + v - 1) / v);
+  const dim3 g = hsc_grid(nrows, k, v, lpr, level_cap);
+  const T *ev;
+  if constexpr (sizeof(T) == 4) ev = H->ch_val32.get(); else ev = H->ch_val.get();
+#define C_(VV, LL) kw_hsc_chain<T, VV, LL><<<g, kBlock, 0, s>>>(nrows, rows, a, m, H->ch_seg_ptr.get(), H->ch_seg.get(), H->ch_col.get(), ev, k, srcA, Z, ctl)
+  if constexpr (sizeof(T) == 4) {
+    if (v == 4) { HSC_LPR(lpr, C_, 4); return; }
+  }
+  if (v >= 2) { HSC_LPR(lpr, C_, 2); } else { HSC_LPR(lpr, C_, 1); }
+#undef C_
+}
+
+constexpr int32_t kChainLevels = 3;  // levels a group of down-sweep launches chains through
+
+// The down sweep up to level t (the coarse level, or the tail's first), in
+// groups of chain_levels levels.
+template <typename T>
+static void down_sweep(cn_hsc_t *H, int32_t t, int64_t k, const T *r, T *z, const int32_t *ctl,
+                       hipStream_t s, const HscShape &sh) {
+  if (hsc_vec<T>(k) == 0) {
+    for (int32_t l = 0; l < t; ++l) level_down_n<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s);
+    return;
+  }
+  if (t == 0) return;
+  if (!H->ch_ready) throw HipError{hipErrorNotReady, "hsc: restriction chains not built"};
+  const int32_t G = sh.chain_levels > 0 ? sh.chain_levels : kChainLevels;
+  for (int32_t a = 0; a < t;) {
+    const int32_t e = std::min(a + G, t);
+    const T *srcA = a == 0 ? r : z;
+    for (int32_t m = a + 1; m < e; ++m)
+      level_chain<T>(H, H->lv[m].Fg.get(), H->lv[m].nF, a, m, k, srcA, z, ctl, s,
+                     sh.level_cap);
+    level_chain<T>(H, H->lv[e - 1].Cg.get(), H->lv[e - 1].nC, a, e, k, srcA, z, ctl, s,
+                   sh.level_cap);
+    a = e;
+  }
 }
 
 // z = M^-1 r (n0 x k), r and z distinct.
@@ -475,8 +529,7 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
     const TailLevel<T> *desc = tail_desc<T>(H, s);
     {
       ProfScope ps("hsc_down", s);
-      for (int32_t l = 0; l < l0; ++l)
-        level_down<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s, sh.level_cap);
+      down_sweep<T>(H, l0, k, r, z, ctl, s, sh);
     }
     {
       ProfScope ps("hsc_tail", s);
@@ -499,8 +552,7 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
   }
   {
     ProfScope ps("hsc_down", s);
-    for (int32_t l = 0; l < L; ++l)
-      level_down<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s, sh.level_cap);
+    down_sweep<T>(H, L, k, r, z, ctl, s, sh);
   }
   if (H->nc > 0) {
     ProfScope ps("hsc_coarse", s);
@@ -565,11 +617,89 @@ unsigned char *hsc_scratch() {
 }
 
 // Everything an apply reads besides its workspace, built at set-up: the
-// fused tail's level descriptors (FP32 and FP64) and this device's scratch
-// address, so cn_hsc_apply_noalloc never allocates.
+// fused tail's level descriptors (FP32 and FP64), the restriction chains
+// and this device's scratch address, so cn_hsc_apply_noalloc never
+// allocates.
+// The restriction chains (cn_hsc_t::ch_*), from the levels' R on the device.
+static void build_chains(cn_hsc_t *H, hipStream_t s) {
+  const int32_t L = (int32_t)H->lv.size(), n0 = H->n0;
+  if (L == 0 || n0 == 0) return;
+  auto down = [&](const auto &d, size_t count, auto &h) {
+    h.resize(count);
+    if (count)
+      CN_HIP(hipMemcpyAsync(h.data(), d.get(), sizeof(h[0]) * count, hipMemcpyDeviceToHost, s));
+  };
+  std::vector<std::vector<int32_t>> Fg(L), Cg(L), rp(L), rc(L);
+  std::vector<std::vector<double>> rv(L);
+  for (int32_t l = 0; l < L; ++l) {
+    const cn_hsc_t::Level &Lv = H->lv[l];
+    down(Lv.Fg, (size_t)Lv.nF, Fg[l]);
+    down(Lv.Cg, (size_t)Lv.nC, Cg[l]);
+    down(Lv.r_ptr, (size_t)Lv.nC + 1, rp[l]);
+  }
+  CN_HIP(hipStreamSynchronize(s));
+  for (int32_t l = 0; l < L; ++l) {
+    down(H->lv[l].r_colg, (size_t)rp[l][H->lv[l].nC], rc[l]);
+    down(H->lv[l].r_val, (size_t)rp[l][H->lv[l].nC], rv[l]);
+  }
+  CN_HIP(hipStreamSynchronize(s));
+  // level of elimination (L: kept to the coarse level) and entry counts
+  std::vector<int32_t> cls((size_t)n0, L);
+  for (int32_t l = 0; l < L; ++l)
+    for (int32_t v : Fg[l]) cls[v] = l;
+  std::vector<int64_t> cnt((size_t)n0, 0);
+  for (int32_t l = 0; l < L; ++l)
+    for (int32_t c = 0; c < H->lv[l].nC; ++c) cnt[Cg[l][c]] += rp[l][c + 1] - rp[l][c];
+  // storage order: by level of elimination, then in that level's F order
+  // (the coarse rows in the last level's C order)
+  std::vector<int32_t> order;
+  order.reserve((size_t)n0);
+  for (int32_t l = 1; l < L; ++l) order.insert(order.end(), Fg[l].begin(), Fg[l].end());
+  order.insert(order.end(), Cg[L - 1].begin(), Cg[L - 1].end());
+  std::vector<int32_t> seg_ptr((size_t)n0, 0);
+  std::vector<int64_t> start((size_t)n0, 0);
+  int64_t nseg = 0, nent = 0;
+  for (int32_t v : order) {
+    seg_ptr[v] = (int32_t)nseg;
+    start[v] = nent;
+    nseg += cls[v] + 1;
+    nent += cnt[v];
+  }
+  if (nseg > INT32_MAX || nent > INT32_MAX)
+    throw HipError{hipErrorInvalidValue, "hsc chains exceed int32"};
+  std::vector<int32_t> seg((size_t)nseg + 1, 0), col((size_t)nent + 1, 0);
+  std::vector<double> val((size_t)nent + 1, 0.0);
+  std::vector<int64_t> cur = start;
+  for (int32_t l = 0; l < L; ++l)
+    for (int32_t c = 0; c < H->lv[l].nC; ++c) {
+      const int32_t v = Cg[l][c];
+      seg[(size_t)seg_ptr[v] + l] = (int32_t)cur[v];
+      for (int32_t e = rp[l][c]; e < rp[l][c + 1]; ++e) {
+        col[(size_t)cur[v]] = rc[l][e];
+        val[(size_t)cur[v]] = rv[l][e];
+        ++cur[v];
+      }
+    }
+  for (int32_t v : order) seg[(size_t)seg_ptr[v] + cls[v]] = (int32_t)cur[v];
+  std::vector<float> val32(val.size());
+  for (size_t i = 0; i < val.size(); ++i) val32[i] = (float)val[i];
+  auto up = [&](auto &d, const auto &h) {
+    d.alloc(h.size());
+    CN_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(h[0]) * h.size(), hipMemcpyHostToDevice, s));
+  };
+  up(H->ch_seg_ptr, seg_ptr);
+  up(H->ch_seg, seg);
+  up(H->ch_col, col);
+  up(H->ch_val, val);
+  up(H->ch_val32, val32);
+  CN_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
+  H->ch_ready = true;
+}
+
 void hsc_prepare(cn_hsc_t *H, hipStream_t s) {
   (void)tail_desc<float>(H, s);
   (void)tail_desc<double>(H, s);
+  build_chains(H, s);
   (void)hsc_scratch();
 }
 

@@ -89,6 +89,7 @@ int64_t exclusive_scan_i64(int64_t *d, int64_t n, hipStream_t s);
 struct HscShape {
   int32_t level_cap = 0;
   int32_t tail_rows = 0;
+  int32_t chain_levels = 0;
 };
 
 // ----------------------------------------------------------- topology --
@@ -313,6 +314,16 @@ struct cn_hsc_t {
   // level descriptors of the fused tail (the small levels and the coarse
   // solve in one launch), FP32 and FP64 forms, built on first use
   cn::DevBuf<unsigned char> tail_desc[2];
+  // restriction chains of the down sweep: every row eliminated at level m
+  // (or kept to the coarse level, m = L) holds its R entries of levels
+  // 0 .. m-1 end to end, by level -- columns as level-0 rows, FP64 and FP32
+  // weights.  ch_seg[ch_seg_ptr[v] + l] is the first entry of row v's
+  // level-l segment (l = 0 .. m; l = m: the end).  Rows are stored by level
+  // of elimination, then in that level's F order.
+  cn::DevBuf<int32_t> ch_seg_ptr, ch_seg, ch_col;
+  cn::DevBuf<double> ch_val;
+  cn::DevBuf<float> ch_val32;
+  bool ch_ready = false;
 };
 
 namespace cn {

@@ -2551,6 +2551,7 @@ static void use_shapes(cn_system_t *S, SysView<T> &V, int64_t k, hipStream_t s) 
   V.gens = S->tune.spmm_gens > 0 ? S->tune.spmm_gens : (int32_t)kSpmmGens;
   V.hsh.level_cap = S->tune.level_cap;
   V.hsh.tail_rows = S->tune.tail_rows;
+  V.hsh.chain_levels = S->tune.chain_levels;
   if (S->tune.spmm_form != 1) use_win<T>(S, V, k, s);
 }
 template <>
@@ -2866,7 +2867,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
           F ? F->serial : 0, G ? G->serial : 0, (uint64_t)V.rpr,
           (uint64_t)(uintptr_t)V.wpk, (uint64_t)(uintptr_t)V.ecol, (uint64_t)V.gens,
           (uint64_t)(uint32_t)V.hsh.level_cap, (uint64_t)(uint32_t)V.hsh.tail_rows,
-          (uint64_t)klf, (uint64_t)cur};
+          (uint64_t)(uint32_t)V.hsh.chain_levels, (uint64_t)klf, (uint64_t)cur};
       for (auto &e : W->graphs)
         if (e.first == key) return e.second;
       if (W->graphs.size() >= 4) {
@@ -3085,7 +3086,8 @@ extern "C" {
 cn_status cn_system_set_tune(cn_system S, const cn_tune *t) {
   if (!S || !t || t->spmm_form < 0 || t->spmm_form > 2 || t->spmm_gens < 0 ||
       t->spmm_gens > (int32_t)kSpmmGens || t->spmm_gens == 3 || t->level_cap < 0 ||
-      t->level_cap > 65536 || t->tail_rows < -1 || t->tail_rows > 16384) {
+      t->level_cap > 65536 || t->tail_rows < -1 || t->tail_rows > 16384 ||
+      t->chain_levels < 0 || t->chain_levels > 24) {
     set_error("cn_system_set_tune: bad arguments");
     return CN_BAD_ARG;
   }

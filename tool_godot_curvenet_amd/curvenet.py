@@ -209,11 +209,11 @@ class _System:
     def n_free(self) -> int:
         return self.info.n_free
 
-    TUNE_FIELDS = ("spmm_form", "spmm_gens", "level_cap", "tail_rows")
+    TUNE_FIELDS = ("spmm_form", "spmm_gens", "level_cap", "tail_rows", "chain_levels")
 
     def set_tune(self, tune: dict | None):
         """Launch shapes of the solver kernels (cn_system_set_tune): a dict of
-        spmm_form, spmm_gens, level_cap, tail_rows; missing keys (or None)
+        spmm_form, spmm_gens, level_cap, tail_rows, chain_levels; missing keys (or None)
         keep the built-in choice."""
         t = L.Tune()
         for f in self.TUNE_FIELDS:
