@@ -1,18 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark: solves/s at 1e-9 relative residual on the ~100k-vertex cut
 Laplacian (capsule union, 16 cut curves), 256 poses x 3 RHS per step,
-mixed-precision IC(0)-PCG with all 768 columns in lockstep (SpMM).
+mixed-precision PCG with all 768 columns in lockstep (SpMM), preconditioned
+by the hierarchical Schur preconditioner (or IC(0) with --precond ic0).
 
 One step = build the harmonic RHS of all 256 poses from their seeded
 constraint displacements (HarmonicSolve RHS, FP64) + solve them to
 ||b - Ax|| <= 1e-9 ||b|| per column (FP64 recomputed residual).  Inputs are
-resident in HBM before the timed region; the IC(0) factor is built once
+resident in HBM before the timed region; the preconditioner is built once
 per mesh (bind time) outside it.
 
 Prints one JSON line (the contract) with `roofline` for the dominant kernel
-(the FP32 SpMM of the inner loop, timed with HIP events around every launch
-inside the timed region) and `cpu_baseline` (the oracle's FP64 IC(0)-PCG on
-one pose, single thread, on this host).
+of the inner loop (the kernel with the most GPU time, timed with HIP events
+around every launch inside the timed region) and `cpu_baseline` (the
+oracle's FP64 IC(0)-PCG on this host's cores).
 """
 from __future__ import annotations
 
@@ -39,8 +40,16 @@ TRAFFIC_KERNELS = {
     "hsc_down": "void cn::kw_hsc_chain<float, 4, 64>",
     "hsc_up": "void cn::kw_hsc_up<float, 4, 64>",
     "pd_update": "void cn::kw_pd_update<float, 4, 64>",
+    "dot": "void cn::kw_dot<float, 4, 64>",
+    "hsc_tail": "void cn::k_hsc_tail<float, 4>",
 }
 FP64_MFMA_PEAK_TFLOPS = 78.6
+# HSC hierarchy per workload: (drop threshold, coarse rows at most, levels
+# at most).  With 768 columns the dense coarse product runs on the FP64
+# matrix cores at about 0.65 of peak, so a small threshold and a large coarse
+# block pay: on the capsule union theta 0.015 / 512 rows takes 24
+# iterations, 0.002 / 4096 rows 10.
+HSC_SHAPE = {"default": (0.015, 512, 24), "capsules100k": (0.002, 4096, 24)}
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 
@@ -99,7 +108,14 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--precond", default="hsc", choices=["hsc", "ic0"],
                     help="hierarchical Schur (default) or IC(0)")
-    ap.add_argument("--theta", type=float, default=0.015, help="HSC drop threshold")
+    ap.add_argument("--theta", type=float, default=None,
+                    help="HSC drop threshold (default: the workload's, see HSC_SHAPE)")
+    ap.add_argument("--coarse-max", type=int, default=None,
+                    help="HSC: stop eliminating at this many rows (dense coarse matrix)")
+    ap.add_argument("--max-levels", type=int, default=None, help="HSC: most elimination levels")
+    ap.add_argument("--inner-tol", type=float, default=None,
+                    help="mixed precision: residual replacement once a column's iterated "
+                         "||r|| has fallen by this factor (default: the autotuner's choice)")
     ap.add_argument("--ordering", default="block", choices=["block", "color", "natural"],
                     help="IC(0) ordering (block multicolour, point colour, natural levels)")
     ap.add_argument("--block-size", type=int, default=16)
@@ -126,6 +142,10 @@ def main():
                     help="no GPU: turn two rocprofv3 --pmc counter files (FETCH_SIZE, "
                          "WRITE_SIZE) of a --copy-mb run into the traffic file")
     args = ap.parse_args()
+    shape = HSC_SHAPE.get(args.workload, HSC_SHAPE["default"])
+    for f, v in zip(("theta", "coarse_max", "max_levels"), shape):
+        if getattr(args, f) is None:
+            setattr(args, f, v)
     if args.traffic_from:
         traffic_from(*args.traffic_from, copy_mb=args.copy_mb or 1024)
         return
@@ -172,7 +192,8 @@ def main():
     mesh = meshgen.config_mesh("capsules100k")
     lap = cn.CutLaplacian.from_mesh(mesh)
     if args.precond == "hsc":
-        pre = cn.HSC(lap, theta=args.theta)
+        pre = cn.HSC(lap, theta=args.theta, coarse_max=args.coarse_max,
+                     max_levels=args.max_levels)
         pre_desc = (f"HSC theta {args.theta} ({pre.info.nlevels} levels, coarse "
                     f"{pre.info.n_coarse})")
     else:
@@ -215,14 +236,15 @@ def main():
     _lib.check(lib.cn_harmonic_rhs(lap.handle, k, C.c_void_p(Xc.data_ptr()),
                                    C.c_void_p(B.data_ptr()), sp), "harmonic_rhs")
     tune = tune_for(lap, B, args, pre)
+    opts.inner_tol = args.inner_tol if args.inner_tol is not None else tune["inner_tol"]
     for _ in range(args.warmup):
         step(info)
     torch.cuda.synchronize()
 
     lib.cn_profile_enable(1)
     prof = {name: [0, 0.0] for name in ("spmm", "spmm_lds", "spmm_dir", "spmm_upd", "fwd",
-                                         "bwd", "resid", "hsc_down", "hsc_up", "hsc_coarse",
-                                         "hsc_tail")}
+                                         "bwd", "resid", "precond", "dot", "hsc_down",
+                                         "hsc_up", "hsc_coarse", "hsc_tail")}
     iters, refines, relmax = 0, 0, 0.0
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -335,6 +357,8 @@ def main():
         nbwd = prof["bwd"][0]
         kern["pd_update"] = ("kw_pd_update<float,4,64>", nbwd, prof["bwd"][1],
                              (nbwd if fused else nwork) * 5 * nk4)
+        # r.z after every apply that does work: reads R and Z
+        kern["dot"] = ("kw_dot<float,4,64>", prof["dot"][0], prof["dot"][1], nwork * 2 * nk4)
         nc = pre.info.n_coarse
         cms = prof["hsc_coarse"][1]
         ncoarse = prof["hsc_coarse"][0]
@@ -367,7 +391,9 @@ def main():
         Bp = B[:, :3].contiguous()
         Xp = torch.zeros_like(Bp)
         plain = {}
-        pres = [("hsc", pre if args.precond == "hsc" else cn.HSC(lap, theta=args.theta)),
+        pres = [("hsc", pre if args.precond == "hsc" else
+                 cn.HSC(lap, theta=args.theta, coarse_max=args.coarse_max,
+                        max_levels=args.max_levels)),
                 ("ic0", pre if args.precond == "ic0" else cn.IC0(lap, "block", block_size=16)),
                 ("jacobi", None)]
         for pname, pobj in pres:
@@ -441,7 +467,8 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
             tk = json.load(fh).get("kernels", {}).get(dom)
-        same = (args.precond == "hsc" and args.theta == 0.015) or \
+        same = (args.precond == "hsc" and
+                (args.theta, args.coarse_max, args.max_levels) == HSC_SHAPE[WORKLOAD]) or \
             (args.precond == "ic0" and args.ordering == "block" and args.block_size == 16)
         if tk and k == 768 and same:
             traffic = tk["bytes"]
@@ -536,12 +563,13 @@ def tune_for(system, B, args, pre, precond=None):
     r = autotune.autotune(system, B, precond=precond, precision=args.precision, tol=args.tol,
                           use_cache=False, **kw)
     built_in = next((c["ms"] for c in r["candidates"]
-                     if c["tune"] == {"spmm_form": 2, "spmm_gens": 0, "level_cap": 0,
-                                      "tail_rows": 0}), None)
-    return {"chosen": r["tune"], "solve_ms": r["ms"], "built_in_solve_ms": built_in,
-            "candidates": len(r["candidates"]),
+                     if c["tune"] == {**autotune.DEFAULT, "spmm_form": 2}
+                     and c["inner_tol"] == autotune.INNER_TOLS[0]), None)
+    return {"chosen": r["tune"], "inner_tol": r["inner_tol"], "solve_ms": r["ms"],
+            "built_in_solve_ms": built_in, "candidates": len(r["candidates"]),
             "how": "coordinate search over SpMM form / stripes, HSC level-pass grid cap, "
-                   "fused-tail size; full solves of this workload, best of 2"}
+                   "fused-tail size, down-sweep chain length, residual-replacement drop; "
+                   "full solves of this workload, best of 2"}
 
 
 def spmv_k1(lib, A, sp, reps=200):
@@ -633,8 +661,10 @@ def meshes64_bench(args):
           f"in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     laps = [cn.CutLaplacian.from_mesh(m) for m in meshes]
     batch = cn.BatchSystem(laps)
-    hb = cn.HSC(batch, theta=args.theta, max_levels=48)
-    hs = [cn.HSC(l, theta=args.theta) for l in laps]
+    hb = cn.HSC(batch, theta=args.theta, coarse_max=args.coarse_max,
+                max_levels=max(args.max_levels, 48))
+    hs = [cn.HSC(l, theta=args.theta, coarse_max=args.coarse_max,
+                 max_levels=args.max_levels) for l in laps]
     print(f"meshes64: systems and hierarchies built at {time.time() - t0:.1f} s",
           file=sys.stderr, flush=True)
     k = 3
@@ -704,6 +734,8 @@ def meshes64_bench(args):
     _lib.check(lib.cn_harmonic_rhs(batch.handle, k, C.c_void_p(Xcs.data_ptr()),
                                    C.c_void_p(Bb.data_ptr()), sp), "harmonic_rhs")
     tune = tune_for(batch, Bb, args, hb, precond="hsc")
+    opts.inner_tol = opts_direct.inner_tol = \
+        args.inner_tol if args.inner_tol is not None else tune["inner_tol"]
     ed, _ = timed(step_batch_direct)
     eb, itb = timed(step_batch)
     print(f"meshes64: batched {len(laps) * args.steps / eb:.1f} solves/s (direct launches "
@@ -774,7 +806,7 @@ def icosphere9_bench(args):
     print(f"icosphere9: {mesh.nv} vertices, {len(mesh.meta['loops'])} loops in "
           f"{time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     lap = cn.CutLaplacian.from_mesh(mesh)
-    hsc = cn.HSC(lap, theta=args.theta)
+    hsc = cn.HSC(lap, theta=args.theta, coarse_max=args.coarse_max, max_levels=args.max_levels)
     print(f"icosphere9: system and hierarchy at {time.time() - t0:.1f} s", file=sys.stderr,
           flush=True)
     P = 16
@@ -813,6 +845,7 @@ def icosphere9_bench(args):
     _lib.check(lib.cn_harmonic_rhs(lap.handle, k, C.c_void_p(Xc.data_ptr()),
                                    C.c_void_p(B.data_ptr()), sp), "harmonic_rhs")
     tune = tune_for(lap, B, args, hsc, precond="hsc")
+    opts.inner_tol = args.inner_tol if args.inner_tol is not None else tune["inner_tol"]
     free0, total = torch.cuda.mem_get_info()
     for _ in range(args.warmup):
         step()

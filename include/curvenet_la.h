@@ -587,7 +587,10 @@ typedef struct cn_pcg_opts {
   double tol_rel;         /* stop when ||b - A x||_2 <= tol_rel ||b||_2 per
                              column, recomputed in FP64                      */
   int32_t max_iter;       /* total inner iterations cap                       */
-  double inner_tol;       /* MIXED: relative reduction per refinement step    */
+  double inner_tol;       /* MIXED: a residual replacement (FP64 x update and
+                             recomputed b - A x) is due once a column's
+                             iterated ||r|| has fallen by this factor since
+                             the last one; in (0, 1), default 0.0316       */
   int32_t max_refine;     /* MIXED: refinement steps cap                      */
   int32_t check_every;    /* device-side stop flags are read every n iters    */
   int32_t warm_start;     /* 1: d_x holds x0 (cgWithGuess), 0: x0 = 0 (cg)    */

@@ -201,6 +201,8 @@ def test_autotune_choice_converges_and_caches(gpu, tmp_path):
     r = autotune.autotune(lap, B, precond="hsc", precision="mixed", hsc=h, path=path, reps=1)
     assert not r["cached"] and len(r["candidates"]) >= 4
     assert lap.get_tune() == r["tune"]
+    assert r["inner_tol"] in autotune.INNER_TOLS and lap.inner_tol == r["inner_tol"]
+    assert {c["inner_tol"] for c in r["candidates"]} == set(autotune.INNER_TOLS)
     x, info = lap.solve(B, precond="hsc", precision="mixed", tol=1e-9, hsc=h)
     assert info.status == 0
     A = lap.A_ff()
@@ -208,8 +210,53 @@ def test_autotune_choice_converges_and_caches(gpu, tmp_path):
     R = Bh - O.spmm(A.row_ptr, A.col_idx, A.values, x.cpu().numpy())
     assert np.all(np.linalg.norm(R, axis=0) <= 1e-9 * np.linalg.norm(Bh, axis=0))
     lap.set_tune(None)
+    lap.inner_tol = lap.INNER_TOL
     r2 = autotune.autotune(lap, B, precond="hsc", precision="mixed", hsc=h, path=path)
     assert r2["cached"] and r2["tune"] == r["tune"] and lap.get_tune() == r["tune"]
+    assert lap.inner_tol == r["inner_tol"]
+
+
+@pytest.mark.parametrize("inner_tol", [0.0316, 1e-2, 1e-3, 1e-4])
+def test_inner_tol_converges(gpu, inner_tol):
+    """Every residual-replacement drop the autotuner may pick reaches 1e-9 in
+    the FP64 residual recomputed on the CPU; a larger drop takes fewer
+    replacements."""
+    import torch
+    from tool_godot_curvenet_amd import HSC, CutLaplacian, meshgen
+    lap = CutLaplacian.from_mesh(meshgen.config_mesh("icosphere6"))
+    B = torch.as_tensor(lap.harmonic_rhs(meshgen.seeded_boundary(lap.n_con, 48, seed=4)),
+                        device="cuda")
+    h = HSC(lap)
+    x, info = lap.solve(B, precond="hsc", precision="mixed", tol=1e-9, hsc=h,
+                        inner_tol=inner_tol)
+    x0, info0 = lap.solve(B, precond="hsc", precision="mixed", tol=1e-9, hsc=h,
+                          inner_tol=0.0316)
+    assert info.status == 0
+    A = lap.A_ff()
+    Bh = B.cpu().numpy()
+    R = Bh - O.spmm(A.row_ptr, A.col_idx, A.values, x.cpu().numpy())
+    assert np.all(np.linalg.norm(R, axis=0) <= 1e-9 * np.linalg.norm(Bh, axis=0))
+    assert info.refinements <= info0.refinements
+
+
+def test_inner_tol_out_of_range_refused(gpu):
+    import ctypes
+    import torch
+    from tool_godot_curvenet_amd import CutLaplacian, _lib, meshgen
+    lap = CutLaplacian.from_mesh(meshgen.config_mesh("cylinder64"))
+    lib = _lib.load()
+    B = torch.ones((lap.n_free, 3), dtype=torch.float64, device="cuda")
+    X = torch.zeros_like(B)
+    for bad in (0.0, 1.0, -0.5):
+        o = _lib.PcgOpts()
+        lib.cn_pcg_default_opts(ctypes.byref(o))
+        o.precond = _lib.CN_PRECOND_JACOBI
+        o.inner_tol = bad
+        info = _lib.SolveInfo()
+        st = lib.cn_pcg_solve(lap.handle, ctypes.c_void_p(0), ctypes.byref(o), 3,
+                              ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(X.data_ptr()),
+                              ctypes.byref(info), ctypes.c_void_p(0))
+        assert st == _lib.CN_BAD_ARG
 
 
 @pytest.mark.parametrize("precond,precision,tol", [("hsc", "fp64", 1e-12), ("hsc", "mixed", 1e-5),

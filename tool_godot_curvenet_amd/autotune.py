@@ -17,9 +17,16 @@ sides k and the precision:
   through (each row is written once a group; longer groups write less and
   gather rows of lower levels again).
 
+With mixed precision it also picks the solve's ``inner_tol``, the factor by
+which a column's iterated residual falls between two residual replacements
+(FP64 x update and recomputed b - A x): a larger drop saves replacements, a
+too large one lets the FP32 recurrence drift from the true residual and
+costs iterations.  The choice is kept on the system (``system.inner_tol``),
+which its solves use unless the caller passes another.
+
 ``autotune`` times full solves of the caller's system and right-hand side
 for a short list of candidates (coordinate search: the SpMM form first, then
-the level cap, the tail size, the chain length), keeps the fastest, and records it in a
+the level cap, the tail size, the chain length, inner_tol), keeps the fastest, and records it in a
 JSON file keyed by device, rows, nonzeros, k, precision and preconditioner.
 ``lookup`` reads that file, ``apply`` sets a choice on a system.  The shapes
 change no result beyond rounding (the SpMM forms compute the same products;
@@ -41,6 +48,7 @@ from . import _lib as L
 
 FIELDS = ("spmm_form", "spmm_gens", "level_cap", "tail_rows", "chain_levels")
 DEFAULT = {"spmm_form": 0, "spmm_gens": 0, "level_cap": 0, "tail_rows": 0, "chain_levels": 0}
+INNER_TOLS = (0.0316, 0.01, 1e-3, 1e-4)   # the first is the built-in one
 
 
 def cache_path() -> str:
@@ -115,9 +123,10 @@ def _tail_applies(k: int, precision: str) -> bool:
 def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: float = 1e-9,
              reps: int = 2, path: str | None = None, use_cache: bool = True, **pre) -> dict:
     """Time full solves of system on right-hand side B (n x k, FP64) for the
-    candidate launch shapes and keep the fastest (set on the system and
-    stored in the cache).  `pre` passes the preconditioner object (hsc=,
-    ic0= or sgs=).  Returns {"key", "tune", "ms", "cached", "candidates"}."""
+    candidate launch shapes (and, in mixed precision, residual-replacement
+    drops) and keep the fastest (set on the system and stored in the
+    cache).  `pre` passes the preconditioner object (hsc=, ic0= or sgs=).
+    Returns {"key", "tune", "inner_tol", "ms", "cached", "candidates"}."""
     import torch
     path = path or cache_path()
     k = int(B.shape[1]) if len(B.shape) == 2 else 1
@@ -127,24 +136,26 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
         if hit is not None:
             system.set_tune(hit)
             rec = _load(path)[key]
-            return {"key": key, "tune": hit, "ms": rec.get("ms"), "cached": True,
-                    "candidates": rec.get("candidates", [])}
+            system.inner_tol = float(rec.get("inner_tol", INNER_TOLS[0]))
+            return {"key": key, "tune": hit, "inner_tol": system.inner_tol, "ms": rec.get("ms"),
+                    "cached": True, "candidates": rec.get("candidates", [])}
     tried = []
 
-    def run(tune: dict) -> float:
+    def run(tune: dict, inner_tol: float = INNER_TOLS[0]) -> float:
         system.set_tune(tune)
-        system.solve(B, precond=precond, precision=precision, tol=tol, **pre)  # warm-up
+        kw = dict(precond=precond, precision=precision, tol=tol, inner_tol=inner_tol, **pre)
+        system.solve(B, **kw)  # warm-up
         best = float("inf")
         for _ in range(reps):
             torch.cuda.synchronize()
             t = time.perf_counter()
-            _, info = system.solve(B, precond=precond, precision=precision, tol=tol, **pre)
+            _, info = system.solve(B, **kw)
             torch.cuda.synchronize()
             el = time.perf_counter() - t
             if info.status != L.CN_OK:
                 return float("inf")     # a shape that does not converge is not a candidate
             best = min(best, el)
-        tried.append({"tune": dict(tune), "ms": 1e3 * best})
+        tried.append({"tune": dict(tune), "inner_tol": inner_tol, "ms": 1e3 * best})
         return best
 
     cur = dict(DEFAULT)
@@ -176,8 +187,17 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
                 t = run({**cur, "chain_levels": g})
                 if t < best:
                     best, cur = t, {**cur, "chain_levels": g}
+    # 5: the residual-replacement drop (mixed precision)
+    itol = INNER_TOLS[0]
+    if precision == "mixed":
+        for it in INNER_TOLS[1:]:
+            t = run(cur, it)
+            if t < best:
+                best, itol = t, it
     system.set_tune(cur)
-    rec = {"tune": cur, "ms": 1e3 * best, "candidates": tried,
+    system.inner_tol = itol
+    rec = {"tune": cur, "inner_tol": itol, "ms": 1e3 * best, "candidates": tried,
            "when": time.strftime("%Y-%m-%dT%H:%M:%S")}
     _store(path, key, rec)
-    return {"key": key, "tune": cur, "ms": 1e3 * best, "cached": False, "candidates": tried}
+    return {"key": key, "tune": cur, "inner_tol": itol, "ms": 1e3 * best, "cached": False,
+            "candidates": tried}

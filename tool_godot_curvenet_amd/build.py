@@ -7,6 +7,8 @@ the C ABI of ``include/curvenet_la.h``.
 Flags:
   * ``-Xclang -target-feature -Xclang -packed-fp32-ops`` keeps the compiler
     from emitting packed FP32 instructions;
+  * the dense FP64 kernels are built with ``-mllvm -amdgpu-mfma-vgpr-form``
+    (MFMA accumulators in VGPRs);
   * files whose FP64 results must match the spec bit for bit (topology,
     assembly, IC(0), SpMV) are built with ``-ffp-contract=off``; the solver
     file keeps contraction on (its FP32 arithmetic uses explicit fma()).
@@ -28,6 +30,9 @@ LIB = HERE / "libcurvenet_la.so"
 OBJ = HERE / "_build"
 
 EXACT = {"cn_topology.hip", "cn_assembly.hip", "cn_ic0.hip", "cn_spmv.hip", "cn_hsc_build.hip", "cn_hsc_host.cpp", "cn_sgs.hip", "cn_deform.hip", "cn_ddm.hip"}
+# files whose MFMA accumulators stay in VGPRs (with the AGPR form the
+# compiler copies loop-carried accumulators between the two files each trip)
+MFMA_VGPR = {"cn_dense.hip"}
 
 
 def _hipcc() -> str:
@@ -50,6 +55,8 @@ def _flags(name: str) -> list[str]:
     ]
     if name in EXACT:
         f.append("-ffp-contract=off")
+    if name in MFMA_VGPR:
+        f += ["-mllvm", "-amdgpu-mfma-vgpr-form"]
     return f
 
 

@@ -38,20 +38,13 @@ __device__ __forceinline__ double to_d(T v) { return (double)v; }
 // op(B)(t, j) = BT ? B[j * ldb + t] : B[t * ldb + j]
 // LOWER: only tiles on or below the diagonal, only entries j <= i written.
 // Block 256 threads = 4 waves; wave w owns the 16x16 subtile (w / 2, w % 2).
-// C = alpha A B + beta C.  With part != nullptr (split K), block z sums the
-// k-range [z kchunk, (z + 1) kchunk) and writes alpha times its sum to
-// part[z][i][j] instead of C.
 template <bool AT, bool BT, bool LOWER, typename TB, typename TC>
 __global__ void __launch_bounds__(256)
 k_gemm_f64(int m, int n, int kk, double alpha, const double *__restrict__ A, int64_t lda,
            const TB *__restrict__ B, int64_t ldb, double beta, TC *__restrict__ C,
-           int64_t ldc, const int32_t *__restrict__ crow = nullptr,
-           const int32_t *__restrict__ ctl = nullptr, double *__restrict__ part = nullptr,
-           int kchunk = 0) {
+           int64_t ldc) {
   if (LOWER && blockIdx.x > blockIdx.y) return;
-  if (ctl && ctl_idle(ctl)) return;
-  const int kbeg = part ? (int)blockIdx.z * kchunk : 0;
-  const int kend = part ? min(kk, kbeg + kchunk) : kk;
+  const int kbeg = 0, kend = kk;
   __shared__ double As[kGT][kGK + 1];
   __shared__ double Bs[kGK][kGT + 1];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
@@ -84,12 +77,8 @@ k_gemm_f64(int m, int n, int kk, double alpha, const double *__restrict__ A, int
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int i = bi + wi + (l >> 4) + 4 * v, j = bj + wj + (l & 15);
-    if (part) {
-      if (i < m && j < n) part[((int64_t)blockIdx.z * m + i) * n + j] = alpha * acc[v];
-      continue;
-    }
     if (i < m && j < n && (!LOWER || j <= i)) {
-      TC *cp = C + (int64_t)(crow ? crow[i] : i) * ldc + j;
+      TC *cp = C + (int64_t)i * ldc + j;
       const double old = beta == 0.0 ? 0.0 : to_d(*cp);
       *cp = (TC)(beta * old + alpha * acc[v]);
     }
@@ -245,74 +234,148 @@ void dense_cholesky_solve(int n, int k, const double *L, int64_t lda, double *B,
   CN_HIP(hipGetLastError());
 }
 
-// Z (m x k, TC) = M (m x m, FP64) X (m x k, TB) on the matrix cores.
-template <typename TB, typename TC>
-void dense_gemm_nn(int m, int k, const double *M, const TB *X, int64_t ldx, TC *Z, int64_t ldz,
-                   hipStream_t s) {
-  k_gemm_f64<false, false, false, TB, TC><<<gemm_grid(m, k), 256, 0, s>>>(
-      m, k, m, 1.0, M, m, X, ldx, 0.0, Z, ldz);
-}
-template void dense_gemm_nn<double, double>(int, int, const double *, const double *, int64_t,
-                                            double *, int64_t, hipStream_t);
-template void dense_gemm_nn<float, float>(int, int, const double *, const float *, int64_t,
-                                          float *, int64_t, hipStream_t);
+// ---- the coarse product of the hierarchical Schur apply -----------------
+// Z[rows[i]] = sum_j M[i][j] S[rows[j]] over the m coarse rows and k
+// columns: M is the dense FP64 coarse inverse (m x m, row-major), S the
+// apply's n x k block (FP32 or FP64), whose coarse rows are read where they
+// lie (no gathered copy).  A block of 256 threads (4 waves) owns a 64 x 64
+// tile of Z and one of sp slices of the j range; each wave owns a 32 x 32
+// quarter of the tile as 2 x 2 accumulators of v_mfma_f64_16x16x4f64.  The
+// j range goes in chunks of 16 through LDS.  The loads of chunk c + 1 are
+// issued right after chunk c is in LDS and before its MFMAs (a scheduling
+// barrier keeps the compiler from sinking them below the MFMAs, which
+// would leave their latency exposed), and they are unconditional: indices
+// are clamped into range, so no load sits under a branch that would make
+// the compiler wait for it.  Rows past m and columns past k compute values
+// nobody stores; for j past the slice the M side is stored as 0 and the S
+// side is a real (finite) row, so those terms add exactly 0.  The slice's
+// coarse row numbers are staged in LDS once, so forming a chunk's S
+// addresses waits on LDS, never on the chunk loads in flight.  Built with
+// the VGPR form of the MFMA (accumulators in VGPRs): with the AGPR form the
+// compiler copied the accumulators out of and back into AGPRs every chunk.
+// Every slice writes its FP64 partial sums; k_coarse_sum adds the slices in
+// slice order and stores the rows (blocks storing rows of z that other
+// blocks still read would race).  At m = 2,043, k = 768 this runs at about
+// 0.55 of the FP64 matrix peak (2 slices, 150 us with the sum).
+constexpr int kCM = 64, kCN = 64, kCK = 16;
+constexpr int kCMaxChunk = 2048;  // longest slice (LDS row-number stage)
 
-// Same product written to rows zrows[i] of Z.
-// Split of the K range for an m x m by m x k product: a grid of a few
-// hundred 32 x 32 tiles walks K one 16-wide step at a time, each step a
-// global load, an LDS round and a barrier; with few tiles (moderate k) the
-// K walk is split over up to 16 blocks a tile and the partial sums are
-// added in order by k_gemm_reduce.
-static int gemm_splits(int m, int k) {
-  const int tiles = ((m + kGT - 1) / kGT) * ((k + kGT - 1) / kGT);
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_coarse_mm(int m, int k, int kchunk, const double *__restrict__ M,
+            const int32_t *__restrict__ rows, const T *__restrict__ S,
+            double *__restrict__ part, const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
+  constexpr int AE = kCM * kCK / 256;  // M entries a thread stages per chunk (4)
+  constexpr int BE = kCK * kCN / 256;  // S values a thread stages per chunk (4)
+  __shared__ double As[kCM][kCK + 1];
+  __shared__ T Bs[kCK][kCN + 1];
+  __shared__ int32_t Rs[kCMaxChunk + kCK];
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int bi = blockIdx.y * kCM, bj = blockIdx.x * kCN;
+  const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
+  const int jb = blockIdx.z * kchunk, je = min(m, jb + kchunk);
+  const int ar = tid / (kCK / AE), ac = (tid % (kCK / AE)) * AE;
+  const int br = tid / (kCN / BE), bc = (tid % (kCN / BE)) * BE;
+  const double *Mrow = M + (int64_t)min(bi + ar, m - 1) * m;
+  int colc[BE];
+#pragma unroll
+  for (int u = 0; u < BE; ++u) colc[u] = min(bj + bc + u, k - 1);
+  for (int q = tid; q < kchunk + kCK; q += 256) Rs[q] = rows[min(jb + q, m - 1)];
+  __syncthreads();
+  double pa[AE];
+  T pb[BE];
+  auto fetch = [&](int j0) {
+#pragma unroll
+    for (int u = 0; u < AE; ++u) pa[u] = Mrow[min(j0 + ac + u, m - 1)];
+    const T *Srow = S + (int64_t)Rs[j0 - jb + br] * k;
+#pragma unroll
+    for (int u = 0; u < BE; ++u) pb[u] = Srow[colc[u]];
+  };
+  v4d acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+  fetch(jb);
+  for (int j0 = jb; j0 < je; j0 += kCK) {
+    __syncthreads();  // every wave is done with the previous chunk
+#pragma unroll
+    for (int u = 0; u < AE; ++u) As[ar][ac + u] = j0 + ac + u < je ? pa[u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < BE; ++u) Bs[br][bc + u] = pb[u];
+    __syncthreads();
+    fetch(j0 + kCK);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int t = 0; t < kCK; t += 4) {
+      const double a0 = As[wi + (l & 15)][t + (l >> 4)];
+      const double a1 = As[wi + 16 + (l & 15)][t + (l >> 4)];
+      const double b0 = (double)Bs[t + (l >> 4)][wj + (l & 15)];
+      const double b1 = (double)Bs[t + (l >> 4)][wj + 16 + (l & 15)];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+  }
+  double *P = part + (int64_t)blockIdx.z * m * k;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = bi + wi + 16 * a + (l >> 4) + 4 * v, j = bj + wj + 16 * b + (l & 15);
+        if (i < m && j < k) P[(int64_t)i * k + j] = acc[a][b][v];
+      }
+}
+
+template <typename T>
+__global__ void k_coarse_sum(int m, int k, int sp, const double *__restrict__ part,
+                             const int32_t *__restrict__ rows, T *__restrict__ Z,
+                             const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
+  const int64_t mk = (int64_t)m * k;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= mk) return;
+  const int i = (int)(id / k), j = (int)(id - (int64_t)i * k);
+  double s = part[id];
+  for (int z = 1; z < sp; ++z) s += part[(int64_t)z * mk + id];
+  Z[(int64_t)rows[i] * k + j] = (T)s;
+}
+
+// Slices of the j range: the fewest that give about 3 blocks a CU (more
+// slices cost more partial sums to add), at most 16, each at least two
+// chunks and at most kCMaxChunk long.  Measured at m = 2,043 and k = 768:
+// 1 / 2 / 4 / 8 slices 198 / 156 / 150 / 160 us with the sum.
+static int coarse_splits(int m, int k) {
+  const int tiles = ((m + kCM - 1) / kCM) * ((k + kCN - 1) / kCN);
   int sp = 1;
-  while (sp < 16 && tiles * sp < 1024 && m / (2 * sp) >= 4 * kGK) sp *= 2;
+  while (sp < 16 && tiles * sp < 768 && m / (2 * sp) >= 2 * kCK) sp *= 2;
+  while ((m + sp - 1) / sp > kCMaxChunk - kCK) ++sp;
   return sp;
 }
 
-size_t dense_gemm_rows_work(int m, int k) {
-  const int sp = gemm_splits(m, k);
-  return sp > 1 ? (size_t)sp * m * k : 0;
+size_t coarse_product_work(int m, int k) {
+  return (size_t)coarse_splits(m, k) * m * k;
 }
 
-template <typename TC>
-__global__ void k_gemm_reduce(int m, int n, int sp, const double *__restrict__ part,
-                              TC *__restrict__ C, int64_t ldc,
-                              const int32_t *__restrict__ crow,
-                              const int32_t *__restrict__ ctl) {
-  if (ctl && ctl_idle(ctl)) return;
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= (int64_t)m * n) return;
-  const int i = (int)(id / n), j = (int)(id - (int64_t)i * n);
-  double s = part[id];
-  for (int z = 1; z < sp; ++z) s += part[(int64_t)z * m * n + id];
-  C[(int64_t)(crow ? crow[i] : i) * ldc + j] = (TC)s;
+template <typename T>
+void coarse_product(int m, int k, const double *M, const int32_t *rows, const T *S, T *Z,
+                    const int32_t *ctl, double *part, hipStream_t s) {
+  if (m <= 0 || k <= 0) return;
+  const int sp = coarse_splits(m, k);
+  const int chunk = ((m + sp - 1) / sp + kCK - 1) / kCK * kCK;
+  const dim3 g((unsigned)((k + kCN - 1) / kCN), (unsigned)((m + kCM - 1) / kCM), (unsigned)sp);
+  k_coarse_mm<T><<<g, 256, 0, s>>>(m, k, chunk, M, rows, S, part, ctl);
+  k_coarse_sum<T><<<grid_for((int64_t)m * k, 256), 256, 0, s>>>(m, k, sp, part, rows, Z, ctl);
 }
-
-template <typename TB, typename TC>
-void dense_gemm_nn_rows(int m, int k, const double *M, const TB *X, int64_t ldx, TC *Z,
-                        int64_t ldz, const int32_t *zrows, const int32_t *ctl, double *work,
-                        hipStream_t s) {
-  const int sp = work ? gemm_splits(m, k) : 1;
-  if (sp <= 1) {
-    k_gemm_f64<false, false, false, TB, TC><<<gemm_grid(m, k), 256, 0, s>>>(
-        m, k, m, 1.0, M, m, X, ldx, 0.0, Z, ldz, zrows, ctl);
-    return;
-  }
-  const int chunk = ((m + sp - 1) / sp + kGK - 1) / kGK * kGK;
-  dim3 g = gemm_grid(m, k);
-  g.z = (unsigned)sp;
-  k_gemm_f64<false, false, false, TB, TC><<<g, 256, 0, s>>>(
-      m, k, m, 1.0, M, m, X, ldx, 0.0, Z, ldz, zrows, ctl, work, chunk);
-  k_gemm_reduce<TC><<<grid_for((int64_t)m * k, 256), 256, 0, s>>>(m, k, sp, work, Z, ldz,
-                                                                  zrows, ctl);
-}
-template void dense_gemm_nn_rows<double, double>(int, int, const double *, const double *,
-                                                 int64_t, double *, int64_t, const int32_t *,
-                                                 const int32_t *, double *, hipStream_t);
-template void dense_gemm_nn_rows<float, float>(int, int, const double *, const float *, int64_t,
-                                               float *, int64_t, const int32_t *, const int32_t *,
-                                               double *, hipStream_t);
+template void coarse_product<float>(int, int, const double *, const int32_t *, const float *,
+                                    float *, const int32_t *, double *, hipStream_t);
+template void coarse_product<double>(int, int, const double *, const int32_t *,
+                                     const double *, double *, const int32_t *, double *,
+                                     hipStream_t);
 
 }  // namespace cn
 

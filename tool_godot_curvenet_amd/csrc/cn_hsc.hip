@@ -27,11 +27,10 @@
 
 namespace cn {
 
-size_t dense_gemm_rows_work(int m, int k);
-template <typename TB, typename TC>
-void dense_gemm_nn_rows(int m, int k, const double *M, const TB *X, int64_t ldx, TC *Z,
-                        int64_t ldz, const int32_t *zrows, const int32_t *ctl, double *work,
-                        hipStream_t s);
+size_t coarse_product_work(int m, int k);
+template <typename T>
+void coarse_product(int m, int k, const double *M, const int32_t *rows, const T *S, T *Z,
+                    const int32_t *ctl, double *part, hipStream_t s);
 
 // ---- wide form: LPR lanes per (row, LPR VEC column tile), 64 / LPR rows
 // per wavefront -----------------------------------------------------------
@@ -498,13 +497,13 @@ static const TailLevel<T> *tail_desc(cn_hsc_t *H, hipStream_t s) {
   return reinterpret_cast<const TailLevel<T> *>(d.get());
 }
 
-// Workspace bytes of an apply of k columns: the gathered coarse rows, then
-// (256-aligned) the split-K partial sums of the coarse product.
+// Workspace bytes of an apply of k columns: the gathered coarse rows of the
+// narrow coarse product (k <= 4), or the per-slice FP64 partial sums of the
+// matrix-core product.
 template <typename T>
-static size_t apply_work_bytes(const cn_hsc_t *H, int64_t k, size_t *wbytes_out = nullptr) {
-  const size_t wbytes = (sizeof(T) * (size_t)H->nc * k + 255) / 256 * 256;
-  if (wbytes_out) *wbytes_out = wbytes;
-  return wbytes + sizeof(double) * dense_gemm_rows_work(H->nc, (int)k) + 256;
+static size_t apply_work_bytes(const cn_hsc_t *H, int64_t k) {
+  if (hsc_vec<T>(k) == 0) return (sizeof(T) * (size_t)H->nc * k + 255) / 256 * 256;
+  return sizeof(double) * coarse_product_work(H->nc, (int)k) + 256;
 }
 
 // ext_work: a caller's workspace of at least apply_work_bytes (the
@@ -513,8 +512,7 @@ template <typename T>
 void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const int32_t *ctl,
                const HscShape &sh, unsigned char *ext_work = nullptr) {
   const int32_t L = (int32_t)H->lv.size();
-  size_t wbytes = 0;
-  const size_t bytes = apply_work_bytes<T>(H, k, &wbytes);
+  const size_t bytes = apply_work_bytes<T>(H, k);
   unsigned char *work = ext_work;
   if (!work) {
     if (H->work_bytes < bytes) {
@@ -556,20 +554,19 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
   }
   if (H->nc > 0) {
     ProfScope ps("hsc_coarse", s);
-    const int64_t tot = (int64_t)H->nc * k;
-    k_gather_rows<T><<<grid_for(tot, 256), 256, 0, s>>>(H->nc, H->coarse_rows.get(), k,
-                                                      L == 0 ? r : z, W, ctl);
     if (k <= 4) {
       // a few columns: one wavefront per coarse row beats padded MFMA tiles
+      const int64_t tot = (int64_t)H->nc * k;
+      k_gather_rows<T><<<grid_for(tot, 256), 256, 0, s>>>(H->nc, H->coarse_rows.get(), k,
+                                                        L == 0 ? r : z, W, ctl);
 #define G_(KK)                                                                            \
   k_coarse_gemv<T, KK><<<(unsigned)((H->nc + kWaves - 1) / kWaves), kBlock, 0, s>>>(      \
       H->nc, H->coarse_inv.get(), W, H->coarse_rows.get(), z, ctl)
       switch (k) { case 1: G_(1); break; case 2: G_(2); break; case 3: G_(3); break; default: G_(4); }
 #undef G_
     } else {
-      dense_gemm_nn_rows<T, T>(H->nc, (int)k, H->coarse_inv.get(), W, k, z, k,
-                               H->coarse_rows.get(), ctl,
-                               reinterpret_cast<double *>(work + wbytes), s);
+      coarse_product<T>(H->nc, (int)k, H->coarse_inv.get(), H->coarse_rows.get(),
+                        L == 0 ? r : z, z, ctl, reinterpret_cast<double *>(work), s);
     }
   }
   {

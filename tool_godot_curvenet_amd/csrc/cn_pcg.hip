@@ -2165,7 +2165,7 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
                           const int32_t *ctl, hipStream_t s, bool no_rupd = false) {
   if (V.hsc || V.sgs || V.sm.of_row) {
     Grid G = grid_rows<T>(V.n, k, kCapSpmm);
-    ProfScope ps("fwd", s);
+    ProfScope ps("precond", s);
 #define N_(KK) kf_rupd<T, KK, FUSED><<<G.g, kFlat(KK), 0, s>>>(V.n, alpha, V.sm, Q, R, part, ctl)
 #define W_(VV, LL) kw_rupd<T, WideV<T, VV>::v, FUSED, LL><<<G.g, kBlock, 0, s>>>(V.n, alpha, V.sm, k, Q, R, part, ctl)
     if (!no_rupd) DISPATCH_K(k, T, G.lpr, N_, W_);
@@ -2181,7 +2181,10 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
                         256, 0, s>>>(total, k, V.dinv, R, Z, ctl);
     }
     // a batch takes its per-segment dots separately (launch_segdot)
-    if (!V.sm.of_row) launch_dot<T>(V.n, k, R, Z, part + stride, s);
+    if (!V.sm.of_row) {
+      ProfScope pd("dot", s);
+      launch_dot<T>(V.n, k, R, Z, part + stride, s);
+    }
     return G.rows;
   }
   if (!V.ic0) {
@@ -2685,7 +2688,9 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   // which is read until the end
   double *X = d_x != d_b ? d_x : W->xp.get(), *R64 = W->rp.get();
   const double theta = 0.5;  // iterated-residual target below tol
-  const double delta2 = sizeof(T) == 4 ? 1e-3 : 1e-8;
+  // reliable update: a residual replacement is due once a column's iterated
+  // ||r|| has dropped by inner_tol (MIXED) or 1e-4 (FP64) since the last one
+  const double delta2 = sizeof(T) == 4 ? o->inner_tol * o->inner_tol : 1e-8;
 
   // x0, true residual, ||b||
   if (o->warm_start) {
@@ -3110,7 +3115,7 @@ void cn_pcg_default_opts(cn_pcg_opts *o) {
   o->precision = CN_PREC_MIXED;
   o->tol_rel = 1e-9;
   o->max_iter = 20000;
-  o->inner_tol = 1e-5;
+  o->inner_tol = 0.0316;  // rr falls by 1e-3 between replacements
   o->max_refine = 30;
   o->check_every = 8;
   o->warm_start = 0;
@@ -3150,6 +3155,10 @@ static cn_status solve_impl(cn_system S, cn_ic0 F, cn_hsc H, cn_sgs G, const cn_
   }
   if (G && (G->sys != S || G->n != S->n_free)) {
     set_error("%s: colouring does not belong to the system", who);
+    return CN_BAD_ARG;
+  }
+  if (o->precision == CN_PREC_MIXED && !(o->inner_tol > 0.0 && o->inner_tol < 1.0)) {
+    set_error("%s: inner_tol must lie in (0, 1)", who);
     return CN_BAD_ARG;
   }
   if (S->nseg > 1 && (F || o->precision == CN_PREC_FP64_PLAIN)) {

@@ -186,8 +186,14 @@ def spmv(A: CsrMatrix, x, precision: str = "f64"):
 class _System:
     """Owner of a cn_system handle."""
 
+    # mixed precision: a residual replacement once the iterated ||r|| has
+    # fallen by this factor (cn_pcg_opts.inner_tol), unless the autotuner set
+    # another for this system or the caller passes one
+    INNER_TOL = 0.0316
+
     def __init__(self, handle: int):
         self._h = C.c_void_p(handle)
+        self.inner_tol = self.INNER_TOL
         info = L.SystemInfo()
         L.check(L.load().cn_system_get_info(self._h, C.byref(info)), "system_get_info")
         self.info = info
@@ -246,11 +252,15 @@ class _System:
 
     def solve(self, b, precond: str = "ic0", precision: str = "mixed", tol: float = 1e-9,
               x0=None, ic0: "IC0 | None" = None, max_iter: int = 20000,
-              inner_tol: float = 1e-5, check_every: int = 8, max_refine: int = 30,
+              inner_tol: float | None = None, check_every: int = 8, max_refine: int = 30,
               hsc: "HSC | None" = None, sgs: "SGS | None" = None, graph: bool = True):
         """Solve A_ff X = B (n x k or n); precond "ic0", "jacobi", "hsc" or
         "sgs".  graph: replay the inner iterations as a captured hipGraph
-        (the same results as direct launches).  Returns (x, SolveInfo)."""
+        (the same results as direct launches).  inner_tol: the mixed
+        precision residual-replacement factor (None: this system's, see
+        INNER_TOL and autotune).  Returns (x, SolveInfo)."""
+        if inner_tol is None:
+            inner_tol = self.inner_tol
         torch = _torch()
         lib = L.load()
         bt = _as_tensor(b, torch.float64)
@@ -282,7 +292,7 @@ class _System:
 
     def adjoint(self, g, x=None, xc=None, precond: str = "ic0", precision: str = "mixed",
                 tol: float = 1e-9, ic0: "IC0 | None" = None, weights: bool = False,
-                max_iter: int = 20000, inner_tol: float = 1e-5, check_every: int = 8,
+                max_iter: int = 20000, inner_tol: float | None = None, check_every: int = 8,
                 max_refine: int = 30, hsc: "HSC | None" = None):
         """Adjoint of the harmonic solve (cn_adjoint_solve, or
         cn_adjoint_solve_hsc with precond "hsc") for g = dL/dx_f
@@ -290,6 +300,8 @@ class _System:
         tensors: lambda = A_ff^-1 g, g_c = -A_fc^T lambda (None without
         constrained columns), g_w = dL/dw per stored entry of A_ff then A_fc
         (only with weights=True, which needs the forward x and x_c)."""
+        if inner_tol is None:
+            inner_tol = self.inner_tol
         torch = _torch()
         lib = L.load()
         gt = _as_tensor(g, torch.float64)
