@@ -141,12 +141,12 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
                     "cached": True, "candidates": rec.get("candidates", [])}
     tried = []
 
-    def run(tune: dict, inner_tol: float = INNER_TOLS[0]) -> float:
+    def run(tune: dict, inner_tol: float = INNER_TOLS[0], n: int = reps) -> float:
         system.set_tune(tune)
         kw = dict(precond=precond, precision=precision, tol=tol, inner_tol=inner_tol, **pre)
         system.solve(B, **kw)  # warm-up
         best = float("inf")
-        for _ in range(reps):
+        for _ in range(n):
             torch.cuda.synchronize()
             t = time.perf_counter()
             _, info = system.solve(B, **kw)
@@ -187,11 +187,14 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
                 t = run({**cur, "chain_levels": g})
                 if t < best:
                     best, cur = t, {**cur, "chain_levels": g}
-    # 5: the residual-replacement drop (mixed precision)
+    # 5: the residual-replacement drop (mixed precision); the candidates
+    # differ by a replacement or two (a few per cent), so each, the built-in
+    # one included, is timed again over more solves
     itol = INNER_TOLS[0]
     if precision == "mixed":
-        for it in INNER_TOLS[1:]:
-            t = run(cur, it)
+        best = float("inf")
+        for it in INNER_TOLS:
+            t = run(cur, it, max(reps, 4))
             if t < best:
                 best, itol = t, it
     system.set_tune(cur)

@@ -238,9 +238,9 @@ void dense_cholesky_solve(int n, int k, const double *L, int64_t lda, double *B,
 // Z[rows[i]] = sum_j M[i][j] S[rows[j]] over the m coarse rows and k
 // columns: M is the dense FP64 coarse inverse (m x m, row-major), S the
 // apply's n x k block (FP32 or FP64), whose coarse rows are read where they
-// lie (no gathered copy).  A block of 256 threads (4 waves) owns a 64 x 64
-// tile of Z and one of sp slices of the j range; each wave owns a 32 x 32
-// quarter of the tile as 2 x 2 accumulators of v_mfma_f64_16x16x4f64.  The
+// lie (no gathered copy).  A block of WM x WN waves owns a BM x BN tile of Z
+// and one of sp slices of the j range; each wave owns a (BM/WM) x (BN/WN)
+// part of the tile as 16 x 16 accumulators of v_mfma_f64_16x16x4f64.  The
 // j range goes in chunks of 16 through LDS.  The loads of chunk c + 1 are
 // issued right after chunk c is in LDS and before its MFMAs (a scheduling
 // barrier keeps the compiler from sinking them below the MFMAs, which
@@ -253,35 +253,49 @@ void dense_cholesky_solve(int n, int k, const double *L, int64_t lda, double *B,
 // addresses waits on LDS, never on the chunk loads in flight.  Built with
 // the VGPR form of the MFMA (accumulators in VGPRs): with the AGPR form the
 // compiler copied the accumulators out of and back into AGPRs every chunk.
+// Workgroups go to the XCDs round-robin, so consecutive tiles are dealt to
+// each XCD in contiguous runs: the column tiles of one row tile then share
+// that XCD's L2 for their M rows (at m = 3,834, k = 768 the fetch beyond L2
+// falls from 1.09 GB to 0.40 GB a launch).
 // Every slice writes its FP64 partial sums; k_coarse_sum adds the slices in
 // slice order and stores the rows (blocks storing rows of z that other
-// blocks still read would race).  At m = 2,043, k = 768 this runs at about
-// 0.55 of the FP64 matrix peak (2 slices, 150 us with the sum).
-constexpr int kCM = 64, kCN = 64, kCK = 16;
+// blocks still read would race).  Measured with the sum: 64 x 64 tiles
+// (4 waves) at m = 2,043, k = 768, 2 slices: 153 us (42 TFLOP/s); 128 x 128
+// tiles (8 waves) at m = 3,834, 4 slices: 452 us (50 TFLOP/s, 0.64 of the
+// FP64 matrix peak), against 505 us with 64 x 64 tiles.
+constexpr int kCK = 16;
 constexpr int kCMaxChunk = 2048;  // longest slice (LDS row-number stage)
 
-template <typename T>
-__global__ void __launch_bounds__(256)
+template <typename T, int BM, int BN, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN)
 k_coarse_mm(int m, int k, int kchunk, const double *__restrict__ M,
             const int32_t *__restrict__ rows, const T *__restrict__ S,
             double *__restrict__ part, const int32_t *__restrict__ ctl) {
   if (ctl && ctl_idle(ctl)) return;
-  constexpr int AE = kCM * kCK / 256;  // M entries a thread stages per chunk (4)
-  constexpr int BE = kCK * kCN / 256;  // S values a thread stages per chunk (4)
-  __shared__ double As[kCM][kCK + 1];
-  __shared__ T Bs[kCK][kCN + 1];
+  constexpr int NT = 64 * WM * WN;
+  constexpr int TM = BM / WM / 16, TN = BN / WN / 16;  // accumulators a wave
+  constexpr int AE = BM * kCK / NT;  // M entries a thread stages per chunk
+  constexpr int BE = kCK * BN / NT;  // S values a thread stages per chunk
+  static_assert(AE >= 1 && BE >= 1 && kCK % AE == 0 && BN % BE == 0, "tile shape");
+  __shared__ double As[BM][kCK + 1];
+  __shared__ T Bs[kCK][BN + 1];
   __shared__ int32_t Rs[kCMaxChunk + kCK];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
-  const int bi = blockIdx.y * kCM, bj = blockIdx.x * kCN;
-  const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
-  const int jb = blockIdx.z * kchunk, je = min(m, jb + kchunk);
+  // XCD-contiguous tile order (x fastest, then y, then the slice z)
+  const int nx = gridDim.x, ny = gridDim.y, nb = nx * ny * gridDim.z;
+  const int hw = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  const int lid = hw < nb - nb % 8 ? (hw % 8) * (nb / 8) + hw / 8 : hw;
+  const int tx = lid % nx, ty = (lid / nx) % ny, tz = lid / (nx * ny);
+  const int bi = ty * BM, bj = tx * BN;
+  const int wi = (wave / WN) * (BM / WM), wj = (wave % WN) * (BN / WN);
+  const int jb = tz * kchunk, je = min(m, jb + kchunk);
   const int ar = tid / (kCK / AE), ac = (tid % (kCK / AE)) * AE;
-  const int br = tid / (kCN / BE), bc = (tid % (kCN / BE)) * BE;
+  const int br = tid / (BN / BE), bc = (tid % (BN / BE)) * BE;
   const double *Mrow = M + (int64_t)min(bi + ar, m - 1) * m;
   int colc[BE];
 #pragma unroll
   for (int u = 0; u < BE; ++u) colc[u] = min(bj + bc + u, k - 1);
-  for (int q = tid; q < kchunk + kCK; q += 256) Rs[q] = rows[min(jb + q, m - 1)];
+  for (int q = tid; q < kchunk + kCK; q += NT) Rs[q] = rows[min(jb + q, m - 1)];
   __syncthreads();
   double pa[AE];
   T pb[BE];
@@ -292,11 +306,11 @@ k_coarse_mm(int m, int k, int kchunk, const double *__restrict__ M,
 #pragma unroll
     for (int u = 0; u < BE; ++u) pb[u] = Srow[colc[u]];
   };
-  v4d acc[2][2];
+  v4d acc[TM][TN];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < TM; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < TN; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
   fetch(jb);
   for (int j0 = jb; j0 < je; j0 += kCK) {
     __syncthreads();  // every wave is done with the previous chunk
@@ -309,25 +323,27 @@ k_coarse_mm(int m, int k, int kchunk, const double *__restrict__ M,
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int t = 0; t < kCK; t += 4) {
-      const double a0 = As[wi + (l & 15)][t + (l >> 4)];
-      const double a1 = As[wi + 16 + (l & 15)][t + (l >> 4)];
-      const double b0 = (double)Bs[t + (l >> 4)][wj + (l & 15)];
-      const double b1 = (double)Bs[t + (l >> 4)][wj + 16 + (l & 15)];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      double a[TM], b[TN];
+#pragma unroll
+      for (int x = 0; x < TM; ++x) a[x] = As[wi + 16 * x + (l & 15)][t + (l >> 4)];
+#pragma unroll
+      for (int y = 0; y < TN; ++y) b[y] = (double)Bs[t + (l >> 4)][wj + 16 * y + (l & 15)];
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TN; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
     }
   }
-  double *P = part + (int64_t)blockIdx.z * m * k;
+  double *P = part + (int64_t)tz * m * k;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int x = 0; x < TM; ++x)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int y = 0; y < TN; ++y)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = bi + wi + 16 * a + (l >> 4) + 4 * v, j = bj + wj + 16 * b + (l & 15);
-        if (i < m && j < k) P[(int64_t)i * k + j] = acc[a][b][v];
+        const int i = bi + wi + 16 * x + (l >> 4) + 4 * v, j = bj + wj + 16 * y + (l & 15);
+        if (i < m && j < k) P[(int64_t)i * k + j] = acc[x][y][v];
       }
 }
 
@@ -345,14 +361,21 @@ __global__ void k_coarse_sum(int m, int k, int sp, const double *__restrict__ pa
   Z[(int64_t)rows[i] * k + j] = (T)s;
 }
 
-// Slices of the j range: the fewest that give about 3 blocks a CU (more
-// slices cost more partial sums to add), at most 16, each at least two
-// chunks and at most kCMaxChunk long.  Measured at m = 2,043 and k = 768:
-// 1 / 2 / 4 / 8 slices 198 / 156 / 150 / 160 us with the sum.
+// Tile shape: 128 x 128 (8 waves) for a large product, 64 x 64 (4 waves)
+// otherwise.
+static bool coarse_big_tiles(int m, int k) { return m >= 3072 && k >= 512; }
+
+// Slices of the j range: the fewest that give about 3 blocks of 4 waves a
+// CU (more slices cost more partial sums to add), at most 16, each at least
+// two chunks and at most kCMaxChunk long.  Measured at m = 2,043 and
+// k = 768: 1 / 2 / 4 / 8 slices 198 / 156 / 150 / 160 us with the sum.
 static int coarse_splits(int m, int k) {
-  const int tiles = ((m + kCM - 1) / kCM) * ((k + kCN - 1) / kCN);
+  const bool big = coarse_big_tiles(m, k);
+  const int bm = big ? 128 : 64, bn = big ? 128 : 64;
+  const int want = big ? 384 : 768;
+  const int tiles = ((m + bm - 1) / bm) * ((k + bn - 1) / bn);
   int sp = 1;
-  while (sp < 16 && tiles * sp < 768 && m / (2 * sp) >= 2 * kCK) sp *= 2;
+  while (sp < 16 && tiles * sp < want && m / (2 * sp) >= 2 * kCK) sp *= 2;
   while ((m + sp - 1) / sp > kCMaxChunk - kCK) ++sp;
   return sp;
 }
@@ -367,8 +390,13 @@ void coarse_product(int m, int k, const double *M, const int32_t *rows, const T 
   if (m <= 0 || k <= 0) return;
   const int sp = coarse_splits(m, k);
   const int chunk = ((m + sp - 1) / sp + kCK - 1) / kCK * kCK;
-  const dim3 g((unsigned)((k + kCN - 1) / kCN), (unsigned)((m + kCM - 1) / kCM), (unsigned)sp);
-  k_coarse_mm<T><<<g, 256, 0, s>>>(m, k, chunk, M, rows, S, part, ctl);
+  if (coarse_big_tiles(m, k)) {
+    const dim3 g((unsigned)((k + 127) / 128), (unsigned)((m + 127) / 128), (unsigned)sp);
+    k_coarse_mm<T, 128, 128, 4, 2><<<g, 512, 0, s>>>(m, k, chunk, M, rows, S, part, ctl);
+  } else {
+    const dim3 g((unsigned)((k + 63) / 64), (unsigned)((m + 63) / 64), (unsigned)sp);
+    k_coarse_mm<T, 64, 64, 2, 2><<<g, 256, 0, s>>>(m, k, chunk, M, rows, S, part, ctl);
+  }
   k_coarse_sum<T><<<grid_for((int64_t)m * k, 256), 256, 0, s>>>(m, k, sp, part, rows, Z, ctl);
 }
 template void coarse_product<float>(int, int, const double *, const int32_t *, const float *,
