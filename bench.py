@@ -51,7 +51,11 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 # matrix cores at about 0.65 of peak, so a small threshold and a large coarse
 # block pay: on the capsule union theta 0.015 / 512 rows takes 24
 # iterations, 0.002 / 4096 rows 10.
-HSC_SHAPE = {"default": (0.015, 512, 24), "capsules100k": (0.002, 4096, 24)}
+# On the level-9 icosphere (k = 48) the 24-level cap binds first, so it gets
+# more levels: theta 0.015 / 512 / 24 takes 29 + 36 iterations (76 poses/s),
+# 0.002 / 8192 / 48 14 + 18 (117 poses/s).
+HSC_SHAPE = {"default": (0.015, 512, 24), "capsules100k": (0.002, 4096, 24),
+             "icosphere9": (0.002, 8192, 48)}
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 

@@ -184,6 +184,32 @@ def test_pcg_hsc_converges(gpu, name, precision, k):
     assert info.iters <= 40, info.iters
 
 
+# A hierarchy of no levels: the coarse block is the whole matrix, and the
+# preconditioner its exact inverse.  One FP32 iteration takes the iterated
+# residual down to rounding noise, unrelated to the true residual, so the
+# column has to restart from the true residual after the replacement rather
+# than continue its direction (which diverged to inf before).
+@pytest.mark.parametrize("k", [3, 8])
+@pytest.mark.parametrize("inner_tol", [0.0316, 1e-4])
+def test_pcg_exact_coarse_converges(gpu, k, inner_tol):
+    from tool_godot_curvenet_amd import HSC
+    from tool_godot_curvenet_amd import meshgen as G
+    lap = _lap("cylinder64")
+    A = lap.A_ff()
+    h = HSC(lap, coarse_max=4096)
+    assert h.info.nlevels == 0 and h.info.n_coarse == lap.n_free
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=11))
+    for precision in ("mixed", "fp64"):
+        x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, hsc=h,
+                            inner_tol=inner_tol)
+        assert info.status == 0, (precision, info.status)
+        X = x.cpu().numpy()
+        R = B - O.spmm(A.row_ptr, A.col_idx, A.values, X)
+        rr = np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
+        assert np.all(rr <= 1e-9), (precision, rr)
+        assert info.iters <= 4, (precision, info.iters)
+
+
 def test_hsc_beats_ic0_on_capsules(gpu):
     """The benchmark mesh: far fewer iterations than block IC(0)."""
     from tool_godot_curvenet_amd import HSC, IC0

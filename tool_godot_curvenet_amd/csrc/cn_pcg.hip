@@ -1401,9 +1401,10 @@ k_reduce_alpha(int32_t nrows, int64_t k, const double *__restrict__ part,
 }
 
 // After the forward pass: r.r and r.M^-1.r partials.
-//   mode 0 (iteration): stop test rr <= tol2, reliable-update trigger
-//          rr < delta2 * rref, beta = rz'/rz, gamma = 1 (inactive: beta 1,
-//          gamma 0 -> p, d frozen)
+//   mode 0 (iteration): stop test rr <= tol2 or rr < floor2 * rref (the
+//          FP32 recurrence at rounding noise), reliable-update trigger
+//          rr < delta2 * rref, pause below delta2^2 * rref, beta = rz'/rz,
+//          gamma = 1 (inactive: beta 1, gamma 0 -> p, d frozen)
 //   mode 1 (after a residual replacement / start): rz = rz', rref = rr;
 //          continuing columns keep p (beta 1, gamma 0), restarted ones take
 //          p = z (beta 0, gamma 1)
@@ -1411,7 +1412,7 @@ __global__ void __launch_bounds__(kBlock)
 k_reduce_beta(int32_t nrows, int32_t nrows_rz, int64_t k, const double *__restrict__ prr,
               const double *__restrict__ prz, double *__restrict__ st,
               int32_t *__restrict__ active, const int32_t *__restrict__ restart,
-              int32_t *__restrict__ ctl, int mode, double delta2) {
+              int32_t *__restrict__ ctl, int mode, double delta2, double floor2) {
   if (ctl_idle(ctl)) return;
   __shared__ double sh[4][64];
   const int64_t c = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
@@ -1424,7 +1425,8 @@ k_reduce_beta(int32_t nrows, int32_t nrows_rz, int64_t k, const double *__restri
         st[ST_RZ * k + c] = rz;
         st[ST_RREF * k + c] = rr;
         if (restart[c]) { beta = 0.0; gamma = 1.0; }
-      } else if (!(rr > st[ST_TOL2 * k + c]) || !isfinite(rr) || !isfinite(rz)) {
+      } else if (!(rr > st[ST_TOL2 * k + c]) || !(rr >= floor2 * st[ST_RREF * k + c]) ||
+                 !isfinite(rr) || !isfinite(rz)) {
         active[c] = 0;
         atomicSub(&ctl[CTL_ACTIVE], 1);
         atomicAdd(&ctl[CTL_JUST], 1);
@@ -1713,6 +1715,12 @@ __global__ void k_dinv(int32_t n, const double *__restrict__ d, double *__restri
   double v = d[i] == 0.0 ? 0.0 : 1.0 / d[i];
   d64[i] = v;
   d32[i] = (float)v;
+}
+
+static uint64_t bits_of(double v) {
+  uint64_t u;
+  std::memcpy(&u, &v, sizeof u);
+  return u;
 }
 
 // ============================================================== profiling ==
@@ -2691,6 +2699,15 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   // reliable update: a residual replacement is due once a column's iterated
   // ||r|| has dropped by inner_tol (MIXED) or 1e-4 (FP64) since the last one
   const double delta2 = sizeof(T) == 4 ? o->inner_tol * o->inner_tol : 1e-8;
+  // A column whose iterated r.r falls below floor2 times its value at the
+  // last replacement stops, like a converged one: in FP32 past 1e-12 (||r||
+  // by 1e-6) the recurrence carries rounding noise only, unrelated to the
+  // true residual, so continuing its direction after the replacement would
+  // not converge.  The next round restarts it (p = z of the true residual)
+  // unless its FP64 residual is already at the tolerance.  A near-exact
+  // preconditioner (a hierarchy whose coarse block is the whole matrix)
+  // gets there in one iteration.
+  const double floor2 = sizeof(T) == 4 ? 1e-12 : 0.0;
 
   // x0, true residual, ||b||
   if (o->warm_start) {
@@ -2726,7 +2743,9 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   CN_HIP(hipMemsetAsync(P, 0, sizeof(T) * total, s));
   CN_HIP(hipGetLastError());
 
-  std::vector<int32_t> act(kk, 0), rst(kk, 0);
+  // act: the columns the host started the round with; dact: those the
+  // device still iterated at the round's end (its stop tests clear them)
+  std::vector<int32_t> act(kk, 0), rst(kk, 0), dact(kk, 0);
   int32_t iters = 0, launched = 0, updates = 0;
   bool r_scaled = false;  // R already holds the scaled residual of this round
   cn_status status = CN_OK;
@@ -2736,6 +2755,9 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     // ---- read the true FP64 residual, decide which columns continue
     CN_HIP(hipMemcpyAsync(hs + ST_BB * kk, st + ST_BB * kk, sizeof(double) * 2 * kk,
                           hipMemcpyDeviceToHost, s));  // BB and RR adjacent
+    if (!first)
+      CN_HIP(hipMemcpyAsync(dact.data(), W->active.get(), sizeof(int32_t) * kk,
+                            hipMemcpyDeviceToHost, s));
     CN_HIP(hipStreamSynchronize(s));
     relmax = 0.0;
     int32_t nact = 0;
@@ -2745,9 +2767,10 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
       if (!std::isfinite(rel)) rel = INFINITY;
       relmax = std::max(relmax, rel);
       bool a = rel > o->tol_rel && bb > 0.0;
-      // a column the iterated residual had stopped, but whose true residual
-      // is not there yet, restarts (p = z)
-      rst[c] = (a && (first || !act[c])) ? 1 : 0;
+      // a column the device stopped (its iterated residual reached 0.5 tol
+      // or FP32 rounding noise), but whose true residual is not there yet,
+      // restarts (p = z)
+      rst[c] = (a && (first || !dact[c])) ? 1 : 0;
       act[c] = a ? 1 : 0;
       nact += a ? 1 : 0;
     }
@@ -2799,11 +2822,11 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
       if (batch) {
         launch_segdot<T, T>(S, W, k, R, R, R, Z, s);
         k_reduce_beta<<<red, kBlock, 0, s>>>(1, 1, kk, sg0, sg1, st, W->active.get(),
-                                             W->restart.get(), ctl, 1, delta2);
+                                             W->restart.get(), ctl, 1, delta2, floor2);
       } else {
         rd = reduce_rows(W, p1, rows, k, stride, 2, s);
         k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, rd.rows, k, rd.p, rd.p + rd.stride, st,
-                                             W->active.get(), W->restart.get(), ctl, 1, delta2);
+                                             W->active.get(), W->restart.get(), ctl, 1, delta2, floor2);
       }
       if (klf) pending = true;   // p = gamma z + beta p in the first KL_DIR
       else launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
@@ -2838,7 +2861,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
         Red b1 = reduce_rows(W, p1, r1, k, stride, 1, q, 0);
         Red b2 = reduce_rows(W, p1 + stride, r2, k, stride, 1, q, 1);
         k_reduce_beta<<<red, kBlock, 0, q>>>(b1.rows, b2.rows, k, b1.p, b2.p, st,
-                                             W->active.get(), W->restart.get(), ctl, 0, delta2);
+                                             W->active.get(), W->restart.get(), ctl, 0, delta2, floor2);
         cur ^= 1;
         return;
       }
@@ -2854,11 +2877,11 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
       if (batch) {
         launch_segdot<T, T>(S, W, k, R, R, R, Z, q);
         k_reduce_beta<<<red, kBlock, 0, q>>>(1, 1, kk, sg0, sg1, st, W->active.get(),
-                                             W->restart.get(), ctl, 0, delta2);
+                                             W->restart.get(), ctl, 0, delta2, floor2);
       } else {
         Red r2 = reduce_rows(W, p1, rw, k, stride, 2, q);
         k_reduce_beta<<<red, kBlock, 0, q>>>(r2.rows, r2.rows, k, r2.p, r2.p + r2.stride, st,
-                                             W->active.get(), W->restart.get(), ctl, 0, delta2);
+                                             W->active.get(), W->restart.get(), ctl, 0, delta2, floor2);
       }
       launch_bwd<T>(V, k, st, Z, P, D, ctl, q);
     };
@@ -2872,7 +2895,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
           F ? F->serial : 0, G ? G->serial : 0, (uint64_t)V.rpr,
           (uint64_t)(uintptr_t)V.wpk, (uint64_t)(uintptr_t)V.ecol, (uint64_t)V.gens,
           (uint64_t)(uint32_t)V.hsh.level_cap, (uint64_t)(uint32_t)V.hsh.tail_rows,
-          (uint64_t)(uint32_t)V.hsh.chain_levels, (uint64_t)klf, (uint64_t)cur};
+          (uint64_t)(uint32_t)V.hsh.chain_levels, (uint64_t)klf, (uint64_t)cur,
+          bits_of(delta2), bits_of(floor2)};
       for (auto &e : W->graphs)
         if (e.first == key) return e.second;
       if (W->graphs.size() >= 4) {
