@@ -54,8 +54,10 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 # On the level-9 icosphere (k = 48) the 24-level cap binds first, so it gets
 # more levels: theta 0.015 / 512 / 24 takes 29 + 36 iterations (76 poses/s),
 # 0.002 / 8192 / 48 14 + 18 (117 poses/s).
+# The 64-mesh batch (k = 3, its hierarchy allowed 48 levels): 0.015 / 512
+# takes 30 iterations (2,455 solves/s), 0.003 / 4096 17 (3,009 solves/s).
 HSC_SHAPE = {"default": (0.015, 512, 24), "capsules100k": (0.002, 4096, 24),
-             "icosphere9": (0.002, 8192, 48)}
+             "icosphere9": (0.002, 8192, 48), "meshes64": (0.003, 4096, 24)}
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 
@@ -865,7 +867,7 @@ def icosphere9_bench(args):
                                         C.c_void_p(B.data_ptr()), C.c_void_p(X.data_ptr()),
                                         C.byref(fi), sp), "pcg_solve_hsc")
         torch.sub(X, target, out=G)              # dL/dx for L = |x - target|^2 / 2
-        _lib.check(lib.cn_adjoint_solve_hsc(lap.handle, hsc.handle, C.byref(opts), k,
+        _lib.check(lib.cn_adjoint_solve_hsc(lap.handle, hsc.handle, C.byref(opts_adj), k,
                                             C.c_void_p(G.data_ptr()), None, None,
                                             C.c_void_p(LAM.data_ptr()),
                                             C.c_void_p(GC.data_ptr()), None, C.byref(ai), sp),
@@ -876,6 +878,20 @@ def icosphere9_bench(args):
                                    C.c_void_p(B.data_ptr()), sp), "harmonic_rhs")
     tune = tune_for(lap, B, args, hsc, precond="hsc")
     opts.inner_tol = args.inner_tol if args.inner_tol is not None else tune["inner_tol"]
+    # the adjoint's right-hand side (x - target) is rougher than the forward
+    # one: its residual-replacement drop is tuned on it (the launch shapes
+    # stay the forward's)
+    from tool_godot_curvenet_amd import autotune
+    _lib.check(lib.cn_pcg_solve_hsc(lap.handle, hsc.handle, C.byref(opts), k,
+                                    C.c_void_p(B.data_ptr()), C.c_void_p(X.data_ptr()),
+                                    C.byref(fi), sp), "pcg_solve_hsc")
+    torch.sub(X, target, out=G)
+    adj = autotune.autotune(lap, G, precond="hsc", precision=args.precision, tol=args.tol,
+                            use_cache=False, shapes=False, hsc=hsc)
+    tune["adjoint_inner_tol"] = adj["inner_tol"]
+    opts_adj = _lib.PcgOpts()
+    C.memmove(C.byref(opts_adj), C.byref(opts), C.sizeof(opts))
+    opts_adj.inner_tol = args.inner_tol if args.inner_tol is not None else adj["inner_tol"]
     free0, total = torch.cuda.mem_get_info()
     for _ in range(args.warmup):
         step()

@@ -214,6 +214,13 @@ def test_autotune_choice_converges_and_caches(gpu, tmp_path):
     r2 = autotune.autotune(lap, B, precond="hsc", precision="mixed", hsc=h, path=path)
     assert r2["cached"] and r2["tune"] == r["tune"] and lap.get_tune() == r["tune"]
     assert lap.inner_tol == r["inner_tol"]
+    # another right-hand side of the same system: only the drop is picked,
+    # the launch shapes stay
+    r3 = autotune.autotune(lap, 2.0 * B + 1.0, precond="hsc", precision="mixed", hsc=h,
+                           path=path, shapes=False, reps=1)
+    assert not r3["cached"] and r3["tune"] == r["tune"] and lap.get_tune() == r["tune"]
+    assert r3["inner_tol"] in autotune.INNER_TOLS and lap.inner_tol == r3["inner_tol"]
+    assert {c["inner_tol"] for c in r3["candidates"]} == set(autotune.INNER_TOLS)
 
 
 @pytest.mark.parametrize("inner_tol", [0.0316, 1e-2, 1e-3, 1e-4])

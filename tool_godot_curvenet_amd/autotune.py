@@ -121,17 +121,21 @@ def _tail_applies(k: int, precision: str) -> bool:
 
 
 def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: float = 1e-9,
-             reps: int = 2, path: str | None = None, use_cache: bool = True, **pre) -> dict:
+             reps: int = 2, path: str | None = None, use_cache: bool = True,
+             shapes: bool = True, **pre) -> dict:
     """Time full solves of system on right-hand side B (n x k, FP64) for the
     candidate launch shapes (and, in mixed precision, residual-replacement
     drops) and keep the fastest (set on the system and stored in the
     cache).  `pre` passes the preconditioner object (hsc=, ic0= or sgs=).
+    shapes=False keeps the system's launch shapes and picks only the drop
+    (for another right-hand side of the same system, such as an adjoint's;
+    nothing is cached then).
     Returns {"key", "tune", "inner_tol", "ms", "cached", "candidates"}."""
     import torch
     path = path or cache_path()
     k = int(B.shape[1]) if len(B.shape) == 2 else 1
     key = key_of(system, k, precision, precond)
-    if use_cache:
+    if use_cache and shapes:
         hit = lookup(system, k, precision, precond, path)
         if hit is not None:
             system.set_tune(hit)
@@ -158,6 +162,17 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
         tried.append({"tune": dict(tune), "inner_tol": inner_tol, "ms": 1e3 * best})
         return best
 
+    if not shapes:
+        cur = system.get_tune()
+        best, itol = float("inf"), INNER_TOLS[0]
+        if precision == "mixed":
+            for it in INNER_TOLS:
+                t = run(cur, it, max(reps, 4))
+                if t < best:
+                    best, itol = t, it
+        system.inner_tol = itol
+        return {"key": key, "tune": cur, "inner_tol": itol, "ms": 1e3 * best, "cached": False,
+                "candidates": tried}
     cur = dict(DEFAULT)
     # 1: the SpMM form (LDS-staged, or the row table with 4, 2 or 1 stripes)
     opts = [{"spmm_form": 2, "spmm_gens": 0}] + [{"spmm_form": 1, "spmm_gens": g}
