@@ -107,6 +107,41 @@ kw_hsc_chain(int32_t nrows, const int32_t *__restrict__ rows, int32_t a, int32_t
   }
 }
 
+// The same launch over prebuilt row records {v, e0, e1, e2}: one 16-byte
+// load per row instead of the dependent chain rows[i] -> seg_ptr[v] ->
+// seg[...] before the gathers can start.
+template <typename T, int VEC, int LPR>
+__global__ void __launch_bounds__(kBlock)
+kw_hsc_chain_rec(int32_t nrows, const int4 *__restrict__ rec,
+                 const int32_t *__restrict__ ecol, const T *__restrict__ eval, int64_t k,
+                 const T *srcA, T *Z, const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
+  using P = Pack<LPR>;
+  const int64_t c0 = P::col0(VEC);
+  const bool valid = c0 < k;
+  const int64_t cl = valid ? c0 : k - VEC;
+  const RowIter ri = row_iter(0, nrows, P::kSlots, P::slot());
+  for (int32_t i = ri.first; i < ri.end; i += ri.stride) {
+    const int4 r = rec[P::u(i)];
+    const int32_t v = P::u(r.x), e0 = P::u(r.y), e1 = P::u(r.z), e2 = P::u(r.w);
+    T acc[VEC];
+    VecIO<T, VEC>::ld(srcA + (int64_t)v * k + cl, acc);
+    gather_wide<T, VEC, 4>(e0, e1, ecol, eval, srcA, k, cl, true, acc);
+    gather_wide<T, VEC, 8>(e1, e2, ecol, eval, Z, k, cl, true, acc);
+    if (valid) VecIO<T, VEC>::st(Z + (int64_t)v * k + c0, acc);
+  }
+}
+
+// rec[i] = {v, seg[sp + a], seg[sp + a + 1], seg[sp + m]}, v = rows[i]
+__global__ void k_chain_rec(int32_t nrows, const int32_t *__restrict__ rows, int32_t a,
+                            int32_t m, const int32_t *__restrict__ seg_ptr,
+                            const int32_t *__restrict__ seg, int4 *__restrict__ rec) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  const int32_t v = rows[i], sp = seg_ptr[v];
+  rec[i] = make_int4(v, seg[sp + a], seg[sp + a + 1], seg[sp + m]);
+}
+
 // ---- narrow form (k <= 4): one thread per row, its K columns in registers
 // (the row's index entries are read once for all columns) ---------------
 // acc[c] += sum_{e in [a,b)} val[e] X[col[e] K + c], CH entries at a time:
@@ -318,11 +353,69 @@ static void level_chain(const cn_hsc_t *H, const int32_t *rows, int32_t nrows, i
 
 constexpr int32_t kChainLevels = 3;  // levels a group of down-sweep launches chains through
 
+// the launches of a down sweep to level t in groups of G levels, in order:
+// (a, m, rows, row count)
+struct ChainLaunch { int32_t a, m; const int32_t *rows; int32_t nrows; };
+static std::vector<ChainLaunch> chain_launches(const cn_hsc_t *H, int32_t t, int32_t G) {
+  std::vector<ChainLaunch> out;
+  for (int32_t a = 0; a < t;) {
+    const int32_t e = std::min(a + G, t);
+    for (int32_t m = a + 1; m < e; ++m)
+      out.push_back({a, m, H->lv[m].Fg.get(), H->lv[m].nF});
+    out.push_back({a, e, H->lv[e - 1].Cg.get(), H->lv[e - 1].nC});
+    a = e;
+  }
+  return out;
+}
+
+// The row records of the sweep (G, t), built on first use (nullptr when
+// may_alloc is false and they are not there yet).
+static const cn_hsc_t::ChainRecs *chain_recs(cn_hsc_t *H, int32_t t, int32_t G,
+                                             bool may_alloc, hipStream_t s) {
+  for (const auto &c : H->ch_recs)
+    if (c.G == G && c.t == t) return &c;
+  if (!may_alloc) return nullptr;
+  const std::vector<ChainLaunch> ls = chain_launches(H, t, G);
+  cn_hsc_t::ChainRecs c;
+  c.G = G;
+  c.t = t;
+  c.off.assign(1, 0);
+  for (const ChainLaunch &l : ls) c.off.push_back(c.off.back() + l.nrows);
+  c.rec.alloc((size_t)std::max<int64_t>(c.off.back(), 1));
+  for (size_t i = 0; i < ls.size(); ++i)
+    if (ls[i].nrows > 0)
+      k_chain_rec<<<grid_for(ls[i].nrows, 256), 256, 0, s>>>(
+          ls[i].nrows, ls[i].rows, ls[i].a, ls[i].m, H->ch_seg_ptr.get(), H->ch_seg.get(),
+          c.rec.get() + c.off[i]);
+  CN_HIP(hipGetLastError());
+  H->ch_recs.push_back(std::move(c));
+  return &H->ch_recs.back();
+}
+
+// one down-sweep launch over row records
+template <typename T>
+static void level_chain_rec(const cn_hsc_t *H, const int4 *rec, int32_t nrows, int64_t k,
+                            const T *srcA, T *Z, const int32_t *ctl, hipStream_t s,
+                            int32_t level_cap) {
+  if (nrows == 0) return;
+  const int v = hsc_vec<T>(k);
+  const int lpr = pack_lpr12((k + v - 1) / v);
+  const dim3 g = hsc_grid(nrows, k, v, lpr, level_cap);
+  const T *ev;
+  if constexpr (sizeof(T) == 4) ev = H->ch_val32.get(); else ev = H->ch_val.get();
+#define C_(VV, LL) kw_hsc_chain_rec<T, VV, LL><<<g, kBlock, 0, s>>>(nrows, rec, H->ch_col.get(), ev, k, srcA, Z, ctl)
+  if constexpr (sizeof(T) == 4) {
+    if (v == 4) { HSC_LPR(lpr, C_, 4); return; }
+  }
+  if (v >= 2) { HSC_LPR(lpr, C_, 2); } else { HSC_LPR(lpr, C_, 1); }
+#undef C_
+}
+
 // The down sweep up to level t (the coarse level, or the tail's first), in
 // groups of chain_levels levels.
 template <typename T>
 static void down_sweep(cn_hsc_t *H, int32_t t, int64_t k, const T *r, T *z, const int32_t *ctl,
-                       hipStream_t s, const HscShape &sh) {
+                       hipStream_t s, const HscShape &sh, bool may_alloc) {
   if (hsc_vec<T>(k) == 0) {
     for (int32_t l = 0; l < t; ++l) level_down_n<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s);
     return;
@@ -330,6 +423,13 @@ static void down_sweep(cn_hsc_t *H, int32_t t, int64_t k, const T *r, T *z, cons
   if (t == 0) return;
   if (!H->ch_ready) throw HipError{hipErrorNotReady, "hsc: restriction chains not built"};
   const int32_t G = sh.chain_levels > 0 ? sh.chain_levels : kChainLevels;
+  if (const cn_hsc_t::ChainRecs *cr = chain_recs(H, t, G, may_alloc, s)) {
+    const std::vector<ChainLaunch> ls = chain_launches(H, t, G);
+    for (size_t i = 0; i < ls.size(); ++i)
+      level_chain_rec<T>(H, cr->rec.get() + cr->off[i], ls[i].nrows, k,
+                         ls[i].a == 0 ? r : z, z, ctl, s, sh.level_cap);
+    return;
+  }
   for (int32_t a = 0; a < t;) {
     const int32_t e = std::min(a + G, t);
     const T *srcA = a == 0 ? r : z;
@@ -527,7 +627,7 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
     const TailLevel<T> *desc = tail_desc<T>(H, s);
     {
       ProfScope ps("hsc_down", s);
-      down_sweep<T>(H, l0, k, r, z, ctl, s, sh);
+      down_sweep<T>(H, l0, k, r, z, ctl, s, sh, ext_work == nullptr);
     }
     {
       ProfScope ps("hsc_tail", s);
@@ -550,7 +650,7 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
   }
   {
     ProfScope ps("hsc_down", s);
-    down_sweep<T>(H, L, k, r, z, ctl, s, sh);
+    down_sweep<T>(H, L, k, r, z, ctl, s, sh, ext_work == nullptr);
   }
   if (H->nc > 0) {
     ProfScope ps("hsc_coarse", s);

@@ -324,6 +324,17 @@ struct cn_hsc_t {
   cn::DevBuf<double> ch_val;
   cn::DevBuf<float> ch_val32;
   bool ch_ready = false;
+  // the down sweep's launches for one (group length, end level): per row of
+  // every launch, in launch order, {level-0 row, first entry of its level-a
+  // segment, end of that segment, end of its chain at the launch's level},
+  // so a row's chain costs one 16-byte load before its gathers; built on
+  // first use of that sweep shape
+  struct ChainRecs {
+    int32_t G = 0, t = 0;
+    cn::DevBuf<int4> rec;
+    std::vector<int64_t> off;  // launch i's records: [off[i], off[i + 1])
+  };
+  std::vector<ChainRecs> ch_recs;
 };
 
 namespace cn {
