@@ -257,7 +257,7 @@ def main():
 
     lib.cn_profile_enable(1)
     prof = {name: [0, 0.0] for name in ("spmm", "spmm_lds", "spmm_dir", "spmm_upd", "fwd",
-                                         "bwd", "resid", "precond", "dot", "hsc_down",
+                                         "bwd", "resid", "dot", "hsc_down",
                                          "hsc_up", "hsc_coarse", "hsc_tail")}
     iters, refines, relmax = 0, 0, 0.0
     torch.cuda.synchronize()

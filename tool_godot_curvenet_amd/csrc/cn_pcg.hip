@@ -1629,14 +1629,15 @@ __global__ void k_scale_in(int64_t total, int64_t k, SegMap sm, const double *__
   }
 }
 
-// X += invs_c * D; D = 0
+// X += invs_c * D; D = 0.  init: X is not read (the first update of a
+// solve from x0 = 0, whose X was never written), the same fma from 0.
 template <typename T>
 __global__ void k_x_update(int64_t total, int64_t k, SegMap sm, const double *__restrict__ invs,
-                           T *__restrict__ D, double *__restrict__ X) {
+                           T *__restrict__ D, double *__restrict__ X, int init) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t row = i / k;
-    X[i] = fma(invs[sm.vc(row, k, i - row * k)], (double)D[i], X[i]);
+    X[i] = fma(invs[sm.vc(row, k, i - row * k)], (double)D[i], init ? 0.0 : X[i]);
     D[i] = T(0);
   }
 }
@@ -1672,7 +1673,7 @@ kw_scale_in(int32_t n, int64_t k, SegMap sm, const double *__restrict__ sc,
 template <typename T, int VEC, int LPR>
 __global__ void __launch_bounds__(kBlock)
 kw_x_update(int32_t n, int64_t k, SegMap sm, const double *__restrict__ invs,
-            T *__restrict__ D, double *__restrict__ X) {
+            T *__restrict__ D, double *__restrict__ X, int init) {
   WIDE_COLS_P(VEC, LPR)
   double iv[VEC];
 #pragma unroll
@@ -1687,7 +1688,12 @@ kw_x_update(int32_t n, int64_t k, SegMap sm, const double *__restrict__ invs,
       const int64_t o = (int64_t)row * k + c0;
       double xv[VEC];
       T dv[VEC];
-      VecIO<double, VEC>::ld(X + o, xv);
+      if (init) {
+#pragma unroll
+        for (int m = 0; m < VEC; ++m) xv[m] = 0.0;
+      } else {
+        VecIO<double, VEC>::ld(X + o, xv);
+      }
       VecIO<T, VEC>::ld(D + o, dv);
 #pragma unroll
       for (int m = 0; m < VEC; ++m) {
@@ -1758,11 +1764,20 @@ struct SolverWork {
   DevBuf<double> part;                // 3 x part_rows x k
   DevBuf<double> red;                 // 2 x kRedRows x k, first-stage sums
   DevBuf<double> st;                  // ST_N x nvc per-column scalars
-  DevBuf<int32_t> active, restart;    // nvc
-  DevBuf<int32_t> ctl;                // CTL_N ints
+  // the control block and the per-column flags in one allocation,
+  // [ctl: CTL_N | active: nvc + 4 | restart: nvc + 4], so a round uploads
+  // them in one copy from the pinned mirror h_flags (same layout)
+  struct View32 {
+    int32_t *p = nullptr;
+    int32_t *get() const { return p; }
+  };
+  DevBuf<int32_t> flags;
+  View32 ctl, active, restart;
   DevBuf<double> dinv64;
   DevBuf<float> dinv32;
-  int32_t *h_ctl = nullptr;           // pinned
+  int32_t *h_flags = nullptr;         // pinned, the layout of flags
+  int32_t *h_ctl = nullptr;           // = h_flags
+  int32_t *h_dact = nullptr;          // pinned, nvc + 4: active read back
   double *h_st = nullptr;             // pinned, ST_N x nvc
   // the captured block of inner iterations (fused solve), replayed while
   // the key -- everything its launches depend on -- is unchanged (a few
@@ -1773,7 +1788,8 @@ struct SolverWork {
   ~SolverWork() {
     for (auto &e : graphs) (void)hipGraphExecDestroy(e.second);
     if (cap) (void)hipStreamDestroy(cap);
-    if (h_ctl) (void)hipHostFree(h_ctl);
+    if (h_flags) (void)hipHostFree(h_flags);
+    if (h_dact) (void)hipHostFree(h_dact);
     if (h_st) (void)hipHostFree(h_st);
   }
 };
@@ -1804,12 +1820,17 @@ static SolverWork *ensure_work(cn_system_t *S, int32_t k, int32_t part_rows) {
   W->part.alloc((size_t)3 * pr * kk + 4);
   W->red.alloc((size_t)2 * kRedRows * kk + 4);
   W->st.alloc((size_t)ST_N * nvc + 4);
-  W->active.alloc((size_t)nvc + 4);
-  W->restart.alloc((size_t)nvc + 4);
-  W->ctl.alloc(CTL_N);
+  const size_t nflags = CTL_N + 2 * ((size_t)nvc + 4);
+  W->flags.alloc(nflags);
+  W->ctl.p = W->flags.get();
+  W->active.p = W->ctl.p + CTL_N;
+  W->restart.p = W->active.p + nvc + 4;
   W->dinv64.alloc((size_t)n + 1);
   W->dinv32.alloc((size_t)n + 1);
-  CN_HIP(hipHostMalloc((void **)&W->h_ctl, sizeof(int32_t) * CTL_N, hipHostMallocDefault));
+  CN_HIP(hipHostMalloc((void **)&W->h_flags, sizeof(int32_t) * nflags, hipHostMallocDefault));
+  W->h_ctl = W->h_flags;
+  CN_HIP(hipHostMalloc((void **)&W->h_dact, sizeof(int32_t) * ((size_t)nvc + 4),
+                       hipHostMallocDefault));
   CN_HIP(hipHostMalloc((void **)&W->h_st, sizeof(double) * ((size_t)ST_N * nvc + 4),
                        hipHostMallocDefault));
   S->work = W;
@@ -2172,8 +2193,9 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
                           const T *Q, T *R, T *Z, double *part, int64_t stride,
                           const int32_t *ctl, hipStream_t s, bool no_rupd = false) {
   if (V.hsc || V.sgs || V.sm.of_row) {
+    // (no scope of its own: each event pair around a launch group leaves a
+    // few microseconds idle, so only the parts inside are timed)
     Grid G = grid_rows<T>(V.n, k, kCapSpmm);
-    ProfScope ps("precond", s);
 #define N_(KK) kf_rupd<T, KK, FUSED><<<G.g, kFlat(KK), 0, s>>>(V.n, alpha, V.sm, Q, R, part, ctl)
 #define W_(VV, LL) kw_rupd<T, WideV<T, VV>::v, FUSED, LL><<<G.g, kBlock, 0, s>>>(V.n, alpha, V.sm, k, Q, R, part, ctl)
     if (!no_rupd) DISPATCH_K(k, T, G.lpr, N_, W_);
@@ -2364,11 +2386,12 @@ static void launch_scale_in(const SysView<T> &V, int64_t k, const double *sc, co
 
 template <typename T>
 static void launch_x_update(const SysView<T> &V, int64_t k, const double *invs, T *D,
-                            double *X, hipStream_t s) {
+                            double *X, hipStream_t s, bool init = false) {
   const int64_t total = (int64_t)V.n * k;
   const Grid G = grid_rows<T>(V.n, k, kCapSpmm);
-#define N_(KK) k_x_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, V.sm, invs, D, X)
-#define W_(VV, LL) kw_x_update<T, WideV<T, VV>::v, LL><<<G.g, kBlock, 0, s>>>(V.n, k, V.sm, invs, D, X)
+  const int ini = init ? 1 : 0;
+#define N_(KK) k_x_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, V.sm, invs, D, X, ini)
+#define W_(VV, LL) kw_x_update<T, WideV<T, VV>::v, LL><<<G.g, kBlock, 0, s>>>(V.n, k, V.sm, invs, D, X, ini)
   DISPATCH_K(k, T, G.lpr, N_, W_);
 #undef N_
 #undef W_
@@ -2713,9 +2736,10 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   if (o->warm_start) {
     if (X != d_x)
       CN_HIP(hipMemcpyAsync(X, d_x, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
-  } else {
-    CN_HIP(hipMemsetAsync(X, 0, sizeof(double) * total, s));
   }
+  // from x0 = 0, X is first written by the first x update (which then does
+  // not read it); a solve that ends before any update zeroes it at the end
+  bool x_unwritten = !o->warm_start;
   int32_t rows;
   Red rd;
   // the FP64 residual the first round scales into R: from x0 = 0 it is b
@@ -2744,8 +2768,13 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   CN_HIP(hipGetLastError());
 
   // act: the columns the host started the round with; dact: those the
-  // device still iterated at the round's end (its stop tests clear them)
-  std::vector<int32_t> act(kk, 0), rst(kk, 0), dact(kk, 0);
+  // device still iterated at the round's end (its stop tests clear them).
+  // act and rst live in the pinned mirror of the flags block.
+  int32_t *act = W->h_flags + CTL_N;
+  int32_t *rst = act + W->nvc + 4;
+  int32_t *dact = W->h_dact;
+  std::fill(act, act + kk, 0);
+  std::fill(rst, rst + kk, 0);
   int32_t iters = 0, launched = 0, updates = 0;
   bool r_scaled = false;  // R already holds the scaled residual of this round
   cn_status status = CN_OK;
@@ -2756,7 +2785,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     CN_HIP(hipMemcpyAsync(hs + ST_BB * kk, st + ST_BB * kk, sizeof(double) * 2 * kk,
                           hipMemcpyDeviceToHost, s));  // BB and RR adjacent
     if (!first)
-      CN_HIP(hipMemcpyAsync(dact.data(), W->active.get(), sizeof(int32_t) * kk,
+      CN_HIP(hipMemcpyAsync(dact, W->active.get(), sizeof(int32_t) * kk,
                             hipMemcpyDeviceToHost, s));
     CN_HIP(hipStreamSynchronize(s));
     relmax = 0.0;
@@ -2796,10 +2825,6 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     for (int64_t c = 0; c < kk; ++c) any_rst = any_rst || rst[c];
     const bool apply_m = first || any_rst;
     first = false;
-    CN_HIP(hipMemcpyAsync(W->active.get(), act.data(), sizeof(int32_t) * kk,
-                          hipMemcpyHostToDevice, s));
-    CN_HIP(hipMemcpyAsync(W->restart.get(), rst.data(), sizeof(int32_t) * kk,
-                          hipMemcpyHostToDevice, s));
     hc[CTL_ACTIVE] = nact;
     hc[CTL_BREAKDOWN] = 0;
     hc[CTL_ONE] = 1;
@@ -2807,7 +2832,9 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     hc[CTL_JUST] = 0;
     hc[CTL_ITERS] = iters;
     hc[CTL_PAUSE] = 0;
-    CN_HIP(hipMemcpyAsync(ctl, hc, sizeof(int32_t) * 7, hipMemcpyHostToDevice, s));
+    // control block, active and restart flags in one copy
+    CN_HIP(hipMemcpyAsync(ctl, hc, sizeof(int32_t) * (CTL_N + 2 * ((size_t)W->nvc + 4)),
+                          hipMemcpyHostToDevice, s));
     CN_HIP(hipMemsetAsync(st + ST_ALPHA * kk, 0, sizeof(double) * kk, s));
     // after the first round the residual kernel writes the scaled R itself
     if (!r_scaled)
@@ -2947,7 +2974,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     // ---- reliable update: x += d / s (FP64), d = 0; r = b - A x (FP64).
     // (One pass that forms x + d / s for the gathered neighbours as well was
     // measured 0.4 % slower a step at k = 768 than these two kernels.)
-    launch_x_update<T>(V, k, st + ST_INVS * kk, D, X, s);
+    launch_x_update<T>(V, k, st + ST_INVS * kk, D, X, s, x_unwritten);
+    x_unwritten = false;
     if (batch) {
       // FP64 residual, per-segment r.r, scaled into R at the top of the loop
       launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
@@ -2964,6 +2992,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     ++updates;
     if (it == 0) { status = CN_MAXIT; break; }
   }
+  if (x_unwritten) CN_HIP(hipMemsetAsync(X, 0, sizeof(double) * total, s));
   if (X != d_x)
     CN_HIP(hipMemcpyAsync(d_x, X, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
   CN_HIP(hipStreamSynchronize(s));
