@@ -29,24 +29,32 @@ def _say(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-@pytest.fixture(scope="module")
-def capsules():
+# hierarchy shapes (drop threshold, coarse rows at most, levels at most):
+# the library default, and the benchmark's for this workload (22 levels, a
+# 3,834-row coarse block whose product runs on the FP64 matrix cores in
+# 128 x 128 tiles)
+SHAPES = {"default": (0.015, 512, 24), "bench": (0.002, 4096, 24)}
+
+
+@pytest.fixture(scope="module", params=sorted(SHAPES))
+def capsules(request):
     from tool_godot_curvenet_amd import HSC, CutLaplacian, meshgen
     lap = CutLaplacian.from_mesh(meshgen.config_mesh("capsules100k"))
-    return lap, HSC(lap)
+    theta, cmax, lmax = SHAPES[request.param]
+    return lap, HSC(lap, theta=theta, coarse_max=cmax, max_levels=lmax), SHAPES[request.param]
 
 
 def test_c3a_capsules_768_columns(gpu, capsules):
     import torch
     from tool_godot_curvenet_amd import meshgen
-    lap, hsc = capsules
+    lap, hsc, _ = capsules
     k = 768
     rng = np.random.default_rng(meshgen.BASE_SEED + 30)
     Xc = torch.as_tensor(rng.uniform(-0.2, 0.2, size=(lap.n_con, k)), device="cuda")
     B = lap.harmonic_rhs(Xc, as_numpy=False)
     x, info = lap.solve(B, precond="hsc", precision="mixed", tol=1e-9, hsc=hsc)
-    _say(f"C3a: n={lap.n_free} k={k} iters={info.iters} refinements={info.refinements} "
-         f"relres_max={info.relres_max:.2e}")
+    _say(f"C3a: n={lap.n_free} k={k} levels={hsc.info.nlevels} coarse={hsc.info.n_coarse} "
+         f"iters={info.iters} refinements={info.refinements} relres_max={info.relres_max:.2e}")
     assert info.status == 0
     A = lap.A_ff()
     rr = _relres(A, x.cpu().numpy(), B.cpu().numpy())
@@ -59,7 +67,7 @@ def test_c3a_first_step_direction(gpu, capsules):
     columns spread over the tiles)."""
     import torch
     from tool_godot_curvenet_amd import meshgen
-    lap, hsc = capsules
+    lap, hsc, (theta, cmax, lmax) = capsules
     k = 768
     rng = np.random.default_rng(meshgen.BASE_SEED + 31)
     Xc = torch.as_tensor(rng.uniform(-0.2, 0.2, size=(lap.n_con, k)), device="cuda")
@@ -69,7 +77,8 @@ def test_c3a_first_step_direction(gpu, capsules):
     cols = [0, 1, 255, 256, 511, 767]
     X = x.cpu().numpy()[:, cols]
     A = lap.A_ff()
-    Z = O.hsc_apply(O.hsc_build(A.row_ptr, A.col_idx, A.values), B[:, cols])
+    Z = O.hsc_apply(O.hsc_build(A.row_ptr, A.col_idx, A.values, theta=theta, coarse_max=cmax,
+                                max_levels=lmax), B[:, cols])
     e = np.linalg.norm(X / np.linalg.norm(X, axis=0) - Z / np.linalg.norm(Z, axis=0), axis=0)
     assert np.all(e <= 2e-5), e
 

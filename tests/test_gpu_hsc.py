@@ -85,8 +85,9 @@ def test_hierarchy_bit_exact(gpu, name, comp):
     assert np.abs(inv @ Co - np.eye(len(Co))).max() <= 1e-9
 
 
-# k <= 4 takes the coarse GEMV; k = 5, 8 and 48 take the coarse GEMM with its
-# K range split over 4 blocks a tile (391 coarse rows, 13 or 26 tiles).
+# k <= 4 takes the coarse GEMV; k = 5, 8 and 48 take the matrix-core coarse
+# product in 64 x 64 tiles with its j range in 8 slices (391 coarse rows:
+# the last slice starts past the last row and adds only zeros).
 @pytest.mark.parametrize("k", [1, 3, 5, 8, 48])
 def test_hsc_apply_matches_oracle(gpu, k):
     from tool_godot_curvenet_amd import HSC
@@ -112,6 +113,23 @@ def test_hsc_apply_chains_match_oracle(gpu, k):
     H = HSC(lap)
     assert H.info.nlevels >= 8
     ho = O.hsc_build(A.row_ptr, A.col_idx, A.values)
+    r = np.random.default_rng(k).normal(size=(lap.n_free, k))
+    zg = H.apply(r)
+    zo = O.hsc_apply(ho, r)
+    assert np.linalg.norm(zg - zo) <= 1e-11 * np.linalg.norm(zo)
+
+
+# A coarse block of more than 3,072 rows and k >= 512 takes the 128 x 128
+# tiles of the coarse product (k = 520 and 3,000+ rows: partial tiles in
+# both directions); 192 columns take the 64 x 64 tiles on the same block.
+@pytest.mark.parametrize("k", [192, 520])
+def test_hsc_apply_large_coarse_matches_oracle(gpu, k):
+    from tool_godot_curvenet_amd import HSC
+    lap = _lap("icosphere6")
+    A = lap.A_ff()
+    H = HSC(lap, coarse_max=4096)
+    assert 3072 <= H.info.n_coarse <= 4096, H.info.n_coarse
+    ho = O.hsc_build(A.row_ptr, A.col_idx, A.values, coarse_max=4096)
     r = np.random.default_rng(k).normal(size=(lap.n_free, k))
     zg = H.apply(r)
     zo = O.hsc_apply(ho, r)
@@ -208,6 +226,37 @@ def test_pcg_exact_coarse_converges(gpu, k, inner_tol):
         rr = np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
         assert np.all(rr <= 1e-9), (precision, rr)
         assert info.iters <= 4, (precision, info.iters)
+
+
+# More than 32 columns that are not a multiple of the vector width (4 FP32,
+# 2 FP64) are solved on a copy padded with zero columns: the solution is the
+# same bits as solving the padded right-hand side directly, a warm start
+# goes through the padding, and the solve converges.
+@pytest.mark.parametrize("precision,k,vec", [("mixed", 37, 4), ("mixed", 101, 4),
+                                             ("fp64", 37, 2)])
+def test_pcg_padded_columns(gpu, precision, k, vec):
+    from tool_godot_curvenet_amd import HSC
+    from tool_godot_curvenet_amd import meshgen as G
+    lap = _lap("icosphere6")
+    A = lap.A_ff()
+    h = HSC(lap)
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=13))
+    kp = (k + vec - 1) // vec * vec
+    Bp = np.zeros((lap.n_free, kp))
+    Bp[:, :k] = B
+    x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, hsc=h)
+    xp, infop = lap.solve(Bp, precond="hsc", precision=precision, tol=1e-9, hsc=h)
+    assert info.status == 0 and infop.status == 0
+    X, XP = x.cpu().numpy(), xp.cpu().numpy()
+    assert np.array_equal(X, XP[:, :k]) and not XP[:, k:].any()
+    assert info.iters == infop.iters
+    rr = np.linalg.norm(B - O.spmm(A.row_ptr, A.col_idx, A.values, X), axis=0) / \
+        np.linalg.norm(B, axis=0)
+    assert np.all(rr <= 1e-9), rr
+    # warm start from the solution: no iteration needed
+    x2, info2 = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, hsc=h, x0=X)
+    assert info2.status == 0 and info2.iters == 0
+    assert np.array_equal(x2.cpu().numpy(), X)
 
 
 def test_hsc_beats_ic0_on_capsules(gpu):

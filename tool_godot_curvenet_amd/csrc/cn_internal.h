@@ -216,6 +216,9 @@ struct cn_system_t {
   cn::DevBuf<int32_t> d_seg_ptr, seg_of_row, blk_ptr, blk_seg;
   cn::DevBuf<double> deform_ff;     // per-face F of cn_deform_rhs, grown on demand
   cn::SolverWork *work = nullptr;   // solver workspace, grown on demand
+  // right-hand sides and solutions of a solve whose column count is padded
+  // to the kernels' vector width (n x k_padded, grown on demand)
+  cn::DevBuf<double> pad_b, pad_x;
   ~cn_system_t() { cn::free_solver_work(work); }
 };
 

@@ -3190,6 +3190,10 @@ int32_t cn_profile_query(const char *name, double *ms_total) {
   return it->second.first;
 }
 
+// fewest columns a fused solve pads to the vector width (below this the
+// padded columns would cost more than the vector loads save)
+constexpr int32_t kPadMinK = 32;
+
 static cn_status solve_impl(cn_system S, cn_ic0 F, cn_hsc H, cn_sgs G, const cn_pcg_opts *o,
                             int32_t k,
                             const double *d_b, double *d_x, cn_solve_info *info, void *stream,
@@ -3224,15 +3228,45 @@ static cn_status solve_impl(cn_system S, cn_ic0 F, cn_hsc H, cn_sgs G, const cn_
     if (info) { info->status = CN_OK; info->iters = 0; info->refinements = 0; info->relres_max = 0.0; }
     return CN_OK;
   }
+  if (o->precision != CN_PREC_FP64_PLAIN && o->precision != CN_PREC_FP64 &&
+      o->precision != CN_PREC_MIXED) {
+    set_error("%s: unknown precision %d", who, o->precision);
+    return CN_BAD_ARG;
+  }
   prof_reset();
+  // The wide kernels move VEC columns per lane (4 FP32, 2 FP64) when k is a
+  // multiple of VEC, one otherwise, and the LDS-staged products need it too:
+  // at k = 765 the mixed solve took 33.5 ms against 22.3 ms at k = 768.  So
+  // a fused solve of more than kPadMinK columns that are not a multiple of
+  // VEC runs on a copy padded with zero columns; those have b = 0 and stay
+  // inactive from the start, and every column's arithmetic is its own.
+  const int32_t vec = o->precision == CN_PREC_MIXED ? 4 : 2;
+  if (o->precision != CN_PREC_FP64_PLAIN && k > kPadMinK && k % vec != 0) {
+    const int32_t kp = (k + vec - 1) / vec * vec;
+    const size_t n = (size_t)S->n_free, need = n * kp;
+    if (S->pad_b.n < need) S->pad_b.alloc(need);
+    if (S->pad_x.n < need) S->pad_x.alloc(need);
+    double *pb = S->pad_b.get(), *px = S->pad_x.get();
+    const size_t w = sizeof(double) * k, wp = sizeof(double) * kp;
+    CN_HIP(hipMemcpy2DAsync(pb, wp, d_b, w, w, n, hipMemcpyDeviceToDevice, s));
+    CN_HIP(hipMemset2DAsync(pb + k, wp, 0, wp - w, n, s));
+    if (o->warm_start) {
+      CN_HIP(hipMemcpy2DAsync(px, wp, d_x, w, w, n, hipMemcpyDeviceToDevice, s));
+      CN_HIP(hipMemset2DAsync(px + k, wp, 0, wp - w, n, s));
+    }
+    cn_status st = o->precision == CN_PREC_MIXED
+                       ? fused_solve<float>(S, F, H, G, o, kp, pb, px, info, s)
+                       : fused_solve<double>(S, F, H, G, o, kp, pb, px, info, s);
+    CN_HIP(hipMemcpy2DAsync(d_x, w, px, wp, w, n, hipMemcpyDeviceToDevice, s));
+    CN_HIP(hipStreamSynchronize(s));
+    prof_collect();
+    return st;
+  }
   cn_status st;
   switch (o->precision) {
     case CN_PREC_FP64_PLAIN: st = plain_solve(S, F, H, G, o, k, d_b, d_x, info, s); break;
     case CN_PREC_FP64: st = fused_solve<double>(S, F, H, G, o, k, d_b, d_x, info, s); break;
-    case CN_PREC_MIXED: st = fused_solve<float>(S, F, H, G, o, k, d_b, d_x, info, s); break;
-    default:
-      set_error("%s: unknown precision %d", who, o->precision);
-      return CN_BAD_ARG;
+    default: st = fused_solve<float>(S, F, H, G, o, k, d_b, d_x, info, s); break;
   }
   prof_collect();
   return st;
