@@ -37,7 +37,7 @@ TRAFFIC_KERNELS = {
     "spmm_row": "void cn::kw_spmm_dot<float, 4, 64>",
     "fwd": "void cn::kb_fwd<float, 4, 16, true>",
     "bwd": "void cn::kb_bwd<float, 4, 16>",
-    "hsc_down": "void cn::kw_hsc_chain<float, 4, 64>",
+    "hsc_down": "void cn::kw_hsc_chain_rec<float, 4, 64>",
     "hsc_up": "void cn::kw_hsc_up<float, 4, 64>",
     "pd_update": "void cn::kw_pd_update<float, 4, 64>",
     "dot": "void cn::kw_dot<float, 4, 64>",
@@ -355,7 +355,7 @@ def main():
         nlev = max(pre.info.nlevels if l0 < 0 else l0, 1)
         napply = prof["hsc_down"][0]
         nwork = iters + args.steps
-        kern["hsc_down"] = ("kw_hsc_chain<float,4,64>", napply * nlev, prof["hsc_down"][1],
+        kern["hsc_down"] = ("kw_hsc_chain_rec<float,4,64>", napply * nlev, prof["hsc_down"][1],
                             nwork * down_b)
         kern["hsc_up"] = ("kw_hsc_up<float,4,64>", napply * nlev, prof["hsc_up"][1],
                           nwork * up_b)
@@ -549,7 +549,7 @@ def main():
 
 def down_sweep_bytes(levels, n0, k, t, G=3):
     """Algorithmic bytes of one HSC down sweep up to level t in groups of G
-    levels (kw_hsc_chain): in group [a, e), launch m (a < m < e) takes the
+    levels (kw_hsc_chain_rec): in group [a, e), launch m (a < m < e) takes the
     rows eliminated at level m, the last launch the rows of level e, each
     from its level-a value through the R entries of levels a .. m-1; each
     row is read and written once (4k bytes each way), every distinct row
