@@ -42,10 +42,15 @@ TRAFFIC_KERNELS = {
     "pd_update": "void cn::kw_pd_update<float, 4, 64>",
     "dot": "void cn::kw_dot<float, 4, 64>",
     "hsc_tail": "void cn::k_hsc_tail<float, 4>",
-    # one coarse product: the MFMA kernel and the sum of its slices
-    "hsc_coarse": ("void cn::k_coarse_mm<float, 128, 128, 4, 2>", "void cn::k_coarse_sum<float>"),
+    # one coarse product: the MFMA kernel and the sum of its slices, on the
+    # FP32 rounding of the coarse inverse (FP32 MFMA) or on the FP64 inverse
+    "hsc_coarse": ("void cn::k_coarse_mm<float, float, 128, 128, 4, 2>",
+                   "void cn::k_coarse_sum<float, float>"),
+    "hsc_coarse_f64": ("void cn::k_coarse_mm<float, double, 128, 128, 4, 2>",
+                       "void cn::k_coarse_sum<float, double>"),
 }
 FP64_MFMA_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
 # HSC hierarchy per workload: (drop threshold, coarse rows at most, levels
 # at most).  With 768 columns the dense coarse product runs on the FP64
 # matrix cores at about 0.65 of peak, so a small threshold and a large coarse
@@ -377,20 +382,26 @@ def main():
         nc = pre.info.n_coarse
         cms = prof["hsc_coarse"][1]
         ncoarse = prof["hsc_coarse"][0]
+        # the coarse product on FP32 (coarse_prec 2) or FP64 matrix cores
+        c32 = args.precision == "mixed" and lap.get_tune()["coarse_prec"] == 2
+        cprec, cpeak = ("FP32", FP32_MFMA_PEAK_TFLOPS) if c32 else ("FP64", FP64_MFMA_PEAK_TFLOPS)
+        ctag = "hsc_coarse" if c32 else "hsc_coarse_f64"
         if l0 < 0 and ncoarse > 0:
-            # the coarse product: 2 n_c^2 k FP64 flops per apply that does work
-            mfma["hsc_coarse"] = ("k_coarse_mm<float,128,128,4,2> + k_coarse_sum (FP64 MFMA)",
-                                  ncoarse, cms, nwork * 2.0 * nc * nc * k)
+            # the coarse product: 2 n_c^2 k flops per apply that does work
+            mt = "float" if c32 else "double"
+            mfma[ctag] = (f"k_coarse_mm<float,{mt},128,128,4,2> + "
+                          f"k_coarse_sum ({cprec} MFMA)", ncoarse, cms,
+                          nwork * 2.0 * nc * nc * k, cpeak)
         tflops = 2.0 * nc * nc * k * nwork / (cms * 1e-3) / 1e12 if cms > 0 else 0.0
         if l0 >= 0:
             coarse = {"kernel": "coarse solve inside k_hsc_tail (FP64 on the vector units, "
                                 "one workgroup per column slice)", "n_coarse": nc,
                       "tail_from_level": int(l0), "levels": int(pre.info.nlevels)}
         else:
-            coarse = {"kernel": "k_coarse_mm (coarse inverse x coarse rows, FP64 MFMA)",
+            coarse = {"kernel": f"k_coarse_mm (coarse inverse x coarse rows, {cprec} MFMA)",
                       "n_coarse": nc, "launches": ncoarse, "avg_ms": cms / max(ncoarse, 1),
-                      "achieved_tflops": tflops, "peak_tflops": FP64_MFMA_PEAK_TFLOPS,
-                      "frac": tflops / FP64_MFMA_PEAK_TFLOPS}
+                      "achieved_tflops": tflops, "peak_tflops": cpeak,
+                      "frac": tflops / cpeak}
     kernels = {}
     for tag, (kname, launches, ms, nbytes) in kern.items():
         gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
@@ -398,12 +409,12 @@ def main():
                         "avg_ms": ms / max(launches, 1),
                         "bytes_per_launch": nbytes / max(launches, 1),
                         "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
-    for tag, (kname, launches, ms, flops) in mfma.items():
+    for tag, (kname, launches, ms, flops, peak) in mfma.items():
         tf = flops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
         kernels[tag] = {"kernel": kname, "bound": "mfma", "launches": launches, "ms": ms,
                         "avg_ms": ms / max(launches, 1),
                         "flops_per_launch": flops / max(launches, 1),
-                        "achieved_tflops": tf, "frac": tf / FP64_MFMA_PEAK_TFLOPS}
+                        "achieved_tflops": tf, "peak": peak, "frac": tf / peak}
     dom = max(kernels, key=lambda t: kernels[t]["ms"])
 
     # --------------------------------------- plain FP64 one-solve-per-launch
@@ -501,7 +512,7 @@ def main():
     kd = kernels[dom]
     if kd["bound"] == "mfma":
         roofline = {"bound": "mfma", "achieved": kd["achieved_tflops"],
-                    "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": kd["frac"],
+                    "peak": kd["peak"], "unit": "TFLOP/s", "frac": kd["frac"],
                     "traffic": traffic, "kernel": kd["kernel"], "avg_ms": kd["avg_ms"],
                     "launches": kd["launches"], "flops_per_launch": kd["flops_per_launch"]}
     else:
@@ -597,11 +608,14 @@ def tune_for(system, B, args, pre, precond=None):
     built_in = next((c["ms"] for c in r["candidates"]
                      if c["tune"] == {**autotune.DEFAULT, "spmm_form": 2}
                      and c["inner_tol"] == autotune.INNER_TOLS[0]), None)
+    tried = [{"set": {f: v for f, v in c["tune"].items() if v != autotune.DEFAULT[f]},
+              "inner_tol": c["inner_tol"], "ms": round(c["ms"], 3)} for c in r["candidates"]]
     return {"chosen": r["tune"], "inner_tol": r["inner_tol"], "solve_ms": r["ms"],
             "built_in_solve_ms": built_in, "candidates": len(r["candidates"]),
+            "tried": tried,
             "how": "coordinate search over SpMM form / stripes, HSC level-pass grid cap, "
-                   "fused-tail size, down-sweep chain length, residual-replacement drop; "
-                   "full solves of this workload, best of 2"}
+                   "fused-tail size, down-sweep chain length, coarse product precision, "
+                   "residual-replacement drop; full solves of this workload, best of 2"}
 
 
 def spmv_k1(lib, A, sp, reps=200):

@@ -211,13 +211,19 @@ cn_status cn_system_batch(int32_t nsys, const cn_system *systems, cn_system *out
  *              up to 16384 (0: 1536; -1: no fused tail);
  *   chain_levels  HSC down sweep (k > 4): levels one group of launches
  *              chains through, each row written once a group, 1 to 24
- *              (0: 3; 1: one in-place pass per level). */
+ *              (0: 3; 1: one in-place pass per level);
+ *   coarse_prec  HSC coarse product of the mixed-precision (FP32) apply,
+ *              k > 4: 1 = the FP64 coarse inverse on FP64 MFMA, 2 = its FP32
+ *              rounding on FP32 MFMA (twice the rate; a preconditioner
+ *              perturbed at FP32 rounding) (0: 1).  FP64 applies always
+ *              take the FP64 inverse. */
 typedef struct cn_tune {
   int32_t spmm_form;
   int32_t spmm_gens;
   int32_t level_cap;
   int32_t tail_rows;
   int32_t chain_levels;
+  int32_t coarse_prec;
 } cn_tune;
 cn_status cn_system_set_tune(cn_system sys, const cn_tune *tune);
 cn_status cn_system_get_tune(cn_system sys, cn_tune *tune);

@@ -161,6 +161,38 @@ def test_chain_levels_direction_and_convergence(gpu, g, precision, tol):
     assert np.all(np.linalg.norm(R, axis=0) <= 1e-9 * np.linalg.norm(B, axis=0))
 
 
+# The FP32 coarse product (coarse_prec 2) on 64 x 64 tiles (k = 64) and
+# 128 x 128 tiles (k = 520, 3,000+ coarse rows): one mixed PCG step from 0
+# follows the oracle hierarchy's apply to FP32 rounding, and the solve
+# converges; an FP64 solve on the same tune keeps the FP64 product.
+@pytest.mark.parametrize("k", [64, 520])
+def test_coarse_fp32_direction_and_convergence(gpu, k):
+    from tool_godot_curvenet_amd import HSC
+    from tool_godot_curvenet_amd import meshgen as G
+    lap = _lap("icosphere6")
+    A = lap.A_ff()
+    H = HSC(lap, coarse_max=4096)
+    assert 3072 <= H.info.n_coarse <= 4096, H.info.n_coarse
+    lap.set_tune({"coarse_prec": 2})
+    assert lap.get_tune()["coarse_prec"] == 2
+    B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, k, seed=300 + k))
+    Z = O.hsc_apply(O.hsc_build(A.row_ptr, A.col_idx, A.values, coarse_max=4096), B)
+    for precision, tol in (("mixed", 2e-5), ("fp64", 1e-11)):
+        x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, max_iter=1,
+                            hsc=H)
+        assert info.iters == 1
+        X = x.cpu().numpy()
+        e = np.linalg.norm(X / np.linalg.norm(X, axis=0) - Z / np.linalg.norm(Z, axis=0),
+                           axis=0)
+        assert np.all(e <= tol), (precision, e.max())
+        x, info = lap.solve(B, precond="hsc", precision=precision, tol=1e-9, hsc=H)
+        assert info.status == 0
+        R = B - O.spmm(A.row_ptr, A.col_idx, A.values, x.cpu().numpy())
+        assert np.all(np.linalg.norm(R, axis=0) <= 1e-9 * np.linalg.norm(B, axis=0))
+    with pytest.raises(Exception):
+        lap.set_tune({"coarse_prec": 3})
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp64_plain", 1e-11),
                                            ("mixed", 2e-5)])
 @pytest.mark.parametrize("k", [3, 100])

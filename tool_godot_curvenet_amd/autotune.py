@@ -46,8 +46,9 @@ import time
 
 from . import _lib as L
 
-FIELDS = ("spmm_form", "spmm_gens", "level_cap", "tail_rows", "chain_levels")
-DEFAULT = {"spmm_form": 0, "spmm_gens": 0, "level_cap": 0, "tail_rows": 0, "chain_levels": 0}
+FIELDS = ("spmm_form", "spmm_gens", "level_cap", "tail_rows", "chain_levels", "coarse_prec")
+DEFAULT = {"spmm_form": 0, "spmm_gens": 0, "level_cap": 0, "tail_rows": 0, "chain_levels": 0,
+           "coarse_prec": 0}
 INNER_TOLS = (0.0316, 0.01, 1e-3, 1e-4)   # the first is the built-in one
 
 
@@ -202,6 +203,13 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
                 t = run({**cur, "chain_levels": g})
                 if t < best:
                     best, cur = t, {**cur, "chain_levels": g}
+        # the coarse product on the FP32 rounding of the inverse (mixed
+        # precision, the MFMA form k > 4): twice the matrix rate, against
+        # what the perturbed preconditioner costs in iterations
+        if k > 4 and precision == "mixed":
+            t = run({**cur, "coarse_prec": 2})
+            if t < best:
+                best, cur = t, {**cur, "coarse_prec": 2}
     # 5: the residual-replacement drop (mixed precision); the candidates
     # differ by a replacement or two (a few per cent), so each, the built-in
     # one included, is timed again over more solves

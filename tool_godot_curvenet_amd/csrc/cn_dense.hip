@@ -25,6 +25,7 @@
 namespace cn {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int kGT = 32;  // GEMM block tile (rows and columns)
 constexpr int kGK = 16;  // GEMM k chunk staged in LDS
@@ -263,21 +264,44 @@ void dense_cholesky_solve(int n, int k, const double *L, int64_t lda, double *B,
 // (4 waves) at m = 2,043, k = 768, 2 slices: 153 us (42 TFLOP/s); 128 x 128
 // tiles (8 waves) at m = 3,834, 4 slices: 452 us (50 TFLOP/s, 0.64 of the
 // FP64 matrix peak), against 505 us with 64 x 64 tiles.
+// The FP32 apply can take M rounded to FP32 (MT = float): the same tiles
+// on v_mfma_f32_16x16x4f32, at twice the FP64 matrix rate, FP32 partial
+// sums (a slice of j), added in FP64 by k_coarse_sum.
 constexpr int kCK = 16;
 constexpr int kCMaxChunk = 2048;  // longest slice (LDS row-number stage)
 
-template <typename T, int BM, int BN, int WM, int WN>
+// The 16 x 16 x 4 MFMA of each matrix type: A (16 x 4) lane l holds
+// A[l % 16][l / 16], B (4 x 16) lane l holds B[l / 16][l % 16]; of the
+// 16 x 16 result lane l holds column l % 16, rows row(l, v), v = 0..3.
+template <typename MT> struct CoarseMfma;
+template <> struct CoarseMfma<double> {
+  using acc_t = v4d;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int v) { return (l >> 4) + 4 * v; }
+};
+template <> struct CoarseMfma<float> {
+  using acc_t = v4f;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int v) { return 4 * (l >> 4) + v; }
+};
+
+template <typename T, typename MT, int BM, int BN, int WM, int WN>
 __global__ void __launch_bounds__(64 * WM * WN)
-k_coarse_mm(int m, int k, int kchunk, const double *__restrict__ M,
+k_coarse_mm(int m, int k, int kchunk, const MT *__restrict__ M,
             const int32_t *__restrict__ rows, const T *__restrict__ S,
-            double *__restrict__ part, const int32_t *__restrict__ ctl) {
+            MT *__restrict__ part, const int32_t *__restrict__ ctl) {
   if (ctl && ctl_idle(ctl)) return;
   constexpr int NT = 64 * WM * WN;
   constexpr int TM = BM / WM / 16, TN = BN / WN / 16;  // accumulators a wave
   constexpr int AE = BM * kCK / NT;  // M entries a thread stages per chunk
   constexpr int BE = kCK * BN / NT;  // S values a thread stages per chunk
   static_assert(AE >= 1 && BE >= 1 && kCK % AE == 0 && BN % BE == 0, "tile shape");
-  __shared__ double As[BM][kCK + 1];
+  using F = CoarseMfma<MT>;
+  __shared__ MT As[BM][kCK + 1];
   __shared__ T Bs[kCK][BN + 1];
   __shared__ int32_t Rs[kCMaxChunk + kCK];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
@@ -291,13 +315,13 @@ k_coarse_mm(int m, int k, int kchunk, const double *__restrict__ M,
   const int jb = tz * kchunk, je = min(m, jb + kchunk);
   const int ar = tid / (kCK / AE), ac = (tid % (kCK / AE)) * AE;
   const int br = tid / (BN / BE), bc = (tid % (BN / BE)) * BE;
-  const double *Mrow = M + (int64_t)min(bi + ar, m - 1) * m;
+  const MT *Mrow = M + (int64_t)min(bi + ar, m - 1) * m;
   int colc[BE];
 #pragma unroll
   for (int u = 0; u < BE; ++u) colc[u] = min(bj + bc + u, k - 1);
   for (int q = tid; q < kchunk + kCK; q += NT) Rs[q] = rows[min(jb + q, m - 1)];
   __syncthreads();
-  double pa[AE];
+  MT pa[AE];
   T pb[BE];
   auto fetch = [&](int j0) {
 #pragma unroll
@@ -306,16 +330,16 @@ k_coarse_mm(int m, int k, int kchunk, const double *__restrict__ M,
 #pragma unroll
     for (int u = 0; u < BE; ++u) pb[u] = Srow[colc[u]];
   };
-  v4d acc[TM][TN];
+  typename F::acc_t acc[TM][TN];
 #pragma unroll
   for (int a = 0; a < TM; ++a)
 #pragma unroll
-    for (int b = 0; b < TN; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < TN; ++b) acc[a][b] = typename F::acc_t{0, 0, 0, 0};
   fetch(jb);
   for (int j0 = jb; j0 < je; j0 += kCK) {
     __syncthreads();  // every wave is done with the previous chunk
 #pragma unroll
-    for (int u = 0; u < AE; ++u) As[ar][ac + u] = j0 + ac + u < je ? pa[u] : 0.0;
+    for (int u = 0; u < AE; ++u) As[ar][ac + u] = j0 + ac + u < je ? pa[u] : MT(0);
 #pragma unroll
     for (int u = 0; u < BE; ++u) Bs[br][bc + u] = pb[u];
     __syncthreads();
@@ -323,42 +347,57 @@ k_coarse_mm(int m, int k, int kchunk, const double *__restrict__ M,
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int t = 0; t < kCK; t += 4) {
-      double a[TM], b[TN];
+      MT a[TM], b[TN];
 #pragma unroll
       for (int x = 0; x < TM; ++x) a[x] = As[wi + 16 * x + (l & 15)][t + (l >> 4)];
 #pragma unroll
-      for (int y = 0; y < TN; ++y) b[y] = (double)Bs[t + (l >> 4)][wj + 16 * y + (l & 15)];
+      for (int y = 0; y < TN; ++y) b[y] = (MT)Bs[t + (l >> 4)][wj + 16 * y + (l & 15)];
 #pragma unroll
       for (int x = 0; x < TM; ++x)
 #pragma unroll
-        for (int y = 0; y < TN; ++y)
-          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+        for (int y = 0; y < TN; ++y) acc[x][y] = F::mma(a[x], b[y], acc[x][y]);
     }
   }
-  double *P = part + (int64_t)tz * m * k;
+  MT *P = part + (int64_t)tz * m * k;
 #pragma unroll
   for (int x = 0; x < TM; ++x)
 #pragma unroll
     for (int y = 0; y < TN; ++y)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = bi + wi + 16 * x + (l >> 4) + 4 * v, j = bj + wj + 16 * y + (l & 15);
+        const int i = bi + wi + 16 * x + F::row(l, v), j = bj + wj + 16 * y + (l & 15);
         if (i < m && j < k) P[(int64_t)i * k + j] = acc[x][y][v];
       }
 }
 
-template <typename T>
-__global__ void k_coarse_sum(int m, int k, int sp, const double *__restrict__ part,
-                             const int32_t *__restrict__ rows, T *__restrict__ Z,
-                             const int32_t *__restrict__ ctl) {
+// Z[rows[i]] = the sum of the slices' partials, in slice order.  A block
+// takes 256 columns of kSumRows rows.  rz != nullptr: it also writes, per
+// column, the sum over its rows of r_c z_c (the row's value in the source
+// Sv before the store, times the stored value) as row blockIdx.y of rz --
+// the coarse term of r . M^-1 r.
+constexpr int kSumRows = 16;
+
+template <typename T, typename PT>
+__global__ void __launch_bounds__(256)
+k_coarse_sum(int m, int k, int sp, const PT *__restrict__ part,
+             const int32_t *__restrict__ rows, const T *Sv, T *Z,
+             const int32_t *__restrict__ ctl, double *__restrict__ rz) {
   if (ctl && ctl_idle(ctl)) return;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= k) return;
   const int64_t mk = (int64_t)m * k;
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= mk) return;
-  const int i = (int)(id / k), j = (int)(id - (int64_t)i * k);
-  double s = part[id];
-  for (int z = 1; z < sp; ++z) s += part[(int64_t)z * mk + id];
-  Z[(int64_t)rows[i] * k + j] = (T)s;
+  const int i0 = blockIdx.y * kSumRows, i1 = min(m, i0 + kSumRows);
+  double e = 0.0;
+  for (int i = i0; i < i1; ++i) {
+    const int64_t id = (int64_t)i * k + j;
+    double sum = (double)part[id];
+    for (int z = 1; z < sp; ++z) sum += (double)part[(int64_t)z * mk + id];
+    const int64_t o = (int64_t)rows[i] * k + j;
+    const T zc = (T)sum;
+    if (rz) e += (double)Sv[o] * (double)zc;
+    Z[o] = zc;
+  }
+  if (rz) rz[(int64_t)blockIdx.y * k + j] = e;
 }
 
 // Tile shape: 128 x 128 (8 waves) for a large product, 64 x 64 (4 waves)
@@ -384,26 +423,46 @@ size_t coarse_product_work(int m, int k) {
   return (size_t)coarse_splits(m, k) * m * k;
 }
 
-template <typename T>
-void coarse_product(int m, int k, const double *M, const int32_t *rows, const T *S, T *Z,
-                    const int32_t *ctl, double *part, hipStream_t s) {
-  if (m <= 0 || k <= 0) return;
+template <typename T, typename MT>
+static int coarse_launch(int m, int k, const MT *M, const int32_t *rows, const T *S, T *Z,
+                         const int32_t *ctl, MT *part, hipStream_t s, double *rz) {
   const int sp = coarse_splits(m, k);
   const int chunk = ((m + sp - 1) / sp + kCK - 1) / kCK * kCK;
   if (coarse_big_tiles(m, k)) {
     const dim3 g((unsigned)((k + 127) / 128), (unsigned)((m + 127) / 128), (unsigned)sp);
-    k_coarse_mm<T, 128, 128, 4, 2><<<g, 512, 0, s>>>(m, k, chunk, M, rows, S, part, ctl);
+    k_coarse_mm<T, MT, 128, 128, 4, 2><<<g, 512, 0, s>>>(m, k, chunk, M, rows, S, part, ctl);
   } else {
     const dim3 g((unsigned)((k + 63) / 64), (unsigned)((m + 63) / 64), (unsigned)sp);
-    k_coarse_mm<T, 64, 64, 2, 2><<<g, 256, 0, s>>>(m, k, chunk, M, rows, S, part, ctl);
+    k_coarse_mm<T, MT, 64, 64, 2, 2><<<g, 256, 0, s>>>(m, k, chunk, M, rows, S, part, ctl);
   }
-  k_coarse_sum<T><<<grid_for((int64_t)m * k, 256), 256, 0, s>>>(m, k, sp, part, rows, Z, ctl);
+  const dim3 gs((unsigned)((k + 255) / 256), (unsigned)((m + kSumRows - 1) / kSumRows));
+  k_coarse_sum<T, MT><<<gs, 256, 0, s>>>(m, k, sp, part, rows, S, Z, ctl, rz);
+  return rz ? (int)gs.y : 0;
 }
-template void coarse_product<float>(int, int, const double *, const int32_t *, const float *,
-                                    float *, const int32_t *, double *, hipStream_t);
-template void coarse_product<double>(int, int, const double *, const int32_t *,
-                                     const double *, double *, const int32_t *, double *,
-                                     hipStream_t);
+
+// M32 != nullptr (FP32 applies only): the product takes M rounded to FP32.
+// Returns the partial rows written to rz (0 with rz = nullptr).
+template <typename T>
+int coarse_product(int m, int k, const double *M, const float *M32, const int32_t *rows,
+                   const T *S, T *Z, const int32_t *ctl, double *part, hipStream_t s,
+                   double *rz) {
+  if (m <= 0 || k <= 0) return 0;
+  if constexpr (sizeof(T) == 4) {
+    if (M32)
+      return coarse_launch<T, float>(m, k, M32, rows, S, Z, ctl,
+                                     reinterpret_cast<float *>(part), s, rz);
+  }
+  return coarse_launch<T, double>(m, k, M, rows, S, Z, ctl, part, s, rz);
+}
+
+int coarse_rz_rows(int m) { return (m + kSumRows - 1) / kSumRows; }
+
+template int coarse_product<float>(int, int, const double *, const float *, const int32_t *,
+                                   const float *, float *, const int32_t *, double *,
+                                   hipStream_t, double *);
+template int coarse_product<double>(int, int, const double *, const float *, const int32_t *,
+                                    const double *, double *, const int32_t *, double *,
+                                    hipStream_t, double *);
 
 }  // namespace cn
 

@@ -28,19 +28,26 @@
 namespace cn {
 
 size_t coarse_product_work(int m, int k);
+int coarse_rz_rows(int m);
 template <typename T>
-void coarse_product(int m, int k, const double *M, const int32_t *rows, const T *S, T *Z,
-                    const int32_t *ctl, double *part, hipStream_t s);
+int coarse_product(int m, int k, const double *M, const float *M32, const int32_t *rows,
+                   const T *S, T *Z, const int32_t *ctl, double *part, hipStream_t s,
+                   double *rz);
 
 // ---- wide form: LPR lanes per (row, LPR VEC column tile), 64 / LPR rows
 // per wavefront -----------------------------------------------------------
 // Z[Fg[f]] = dinv[f] src[Fg[f]] + sum_e pval[e] Z[pcol[e]]   (src = r or Z)
+// rz != nullptr: the block also writes its partial sums of d_f r_f^2 per
+// column (row blockIdx.x of rz): over the levels these and the coarse
+// rows' r_c . z_c add up to r . M^-1 r, since the apply is the symmetric
+// block factorisation M^-1 = L^T diag(D_F^-1 ..., S_c^-1) L whose L r the
+// down sweep leaves in the eliminated rows (the up pass's src).
 template <typename T, int VEC, int LPR>
 __global__ void __launch_bounds__(kBlock)
 kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv,
           const int32_t *__restrict__ pptr, const int32_t *__restrict__ pcol,
           const T *__restrict__ pval, int64_t k, const T *src, T *Z,
-          const int32_t *__restrict__ ctl) {
+          const int32_t *__restrict__ ctl, double *__restrict__ rz, int32_t rz_add) {
   if (ctl && ctl_idle(ctl)) return;
   using P = Pack<LPR>;
   const int64_t c0 = P::col0(VEC);
@@ -49,6 +56,7 @@ kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv
   // columns), only the store is not: a load consumed under a branch makes
   // the compiler wait for the previous row's store before the next row
   const int64_t cl = valid ? c0 : k - VEC;
+  double pd[1][VEC] = {};
   const RowIter ri = row_iter(0, nF, P::kSlots, P::slot());
   for (int32_t row = ri.first; row < ri.end; row += ri.stride) {
     const int32_t f = P::u(row);
@@ -57,10 +65,14 @@ kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv
     T acc[VEC];
     VecIO<T, VEC>::ld(src + g + cl, acc);
 #pragma unroll
-    for (int m = 0; m < VEC; ++m) acc[m] = d * acc[m];
+    for (int m = 0; m < VEC; ++m) {
+      pd[0][m] += (double)d * (double)acc[m] * (double)acc[m];
+      acc[m] = d * acc[m];
+    }
     gather_wide<T, VEC>(P::u(pptr[f]), P::u(pptr[f + 1]), pcol, pval, Z, k, cl, true, acc);
     if (valid) VecIO<T, VEC>::st(Z + g + c0, acc);
   }
+  if (rz) block_partials_wide<VEC, 1, LPR>(pd, rz, k, c0, 0, rz_add);
 }
 
 // ---- the down sweep (wide form) -----------------------------------------
@@ -309,27 +321,39 @@ static void level_down_n(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z
 #undef N_
 }
 
+// rz != nullptr: block b adds its r . z partials to row b of rz for
+// b < rz_rows (rows earlier launches wrote) and stores them past it.
+// Returns the rows of rz written so far (rz_rows, or the grid's blocks a
+// column tile if more).  The narrow form (k <= 4) takes rz = nullptr.
 template <typename T>
-static void level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
-                     const int32_t *ctl, hipStream_t s,
-                     int32_t level_cap) {
-  if (L.nF == 0) return;
+static int32_t level_up(const cn_hsc_t::Level &L, int64_t k, const T *src, T *Z,
+                        const int32_t *ctl, hipStream_t s, int32_t level_cap,
+                        double *rz = nullptr, int32_t rz_rows = 0) {
+  if (L.nF == 0) return rz_rows;
   const int v = hsc_vec<T>(k);
   if (v == 0) {
 #define N_(KK) k_hsc_up_n<T, KK><<<grid_for(L.nF, 256), 256, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L), L.p_ptr.get(), L.p_colg.get(), LvVals<T>::p(L), src, Z, ctl)
     switch (k) { case 1: N_(1); break; case 2: N_(2); break; case 3: N_(3); break; default: N_(4); }
 #undef N_
-    return;
+    return 0;
   }
   const int lpr = pack_lpr12((k + v - 1) / v);
   const dim3 g = hsc_grid(L.nF, k, v, lpr, level_cap);
-#define U_(VV, LL) kw_hsc_up<T, VV, LL><<<g, kBlock, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L), L.p_ptr.get(), L.p_colg.get(), LvVals<T>::p(L), k, src, Z, ctl)
+#define U_(VV, LL) kw_hsc_up<T, VV, LL><<<g, kBlock, 0, s>>>(L.nF, L.Fg.get(), LvVals<T>::d(L), L.p_ptr.get(), L.p_colg.get(), LvVals<T>::p(L), k, src, Z, ctl, rz, rz_rows)
+  const int32_t rows = rz ? std::max(rz_rows, (int32_t)g.x) : 0;
   if constexpr (sizeof(T) == 4) {
-    if (v == 4) { HSC_LPR(lpr, U_, 4); return; }
+    if (v == 4) { HSC_LPR(lpr, U_, 4); return rows; }
   }
   if (v >= 2) { HSC_LPR(lpr, U_, 2); } else { HSC_LPR(lpr, U_, 1); }
 #undef U_
+  return rows;
 }
+
+// Partial rows an apply of k columns writes for r . M^-1 r (hsc_apply with
+// rz_part), or -1 where it computes none (the narrow form k <= 4, or the
+// fused tail).
+template <typename T>
+int32_t hsc_rz_rows(const cn_hsc_t *H, int64_t k, const HscShape &sh);
 
 // the rows `rows` (level-0 numbers) from level a to the ends of their
 // chains at level m
@@ -608,9 +632,27 @@ static size_t apply_work_bytes(const cn_hsc_t *H, int64_t k) {
 
 // ext_work: a caller's workspace of at least apply_work_bytes (the
 // hierarchy's own buffer, grown on demand, otherwise)
+// rz_part != nullptr (only where hsc_rz_rows is not -1): the apply also
+// writes the FP64 partial sums of r . z per column, hsc_rz_rows rows of k:
+// the coarse rows' r_c z_c, then every up-pass block's d_f r_f^2 added
+// into row blockIdx.x, levels from the coarsest down.
+template <typename T>
+int32_t hsc_rz_rows(const cn_hsc_t *H, int64_t k, const HscShape &sh) {
+  const int v = hsc_vec<T>(k);
+  if (v == 0 || tail_level<T>(H, k, sh.tail_rows) >= 0) return -1;
+  int32_t rows = H->nc > 0 ? coarse_rz_rows(H->nc) : 0;
+  const int lpr = pack_lpr12((k + v - 1) / v);
+  for (const cn_hsc_t::Level &L : H->lv)
+    if (L.nF > 0) rows = std::max(rows, (int32_t)hsc_grid(L.nF, k, v, lpr, sh.level_cap).x);
+  return rows;
+}
+template int32_t hsc_rz_rows<float>(const cn_hsc_t *, int64_t, const HscShape &);
+template int32_t hsc_rz_rows<double>(const cn_hsc_t *, int64_t, const HscShape &);
+
 template <typename T>
 void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const int32_t *ctl,
-               const HscShape &sh, unsigned char *ext_work = nullptr) {
+               const HscShape &sh, unsigned char *ext_work = nullptr,
+               double *rz_part = nullptr) {
   const int32_t L = (int32_t)H->lv.size();
   const size_t bytes = apply_work_bytes<T>(H, k);
   unsigned char *work = ext_work;
@@ -623,6 +665,8 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
   }
   T *W = reinterpret_cast<T *>(work);
   const int32_t l0 = tail_level<T>(H, k, sh.tail_rows);
+  if (l0 >= 0 || hsc_vec<T>(k) == 0) rz_part = nullptr;  // hsc_rz_rows is -1 there
+  int32_t rzr = 0;  // rows of rz_part written so far
   if (l0 >= 0) {
     const TailLevel<T> *desc = tail_desc<T>(H, s);
     {
@@ -665,20 +709,23 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
       switch (k) { case 1: G_(1); break; case 2: G_(2); break; case 3: G_(3); break; default: G_(4); }
 #undef G_
     } else {
-      coarse_product<T>(H->nc, (int)k, H->coarse_inv.get(), H->coarse_rows.get(),
-                        L == 0 ? r : z, z, ctl, reinterpret_cast<double *>(work), s);
+      const float *M32 = sizeof(T) == 4 && sh.coarse_prec == 2 ? H->coarse_inv32.get() : nullptr;
+      rzr = coarse_product<T>(H->nc, (int)k, H->coarse_inv.get(), M32, H->coarse_rows.get(),
+                               L == 0 ? r : z, z, ctl, reinterpret_cast<double *>(work), s,
+                               rz_part);
     }
   }
   {
     ProfScope ps("hsc_up", s);
     for (int32_t l = L - 1; l >= 0; --l)
-      level_up<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s, sh.level_cap);
+      rzr = level_up<T>(H->lv[l], k, l == 0 ? r : z, z, ctl, s, sh.level_cap, rz_part, rzr);
   }
 }
 template void hsc_apply<float>(cn_hsc_t *, int64_t, const float *, float *, hipStream_t,
-                               const int32_t *, const HscShape &, unsigned char *);
+                               const int32_t *, const HscShape &, unsigned char *, double *);
 template void hsc_apply<double>(cn_hsc_t *, int64_t, const double *, double *, hipStream_t,
-                                const int32_t *, const HscShape &, unsigned char *);
+                                const int32_t *, const HscShape &, unsigned char *,
+                                double *);
 
 // ---- workspace of the apply that allocates nothing (cn_hsc_apply_noalloc)
 // A fixed array in device memory, one copy per GPU the library runs on,
@@ -793,7 +840,19 @@ static void build_chains(cn_hsc_t *H, hipStream_t s) {
   H->ch_ready = true;
 }
 
+__global__ void k_round_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
 void hsc_prepare(cn_hsc_t *H, hipStream_t s) {
+  if (H->nc > 0) {
+    const int64_t nn = (int64_t)H->nc * H->nc;
+    H->coarse_inv32.alloc((size_t)nn);
+    k_round_f32<<<grid_for(nn, 256), 256, 0, s>>>(nn, H->coarse_inv.get(),
+                                                 H->coarse_inv32.get());
+    CN_HIP(hipGetLastError());
+  }
   (void)tail_desc<float>(H, s);
   (void)tail_desc<double>(H, s);
   build_chains(H, s);

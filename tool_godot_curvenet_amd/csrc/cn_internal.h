@@ -90,6 +90,7 @@ struct HscShape {
   int32_t level_cap = 0;
   int32_t tail_rows = 0;
   int32_t chain_levels = 0;
+  int32_t coarse_prec = 0;  // 2: FP32 coarse product in FP32 applies
 };
 
 // ----------------------------------------------------------- topology --
@@ -310,6 +311,7 @@ struct cn_hsc_t {
   int32_t nc = 0;                   // coarse rows
   cn::DevBuf<int32_t> coarse_rows;  // their level-0 rows
   cn::DevBuf<double> coarse_L, coarse_inv;
+  cn::DevBuf<float> coarse_inv32;   // coarse_inv rounded to FP32
   int64_t nnz = 0;
   // apply workspace: the gathered coarse rows (nc x k), grown on demand
   cn::DevBuf<unsigned char> work;

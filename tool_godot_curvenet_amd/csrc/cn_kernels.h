@@ -247,11 +247,15 @@ inline int pack_lpr12(int64_t lanes) {
 // the wave's 64 / LPR row groups (waves outer, groups inner: a fixed order;
 // for LPR = 12 the idle lanes 60-63 form no group).
 // `sh` holds kWaves * NP * 64 * VEC doubles of shared memory.
+// Blocks blockIdx.x < add_below add their sums to the row instead of
+// storing them (the row holds an earlier launch's partials: launches in
+// stream order accumulate into one set of rows, in launch order).
 template <int VEC, int NP, int LPR = 64>
 __device__ __forceinline__ void block_partials_wide_in(double (&p)[NP][VEC],
                                                        double *__restrict__ out,
                                                        int64_t k, int64_t c0,
-                                                       int64_t stride_np, double *sh) {
+                                                       int64_t stride_np, double *sh,
+                                                       int32_t add_below = 0) {
   constexpr int TW = 64 * VEC, G = 64 / LPR;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -272,7 +276,10 @@ __device__ __forceinline__ void block_partials_wide_in(double (&p)[NP][VEC],
 #pragma unroll
           for (int g = 0; g < G; ++g)
             if (ww + g > 0) s += sh[ww * NP * TW + g * LPR * VEC + o];
-        if (c < k) out[q * stride_np + (int64_t)blockIdx.x * k + c] = s;
+        if (c < k) {
+          double *o_ = out + q * stride_np + (int64_t)blockIdx.x * k + c;
+          *o_ = (int32_t)blockIdx.x < add_below ? *o_ + s : s;
+        }
       }
   }
 }
@@ -281,9 +288,10 @@ template <int VEC, int NP, int LPR = 64>
 __device__ __forceinline__ void block_partials_wide(double (&p)[NP][VEC],
                                                     double *__restrict__ out,
                                                     int64_t k, int64_t c0,
-                                                    int64_t stride_np) {
+                                                    int64_t stride_np,
+                                                    int32_t add_below = 0) {
   __shared__ double sh[kWaves * NP * 64 * VEC];
-  block_partials_wide_in<VEC, NP, LPR>(p, out, k, c0, stride_np, sh);
+  block_partials_wide_in<VEC, NP, LPR>(p, out, k, c0, stride_np, sh, add_below);
 }
 
 // narrow: lane g < K of every 8-lane group holds the partials of column g;

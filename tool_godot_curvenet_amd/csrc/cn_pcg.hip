@@ -2080,7 +2080,10 @@ static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
 
 template <typename T>
 void hsc_apply(cn_hsc_t *H, int64_t k, const T *r0, T *z0, hipStream_t s, const int32_t *ctl,
-               const HscShape &sh, unsigned char *ext_work = nullptr);
+               const HscShape &sh, unsigned char *ext_work = nullptr,
+               double *rz_part = nullptr);
+template <typename T>
+int32_t hsc_rz_rows(const cn_hsc_t *H, int64_t k, const HscShape &sh);
 template <typename T>
 void sgs_apply(const cn_sgs_t *G, int64_t k, const T *r, T *z, hipStream_t s,
                const int32_t *ctl);
@@ -2187,7 +2190,11 @@ static LevelGrid level_grid(const SysView<T> &V, int64_t k, int32_t nb, bool bwd
 // or z = r / d (Jacobi); partials r.r -> part[0..rows) and r.M^-1.r ->
 // part[stride..].  Returns the partial rows written.
 // no_rupd: the residual update and its r.r partials were done already (the
-// LDS-staged KL_UPD product); only the preconditioner and r.z run.
+// LDS-staged KL_UPD product); only the preconditioner and r.z run, and the
+// return value is the r.z partial rows alone.  The hierarchical apply then
+// forms r.z from its own passes (r . M^-1 r = sum over the eliminated rows
+// of d_f (L r)_f^2 plus the coarse rows' r_c z_c) instead of reading r and z
+// again.
 template <typename T, bool FUSED>
 static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
                           const T *Q, T *R, T *Z, double *part, int64_t stride,
@@ -2202,7 +2209,9 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
 #undef N_
 #undef W_
     if (V.hsc) {
-      hsc_apply<T>(V.hsc, k, R, Z, s, ctl, V.hsh);
+      const int32_t er = no_rupd && !V.sm.of_row ? hsc_rz_rows<T>(V.hsc, k, V.hsh) : -1;
+      hsc_apply<T>(V.hsc, k, R, Z, s, ctl, V.hsh, nullptr, er >= 0 ? part + stride : nullptr);
+      if (er >= 0) return er;
     } else if (V.sgs) {
       sgs_apply<T>(V.sgs, k, R, Z, s, ctl);
     } else {
@@ -2586,6 +2595,7 @@ static void use_shapes(cn_system_t *S, SysView<T> &V, int64_t k, hipStream_t s) 
   V.hsh.level_cap = S->tune.level_cap;
   V.hsh.tail_rows = S->tune.tail_rows;
   V.hsh.chain_levels = S->tune.chain_levels;
+  V.hsh.coarse_prec = S->tune.coarse_prec;
   if (S->tune.spmm_form != 1) use_win<T>(S, V, k, s);
 }
 template <>
@@ -2681,6 +2691,9 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   need = std::max<int32_t>(need, grid_rows<T>(n, k, kCapSpmm).rows);
   need = std::max<int32_t>(need, grid_rows<double>(n, k, kCapSpmm).rows);
   need = std::max<int32_t>(need, S->seg_blocks);
+  if (H) need = std::max<int32_t>(need, hsc_rz_rows<T>(H, k, HscShape{S->tune.level_cap,
+                                                                     S->tune.tail_rows,
+                                                                     S->tune.chain_levels}));
   SolverWork *W = ensure_work(S, k, need);
   k_dinv<<<grid_for(n, 256), 256, 0, s>>>(n, S->diag.get(), W->dinv64.get(),
                                          W->dinv32.get());
@@ -2922,7 +2935,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
           F ? F->serial : 0, G ? G->serial : 0, (uint64_t)V.rpr,
           (uint64_t)(uintptr_t)V.wpk, (uint64_t)(uintptr_t)V.ecol, (uint64_t)V.gens,
           (uint64_t)(uint32_t)V.hsh.level_cap, (uint64_t)(uint32_t)V.hsh.tail_rows,
-          (uint64_t)(uint32_t)V.hsh.chain_levels, (uint64_t)klf, (uint64_t)cur,
+          (uint64_t)(uint32_t)V.hsh.chain_levels, (uint64_t)(uint32_t)V.hsh.coarse_prec,
+          (uint64_t)klf, (uint64_t)cur,
           bits_of(delta2), bits_of(floor2)};
       for (auto &e : W->graphs)
         if (e.first == key) return e.second;
@@ -3145,7 +3159,8 @@ cn_status cn_system_set_tune(cn_system S, const cn_tune *t) {
   if (!S || !t || t->spmm_form < 0 || t->spmm_form > 2 || t->spmm_gens < 0 ||
       t->spmm_gens > (int32_t)kSpmmGens || t->spmm_gens == 3 || t->level_cap < 0 ||
       t->level_cap > 65536 || t->tail_rows < -1 || t->tail_rows > 16384 ||
-      t->chain_levels < 0 || t->chain_levels > 24) {
+      t->chain_levels < 0 || t->chain_levels > 24 || t->coarse_prec < 0 ||
+      t->coarse_prec > 2) {
     set_error("cn_system_set_tune: bad arguments");
     return CN_BAD_ARG;
   }
