@@ -267,6 +267,9 @@ void dense_cholesky_solve(int n, int k, const double *L, int64_t lda, double *B,
 // The FP32 apply can take M rounded to FP32 (MT = float): the same tiles
 // on v_mfma_f32_16x16x4f32, at twice the FP64 matrix rate, FP32 partial
 // sums (a slice of j), added in FP64 by k_coarse_sum.
+// The j range goes through LDS in chunks of kCK = 16.  For FP32, chunks of
+// 32, or a second chunk of look-ahead in registers, took the kernel from 7 to
+// 4-5 waves a SIMD (registers, LDS) and ran 3-10 % slower.
 constexpr int kCK = 16;
 constexpr int kCMaxChunk = 2048;  // longest slice (LDS row-number stage)
 
@@ -316,9 +319,7 @@ k_coarse_mm(int m, int k, int kchunk, const MT *__restrict__ M,
   const int ar = tid / (kCK / AE), ac = (tid % (kCK / AE)) * AE;
   const int br = tid / (BN / BE), bc = (tid % (BN / BE)) * BE;
   const MT *Mrow = M + (int64_t)min(bi + ar, m - 1) * m;
-  int colc[BE];
-#pragma unroll
-  for (int u = 0; u < BE; ++u) colc[u] = min(bj + bc + u, k - 1);
+  const int cb = bj + bc;  // the thread's first S column (clamped below)
   for (int q = tid; q < kchunk + kCK; q += NT) Rs[q] = rows[min(jb + q, m - 1)];
   __syncthreads();
   MT pa[AE];
@@ -328,7 +329,7 @@ k_coarse_mm(int m, int k, int kchunk, const MT *__restrict__ M,
     for (int u = 0; u < AE; ++u) pa[u] = Mrow[min(j0 + ac + u, m - 1)];
     const T *Srow = S + (int64_t)Rs[j0 - jb + br] * k;
 #pragma unroll
-    for (int u = 0; u < BE; ++u) pb[u] = Srow[colc[u]];
+    for (int u = 0; u < BE; ++u) pb[u] = Srow[min(cb + u, k - 1)];
   };
   typename F::acc_t acc[TM][TN];
 #pragma unroll
@@ -371,11 +372,12 @@ k_coarse_mm(int m, int k, int kchunk, const MT *__restrict__ M,
 }
 
 // Z[rows[i]] = the sum of the slices' partials, in slice order.  A block
-// takes 256 columns of kSumRows rows.  rz != nullptr: it also writes, per
-// column, the sum over its rows of r_c z_c (the row's value in the source
-// Sv before the store, times the stored value) as row blockIdx.y of rz --
-// the coarse term of r . M^-1 r.
-constexpr int kSumRows = 16;
+// takes 256 columns of kSumRows rows; a thread's rows are loaded together,
+// slice by slice.  rz != nullptr: it also writes, per column, the sum over
+// its rows of r_c z_c (the row's value in the source Sv before the store,
+// times the stored value) as row blockIdx.y of rz -- the coarse term of
+// r . M^-1 r.
+constexpr int kSumRows = 4;
 
 template <typename T, typename PT>
 __global__ void __launch_bounds__(256)
@@ -386,16 +388,27 @@ k_coarse_sum(int m, int k, int sp, const PT *__restrict__ part,
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= k) return;
   const int64_t mk = (int64_t)m * k;
-  const int i0 = blockIdx.y * kSumRows, i1 = min(m, i0 + kSumRows);
+  const int i0 = blockIdx.y * kSumRows;
+  // rows past m repeat row m - 1 and are not stored
+  int64_t id[kSumRows], o[kSumRows];
+  double sum[kSumRows];
+#pragma unroll
+  for (int u = 0; u < kSumRows; ++u) {
+    const int i = min(i0 + u, m - 1);
+    id[u] = (int64_t)i * k + j;
+    o[u] = (int64_t)rows[i] * k + j;
+    sum[u] = (double)part[id[u]];
+  }
+  for (int z = 1; z < sp; ++z)
+#pragma unroll
+    for (int u = 0; u < kSumRows; ++u) sum[u] += (double)part[(int64_t)z * mk + id[u]];
   double e = 0.0;
-  for (int i = i0; i < i1; ++i) {
-    const int64_t id = (int64_t)i * k + j;
-    double sum = (double)part[id];
-    for (int z = 1; z < sp; ++z) sum += (double)part[(int64_t)z * mk + id];
-    const int64_t o = (int64_t)rows[i] * k + j;
-    const T zc = (T)sum;
-    if (rz) e += (double)Sv[o] * (double)zc;
-    Z[o] = zc;
+#pragma unroll
+  for (int u = 0; u < kSumRows; ++u) {
+    if (i0 + u >= m) break;
+    const T zc = (T)sum[u];
+    if (rz) e += (double)Sv[o[u]] * (double)zc;
+    Z[o[u]] = zc;
   }
   if (rz) rz[(int64_t)blockIdx.y * k + j] = e;
 }
@@ -427,6 +440,8 @@ template <typename T, typename MT>
 static int coarse_launch(int m, int k, const MT *M, const int32_t *rows, const T *S, T *Z,
                          const int32_t *ctl, MT *part, hipStream_t s, double *rz) {
   const int sp = coarse_splits(m, k);
+  // a slice is at most kCMaxChunk - kCK rows, so the rounded chunk plus one
+  // chunk of look-ahead fits the kernel's row-number stage
   const int chunk = ((m + sp - 1) / sp + kCK - 1) / kCK * kCK;
   if (coarse_big_tiles(m, k)) {
     const dim3 g((unsigned)((k + 127) / 128), (unsigned)((m + 127) / 128), (unsigned)sp);
