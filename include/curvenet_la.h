@@ -717,6 +717,10 @@ cn_status cn_st_write_file(const char *path, int32_t n, const cn_st_tensor *tens
  * Returns the scopes timed; *ms_total gets their summed duration. */
 void cn_profile_enable(int32_t on);
 int32_t cn_profile_query(const char *name, double *ms_total);
+/* Record events only around the scopes of this name (NULL or "": all).
+ * Each recorded event pair leaves the GPU idle a few microseconds, so a
+ * timed run that needs one kernel's duration records only that kernel. */
+void cn_profile_only(const char *name);
 
 #ifdef __cplusplus
 }

@@ -196,6 +196,7 @@ _SIGS = {
     "cn_bind_weights": (i32, [vp, i32, i32, vp, vp, i64, vp]),
     "cn_profile_enable": (None, [i32]),
     "cn_profile_query": (i32, [C.c_char_p, C.POINTER(f64)]),
+    "cn_profile_only": (None, [C.c_char_p]),
 }
 
 _lib = None

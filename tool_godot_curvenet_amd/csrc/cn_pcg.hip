@@ -3195,6 +3195,8 @@ void cn_profile_enable(int32_t on) {
   prof_reset();
 }
 
+void cn_profile_only(const char *name) { g_prof.only = name ? name : ""; }
+
 int32_t cn_profile_query(const char *name, double *ms_total) {
   auto it = g_prof.result.find(name ? name : "");
   if (it == g_prof.result.end()) {

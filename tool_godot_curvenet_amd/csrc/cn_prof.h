@@ -17,6 +17,7 @@ namespace cn {
 
 struct Prof {
   bool on = false;
+  std::string only;  // non-empty: only scopes of this name record events
   std::vector<hipEvent_t> pool;
   struct Rec { std::string name; hipEvent_t a, b; };
   std::vector<Rec> recs;
@@ -34,7 +35,7 @@ struct ProfScope {
   hipStream_t s;
   ProfScope(const char *name, hipStream_t st) : s(st) {
     Prof &P = g_prof;
-    if (!P.on) return;
+    if (!P.on || (!P.only.empty() && P.only != name)) return;
     if (P.used + 2 > P.pool.size()) {
       if (P.pool.size() >= 65536) return;
       size_t add = std::max<size_t>(256, P.pool.size());
