@@ -2191,14 +2191,17 @@ static LevelGrid level_grid(const SysView<T> &V, int64_t k, int32_t nb, bool bwd
 // part[stride..].  Returns the partial rows written.
 // no_rupd: the residual update and its r.r partials were done already (the
 // LDS-staged KL_UPD product); only the preconditioner and r.z run, and the
-// return value is the r.z partial rows alone.  The hierarchical apply then
-// forms r.z from its own passes (r . M^-1 r = sum over the eliminated rows
-// of d_f (L r)_f^2 plus the coarse rows' r_c z_c) instead of reading r and z
-// again.
+// return value is the r.z partial rows alone.  rz_rows != nullptr: the r.z
+// partials may have a row count of their own, returned there.  In both cases
+// the hierarchical apply forms r.z from its own passes (r . M^-1 r = sum over
+// the eliminated rows of d_f (L r)_f^2 plus the coarse rows' r_c z_c)
+// instead of reading r and z again.
 template <typename T, bool FUSED>
 static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
                           const T *Q, T *R, T *Z, double *part, int64_t stride,
-                          const int32_t *ctl, hipStream_t s, bool no_rupd = false) {
+                          const int32_t *ctl, hipStream_t s, bool no_rupd = false,
+                          int32_t *rz_rows = nullptr) {
+  if (rz_rows) *rz_rows = -1;  // -1: the r.z partials have the returned rows
   if (V.hsc || V.sgs || V.sm.of_row) {
     // (no scope of its own: each event pair around a launch group leaves a
     // few microseconds idle, so only the parts inside are timed)
@@ -2209,9 +2212,14 @@ static int32_t launch_fwd(const SysView<T> &V, int64_t k, const double *alpha,
 #undef N_
 #undef W_
     if (V.hsc) {
-      const int32_t er = no_rupd && !V.sm.of_row ? hsc_rz_rows<T>(V.hsc, k, V.hsh) : -1;
+      const int32_t er =
+          (no_rupd || rz_rows) && !V.sm.of_row ? hsc_rz_rows<T>(V.hsc, k, V.hsh) : -1;
       hsc_apply<T>(V.hsc, k, R, Z, s, ctl, V.hsh, nullptr, er >= 0 ? part + stride : nullptr);
-      if (er >= 0) return er;
+      if (er >= 0) {
+        if (no_rupd) return er;
+        *rz_rows = er;
+        return G.rows;
+      }
     } else if (V.sgs) {
       sgs_apply<T>(V.sgs, k, R, Z, s, ctl);
     } else {
@@ -2858,15 +2866,19 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
       if (klf && pending) launch_bwd<T>(V, k, st, Z, Pb[cur], D, ctl, s);
       pending = false;
       // z = M^-1 r; rz; continuing columns keep p, restarted ones take p = z
-      rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, s);
+      int32_t rzr = 0;
+      rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, s,
+                                  false, batch ? nullptr : &rzr);
       if (batch) {
         launch_segdot<T, T>(S, W, k, R, R, R, Z, s);
         k_reduce_beta<<<red, kBlock, 0, s>>>(1, 1, kk, sg0, sg1, st, W->active.get(),
                                              W->restart.get(), ctl, 1, delta2, floor2);
       } else {
-        rd = reduce_rows(W, p1, rows, k, stride, 2, s);
-        k_reduce_beta<<<red, kBlock, 0, s>>>(rd.rows, rd.rows, k, rd.p, rd.p + rd.stride, st,
-                                             W->active.get(), W->restart.get(), ctl, 1, delta2, floor2);
+        Red b1 = reduce_rows(W, p1, rows, k, stride, 1, s, 0);
+        Red b2 = reduce_rows(W, p1 + stride, rzr >= 0 ? rzr : rows, k, stride, 1, s, 1);
+        k_reduce_beta<<<red, kBlock, 0, s>>>(b1.rows, b2.rows, k, b1.p, b2.p, st,
+                                             W->active.get(), W->restart.get(), ctl, 1, delta2,
+                                             floor2);
       }
       if (klf) pending = true;   // p = gamma z + beta p in the first KL_DIR
       else launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
@@ -2913,15 +2925,19 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
         Red r1 = reduce_rows(W, p0, rw, k, stride, 1, q);
         k_reduce_alpha<<<red, kBlock, 0, q>>>(r1.rows, k, r1.p, st, W->active.get(), ctl);
       }
-      rw = launch_fwd<T, true>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, q);
+      int32_t rzr = 0;
+      rw = launch_fwd<T, true>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, q, false,
+                               batch ? nullptr : &rzr);
       if (batch) {
         launch_segdot<T, T>(S, W, k, R, R, R, Z, q);
         k_reduce_beta<<<red, kBlock, 0, q>>>(1, 1, kk, sg0, sg1, st, W->active.get(),
                                              W->restart.get(), ctl, 0, delta2, floor2);
       } else {
-        Red r2 = reduce_rows(W, p1, rw, k, stride, 2, q);
-        k_reduce_beta<<<red, kBlock, 0, q>>>(r2.rows, r2.rows, k, r2.p, r2.p + r2.stride, st,
-                                             W->active.get(), W->restart.get(), ctl, 0, delta2, floor2);
+        Red b1 = reduce_rows(W, p1, rw, k, stride, 1, q, 0);
+        Red b2 = reduce_rows(W, p1 + stride, rzr >= 0 ? rzr : rw, k, stride, 1, q, 1);
+        k_reduce_beta<<<red, kBlock, 0, q>>>(b1.rows, b2.rows, k, b1.p, b2.p, st,
+                                             W->active.get(), W->restart.get(), ctl, 0, delta2,
+                                             floor2);
       }
       launch_bwd<T>(V, k, st, Z, P, D, ctl, q);
     };
