@@ -388,8 +388,9 @@ def main():
             nbwd = prof["bwd"][0]
             kern["pd_update"] = ("kw_pd_update<float,4,64>", nbwd, prof["bwd"][1],
                                  (nbwd if fused else nwork) * 5 * nk4)
-            # r.z after every apply that does work: reads R and Z
-            kern["dot"] = ("kw_dot<float,4,64>", prof["dot"][0], prof["dot"][1], nwork * 2 * nk4)
+            # r.z where the apply does not form it from its passes: reads R and Z
+            kern["dot"] = ("kw_dot<float,4,64>", prof["dot"][0], prof["dot"][1],
+                           prof["dot"][0] * 2 * nk4)
             nc = pre.info.n_coarse
             cms = prof["hsc_coarse"][1]
             ncoarse = prof["hsc_coarse"][0]
@@ -427,6 +428,9 @@ def main():
                             "avg_ms": ms / max(launches, 1),
                             "flops_per_launch": flops / max(launches, 1),
                             "achieved_tflops": tf, "peak": peak, "frac": tf / peak}
+        # kernels this workload does not launch (the fused iteration has no
+        # kw_pd_update; the apply forms r.z itself) are left out
+        kernels = {t: v for t, v in kernels.items() if v["launches"] > 0}
         scope = {t: t for t in kernels}
         scope["pd_update"] = "bwd"
         if "spmm" in kernels:
