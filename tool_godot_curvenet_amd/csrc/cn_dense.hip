@@ -304,8 +304,10 @@ k_coarse_mm(int m, int k, int kchunk, const MT *__restrict__ M,
   constexpr int BE = kCK * BN / NT;  // S values a thread stages per chunk
   static_assert(AE >= 1 && BE >= 1 && kCK % AE == 0 && BN % BE == 0, "tile shape");
   using F = CoarseMfma<MT>;
-  __shared__ MT As[BM][kCK + 1];
-  __shared__ T Bs[kCK][BN + 1];
+  // row strides that spread a wave's operand reads (16 rows x 4 k of A,
+  // 4 k x 16 columns of B) over distinct LDS banks
+  __shared__ MT As[BM][kCK + (sizeof(MT) == 4 ? 4 : 1)];
+  __shared__ T Bs[kCK][BN + 16];
   __shared__ int32_t Rs[kCMaxChunk + kCK];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
   // XCD-contiguous tile order (x fastest, then y, then the slice z)
