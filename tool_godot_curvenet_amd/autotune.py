@@ -199,7 +199,7 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
                     best, cur = t, {**cur, "tail_rows": rows}
         # 4: the down sweep's chain length (the wide form, k > 4)
         if k > 4:
-            for g in (2, 4):
+            for g in (2, 4, 5, 6):
                 t = run({**cur, "chain_levels": g})
                 if t < best:
                     best, cur = t, {**cur, "chain_levels": g}

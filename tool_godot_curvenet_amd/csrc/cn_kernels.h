@@ -45,8 +45,11 @@ constexpr int kEll = 7;
 // set when a column's iterated residual has fallen delta^4 below the last
 // true one (far past the delta^2 trigger), so a fast-converging solve does
 // not run on in FP32 between two host checks.
+// CTL_DFRESH (the fused iteration): the round's d holds nothing yet, so its
+// first d += alpha p reads no d and the residual replacement adds no d; the
+// first iteration's r.z reduction clears it (d is never zeroed in memory).
 enum { CTL_ACTIVE = 0, CTL_BREAKDOWN = 1, CTL_ONE = 2, CTL_TRIGGER = 3, CTL_JUST = 4,
-       CTL_ITERS = 5, CTL_PAUSE = 6, CTL_N = 8 };
+       CTL_ITERS = 5, CTL_PAUSE = 6, CTL_DFRESH = 7, CTL_N = 8 };
 __device__ __forceinline__ bool ctl_idle(const int32_t *ctl) {
   return ctl[CTL_ACTIVE] == 0 || ctl[CTL_PAUSE] != 0;
 }

@@ -349,10 +349,17 @@ kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
     ib[u] = reinterpret_cast<const IW *>(wpk + (int64_t)gi * IB)[iw];
   };
   // own-row r and d of compute chunk t + D, after compute(t) read the buffer
+  // (d is 0 in a round's first iteration, which reads none)
+  const bool dfresh = MODE == KL_UPD && ctl[CTL_DFRESH] != 0;
   auto issue_rd = [&](int32_t t, int u) {
     const int32_t row = min(max(row_of((t + D) * kLdsRows + rsub), 0), n - 1);
     VecIO<T, VEC>::ld(Rv + (int64_t)row * k + cl, rb[u]);
-    VecIO<T, VEC>::ld(Dv + (int64_t)row * k + cl, db[u]);
+    if (dfresh) {
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) db[u][m] = T(0);
+    } else {
+      VecIO<T, VEC>::ld(Dv + (int64_t)row * k + cl, db[u]);
+    }
   };
   auto land = [&](int32_t t, int u) {
     const int32_t g = t + kLdsAhead;
@@ -1414,6 +1421,8 @@ k_reduce_beta(int32_t nrows, int32_t nrows_rz, int64_t k, const double *__restri
               int32_t *__restrict__ active, const int32_t *__restrict__ restart,
               int32_t *__restrict__ ctl, int mode, double delta2, double floor2) {
   if (ctl_idle(ctl)) return;
+  // an iteration has run its d update: the round's d holds it from now on
+  if (mode == 0 && blockIdx.x == 0 && threadIdx.x == 0) ctl[CTL_DFRESH] = 0;
   __shared__ double sh[4][64];
   const int64_t c = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   double rr = col_reduce(prr, nrows, k, c, sh);
@@ -1632,13 +1641,18 @@ __global__ void k_scale_in(int64_t total, int64_t k, SegMap sm, const double *__
 // X += invs_c * D; D = 0.  init: X is not read (the first update of a
 // solve from x0 = 0, whose X was never written), the same fma from 0.
 template <typename T>
+// dfresh != nullptr (the fused iteration): d is left as it is, and a round
+// whose d holds nothing yet (*dfresh != 0) adds none.
 __global__ void k_x_update(int64_t total, int64_t k, SegMap sm, const double *__restrict__ invs,
-                           T *__restrict__ D, double *__restrict__ X, int init) {
+                           T *__restrict__ D, double *__restrict__ X, int init,
+                           const int32_t *__restrict__ dfresh) {
+  const bool skip = dfresh && *dfresh != 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t row = i / k;
-    X[i] = fma(invs[sm.vc(row, k, i - row * k)], (double)D[i], init ? 0.0 : X[i]);
-    D[i] = T(0);
+    const double d = skip ? 0.0 : (double)D[i];
+    X[i] = fma(invs[sm.vc(row, k, i - row * k)], d, init ? 0.0 : X[i]);
+    if (!dfresh) D[i] = T(0);
   }
 }
 
@@ -1673,7 +1687,9 @@ kw_scale_in(int32_t n, int64_t k, SegMap sm, const double *__restrict__ sc,
 template <typename T, int VEC, int LPR>
 __global__ void __launch_bounds__(kBlock)
 kw_x_update(int32_t n, int64_t k, SegMap sm, const double *__restrict__ invs,
-            T *__restrict__ D, double *__restrict__ X, int init) {
+            T *__restrict__ D, double *__restrict__ X, int init,
+            const int32_t *__restrict__ dfresh) {
+  const bool skip = dfresh && *dfresh != 0;
   WIDE_COLS_P(VEC, LPR)
   double iv[VEC];
 #pragma unroll
@@ -1694,14 +1710,19 @@ kw_x_update(int32_t n, int64_t k, SegMap sm, const double *__restrict__ invs,
       } else {
         VecIO<double, VEC>::ld(X + o, xv);
       }
-      VecIO<T, VEC>::ld(D + o, dv);
+      if (skip) {
+#pragma unroll
+        for (int m = 0; m < VEC; ++m) dv[m] = T(0);
+      } else {
+        VecIO<T, VEC>::ld(D + o, dv);
+      }
 #pragma unroll
       for (int m = 0; m < VEC; ++m) {
         xv[m] = fma(iv[m], (double)dv[m], xv[m]);
         dv[m] = T(0);
       }
       VecIO<double, VEC>::st(X + o, xv);
-      VecIO<T, VEC>::st(D + o, dv);
+      if (!dfresh) VecIO<T, VEC>::st(D + o, dv);
     }
   }
 }
@@ -2403,12 +2424,13 @@ static void launch_scale_in(const SysView<T> &V, int64_t k, const double *sc, co
 
 template <typename T>
 static void launch_x_update(const SysView<T> &V, int64_t k, const double *invs, T *D,
-                            double *X, hipStream_t s, bool init = false) {
+                            double *X, hipStream_t s, bool init = false,
+                            const int32_t *dfresh = nullptr) {
   const int64_t total = (int64_t)V.n * k;
   const Grid G = grid_rows<T>(V.n, k, kCapSpmm);
   const int ini = init ? 1 : 0;
-#define N_(KK) k_x_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, V.sm, invs, D, X, ini)
-#define W_(VV, LL) kw_x_update<T, WideV<T, VV>::v, LL><<<G.g, kBlock, 0, s>>>(V.n, k, V.sm, invs, D, X, ini)
+#define N_(KK) k_x_update<T><<<ew_blocks(total), 256, 0, s>>>(total, k, V.sm, invs, D, X, ini, dfresh)
+#define W_(VV, LL) kw_x_update<T, WideV<T, VV>::v, LL><<<G.g, kBlock, 0, s>>>(V.n, k, V.sm, invs, D, X, ini, dfresh)
   DISPATCH_K(k, T, G.lpr, N_, W_);
 #undef N_
 #undef W_
@@ -2784,7 +2806,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     }
     k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
   }
-  CN_HIP(hipMemsetAsync(D, 0, sizeof(T) * total, s));
+  // (the fused iteration's d needs no zeros: CTL_DFRESH)
+  if (!klf) CN_HIP(hipMemsetAsync(D, 0, sizeof(T) * total, s));
   CN_HIP(hipMemsetAsync(P, 0, sizeof(T) * total, s));
   CN_HIP(hipGetLastError());
 
@@ -2853,6 +2876,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     hc[CTL_JUST] = 0;
     hc[CTL_ITERS] = iters;
     hc[CTL_PAUSE] = 0;
+    hc[CTL_DFRESH] = klf ? 1 : 0;
     // control block, active and restart flags in one copy
     CN_HIP(hipMemcpyAsync(ctl, hc, sizeof(int32_t) * (CTL_N + 2 * ((size_t)W->nvc + 4)),
                           hipMemcpyHostToDevice, s));
@@ -3004,7 +3028,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     // ---- reliable update: x += d / s (FP64), d = 0; r = b - A x (FP64).
     // (One pass that forms x + d / s for the gathered neighbours as well was
     // measured 0.4 % slower a step at k = 768 than these two kernels.)
-    launch_x_update<T>(V, k, st + ST_INVS * kk, D, X, s, x_unwritten);
+    launch_x_update<T>(V, k, st + ST_INVS * kk, D, X, s, x_unwritten,
+                       klf ? ctl + CTL_DFRESH : nullptr);
     x_unwritten = false;
     if (batch) {
       // FP64 residual, per-segment r.r, scaled into R at the top of the loop
@@ -3068,6 +3093,7 @@ static cn_status plain_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   W->h_ctl[CTL_JUST] = 0;
   W->h_ctl[CTL_ITERS] = 0;
   W->h_ctl[CTL_PAUSE] = 0;
+  W->h_ctl[CTL_DFRESH] = 0;
   CN_HIP(hipMemcpyAsync(ctl, W->h_ctl, sizeof(int32_t) * 7, hipMemcpyHostToDevice, s));
   // preconditioner-only coefficients: alpha 0, beta 1, gamma 0 (p, d untouched)
   {
