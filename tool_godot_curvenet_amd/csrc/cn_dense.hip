@@ -242,10 +242,13 @@ void dense_cholesky_solve(int n, int k, const double *L, int64_t lda, double *B,
 // lie (no gathered copy).  A block of WM x WN waves owns a BM x BN tile of Z
 // and one of sp slices of the j range; each wave owns a (BM/WM) x (BN/WN)
 // part of the tile as 16 x 16 accumulators of v_mfma_f64_16x16x4f64.  The
-// j range goes in chunks of 16 through LDS.  The loads of chunk c + 1 are
-// issued right after chunk c is in LDS and before its MFMAs (a scheduling
-// barrier keeps the compiler from sinking them below the MFMAs, which
-// would leave their latency exposed), and they are unconditional: indices
+// j range goes in chunks of 16 through two LDS stages: chunk c multiplies
+// out of one stage while chunk c + 1, whose loads were issued before chunk
+// c's MFMAs, is stored to the other, with one barrier a chunk (one stage and
+// two barriers: 254 against 236 us a launch in FP32 at m = 3,834, k = 768).
+// A scheduling barrier keeps the compiler from sinking the loads below the
+// MFMAs, which would leave their latency exposed, and they are
+// unconditional: indices
 // are clamped into range, so no load sits under a branch that would make
 // the compiler wait for it.  Rows past m and columns past k compute values
 // nobody stores; for j past the slice the M side is stored as 0 and the S
@@ -267,9 +270,9 @@ void dense_cholesky_solve(int n, int k, const double *L, int64_t lda, double *B,
 // The FP32 apply can take M rounded to FP32 (MT = float): the same tiles
 // on v_mfma_f32_16x16x4f32, at twice the FP64 matrix rate, FP32 partial
 // sums (a slice of j), added in FP64 by k_coarse_sum.
-// The j range goes through LDS in chunks of kCK = 16.  For FP32, chunks of
-// 32, or a second chunk of look-ahead in registers, took the kernel from 7 to
-// 4-5 waves a SIMD (registers, LDS) and ran 3-10 % slower.
+// Chunks of kCK = 16.  For FP32, chunks of 32, or a second chunk of
+// look-ahead in registers, took the kernel from 7 to 4-5 waves a SIMD
+// (registers, LDS) and ran 3-10 % slower.
 constexpr int kCK = 16;
 constexpr int kCMaxChunk = 2048;  // longest slice (LDS row-number stage)
 
@@ -306,9 +309,9 @@ k_coarse_mm(int m, int k, int kchunk, const MT *__restrict__ M,
   using F = CoarseMfma<MT>;
   // row strides that spread a wave's operand reads (16 rows x 4 k of A,
   // 4 k x 16 columns of B) over distinct LDS banks
-  __shared__ MT As[BM][kCK + (sizeof(MT) == 4 ? 4 : 1)];
-  __shared__ T Bs[kCK][BN + 16];
-  __shared__ int32_t Rs[kCMaxChunk + kCK];
+  __shared__ MT As[2][BM][kCK + (sizeof(MT) == 4 ? 4 : 1)];
+  __shared__ T Bs[2][kCK][BN + 16];
+  __shared__ int32_t Rs[kCMaxChunk + 2 * kCK];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
   // XCD-contiguous tile order (x fastest, then y, then the slice z)
   const int nx = gridDim.x, ny = gridDim.y, nb = nx * ny * gridDim.z;
@@ -322,7 +325,7 @@ k_coarse_mm(int m, int k, int kchunk, const MT *__restrict__ M,
   const int br = tid / (BN / BE), bc = (tid % (BN / BE)) * BE;
   const MT *Mrow = M + (int64_t)min(bi + ar, m - 1) * m;
   const int cb = bj + bc;  // the thread's first S column (clamped below)
-  for (int q = tid; q < kchunk + kCK; q += NT) Rs[q] = rows[min(jb + q, m - 1)];
+  for (int q = tid; q < kchunk + 2 * kCK; q += NT) Rs[q] = rows[min(jb + q, m - 1)];
   __syncthreads();
   MT pa[AE];
   T pb[BE];
@@ -338,28 +341,37 @@ k_coarse_mm(int m, int k, int kchunk, const MT *__restrict__ M,
   for (int a = 0; a < TM; ++a)
 #pragma unroll
     for (int b = 0; b < TN; ++b) acc[a][b] = typename F::acc_t{0, 0, 0, 0};
+  // two LDS stages: chunk c multiplies out of stage c & 1 while chunk c + 1,
+  // loaded during chunk c - 1, is stored to the other stage; one barrier a
+  // chunk
+  auto stage = [&](int j0, int sb) {
+#pragma unroll
+    for (int u = 0; u < AE; ++u) As[sb][ar][ac + u] = j0 + ac + u < je ? pa[u] : MT(0);
+#pragma unroll
+    for (int u = 0; u < BE; ++u) Bs[sb][br][bc + u] = pb[u];
+  };
   fetch(jb);
-  for (int j0 = jb; j0 < je; j0 += kCK) {
-    __syncthreads();  // every wave is done with the previous chunk
-#pragma unroll
-    for (int u = 0; u < AE; ++u) As[ar][ac + u] = j0 + ac + u < je ? pa[u] : MT(0);
-#pragma unroll
-    for (int u = 0; u < BE; ++u) Bs[br][bc + u] = pb[u];
-    __syncthreads();
-    fetch(j0 + kCK);
+  stage(jb, 0);
+  __syncthreads();
+  fetch(jb + kCK);
+  int sb = 0;
+  for (int j0 = jb; j0 < je; j0 += kCK, sb ^= 1) {
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int t = 0; t < kCK; t += 4) {
       MT a[TM], b[TN];
 #pragma unroll
-      for (int x = 0; x < TM; ++x) a[x] = As[wi + 16 * x + (l & 15)][t + (l >> 4)];
+      for (int x = 0; x < TM; ++x) a[x] = As[sb][wi + 16 * x + (l & 15)][t + (l >> 4)];
 #pragma unroll
-      for (int y = 0; y < TN; ++y) b[y] = (MT)Bs[t + (l >> 4)][wj + 16 * y + (l & 15)];
+      for (int y = 0; y < TN; ++y) b[y] = (MT)Bs[sb][t + (l >> 4)][wj + 16 * y + (l & 15)];
 #pragma unroll
       for (int x = 0; x < TM; ++x)
 #pragma unroll
         for (int y = 0; y < TN; ++y) acc[x][y] = F::mma(a[x], b[y], acc[x][y]);
     }
+    stage(j0 + kCK, sb ^ 1);
+    __syncthreads();  // stage sb read by every wave, stage sb ^ 1 written
+    fetch(j0 + 2 * kCK);
   }
   MT *P = part + (int64_t)tz * m * k;
 #pragma unroll
