@@ -1415,6 +1415,7 @@ k_reduce_alpha(int32_t nrows, int64_t k, const double *__restrict__ part,
 //   mode 1 (after a residual replacement / start): rz = rz', rref = rr;
 //          continuing columns keep p (beta 1, gamma 0), restarted ones take
 //          p = z (beta 0, gamma 1)
+//   mode 2: mode 1 with rref as the host set it (no r.r partials)
 __global__ void __launch_bounds__(kBlock)
 k_reduce_beta(int32_t nrows, int32_t nrows_rz, int64_t k, const double *__restrict__ prr,
               const double *__restrict__ prz, double *__restrict__ st,
@@ -1430,9 +1431,9 @@ k_reduce_beta(int32_t nrows, int32_t nrows_rz, int64_t k, const double *__restri
   if (threadIdx.x < 64 && c < k) {
     double beta = 1.0, gamma = 0.0;
     if (active[c]) {
-      if (mode == 1) {
+      if (mode >= 1) {
         st[ST_RZ * k + c] = rz;
-        st[ST_RREF * k + c] = rr;
+        if (mode == 1) st[ST_RREF * k + c] = rr;
         if (restart[c]) { beta = 0.0; gamma = 1.0; }
       } else if (!(rr > st[ST_TOL2 * k + c]) || !(rr >= floor2 * st[ST_RREF * k + c]) ||
                  !isfinite(rr) || !isfinite(rz)) {
@@ -2858,7 +2859,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
         hs[ST_INVS * kk + c] = bb > 0.0 ? std::sqrt(bb) : 0.0;
         double t = theta * o->tol_rel;
         hs[ST_TOL2 * kk + c] = t * t;
-        hs[ST_RREF * kk + c] = 0.0;
+        // the reference r.r of the round: the true one, scaled
+        hs[ST_RREF * kk + c] = hs[ST_RR * kk + c] * hs[ST_S * kk + c] * hs[ST_S * kk + c];
       }
       CN_HIP(hipMemcpyAsync(st + ST_TOL2 * kk, hs + ST_TOL2 * kk, sizeof(double) * 4 * kk,
                             hipMemcpyHostToDevice, s));  // TOL2, RREF, S, INVS
@@ -2868,6 +2870,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     bool any_rst = false;
     for (int64_t c = 0; c < kk; ++c) any_rst = any_rst || rst[c];
     const bool apply_m = first || any_rst;
+    const bool round0 = first;
     first = false;
     hc[CTL_ACTIVE] = nact;
     hc[CTL_BREAKDOWN] = 0;
@@ -2890,19 +2893,27 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
       if (klf && pending) launch_bwd<T>(V, k, st, Z, Pb[cur], D, ctl, s);
       pending = false;
       // z = M^-1 r; rz; continuing columns keep p, restarted ones take p = z
-      int32_t rzr = 0;
-      rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, s,
-                                  false, batch ? nullptr : &rzr);
       if (batch) {
+        rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, s);
         launch_segdot<T, T>(S, W, k, R, R, R, Z, s);
         k_reduce_beta<<<red, kBlock, 0, s>>>(1, 1, kk, sg0, sg1, st, W->active.get(),
                                              W->restart.get(), ctl, 1, delta2, floor2);
       } else {
-        Red b1 = reduce_rows(W, p1, rows, k, stride, 1, s, 0);
-        Red b2 = reduce_rows(W, p1 + stride, rzr >= 0 ? rzr : rows, k, stride, 1, s, 1);
-        k_reduce_beta<<<red, kBlock, 0, s>>>(b1.rows, b2.rows, k, b1.p, b2.p, st,
-                                             W->active.get(), W->restart.get(), ctl, 1, delta2,
-                                             floor2);
+        // the reference r.r is the true one, scaled (the first round
+        // uploaded it with the scales): no pass over r for it
+        if (!round0) {
+          for (int64_t c = 0; c < kk; ++c) {
+            const double sc = hs[ST_S * kk + c];
+            hs[ST_RREF * kk + c] = hs[ST_RR * kk + c] * sc * sc;
+          }
+          CN_HIP(hipMemcpyAsync(st + ST_RREF * kk, hs + ST_RREF * kk, sizeof(double) * kk,
+                                hipMemcpyHostToDevice, s));
+        }
+        rows = launch_fwd<T, false>(V, k, st + ST_ALPHA * kk, Q, R, Z, p1, stride, ctl, s,
+                                    true);
+        Red b2 = reduce_rows(W, p1 + stride, rows, k, stride, 1, s, 1);
+        k_reduce_beta<<<red, kBlock, 0, s>>>(0, b2.rows, k, p1, b2.p, st, W->active.get(),
+                                             W->restart.get(), ctl, 2, delta2, floor2);
       }
       if (klf) pending = true;   // p = gamma z + beta p in the first KL_DIR
       else launch_bwd<T>(V, k, st, Z, P, D, ctl, s);
