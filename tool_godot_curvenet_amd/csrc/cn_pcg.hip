@@ -1330,6 +1330,31 @@ kw_dot(int32_t n, int64_t k, const T *__restrict__ A, const T *__restrict__ B,
   block_partials_wide<VEC, 1, LPR>(pd, part, k, c0, 0);
 }
 
+// partial b.b of FP64 b with b stored as TO into Y (kw_dot<double>'s sums):
+// a solve from x0 = 0 takes ||b|| and its unscaled first residual in one
+// pass over b
+template <typename TO, int VEC, int LPR>
+__global__ void __launch_bounds__(kBlock)
+kw_dot_copy(int32_t n, int64_t k, const double *__restrict__ B, TO *__restrict__ Y,
+            double *__restrict__ part) {
+  WIDE_COLS_P(VEC, LPR)
+  double pd[1][VEC] = {};
+  WIDE_LOOP_P(row, 0, n, LPR) {
+    if (valid) {
+      double b[VEC];
+      TO y[VEC];
+      VecIO<double, VEC>::ld(B + (int64_t)row * k + c0, b);
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) {
+        pd[0][m] += b[m] * b[m];
+        y[m] = (TO)b[m];
+      }
+      VecIO<TO, VEC>::st(Y + (int64_t)row * k + c0, y);
+    }
+  }
+  block_partials_wide<VEC, 1, LPR>(pd, part, k, c0, 0);
+}
+
 // ---- column reductions of the [rows][k] partial buffers (fixed order) ----
 __device__ __forceinline__ double col_reduce(const double *__restrict__ part,
                                              int32_t nrows, int64_t k, int64_t c,
@@ -2374,6 +2399,19 @@ static int32_t launch_dot(int32_t n, int64_t k, const T *A, const T *B, double *
   return G.rows;
 }
 
+// k > 4 only (the wide mapping)
+template <typename T>
+static int32_t launch_dot_copy(int32_t n, int64_t k, const double *B, T *Y, double *part,
+                               hipStream_t s) {
+  Grid G = grid_rows<double>(n, k, kCapSpmm);
+#define N_(KK) (void)0
+#define W_(VV, LL) kw_dot_copy<T, WideV<double, VV>::v, LL><<<G.g, kBlock, 0, s>>>(n, k, B, Y, part)
+  DISPATCH_K(k, double, G.lpr, N_, W_);
+#undef N_
+#undef W_
+  return G.rows;
+}
+
 // Per-(segment, column) dots a.b (and c.d when c != nullptr) of a batch
 // into W->seg (then W->seg + nvc): one partial row per segment-aligned
 // block, summed per segment in block order.
@@ -2789,6 +2827,11 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   // the FP64 residual the first round scales into R: from x0 = 0 it is b
   // itself (b - A 0 = b exactly), so no SpMM, and ||r|| is ||b||
   const double *rsrc = o->warm_start ? R64 : d_b;
+  // from x0 = 0 (one system, k > 4) the pass that forms ||b|| also stores b
+  // into R, unscaled: a column whose ||b|| is within [1e-15, 1e15] then runs
+  // unscaled (scale 1), far from the FP32 range's ends, and needs no
+  // scaling pass
+  const bool b_in_r = !batch && !o->warm_start && k > 4;
   if (batch) {
     launch_segdot<double, double>(S, W, k, d_b, d_b, nullptr, nullptr, s);
     k_reduce_plain<<<red, kBlock, 0, s>>>(1, kk, sg0, st + ST_BB * kk);
@@ -2798,7 +2841,8 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     }
     k_reduce_plain<<<red, kBlock, 0, s>>>(1, kk, sg0, st + ST_RR * kk);
   } else {
-    rows = launch_dot<double>(n, k, d_b, d_b, p0, s);
+    rows = b_in_r ? launch_dot_copy<T>(n, k, d_b, R, p0, s)
+                  : launch_dot<double>(n, k, d_b, d_b, p0, s);
     rd = reduce_rows(W, p0, rows, k, stride, 1, s);
     k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_BB * k);
     if (o->warm_start) {
@@ -2851,19 +2895,27 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     if (nact == 0) { status = CN_OK; break; }
     if (!std::isfinite(relmax)) { status = CN_BREAKDOWN; break; }
     if (launched >= o->max_iter || updates >= o->max_refine * 8) { status = CN_MAXIT; break; }
-    // ---- per-column scale s = 1/||b|| (fixed for the solve)
+    // ---- per-column scale s = 1/||b|| (fixed for the solve), or 1 for a
+    // column whose unscaled b is already in R
     if (first) {
+      bool rescale = false;
       for (int64_t c = 0; c < kk; ++c) {
         double bb = hs[ST_BB * kk + c];
-        hs[ST_S * kk + c] = bb > 0.0 ? 1.0 / std::sqrt(bb) : 0.0;
-        hs[ST_INVS * kk + c] = bb > 0.0 ? std::sqrt(bb) : 0.0;
+        // (b = 0: the column never runs, its R column is 0 either way)
+        const bool one = b_in_r && (bb == 0.0 || (bb >= 1e-30 && bb <= 1e30));
+        rescale = rescale || !one;
+        hs[ST_S * kk + c] = one ? 1.0 : (bb > 0.0 ? 1.0 / std::sqrt(bb) : 0.0);
+        hs[ST_INVS * kk + c] = one ? 1.0 : (bb > 0.0 ? std::sqrt(bb) : 0.0);
+        // the stop test on the scaled residual: ||r s|| <= theta tol ||b|| s
         double t = theta * o->tol_rel;
-        hs[ST_TOL2 * kk + c] = t * t;
+        hs[ST_TOL2 * kk + c] = one ? t * t * bb : t * t;
         // the reference r.r of the round: the true one, scaled
         hs[ST_RREF * kk + c] = hs[ST_RR * kk + c] * hs[ST_S * kk + c] * hs[ST_S * kk + c];
       }
       CN_HIP(hipMemcpyAsync(st + ST_TOL2 * kk, hs + ST_TOL2 * kk, sizeof(double) * 4 * kk,
                             hipMemcpyHostToDevice, s));  // TOL2, RREF, S, INVS
+      // R holds b: scaled already unless some column needs 1/||b||
+      if (b_in_r && !rescale) r_scaled = true;
     }
     // Continuing columns keep p and the iterated r.z (van der Vorst-Ye
     // reliable update): only a (re)start needs z = M^-1 r of the new r.
