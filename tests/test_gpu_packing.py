@@ -308,3 +308,36 @@ def test_fused_graph_odd_block(gpu):
         x1, i1 = lap.solve(B, precond="sgs", precision="mixed", tol=1e-9, sgs=g, check_every=3,
                            graph=True)
         assert i1.iters == i0.iters and np.array_equal(x0.cpu().numpy(), x1.cpu().numpy())
+
+
+# Column scales: from x0 = 0 a column whose ||b|| lies within [1e-15, 1e15]
+# runs unscaled, others at 1/||b||; a solve with columns of both kinds
+# scales every column by its own rule.  Every column converges to the
+# tolerance relative to its own ||b||, and its solution divided by its scale
+# is the unit-scale column's solution to what that tolerance allows (the
+# columns' FP32 roundings differ, so their iterates do).
+@pytest.mark.parametrize("precond", ["hsc", "jacobi"])
+@pytest.mark.parametrize("scales", [
+    (1.0, 1e-12, 1e12, 1e-14, 1e14, 3.0, 1e-7, 1e7),       # all at scale 1
+    (1.0, 1e-25, 1e25, 1e-18, 1e18, 1e-40, 1e40, 1e5),     # mixed rules
+])
+def test_pcg_column_scales(gpu, precond, scales):
+    from tool_godot_curvenet_amd import HSC
+    from tool_godot_curvenet_amd import meshgen as G
+    lap = _lap("icosphere6")
+    A = lap.A_ff()
+    k = len(scales)
+    B0 = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, 1, seed=77))
+    B = B0 * np.asarray(scales)[None, :]
+    kw = {"hsc": HSC(lap)} if precond == "hsc" else {}
+    x, info = lap.solve(B, precond=precond, precision="mixed", tol=1e-9, **kw)
+    assert info.status == 0
+    X = x.cpu().numpy()
+    R = B - O.spmm(A.row_ptr, A.col_idx, A.values, X)
+    rr = np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
+    assert np.all(rr <= 1e-9), rr
+    assert np.all(np.isfinite(X))
+    # every column is the same system: X[:, j] / scale_j agrees with column 0
+    X1 = X / np.asarray(scales)[None, :]
+    dev = np.linalg.norm(X1 - X1[:, :1], axis=0) / np.linalg.norm(X1[:, 0])
+    assert np.all(dev <= 1e-4), dev
