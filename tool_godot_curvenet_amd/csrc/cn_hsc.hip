@@ -57,6 +57,13 @@ kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv
   // the compiler wait for the previous row's store before the next row
   const int64_t cl = valid ? c0 : k - VEC;
   double pd[1][VEC] = {};
+  // the row of rz this block adds to, loaded before the block's rows (at
+  // the end its latency would idle the block)
+  double pre[VEC] = {};
+  if (rz && (int32_t)blockIdx.x < rz_add) {
+#pragma unroll
+    for (int m = 0; m < VEC; ++m) pre[m] = rz[(int64_t)blockIdx.x * k + min(c0 + m, k - 1)];
+  }
   const RowIter ri = row_iter(0, nF, P::kSlots, P::slot());
   for (int32_t row = ri.first; row < ri.end; row += ri.stride) {
     const int32_t f = P::u(row);
@@ -72,7 +79,7 @@ kw_hsc_up(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ dinv
     gather_wide<T, VEC>(P::u(pptr[f]), P::u(pptr[f + 1]), pcol, pval, Z, k, cl, true, acc);
     if (valid) VecIO<T, VEC>::st(Z + g + c0, acc);
   }
-  if (rz) block_partials_wide<VEC, 1, LPR>(pd, rz, k, c0, 0, rz_add);
+  if (rz) block_partials_wide<VEC, 1, LPR>(pd, rz, k, c0, 0, rz_add, pre);
 }
 
 // ---- the down sweep (wide form) -----------------------------------------

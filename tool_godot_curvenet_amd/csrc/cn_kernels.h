@@ -252,13 +252,16 @@ inline int pack_lpr12(int64_t lanes) {
 // `sh` holds kWaves * NP * 64 * VEC doubles of shared memory.
 // Blocks blockIdx.x < add_below add their sums to the row instead of
 // storing them (the row holds an earlier launch's partials: launches in
-// stream order accumulate into one set of rows, in launch order).
+// stream order accumulate into one set of rows, in launch order); pre, if
+// given, holds the thread's values of that row, loaded earlier by the
+// caller (the writing threads' columns c0 + m, NP x VEC of them).
 template <int VEC, int NP, int LPR = 64>
 __device__ __forceinline__ void block_partials_wide_in(double (&p)[NP][VEC],
                                                        double *__restrict__ out,
                                                        int64_t k, int64_t c0,
                                                        int64_t stride_np, double *sh,
-                                                       int32_t add_below = 0) {
+                                                       int32_t add_below = 0,
+                                                       const double *pre = nullptr) {
   constexpr int TW = 64 * VEC, G = 64 / LPR;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -281,7 +284,7 @@ __device__ __forceinline__ void block_partials_wide_in(double (&p)[NP][VEC],
             if (ww + g > 0) s += sh[ww * NP * TW + g * LPR * VEC + o];
         if (c < k) {
           double *o_ = out + q * stride_np + (int64_t)blockIdx.x * k + c;
-          *o_ = (int32_t)blockIdx.x < add_below ? *o_ + s : s;
+          *o_ = (int32_t)blockIdx.x < add_below ? (pre ? pre[q * VEC + m] : *o_) + s : s;
         }
       }
   }
@@ -292,9 +295,10 @@ __device__ __forceinline__ void block_partials_wide(double (&p)[NP][VEC],
                                                     double *__restrict__ out,
                                                     int64_t k, int64_t c0,
                                                     int64_t stride_np,
-                                                    int32_t add_below = 0) {
+                                                    int32_t add_below = 0,
+                                                    const double *pre = nullptr) {
   __shared__ double sh[kWaves * NP * 64 * VEC];
-  block_partials_wide_in<VEC, NP, LPR>(p, out, k, c0, stride_np, sh, add_below);
+  block_partials_wide_in<VEC, NP, LPR>(p, out, k, c0, stride_np, sh, add_below, pre);
 }
 
 // narrow: lane g < K of every 8-lane group holds the partials of column g;
