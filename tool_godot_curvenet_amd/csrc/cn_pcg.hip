@@ -258,13 +258,17 @@ constexpr int win_block_bytes() { return kWinEnt * (4 + (int)sizeof(T)); }
 //            partial p.q and no Q stored;
 //  KL_UPD    q = A p recomputed in the same order (so the same bits as
 //            KL_DIR's), then r -= alpha q and d += alpha p on the block's
-//            own rows (kw_rupd's and kw_pd_update's arithmetic), partial r.r.
+//            own rows (kw_rupd's and kw_pd_update's arithmetic), partial r.r;
+//  KL_RES    (FP64) the residual r = b - A x of a residual replacement: P is
+//            x, Z is b (own rows, loaded like KL_UPD's r), r scaled by the
+//            column scale into Rf (FP32), partial r.r of the unscaled r
+//            (kw_resid's arithmetic, the product in the window's order).
 // With KL_DIR then KL_UPD an iteration's SpMM, residual and iterate updates
 // and direction update move P in, P out, then P, R, D in and R, D out:
 // 8 n x k passes instead of the 10 of the SpMM, kw_rupd and kw_pd_update.
 // The own-row R and D values KL_UPD needs are loaded D steps ahead in
 // registers (the thread that loads a row is the one that updates it).
-enum { KL_PLAIN = 0, KL_DIR = 1, KL_UPD = 2 };
+enum { KL_PLAIN = 0, KL_DIR = 1, KL_UPD = 2, KL_RES = 3 };
 
 template <typename T, int D, int MODE>
 __global__ void __launch_bounds__(512)
@@ -273,8 +277,8 @@ kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
         const T *__restrict__ P, const T *__restrict__ Z, T *__restrict__ Pout,
         T *__restrict__ Q, T *__restrict__ Rv, T *__restrict__ Dv,
         const double *__restrict__ st, double *__restrict__ part,
-        const int32_t *__restrict__ ctl) {
-  if (ctl_idle(ctl)) return;
+        const int32_t *__restrict__ ctl, float *__restrict__ Rf = nullptr) {
+  if (ctl && ctl_idle(ctl)) return;
   constexpr int VEC = 16 / sizeof(T);
   constexpr int W = 16 * VEC;          // columns of a tile (256 bytes)
   constexpr int CH = kLdsRows * W;     // elements of a chunk
@@ -312,8 +316,10 @@ kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
   auto row_of = [&](int32_t p) { return down ? re - 1 - p : rs + p; };
   // the column scalars (single system: slot c of k)
   T al[VEC], be[VEC], ga[VEC];
+  double sc[VEC];
 #pragma unroll
   for (int m = 0; m < VEC; ++m) {
+    sc[m] = (MODE == KL_RES && valid) ? st[ST_S * k + c0 + m] : 0.0;
     al[m] = (MODE == KL_UPD && valid) ? (T)st[ST_ALPHA * k + c0 + m] : T(0);
     be[m] = (MODE == KL_DIR && valid) ? (T)st[ST_BETA * k + c0 + m] : T(0);
     ga[m] = (MODE == KL_DIR && valid) ? (T)st[ST_GAMMA * k + c0 + m] : T(0);
@@ -337,7 +343,7 @@ kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
     }
   }
   constexpr int DZ = MODE == KL_DIR ? D : 1;   // z of the landing rows
-  constexpr int DR = MODE == KL_UPD ? D : 1;   // own-row r and d
+  constexpr int DR = (MODE == KL_UPD || MODE == KL_RES) ? D : 1;  // own-row r, d (b)
   T pb[D][VEC], zb[DZ][VEC], rb[DR][VEC], db[DR][VEC];
   IW ib[D];
   auto issue = [&](int32_t t, int u) {
@@ -353,6 +359,10 @@ kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
   const bool dfresh = MODE == KL_UPD && ctl[CTL_DFRESH] != 0;
   auto issue_rd = [&](int32_t t, int u) {
     const int32_t row = min(max(row_of((t + D) * kLdsRows + rsub), 0), n - 1);
+    if constexpr (MODE == KL_RES) {
+      VecIO<T, VEC>::ld(Z + (int64_t)row * k + cl, rb[u]);  // b
+      return;
+    }
     VecIO<T, VEC>::ld(Rv + (int64_t)row * k + cl, rb[u]);
     if (dfresh) {
 #pragma unroll
@@ -406,6 +416,15 @@ kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
         VecIO<T, VEC>::st(Rv + (int64_t)row * k + c0, rv);
         VecIO<T, VEC>::st(Dv + (int64_t)row * k + c0, dv);
       }
+    } else if constexpr (MODE == KL_RES) {
+      float out[VEC];
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) {
+        const double rv = (double)rb[u][m] - (double)acc[m];
+        pd[m] += rv * rv;
+        out[m] = (float)(rv * sc[m]);
+      }
+      if (valid) VecIO<float, VEC>::st(Rf + (int64_t)row * k + c0, out);
     } else {
 #pragma unroll
       for (int m = 0; m < VEC; ++m) pd[m] += (double)x0[m] * (double)acc[m];
@@ -429,7 +448,7 @@ kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
       issue(t, u);
       __syncthreads();
       if (t >= 0 && t < nch) compute(t, u);
-      if constexpr (MODE == KL_UPD) issue_rd(t, u);
+      if constexpr (MODE == KL_UPD || MODE == KL_RES) issue_rd(t, u);
     }
   }
   // block partials (p.q, or r.r for KL_UPD): the 32 row slots of each
@@ -2109,6 +2128,23 @@ static int32_t launch_kl(const SysView<T> &V, int64_t k, const T *P, const T *Z,
   return (int32_t)nr;
 }
 
+// The FP64 residual of a replacement with the LDS-staged product, over the
+// system's window (the FP32 product's row ranges, FP64 weights): R = r s
+// (FP32), partial r.r a range.  Returns the partial rows.
+static int32_t launch_resid_kl(const cn_system_t *S, int32_t n, int32_t rpr, int64_t k,
+                               const double *X, const double *B, float *R,
+                               const double *st, double *part, hipStream_t s) {
+  const int64_t W = 256 / sizeof(double);
+  const int64_t nr = (n + rpr - 1) / rpr, pairs = nr * ((k + W - 1) / W);
+  const dim3 g((unsigned)(kXcd * ((pairs + kXcd - 1) / kXcd)));
+  ProfScope ps("resid", s);
+  kl_spmm<double, 6, KL_RES><<<g, 512, 0, s>>>(n, rpr, S->win_pk64.get(), S->win_far_ptr.get(),
+                                               S->win_far_rows.get(), k, X, B, nullptr,
+                                               nullptr, nullptr, nullptr, st, part, nullptr,
+                                               R);
+  return (int32_t)nr;
+}
+
 template <typename TO>
 static int32_t launch_resid(int32_t n, const int32_t *ptr, const int32_t *col,
                             const double *val, int64_t k, const double *X,
@@ -3101,7 +3137,15 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
       launch_segdot<double, double>(S, W, k, R64, R64, nullptr, nullptr, s);
       k_reduce_plain<<<red, kBlock, 0, s>>>(1, kk, sg0, st + ST_RR * kk);
     } else {
-      rows = launch_resid<T>(n, V.ptr, V.col, V.val64, k, X, d_b, R, st + ST_S * k, p0, s);
+      if constexpr (sizeof(T) == 4) {
+        // the window the FP32 product uses holds FP64 weights too
+        rows = V.rpr > 0 && win_applies<double>(n, k)
+                   ? launch_resid_kl(S, n, V.rpr, k, X, d_b, R, st, p0, s)
+                   : launch_resid<T>(n, V.ptr, V.col, V.val64, k, X, d_b, R, st + ST_S * k,
+                                     p0, s);
+      } else {
+        rows = launch_resid<T>(n, V.ptr, V.col, V.val64, k, X, d_b, R, st + ST_S * k, p0, s);
+      }
       r_scaled = true;
       rd = reduce_rows(W, p0, rows, k, stride, 1, s);
       k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
