@@ -259,8 +259,12 @@ constexpr int win_block_bytes() { return kWinEnt * (4 + (int)sizeof(T)); }
 //  KL_UPD    q = A p recomputed in the same order (so the same bits as
 //            KL_DIR's), then r -= alpha q and d += alpha p on the block's
 //            own rows (kw_rupd's and kw_pd_update's arithmetic), partial r.r;
-//  KL_RES    (FP64) the residual r = b - A x of a residual replacement: P is
-//            x, Z is b (own rows, loaded like KL_UPD's r), r scaled by the
+//  KL_RES    (FP64) the x update and residual of a residual replacement:
+//            every row that lands in LDS is x = x_old + d / s (P is x_old, Df
+//            the FP32 d; kw_x_update's arithmetic; no x_old read when x0 = 0,
+//            no d when the round's d holds nothing, *dnone), the block's own
+//            rows are written to Pout (another x buffer), then r = b - A x
+//            (Z is b, own rows, loaded like KL_UPD's r), r scaled by the
 //            column scale into Rf (FP32), partial r.r of the unscaled r
 //            (kw_resid's arithmetic, the product in the window's order).
 // With KL_DIR then KL_UPD an iteration's SpMM, residual and iterate updates
@@ -277,7 +281,9 @@ kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
         const T *__restrict__ P, const T *__restrict__ Z, T *__restrict__ Pout,
         T *__restrict__ Q, T *__restrict__ Rv, T *__restrict__ Dv,
         const double *__restrict__ st, double *__restrict__ part,
-        const int32_t *__restrict__ ctl, float *__restrict__ Rf = nullptr) {
+        const int32_t *__restrict__ ctl, float *__restrict__ Rf = nullptr,
+        const float *__restrict__ Df = nullptr, const int32_t *__restrict__ dnone = nullptr,
+        int xinit = 0) {
   if (ctl && ctl_idle(ctl)) return;
   constexpr int VEC = 16 / sizeof(T);
   constexpr int W = 16 * VEC;          // columns of a tile (256 bytes)
@@ -316,10 +322,11 @@ kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
   auto row_of = [&](int32_t p) { return down ? re - 1 - p : rs + p; };
   // the column scalars (single system: slot c of k)
   T al[VEC], be[VEC], ga[VEC];
-  double sc[VEC];
+  double sc[VEC], iv[VEC];
 #pragma unroll
   for (int m = 0; m < VEC; ++m) {
     sc[m] = (MODE == KL_RES && valid) ? st[ST_S * k + c0 + m] : 0.0;
+    iv[m] = (MODE == KL_RES && valid) ? st[ST_INVS * k + c0 + m] : 0.0;
     al[m] = (MODE == KL_UPD && valid) ? (T)st[ST_ALPHA * k + c0 + m] : T(0);
     be[m] = (MODE == KL_DIR && valid) ? (T)st[ST_BETA * k + c0 + m] : T(0);
     ga[m] = (MODE == KL_DIR && valid) ? (T)st[ST_GAMMA * k + c0 + m] : T(0);
@@ -328,12 +335,42 @@ kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
 #pragma unroll
     for (int m = 0; m < VEC; ++m) v[m] = fma(be[m], v[m], ga[m] * zv[m]);
   };
+  // KL_RES: x_old is not read from x0 = 0, d not when it holds nothing
+  const bool xskip = MODE == KL_RES && xinit != 0;
+  const bool dskip = MODE == KL_RES && dnone && *dnone != 0;
+  auto ld_x = [&](int32_t row, T (&v)[VEC]) {
+    if (xskip) {
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) v[m] = T(0);
+    } else {
+      VecIO<T, VEC>::ld(P + (int64_t)row * k + cl, v);
+    }
+  };
+  auto ld_d = [&](int32_t row, float (&v)[VEC]) {
+    if (dskip) {
+#pragma unroll
+      for (int m = 0; m < VEC; ++m) v[m] = 0.0f;
+    } else {
+      VecIO<float, VEC>::ld(Df + (int64_t)row * k + cl, v);
+    }
+  };
+  auto xupd = [&](T (&v)[VEC], const float (&dv)[VEC]) {
+#pragma unroll
+    for (int m = 0; m < VEC; ++m) v[m] = fma(iv[m], (double)dv[m], (double)v[m]);
+  };
   // far rows of the range, once
   {
     const int32_t fa = far_ptr[x], fb = far_ptr[x + 1];
     for (int32_t f = fa + rsub; f < fb; f += kLdsRows) {
       T v[VEC];
-      VecIO<T, VEC>::ld(P + (int64_t)far_rows[f] * k + cl, v);
+      if constexpr (MODE == KL_RES) {
+        float dv[VEC];
+        ld_x(far_rows[f], v);
+        ld_d(far_rows[f], dv);
+        xupd(v, dv);
+      } else {
+        VecIO<T, VEC>::ld(P + (int64_t)far_rows[f] * k + cl, v);
+      }
       if constexpr (MODE == KL_DIR) {
         T zv[VEC];
         VecIO<T, VEC>::ld(Z + (int64_t)far_rows[f] * k + cl, zv);
@@ -344,12 +381,19 @@ kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
   }
   constexpr int DZ = MODE == KL_DIR ? D : 1;   // z of the landing rows
   constexpr int DR = (MODE == KL_UPD || MODE == KL_RES) ? D : 1;  // own-row r, d (b)
+  constexpr int DX = MODE == KL_RES ? D : 1;   // d of the landing rows (FP32)
   T pb[D][VEC], zb[DZ][VEC], rb[DR][VEC], db[DR][VEC];
+  float xd[DX][VEC];
   IW ib[D];
   auto issue = [&](int32_t t, int u) {
     const int32_t g = t + kLdsAhead + D;                 // P chunk
     const int32_t row = min(max(row_of(g * kLdsRows + rsub), 0), n - 1);
-    VecIO<T, VEC>::ld(P + (int64_t)row * k + cl, pb[u]);
+    if constexpr (MODE == KL_RES) {
+      ld_x(row, pb[u]);
+      ld_d(row, xd[u]);
+    } else {
+      VecIO<T, VEC>::ld(P + (int64_t)row * k + cl, pb[u]);
+    }
     if constexpr (MODE == KL_DIR) VecIO<T, VEC>::ld(Z + (int64_t)row * k + cl, zb[u]);
     const int32_t gi = min(max(cbase + t + D, 0), clast);  // index chunk
     ib[u] = reinterpret_cast<const IW *>(wpk + (int64_t)gi * IB)[iw];
@@ -373,8 +417,9 @@ kl_spmm(int32_t n, int32_t rpr, const unsigned char *__restrict__ wpk,
   };
   auto land = [&](int32_t t, int u) {
     const int32_t g = t + kLdsAhead;
-    if constexpr (MODE == KL_DIR) {
-      dir(pb[u], zb[u]);
+    if constexpr (MODE == KL_DIR || MODE == KL_RES) {
+      if constexpr (MODE == KL_DIR) dir(pb[u], zb[u]);
+      else xupd(pb[u], xd[u]);
       // the block's own rows go out once (halo rows belong to another range)
       const int32_t p = g * kLdsRows + rsub;
       if (valid && p >= 0 && p < len)
@@ -2128,20 +2173,23 @@ static int32_t launch_kl(const SysView<T> &V, int64_t k, const T *P, const T *Z,
   return (int32_t)nr;
 }
 
-// The FP64 residual of a replacement with the LDS-staged product, over the
-// system's window (the FP32 product's row ranges, FP64 weights): R = r s
-// (FP32), partial r.r a range.  Returns the partial rows.
+// The x update and FP64 residual of a replacement with the LDS-staged
+// product, over the system's window (the FP32 product's row ranges, FP64
+// weights): Xout = X + d / s (X not read when xinit, d not when *dfresh),
+// R = (b - A Xout) s (FP32), partial r.r a range.  Returns the partial rows.
 static int32_t launch_resid_kl(const cn_system_t *S, int32_t n, int32_t rpr, int64_t k,
                                const double *X, const double *B, float *R,
-                               const double *st, double *part, hipStream_t s) {
+                               const double *st, double *part, hipStream_t s,
+                               const float *D, double *Xout, const int32_t *dfresh,
+                               bool xinit) {
   const int64_t W = 256 / sizeof(double);
   const int64_t nr = (n + rpr - 1) / rpr, pairs = nr * ((k + W - 1) / W);
   const dim3 g((unsigned)(kXcd * ((pairs + kXcd - 1) / kXcd)));
   ProfScope ps("resid", s);
   kl_spmm<double, 6, KL_RES><<<g, 512, 0, s>>>(n, rpr, S->win_pk64.get(), S->win_far_ptr.get(),
-                                               S->win_far_rows.get(), k, X, B, nullptr,
+                                               S->win_far_rows.get(), k, X, B, Xout,
                                                nullptr, nullptr, nullptr, st, part, nullptr,
-                                               R);
+                                               R, D, dfresh, xinit ? 1 : 0);
   return (int32_t)nr;
 }
 
@@ -2836,6 +2884,10 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
   // x accumulates in the caller's d_x (no copy in or out) unless d_x is d_b,
   // which is read until the end
   double *X = d_x != d_b ? d_x : W->xp.get(), *R64 = W->rp.get();
+  // the two x buffers of the fused x update + residual (KL_RES): the
+  // caller's x (or xp) and xp (or rp, free once the first round has scaled
+  // its residual into R); the solve copies x into d_x at the end if needed
+  double *const Xa = X, *const Xb = X == W->xp.get() ? W->rp.get() : W->xp.get();
   const double theta = 0.5;  // iterated-residual target below tol
   // reliable update: a residual replacement is due once a column's iterated
   // ||r|| has dropped by inner_tol (MIXED) or 1e-4 (FP64) since the last one
@@ -3125,30 +3177,39 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
     launched += it;
     iters = hc[CTL_ITERS];
     // ---- reliable update: x += d / s (FP64), d = 0; r = b - A x (FP64).
-    // (One pass that forms x + d / s for the gathered neighbours as well was
-    // measured 0.4 % slower a step at k = 768 than these two kernels.)
-    launch_x_update<T>(V, k, st + ST_INVS * kk, D, X, s, x_unwritten,
-                       klf ? ctl + CTL_DFRESH : nullptr);
-    x_unwritten = false;
-    if (batch) {
-      // FP64 residual, per-segment r.r, scaled into R at the top of the loop
-      launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
-      rsrc = R64;
-      launch_segdot<double, double>(S, W, k, R64, R64, nullptr, nullptr, s);
-      k_reduce_plain<<<red, kBlock, 0, s>>>(1, kk, sg0, st + ST_RR * kk);
-    } else {
-      if constexpr (sizeof(T) == 4) {
-        // the window the FP32 product uses holds FP64 weights too
-        rows = V.rpr > 0 && win_applies<double>(n, k)
-                   ? launch_resid_kl(S, n, V.rpr, k, X, d_b, R, st, p0, s)
-                   : launch_resid<T>(n, V.ptr, V.col, V.val64, k, X, d_b, R, st + ST_S * k,
-                                     p0, s);
-      } else {
-        rows = launch_resid<T>(n, V.ptr, V.col, V.val64, k, X, d_b, R, st + ST_S * k, p0, s);
-      }
+    // The fused iteration with the LDS-staged product does both in one
+    // pass (KL_RES: x lands in LDS as x + d / s and goes to the other x
+    // buffer); otherwise two kernels.  (One row-table pass that formed
+    // x + d / s for the gathered neighbours as well was 0.4 % slower a step
+    // at k = 768 than the two.)
+    bool kl_res = false;
+    if constexpr (sizeof(T) == 4) kl_res = klf && win_applies<double>(n, k);
+    if (kl_res) {
+      double *Xo = X == Xa ? Xb : Xa;
+      rows = launch_resid_kl(S, n, V.rpr, k, X, d_b, reinterpret_cast<float *>(R), st, p0, s,
+                             reinterpret_cast<const float *>(D), Xo, ctl + CTL_DFRESH,
+                             x_unwritten);
+      X = Xo;
+      x_unwritten = false;
       r_scaled = true;
       rd = reduce_rows(W, p0, rows, k, stride, 1, s);
       k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
+    } else {
+      launch_x_update<T>(V, k, st + ST_INVS * kk, D, X, s, x_unwritten,
+                         klf ? ctl + CTL_DFRESH : nullptr);
+      x_unwritten = false;
+      if (batch) {
+        // FP64 residual, per-segment r.r, scaled into R at the top of the loop
+        launch_resid<double>(n, V.ptr, V.col, V.val64, k, X, d_b, R64, nullptr, p0, s);
+        rsrc = R64;
+        launch_segdot<double, double>(S, W, k, R64, R64, nullptr, nullptr, s);
+        k_reduce_plain<<<red, kBlock, 0, s>>>(1, kk, sg0, st + ST_RR * kk);
+      } else {
+        rows = launch_resid<T>(n, V.ptr, V.col, V.val64, k, X, d_b, R, st + ST_S * k, p0, s);
+        r_scaled = true;
+        rd = reduce_rows(W, p0, rows, k, stride, 1, s);
+        k_reduce_plain<<<red, kBlock, 0, s>>>(rd.rows, k, rd.p, st + ST_RR * k);
+      }
     }
     CN_HIP(hipGetLastError());
     ++updates;
