@@ -591,6 +591,31 @@ def test_solve_after_set_positions(gpu):
     assert np.all(_relres(A, x.cpu().numpy(), B) <= 1e-9)
 
 
+# The solver's row tables (the LDS window's packed weights, the row table's
+# values) are refilled only after an assembly: each form and preconditioner
+# solves with the values of the positions set last, before and after a
+# second change, and a solve that changes nothing reuses them.
+@pytest.mark.parametrize("precond,form", [("jacobi", 0), ("jacobi", 1), ("hsc", 0),
+                                          ("hsc", 1)])
+def test_solve_tables_follow_set_positions(gpu, precond, form):
+    from tool_godot_curvenet_amd import HSC
+    from tool_godot_curvenet_amd import meshgen as G
+    m, lap = _system("cylinder64")
+    lap.set_tune({"spmm_form": form})
+    rng = np.random.default_rng(21)
+    pos0 = lap.pos.copy()
+    for step in range(3):
+        if step:
+            lap.set_positions(pos0 * rng.uniform(0.8, 1.25, size=(1, 3)))
+        A = lap.A_ff()
+        B = lap.harmonic_rhs(G.seeded_boundary(lap.n_con, 96, seed=30 + step))
+        kw = {"hsc": HSC(lap)} if precond == "hsc" else {}
+        for _ in range(2):  # the second solve reuses the tables
+            x, info = lap.solve(B, precond=precond, precision="mixed", tol=1e-9, **kw)
+            assert info.status == 0
+            assert np.all(_relres(A, x.cpu().numpy(), B) <= 1e-9), (step, precond, form)
+
+
 @pytest.mark.parametrize("k", [1, 2, 3, 4])
 def test_spmm_f32_long_rows_and_grid_stride(gpu, k):
     """Thread-per-row SpMM (k <= 4): rows longer than its 8-entry chunk (hub

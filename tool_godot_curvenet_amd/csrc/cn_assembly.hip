@@ -556,6 +556,7 @@ static void assemble_values(cn_system_t *S, hipStream_t s) {
     k_diag_at<<<grid_for(S->n_free, 256), 256, 0, s>>>(S->n_free, S->diag_pos.get(),
                                                        S->ff_val.get(), S->diag.get());
   }
+  ++S->vals_gen;  // the solver's row tables refill from the new values
   CN_HIP(hipGetLastError());
   CN_HIP(hipStreamSynchronize(s));
 }

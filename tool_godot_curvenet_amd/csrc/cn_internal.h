@@ -191,7 +191,9 @@ struct cn_system_t {
   // A_ff as a fixed-width row table for the solver's multi-column SpMM:
   // the first kEll entries of every row (short rows padded with their last
   // column and weight 0), the rest in an overflow CSR.  The pattern is built
-  // on first use (ov_nnz = -1 until then), the values at every solve.
+  // on first use (ov_nnz = -1 until then), the values at the first solve
+  // after they change (vals_gen counts the value assemblies; ell_gen and
+  // win_gen are the ones the tables hold).
   cn::DevBuf<int32_t> ell_col, ov_ptr, ov_col;
   cn::DevBuf<float> ell_val32, ov_val32;
   cn::DevBuf<double> ell_val64, ov_val64;
@@ -206,6 +208,7 @@ struct cn_system_t {
   // FP64 weights), refilled from the values at every solve
   cn::DevBuf<unsigned char> win_pk32, win_pk64;
   int32_t win_rpr = 0, win_ok = 0;
+  uint64_t vals_gen = 1, ell_gen = 0, win_gen = 0;
   cn_tune tune{};                   // launch shapes (cn_system_set_tune); 0 = built-in
   // A batch of independent systems (cn_system_batch) is one block-diagonal
   // A_ff: segment s owns rows [seg_ptr[s], seg_ptr[s+1]).  The solver keeps

@@ -2584,8 +2584,8 @@ static Red reduce_rows(SolverWork *W, const double *part, int32_t nrows, int64_t
   return Red{out, S, ostride};
 }
 
-// Row table of A_ff for the SpMM: pattern once per system, values at every
-// solve (cn_system_set_positions changes them).
+// Row table of A_ff for the SpMM: pattern once per system, values at the
+// first solve after an assembly (cn_system_set_positions changes them).
 static void ensure_ell(cn_system_t *S, hipStream_t s) {
   const int32_t n = S->n_free;
   if (n <= 0) return;
@@ -2621,6 +2621,8 @@ static void ensure_ell(cn_system_t *S, hipStream_t s) {
     }
     S->ov_nnz = op[n];
   }
+  if (S->ell_gen == S->vals_gen) return;  // the table holds these values
+  S->ell_gen = S->vals_gen;
   const bool ov = S->ov_nnz > 0;
   k_ell_fill<<<grid_for(n, 256), 256, 0, s>>>(
       n, S->ff_ptr.get(), S->ff_col.get(), S->ff_val.get(), S->ell_col.get(),
@@ -2630,7 +2632,7 @@ static void ensure_ell(cn_system_t *S, hipStream_t s) {
 }
 
 // Window row table for ranges of rpr rows (see kl_spmm_dot); refills the
-// weights on every call.  Returns false when the matrix does not fit the
+// weights at the first call after an assembly.  Returns false when the matrix does not fit the
 // window (a row of more than kWinEll entries, or more than kLdsFar far rows
 // in a range).
 static bool ensure_win(cn_system_t *S, int32_t rpr, hipStream_t s) {
@@ -2707,6 +2709,7 @@ static bool ensure_win(cn_system_t *S, int32_t rpr, hipStream_t s) {
     S->win_q.alloc(wq.size());
     S->win_pk32.release();
     S->win_pk64.release();
+    S->win_gen = 0;
     S->win_far_ptr.alloc(fp.size());
     S->win_far_rows.alloc(std::max<size_t>(fr.size(), 1));
     CN_HIP(hipMemcpyAsync(S->win_col.get(), wc.data(), sizeof(int32_t) * wc.size(),
@@ -2728,6 +2731,8 @@ static bool ensure_win(cn_system_t *S, int32_t rpr, hipStream_t s) {
     S->win_pk32.alloc((size_t)nblocks * win_block_bytes<float>());
     S->win_pk64.alloc((size_t)nblocks * win_block_bytes<double>());
   }
+  if (S->win_gen == S->vals_gen) return true;  // the packs hold these values
+  S->win_gen = S->vals_gen;
   k_win_pack<float><<<grid_for(total, 256), 256, 0, s>>>(total, S->win_col.get(),
                                                         S->win_q.get(), S->ff_val.get(),
                                                         S->win_pk32.get());
