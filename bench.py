@@ -275,9 +275,10 @@ def main():
         rel = 0.0
         torch.cuda.synchronize()
         t = time.perf_counter()
+        names = [only] if only else list(prof)  # the scopes that record events
         for _ in range(nsteps):
             step(info)
-            for name in prof:
+            for name in names:
                 ms = C.c_double(0.0)
                 cnt = lib.cn_profile_query(name.encode(), C.byref(ms))
                 prof[name][0] += cnt
