@@ -41,6 +41,8 @@ TRAFFIC_KERNELS = {
     "hsc_up": "void cn::kw_hsc_up<float, 4, 64>",
     "pd_update": "void cn::kw_pd_update<float, 4, 64>",
     "dot": "void cn::kw_dot<float, 4, 64>",
+    # the x update and FP64 residual of a replacement (KL_RES)
+    "resid": "void cn::kl_spmm<double, 6, 3>",
     "hsc_tail": "void cn::k_hsc_tail<float, 4>",
     # one coarse product: the MFMA kernel and the sum of its slices, on the
     # FP32 rounding of the coarse inverse (FP32 MFMA) or on the FP64 inverse
