@@ -660,7 +660,8 @@ def tune_for(system, B, args, pre, precond=None):
             "built_in_solve_ms": built_in, "candidates": len(r["candidates"]),
             "tried": tried,
             "how": "coordinate search over SpMM form / stripes, HSC level-pass grid cap, "
-                   "fused-tail size, down-sweep chain length, coarse product precision, "
+                   "fused-tail size, down-sweep chain length and grid cap, coarse product "
+                   "precision, "
                    "residual-replacement drop; full solves of this workload, best of 2"}
 
 

@@ -212,6 +212,8 @@ cn_status cn_system_batch(int32_t nsys, const cn_system *systems, cn_system *out
  *   chain_levels  HSC down sweep (k > 4): levels one group of launches
  *              chains through, each row written once a group, 1 to 24
  *              (0: 3; 1: one in-place pass per level);
+ *   down_cap   HSC down sweep: its launches' grid cap, as level_cap (0: the
+ *              level_cap); the up passes keep level_cap;
  *   coarse_prec  HSC coarse product of the mixed-precision (FP32) apply,
  *              k > 4: 1 = the FP64 coarse inverse on FP64 MFMA, 2 = its FP32
  *              rounding on FP32 MFMA (twice the rate; a preconditioner
@@ -224,6 +226,7 @@ typedef struct cn_tune {
   int32_t tail_rows;
   int32_t chain_levels;
   int32_t coarse_prec;
+  int32_t down_cap;
 } cn_tune;
 cn_status cn_system_set_tune(cn_system sys, const cn_tune *tune);
 cn_status cn_system_get_tune(cn_system sys, cn_tune *tune);

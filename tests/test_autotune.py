@@ -29,7 +29,7 @@ def test_cache_round_trip(tmp_path, fake_device):
     A._store(path, key, {"tune": {"spmm_form": 2, "level_cap": 4096}, "ms": 40.0})
     got = A.lookup(_Sys(), 768, "mixed", "hsc", path)
     assert got == {"spmm_form": 2, "spmm_gens": 0, "level_cap": 4096, "tail_rows": 0,
-                   "chain_levels": 0, "coarse_prec": 0}
+                   "chain_levels": 0, "coarse_prec": 0, "down_cap": 0}
     # another k, precision or preconditioner is another key
     assert A.lookup(_Sys(), 48, "mixed", "hsc", path) is None
     assert A.lookup(_Sys(), 768, "fp64", "hsc", path) is None

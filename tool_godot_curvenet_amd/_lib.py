@@ -67,7 +67,7 @@ class PcgOpts(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("spmm_form", i32), ("spmm_gens", i32), ("level_cap", i32), ("tail_rows", i32),
-                ("chain_levels", i32), ("coarse_prec", i32)]
+                ("chain_levels", i32), ("coarse_prec", i32), ("down_cap", i32)]
 
 
 class HscOpts(C.Structure):

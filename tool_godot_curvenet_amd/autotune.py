@@ -15,7 +15,10 @@ sides k and the precision:
   launch (one workgroup per column slice walks them with barriers);
 * ``chain_levels``: the levels one group of down-sweep launches chains
   through (each row is written once a group; longer groups write less and
-  gather rows of lower levels again).
+  gather rows of lower levels again), and ``down_cap``, the down sweep's own
+  grid cap;
+* ``coarse_prec``: the HSC coarse product of the mixed-precision apply on
+  the FP64 inverse or its FP32 rounding.
 
 With mixed precision it also picks the solve's ``inner_tol``, the factor by
 which a column's iterated residual falls between two residual replacements
@@ -46,9 +49,10 @@ import time
 
 from . import _lib as L
 
-FIELDS = ("spmm_form", "spmm_gens", "level_cap", "tail_rows", "chain_levels", "coarse_prec")
+FIELDS = ("spmm_form", "spmm_gens", "level_cap", "tail_rows", "chain_levels", "coarse_prec",
+          "down_cap")
 DEFAULT = {"spmm_form": 0, "spmm_gens": 0, "level_cap": 0, "tail_rows": 0, "chain_levels": 0,
-           "coarse_prec": 0}
+           "coarse_prec": 0, "down_cap": 0}
 INNER_TOLS = (0.0316, 0.01, 1e-3, 1e-4)   # the first is the built-in one
 
 
@@ -203,6 +207,11 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
                 t = run({**cur, "chain_levels": g})
                 if t < best:
                     best, cur = t, {**cur, "chain_levels": g}
+            # the down sweep's own grid cap
+            for cap in (16384, 65536):
+                t = run({**cur, "down_cap": cap})
+                if t < best:
+                    best, cur = t, {**cur, "down_cap": cap}
         # the coarse product on the FP32 rounding of the inverse (mixed
         # precision, the MFMA form k > 4): twice the matrix rate, against
         # what the perturbed preconditioner costs in iterations

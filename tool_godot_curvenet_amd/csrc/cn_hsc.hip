@@ -456,9 +456,13 @@ static void down_sweep(cn_hsc_t *H, int32_t t, int64_t k, const T *r, T *z, cons
   const int32_t G = sh.chain_levels > 0 ? sh.chain_levels : kChainLevels;
   if (const cn_hsc_t::ChainRecs *cr = chain_recs(H, t, G, may_alloc, s)) {
     const std::vector<ChainLaunch> ls = chain_launches(H, t, G);
+    // (the down sweep's launches take their own grid cap: a group's last
+    // launch gathers whole chains and wants more waves in flight than the
+    // up passes, whose blocks end with the r.z partial sums)
+    const int32_t cap = sh.down_cap > 0 ? sh.down_cap : sh.level_cap;
     for (size_t i = 0; i < ls.size(); ++i)
       level_chain_rec<T>(H, cr->rec.get() + cr->off[i], ls[i].nrows, k,
-                         ls[i].a == 0 ? r : z, z, ctl, s, sh.level_cap);
+                         ls[i].a == 0 ? r : z, z, ctl, s, cap);
     return;
   }
   for (int32_t a = 0; a < t;) {

@@ -2754,6 +2754,7 @@ static void use_shapes(cn_system_t *S, SysView<T> &V, int64_t k, hipStream_t s) 
   V.hsh.tail_rows = S->tune.tail_rows;
   V.hsh.chain_levels = S->tune.chain_levels;
   V.hsh.coarse_prec = S->tune.coarse_prec;
+  V.hsh.down_cap = S->tune.down_cap;
   if (S->tune.spmm_form != 1) use_win<T>(S, V, k, s);
 }
 template <>
@@ -3132,6 +3133,7 @@ static cn_status fused_solve(cn_system_t *S, cn_ic0_t *F, cn_hsc_t *H, const cn_
           (uint64_t)(uintptr_t)V.wpk, (uint64_t)(uintptr_t)V.ecol, (uint64_t)V.gens,
           (uint64_t)(uint32_t)V.hsh.level_cap, (uint64_t)(uint32_t)V.hsh.tail_rows,
           (uint64_t)(uint32_t)V.hsh.chain_levels, (uint64_t)(uint32_t)V.hsh.coarse_prec,
+          (uint64_t)(uint32_t)V.hsh.down_cap,
           (uint64_t)klf, (uint64_t)cur,
           bits_of(delta2), bits_of(floor2)};
       for (auto &e : W->graphs)
@@ -3375,7 +3377,7 @@ cn_status cn_system_set_tune(cn_system S, const cn_tune *t) {
       t->spmm_gens > (int32_t)kSpmmGens || t->spmm_gens == 3 || t->level_cap < 0 ||
       t->level_cap > 65536 || t->tail_rows < -1 || t->tail_rows > 16384 ||
       t->chain_levels < 0 || t->chain_levels > 24 || t->coarse_prec < 0 ||
-      t->coarse_prec > 2) {
+      t->coarse_prec > 2 || t->down_cap < 0 || t->down_cap > 65536) {
     set_error("cn_system_set_tune: bad arguments");
     return CN_BAD_ARG;
   }

@@ -216,12 +216,12 @@ class _System:
         return self.info.n_free
 
     TUNE_FIELDS = ("spmm_form", "spmm_gens", "level_cap", "tail_rows", "chain_levels",
-                   "coarse_prec")
+                   "coarse_prec", "down_cap")
 
     def set_tune(self, tune: dict | None):
         """Launch shapes of the solver kernels (cn_system_set_tune): a dict of
-        spmm_form, spmm_gens, level_cap, tail_rows, chain_levels, coarse_prec; missing
-        keys (or None) keep the built-in choice."""
+        spmm_form, spmm_gens, level_cap, tail_rows, chain_levels, coarse_prec, down_cap;
+        missing keys (or None) keep the built-in choice."""
         t = L.Tune()
         for f in self.TUNE_FIELDS:
             setattr(t, f, int((tune or {}).get(f, 0)))
