@@ -191,7 +191,7 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
         cur.update(best_t)
     # 2: the HSC level-pass grid cap
     if precond == "hsc":
-        for cap in (4096, 65536):
+        for cap in (2048, 4096, 65536):
             t = run({**cur, "level_cap": cap})
             if t < best:
                 best, cur = t, {**cur, "level_cap": cap}
