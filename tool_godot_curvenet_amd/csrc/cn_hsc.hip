@@ -242,10 +242,23 @@ k_coarse_gemv(int32_t nc, const double *__restrict__ inv, const T *__restrict__ 
   if (i >= nc) return;
   double acc[K] = {};
   const double *a = inv + (int64_t)i * nc;
-  for (int32_t j = lane; j < nc; j += 64) {
-    const double m = a[j];
+  // the lane's entries j = lane, lane + 64, ... in order, 4 of them loaded
+  // at a time (clamped into range, the clamped ones multiplied by 0)
+  for (int32_t j0 = lane; j0 < nc; j0 += 4 * 64) {
+    double m[4];
+    T w[4][K];
 #pragma unroll
-    for (int c = 0; c < K; ++c) acc[c] += m * (double)W[(int64_t)j * K + c];
+    for (int u = 0; u < 4; ++u) {
+      const int32_t j = min(j0 + u * 64, nc - 1);
+      const double mj = a[j];
+      m[u] = j0 + u * 64 < nc ? mj : 0.0;
+#pragma unroll
+      for (int c = 0; c < K; ++c) w[u][c] = W[(int64_t)j * K + c];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[c] += m[u] * (double)w[u][c];
   }
 #pragma unroll
   for (int c = 0; c < K; ++c) {
