@@ -60,11 +60,17 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 # iterations, 0.002 / 4096 rows 10.
 # On the level-9 icosphere (k = 48) the 24-level cap binds first, so it gets
 # more levels: theta 0.015 / 512 / 24 takes 29 + 36 iterations (76 poses/s),
-# 0.002 / 8192 / 48 14 + 18 (117 poses/s).
+# 0.002 / 8192 / 48 14 + 18 (117 poses/s).  With the FP32 coarse product the
+# larger blocks pay: 0.002 at 8192 / 12288 / 16384 rows 130.4 / 133.9 / 136.5
+# poses/s, 0.0015 at 12288 / 16384 137.3 / 132.2 (12 + 16 iterations), 0.001
+# at 12288 135.6.
 # The 64-mesh batch (k = 3, its hierarchy allowed 48 levels): 0.015 / 512
-# takes 30 iterations (2,455 solves/s), 0.003 / 4096 17 (3,009 solves/s).
+# takes 30 iterations (2,455 solves/s), 0.003 / 4096 17 (3,009 solves/s);
+# with the FP32 coarse inverse 0.003 / 4096 / 5120 / 6144 / 8192 / 12288
+# rows gave 3,130 / 3,154 / 3,350 / 3,313 / 3,188 solves/s (17, 17, 16, 16,
+# 16 iterations), 0.0025 / 0.0035 / 0.004 at 6144 3,172 / 3,245 / 3,236.
 HSC_SHAPE = {"default": (0.015, 512, 24), "capsules100k": (0.002, 4096, 24),
-             "icosphere9": (0.002, 8192, 48), "meshes64": (0.003, 4096, 24)}
+             "icosphere9": (0.0015, 12288, 48), "meshes64": (0.003, 6144, 24)}
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 

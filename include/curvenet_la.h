@@ -214,11 +214,12 @@ cn_status cn_system_batch(int32_t nsys, const cn_system *systems, cn_system *out
  *              (0: 3; 1: one in-place pass per level);
  *   down_cap   HSC down sweep: its launches' grid cap, as level_cap (0: the
  *              level_cap); the up passes keep level_cap;
- *   coarse_prec  HSC coarse product of the mixed-precision (FP32) apply,
- *              k > 4: 1 = the FP64 coarse inverse on FP64 MFMA, 2 = its FP32
- *              rounding on FP32 MFMA (twice the rate; a preconditioner
- *              perturbed at FP32 rounding) (0: 1).  FP64 applies always
- *              take the FP64 inverse. */
+ *   coarse_prec  HSC coarse product of the mixed-precision (FP32) apply:
+ *              1 = the FP64 coarse inverse (on FP64 MFMA for k > 4), 2 = its
+ *              FP32 rounding (on FP32 MFMA for k > 4, twice the rate; for
+ *              k <= 4 half the bytes read; a preconditioner perturbed at
+ *              FP32 rounding) (0: 1).  FP64 applies and the fused tail
+ *              always take the FP64 inverse. */
 typedef struct cn_tune {
   int32_t spmm_form;
   int32_t spmm_gens;

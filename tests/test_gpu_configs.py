@@ -95,15 +95,25 @@ def _meshes64_rhs(laps, seed):
     return [l.harmonic_rhs(rng.uniform(-0.2, 0.2, size=(l.n_con, 3))) for l in laps]
 
 
-def test_c3b_meshes64_batch_hsc(gpu, meshes64):
+# the default hierarchy, and the benched one (theta 0.003, up to 6,144
+# coarse rows) with the coarse product on the FP32 rounding of the inverse
+@pytest.mark.parametrize("shape,tune", [({}, {}),
+                                        ({"theta": 0.003, "coarse_max": 6144},
+                                         {"coarse_prec": 2})])
+def test_c3b_meshes64_batch_hsc(gpu, meshes64, shape, tune):
     from tool_godot_curvenet_amd import HSC, meshgen
     laps, batch = meshes64
     sizes = [l.n_free for l in laps]
     assert len(laps) == 64 and min(sizes) < 4000 and max(sizes) > 100_000
     Bs = _meshes64_rhs(laps, meshgen.BASE_SEED + 40)
-    x, info = batch.solve(batch.stack(Bs), precond="hsc", precision="mixed", tol=1e-9,
-                          hsc=HSC(batch, max_levels=48))
-    _say(f"C3b: {len(laps)} meshes, n={batch.n_free} iters={info.iters} "
+    saved = batch.get_tune()
+    batch.set_tune(tune)
+    try:
+        x, info = batch.solve(batch.stack(Bs), precond="hsc", precision="mixed", tol=1e-9,
+                              hsc=HSC(batch, max_levels=48, **shape))
+    finally:
+        batch.set_tune(saved)
+    _say(f"C3b {shape} {tune}: {len(laps)} meshes, n={batch.n_free} iters={info.iters} "
          f"relres_max={info.relres_max:.2e}")
     assert info.status == 0
     worst = 0.0

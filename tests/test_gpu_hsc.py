@@ -161,11 +161,12 @@ def test_chain_levels_direction_and_convergence(gpu, g, precision, tol):
     assert np.all(np.linalg.norm(R, axis=0) <= 1e-9 * np.linalg.norm(B, axis=0))
 
 
-# The FP32 coarse product (coarse_prec 2) on 64 x 64 tiles (k = 64) and
-# 128 x 128 tiles (k = 520, 3,000+ coarse rows): one mixed PCG step from 0
-# follows the oracle hierarchy's apply to FP32 rounding, and the solve
-# converges; an FP64 solve on the same tune keeps the FP64 product.
-@pytest.mark.parametrize("k", [64, 520])
+# The FP32 coarse product (coarse_prec 2) in the one-wavefront-a-row vector
+# form (k = 3), on 64 x 64 tiles (k = 64) and 128 x 128 tiles (k = 520;
+# 3,000+ coarse rows): one mixed PCG step from 0 follows the oracle
+# hierarchy's apply to FP32 rounding, and the solve converges; an FP64 solve
+# on the same tune keeps the FP64 product.
+@pytest.mark.parametrize("k", [3, 64, 520])
 def test_coarse_fp32_direction_and_convergence(gpu, k):
     from tool_godot_curvenet_amd import HSC
     from tool_godot_curvenet_amd import meshgen as G

@@ -203,7 +203,7 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
                     best, cur = t, {**cur, "tail_rows": rows}
         # 4: the down sweep's chain length (the wide form, k > 4)
         if k > 4:
-            for g in (2, 4, 5, 6):
+            for g in (2, 4, 5, 6, 8):
                 t = run({**cur, "chain_levels": g})
                 if t < best:
                     best, cur = t, {**cur, "chain_levels": g}
@@ -213,9 +213,10 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
                 if t < best:
                     best, cur = t, {**cur, "down_cap": cap}
         # the coarse product on the FP32 rounding of the inverse (mixed
-        # precision, the MFMA form k > 4): twice the matrix rate, against
-        # what the perturbed preconditioner costs in iterations
-        if k > 4 and precision == "mixed":
+        # precision): twice the matrix rate (k > 4, MFMA) or half the bytes
+        # (k <= 4), against what the perturbed preconditioner costs in
+        # iterations
+        if precision == "mixed":
             t = run({**cur, "coarse_prec": 2})
             if t < best:
                 best, cur = t, {**cur, "coarse_prec": 2}

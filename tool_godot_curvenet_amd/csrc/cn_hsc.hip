@@ -229,11 +229,12 @@ k_hsc_up_n(int32_t nF, const int32_t *__restrict__ Fg, const T *__restrict__ din
 }
 
 // z[rows[i]] = sum_j inv[i][j] W[j] for K <= 4 columns: wavefront i of the
-// grid takes coarse row i, lane l the entries j = l, l + 64, ... (FP64),
-// and the lane sums meet in a fixed butterfly
-template <typename T, int K>
+// grid takes coarse row i, lane l the entries j = l, l + 64, ... (summed in
+// FP64 over the FP64 inverse or its FP32 rounding, MT), and the lane sums
+// meet in a fixed butterfly
+template <typename T, typename MT, int K>
 __global__ void __launch_bounds__(kBlock)
-k_coarse_gemv(int32_t nc, const double *__restrict__ inv, const T *__restrict__ W,
+k_coarse_gemv(int32_t nc, const MT *__restrict__ inv, const T *__restrict__ W,
               const int32_t *__restrict__ rows, T *__restrict__ z,
               const int32_t *__restrict__ ctl) {
   if (ctl && ctl_idle(ctl)) return;
@@ -241,7 +242,7 @@ k_coarse_gemv(int32_t nc, const double *__restrict__ inv, const T *__restrict__ 
   const int lane = threadIdx.x & 63;
   if (i >= nc) return;
   double acc[K] = {};
-  const double *a = inv + (int64_t)i * nc;
+  const MT *a = inv + (int64_t)i * nc;
   // the lane's entries j = lane, lane + 64, ... in order, 4 of them loaded
   // at a time (clamped into range, the clamped ones multiplied by 0)
   for (int32_t j0 = lane; j0 < nc; j0 += 4 * 64) {
@@ -250,7 +251,7 @@ k_coarse_gemv(int32_t nc, const double *__restrict__ inv, const T *__restrict__ 
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int32_t j = min(j0 + u * 64, nc - 1);
-      const double mj = a[j];
+      const double mj = (double)a[j];
       m[u] = j0 + u * 64 < nc ? mj : 0.0;
 #pragma unroll
       for (int c = 0; c < K; ++c) w[u][c] = W[(int64_t)j * K + c];
@@ -727,11 +728,22 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
       const int64_t tot = (int64_t)H->nc * k;
       k_gather_rows<T><<<grid_for(tot, 256), 256, 0, s>>>(H->nc, H->coarse_rows.get(), k,
                                                         L == 0 ? r : z, W, ctl);
-#define G_(KK)                                                                            \
-  k_coarse_gemv<T, KK><<<(unsigned)((H->nc + kWaves - 1) / kWaves), kBlock, 0, s>>>(      \
-      H->nc, H->coarse_inv.get(), W, H->coarse_rows.get(), z, ctl)
-      switch (k) { case 1: G_(1); break; case 2: G_(2); break; case 3: G_(3); break; default: G_(4); }
+      // (coarse_prec 2: the FP32 rounding of the inverse in FP32 applies,
+      // half the bytes of the product's one pass over it)
+      const unsigned gg = (unsigned)((H->nc + kWaves - 1) / kWaves);
+      if (sizeof(T) == 4 && sh.coarse_prec == 2) {
+#define G_(KK)                                                                  \
+  k_coarse_gemv<T, float, KK><<<gg, kBlock, 0, s>>>(H->nc, H->coarse_inv32.get(), W, \
+                                                    H->coarse_rows.get(), z, ctl)
+        switch (k) { case 1: G_(1); break; case 2: G_(2); break; case 3: G_(3); break; default: G_(4); }
 #undef G_
+      } else {
+#define G_(KK)                                                                   \
+  k_coarse_gemv<T, double, KK><<<gg, kBlock, 0, s>>>(H->nc, H->coarse_inv.get(), W, \
+                                                     H->coarse_rows.get(), z, ctl)
+        switch (k) { case 1: G_(1); break; case 2: G_(2); break; case 3: G_(3); break; default: G_(4); }
+#undef G_
+      }
     } else {
       const float *M32 = sizeof(T) == 4 && sh.coarse_prec == 2 ? H->coarse_inv32.get() : nullptr;
       rzr = coarse_product<T>(H->nc, (int)k, H->coarse_inv.get(), M32, H->coarse_rows.get(),
