@@ -311,7 +311,13 @@ k_coarse_mm(int m, int k, int kchunk, const MT *__restrict__ M,
   // 4 k x 16 columns of B) over distinct LDS banks
   __shared__ MT As[2][BM][kCK + (sizeof(MT) == 4 ? 4 : 1)];
   __shared__ T Bs[2][kCK][BN + 16];
-  __shared__ int32_t Rs[kCMaxChunk + 2 * kCK];
+  // chunks loaded ahead in registers: the 64 x 64 tiles (small k, or a
+  // small coarse block) have few MFMAs a chunk to hide a load behind and
+  // take 4 (at m = 12,048, k = 48: FP32 302.6 -> 295.7 us, FP64 407.4 ->
+  // 378.1 us; the occupancy is set by LDS either way); the 128 x 128 tiles
+  // are bound by the matrix cores and keep one (more costs them occupancy)
+  constexpr int NS = BM == 64 ? 4 : 1;
+  __shared__ int32_t Rs[kCMaxChunk + (NS + 1) * kCK];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
   // XCD-contiguous tile order (x fastest, then y, then the slice z)
   const int nx = gridDim.x, ny = gridDim.y, nb = nx * ny * gridDim.z;
@@ -325,7 +331,7 @@ k_coarse_mm(int m, int k, int kchunk, const MT *__restrict__ M,
   const int br = tid / (BN / BE), bc = (tid % (BN / BE)) * BE;
   const MT *Mrow = M + (int64_t)min(bi + ar, m - 1) * m;
   const int cb = bj + bc;  // the thread's first S column (clamped below)
-  for (int q = tid; q < kchunk + 2 * kCK; q += NT) Rs[q] = rows[min(jb + q, m - 1)];
+  for (int q = tid; q < kchunk + (NS + 1) * kCK; q += NT) Rs[q] = rows[min(jb + q, m - 1)];
   __syncthreads();
   MT pa[AE];
   T pb[BE];
@@ -350,28 +356,77 @@ k_coarse_mm(int m, int k, int kchunk, const MT *__restrict__ M,
 #pragma unroll
     for (int u = 0; u < BE; ++u) Bs[sb][br][bc + u] = pb[u];
   };
-  fetch(jb);
-  stage(jb, 0);
-  __syncthreads();
-  fetch(jb + kCK);
-  int sb = 0;
-  for (int j0 = jb; j0 < je; j0 += kCK, sb ^= 1) {
-    __builtin_amdgcn_sched_barrier(0);
+  if constexpr (NS == 1) {
+    fetch(jb);
+    stage(jb, 0);
+    __syncthreads();
+    fetch(jb + kCK);
+    int sb = 0;
+    for (int j0 = jb; j0 < je; j0 += kCK, sb ^= 1) {
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int t = 0; t < kCK; t += 4) {
-      MT a[TM], b[TN];
+      for (int t = 0; t < kCK; t += 4) {
+        MT a[TM], b[TN];
 #pragma unroll
-      for (int x = 0; x < TM; ++x) a[x] = As[sb][wi + 16 * x + (l & 15)][t + (l >> 4)];
+        for (int x = 0; x < TM; ++x) a[x] = As[sb][wi + 16 * x + (l & 15)][t + (l >> 4)];
 #pragma unroll
-      for (int y = 0; y < TN; ++y) b[y] = (MT)Bs[sb][t + (l >> 4)][wj + 16 * y + (l & 15)];
+        for (int y = 0; y < TN; ++y) b[y] = (MT)Bs[sb][t + (l >> 4)][wj + 16 * y + (l & 15)];
 #pragma unroll
-      for (int x = 0; x < TM; ++x)
+        for (int x = 0; x < TM; ++x)
 #pragma unroll
-        for (int y = 0; y < TN; ++y) acc[x][y] = F::mma(a[x], b[y], acc[x][y]);
+          for (int y = 0; y < TN; ++y) acc[x][y] = F::mma(a[x], b[y], acc[x][y]);
+      }
+      stage(j0 + kCK, sb ^ 1);
+      __syncthreads();  // stage sb read by every wave, stage sb ^ 1 written
+      fetch(j0 + 2 * kCK);
     }
-    stage(j0 + kCK, sb ^ 1);
-    __syncthreads();  // stage sb read by every wave, stage sb ^ 1 written
-    fetch(j0 + 2 * kCK);
+  } else {
+    // register slot q holds chunk c + 1 at chunk c = q (mod NS)
+    MT ra[NS][AE];
+    T rb[NS][BE];
+    auto fetch_to = [&](int j0, MT *qa, T *qb) {
+#pragma unroll
+      for (int u = 0; u < AE; ++u) qa[u] = Mrow[min(j0 + ac + u, m - 1)];
+      const T *Srow = S + (int64_t)Rs[j0 - jb + br] * k;
+#pragma unroll
+      for (int u = 0; u < BE; ++u) qb[u] = Srow[min(cb + u, k - 1)];
+    };
+    auto stage_from = [&](int j0, int sb, const MT *qa, const T *qb) {
+#pragma unroll
+      for (int u = 0; u < AE; ++u) As[sb][ar][ac + u] = j0 + ac + u < je ? qa[u] : MT(0);
+#pragma unroll
+      for (int u = 0; u < BE; ++u) Bs[sb][br][bc + u] = qb[u];
+    };
+    fetch_to(jb, ra[0], rb[0]);
+    stage_from(jb, 0, ra[0], rb[0]);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NS; ++q) fetch_to(jb + (q + 1) * kCK, ra[q], rb[q]);
+    int sb = 0;
+    for (int jg = jb; jg < je; jg += NS * kCK) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const int j0 = jg + q * kCK;
+        if (j0 >= je) break;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < kCK; t += 4) {
+          MT a[TM], b[TN];
+#pragma unroll
+          for (int x = 0; x < TM; ++x) a[x] = As[sb][wi + 16 * x + (l & 15)][t + (l >> 4)];
+#pragma unroll
+          for (int y = 0; y < TN; ++y) b[y] = (MT)Bs[sb][t + (l >> 4)][wj + 16 * y + (l & 15)];
+#pragma unroll
+          for (int x = 0; x < TM; ++x)
+#pragma unroll
+            for (int y = 0; y < TN; ++y) acc[x][y] = F::mma(a[x], b[y], acc[x][y]);
+        }
+        stage_from(j0 + kCK, sb ^ 1, ra[q], rb[q]);
+        __syncthreads();
+        fetch_to(j0 + (NS + 1) * kCK, ra[q], rb[q]);
+        sb ^= 1;
+      }
+    }
   }
   MT *P = part + (int64_t)tz * m * k;
 #pragma unroll
@@ -434,7 +489,10 @@ static bool coarse_big_tiles(int m, int k) { return m >= 3072 && k >= 512; }
 // Slices of the j range: the fewest that give about 3 blocks of 4 waves a
 // CU (more slices cost more partial sums to add), at most 16, each at least
 // two chunks and at most kCMaxChunk long.  Measured at m = 2,043 and
-// k = 768: 1 / 2 / 4 / 8 slices 198 / 156 / 150 / 160 us with the sum.
+// k = 768: 1 / 2 / 4 / 8 slices 198 / 156 / 150 / 160 us with the sum.  At
+// m = 12,048, k = 48 (FP32) 8 slices take 296 us, and 13 slices, one block
+// a CU more than fits at once in two full rounds, 304 us: that product is
+// bound by neither the chunk count nor the matrix cores.
 static int coarse_splits(int m, int k) {
   const bool big = coarse_big_tiles(m, k);
   const int bm = big ? 128 : 64, bn = big ? 128 : 64;
