@@ -50,9 +50,13 @@ TRAFFIC_KERNELS = {
                    "void cn::k_coarse_sum<float, float>"),
     "hsc_coarse_f64": ("void cn::k_coarse_mm<float, double, 128, 128, 4, 2>",
                        "void cn::k_coarse_sum<float, double>"),
+    # on the inverse split into two bfloat16 matrices (coarse_prec 3)
+    "hsc_coarse_bf3": ("void cn::k_coarse_mm_bf3<float, 128, 128, 4, 2>",
+                       "void cn::k_coarse_sum<float, float>"),
 }
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 # HSC hierarchy per workload: (drop threshold, coarse rows at most, levels
 # at most).  With 768 columns the dense coarse product runs on the FP64
 # matrix cores at about 0.65 of peak, so a small threshold and a large coarse
@@ -403,16 +407,22 @@ def main():
             nc = pre.info.n_coarse
             cms = prof["hsc_coarse"][1]
             ncoarse = prof["hsc_coarse"][0]
-            # the coarse product on FP32 (coarse_prec 2) or FP64 matrix cores
-            c32 = args.precision == "mixed" and lap.get_tune()["coarse_prec"] == 2
-            cprec, cpeak = (("FP32", FP32_MFMA_PEAK_TFLOPS) if c32 else
+            # the coarse product on FP32 (coarse_prec 2), split-bfloat16
+            # (coarse_prec 3, products of at least 3,072 coarse rows and 512
+            # columns) or FP64 matrix cores
+            cp = lap.get_tune()["coarse_prec"] if args.precision == "mixed" else 1
+            cbf = cp == 3 and nc >= 3072 and k >= 512
+            c32 = cp >= 2 and not cbf
+            cprec, cpeak = (("split bfloat16, 3 products", BF16_MFMA_PEAK_TFLOPS) if cbf else
+                            ("FP32", FP32_MFMA_PEAK_TFLOPS) if c32 else
                             ("FP64", FP64_MFMA_PEAK_TFLOPS))
-            ctag = "hsc_coarse" if c32 else "hsc_coarse_f64"
+            ctag = "hsc_coarse_bf3" if cbf else "hsc_coarse" if c32 else "hsc_coarse_f64"
             if l0 < 0 and ncoarse > 0:
                 # the coarse product: 2 n_c^2 k flops per apply that does work
-                mt = "float" if c32 else "double"
-                mfma[ctag] = (f"k_coarse_mm<float,{mt},128,128,4,2> + "
-                              f"k_coarse_sum ({cprec} MFMA)", ncoarse, cms,
+                # (the split product runs three times as many on the MFMAs)
+                kn = ("k_coarse_mm_bf3<float,128,128,4,2>" if cbf else
+                      f"k_coarse_mm<float,{'float' if c32 else 'double'},128,128,4,2>")
+                mfma[ctag] = (f"{kn} + k_coarse_sum ({cprec} MFMA)", ncoarse, cms,
                               nwork * 2.0 * nc * nc * k, cpeak)
             tflops = 2.0 * nc * nc * k * nwork / (cms * 1e-3) / 1e12 if cms > 0 else 0.0
             if l0 >= 0:
@@ -444,7 +454,7 @@ def main():
         scope["pd_update"] = "bwd"
         if "spmm" in kernels:
             scope["spmm"] = "spmm_lds" if prof["spmm_lds"][0] > 0 else "spmm"
-        for t in ("hsc_coarse", "hsc_coarse_f64"):
+        for t in ("hsc_coarse", "hsc_coarse_f64", "hsc_coarse_bf3"):
             scope[t] = "hsc_coarse"
         return kernels, coarse, scope
 

@@ -18,7 +18,8 @@ sides k and the precision:
   gather rows of lower levels again), and ``down_cap``, the down sweep's own
   grid cap;
 * ``coarse_prec``: the HSC coarse product of the mixed-precision apply on
-  the FP64 inverse or its FP32 rounding.
+  the FP64 inverse, its FP32 rounding, or (wide products) its split into
+  two bfloat16 matrices.
 
 With mixed precision it also picks the solve's ``inner_tol``, the factor by
 which a column's iterated residual falls between two residual replacements
@@ -216,10 +217,13 @@ def autotune(system, B, precond: str = "hsc", precision: str = "mixed", tol: flo
         # precision): twice the matrix rate (k > 4, MFMA) or half the bytes
         # (k <= 4), against what the perturbed preconditioner costs in
         # iterations
+        # (k > 4 also the split-bfloat16 product, 3: three bfloat16 MFMAs a
+        # product at eight times the FP32 matrix rate each, a b to 2^-16)
         if precision == "mixed":
-            t = run({**cur, "coarse_prec": 2})
-            if t < best:
-                best, cur = t, {**cur, "coarse_prec": 2}
+            for cp in ((2, 3) if k > 4 else (2,)):
+                t = run({**cur, "coarse_prec": cp})
+                if t < best:
+                    best, cur = t, {**cur, "coarse_prec": cp}
     # 5: the residual-replacement drop (mixed precision); the candidates
     # differ by a replacement or two (a few per cent), so each, the built-in
     # one included, is timed again over more solves

@@ -440,6 +440,147 @@ k_coarse_mm(int m, int k, int kchunk, const MT *__restrict__ M,
       }
 }
 
+// bfloat16 (the top 16 bits of an FP32) nearest to a finite f, ties to even
+__device__ __forceinline__ uint16_t bf16_rn(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+// The coarse inverse as two bfloat16 matrices, a = hi + lo with hi the
+// nearest bfloat16 to a and lo the nearest to a - hi (FP64 difference), each
+// m rows of ld >= m entries, zero past column m.
+__global__ void k_split_bf16(int m, int ld, const double *__restrict__ a,
+                             uint16_t *__restrict__ hi, uint16_t *__restrict__ lo) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)m * ld) return;
+  const int i = (int)(t / ld), j = (int)(t % ld);
+  const double v = j < m ? a[(int64_t)i * m + j] : 0.0;
+  const uint16_t h = bf16_rn((float)v);
+  hi[t] = h;
+  lo[t] = bf16_rn((float)(v - (double)bf16_f(h)));
+}
+
+// The coarse product on bfloat16 matrix cores (v_mfma_f32_16x16x16bf16_1k)
+// to about FP32 accuracy: with a = a_h + a_l from the split inverse and each
+// staged coarse value b split the same way in registers, a b is taken as
+// a_h b_h + a_h b_l + a_l b_h (the dropped a_l b_l and the roundings of the
+// low parts are 2^-16 of a b), three MFMAs into one FP32 accumulator.  The
+// tile, slice and staging scheme is k_coarse_mm's with chunks of 16 j and
+// both halves of the inverse staged in LDS; A lane l holds rows l % 16 and
+// j = 4 (l / 16) .. + 3, B lane l the same j of column l % 16, and the
+// result the layout of the 16 x 16 x 4 FP32 MFMA.
+template <typename T, int BM, int BN, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN)
+k_coarse_mm_bf3(int m, int ld, int k, int kchunk, const uint16_t *__restrict__ Mh,
+                const uint16_t *__restrict__ Ml, const int32_t *__restrict__ rows,
+                const T *__restrict__ S, float *__restrict__ part,
+                const int32_t *__restrict__ ctl) {
+  if (ctl && ctl_idle(ctl)) return;
+  constexpr int NT = 64 * WM * WN;
+  constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
+  constexpr int AE = BM * kCK / NT;  // inverse entries a thread stages per chunk
+  constexpr int BE = kCK * BN / NT;  // S values a thread stages per chunk
+  static_assert(kCK == 16 && AE == 4 && BN % BE == 0, "tile shape");
+  __shared__ __align__(16) uint16_t Ah[2][BM][kCK + 8];
+  __shared__ __align__(16) uint16_t Al[2][BM][kCK + 8];
+  __shared__ T Bs[2][kCK][BN + 16];
+  __shared__ int32_t Rs[kCMaxChunk + 2 * kCK];
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int nx = gridDim.x, ny = gridDim.y, nb = nx * ny * gridDim.z;
+  const int hw = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  const int lid = hw < nb - nb % 8 ? (hw % 8) * (nb / 8) + hw / 8 : hw;
+  const int tx = lid % nx, ty = (lid / nx) % ny, tz = lid / (nx * ny);
+  const int bi = ty * BM, bj = tx * BN;
+  const int wi = (wave / WN) * (BM / WM), wj = (wave % WN) * (BN / WN);
+  const int jb = tz * kchunk, je = min(m, jb + kchunk);
+  const int ar = tid / (kCK / AE), ac = (tid % (kCK / AE)) * AE;
+  const int br = tid / (BN / BE), bc = (tid % (BN / BE)) * BE;
+  const int64_t arow = (int64_t)min(bi + ar, m - 1) * ld;
+  const int cb = bj + bc;
+  for (int q = tid; q < kchunk + 2 * kCK; q += NT) Rs[q] = rows[min(jb + q, m - 1)];
+  __syncthreads();
+  uint2 ph, pl;
+  T pb[BE];
+  // j0 + ac is a multiple of 4 and, where any of its 4 entries is < m,
+  // at most ld - 4; past the slice the entries are zeroed when staged
+  auto fetch = [&](int j0) {
+    const int jj = min(j0 + ac, ld - 4);
+    ph = *reinterpret_cast<const uint2 *>(Mh + arow + jj);
+    pl = *reinterpret_cast<const uint2 *>(Ml + arow + jj);
+    const T *Srow = S + (int64_t)Rs[j0 - jb + br] * k;
+#pragma unroll
+    for (int u = 0; u < BE; ++u) pb[u] = Srow[min(cb + u, k - 1)];
+  };
+  auto stage = [&](int j0, int sb) {
+    uint2 h = ph, lo = pl;
+    const int n_in = je - (j0 + ac);  // entries of the 4 inside the slice
+    if (n_in < 4) {
+      const uint64_t keep = n_in <= 0 ? 0ull : (~0ull >> (64 - 16 * n_in));
+      uint64_t hv = ((uint64_t)h.y << 32 | h.x) & keep, lv = ((uint64_t)lo.y << 32 | lo.x) & keep;
+      h = make_uint2((uint32_t)hv, (uint32_t)(hv >> 32));
+      lo = make_uint2((uint32_t)lv, (uint32_t)(lv >> 32));
+    }
+    *reinterpret_cast<uint2 *>(&Ah[sb][ar][ac]) = h;
+    *reinterpret_cast<uint2 *>(&Al[sb][ar][ac]) = lo;
+#pragma unroll
+    for (int u = 0; u < BE; ++u) Bs[sb][br][bc + u] = pb[u];
+  };
+  v4f acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = v4f{0, 0, 0, 0};
+  fetch(jb);
+  stage(jb, 0);
+  __syncthreads();
+  fetch(jb + kCK);
+  int sb = 0;
+  for (int j0 = jb; j0 < je; j0 += kCK, sb ^= 1) {
+    __builtin_amdgcn_sched_barrier(0);
+    s4 ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+    for (int x = 0; x < TM; ++x) {
+      ah[x] = *reinterpret_cast<const s4 *>(&Ah[sb][wi + 16 * x + (l & 15)][4 * (l >> 4)]);
+      al[x] = *reinterpret_cast<const s4 *>(&Al[sb][wi + 16 * x + (l & 15)][4 * (l >> 4)]);
+    }
+#pragma unroll
+    for (int y = 0; y < TN; ++y) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float b = (float)Bs[sb][4 * (l >> 4) + u][wj + 16 * y + (l & 15)];
+        const uint16_t h = bf16_rn(b);
+        bh[y][u] = (short)h;
+        bl[y][u] = (short)bf16_rn(b - bf16_f(h));
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TN; ++y) {
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[x], bh[y], acc[x][y], 0, 0, 0);
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[x], bl[y], acc[x][y], 0, 0, 0);
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[x], bh[y], acc[x][y], 0, 0, 0);
+      }
+    stage(j0 + kCK, sb ^ 1);
+    __syncthreads();  // stage sb read by every wave, stage sb ^ 1 written
+    fetch(j0 + 2 * kCK);
+  }
+  float *P = part + (int64_t)tz * m * k;
+#pragma unroll
+  for (int x = 0; x < TM; ++x)
+#pragma unroll
+    for (int y = 0; y < TN; ++y)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = bi + wi + 16 * x + 4 * (l >> 4) + v, j = bj + wj + 16 * y + (l & 15);
+        if (i < m && j < k) P[(int64_t)i * k + j] = acc[x][y][v];
+      }
+}
+
 // Z[rows[i]] = the sum of the slices' partials, in slice order.  A block
 // takes 256 columns of kSumRows rows; a thread's rows are loaded together,
 // slice by slice.  rz != nullptr: it also writes, per column, the sum over
@@ -543,6 +684,36 @@ int coarse_product(int m, int k, const double *M, const float *M32, const int32_
 }
 
 int coarse_rz_rows(int m) { return (m + kSumRows - 1) / kSumRows; }
+
+// The split-bfloat16 product (k_coarse_mm_bf3) runs where the 128 x 128
+// tiles do; other shapes keep the FP32 one.
+bool coarse_bf3_applies(int m, int k) { return coarse_big_tiles(m, k); }
+
+void coarse_split_bf16(int m, int ld, const double *a, uint16_t *hi, uint16_t *lo,
+                       hipStream_t s) {
+  const int64_t n = (int64_t)m * ld;
+  if (n == 0) return;
+  k_split_bf16<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, ld, a, hi, lo);
+  CN_HIP(hipGetLastError());
+}
+
+// z[rows] = S_c^-1 s[rows] on the split inverse (hi, lo: m rows of ld);
+// FP32 applies where coarse_bf3_applies.  Returns the partial rows written
+// to rz (0 with rz = nullptr).
+int coarse_product_bf3(int m, int ld, int k, const uint16_t *Mh, const uint16_t *Ml,
+                       const int32_t *rows, const float *S, float *Z, const int32_t *ctl,
+                       double *part, hipStream_t s, double *rz) {
+  if (m <= 0 || k <= 0) return 0;
+  const int sp = coarse_splits(m, k);
+  const int chunk = ((m + sp - 1) / sp + kCK - 1) / kCK * kCK;
+  float *pf = reinterpret_cast<float *>(part);
+  const dim3 g((unsigned)((k + 127) / 128), (unsigned)((m + 127) / 128), (unsigned)sp);
+  k_coarse_mm_bf3<float, 128, 128, 4, 2><<<g, 512, 0, s>>>(m, ld, k, chunk, Mh, Ml, rows, S,
+                                                           pf, ctl);
+  const dim3 gs((unsigned)((k + 255) / 256), (unsigned)((m + kSumRows - 1) / kSumRows));
+  k_coarse_sum<float, float><<<gs, 256, 0, s>>>(m, k, sp, pf, rows, S, Z, ctl, rz);
+  return rz ? (int)gs.y : 0;
+}
 
 template int coarse_product<float>(int, int, const double *, const float *, const int32_t *,
                                    const float *, float *, const int32_t *, double *,

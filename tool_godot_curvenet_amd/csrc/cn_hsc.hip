@@ -28,6 +28,12 @@
 namespace cn {
 
 size_t coarse_product_work(int m, int k);
+bool coarse_bf3_applies(int m, int k);
+void coarse_split_bf16(int m, int ld, const double *a, uint16_t *hi, uint16_t *lo,
+                       hipStream_t s);
+int coarse_product_bf3(int m, int ld, int k, const uint16_t *Mh, const uint16_t *Ml,
+                       const int32_t *rows, const float *S, float *Z, const int32_t *ctl,
+                       double *part, hipStream_t s, double *rz);
 int coarse_rz_rows(int m);
 template <typename T>
 int coarse_product(int m, int k, const double *M, const float *M32, const int32_t *rows,
@@ -731,7 +737,7 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
       // (coarse_prec 2: the FP32 rounding of the inverse in FP32 applies,
       // half the bytes of the product's one pass over it)
       const unsigned gg = (unsigned)((H->nc + kWaves - 1) / kWaves);
-      if (sizeof(T) == 4 && sh.coarse_prec == 2) {
+      if (sizeof(T) == 4 && sh.coarse_prec >= 2) {
 #define G_(KK)                                                                  \
   k_coarse_gemv<T, float, KK><<<gg, kBlock, 0, s>>>(H->nc, H->coarse_inv32.get(), W, \
                                                     H->coarse_rows.get(), z, ctl)
@@ -745,10 +751,23 @@ void hsc_apply(cn_hsc_t *H, int64_t k, const T *r, T *z, hipStream_t s, const in
 #undef G_
       }
     } else {
-      const float *M32 = sizeof(T) == 4 && sh.coarse_prec == 2 ? H->coarse_inv32.get() : nullptr;
-      rzr = coarse_product<T>(H->nc, (int)k, H->coarse_inv.get(), M32, H->coarse_rows.get(),
-                               L == 0 ? r : z, z, ctl, reinterpret_cast<double *>(work), s,
-                               rz_part);
+      // coarse_prec 3: the split-bfloat16 product where it runs, else FP32
+      const bool bf3 = sh.coarse_prec == 3 && H->coarse_bh.get() &&
+                       coarse_bf3_applies(H->nc, (int)k);
+      if constexpr (sizeof(T) == 4) {
+        if (bf3) {
+          rzr = coarse_product_bf3(H->nc, H->coarse_ld, (int)k, H->coarse_bh.get(),
+                                   H->coarse_bl.get(), H->coarse_rows.get(), L == 0 ? r : z,
+                                   z, ctl, reinterpret_cast<double *>(work), s, rz_part);
+        }
+      }
+      if (sizeof(T) != 4 || !bf3) {
+        const float *M32 =
+            sizeof(T) == 4 && sh.coarse_prec >= 2 ? H->coarse_inv32.get() : nullptr;
+        rzr = coarse_product<T>(H->nc, (int)k, H->coarse_inv.get(), M32,
+                                H->coarse_rows.get(), L == 0 ? r : z, z, ctl,
+                                reinterpret_cast<double *>(work), s, rz_part);
+      }
     }
   }
   {
@@ -888,6 +907,14 @@ void hsc_prepare(cn_hsc_t *H, hipStream_t s) {
     k_round_f32<<<grid_for(nn, 256), 256, 0, s>>>(nn, H->coarse_inv.get(),
                                                  H->coarse_inv32.get());
     CN_HIP(hipGetLastError());
+    if (H->nc >= 3072) {
+      H->coarse_ld = (H->nc + 7) / 8 * 8;
+      const size_t nb = (size_t)H->nc * H->coarse_ld;
+      H->coarse_bh.alloc(nb);
+      H->coarse_bl.alloc(nb);
+      coarse_split_bf16(H->nc, H->coarse_ld, H->coarse_inv.get(), H->coarse_bh.get(),
+                        H->coarse_bl.get(), s);
+    }
   }
   (void)tail_desc<float>(H, s);
   (void)tail_desc<double>(H, s);

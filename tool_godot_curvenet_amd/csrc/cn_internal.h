@@ -316,6 +316,10 @@ struct cn_hsc_t {
   cn::DevBuf<int32_t> coarse_rows;  // their level-0 rows
   cn::DevBuf<double> coarse_L, coarse_inv;
   cn::DevBuf<float> coarse_inv32;   // coarse_inv rounded to FP32
+  // coarse_inv as hi + lo bfloat16 matrices (rows of coarse_ld entries),
+  // for hierarchies with at least 3,072 coarse rows
+  cn::DevBuf<uint16_t> coarse_bh, coarse_bl;
+  int32_t coarse_ld = 0;
   int64_t nnz = 0;
   // apply workspace: the gathered coarse rows (nc x k), grown on demand
   cn::DevBuf<unsigned char> work;

@@ -3391,7 +3391,7 @@ cn_status cn_system_set_tune(cn_system S, const cn_tune *t) {
       t->spmm_gens > (int32_t)kSpmmGens || t->spmm_gens == 3 || t->level_cap < 0 ||
       t->level_cap > 65536 || t->tail_rows < -1 || t->tail_rows > 16384 ||
       t->chain_levels < 0 || t->chain_levels > 24 || t->coarse_prec < 0 ||
-      t->coarse_prec > 2 || t->down_cap < 0 || t->down_cap > 65536) {
+      t->coarse_prec > 3 || t->down_cap < 0 || t->down_cap > 65536) {
     set_error("cn_system_set_tune: bad arguments");
     return CN_BAD_ARG;
   }
