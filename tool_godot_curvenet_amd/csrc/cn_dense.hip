@@ -472,7 +472,9 @@ __global__ void k_split_bf16(int m, int ld, const double *__restrict__ a,
 // tile, slice and staging scheme is k_coarse_mm's with chunks of 16 j and
 // both halves of the inverse staged in LDS; A lane l holds rows l % 16 and
 // j = 4 (l / 16) .. + 3, B lane l the same j of column l % 16, and the
-// result the layout of the 16 x 16 x 4 FP32 MFMA.
+// result the layout of the 16 x 16 x 4 FP32 MFMA.  (gfx950's 16 x 16 x 32
+// bfloat16 MFMA with chunks of 32 needs 150 VGPRs and 78 KB of LDS, one
+// block a CU, and made the solve 4 % slower.)
 template <typename T, int BM, int BN, int WM, int WN>
 __global__ void __launch_bounds__(64 * WM * WN)
 k_coarse_mm_bf3(int m, int ld, int k, int kchunk, const uint16_t *__restrict__ Mh,
@@ -487,7 +489,9 @@ k_coarse_mm_bf3(int m, int ld, int k, int kchunk, const uint16_t *__restrict__ M
   static_assert(kCK == 16 && AE == 4 && BN % BE == 0, "tile shape");
   __shared__ __align__(16) uint16_t Ah[2][BM][kCK + 8];
   __shared__ __align__(16) uint16_t Al[2][BM][kCK + 8];
-  __shared__ T Bs[2][kCK][BN + 16];
+  // B lane l reads rows 4 (l / 16) + u: a row stride of 4 mod 16 words puts
+  // the 4 row groups on distinct banks
+  __shared__ T Bs[2][kCK][BN + 4];
   __shared__ int32_t Rs[kCMaxChunk + 2 * kCK];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
   const int nx = gridDim.x, ny = gridDim.y, nb = nx * ny * gridDim.z;
