@@ -90,7 +90,7 @@ struct HscShape {
   int32_t level_cap = 0;
   int32_t tail_rows = 0;
   int32_t chain_levels = 0;
-  int32_t coarse_prec = 0;  // 2: FP32 coarse product in FP32 applies
+  int32_t coarse_prec = 0;  // 2: FP32 coarse product in FP32 applies, 3: split bfloat16
   int32_t down_cap = 0;     // the down sweep's grid cap (0: level_cap)
 };
 
