@@ -173,6 +173,26 @@ def test_graph_replay_bit_identical(gpu, name, k, precond, precision):
     assert np.all(np.linalg.norm(R, axis=0) <= 1e-9 * np.linalg.norm(B, axis=0))
 
 
+def test_graph_replay_split_coarse_product(gpu):
+    """The benched capsule hierarchy (theta 0.002, 3,834 coarse rows) with the
+    split-bfloat16 coarse product (coarse_prec 3) at k = 768: graph replay
+    gives the direct launches' bits, and the solve reaches 1e-9."""
+    from tool_godot_curvenet_amd import HSC, CutLaplacian, meshgen
+    lap = CutLaplacian.from_mesh(meshgen.config_mesh("capsules100k"))
+    h = HSC(lap, theta=0.002, coarse_max=4096)
+    assert h.info.n_coarse >= 3072
+    lap.set_tune({"spmm_form": 2, "coarse_prec": 3})
+    B = lap.harmonic_rhs(meshgen.seeded_boundary(lap.n_con, 768, seed=77))
+    x0, i0 = lap.solve(B, precond="hsc", precision="mixed", tol=1e-9, graph=False, hsc=h)
+    for rep in range(2):
+        x1, i1 = lap.solve(B, precond="hsc", precision="mixed", tol=1e-9, graph=True, hsc=h)
+        assert i1.status == 0 and i1.iters == i0.iters
+        assert np.array_equal(x0.cpu().numpy(), x1.cpu().numpy()), rep
+    A = lap.A_ff()
+    R = B - O.spmm(A.row_ptr, A.col_idx, A.values, x1.cpu().numpy())
+    assert np.all(np.linalg.norm(R, axis=0) <= 1e-9 * np.linalg.norm(B, axis=0))
+
+
 def test_graph_replay_batch(gpu):
     """A batch of systems (per-member scalars): replayed equals direct."""
     from tool_godot_curvenet_amd import HSC, BatchSystem, CutLaplacian, meshgen
