@@ -591,7 +591,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner / f64 refinement" if args.precision == "mixed" else "f64",
+        "dtype": ("f32 inner / f64 refinement" + (
+            " (coarse product: bf16 MFMA on hi + lo splits, 2^-16)"
+            if args.precond == "hsc" and lap.get_tune()["coarse_prec"] == 3 and
+            pre.info.n_coarse >= 3072 and k >= 512 else "")
+            if args.precision == "mixed" else "f64"),
         "data": "synthetic (seeded capsule union, seeded constraint displacements)",
         "config": {"workload": "capsule union 12 capsules / 16 ring cuts, "
                                f"{mesh.nv} verts, n_free={n}, nnz={nnz}, "
