@@ -408,10 +408,10 @@ def main():
             cms = prof["hsc_coarse"][1]
             ncoarse = prof["hsc_coarse"][0]
             # the coarse product on FP32 (coarse_prec 2), split-bfloat16
-            # (coarse_prec 3, products of at least 3,072 coarse rows and 512
-            # columns) or FP64 matrix cores
+            # (coarse_prec 3, hierarchies of at least 3,072 coarse rows) or
+            # FP64 matrix cores
             cp = lap.get_tune()["coarse_prec"] if args.precision == "mixed" else 1
-            cbf = cp == 3 and nc >= 3072 and k >= 512
+            cbf = cp == 3 and nc >= 3072 and k > 4
             c32 = cp >= 2 and not cbf
             cprec, cpeak = (("split bfloat16, 3 products", BF16_MFMA_PEAK_TFLOPS) if cbf else
                             ("FP32", FP32_MFMA_PEAK_TFLOPS) if c32 else
@@ -420,7 +420,8 @@ def main():
             if l0 < 0 and ncoarse > 0:
                 # the coarse product: 2 n_c^2 k flops per apply that does work
                 # (the split product runs three times as many on the MFMAs)
-                kn = ("k_coarse_mm_bf3<float,128,128,4,2>" if cbf else
+                big = nc >= 3072 and k >= 512
+                kn = (f"k_coarse_mm_bf3<float,{'128,128,4,2' if big else '64,64,2,2'}>" if cbf else
                       f"k_coarse_mm<float,{'float' if c32 else 'double'},128,128,4,2>")
                 mfma[ctag] = (f"{kn} + k_coarse_sum ({cprec} MFMA)", ncoarse, cms,
                               nwork * 2.0 * nc * nc * k, cpeak)
@@ -594,7 +595,7 @@ def main():
         "dtype": ("f32 inner / f64 refinement" + (
             " (coarse product: bf16 MFMA on hi + lo splits, 2^-16)"
             if args.precond == "hsc" and lap.get_tune()["coarse_prec"] == 3 and
-            pre.info.n_coarse >= 3072 and k >= 512 else "")
+            pre.info.n_coarse >= 3072 and k > 4 else "")
             if args.precision == "mixed" else "f64"),
         "data": "synthetic (seeded capsule union, seeded constraint displacements)",
         "config": {"workload": "capsule union 12 capsules / 16 ring cuts, "

@@ -221,7 +221,7 @@ cn_status cn_system_batch(int32_t nsys, const cn_system *systems, cn_system *out
  *              FP32 rounding), 3 = the inverse split into two bfloat16
  *              matrices, each product a b as a_h b_h + a_h b_l + a_l b_h on
  *              bfloat16 MFMA (2^-16 of a b dropped; hierarchies of at least
- *              3,072 coarse rows and k >= 512, else as 2) (0: 1).  FP64
+ *              3,072 coarse rows and k > 4, else as 2) (0: 1).  FP64
  *              applies and the fused tail always take the FP64 inverse. */
 typedef struct cn_tune {
   int32_t spmm_form;

@@ -169,12 +169,11 @@ def test_chain_levels_direction_and_convergence(gpu, g, precision, tol):
 
 # The FP32 coarse product (coarse_prec 2) in the one-wavefront-a-row vector
 # form (k = 3), on 64 x 64 tiles (k = 64) and 128 x 128 tiles (k = 520;
-# 3,000+ coarse rows), and the split-bfloat16 product (coarse_prec 3, the
-# 128 x 128 tiles; a b as a_h b_h + a_h b_l + a_l b_h, 2^-16 of a b
-# dropped): one mixed PCG step from 0 follows the oracle hierarchy's apply
+# 3,000+ coarse rows), and the split-bfloat16 product (coarse_prec 3, both
+# tile shapes; a b as a_h b_h + a_h b_l + a_l b_h, 2^-16 of a b dropped): one mixed PCG step from 0 follows the oracle hierarchy's apply
 # to FP32 rounding (3e-4 for the split product), and the solve converges;
 # an FP64 solve on the same tune keeps the FP64 product.
-@pytest.mark.parametrize("k,prec", [(3, 2), (64, 2), (520, 2), (520, 3)])
+@pytest.mark.parametrize("k,prec", [(3, 2), (64, 2), (64, 3), (520, 2), (520, 3)])
 def test_coarse_fp32_direction_and_convergence(gpu, k, prec):
     from tool_godot_curvenet_amd import HSC
     from tool_godot_curvenet_amd import meshgen as G

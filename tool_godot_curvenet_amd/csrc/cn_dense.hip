@@ -691,9 +691,10 @@ int coarse_product(int m, int k, const double *M, const float *M32, const int32_
 
 int coarse_rz_rows(int m) { return (m + kSumRows - 1) / kSumRows; }
 
-// The split-bfloat16 product (k_coarse_mm_bf3) runs where the 128 x 128
-// tiles do; other shapes keep the FP32 one.
-bool coarse_bf3_applies(int m, int k) { return coarse_big_tiles(m, k); }
+// The split-bfloat16 product (k_coarse_mm_bf3) runs on hierarchies that
+// keep the split inverse (at least 3,072 coarse rows), in the tiles the
+// FP32 product would take there.
+bool coarse_bf3_applies(int m, int k) { return m >= 3072 && k > 4; }
 
 void coarse_split_bf16(int m, int ld, const double *a, uint16_t *hi, uint16_t *lo,
                        hipStream_t s) {
@@ -713,9 +714,15 @@ int coarse_product_bf3(int m, int ld, int k, const uint16_t *Mh, const uint16_t 
   const int sp = coarse_splits(m, k);
   const int chunk = ((m + sp - 1) / sp + kCK - 1) / kCK * kCK;
   float *pf = reinterpret_cast<float *>(part);
-  const dim3 g((unsigned)((k + 127) / 128), (unsigned)((m + 127) / 128), (unsigned)sp);
-  k_coarse_mm_bf3<float, 128, 128, 4, 2><<<g, 512, 0, s>>>(m, ld, k, chunk, Mh, Ml, rows, S,
+  if (coarse_big_tiles(m, k)) {
+    const dim3 g((unsigned)((k + 127) / 128), (unsigned)((m + 127) / 128), (unsigned)sp);
+    k_coarse_mm_bf3<float, 128, 128, 4, 2><<<g, 512, 0, s>>>(m, ld, k, chunk, Mh, Ml, rows,
+                                                             S, pf, ctl);
+  } else {
+    const dim3 g((unsigned)((k + 63) / 64), (unsigned)((m + 63) / 64), (unsigned)sp);
+    k_coarse_mm_bf3<float, 64, 64, 2, 2><<<g, 256, 0, s>>>(m, ld, k, chunk, Mh, Ml, rows, S,
                                                            pf, ctl);
+  }
   const dim3 gs((unsigned)((k + 255) / 256), (unsigned)((m + kSumRows - 1) / kSumRows));
   k_coarse_sum<float, float><<<gs, 256, 0, s>>>(m, k, sp, pf, rows, S, Z, ctl, rz);
   return rz ? (int)gs.y : 0;
